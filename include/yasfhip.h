@@ -1,0 +1,110 @@
+/*
+ * yasfhip: C ABI of the MI355X (gfx950) multiple-sphere T-matrix solve path.
+ *
+ * Drop-in boundary for the reference's coupling-backend seam
+ * (yasfpy/coupling/backends.py:36-66 CouplingBackend.multiply, selected by
+ * yasfpy/simulation.py:95-106) and for the solve it drives
+ * (yasfpy/scattered_field_coefficients.py:95-156, yasfpy/solver.py:108-121).
+ *
+ * Conventions
+ *  - Complex arrays are interleaved (re, im) doubles; complex64 arrays are
+ *    interleaved floats.  Vectors are particle-major: j = s * nmax + n with
+ *    nmax = 2 L (L + 2) and the reference's mode order
+ *    n = (tau-1) L(L+2) + l^2 - 1 + l + m (yasfpy/functions/cpu_numba.py:395-405).
+ *  - Every call returns 0 on success, otherwise a status code; the message of
+ *    the last failure on the calling thread is yasfhip_last_error().
+ *  - A session binds one HIP stream on the current device.  Calls on one
+ *    session are synchronous with respect to the host and not thread-safe.
+ *  - Pointers are host pointers unless a function says otherwise.
+ */
+#ifndef YASFHIP_H
+#define YASFHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct yasfhip_session yasfhip_session;
+
+enum {
+    YASFHIP_OK = 0,
+    YASFHIP_INVALID_ARGUMENT = 1,
+    YASFHIP_DEVICE_ERROR = 2,
+    YASFHIP_NOT_PREPARED = 3,
+    YASFHIP_INTERNAL = 4
+};
+
+enum { YASFHIP_FP64 = 0, YASFHIP_FP32 = 1 };
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *yasfhip_last_error(void);
+
+/* ab5[j1][j2][p] (nmax x nmax x (2L+1) complex) into out.
+ * Replaces Numerics.compute_translation_table (yasfpy/numerics.py:187-280). */
+int yasfhip_translation_table(int lmax, double *out);
+
+/* Work counts of the coupling plan: complex MACs the pair kernel issues per
+ * particle pair, and the number of nonzero ab5 entries. */
+int yasfhip_plan_stats(int lmax, long long *pair_macs_per_pair, long long *nnz_ab5);
+
+/* Create a session for particle positions (N x 3), expansion order lmax
+ * (1..8) and real medium wave numbers k_medium[channels].  Replaces the
+ * Simulation set-up of the coupling backend (yasfpy/simulation.py:95-167);
+ * no O(N^2) tables are built. */
+int yasfhip_create(const double *positions, int particle_number, int lmax, const double *k_medium,
+                   int channels, yasfhip_session **out);
+int yasfhip_destroy(yasfhip_session *s);
+
+/* Diagonal T-matrix per channel, t[channels][N*nmax] complex
+ * (Simulation.mie_coefficients[single_unique_array_idx, :, w].ravel(),
+ * yasfpy/matrix_apply.py:104-107). */
+int yasfhip_set_tmatrix(yasfhip_session *s, const double *t);
+
+/* y = W x for one channel (CouplingBackend.multiply(x, idx),
+ * yasfpy/coupling/backends.py:99-125).  precision: YASFHIP_FP64 computes in
+ * complex128; YASFHIP_FP32 runs the pair GEMM in complex64 (geometry and
+ * special functions stay FP64) with complex128 in/out. */
+int yasfhip_coupling_matvec(yasfhip_session *s, int channel, const double *x, double *y,
+                            int precision);
+
+/* y = x - t * (W x) (master_matrix_multiply, yasfpy/matrix_apply.py:66-120). */
+int yasfhip_master_matvec(yasfhip_session *s, int channel, const double *x, double *y,
+                          int precision);
+
+/* Solve (I - T W) x = b for one channel with GMRES on the device, following
+ * SciPy's gmres as the reference calls it (solver.py:108-121: rtol=tol,
+ * atol=0, restart, maxiter in restart cycles; left preconditioning, modified
+ * Gram-Schmidt, LAPACK-style Givens rotations, true-residual outer check).
+ * x_inout holds x0 on entry (the reference defaults x0 = b) and the solution
+ * on exit.  use_precond != 0 applies the block-diagonal preconditioner built
+ * by yasfhip_precond_build.  info: 0 converged, else maxiter (SciPy).
+ * history (may be NULL) receives up to history_cap values presid/||b|| per
+ * inner iteration (SciPy's 'pr_norm' callback).  iterations = inner
+ * iterations; residual = final ||b - A x|| / ||b||. */
+int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double *x_inout,
+                        double rtol, int restart, int maxiter, int use_precond, int *info,
+                        int *iterations, double *residual, double *history, int history_cap);
+
+/* Block-diagonal preconditioner for one channel: particles are grouped into
+ * boxes by box_of_particle[N] (values 0..num_boxes-1); each box's block of
+ * I - T W restricted to intra-box pairs is assembled and LU-factored on the
+ * device.  Intended semantics of yasfpy/preconditioner.py:1-14, 92-219
+ * (receiver T, particle-major indices, FP64). */
+int yasfhip_precond_build(yasfhip_session *s, int channel, const int *box_of_particle,
+                          int num_boxes);
+
+/* z = M^-1 r with the prepared preconditioner (BlockDiagonalPreconditioner.apply,
+ * yasfpy/preconditioner.py:177-219). */
+int yasfhip_precond_apply(yasfhip_session *s, const double *r, double *z);
+
+/* Timing of `reps` device-resident matvecs of one channel on the session
+ * stream with HIP events: ms_pair = mean duration of the pair kernel,
+ * ms_total = mean duration of the whole matvec (pair + contraction). */
+int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps, double *ms_pair,
+                        double *ms_total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YASFHIP_H */

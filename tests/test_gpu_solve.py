@@ -1,0 +1,119 @@
+"""GPU solve parity: device GMRES (and block preconditioner) through the
+Simulation API and the C ABI against the reference's fixtures and SciPy GMRES
+driving the oracle."""
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+from oracle import restate as R
+from yasf_amd import (BlockDiagonalPreconditioner, InitialField, Numerics, Parameters, Particles,
+                      Simulation, Solver, edge_size_for)
+from yasf_amd import _native
+from yasf_amd.clusters import cluster
+
+pytestmark = pytest.mark.gpu
+
+
+def build_sim(positions, radii, n, lam, nmed, lmax, tol, restart=100, maxiter=1000, pc=None):
+    particles = Particles(positions, radii, n)
+    field = InitialField(beam_width=0, focal_point=np.zeros(3), polar_angle=0.0,
+                         azimuthal_angle=0.0, polarization="TE")
+    params = Parameters(np.atleast_1d(lam), np.atleast_1d(nmed), particles, field)
+    solver = Solver("gmres", tolerance=tol, max_iter=maxiter, restart=restart, preconditioner=pc)
+    sim = Simulation(params, Numerics(lmax=lmax, solver=solver, coupling_backend="hip"))
+    sim.compute_mie_coefficients()
+    sim.compute_initial_field_coefficients()
+    sim.compute_right_hand_side()
+    return sim
+
+
+def test_full_simulation_fixture(golden):
+    d = golden("full_simulation")
+    sim = build_sim(d["positions"], d["radii"], d["refractive_index"], float(d["wavelength"]),
+                    float(d["medium_refractive_index"]), int(d["lmax"]), 1e-4, restart=1000)
+    sim.compute_scattered_field_coefficients()
+    assert sim.scattered_field_err_codes[0] == 0
+    npt.assert_allclose(sim.scattered_field_coefficients[:, :, 0], d["scattered_field_coefficients"],
+                        rtol=1e-3, atol=1e-5)
+
+
+def test_celes_500_fixture(golden):
+    c = golden("celes500")
+    sim = build_sim(c["positions"], c["radii"], c["refractive_index"], c["wavelength"],
+                    c["medium_refractive_index"].real, int(c["lmax"]), float(c["tolerance"]),
+                    restart=int(c["restart"]), maxiter=int(c["max_iter"]))
+    sim.compute_scattered_field_coefficients()
+    npt.assert_allclose(sim.initial_field_coefficients[:, :, 0], c["initial_field_coefficients"],
+                        rtol=2e-3)
+    npt.assert_allclose(sim.scattered_field_coefficients[:, :, 0], c["scattered_field_coefficients"],
+                        rtol=2e-3)
+
+
+def _cfg1():
+    pos, radii = cluster(1, 50, 0.1)
+    return pos, radii, np.full(50, 1.5 + 0.01j), 550.0, 1.0, 4
+
+
+def test_gmres_matches_scipy_over_oracle():
+    pos, radii, n, lam, nmed, L = _cfg1()
+    sim = build_sim(pos, radii, n, lam, nmed, L, 1e-6, restart=100)
+    sim.compute_scattered_field_coefficients()
+    x = sim.scattered_field_coefficients[:, :, 0].ravel()
+    b = sim.right_hand_side[:, :, 0].ravel()
+    k = 2 * np.pi / lam
+    t = sim.tmatrix_flat()[0]
+    res = np.linalg.norm(b - R.master_matvec(L, pos, k, t, x)) / np.linalg.norm(b)
+    assert res <= 1e-6
+    xr, info, it_ref = R.gmres_solve(lambda v: R.master_matvec(L, pos, k, t, v), b, rtol=1e-6,
+                                     restart=100)
+    assert info == 0
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-5
+    its = sim.numerics.solver.last_iterations[0]
+    assert abs(its - it_ref) <= 1
+
+
+def test_block_preconditioner_apply_and_solve():
+    pos, radii, n, lam, nmed, L = _cfg1()
+    pc = BlockDiagonalPreconditioner(edge_size_for(pos, 8))
+    sim = build_sim(pos, radii, n, lam, nmed, L, 1e-6, restart=100, pc=pc)
+    pc.prepare(sim)
+    k = 2 * np.pi / lam
+    t = sim.tmatrix_flat()[0]
+    ref_apply = R.block_preconditioner(L, pos, k, t, pc.partitioning)
+    r = np.random.default_rng(9).normal(size=t.size) + 1j
+    npt.assert_allclose(pc.apply(r), ref_apply(r), rtol=1e-10, atol=1e-10 * np.abs(r).max())
+    sim.compute_scattered_field_coefficients()
+    x_pc = sim.scattered_field_coefficients[:, :, 0].ravel().copy()
+    b = sim.right_hand_side[:, :, 0].ravel()
+    assert np.linalg.norm(b - R.master_matvec(L, pos, k, t, x_pc)) / np.linalg.norm(b) <= 1e-6
+    its_pc = sim.numerics.solver.last_iterations[0]
+    sim2 = build_sim(pos, radii, n, lam, nmed, L, 1e-6, restart=100)
+    sim2.compute_scattered_field_coefficients()
+    x = sim2.scattered_field_coefficients[:, :, 0].ravel()
+    assert np.linalg.norm(x - x_pc) / np.linalg.norm(x) < 1e-5
+    assert its_pc <= sim2.numerics.solver.last_iterations[0]
+
+
+def test_single_box_preconditioner_is_exact():
+    pos, radii = cluster(11, 6, 0.1)
+    sim = build_sim(pos, radii, np.full(6, 1.6 + 0.02j), 600.0, 1.0, 3, 1e-8, restart=50)
+    s = sim.session
+    s.precond_build(np.zeros(6, dtype=np.int32))
+    b = sim.right_hand_side[:, :, 0].ravel()
+    x, info, iters, res, _ = s.gmres(b, b, rtol=1e-8, restart=50, maxiter=10, use_precond=True)
+    assert info == 0 and iters == 1 and res < 1e-8
+
+
+def test_gmres_errors_and_nonconvergence():
+    pos, radii, n, lam, nmed, L = _cfg1()
+    sim = build_sim(pos, radii, n, lam, nmed, L, 1e-12, restart=2, maxiter=1)
+    sim.compute_scattered_field_coefficients()
+    assert sim.scattered_field_err_codes[0] == 1  # SciPy: info = maxiter when not converged
+    with pytest.raises(ValueError):
+        sim.session.gmres(sim.right_hand_side[:, :, 0].ravel(), channel=3)
+    with pytest.raises(ValueError):
+        sim.session.gmres(sim.right_hand_side[:, :, 0].ravel(), use_precond=True)
+    zero = np.zeros(sim.session.n, dtype=complex)
+    x, info, iters, res, _ = sim.session.gmres(zero)
+    assert info == 0 and np.all(x == 0)

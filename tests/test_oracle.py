@@ -1,0 +1,202 @@
+"""Pins of the CPU oracle against the reference's own fixtures, library routines
+and closed forms (no GPU).  Each test names what it pins."""
+
+import math
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+from scipy.special import lpmv, spherical_jn, spherical_yn
+
+from oracle import restate as R
+
+
+def test_multi2single_index(golden):
+    d = golden("index")  # MATLAB 1-based (tests/test_comparison.py:59-67)
+    for (s, tau, l, m), expect in zip(d["inputs"], d["outputs"]):
+        assert R.multi2single_index(s - 1, tau, l, m, int(d["lmax"])) == expect - 1
+    idx = R.idx_lookup(3, 2)
+    for j, (s, n, tau, l, m) in enumerate(idx):
+        assert R.multi2single_index(s, tau, l, m, 3) == j
+
+
+def test_legendre_fixture(golden):
+    d = golden("legendre")
+    theta = d["theta"]
+    plm = R.legendre_normalized_trigon(int(d["lmax"]), np.cos(theta), np.sin(theta))
+    npt.assert_allclose(plm, d["plm"], rtol=1e-5, atol=1e-12)
+
+
+def test_legendre_closed_form():
+    """Normalised P_l^m without Condon-Shortley phase = (-1)^m norm * scipy lpmv."""
+    th = np.linspace(0.05, 3.1, 17)
+    ct, st = np.cos(th), np.sin(th)
+    plm = R.legendre_normalized_trigon(16, ct, st)
+    for l in range(17):
+        for m in range(l + 1):
+            norm = math.sqrt((2 * l + 1) / 2 * math.factorial(l - m) / math.factorial(l + m))
+            npt.assert_allclose(plm[l, m], (-1) ** m * norm * lpmv(m, l, ct), rtol=1e-10, atol=1e-13)
+
+
+def test_hankel_tables(golden):
+    d = golden("sph_bessel")
+    npt.assert_allclose(spherical_jn(int(d["l"]), d["z"]), d["j"], rtol=1e-5)
+    npt.assert_allclose(spherical_jn(int(d["l"]), d["z"]) + 1j * spherical_yn(int(d["l"]), d["z"]),
+                        d["h"], rtol=1e-5)
+    # closed forms h_0 = -i e^{ix}/x, h_1 = -(x + i) e^{ix}/x^2
+    pos = np.array([[0.0, 0.0, 0.0], [30.0, -40.0, 120.0]])
+    k = 0.011
+    _, sph_h, _, _ = R.mutual_lookup(2, pos, pos, np.array([k]))
+    x = k * np.linalg.norm(pos[0] - pos[1])
+    npt.assert_allclose(sph_h[0, 0, 1, 0], -1j * np.exp(1j * x) / x, rtol=1e-14)
+    npt.assert_allclose(sph_h[1, 0, 1, 0], -(x + 1j) * np.exp(1j * x) / x ** 2, rtol=1e-14)
+
+
+def test_wigner3j_against_sympy():
+    from sympy import Rational
+    from sympy.physics.wigner import wigner_3j
+
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        j1, j2 = rng.integers(0, 8, 2)
+        j3 = rng.integers(abs(j1 - j2), j1 + j2 + 1)
+        m1 = rng.integers(-j1, j1 + 1)
+        m2 = rng.integers(-j2, j2 + 1)
+        m3 = -m1 - m2
+        if abs(m3) > j3:
+            continue
+        exact = float(wigner_3j(int(j1), int(j2), int(j3), int(m1), int(m2), int(m3)))
+        assert R.wigner3j(j1, j2, j3, m1, m2, m3) == pytest.approx(exact, rel=1e-14, abs=1e-16)
+    assert R.wigner3j(1, 1, 1, 0, 0, 0) == 0.0  # odd sum with zero m vanishes
+    del Rational
+
+
+def test_wigner3j_orthogonality():
+    """sum_{m1} (2 j3 + 1) (j1 j2 j3; m1 m2 m3)(j1 j2 j3'; m1 m2 m3) = delta_{j3 j3'} at fixed m3."""
+    for j1, j2 in [(2, 3), (4, 4), (5, 2)]:
+        for m3 in (-1, 0, 2):
+            js = [j for j in range(abs(j1 - j2), j1 + j2 + 1) if abs(m3) <= j]
+            for j3 in js:
+                for j3b in js:
+                    s = sum((2 * j3 + 1) * R.wigner3j(j1, j2, j3, m1, -m1 - m3, m3)
+                            * R.wigner3j(j1, j2, j3b, m1, -m1 - m3, m3)
+                            for m1 in range(-j1, j1 + 1) if abs(m1 + m3) <= j2)
+                    assert s == pytest.approx(1.0 if j3 == j3b else 0.0, abs=1e-13)
+
+
+def test_translation_table_fixture(golden):
+    d = golden("ab5_lmax3")
+    npt.assert_allclose(R.translation_table(3), d["ab5"], rtol=1e-5, atol=1e-7)
+
+
+def test_mie_t_entry(golden):
+    d = golden("t_entry")  # t_entry(tau, l, kM, kS, R) (tests/test_comparison.py:92-110)
+    lam = 2 * np.pi / d["k_medium"]
+    mie = R.mie_coefficients(1, np.array([d["radius"]]), np.array([d["k_sphere"] / d["k_medium"]]),
+                             np.array([lam]), 1.0)
+    l = int(d["l"])
+    npt.assert_allclose(mie[0, l * l - 1, 0], d["scattered_1"], rtol=1e-5)
+    npt.assert_allclose(mie[0, 3 + l * l - 1, 0], d["scattered_2"], rtol=1e-5)
+
+
+def test_mie_series_single_sphere():
+    """Q_ext, Q_sca of one sphere from the textbook Mie series vs -Re sum(conj(a) f)."""
+    lmax, r, n, lam = 10, 150.0, 1.5 + 0.02j, 600.0
+    k = 2 * np.pi / lam
+    mie = R.mie_coefficients(lmax, np.array([r]), np.array([n]), np.array([lam]), 1.0)[0, :, 0]
+    a = R.planewave_coefficients(lmax, np.zeros((1, 3)), np.array([k]), 0.0, 0.0, 1)[0, :, 0]
+    f = mie * a
+    c_ext = float(np.real(-np.sum(np.conj(a) * f))) / k ** 2 * np.pi
+    x = k * r
+    qs = 0.0
+    for l in range(1, lmax + 1):
+        b_l = -mie[l * l - 1]
+        a_l = -mie[lmax * (lmax + 2) + l * l - 1]
+        qs += (2 * l + 1) * np.real(a_l + b_l)
+    q_ext_series = 2 / x ** 2 * qs
+    assert c_ext / (np.pi * r ** 2) == pytest.approx(q_ext_series, rel=1e-10)
+
+
+def test_planewave_fixtures(golden):
+    d = golden("full_simulation")
+    k = 2 * np.pi / float(d["wavelength"]) * float(d["medium_refractive_index"])
+    a = R.planewave_coefficients(int(d["lmax"]), d["positions"], np.array([k]), 0.0, 0.0, 1)[:, :, 0]
+    npt.assert_allclose(a, d["initial_field_coefficients"], rtol=1e-4, atol=1e-6)
+    c = golden("celes500")
+    k = (2 * np.pi / c["wavelength"] * c["medium_refractive_index"]).real
+    a = R.planewave_coefficients(int(c["lmax"]), c["positions"], k, 0.0, 0.0, 1)[:, :, 0]
+    npt.assert_allclose(a, c["initial_field_coefficients"], rtol=2e-3)
+
+
+def test_mie_fixture(golden):
+    d = golden("full_simulation")
+    mie = R.mie_coefficients(int(d["lmax"]), d["radii"], d["refractive_index"],
+                             np.array([float(d["wavelength"])]), float(d["medium_refractive_index"]))
+    npt.assert_allclose(mie[:, :, 0], d["mie_coefficients"], rtol=1e-4, atol=1e-6)
+
+
+def test_coupling_matvec_fixture(golden):
+    d = golden("coupling_matvec")
+    k = 2 * np.pi / float(d["wavelength"]) * float(d["medium_refractive_index"])
+    wx = R.coupling_matvec(int(d["lmax"]), d["positions"], k, d["x"])
+    npt.assert_allclose(wx, d["wx"], rtol=2e-4, atol=1e-5)
+
+
+def _dense_w(lmax, pos, k):
+    n = pos.shape[0] * R.nmax_of(lmax)
+    W = np.zeros((n, n), dtype=complex)
+    for c in range(n):
+        e = np.zeros(n, dtype=complex)
+        e[c] = 1.0
+        W[:, c] = R.coupling_matvec(lmax, pos, k, e)
+    return W
+
+
+def test_coupling_reciprocity():
+    """W[j1, j2] = W[P j2, P j1] with P: m -> -m (reciprocity of the translation)."""
+    lmax = 3
+    pos = np.random.default_rng(1).normal(size=(3, 3)) * 250.0
+    W = _dense_w(lmax, pos, 2 * np.pi / 480.0)
+    idx = R.idx_lookup(lmax, 3)
+    perm = np.array([R.multi2single_index(s, t, l, -m, lmax) for s, _, t, l, m in idx])
+    npt.assert_allclose(W.T[perm][:, perm], W, rtol=0, atol=1e-14 * np.abs(W).max())
+
+
+def test_coupling_axis_selection_rule():
+    """Spheres on the z axis couple only equal m (P_p^{|dm|}(+-1) = 0 for dm != 0)."""
+    lmax = 3
+    pos = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 350.0]])
+    W = _dense_w(lmax, pos, 2 * np.pi / 550.0)
+    m = R.idx_lookup(lmax, 2)[:, 4]
+    assert np.abs(W[m[:, None] != m[None, :]]).max() == 0.0
+    assert np.abs(W[m[:, None] == m[None, :]]).max() > 1e-3
+
+
+def test_full_simulation_solve(golden):
+    d = golden("full_simulation")
+    L = int(d["lmax"])
+    lam = float(d["wavelength"])
+    k = 2 * np.pi / lam * float(d["medium_refractive_index"])
+    mie = R.mie_coefficients(L, d["radii"], d["refractive_index"], np.array([lam]),
+                             float(d["medium_refractive_index"]))[:, :, 0]
+    a = R.planewave_coefficients(L, d["positions"], np.array([k]), 0.0, 0.0, 1)[:, :, 0]
+    t = mie.ravel()
+    x, info, _ = R.gmres_solve(lambda v: R.master_matvec(L, d["positions"], k, t, v), t * a.ravel(),
+                               rtol=1e-4, restart=1000, maxiter=1000)
+    assert info == 0
+    npt.assert_allclose(x.reshape(a.shape), d["scattered_field_coefficients"], rtol=1e-3, atol=1e-5)
+
+
+def test_block_preconditioner_single_box_is_exact():
+    """One box holding every particle: M^-1 is the exact inverse (1 iteration)."""
+    L = 2
+    rng = np.random.default_rng(5)
+    pos = rng.normal(size=(4, 3)) * 300.0
+    lam = 600.0
+    k = 2 * np.pi / lam
+    mie = R.mie_coefficients(L, np.full(4, 90.0), np.full(4, 1.5 + 0.01j), np.array([lam]), 1.0)[:, :, 0]
+    t = mie.ravel()
+    apply = R.block_preconditioner(L, pos, k, t, [np.arange(4)])
+    b = rng.normal(size=t.size) + 1j * rng.normal(size=t.size)
+    z = apply(b)
+    npt.assert_allclose(R.master_matvec(L, pos, k, t, z), b, rtol=1e-12, atol=1e-12)

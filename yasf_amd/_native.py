@@ -1,0 +1,213 @@
+"""ctypes binding of ``libyasfhip.so`` (the C ABI declared in include/yasfhip.h).
+
+The library is required: importing a function that needs it raises
+``NativeLibraryError`` when it is missing, and every device call raises when
+no GPU is usable.  There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libyasfhip.so"
+
+FP64 = 0
+FP32 = 1
+
+_lib = None
+_lock = threading.Lock()
+
+
+class NativeLibraryError(RuntimeError):
+    """libyasfhip.so is missing or a call into it failed."""
+
+
+class DeviceError(NativeLibraryError):
+    """A HIP call inside the library failed (no GPU, out of memory, ...)."""
+
+
+def _load():
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise NativeLibraryError(
+                f"{LIB_PATH} is not built; run `python -m yasf_amd.build` (hipcc, gfx950)")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        P = ctypes.c_void_p
+        i = ctypes.c_int
+        d = ctypes.c_double
+        ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
+        llp = ctypes.POINTER(ctypes.c_longlong)
+        sig = {
+            "yasfhip_last_error": ([], ctypes.c_char_p),
+            "yasfhip_translation_table": ([i, P], i),
+            "yasfhip_plan_stats": ([i, llp, llp], i),
+            "yasfhip_create": ([P, i, i, P, i, ctypes.POINTER(P)], i),
+            "yasfhip_destroy": ([P], i),
+            "yasfhip_set_tmatrix": ([P, P], i),
+            "yasfhip_coupling_matvec": ([P, i, P, P, i], i),
+            "yasfhip_master_matvec": ([P, i, P, P, i], i),
+            "yasfhip_gmres_solve": ([P, i, P, P, d, i, i, i, ip, ip, dp, P, i], i),
+            "yasfhip_precond_build": ([P, i, P, i], i),
+            "yasfhip_precond_apply": ([P, P, P], i),
+            "yasfhip_time_matvec": ([P, i, i, i, dp, dp], i),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+        return lib
+
+
+def library():
+    """The loaded ctypes library (loads it on first use)."""
+    return _load()
+
+
+def exported_symbols() -> list[str]:
+    return [
+        "yasfhip_last_error", "yasfhip_translation_table", "yasfhip_plan_stats", "yasfhip_create",
+        "yasfhip_destroy", "yasfhip_set_tmatrix", "yasfhip_coupling_matvec", "yasfhip_master_matvec",
+        "yasfhip_gmres_solve", "yasfhip_precond_build", "yasfhip_precond_apply",
+        "yasfhip_time_matvec",
+    ]
+
+
+def _check(status: int):
+    if status == 0:
+        return
+    msg = _load().yasfhip_last_error().decode(errors="replace")
+    if status == 1:
+        raise ValueError(msg)
+    if status == 3:
+        raise RuntimeError(msg)
+    if status == 2:
+        raise DeviceError(msg)
+    raise NativeLibraryError(msg)
+
+
+def _ptr(a: np.ndarray):
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def translation_table(lmax: int) -> np.ndarray:
+    """ab5[j1, j2, p] as complex128 ``(nmax, nmax, 2 lmax + 1)``."""
+    nmax = 2 * lmax * (lmax + 2)
+    out = np.zeros((nmax, nmax, 2 * lmax + 1), dtype=np.complex128)
+    _check(_load().yasfhip_translation_table(int(lmax), _ptr(out)))
+    return out
+
+
+def plan_stats(lmax: int) -> tuple[int, int]:
+    """(complex MACs per particle pair issued by the pair kernel, nnz(ab5))."""
+    a = ctypes.c_longlong()
+    b = ctypes.c_longlong()
+    _check(_load().yasfhip_plan_stats(int(lmax), ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+class Session:
+    """One device session: geometry, expansion order and channels.
+
+    Owns the device buffers; ``close()`` (or garbage collection) frees them.
+    """
+
+    def __init__(self, positions: np.ndarray, lmax: int, k_medium):
+        lib = _load()
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        if pos.ndim != 2 or pos.shape[1] != 3:
+            raise ValueError("positions must have shape (N, 3)")
+        k = np.atleast_1d(np.asarray(k_medium))
+        if np.iscomplexobj(k):
+            if np.any(k.imag != 0):
+                raise NotImplementedError("the hip backend supports real medium wave numbers only")
+            k = k.real
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        self.N = pos.shape[0]
+        self.lmax = int(lmax)
+        self.nmax = 2 * self.lmax * (self.lmax + 2)
+        self.n = self.N * self.nmax
+        self.channels = k.size
+        h = ctypes.c_void_p()
+        _check(lib.yasfhip_create(_ptr(pos), self.N, self.lmax, _ptr(k), self.channels, ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.yasfhip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _vec(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex128).ravel()
+        if x.size != self.n:
+            raise ValueError(f"vector has {x.size} entries, expected {self.n}")
+        return x
+
+    def set_tmatrix(self, t: np.ndarray):
+        t = np.ascontiguousarray(t, dtype=np.complex128).reshape(self.channels, self.n)
+        _check(self._lib.yasfhip_set_tmatrix(self._h, _ptr(t)))
+
+    def coupling_matvec(self, x, channel: int = 0, precision: int = FP64) -> np.ndarray:
+        x = self._vec(x)
+        y = np.empty_like(x)
+        _check(self._lib.yasfhip_coupling_matvec(self._h, int(channel), _ptr(x), _ptr(y), int(precision)))
+        return y
+
+    def master_matvec(self, x, channel: int = 0, precision: int = FP64) -> np.ndarray:
+        x = self._vec(x)
+        y = np.empty_like(x)
+        _check(self._lib.yasfhip_master_matvec(self._h, int(channel), _ptr(x), _ptr(y), int(precision)))
+        return y
+
+    def gmres(self, b, x0=None, channel: int = 0, rtol: float = 1e-6, restart: int = 20,
+              maxiter: int = 1000, use_precond: bool = False, history_cap: int = 100000):
+        """Returns (x, info, inner_iterations, relative_residual, pr_norm history)."""
+        b = self._vec(b)
+        x = self._vec(b if x0 is None else x0).copy()
+        info = ctypes.c_int()
+        iters = ctypes.c_int()
+        res = ctypes.c_double()
+        hist = np.zeros(max(1, int(history_cap)), dtype=np.float64)
+        _check(self._lib.yasfhip_gmres_solve(
+            self._h, int(channel), _ptr(b), _ptr(x), float(rtol), int(restart), int(maxiter),
+            int(bool(use_precond)), ctypes.byref(info), ctypes.byref(iters), ctypes.byref(res),
+            _ptr(hist), hist.size))
+        return x, info.value, iters.value, res.value, hist[: min(iters.value, hist.size)].copy()
+
+    def precond_build(self, box_of_particle, channel: int = 0):
+        box = np.ascontiguousarray(box_of_particle, dtype=np.int32)
+        if box.size != self.N:
+            raise ValueError("box_of_particle must have one entry per particle")
+        nbox = int(box.max()) + 1 if box.size else 0
+        _check(self._lib.yasfhip_precond_build(self._h, int(channel), _ptr(box), nbox))
+
+    def precond_apply(self, r) -> np.ndarray:
+        r = self._vec(r)
+        z = np.empty_like(r)
+        _check(self._lib.yasfhip_precond_apply(self._h, _ptr(r), _ptr(z)))
+        return z
+
+    def time_matvec(self, channel: int = 0, precision: int = FP64, reps: int = 5) -> tuple[float, float]:
+        """Mean (pair-kernel ms, whole-matvec ms) over `reps` device-resident matvecs."""
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        _check(self._lib.yasfhip_time_matvec(self._h, int(channel), int(precision), int(reps),
+                                             ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value

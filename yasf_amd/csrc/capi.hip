@@ -1,0 +1,607 @@
+// C ABI (include/yasfhip.h): session state, device buffers and the GMRES
+// driver.  The GMRES control flow restates SciPy's gmres
+// (scipy/sparse/linalg/_isolve/iterative.py:692-840, the solver the reference
+// calls from yasfpy/solver.py:108-121) with every vector operation on the
+// device; only the (restart+1)-sized Hessenberg/Givens work runs on the host.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "../../include/yasfhip.h"
+#include "internal.hpp"
+
+namespace yasfhip {
+
+// coupling.hip
+struct DevicePlan {
+    int lmax, nmax, maxcols, nq, njobs;
+    PairJob *jobs;
+    int *colmap, *crow, *cidx;
+    double2 *cval;
+    double *la, *lb, *ld;
+};
+void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
+                 void *Q, hipStream_t s);
+void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
+                     int mode, void *y, bool fp32, hipStream_t s);
+// krylov.hip
+struct Reducer {
+    double *partials;
+    unsigned *counter;
+};
+void launch_mgs_step(double2 *w, const double2 *v_prev, const double2 *h_prev, const double2 *v_next,
+                     long n, Reducer red, double *out, hipStream_t s);
+void launch_scale(double2 *y, const double2 *x, double a, long n, hipStream_t s);
+void launch_update(double2 *x, const double2 *V, const double2 *y, int m, long n, long ld,
+                   hipStream_t s);
+void launch_residual(double2 *r, const double2 *b, const double2 *ax, long n, Reducer red, double *out,
+                     hipStream_t s);
+void launch_norm2(const double2 *x, long n, Reducer red, double *out, hipStream_t s);
+void launch_to_fp32(float *dst, const double *src, long count, hipStream_t s);
+void launch_to_fp64(double *dst, const float *src, long count, hipStream_t s);
+// precond.hip
+struct BlockPrecond;
+BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &ab5,
+                             const double *pos_host, int N, const int *box_of_particle, int num_boxes,
+                             double k, const double2 *t_dev, hipStream_t s);
+void precond_destroy(BlockPrecond *p);
+void precond_apply(BlockPrecond *p, const double2 *r, double2 *z, hipStream_t s);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename T>
+T *dmalloc(size_t count) {
+    void *p = nullptr;
+    if (count == 0)
+        count = 1;
+    YH_CHECK(hipMalloc(&p, count * sizeof(T)));
+    return static_cast<T *>(p);
+}
+
+template <typename T>
+T *upload(const std::vector<T> &v, hipStream_t s) {
+    T *d = dmalloc<T>(v.size());
+    if (!v.empty())
+        YH_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return d;
+}
+
+}  // namespace
+
+}  // namespace yasfhip
+
+using namespace yasfhip;
+
+struct yasfhip_session {
+    int N = 0, L = 0, nmax = 0, W = 0, Npad = 0, ntiles1 = 0;
+    long n = 0;
+    std::vector<double> k;
+    std::vector<double> pos_host;
+    hipStream_t stream = nullptr;
+    CouplingPlan plan;
+    DevicePlan dp{};
+    double4 *pos = nullptr;
+    void *Q = nullptr;  // pair-GEMM output, sized for complex128
+    double2 *t = nullptr;
+    bool t_set = false;
+    double2 *xin = nullptr, *yout = nullptr;
+    float2 *x32 = nullptr, *y32 = nullptr;
+    // GMRES workspace
+    int vcap = 0;
+    double2 *V = nullptr, *bvec = nullptr, *xvec = nullptr, *rvec = nullptr, *avec = nullptr,
+            *ycoef = nullptr;
+    double *scal = nullptr, *partials = nullptr;
+    unsigned *counter = nullptr;
+    double *scal_host = nullptr;
+    double *hdev = nullptr, *hhost = nullptr;  // per-step MGS results, 4 doubles per slot
+    BlockPrecond *precond = nullptr;
+    int precond_channel = -1;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+
+    ~yasfhip_session() {
+        if (precond)
+            precond_destroy(precond);
+        void *bufs[] = {pos,  Q,    t,    xin,   yout, x32,      y32,     V,    bvec,
+                        xvec, rvec, avec, ycoef, scal, partials, counter, hdev};
+        for (void *b : bufs)
+            if (b)
+                (void)hipFree(b);
+        void *plan_bufs[] = {dp.jobs, dp.colmap, dp.crow, dp.cidx, dp.cval, dp.la, dp.lb, dp.ld};
+        for (void *b : plan_bufs)
+            if (b)
+                (void)hipFree(b);
+        if (scal_host)
+            (void)hipHostFree(scal_host);
+        if (hhost)
+            (void)hipHostFree(hhost);
+        for (auto &e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    Reducer reducer() const { return Reducer{partials, counter}; }
+
+    // y = W x (mode 0) or x - t W x (mode 1) for device vectors of complex128.
+    void matvec_dev(int channel, const double2 *x, double2 *y, int mode, bool fp32) {
+        const double kk = k[channel];
+        const double2 *tt = t + static_cast<size_t>(channel) * n;
+        if (!fp32) {
+            launch_pair(dp, pos, N, kk, x, false, Q, stream);
+            launch_contract(dp, N, Q, x, tt, mode, y, false, stream);
+        } else {
+            launch_to_fp32(reinterpret_cast<float *>(x32), reinterpret_cast<const double *>(x), 2 * n,
+                           stream);
+            launch_pair(dp, pos, N, kk, x32, true, Q, stream);
+            launch_contract(dp, N, Q, x32, tt, mode, y32, true, stream);
+            launch_to_fp64(reinterpret_cast<double *>(y), reinterpret_cast<const float *>(y32), 2 * n,
+                           stream);
+        }
+    }
+
+    void ensure_gmres(int restart) {
+        if (vcap >= restart + 1)
+            return;
+        if (V)
+            YH_CHECK(hipFree(V));
+        if (ycoef)
+            YH_CHECK(hipFree(ycoef));
+        if (hdev)
+            YH_CHECK(hipFree(hdev));
+        if (hhost)
+            YH_CHECK(hipHostFree(hhost));
+        V = dmalloc<double2>(static_cast<size_t>(restart + 1) * n);
+        ycoef = dmalloc<double2>(restart + 1);
+        hdev = dmalloc<double>(4 * static_cast<size_t>(restart + 2));
+        YH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&hhost), 4 * (restart + 2) * sizeof(double),
+                               hipHostMallocDefault));
+        vcap = restart + 1;
+    }
+
+    // Host-visible reduction results after a stream sync.
+    void fetch_scalars(int count) {
+        YH_CHECK(hipMemcpyAsync(scal_host, scal, count * sizeof(double), hipMemcpyDeviceToHost, stream));
+        YH_CHECK(hipStreamSynchronize(stream));
+    }
+
+    double norm_dev(const double2 *x) {
+        launch_norm2(x, n, reducer(), scal, stream);
+        fetch_scalars(1);
+        return std::sqrt(scal_host[0]);
+    }
+
+    void psolve(const double2 *r, double2 *z, bool use_precond) {
+        if (use_precond)
+            precond_apply(precond, r, z, stream);
+        else if (z != r)
+            YH_CHECK(hipMemcpyAsync(z, r, n * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+    }
+};
+
+namespace {
+
+int fail(const yasfhip::Error &e) {
+    g_last_error = e.what();
+    return e.code;
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        g_last_error.clear();
+        return kOk;
+    } catch (const yasfhip::Error &e) {
+        return fail(e);
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return kInternal;
+    }
+}
+
+// LAPACK zlartg semantics: [c s; -conj(s) c] [f; g] = [r; 0], c real.
+void lartg(cplx f, cplx g, double &c, cplx &s, cplx &r) {
+    if (g == cplx(0, 0)) {
+        c = 1.0;
+        s = cplx(0, 0);
+        r = f;
+        return;
+    }
+    if (f == cplx(0, 0)) {
+        c = 0.0;
+        const double ag = std::abs(g);
+        s = std::conj(g) / ag;
+        r = cplx(ag, 0.0);
+        return;
+    }
+    const double af = std::abs(f), ag = std::abs(g);
+    const double d = std::sqrt(af * af + ag * ag);
+    c = af / d;
+    const cplx sgn = f / af;
+    s = sgn * std::conj(g) / d;
+    r = sgn * d;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *yasfhip_last_error(void) { return g_last_error.c_str(); }
+
+int yasfhip_translation_table(int lmax, double *out) {
+    return guarded([&] {
+        YH_REQUIRE(out, "out is NULL");
+        YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
+        const auto ab5 = translation_table(lmax);
+        std::memcpy(out, ab5.data(), ab5.size() * sizeof(cplx));
+    });
+}
+
+int yasfhip_plan_stats(int lmax, long long *pair_macs_per_pair, long long *nnz_ab5) {
+    return guarded([&] {
+        const auto plan = build_coupling_plan(lmax);
+        if (pair_macs_per_pair)
+            *pair_macs_per_pair = plan.pair_macs_per_pair;
+        if (nnz_ab5)
+            *nnz_ab5 = plan.nnz_ab5;
+    });
+}
+
+int yasfhip_create(const double *positions, int particle_number, int lmax, const double *k_medium,
+                   int channels, yasfhip_session **out) {
+    return guarded([&] {
+        YH_REQUIRE(out, "out is NULL");
+        YH_REQUIRE(positions && k_medium, "positions/k_medium is NULL");
+        YH_REQUIRE(particle_number >= 1, "particle_number must be >= 1");
+        YH_REQUIRE(channels >= 1, "channels must be >= 1");
+        for (int w = 0; w < channels; ++w)
+            YH_REQUIRE(std::isfinite(k_medium[w]) && k_medium[w] > 0.0,
+                       "k_medium must be real, finite and positive");
+        auto s = std::make_unique<yasfhip_session>();
+        s->N = particle_number;
+        s->L = lmax;
+        s->W = channels;
+        s->plan = build_coupling_plan(lmax);
+        s->nmax = s->plan.nmax;
+        s->n = static_cast<long>(s->N) * s->nmax;
+        s->ntiles1 = (s->N + 15) / 16;
+        s->Npad = s->ntiles1 * 16;
+        s->k.assign(k_medium, k_medium + channels);
+        s->pos_host.assign(positions, positions + 3 * static_cast<size_t>(particle_number));
+        YH_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        for (auto &e : s->ev)
+            YH_CHECK(hipEventCreate(&e));
+        hipStream_t st = s->stream;
+
+        std::vector<double4> pos(s->Npad, make_double4(0, 0, 0, 0));
+        for (int i = 0; i < s->N; ++i)
+            pos[i] = make_double4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 0.0);
+        s->pos = upload(pos, st);
+
+        const auto &pl = s->plan;
+        s->dp.lmax = pl.lmax;
+        s->dp.nmax = pl.nmax;
+        s->dp.maxcols = pl.maxcols;
+        s->dp.nq = pl.nq;
+        s->dp.njobs = static_cast<int>(pl.jobs.size());
+        s->dp.jobs = upload(pl.jobs, st);
+        s->dp.colmap = upload(pl.colmap, st);
+        s->dp.crow = upload(pl.crow, st);
+        s->dp.cidx = upload(pl.cidx, st);
+        std::vector<double2> cval(pl.cval.size() / 2);
+        for (size_t i = 0; i < cval.size(); ++i)
+            cval[i] = make_double2(pl.cval[2 * i], pl.cval[2 * i + 1]);
+        s->dp.cval = upload(cval, st);
+        s->dp.la = upload(pl.leg_alpha, st);
+        s->dp.lb = upload(pl.leg_beta, st);
+        s->dp.ld = upload(pl.leg_diag, st);
+
+        s->Q = dmalloc<double2>(static_cast<size_t>(s->ntiles1) * pl.nq * 256);
+        s->t = dmalloc<double2>(static_cast<size_t>(s->W) * s->n);
+        s->xin = dmalloc<double2>(s->n);
+        s->yout = dmalloc<double2>(s->n);
+        s->x32 = dmalloc<float2>(s->n);
+        s->y32 = dmalloc<float2>(s->n);
+        s->bvec = dmalloc<double2>(s->n);
+        s->xvec = dmalloc<double2>(s->n);
+        s->rvec = dmalloc<double2>(s->n);
+        s->avec = dmalloc<double2>(s->n);
+        s->scal = dmalloc<double>(8);
+        s->partials = dmalloc<double>(512 * 4);
+        s->counter = dmalloc<unsigned>(1);
+        YH_CHECK(hipMemsetAsync(s->counter, 0, sizeof(unsigned), st));
+        YH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&s->scal_host), 8 * sizeof(double),
+                               hipHostMallocDefault));
+        YH_CHECK(hipStreamSynchronize(st));
+        *out = s.release();
+    });
+}
+
+int yasfhip_destroy(yasfhip_session *s) {
+    return guarded([&] { delete s; });
+}
+
+int yasfhip_set_tmatrix(yasfhip_session *s, const double *t) {
+    return guarded([&] {
+        YH_REQUIRE(s && t, "NULL argument");
+        YH_CHECK(hipMemcpyAsync(s->t, t, static_cast<size_t>(s->W) * s->n * sizeof(double2),
+                                hipMemcpyHostToDevice, s->stream));
+        YH_CHECK(hipStreamSynchronize(s->stream));
+        s->t_set = true;
+    });
+}
+
+static void host_matvec(yasfhip_session *s, int channel, const double *x, double *y, int precision,
+                        int mode) {
+    YH_REQUIRE(s && x && y, "NULL argument");
+    YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
+    YH_REQUIRE(precision == YASFHIP_FP64 || precision == YASFHIP_FP32, "unknown precision");
+    if (mode == 1)
+        YH_REQUIRE(s->t_set, "T-matrix not set (yasfhip_set_tmatrix)");
+    YH_CHECK(hipMemcpyAsync(s->xin, x, s->n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+    s->matvec_dev(channel, s->xin, s->yout, mode, precision == YASFHIP_FP32);
+    YH_CHECK(hipMemcpyAsync(y, s->yout, s->n * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+    YH_CHECK(hipStreamSynchronize(s->stream));
+}
+
+int yasfhip_coupling_matvec(yasfhip_session *s, int channel, const double *x, double *y,
+                            int precision) {
+    return guarded([&] { host_matvec(s, channel, x, y, precision, 0); });
+}
+
+int yasfhip_master_matvec(yasfhip_session *s, int channel, const double *x, double *y,
+                          int precision) {
+    return guarded([&] { host_matvec(s, channel, x, y, precision, 1); });
+}
+
+int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double *x_inout,
+                        double rtol, int restart, int maxiter, int use_precond, int *info,
+                        int *iterations, double *residual, double *history, int history_cap) {
+    return guarded([&] {
+        YH_REQUIRE(s && b && x_inout, "NULL argument");
+        YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
+        YH_REQUIRE(s->t_set, "T-matrix not set (yasfhip_set_tmatrix)");
+        YH_REQUIRE(rtol >= 0.0, "rtol must be >= 0");
+        YH_REQUIRE(maxiter >= 1, "maxiter must be >= 1");
+        if (use_precond)
+            YH_REQUIRE(s->precond && s->precond_channel == channel,
+                       "preconditioner not prepared for this channel");
+        const long n = s->n;
+        if (restart <= 0)
+            restart = 20;
+        restart = static_cast<int>(std::min<long>(restart, n));
+        s->ensure_gmres(restart);
+        hipStream_t st = s->stream;
+        const bool pc = use_precond != 0;
+        double2 *x = s->xvec, *bb = s->bvec, *r = s->rvec, *av = s->avec, *V = s->V;
+        YH_CHECK(hipMemcpyAsync(bb, b, n * sizeof(double2), hipMemcpyHostToDevice, st));
+        YH_CHECK(hipMemcpyAsync(x, x_inout, n * sizeof(double2), hipMemcpyHostToDevice, st));
+        int inner_iter = 0, nhist = 0, out_info = 0;
+        double rnorm = 0.0;
+        const double bnrm2 = s->norm_dev(bb);
+        auto finish = [&](double rn) {
+            YH_CHECK(hipMemcpyAsync(x_inout, x, n * sizeof(double2), hipMemcpyDeviceToHost, st));
+            YH_CHECK(hipStreamSynchronize(st));
+            if (info)
+                *info = out_info;
+            if (iterations)
+                *iterations = inner_iter;
+            if (residual)
+                *residual = bnrm2 > 0 ? rn / bnrm2 : 0.0;
+        };
+        if (bnrm2 == 0.0) {  // SciPy returns b
+            YH_CHECK(hipMemcpyAsync(x, bb, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+            finish(0.0);
+            return;
+        }
+        const double atol = rtol * bnrm2;
+        const double eps = DBL_EPSILON;
+        s->psolve(bb, av, pc);
+        const double Mb_nrm2 = s->norm_dev(av);
+        double ptol_max_factor = 1.0;
+        double ptol = Mb_nrm2 * std::min(ptol_max_factor, atol / bnrm2);
+        double presid = 0.0;
+        std::vector<cplx> h(static_cast<size_t>(restart) * (restart + 1));
+        std::vector<double> gc(restart);
+        std::vector<cplx> gs(restart), S(restart + 1), y(restart);
+        auto H = [&](int col, int k) -> cplx & { return h[static_cast<size_t>(col) * (restart + 1) + k]; };
+
+        for (int iteration = 0; iteration < maxiter; ++iteration) {
+            if (iteration == 0) {
+                // r = b - A x if x.any() else b
+                launch_norm2(x, n, s->reducer(), s->scal, st);
+                s->fetch_scalars(1);
+                if (s->scal_host[0] != 0.0) {
+                    s->matvec_dev(channel, x, av, 1, false);
+                    launch_residual(r, bb, av, n, s->reducer(), s->scal, st);
+                    s->fetch_scalars(1);
+                } else {
+                    YH_CHECK(hipMemcpyAsync(r, bb, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+                    launch_norm2(r, n, s->reducer(), s->scal, st);
+                    s->fetch_scalars(1);
+                }
+                rnorm = std::sqrt(s->scal_host[0]);
+                if (rnorm < atol) {
+                    out_info = 0;
+                    finish(rnorm);
+                    return;
+                }
+            }
+            s->psolve(r, V, pc);
+            const double tmp0 = s->norm_dev(V);
+            launch_scale(V, V, 1.0 / tmp0, n, st);
+            std::fill(S.begin(), S.end(), cplx(0, 0));
+            S[0] = tmp0;
+            bool breakdown = false;
+            int col = 0;
+            for (col = 0; col < restart; ++col) {
+                double2 *vc = V + static_cast<size_t>(col) * n;
+                double2 *w = V + static_cast<size_t>(col + 1) * n;
+                if (pc) {
+                    s->matvec_dev(channel, vc, av, 1, false);
+                    s->psolve(av, w, true);
+                } else {
+                    s->matvec_dev(channel, vc, w, 1, false);
+                }
+                // Modified Gram-Schmidt, one fused launch per basis vector:
+                // launch kq subtracts h[kq-1] v[kq-1] and forms <v[kq], w>;
+                // slot kq of hdev holds (Re, Im, ||w||^2) of launch kq.
+                for (int kq = 0; kq <= col; ++kq) {
+                    const double2 *vk = V + static_cast<size_t>(kq) * n;
+                    const double2 *vp = kq > 0 ? V + static_cast<size_t>(kq - 1) * n : nullptr;
+                    const double2 *hp =
+                        kq > 0 ? reinterpret_cast<const double2 *>(s->hdev + 4 * (kq - 1)) : nullptr;
+                    launch_mgs_step(w, vp, hp, vk, n, s->reducer(), s->hdev + 4 * kq, st);
+                }
+                launch_mgs_step(w, vc, reinterpret_cast<const double2 *>(s->hdev + 4 * col), nullptr, n,
+                                s->reducer(), s->hdev + 4 * (col + 1), st);
+                YH_CHECK(hipMemcpyAsync(s->hhost, s->hdev, 4 * (col + 2) * sizeof(double),
+                                        hipMemcpyDeviceToHost, st));
+                YH_CHECK(hipStreamSynchronize(st));
+                const double h0 = std::sqrt(s->hhost[2]);
+                for (int kq = 0; kq <= col; ++kq)
+                    H(col, kq) = cplx(s->hhost[4 * kq], s->hhost[4 * kq + 1]);
+                const double h1 = std::sqrt(s->hhost[4 * (col + 1) + 2]);
+                H(col, col + 1) = h1;
+                if (h1 <= eps * h0) {
+                    H(col, col + 1) = 0.0;
+                    breakdown = true;
+                } else {
+                    launch_scale(w, w, 1.0 / h1, n, st);
+                }
+                for (int kq = 0; kq < col; ++kq) {
+                    const double c = gc[kq];
+                    const cplx sn = gs[kq];
+                    const cplx n0 = H(col, kq), n1 = H(col, kq + 1);
+                    H(col, kq) = c * n0 + sn * n1;
+                    H(col, kq + 1) = -std::conj(sn) * n0 + c * n1;
+                }
+                double c;
+                cplx sn, mag;
+                lartg(H(col, col), H(col, col + 1), c, sn, mag);
+                gc[col] = c;
+                gs[col] = sn;
+                H(col, col) = mag;
+                H(col, col + 1) = 0.0;
+                const cplx tmp = -std::conj(sn) * S[col];
+                S[col] = c * S[col];
+                S[col + 1] = tmp;
+                presid = std::abs(tmp);
+                ++inner_iter;
+                if (history && nhist < history_cap)
+                    history[nhist++] = presid / bnrm2;
+                if (presid <= ptol || breakdown)
+                    break;
+            }
+            if (col == restart)
+                col = restart - 1;
+            if (H(col, col) == cplx(0, 0))
+                S[col] = 0.0;
+            for (int kq = 0; kq <= col; ++kq)
+                y[kq] = S[kq];
+            for (int kq = col; kq > 0; --kq) {
+                if (y[kq] != cplx(0, 0)) {
+                    y[kq] /= H(kq, kq);
+                    const cplx tk = y[kq];
+                    for (int i = 0; i < kq; ++i)
+                        y[i] -= tk * H(kq, i);
+                }
+            }
+            if (y[0] != cplx(0, 0))
+                y[0] /= H(0, 0);
+            std::vector<double2> ydev(col + 1);
+            for (int kq = 0; kq <= col; ++kq)
+                ydev[kq] = make_double2(y[kq].real(), y[kq].imag());
+            YH_CHECK(hipMemcpyAsync(s->ycoef, ydev.data(), (col + 1) * sizeof(double2),
+                                    hipMemcpyHostToDevice, st));
+            launch_update(x, V, s->ycoef, col + 1, n, n, st);
+            s->matvec_dev(channel, x, av, 1, false);
+            launch_residual(r, bb, av, n, s->reducer(), s->scal, st);
+            s->fetch_scalars(1);
+            rnorm = std::sqrt(s->scal_host[0]);
+            if (rnorm <= atol)
+                break;
+            else if (breakdown)
+                break;
+            else if (presid <= ptol)
+                ptol_max_factor = std::max(eps, 0.25 * ptol_max_factor);
+            else
+                ptol_max_factor = std::min(1.0, 1.5 * ptol_max_factor);
+            ptol = presid * std::min(ptol_max_factor, atol / rnorm);
+        }
+        out_info = (rnorm <= atol) ? 0 : maxiter;
+        finish(rnorm);
+    });
+}
+
+int yasfhip_precond_build(yasfhip_session *s, int channel, const int *box_of_particle,
+                          int num_boxes) {
+    return guarded([&] {
+        YH_REQUIRE(s && box_of_particle, "NULL argument");
+        YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
+        YH_REQUIRE(s->t_set, "T-matrix not set (yasfhip_set_tmatrix)");
+        YH_REQUIRE(num_boxes >= 1 && num_boxes <= s->N, "num_boxes out of range");
+        for (int i = 0; i < s->N; ++i)
+            YH_REQUIRE(box_of_particle[i] >= 0 && box_of_particle[i] < num_boxes,
+                       "box index out of range");
+        if (s->precond) {
+            precond_destroy(s->precond);
+            s->precond = nullptr;
+            s->precond_channel = -1;
+        }
+        s->precond = precond_create(s->plan, translation_table(s->L), s->pos_host.data(), s->N,
+                                    box_of_particle, num_boxes, s->k[channel],
+                                    s->t + static_cast<size_t>(channel) * s->n, s->stream);
+        s->precond_channel = channel;
+    });
+}
+
+int yasfhip_precond_apply(yasfhip_session *s, const double *r, double *z) {
+    return guarded([&] {
+        YH_REQUIRE(s && r && z, "NULL argument");
+        YH_REQUIRE(s->precond, "preconditioner not prepared (yasfhip_precond_build)");
+        YH_CHECK(hipMemcpyAsync(s->xin, r, s->n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+        precond_apply(s->precond, s->xin, s->yout, s->stream);
+        YH_CHECK(hipMemcpyAsync(z, s->yout, s->n * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+        YH_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps, double *ms_pair,
+                        double *ms_total) {
+    return guarded([&] {
+        YH_REQUIRE(s && ms_pair && ms_total, "NULL argument");
+        YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
+        YH_REQUIRE(reps >= 1, "reps must be >= 1");
+        const bool fp32 = precision == YASFHIP_FP32;
+        hipStream_t st = s->stream;
+        double pair_ms = 0.0, total_ms = 0.0;
+        if (fp32)
+            launch_to_fp32(reinterpret_cast<float *>(s->x32), reinterpret_cast<const double *>(s->xin),
+                           2 * s->n, st);
+        for (int r = 0; r < reps; ++r) {
+            YH_CHECK(hipEventRecord(s->ev[0], st));
+            const void *xv = fp32 ? static_cast<const void *>(s->x32) : static_cast<const void *>(s->xin);
+            launch_pair(s->dp, s->pos, s->N, s->k[channel], xv, fp32, s->Q, st);
+            YH_CHECK(hipEventRecord(s->ev[1], st));
+            launch_contract(s->dp, s->N, s->Q, xv, s->t + static_cast<size_t>(channel) * s->n, 0,
+                            fp32 ? static_cast<void *>(s->y32) : static_cast<void *>(s->yout), fp32, st);
+            YH_CHECK(hipEventRecord(s->ev[2], st));
+            YH_CHECK(hipEventSynchronize(s->ev[2]));
+            float a = 0.f, b = 0.f;
+            YH_CHECK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+            YH_CHECK(hipEventElapsedTime(&b, s->ev[0], s->ev[2]));
+            pair_ms += a;
+            total_ms += b;
+        }
+        *ms_pair = pair_ms / reps;
+        *ms_total = total_ms / reps;
+    });
+}
+
+}  // extern "C"

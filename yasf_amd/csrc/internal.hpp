@@ -1,0 +1,86 @@
+// Internal declarations shared by the yasfhip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace yasfhip {
+
+using cplx = std::complex<double>;
+
+// Largest expansion order the device kernels are instantiated for.
+constexpr int kMaxLmax = 8;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+// Status codes returned through the C ABI (see include/yasfhip.h).
+enum Status : int {
+    kOk = 0,
+    kInvalidArgument = 1,
+    kDeviceError = 2,
+    kNotPrepared = 3,
+    kInternal = 4,
+};
+
+#define YH_CHECK(expr)                                                                   \
+    do {                                                                                 \
+        hipError_t yh_e_ = (expr);                                                       \
+        if (yh_e_ != hipSuccess)                                                         \
+            throw ::yasfhip::Error(::yasfhip::kDeviceError,                              \
+                                   std::string(#expr) + ": " + hipGetErrorString(yh_e_)); \
+    } while (0)
+
+#define YH_REQUIRE(cond, msg)                                                            \
+    do {                                                                                 \
+        if (!(cond))                                                                     \
+            throw ::yasfhip::Error(::yasfhip::kInvalidArgument, msg);                    \
+    } while (0)
+
+// ---------------------------------------------------------------- host tables
+
+inline int nmax_of(int lmax) { return 2 * lmax * (lmax + 2); }
+
+struct Mode {
+    int tau, l, m;
+};
+std::vector<Mode> modes_of(int lmax);
+
+// ab5[j1][j2][p] (reference yasfpy/numerics.py:224-272), row-major complex.
+std::vector<cplx> translation_table(int lmax);
+
+// One wave's share of the pair GEMM: for one signed dm, expansion orders
+// p in [p_lo, p_lo+np) and column tiles [tile0, tile0+ntile) of the dm's
+// compatible source columns.  Its C tiles are written to Q slots
+// qbase + ip*ntile + it of the target tile's Q block.
+struct PairJob {
+    int dm, p_lo, np, tile0, ntile, qbase, cost, pad;
+};
+
+constexpr int kJobMaxP = 4;
+constexpr int kJobMaxTiles = 2;
+
+// Everything the coupling kernels need for one lmax.
+struct CouplingPlan {
+    int lmax = 0, nmax = 0, ndm = 0, maxcols = 0, nq = 0;
+    std::vector<int> colmap;          // [ndm][maxcols] -> source mode n2 or -1
+    std::vector<PairJob> jobs;        // per 16-particle target tile, sorted by cost
+    std::vector<int> crow;            // CSR row offsets per target mode n1 (nmax+1)
+    std::vector<int> cidx;            // Q column index q*16 + col
+    std::vector<double> cval;         // ab5 entry (re, im) interleaved
+    std::vector<double> leg_alpha;    // [(2L+1)^2] Legendre recurrence alpha(l, m)
+    std::vector<double> leg_beta;     // [(2L+1)^2] Legendre recurrence beta(l, m)
+    std::vector<double> leg_diag;     // [2L+1] P_m^m = leg_diag[m] * sin^m
+    long long pair_macs_per_pair = 0; // complex MACs the pair kernel issues per pair
+    long long nnz_ab5 = 0;
+};
+CouplingPlan build_coupling_plan(int lmax);
+
+}  // namespace yasfhip

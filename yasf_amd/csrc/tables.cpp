@@ -1,0 +1,224 @@
+// Host-side tables for the coupling operator: mode indexing, the Wigner-3j
+// translation table ab5, Legendre recurrence coefficients and the pair-GEMM
+// work plan with its ab5 contraction lists.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <map>
+#include <tuple>
+
+#include "internal.hpp"
+
+namespace yasfhip {
+
+// Mode order n = (tau-1) L(L+2) + l^2 - 1 + l + m
+// (reference yasfpy/functions/cpu_numba.py:395-405, misc.py:23-43).
+std::vector<Mode> modes_of(int lmax) {
+    std::vector<Mode> out;
+    for (int tau = 1; tau <= 2; ++tau)
+        for (int l = 1; l <= lmax; ++l)
+            for (int m = -l; m <= l; ++m)
+                out.push_back({tau, l, m});
+    return out;
+}
+
+namespace {
+
+long double factorial(int n) {
+    static std::vector<long double> table = [] {
+        std::vector<long double> t(80);
+        t[0] = 1.0L;
+        for (int i = 1; i < 80; ++i)
+            t[i] = t[i - 1] * static_cast<long double>(i);
+        return t;
+    }();
+    if (n < 0 || n >= 80)
+        throw Error(kInternal, "factorial argument out of range");
+    return table[n];
+}
+
+// Wigner 3j symbol by Racah's single-sum formula, evaluated in long double.
+// The alternating sum is accumulated from its largest-index term downwards.
+long double wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+    if (m1 + m2 + m3 != 0)
+        return 0.0L;
+    if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(m3) > j3)
+        return 0.0L;
+    if (j3 < std::abs(j1 - j2) || j3 > j1 + j2)
+        return 0.0L;
+    const int kmin = std::max({0, j2 - j3 - m1, j1 - j3 + m2});
+    const int kmax = std::min({j1 + j2 - j3, j1 - m1, j2 + m2});
+    long double sum = 0.0L, mag = 0.0L;
+    for (int k = kmax; k >= kmin; --k) {
+        const long double den = factorial(k) * factorial(j3 - j2 + k + m1) *
+                                factorial(j3 - j1 + k - m2) * factorial(j1 + j2 - j3 - k) *
+                                factorial(j1 - k - m1) * factorial(j2 - k + m2);
+        sum += ((k & 1) ? -1.0L : 1.0L) / den;
+        mag += 1.0L / den;
+    }
+    // Non-trivial zeros of the 3j symbol cancel exactly in exact arithmetic;
+    // in long double they leave a residue of a few ulps of the term sum.
+    if (std::fabs(sum) <= 256.0L * LDBL_EPSILON * mag)
+        return 0.0L;
+    const long double delta = factorial(j1 + j2 - j3) * factorial(j1 - j2 + j3) *
+                              factorial(-j1 + j2 + j3) / factorial(j1 + j2 + j3 + 1);
+    const long double pref = factorial(j1 + m1) * factorial(j1 - m1) * factorial(j2 + m2) *
+                             factorial(j2 - m2) * factorial(j3 + m3) * factorial(j3 - m3);
+    const int e = j1 - j2 - m3;
+    const long double sign = (((e % 2) + 2) % 2) ? -1.0L : 1.0L;
+    return sign * std::sqrt(delta * pref) * sum;
+}
+
+}  // namespace
+
+// ab5[j1, j2, p] for (tau1,l1,m1) = modes[j1], (tau2,l2,m2) = modes[j2]
+// (reference yasfpy/numerics.py:224-272).  i^k is applied exactly.
+std::vector<cplx> translation_table(int lmax) {
+    const auto modes = modes_of(lmax);
+    const int nmax = static_cast<int>(modes.size());
+    const int P = 2 * lmax + 1;
+    std::vector<cplx> ab5(static_cast<size_t>(nmax) * nmax * P, cplx(0.0, 0.0));
+    const cplx ipow[4] = {cplx(1, 0), cplx(0, 1), cplx(-1, 0), cplx(0, -1)};
+    for (int j1 = 0; j1 < nmax; ++j1) {
+        const auto [tau1, l1, m1] = modes[j1];
+        for (int j2 = 0; j2 < nmax; ++j2) {
+            const auto [tau2, l2, m2] = modes[j2];
+            for (int p = 0; p < P; ++p) {
+                const int e = std::abs(m1 - m2) - std::abs(m1) - std::abs(m2) + l2 - l1 + p;
+                const cplx phase = ipow[((e % 4) + 4) % 4];
+                const double sgn = ((m1 - m2) % 2 != 0) ? -1.0 : 1.0;
+                const double norm = std::sqrt(static_cast<double>((2 * l1 + 1) * (2 * l2 + 1)) /
+                                              (2.0 * l1 * (l1 + 1) * l2 * (l2 + 1)));
+                double mag = 0.0;
+                if (tau1 == tau2) {
+                    const long double wa = wigner3j(l1, l2, p, m1, -m2, -m1 + m2);
+                    const long double wb = wigner3j(l1, l2, p, 0, 0, 0);
+                    mag = static_cast<double>(l1 * (l1 + 1) + l2 * (l2 + 1) - p * (p + 1)) *
+                          std::sqrt(2.0 * p + 1.0) * static_cast<double>(wa * wb);
+                } else if (p > 0) {
+                    const long rad = static_cast<long>(l1 + l2 + 1 + p) * (l1 + l2 + 1 - p) *
+                                     (p + l1 - l2) * (p - l1 + l2) * (2 * p + 1);
+                    const long double wa = wigner3j(l1, l2, p, m1, -m2, -m1 + m2);
+                    const long double wb = wigner3j(l1, l2, p - 1, 0, 0, 0);
+                    // A negative radicand only occurs where a 3j factor vanishes.
+                    if (rad > 0)
+                        mag = std::sqrt(static_cast<double>(rad)) * static_cast<double>(wa * wb);
+                }
+                ab5[(static_cast<size_t>(j1) * nmax + j2) * P + p] = phase * (sgn * norm * mag);
+            }
+        }
+    }
+    return ab5;
+}
+
+// Work plan of the factorised coupling matvec
+//   Q[s1][p][dm][n2] = sum_{s2 != s1} Ptilde_p^{|dm|}(cos th) h_p(k d) e^{i dm phi} x[s2][n2]
+//   (W x)[s1][n1]   = sum_{p, n2} ab5[n2][n1][p] Q[s1][p][m2-m1][n2]
+// which regroups the reference's quadruple sum (cpu_numba.py:50-93) so the
+// pair-independent ab5 contraction leaves the O(N^2) loop.
+CouplingPlan build_coupling_plan(int lmax) {
+    YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
+    CouplingPlan plan;
+    plan.lmax = lmax;
+    plan.nmax = nmax_of(lmax);
+    plan.ndm = 4 * lmax + 1;
+    const auto modes = modes_of(lmax);
+    const int P = 2 * lmax + 1;
+
+    // Source columns compatible with each signed dm: |m2 - dm| <= L.
+    std::vector<std::vector<int>> cols(plan.ndm);
+    for (int dmi = 0; dmi < plan.ndm; ++dmi) {
+        const int dm = dmi - 2 * lmax;
+        for (int n2 = 0; n2 < plan.nmax; ++n2)
+            if (std::abs(modes[n2].m - dm) <= lmax)
+                cols[dmi].push_back(n2);
+    }
+    int maxcols = 0;
+    for (auto &c : cols)
+        maxcols = std::max<int>(maxcols, ((static_cast<int>(c.size()) + 15) / 16) * 16);
+    plan.maxcols = maxcols;
+    plan.colmap.assign(static_cast<size_t>(plan.ndm) * maxcols, -1);
+    for (int dmi = 0; dmi < plan.ndm; ++dmi)
+        for (size_t c = 0; c < cols[dmi].size(); ++c)
+            plan.colmap[static_cast<size_t>(dmi) * maxcols + c] = cols[dmi][c];
+
+    // Jobs: (dm, p-chunk <= kJobMaxP, tile-chunk <= kJobMaxTiles).
+    // slot[(dmi, p, col)] -> Q column index q*16 + col%16.
+    std::map<std::tuple<int, int, int>, int> slot;
+    int q = 0;
+    long long macs = 0;
+    for (int dmi = 0; dmi < plan.ndm; ++dmi) {
+        const int dm = dmi - 2 * lmax;
+        const int ntiles = (static_cast<int>(cols[dmi].size()) + 15) / 16;
+        for (int p_lo = std::abs(dm); p_lo < P; p_lo += kJobMaxP) {
+            const int np = std::min(kJobMaxP, P - p_lo);
+            for (int t0 = 0; t0 < ntiles; t0 += kJobMaxTiles) {
+                const int nt = std::min(kJobMaxTiles, ntiles - t0);
+                PairJob job{dm, p_lo, np, t0, nt, q, np * nt, 0};
+                for (int ip = 0; ip < np; ++ip)
+                    for (int it = 0; it < nt; ++it) {
+                        for (int c = 0; c < 16; ++c)
+                            slot[{dmi, p_lo + ip, (t0 + it) * 16 + c}] = (q + ip * nt + it) * 16 + c;
+                    }
+                q += np * nt;
+                macs += static_cast<long long>(np) * nt * 16;
+                plan.jobs.push_back(job);
+            }
+        }
+    }
+    plan.nq = q;
+    plan.pair_macs_per_pair = macs;
+    std::stable_sort(plan.jobs.begin(), plan.jobs.end(),
+                     [](const PairJob &a, const PairJob &b) { return a.cost > b.cost; });
+
+    // Contraction lists: for each n1, the nonzero ab5[n2][n1][p].
+    const auto ab5 = translation_table(lmax);
+    plan.crow.assign(plan.nmax + 1, 0);
+    for (int n1 = 0; n1 < plan.nmax; ++n1) {
+        const int m1 = modes[n1].m;
+        for (int n2 = 0; n2 < plan.nmax; ++n2) {
+            const int dm = modes[n2].m - m1;
+            const int dmi = dm + 2 * lmax;
+            for (int p = 0; p < P; ++p) {
+                const cplx a = ab5[(static_cast<size_t>(n2) * plan.nmax + n1) * P + p];
+                if (a == cplx(0.0, 0.0))
+                    continue;
+                const auto &cl = cols[dmi];
+                const int c = static_cast<int>(std::find(cl.begin(), cl.end(), n2) - cl.begin());
+                auto it = slot.find({dmi, p, c});
+                if (it == slot.end() || c >= static_cast<int>(cl.size()))
+                    throw Error(kInternal, "ab5 entry outside the pair plan");
+                plan.cidx.push_back(it->second);
+                plan.cval.push_back(a.real());
+                plan.cval.push_back(a.imag());
+                ++plan.nnz_ab5;
+            }
+        }
+        plan.crow[n1 + 1] = static_cast<int>(plan.cidx.size());
+    }
+
+    // Normalised Legendre recurrences (legendre_normalized_trigon.py:59-91):
+    //   P_m^m = sqrt((2m+1)/2/(2m)!) (2m-1)!! sin^m
+    //   P_{l+1}^m = alpha(l,m) cos P_l^m - beta(l,m) P_{l-1}^m
+    plan.leg_alpha.assign(P * P, 0.0);
+    plan.leg_beta.assign(P * P, 0.0);
+    plan.leg_diag.assign(P, 0.0);
+    for (int m = 0; m < P; ++m) {
+        long double f2m = 1.0L, dfac = 1.0L;
+        for (int i = 1; i <= 2 * m; ++i)
+            f2m *= i;
+        for (int i = 1; i < 2 * m; i += 2)
+            dfac *= i;
+        plan.leg_diag[m] = static_cast<double>(std::sqrt((2.0L * m + 1.0L) / 2.0L / f2m) * dfac);
+        for (int l = m; l < P; ++l) {
+            plan.leg_alpha[l * P + m] =
+                std::sqrt(static_cast<double>((2 * l + 1) * (2 * l + 3)) / ((l + 1 - m) * (l + 1 + m)));
+            if (l > m)
+                plan.leg_beta[l * P + m] = std::sqrt(static_cast<double>((2 * l + 3) * (l - m) * (l + m)) /
+                                                     ((2 * l - 1) * (l + 1 - m) * (l + 1 + m)));
+        }
+    }
+    return plan;
+}
+
+}  // namespace yasfhip

@@ -1,0 +1,83 @@
+"""Block-diagonal preconditioner (mirror of yasfpy/preconditioner.py:58-265).
+
+Particles are binned into cuboids of ``partition_edge_sizes``
+(``make_particle_partition``, preconditioner.py:222-265); each box's block of
+``I - T W`` restricted to intra-box pairs is built and LU-factored on the
+device in FP64 and applied inside the device GMRES.  This implements the
+intended semantics of the reference's docstrings (particle-major indices,
+receiver T, P^T on the right-hand side); see the module docstring of
+``csrc/precond.hip`` for the differences from the reference's code.
+"""
+
+from __future__ import annotations
+
+import logging
+from time import perf_counter
+
+import numpy as np
+
+
+def make_particle_partition(positions: np.ndarray, edge_sizes) -> list[np.ndarray]:
+    """Cuboid binning with ``np.digitize`` over edges starting at min - 1."""
+    edge_sizes = np.asarray(edge_sizes, dtype=float)
+    lo = positions.min(axis=0) - 1.0
+    hi = positions.max(axis=0) + 1.0
+    bins = [np.digitize(positions[:, d], np.arange(lo[d], hi[d] + edge_sizes[d], edge_sizes[d]))
+            for d in range(3)]
+    groups: dict = {}
+    for i in range(positions.shape[0]):
+        groups.setdefault((int(bins[0][i]), int(bins[1][i]), int(bins[2][i])), []).append(i)
+    return [np.array(v, dtype=int) for v in groups.values()]
+
+
+def edge_size_for(positions: np.ndarray, particles_per_box: float) -> np.ndarray:
+    """Cube edge aiming at ``particles_per_box`` particles per box (the heuristic of
+    the reference's tests/test_preconditioner.py:80-91, with the target as a parameter)."""
+    span = positions.max(axis=0) - positions.min(axis=0)
+    volume = float(np.prod(np.maximum(span, 1e-12)))
+    boxes = max(1, int(positions.shape[0] // particles_per_box))
+    edge = (volume / boxes) ** (1.0 / 3.0)
+    return np.array([edge, edge, edge])
+
+
+class BlockDiagonalPreconditioner:
+    def __init__(self, partition_edge_sizes):
+        self.partition_edge_sizes = np.asarray(partition_edge_sizes, dtype=float)
+        if self.partition_edge_sizes.shape != (3,):
+            raise ValueError("partition_edge_sizes must be array of shape (3,)")
+        self.partitioning = None
+        self.box_of_particle = None
+        self._sim = None
+        self._prepared_channel = None
+        self.prepare_seconds = 0.0
+        self.log = logging.getLogger(self.__class__.__module__)
+
+    def prepare(self, simulation, channel: int = 0):
+        """Partition, build and factor the blocks for one wavelength channel."""
+        t0 = perf_counter()
+        pos = simulation.parameters.particles.position
+        self.partitioning = make_particle_partition(pos, self.partition_edge_sizes)
+        box = np.empty(pos.shape[0], dtype=np.int32)
+        for b, members in enumerate(self.partitioning):
+            box[members] = b
+        self.box_of_particle = box
+        self._sim = simulation
+        simulation.session.precond_build(box, channel=channel)
+        self._prepared_channel = channel
+        self.prepare_seconds = perf_counter() - t0
+        self.log.info("block preconditioner: %d boxes, largest %d particles, %.3f s",
+                      len(self.partitioning), max(len(p) for p in self.partitioning),
+                      self.prepare_seconds)
+        return simulation
+
+    def ensure_prepared(self, channel: int):
+        if self._sim is None:
+            raise RuntimeError("Preconditioner not prepared. Call prepare() first.")
+        if self._prepared_channel != channel:
+            self.prepare(self._sim, channel)
+
+    def apply(self, rhs: np.ndarray) -> np.ndarray:
+        """M^-1 rhs for the prepared channel."""
+        if self._sim is None:
+            raise RuntimeError("Preconditioner not prepared. Call prepare() first.")
+        return self._sim.session.precond_apply(np.asarray(rhs).ravel())
