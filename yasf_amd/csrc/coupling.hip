@@ -24,12 +24,16 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Mfma;
+// A operand: lane holds A[lane & 15][lane >> 4]; B: B[lane >> 4][lane & 15].
+// D: accumulator element i of a lane is column lane & 15 and row
+// row(i, lane >> 4), which differs between the f64 and f32 instructions.
 template <>
 struct Mfma<double> {
     typedef v4d acc_t;
     static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int i, int hi) { return 4 * i + hi; }
 };
 template <>
 struct Mfma<float> {
@@ -37,6 +41,7 @@ struct Mfma<float> {
     static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int i, int hi) { return 4 * hi + i; }
 };
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -215,18 +220,19 @@ __global__ __launch_bounds__(256) void pair_kernel(const double4 *__restrict__ p
         }
     }
 
-    // D[4*hi + i][lo]: target row 4*hi+i, column lo.
+    // Accumulator element i: target row Mfma<T>::row(i, hi), column lo.
 #pragma unroll
     for (int ip = 0; ip < kJobMaxP; ++ip)
 #pragma unroll
         for (int it = 0; it < kJobMaxTiles; ++it)
             if (ip < np && it < ntile) {
                 const int q = qbase + ip * ntile + it;
-                T *dst = Q + 2 * (((static_cast<size_t>(s1t) * nq + q) * 16 + lo) * 16 + 4 * hi);
+                T *dst = Q + 2 * (((static_cast<size_t>(s1t) * nq + q) * 16 + lo) * 16);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    dst[2 * i] = acc[ip][it][0][i];
-                    dst[2 * i + 1] = acc[ip][it][1][i];
+                    const int r = Mfma<T>::row(i, hi);
+                    dst[2 * r] = acc[ip][it][0][i];
+                    dst[2 * r + 1] = acc[ip][it][1][i];
                 }
             }
 }
