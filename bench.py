@@ -106,7 +106,8 @@ def main():
     torch.cuda.set_device(0)
 
     from yasf_amd import _native
-    from yasf_amd.preconditioner import edge_size_for, make_particle_partition
+    from yasf_amd.preconditioner import (MAX_BLOCK_ROWS, edge_size_for, make_particle_partition,
+                                         split_oversized)
 
     lmax, N = 6, 2000
     pos, radii, k, t, b = workload()
@@ -115,7 +116,8 @@ def main():
     use_pc = not args.no_precond
     boxes = None
     if use_pc:
-        part = make_particle_partition(pos, edge_size_for(pos, args.per_box))
+        part = split_oversized(make_particle_partition(pos, edge_size_for(pos, args.per_box)), pos,
+                               MAX_BLOCK_ROWS // 96)
         boxes = np.empty(N, dtype=np.int32)
         for i, members in enumerate(part):
             boxes[members] = i

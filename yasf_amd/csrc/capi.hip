@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -16,7 +17,8 @@ namespace yasfhip {
 
 // coupling.hip
 struct DevicePlan {
-    int lmax, nmax, maxcols, nq, njobs;
+    int lmax, nmax, maxcols, nq, njobs, max_job_cols;
+    int ks;
     PairJob *jobs;
     int *colmap, *crow, *cidx;
     double2 *cval;
@@ -289,6 +291,10 @@ int yasfhip_create(const double *positions, int particle_number, int lmax, const
         s->dp.maxcols = pl.maxcols;
         s->dp.nq = pl.nq;
         s->dp.njobs = static_cast<int>(pl.jobs.size());
+        s->dp.max_job_cols = pl.max_job_cols;
+        s->dp.ks = 0;
+        if (const char *e = std::getenv("YASFHIP_PAIR_KSTEPS"))
+            s->dp.ks = std::atoi(e);
         s->dp.jobs = upload(pl.jobs, st);
         s->dp.colmap = upload(pl.colmap, st);
         s->dp.crow = upload(pl.crow, st);

@@ -46,19 +46,29 @@ struct Mfma<float> {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// G_{s1 s2}[p, dm] for p in [p_lo, p_lo + NP) (entries past np are unused).
+template <typename T>
+struct Cplx;
+template <>
+struct Cplx<double> {
+    typedef double2 type;
+};
+template <>
+struct Cplx<float> {
+    typedef float2 type;
+};
+
+// G_{s1 s2}[p, dm] for every p in [|dm|, 2L], stored to dst[(p - |dm|) * stride].
 // Geometry follows the reference's mutual_lookup (misc.py:149-229):
 // d = r_s1 - r_s2, cos th = d_z/|d| clipped, sin th = sqrt(1 - cos^2),
 // phi = atan2(d_y, d_x), h_p = j_p + i y_p; Legendre per
 // legendre_normalized_trigon.py:59-91.  h_p comes from the upward recurrence
 // h_{p+1} = (2p+1)/x h_p - h_{p-1} seeded with h_0, h_1 (stable for the y_p
-// part that dominates |h_p|).
-template <int NP>
-__device__ __forceinline__ void pair_series(double dx, double dy, double dz, double k, int dm, int p_lo,
-                                            int P, const double *__restrict__ la,
+// part that dominates |h_p|).  All arithmetic is FP64; invalid pairs store 0.
+template <typename T2>
+__device__ __forceinline__ void pair_series(double dx, double dy, double dz, double k, int dm, int P,
+                                            bool valid, const double *__restrict__ la,
                                             const double *__restrict__ lb,
-                                            const double *__restrict__ ld, double (&gre)[NP],
-                                            double (&gim)[NP]) {
+                                            const double *__restrict__ ld, T2 *dst, int stride) {
     const double d = sqrt(dx * dx + dy * dy + dz * dz);
     const bool zero_dist = !(d > 0.0);
     double ct = zero_dist ? 0.0 : dz / d;
@@ -72,7 +82,6 @@ __device__ __forceinline__ void pair_series(double dx, double dy, double dz, dou
         s1 = dy * ir;
     }
     const int adm = dm < 0 ? -dm : dm;
-
     // e^{i dm phi} by binary powering of e^{i phi}.
     double er = 1.0, ei = 0.0, br = c1, bi = s1;
     for (int n = adm; n; n >>= 1) {
@@ -87,16 +96,38 @@ __device__ __forceinline__ void pair_series(double dx, double dy, double dz, dou
     }
     if (dm < 0)
         ei = -ei;
-
-    // Hankel h_p(x): (ar, ai) = h_p, (br2, bi2) = h_{p+1}.
+    // Hankel: (ar, ai) = h_p, (hr, hi) = h_{p+1}.
     const double x = zero_dist ? 1.0 : k * d;
     double sx, cx;
     sincos(x, &sx, &cx);
     const double inv = 1.0 / x;
     double ar = sx * inv, ai = -cx * inv;
     double hr = (ar - cx) * inv, hi = (ai - sx) * inv;
-    int p = 0;
-    for (; p < p_lo; ++p) {
+    // Legendre P_l^adm: cur = P_l, prv = P_{l-1}, starting at l = adm.
+    double cur = ld[adm], prv = 0.0;
+    for (int i = 0; i < adm; ++i)
+        cur *= st;
+    for (int p = 0; p < P; ++p) {
+        if (p >= adm) {
+            const double phr = cur * ar, phi = cur * ai;
+            double gr = phr * er - phi * ei;
+            double gi = phr * ei + phi * er;
+            if (zero_dist) {  // h_p(0) = j_p(0) = delta_p0, phi = 0, cos th = 0
+                gr = (p == 0) ? cur : 0.0;
+                gi = 0.0;
+            }
+            if (!valid) {
+                gr = 0.0;
+                gi = 0.0;
+            }
+            T2 g;
+            g.x = gr;
+            g.y = gi;
+            dst[(p - adm) * stride] = g;
+            const double nx = la[p * P + adm] * ct * cur - lb[p * P + adm] * prv;
+            prv = cur;
+            cur = nx;
+        }
         const double f = (2 * p + 3) * inv;
         const double nr = f * hr - ar, ni = f * hi - ai;
         ar = hr;
@@ -104,137 +135,109 @@ __device__ __forceinline__ void pair_series(double dx, double dy, double dz, dou
         hr = nr;
         hi = ni;
     }
-    // Legendre P_l^adm: cur = P_l, prv = P_{l-1}, starting at l = adm.
-    double cur = ld[adm], prv = 0.0;
-    for (int i = 0; i < adm; ++i)
-        cur *= st;
-    for (int l = adm; l < p_lo; ++l) {
-        const double nx = la[l * P + adm] * ct * cur - lb[l * P + adm] * prv;
-        prv = cur;
-        cur = nx;
-    }
-#pragma unroll
-    for (int ip = 0; ip < NP; ++ip) {
-        const int pp = p_lo + ip;
-        // G = P h e
-        const double phr = cur * ar, phi = cur * ai;
-        double gr = phr * er - phi * ei;
-        double gi = phr * ei + phi * er;
-        if (zero_dist) {  // h_p(0) = j_p(0) = delta_p0, phi = 0, cos th = 0
-            gr = (pp == 0) ? cur : 0.0;
-            gi = 0.0;
-        }
-        gre[ip] = gr;
-        gim[ip] = gi;
-        if (pp + 1 < P) {
-            const double f = (2 * pp + 3) * inv;
-            const double nr = f * hr - ar, ni = f * hi - ai;
-            ar = hr;
-            ai = hi;
-            hr = nr;
-            hi = ni;
-            const double nx = la[pp * P + adm] * ct * cur - lb[pp * P + adm] * prv;
-            prv = cur;
-            cur = nx;
-        }
-    }
 }
 
-// One wave per (job, target tile).  Block = 4 independent waves.
+// One block (4 waves) per (job, 16-particle target tile); jobs are sorted by
+// cost and the block index is job-major so the heaviest jobs start first.
+// Per chunk of 4*KS sources: (1) every pair's G[p] for all of the job's
+// orders is generated once into LDS in the MFMA A-operand layout and the
+// job's source columns of x are staged as the B operand; (2) each wave runs
+// the complex 16x16x4 MFMAs of its C tiles (c = wave, wave+4, ...).
 // Q layout: [target tile][q][col 16][row 16] complex T (interleaved).
-template <typename T>
-__global__ __launch_bounds__(256) void pair_kernel(const double4 *__restrict__ pos, const T *__restrict__ x,
-                                                   int N, int nmax, double k,
-                                                   const PairJob *__restrict__ jobs, int njobs,
+template <typename T, int KS>
+__global__ __launch_bounds__(256) void pair_kernel(const double4 *__restrict__ pos,
+                                                   const typename Cplx<T>::type *__restrict__ x, int N,
+                                                   int nmax, double k, const PairJob *__restrict__ jobs,
                                                    int ntiles1, const int *__restrict__ colmap,
                                                    int maxcols, int lmax, const double *__restrict__ la,
                                                    const double *__restrict__ lb,
                                                    const double *__restrict__ ld, T *__restrict__ Q,
                                                    int nq) {
     typedef typename Mfma<T>::acc_t acc_t;
-    const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (wave >= njobs * ntiles1)
-        return;
-    const int jid = uniform(wave / ntiles1);
-    const int s1t = uniform(wave % ntiles1);
+    typedef typename Cplx<T>::type T2;
+    constexpr int SRC = 4 * KS;  // sources per chunk
+    extern __shared__ unsigned char lds_raw[];
+    T2 *Ash = reinterpret_cast<T2 *>(lds_raw);  // [R][64*KS]
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int jid = blockIdx.x / ntiles1;
+    const int s1t = blockIdx.x - jid * ntiles1;
     const int dm = uniform(jobs[jid].dm);
-    const int p_lo = uniform(jobs[jid].p_lo);
-    const int np = uniform(jobs[jid].np);
     const int tile0 = uniform(jobs[jid].tile0);
     const int ntile = uniform(jobs[jid].ntile);
     const int qbase = uniform(jobs[jid].qbase);
     const int P = 2 * lmax + 1;
+    const int R = P - (dm < 0 ? -dm : dm);
+    const int ncols = ntile * 16;
+    const int total = R * ntile;
+    T2 *Bsh = Ash + R * 64 * KS;  // [SRC][ncols]
+    const int *cmap = colmap + (dm + 2 * lmax) * maxcols + tile0 * 16;
 
-    const int lo = lane & 15;  // A row (target in tile) / B,D column
-    const int hi = lane >> 4;  // A,B k index (source in K-step)
-    const int s1 = s1t * 16 + lo;
+    const int lo = lane & 15, hi = lane >> 4;
+    const int gr = tid & 15, gk = tid >> 4;  // generation: target row, source in chunk
+    const int s1 = s1t * 16 + gr;
     const double4 p1 = pos[s1];  // pos is padded to a multiple of 16
 
-    int n2c[kJobMaxTiles];
+    acc_t acc[kWaveMaxTiles][2];
 #pragma unroll
-    for (int it = 0; it < kJobMaxTiles; ++it) {
-        const int col = (tile0 + it) * 16 + lo;
-        n2c[it] = (it < ntile) ? colmap[(dm + 2 * lmax) * maxcols + col] : -1;
+    for (int j = 0; j < kWaveMaxTiles; ++j) {
+        acc[j][0] = acc_t{0, 0, 0, 0};
+        acc[j][1] = acc_t{0, 0, 0, 0};
     }
 
-    acc_t acc[kJobMaxP][kJobMaxTiles][2];
-#pragma unroll
-    for (int ip = 0; ip < kJobMaxP; ++ip)
-#pragma unroll
-        for (int it = 0; it < kJobMaxTiles; ++it) {
-            acc[ip][it][0] = acc_t{0, 0, 0, 0};
-            acc[ip][it][1] = acc_t{0, 0, 0, 0};
+    for (int c0 = 0; c0 < N; c0 += SRC) {
+        if (gk < SRC) {
+            const int s2 = c0 + gk;
+            const bool valid = (s2 < N) && (s1 < N) && (s2 != s1);
+            const double4 p2 = pos[s2 < N ? s2 : 0];
+            pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld,
+                            Ash + tid, 64 * KS);
         }
-
-    for (int k0 = 0; k0 < N; k0 += 4) {
-        const int s2 = k0 + hi;
-        const bool valid = (s2 < N) && (s1 < N) && (s2 != s1);
-        const double4 p2 = pos[s2 < N ? s2 : 0];
-        double gre[kJobMaxP], gim[kJobMaxP];
-        pair_series<kJobMaxP>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, p_lo, P, la, lb, ld, gre,
-                              gim);
-        T bre[kJobMaxTiles], bim[kJobMaxTiles];
-#pragma unroll
-        for (int it = 0; it < kJobMaxTiles; ++it) {
-            const bool ok = (n2c[it] >= 0) && (s2 < N);
-            const size_t o = 2 * (static_cast<size_t>(ok ? s2 : 0) * nmax + (ok ? n2c[it] : 0));
-            bre[it] = ok ? x[o] : T(0);
-            bim[it] = ok ? x[o + 1] : T(0);
+        for (int e = tid; e < SRC * ncols; e += blockDim.x) {
+            const int kk = e / ncols, c = e - kk * ncols;
+            const int n2 = cmap[c];
+            const int s2 = c0 + kk;
+            T2 v;
+            v.x = T(0);
+            v.y = T(0);
+            if (n2 >= 0 && s2 < N)
+                v = x[static_cast<size_t>(s2) * nmax + n2];
+            Bsh[e] = v;
         }
+        __syncthreads();
 #pragma unroll
-        for (int ip = 0; ip < kJobMaxP; ++ip) {
-            if (ip < np) {
-                const T are = valid ? static_cast<T>(gre[ip]) : T(0);
-                const T aim = valid ? static_cast<T>(gim[ip]) : T(0);
+        for (int j = 0; j < kWaveMaxTiles; ++j) {
+            const int c = wave + kWavesPerJob * j;
+            if (c < total) {
+                const int ip = c / ntile, it = c - ip * ntile;
 #pragma unroll
-                for (int it = 0; it < kJobMaxTiles; ++it) {
-                    if (it < ntile) {
-                        acc[ip][it][0] = Mfma<T>::mac(are, bre[it], acc[ip][it][0]);
-                        acc[ip][it][0] = Mfma<T>::mac(-aim, bim[it], acc[ip][it][0]);
-                        acc[ip][it][1] = Mfma<T>::mac(are, bim[it], acc[ip][it][1]);
-                        acc[ip][it][1] = Mfma<T>::mac(aim, bre[it], acc[ip][it][1]);
-                    }
+                for (int ks = 0; ks < KS; ++ks) {
+                    const T2 a = Ash[ip * 64 * KS + ks * 64 + lane];
+                    const T2 bv = Bsh[(ks * 4 + hi) * ncols + it * 16 + lo];
+                    acc[j][0] = Mfma<T>::mac(a.x, bv.x, acc[j][0]);
+                    acc[j][0] = Mfma<T>::mac(-a.y, bv.y, acc[j][0]);
+                    acc[j][1] = Mfma<T>::mac(a.x, bv.y, acc[j][1]);
+                    acc[j][1] = Mfma<T>::mac(a.y, bv.x, acc[j][1]);
                 }
             }
         }
+        __syncthreads();
     }
 
     // Accumulator element i: target row Mfma<T>::row(i, hi), column lo.
 #pragma unroll
-    for (int ip = 0; ip < kJobMaxP; ++ip)
+    for (int j = 0; j < kWaveMaxTiles; ++j) {
+        const int c = wave + kWavesPerJob * j;
+        if (c < total) {
+            T *dst = Q + 2 * (((static_cast<size_t>(s1t) * nq + qbase + c) * 16 + lo) * 16);
 #pragma unroll
-        for (int it = 0; it < kJobMaxTiles; ++it)
-            if (ip < np && it < ntile) {
-                const int q = qbase + ip * ntile + it;
-                T *dst = Q + 2 * (((static_cast<size_t>(s1t) * nq + q) * 16 + lo) * 16);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int r = Mfma<T>::row(i, hi);
-                    dst[2 * r] = acc[ip][it][0][i];
-                    dst[2 * r + 1] = acc[ip][it][1][i];
-                }
+            for (int i = 0; i < 4; ++i) {
+                const int r = Mfma<T>::row(i, hi);
+                dst[2 * r] = acc[j][0][i];
+                dst[2 * r + 1] = acc[j][1][i];
             }
+        }
+    }
 }
 
 // (W x)[s1, n1] = sum_e ab5_e Q[e], then either y = W x (mode 0) or the
@@ -281,29 +284,49 @@ __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q,
 
 // Device-side views of a CouplingPlan.
 struct DevicePlan {
-    int lmax, nmax, maxcols, nq, njobs;
+    int lmax, nmax, maxcols, nq, njobs, max_job_cols;
+    int ks;  // source K-steps per LDS chunk: 2, 4, or 0 = largest that fits
     PairJob *jobs;
     int *colmap, *crow, *cidx;
     double2 *cval;
     double *la, *lb, *ld;
 };
 
+template <typename T, int KS>
+static size_t pair_lds_bytes(const DevicePlan &dp) {
+    return (static_cast<size_t>(2 * dp.lmax + 1) * 64 * KS + 4 * KS * dp.max_job_cols) *
+           sizeof(typename Cplx<T>::type);
+}
+
+template <typename T, int KS>
+static void launch_pair_t(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
+                          void *Q, hipStream_t s) {
+    const int ntiles1 = (N + 15) / 16;
+    const size_t lds = pair_lds_bytes<T, KS>(dp);
+    hipLaunchKernelGGL((pair_kernel<T, KS>), dim3(dp.njobs * ntiles1), dim3(256), lds, s, pos, static_cast<const typename Cplx<T>::type *>(x), N, dp.nmax, k, dp.jobs,
+                       ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.la, dp.lb, dp.ld, static_cast<T *>(Q),
+                       dp.nq);
+    YH_CHECK(hipGetLastError());
+}
+
+constexpr size_t kPairLdsLimit = 64 * 1024;
+
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
                  void *Q, hipStream_t s) {
-    const int ntiles1 = (N + 15) / 16;
-    const long waves = static_cast<long>(dp.njobs) * ntiles1;
-    const int blocks = static_cast<int>((waves + 3) / 4);
-    if (fp32)
-        hipLaunchKernelGGL(pair_kernel<float>, dim3(blocks), dim3(256), 0, s, pos,
-                           static_cast<const float *>(x), N, dp.nmax, k, dp.jobs, dp.njobs, ntiles1,
-                           dp.colmap, dp.maxcols, dp.lmax, dp.la, dp.lb, dp.ld, static_cast<float *>(Q),
-                           dp.nq);
-    else
-        hipLaunchKernelGGL(pair_kernel<double>, dim3(blocks), dim3(256), 0, s, pos,
-                           static_cast<const double *>(x), N, dp.nmax, k, dp.jobs, dp.njobs, ntiles1,
-                           dp.colmap, dp.maxcols, dp.lmax, dp.la, dp.lb, dp.ld,
-                           static_cast<double *>(Q), dp.nq);
-    YH_CHECK(hipGetLastError());
+    const bool ks4_fits = fp32 ? pair_lds_bytes<float, 4>(dp) <= kPairLdsLimit
+                               : pair_lds_bytes<double, 4>(dp) <= kPairLdsLimit;
+    const bool use4 = ks4_fits && dp.ks != 2;
+    if (fp32) {
+        if (use4)
+            launch_pair_t<float, 4>(dp, pos, N, k, x, Q, s);
+        else
+            launch_pair_t<float, 2>(dp, pos, N, k, x, Q, s);
+    } else {
+        if (use4)
+            launch_pair_t<double, 4>(dp, pos, N, k, x, Q, s);
+        else
+            launch_pair_t<double, 2>(dp, pos, N, k, x, Q, s);
+    }
 }
 
 // fp32: Q, x and y are complex64; otherwise complex128.

@@ -56,22 +56,25 @@ std::vector<Mode> modes_of(int lmax);
 // ab5[j1][j2][p] (reference yasfpy/numerics.py:224-272), row-major complex.
 std::vector<cplx> translation_table(int lmax);
 
-// One wave's share of the pair GEMM: for one signed dm, expansion orders
-// p in [p_lo, p_lo+np) and column tiles [tile0, tile0+ntile) of the dm's
-// compatible source columns.  Its C tiles are written to Q slots
-// qbase + ip*ntile + it of the target tile's Q block.
+// One workgroup's share of the pair GEMM for one 16-particle target tile:
+// a signed dm, all orders p in [|dm|, 2L] (R = 2L+1-|dm| of them) and the
+// column tiles [tile0, tile0+ntile) of the dm's compatible source columns.
+// Its R*ntile C tiles, flattened as c = ip*ntile + it, go to Q slots qbase + c.
 struct PairJob {
-    int dm, p_lo, np, tile0, ntile, qbase, cost, pad;
+    int dm, tile0, ntile, qbase, cost, pad0, pad1, pad2;
 };
 
-constexpr int kJobMaxP = 4;
-constexpr int kJobMaxTiles = 2;
+constexpr int kWavesPerJob = 4;
+constexpr int kWaveMaxTiles = 8;                                // C tiles per wave
+constexpr int kJobMaxTiles = kWavesPerJob * kWaveMaxTiles;      // C tiles per job
+constexpr int kJobMaxColTiles = 8;                              // column tiles per job
 
 // Everything the coupling kernels need for one lmax.
 struct CouplingPlan {
     int lmax = 0, nmax = 0, ndm = 0, maxcols = 0, nq = 0;
     std::vector<int> colmap;          // [ndm][maxcols] -> source mode n2 or -1
     std::vector<PairJob> jobs;        // per 16-particle target tile, sorted by cost
+    int max_job_cols = 0;             // widest job in columns (ntile*16)
     std::vector<int> crow;            // CSR row offsets per target mode n1 (nmax+1)
     std::vector<int> cidx;            // Q column index q*16 + col
     std::vector<double> cval;         // ab5 entry (re, im) interleaved

@@ -142,28 +142,27 @@ CouplingPlan build_coupling_plan(int lmax) {
         for (size_t c = 0; c < cols[dmi].size(); ++c)
             plan.colmap[static_cast<size_t>(dmi) * maxcols + c] = cols[dmi][c];
 
-    // Jobs: (dm, p-chunk <= kJobMaxP, tile-chunk <= kJobMaxTiles).
-    // slot[(dmi, p, col)] -> Q column index q*16 + col%16.
+    // Jobs: (dm, all its orders, a group of column tiles) with at most
+    // kJobMaxTiles C tiles.  slot[(dmi, p, col)] -> Q column index q*16 + col%16.
     std::map<std::tuple<int, int, int>, int> slot;
     int q = 0;
     long long macs = 0;
     for (int dmi = 0; dmi < plan.ndm; ++dmi) {
         const int dm = dmi - 2 * lmax;
+        const int R = P - std::abs(dm);
         const int ntiles = (static_cast<int>(cols[dmi].size()) + 15) / 16;
-        for (int p_lo = std::abs(dm); p_lo < P; p_lo += kJobMaxP) {
-            const int np = std::min(kJobMaxP, P - p_lo);
-            for (int t0 = 0; t0 < ntiles; t0 += kJobMaxTiles) {
-                const int nt = std::min(kJobMaxTiles, ntiles - t0);
-                PairJob job{dm, p_lo, np, t0, nt, q, np * nt, 0};
-                for (int ip = 0; ip < np; ++ip)
-                    for (int it = 0; it < nt; ++it) {
-                        for (int c = 0; c < 16; ++c)
-                            slot[{dmi, p_lo + ip, (t0 + it) * 16 + c}] = (q + ip * nt + it) * 16 + c;
-                    }
-                q += np * nt;
-                macs += static_cast<long long>(np) * nt * 16;
-                plan.jobs.push_back(job);
-            }
+        const int group = std::max(1, std::min(kJobMaxColTiles, kJobMaxTiles / R));
+        for (int t0 = 0; t0 < ntiles; t0 += group) {
+            const int nt = std::min(group, ntiles - t0);
+            PairJob job{dm, t0, nt, q, R * nt, 0, 0, 0};
+            for (int ip = 0; ip < R; ++ip)
+                for (int it = 0; it < nt; ++it)
+                    for (int c = 0; c < 16; ++c)
+                        slot[{dmi, std::abs(dm) + ip, (t0 + it) * 16 + c}] = (q + ip * nt + it) * 16 + c;
+            q += R * nt;
+            macs += static_cast<long long>(R) * nt * 16;
+            plan.max_job_cols = std::max(plan.max_job_cols, nt * 16);
+            plan.jobs.push_back(job);
         }
     }
     plan.nq = q;

@@ -30,6 +30,27 @@ def make_particle_partition(positions: np.ndarray, edge_sizes) -> list[np.ndarra
     return [np.array(v, dtype=int) for v in groups.values()]
 
 
+def split_oversized(partition: list[np.ndarray], positions: np.ndarray, cap: int) -> list[np.ndarray]:
+    """Bisect any box holding more than ``cap`` particles at the median of its
+    longest extent until every box fits (device blocks are limited in size)."""
+    out = []
+    stack = list(partition)
+    while stack:
+        box = stack.pop()
+        if box.size <= cap:
+            out.append(box)
+            continue
+        pts = positions[box]
+        axis = int(np.argmax(pts.max(axis=0) - pts.min(axis=0)))
+        order = box[np.argsort(pts[:, axis], kind="stable")]
+        half = order.size // 2
+        stack.extend([order[half:], order[:half]])
+    return out
+
+
+MAX_BLOCK_ROWS = 3840  # device limit on a box's block size (padded rows)
+
+
 def edge_size_for(positions: np.ndarray, particles_per_box: float) -> np.ndarray:
     """Cube edge aiming at ``particles_per_box`` particles per box (the heuristic of
     the reference's tests/test_preconditioner.py:80-91, with the target as a parameter)."""
@@ -56,7 +77,9 @@ class BlockDiagonalPreconditioner:
         """Partition, build and factor the blocks for one wavelength channel."""
         t0 = perf_counter()
         pos = simulation.parameters.particles.position
-        self.partitioning = make_particle_partition(pos, self.partition_edge_sizes)
+        cap = max(1, MAX_BLOCK_ROWS // simulation.numerics.nmax)
+        self.partitioning = split_oversized(make_particle_partition(pos, self.partition_edge_sizes),
+                                            pos, cap)
         box = np.empty(pos.shape[0], dtype=np.int32)
         for b, members in enumerate(self.partitioning):
             box[members] = b
