@@ -36,6 +36,13 @@ enum {
 
 enum { YASFHIP_FP64 = 0, YASFHIP_FP32 = 1 };
 
+/* How the coupling matvec obtains the pair factors P_p^{|dm|} h_p e^{i dm phi}:
+ * generated inside the matvec kernel, or generated once per session and
+ * channel into a resident HBM table (ntiles*(2L+1)^2*ceil(N/4)*64 complex
+ * values) that every later matvec streams.  AUTO uses the table when it needs
+ * at most half of the free device memory. */
+enum { YASFHIP_MATVEC_AUTO = 0, YASFHIP_MATVEC_ON_THE_FLY = 1, YASFHIP_MATVEC_TABLE = 2 };
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *yasfhip_last_error(void);
 
@@ -96,6 +103,12 @@ int yasfhip_precond_build(yasfhip_session *s, int channel, const int *box_of_par
 /* z = M^-1 r with the prepared preconditioner (BlockDiagonalPreconditioner.apply,
  * yasfpy/preconditioner.py:177-219). */
 int yasfhip_precond_apply(yasfhip_session *s, const double *r, double *z);
+
+/* Select the pair-factor strategy (YASFHIP_MATVEC_*); results agree to
+ * rounding.  Switching to ON_THE_FLY frees the resident tables.  No
+ * counterpart in the reference (its dense backend stores raw per-pair tables,
+ * yasfpy/lookups.py:124-171). */
+int yasfhip_set_matvec_mode(yasfhip_session *s, int mode);
 
 /* Timing of `reps` device-resident matvecs of one channel on the session
  * stream with HIP events: ms_pair = mean duration of the pair kernel,

@@ -33,14 +33,19 @@ def test_fixture_matvec(golden):
     assert rel(y, R.coupling_matvec(L, d["positions"], k, d["x"])) < 1e-12
 
 
+MODES = [_native.MATVEC_ON_THE_FLY, _native.MATVEC_TABLE]
+
+
+@pytest.mark.parametrize("mode", MODES, ids=["on_the_fly", "table"])
 @pytest.mark.parametrize("lmax", [1, 2, 3, 4, 5, 6, 7, 8])
-def test_matvec_all_orders(lmax):
+def test_matvec_all_orders(lmax, mode):
     rng = np.random.default_rng(100 + lmax)
     N = 21  # not a multiple of the 16-particle tile or the 4-source step
     pos, _ = cluster(lmax, N, 0.1)
     k = 2 * np.pi / 550.0
     x = rand_vec(rng, N * 2 * lmax * (lmax + 2))
     s = _native.Session(pos, lmax, [k])
+    s.set_matvec_mode(mode)
     y = s.coupling_matvec(x)
     ref = R.coupling_matvec(lmax, pos, k, x)
     assert rel(y, ref) < 1e-12
@@ -63,13 +68,15 @@ def test_small_cluster_config():
     assert rel(s.coupling_matvec(x), R.coupling_matvec(4, pos, k, x)) < 1e-12
 
 
-def test_full_size_sampled_rows():
+@pytest.mark.parametrize("mode", MODES, ids=["on_the_fly", "table"])
+def test_full_size_sampled_rows(mode):
     """2000 spheres, lmax 6: full matvec on the GPU, oracle on sampled target rows."""
     pos, _ = cluster(2, 2000, 0.2)
     k = 2 * np.pi / 550.0
     nmax = 96
     x = rand_vec(np.random.default_rng(2), 2000 * nmax)
     s = _native.Session(pos, 6, [k])
+    s.set_matvec_mode(mode)
     y = s.coupling_matvec(x)
     for s1 in (0, 777, 1999):
         ref = R.coupling_matvec(6, pos, k, x, s1_lo=s1, n_s1=1)
@@ -79,6 +86,22 @@ def test_full_size_sampled_rows():
     a, b = 0.3 - 1.1j, -2.0 + 0.5j
     y2 = s.coupling_matvec(x2)
     assert rel(s.coupling_matvec(a * x + b * x2), a * y + b * y2) < 1e-13
+
+
+def test_modes_agree_and_table_follows_channel():
+    pos, _ = cluster(8, 40, 0.15)
+    k = 2 * np.pi / np.array([400.0, 650.0])
+    s = _native.Session(pos, 5, k)
+    x = rand_vec(np.random.default_rng(8), 40 * 70)
+    out = {}
+    for mode in MODES:
+        s.set_matvec_mode(mode)
+        out[mode] = [s.coupling_matvec(x, channel=w) for w in (0, 1, 0)]
+    for w in range(3):
+        assert rel(out[MODES[1]][w], out[MODES[0]][w]) < 1e-13
+    assert rel(out[MODES[1]][0], out[MODES[1]][1]) > 1e-3  # the channels differ
+    with pytest.raises(ValueError):
+        s.set_matvec_mode(7)
 
 
 def test_master_and_multichannel():

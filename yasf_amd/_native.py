@@ -18,6 +18,10 @@ LIB_PATH = Path(__file__).resolve().parent / "libyasfhip.so"
 FP64 = 0
 FP32 = 1
 
+MATVEC_AUTO = 0
+MATVEC_ON_THE_FLY = 1
+MATVEC_TABLE = 2
+
 _lib = None
 _lock = threading.Lock()
 
@@ -58,6 +62,7 @@ def _load():
             "yasfhip_precond_build": ([P, i, P, i], i),
             "yasfhip_precond_apply": ([P, P, P], i),
             "yasfhip_time_matvec": ([P, i, i, i, dp, dp], i),
+            "yasfhip_set_matvec_mode": ([P, i], i),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -77,7 +82,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_last_error", "yasfhip_translation_table", "yasfhip_plan_stats", "yasfhip_create",
         "yasfhip_destroy", "yasfhip_set_tmatrix", "yasfhip_coupling_matvec", "yasfhip_master_matvec",
         "yasfhip_gmres_solve", "yasfhip_precond_build", "yasfhip_precond_apply",
-        "yasfhip_time_matvec",
+        "yasfhip_time_matvec", "yasfhip_set_matvec_mode",
     ]
 
 
@@ -203,6 +208,10 @@ class Session:
         z = np.empty_like(r)
         _check(self._lib.yasfhip_precond_apply(self._h, _ptr(r), _ptr(z)))
         return z
+
+    def set_matvec_mode(self, mode: int):
+        """MATVEC_AUTO, MATVEC_ON_THE_FLY or MATVEC_TABLE (resident pair-factor table)."""
+        _check(self._lib.yasfhip_set_matvec_mode(self._h, int(mode)))
 
     def time_matvec(self, channel: int = 0, precision: int = FP64, reps: int = 5) -> tuple[float, float]:
         """Mean (pair-kernel ms, whole-matvec ms) over `reps` device-resident matvecs."""

@@ -15,19 +15,6 @@
 
 namespace yasfhip {
 
-// coupling.hip
-struct DevicePlan {
-    int lmax, nmax, maxcols, nq, njobs, max_job_cols;
-    int ks;
-    PairJob *jobs;
-    int *colmap, *crow, *cidx;
-    double2 *cval;
-    double *la, *lb, *ld;
-};
-void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
-                 void *Q, hipStream_t s);
-void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
-                     int mode, void *y, bool fp32, hipStream_t s);
 // krylov.hip
 struct Reducer {
     double *partials;
@@ -102,6 +89,11 @@ struct yasfhip_session {
     double *hdev = nullptr, *hhost = nullptr;  // per-step MGS results, 4 doubles per slot
     BlockPrecond *precond = nullptr;
     int precond_channel = -1;
+    // Resident pair-factor tables, one per precision (0 = FP64, 1 = FP32).
+    int matvec_mode = YASFHIP_MATVEC_AUTO;
+    void *table[2] = {nullptr, nullptr};
+    int table_channel[2] = {-1, -1};
+    bool table_refused[2] = {false, false};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 
     ~yasfhip_session() {
@@ -112,7 +104,8 @@ struct yasfhip_session {
         for (void *b : bufs)
             if (b)
                 (void)hipFree(b);
-        void *plan_bufs[] = {dp.jobs, dp.colmap, dp.crow, dp.cidx, dp.cval, dp.la, dp.lb, dp.ld};
+        void *plan_bufs[] = {dp.jobs, dp.colmap, dp.crow, dp.cidx, dp.cval, dp.la, dp.lb, dp.ld, dp.dmoff,
+                             table[0], table[1]};
         for (void *b : plan_bufs)
             if (b)
                 (void)hipFree(b);
@@ -129,17 +122,50 @@ struct yasfhip_session {
 
     Reducer reducer() const { return Reducer{partials, counter}; }
 
+    // Makes the resident pair table of this precision hold `channel`; false
+    // when the matvec should generate the pair factors on the fly instead.
+    bool table_for(int channel, bool fp32) {
+        const int ix = fp32 ? 1 : 0;
+        if (matvec_mode == YASFHIP_MATVEC_ON_THE_FLY)
+            return false;
+        if (table[ix] && table_channel[ix] == channel)
+            return true;
+        const size_t bytes = pair_table_bytes(dp, N, fp32);
+        if (!table[ix]) {
+            if (matvec_mode == YASFHIP_MATVEC_AUTO) {
+                if (table_refused[ix])
+                    return false;
+                size_t free_b = 0, total_b = 0;
+                YH_CHECK(hipMemGetInfo(&free_b, &total_b));
+                if (bytes > free_b / 2) {  // leave room for other sessions and solver state
+                    table_refused[ix] = true;
+                    return false;
+                }
+            }
+            YH_CHECK(hipMalloc(&table[ix], bytes));
+        }
+        launch_pair_table_build(dp, pos, N, k[channel], fp32, table[ix], stream);
+        table_channel[ix] = channel;
+        return true;
+    }
+
+    void pair_gemm(int channel, const void *xv, bool fp32) {
+        if (table_for(channel, fp32))
+            launch_pair_table_gemm(dp, N, table[fp32 ? 1 : 0], xv, fp32, Q, stream);
+        else
+            launch_pair(dp, pos, N, k[channel], xv, fp32, Q, stream);
+    }
+
     // y = W x (mode 0) or x - t W x (mode 1) for device vectors of complex128.
     void matvec_dev(int channel, const double2 *x, double2 *y, int mode, bool fp32) {
-        const double kk = k[channel];
         const double2 *tt = t + static_cast<size_t>(channel) * n;
         if (!fp32) {
-            launch_pair(dp, pos, N, kk, x, false, Q, stream);
+            pair_gemm(channel, x, false);
             launch_contract(dp, N, Q, x, tt, mode, y, false, stream);
         } else {
             launch_to_fp32(reinterpret_cast<float *>(x32), reinterpret_cast<const double *>(x), 2 * n,
                            stream);
-            launch_pair(dp, pos, N, kk, x32, true, Q, stream);
+            pair_gemm(channel, x32, true);
             launch_contract(dp, N, Q, x32, tt, mode, y32, true, stream);
             launch_to_fp64(reinterpret_cast<double *>(y), reinterpret_cast<const float *>(y32), 2 * n,
                            stream);
@@ -291,10 +317,7 @@ int yasfhip_create(const double *positions, int particle_number, int lmax, const
         s->dp.maxcols = pl.maxcols;
         s->dp.nq = pl.nq;
         s->dp.njobs = static_cast<int>(pl.jobs.size());
-        s->dp.max_job_cols = pl.max_job_cols;
-        s->dp.ks = 0;
-        if (const char *e = std::getenv("YASFHIP_PAIR_KSTEPS"))
-            s->dp.ks = std::atoi(e);
+        s->dp.stage_elems = pl.stage_elems;
         s->dp.jobs = upload(pl.jobs, st);
         s->dp.colmap = upload(pl.colmap, st);
         s->dp.crow = upload(pl.crow, st);
@@ -306,6 +329,12 @@ int yasfhip_create(const double *positions, int particle_number, int lmax, const
         s->dp.la = upload(pl.leg_alpha, st);
         s->dp.lb = upload(pl.leg_beta, st);
         s->dp.ld = upload(pl.leg_diag, st);
+        std::vector<int> dmoff(4 * lmax + 1);
+        for (int d = 0, acc = 0; d < 4 * lmax + 1; ++d) {
+            dmoff[d] = acc;
+            acc += 2 * lmax + 1 - std::abs(d - 2 * lmax);
+        }
+        s->dp.dmoff = upload(dmoff, st);
 
         s->Q = dmalloc<double2>(static_cast<size_t>(s->ntiles1) * pl.nq * 256);
         s->t = dmalloc<double2>(static_cast<size_t>(s->W) * s->n);
@@ -578,6 +607,24 @@ int yasfhip_precond_apply(yasfhip_session *s, const double *r, double *z) {
     });
 }
 
+int yasfhip_set_matvec_mode(yasfhip_session *s, int mode) {
+    return guarded([&] {
+        YH_REQUIRE(s, "NULL argument");
+        YH_REQUIRE(mode == YASFHIP_MATVEC_AUTO || mode == YASFHIP_MATVEC_ON_THE_FLY ||
+                       mode == YASFHIP_MATVEC_TABLE,
+                   "unknown matvec mode");
+        s->matvec_mode = mode;
+        if (mode == YASFHIP_MATVEC_ON_THE_FLY)
+            for (int ix = 0; ix < 2; ++ix)
+                if (s->table[ix]) {
+                    YH_CHECK(hipFree(s->table[ix]));
+                    s->table[ix] = nullptr;
+                    s->table_channel[ix] = -1;
+                }
+        s->table_refused[0] = s->table_refused[1] = false;
+    });
+}
+
 int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps, double *ms_pair,
                         double *ms_total) {
     return guarded([&] {
@@ -591,9 +638,10 @@ int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps
             launch_to_fp32(reinterpret_cast<float *>(s->x32), reinterpret_cast<const double *>(s->xin),
                            2 * s->n, st);
         for (int r = 0; r < reps; ++r) {
-            YH_CHECK(hipEventRecord(s->ev[0], st));
             const void *xv = fp32 ? static_cast<const void *>(s->x32) : static_cast<const void *>(s->xin);
-            launch_pair(s->dp, s->pos, s->N, s->k[channel], xv, fp32, s->Q, st);
+            s->table_for(channel, fp32);  // a (re)build is not part of the timed matvec
+            YH_CHECK(hipEventRecord(s->ev[0], st));
+            s->pair_gemm(channel, xv, fp32);
             YH_CHECK(hipEventRecord(s->ev[1], st));
             launch_contract(s->dp, s->N, s->Q, xv, s->t + static_cast<size_t>(channel) * s->n, 0,
                             fp32 ? static_cast<void *>(s->y32) : static_cast<void *>(s->yout), fp32, st);

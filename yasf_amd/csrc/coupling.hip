@@ -15,151 +15,38 @@
 // straight in as the A operand -- no LDS, no barriers.  The B operand is the
 // source coefficients x[s2, n2] of the wave's 16 columns.  Each wave owns one
 // PairJob (dm, <=4 orders p, <=2 column tiles) and sweeps all sources.
-#include "internal.hpp"
+#include "device_common.hpp"
 
 namespace yasfhip {
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct Mfma;
-// A operand: lane holds A[lane & 15][lane >> 4]; B: B[lane >> 4][lane & 15].
-// D: accumulator element i of a lane is column lane & 15 and row
-// row(i, lane >> 4), which differs between the f64 and f32 instructions.
-template <>
-struct Mfma<double> {
-    typedef v4d acc_t;
-    static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int i, int hi) { return 4 * i + hi; }
-};
-template <>
-struct Mfma<float> {
-    typedef v4f acc_t;
-    static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int i, int hi) { return 4 * hi + i; }
-};
-
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-template <typename T>
-struct Cplx;
-template <>
-struct Cplx<double> {
-    typedef double2 type;
-};
-template <>
-struct Cplx<float> {
-    typedef float2 type;
-};
-
-// G_{s1 s2}[p, dm] for every p in [|dm|, 2L], stored to dst[(p - |dm|) * stride].
-// Geometry follows the reference's mutual_lookup (misc.py:149-229):
-// d = r_s1 - r_s2, cos th = d_z/|d| clipped, sin th = sqrt(1 - cos^2),
-// phi = atan2(d_y, d_x), h_p = j_p + i y_p; Legendre per
-// legendre_normalized_trigon.py:59-91.  h_p comes from the upward recurrence
-// h_{p+1} = (2p+1)/x h_p - h_{p-1} seeded with h_0, h_1 (stable for the y_p
-// part that dominates |h_p|).  All arithmetic is FP64; invalid pairs store 0.
-template <typename T2>
-__device__ __forceinline__ void pair_series(double dx, double dy, double dz, double k, int dm, int P,
-                                            bool valid, const double *__restrict__ la,
-                                            const double *__restrict__ lb,
-                                            const double *__restrict__ ld, T2 *dst, int stride) {
-    const double d = sqrt(dx * dx + dy * dy + dz * dz);
-    const bool zero_dist = !(d > 0.0);
-    double ct = zero_dist ? 0.0 : dz / d;
-    ct = fmin(1.0, fmax(-1.0, ct));
-    const double st = sqrt(1.0 - ct * ct);
-    const double rho2 = dx * dx + dy * dy;
-    double c1 = 1.0, s1 = 0.0;
-    if (rho2 > 0.0) {
-        const double ir = 1.0 / sqrt(rho2);
-        c1 = dx * ir;
-        s1 = dy * ir;
-    }
-    const int adm = dm < 0 ? -dm : dm;
-    // e^{i dm phi} by binary powering of e^{i phi}.
-    double er = 1.0, ei = 0.0, br = c1, bi = s1;
-    for (int n = adm; n; n >>= 1) {
-        if (n & 1) {
-            const double t = er * br - ei * bi;
-            ei = er * bi + ei * br;
-            er = t;
-        }
-        const double t = br * br - bi * bi;
-        bi = 2.0 * br * bi;
-        br = t;
-    }
-    if (dm < 0)
-        ei = -ei;
-    // Hankel: (ar, ai) = h_p, (hr, hi) = h_{p+1}.
-    const double x = zero_dist ? 1.0 : k * d;
-    double sx, cx;
-    sincos(x, &sx, &cx);
-    const double inv = 1.0 / x;
-    double ar = sx * inv, ai = -cx * inv;
-    double hr = (ar - cx) * inv, hi = (ai - sx) * inv;
-    // Legendre P_l^adm: cur = P_l, prv = P_{l-1}, starting at l = adm.
-    double cur = ld[adm], prv = 0.0;
-    for (int i = 0; i < adm; ++i)
-        cur *= st;
-    for (int p = 0; p < P; ++p) {
-        if (p >= adm) {
-            const double phr = cur * ar, phi = cur * ai;
-            double gr = phr * er - phi * ei;
-            double gi = phr * ei + phi * er;
-            if (zero_dist) {  // h_p(0) = j_p(0) = delta_p0, phi = 0, cos th = 0
-                gr = (p == 0) ? cur : 0.0;
-                gi = 0.0;
-            }
-            if (!valid) {
-                gr = 0.0;
-                gi = 0.0;
-            }
-            T2 g;
-            g.x = gr;
-            g.y = gi;
-            dst[(p - adm) * stride] = g;
-            const double nx = la[p * P + adm] * ct * cur - lb[p * P + adm] * prv;
-            prv = cur;
-            cur = nx;
-        }
-        const double f = (2 * p + 3) * inv;
-        const double nr = f * hr - ar, ni = f * hi - ai;
-        ar = hr;
-        ai = hi;
-        hr = nr;
-        hi = ni;
-    }
-}
-
-// One block (4 waves) per (job, 16-particle target tile); jobs are sorted by
+// One block of 8 waves per (job, 16-particle target tile); jobs are sorted by
 // cost and the block index is job-major so the heaviest jobs start first.
-// Per chunk of 4*KS sources: (1) every pair's G[p] for all of the job's
-// orders is generated once into LDS in the MFMA A-operand layout and the
-// job's source columns of x are staged as the B operand; (2) each wave runs
-// the complex 16x16x4 MFMAs of its C tiles (c = wave, wave+4, ...).
+// Waves 4..7 produce, waves 0..3 consume, over chunks of 16 sources:
+//   producers: chunk i+1 -> LDS buffer (i+1)&1: every pair's G[p] for all of
+//              the job's orders (FP64 series, stored in the MFMA A layout) and
+//              the job's source columns of x (the B operand);
+//   consumers: complex 16x16x4 MFMAs of their C tiles (c = wave, wave+4, ...)
+//              on chunk i from buffer i&1.
+// One barrier per chunk hands the buffers over, so generation runs on the
+// VALU while the matrix cores work.
 // Q layout: [target tile][q][col 16][row 16] complex T (interleaved).
-template <typename T, int KS>
-__global__ __launch_bounds__(256) void pair_kernel(const double4 *__restrict__ pos,
+template <typename T>
+__global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ pos,
                                                    const typename Cplx<T>::type *__restrict__ x, int N,
                                                    int nmax, double k, const PairJob *__restrict__ jobs,
                                                    int ntiles1, const int *__restrict__ colmap,
                                                    int maxcols, int lmax, const double *__restrict__ la,
                                                    const double *__restrict__ lb,
-                                                   const double *__restrict__ ld, T *__restrict__ Q,
-                                                   int nq) {
+                                                   const double *__restrict__ ld, T *__restrict__ Q, int nq,
+                                                   int stage_elems) {
     typedef typename Mfma<T>::acc_t acc_t;
     typedef typename Cplx<T>::type T2;
-    constexpr int SRC = 4 * KS;  // sources per chunk
+    constexpr int SRC = 16;  // sources per chunk (4 MFMA K-steps)
     extern __shared__ unsigned char lds_raw[];
-    T2 *Ash = reinterpret_cast<T2 *>(lds_raw);  // [R][64*KS]
+    T2 *stage = reinterpret_cast<T2 *>(lds_raw);  // [2][stage_elems]: A [R][256] then B [16][ncols]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool producer = wave >= kWavesPerJob;
     const int jid = blockIdx.x / ntiles1;
     const int s1t = blockIdx.x - jid * ntiles1;
     const int dm = uniform(jobs[jid].dm);
@@ -170,13 +57,56 @@ __global__ __launch_bounds__(256) void pair_kernel(const double4 *__restrict__ p
     const int R = P - (dm < 0 ? -dm : dm);
     const int ncols = ntile * 16;
     const int total = R * ntile;
-    T2 *Bsh = Ash + R * 64 * KS;  // [SRC][ncols]
+    const int boff = R * 256;
     const int *cmap = colmap + (dm + 2 * lmax) * maxcols + tile0 * 16;
+    const int nchunks = (N + SRC - 1) / SRC;
 
-    const int lo = lane & 15, hi = lane >> 4;
-    const int gr = tid & 15, gk = tid >> 4;  // generation: target row, source in chunk
+    // Producers (waves 4..7): one (target gr, source gk) pair per thread; they
+    // generate G for chunk ch+1 while the consumers multiply chunk ch, and
+    // prefetch the next source position one chunk ahead.
+    // Consumers (waves 0..3): besides their MFMAs they stage the B operand:
+    // thread (source bk, column bc + 16 j) keeps chunk ch+1's x values in
+    // registers, loaded before the MFMAs of chunk ch and stored after them.
+    const int ptid = tid - kWavesPerJob * 64;
+    const int gr = ptid & 15, gk = ptid >> 4;
     const int s1 = s1t * 16 + gr;
-    const double4 p1 = pos[s1];  // pos is padded to a multiple of 16
+    const int bc = tid & 15, bk = tid >> 4;
+    double4 p1 = make_double4(0, 0, 0, 0), p2next = make_double4(0, 0, 0, 0);
+    int n2reg[kJobMaxColTiles];
+    T2 breg[kJobMaxColTiles];
+    if (producer) {
+        p1 = pos[s1];  // pos is padded to a multiple of 16
+        p2next = pos[gk < N ? gk : 0];
+    } else {
+#pragma unroll
+        for (int j = 0; j < kJobMaxColTiles; ++j)
+            n2reg[j] = (j < ntile) ? cmap[j * 16 + bc] : -1;
+    }
+    auto load_b = [&](int chunk) {
+        const int s2 = chunk * SRC + bk;
+#pragma unroll
+        for (int j = 0; j < kJobMaxColTiles; ++j) {
+            breg[j].x = T(0);
+            breg[j].y = T(0);
+            if (n2reg[j] >= 0 && s2 < N)
+                breg[j] = x[static_cast<size_t>(s2) * nmax + n2reg[j]];
+        }
+    };
+    auto store_b = [&](T2 *buf) {
+#pragma unroll
+        for (int j = 0; j < kJobMaxColTiles; ++j)
+            if (j < ntile)
+                buf[boff + bk * ncols + j * 16 + bc] = breg[j];
+    };
+    auto produce = [&](int chunk, T2 *buf) {
+        const int s2 = chunk * SRC + gk;
+        const double4 p2 = p2next;
+        const int s2n = s2 + SRC;
+        p2next = pos[s2n < N ? s2n : 0];
+        const bool valid = (s2 < N) && (s1 < N) && (s2 != s1);
+        pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld,
+                        buf + ptid, 256);
+    };
 
     acc_t acc[kWaveMaxTiles][2];
 #pragma unroll
@@ -184,45 +114,50 @@ __global__ __launch_bounds__(256) void pair_kernel(const double4 *__restrict__ p
         acc[j][0] = acc_t{0, 0, 0, 0};
         acc[j][1] = acc_t{0, 0, 0, 0};
     }
+    const int lo = lane & 15, hi = lane >> 4;
 
-    for (int c0 = 0; c0 < N; c0 += SRC) {
-        if (gk < SRC) {
-            const int s2 = c0 + gk;
-            const bool valid = (s2 < N) && (s1 < N) && (s2 != s1);
-            const double4 p2 = pos[s2 < N ? s2 : 0];
-            pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld,
-                            Ash + tid, 64 * KS);
-        }
-        for (int e = tid; e < SRC * ncols; e += blockDim.x) {
-            const int kk = e / ncols, c = e - kk * ncols;
-            const int n2 = cmap[c];
-            const int s2 = c0 + kk;
-            T2 v;
-            v.x = T(0);
-            v.y = T(0);
-            if (n2 >= 0 && s2 < N)
-                v = x[static_cast<size_t>(s2) * nmax + n2];
-            Bsh[e] = v;
-        }
-        __syncthreads();
+    if (producer) {
+        produce(0, stage);
+    } else {
+        load_b(0);
+        store_b(stage);
+        if (nchunks > 1)
+            load_b(1);
+    }
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ++ch) {
+        T2 *cur = stage + (ch & 1) * stage_elems;
+        T2 *nxt = stage + ((ch + 1) & 1) * stage_elems;
+        if (producer) {
+            if (ch + 1 < nchunks)
+                produce(ch + 1, nxt);
+        } else {
 #pragma unroll
-        for (int j = 0; j < kWaveMaxTiles; ++j) {
-            const int c = wave + kWavesPerJob * j;
-            if (c < total) {
-                const int ip = c / ntile, it = c - ip * ntile;
+            for (int j = 0; j < kWaveMaxTiles; ++j) {
+                const int c = wave + kWavesPerJob * j;
+                if (c < total) {
+                    const int ip = c / ntile, it = c - ip * ntile;
 #pragma unroll
-                for (int ks = 0; ks < KS; ++ks) {
-                    const T2 a = Ash[ip * 64 * KS + ks * 64 + lane];
-                    const T2 bv = Bsh[(ks * 4 + hi) * ncols + it * 16 + lo];
-                    acc[j][0] = Mfma<T>::mac(a.x, bv.x, acc[j][0]);
-                    acc[j][0] = Mfma<T>::mac(-a.y, bv.y, acc[j][0]);
-                    acc[j][1] = Mfma<T>::mac(a.x, bv.y, acc[j][1]);
-                    acc[j][1] = Mfma<T>::mac(a.y, bv.x, acc[j][1]);
+                    for (int ks = 0; ks < 4; ++ks) {
+                        const T2 a = cur[ip * 256 + ks * 64 + lane];
+                        const T2 bv = cur[boff + (ks * 4 + hi) * ncols + it * 16 + lo];
+                        acc[j][0] = Mfma<T>::mac(a.x, bv.x, acc[j][0]);
+                        acc[j][0] = Mfma<T>::mac(-a.y, bv.y, acc[j][0]);
+                        acc[j][1] = Mfma<T>::mac(a.x, bv.y, acc[j][1]);
+                        acc[j][1] = Mfma<T>::mac(a.y, bv.x, acc[j][1]);
+                    }
                 }
+            }
+            if (ch + 1 < nchunks) {
+                store_b(nxt);
+                if (ch + 2 < nchunks)
+                    load_b(ch + 2);
             }
         }
         __syncthreads();
     }
+    if (producer)
+        return;
 
     // Accumulator element i: target row Mfma<T>::row(i, hi), column lo.
 #pragma unroll
@@ -282,52 +217,34 @@ __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q,
     }
 }
 
-// Device-side views of a CouplingPlan.
-struct DevicePlan {
-    int lmax, nmax, maxcols, nq, njobs, max_job_cols;
-    int ks;  // source K-steps per LDS chunk: 2, 4, or 0 = largest that fits
-    PairJob *jobs;
-    int *colmap, *crow, *cidx;
-    double2 *cval;
-    double *la, *lb, *ld;
-};
-
-template <typename T, int KS>
-static size_t pair_lds_bytes(const DevicePlan &dp) {
-    return (static_cast<size_t>(2 * dp.lmax + 1) * 64 * KS + 4 * KS * dp.max_job_cols) *
-           sizeof(typename Cplx<T>::type);
-}
-
-template <typename T, int KS>
+template <typename T>
 static void launch_pair_t(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
                           void *Q, hipStream_t s) {
     const int ntiles1 = (N + 15) / 16;
-    const size_t lds = pair_lds_bytes<T, KS>(dp);
-    hipLaunchKernelGGL((pair_kernel<T, KS>), dim3(dp.njobs * ntiles1), dim3(256), lds, s, pos, static_cast<const typename Cplx<T>::type *>(x), N, dp.nmax, k, dp.jobs,
+    const size_t lds = 2 * static_cast<size_t>(dp.stage_elems) * sizeof(typename Cplx<T>::type);
+    static bool lds_opt_in = false;  // gfx950: up to 160 KB of LDS per workgroup
+    if (!lds_opt_in) {
+        YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pair_kernel<T>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        lds_opt_in = true;
+    }
+    hipLaunchKernelGGL((pair_kernel<T>), dim3(dp.njobs * ntiles1), dim3(2 * kWavesPerJob * 64), lds, s,
+                       pos, static_cast<const typename Cplx<T>::type *>(x), N, dp.nmax, k, dp.jobs,
                        ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.la, dp.lb, dp.ld, static_cast<T *>(Q),
-                       dp.nq);
+                       dp.nq, dp.stage_elems);
     YH_CHECK(hipGetLastError());
 }
 
-constexpr size_t kPairLdsLimit = 64 * 1024;
-
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
                  void *Q, hipStream_t s) {
-    const bool ks4_fits = fp32 ? pair_lds_bytes<float, 4>(dp) <= kPairLdsLimit
-                               : pair_lds_bytes<double, 4>(dp) <= kPairLdsLimit;
-    const bool use4 = ks4_fits && dp.ks != 2;
-    if (fp32) {
-        if (use4)
-            launch_pair_t<float, 4>(dp, pos, N, k, x, Q, s);
-        else
-            launch_pair_t<float, 2>(dp, pos, N, k, x, Q, s);
-    } else {
-        if (use4)
-            launch_pair_t<double, 4>(dp, pos, N, k, x, Q, s);
-        else
-            launch_pair_t<double, 2>(dp, pos, N, k, x, Q, s);
-    }
+    if (fp32)
+        launch_pair_t<float>(dp, pos, N, k, x, Q, s);
+    else
+        launch_pair_t<double>(dp, pos, N, k, x, Q, s);
 }
+
+// Largest LDS the pair kernel asks for, bytes (2 stages of complex128).
+size_t pair_lds_bytes(const DevicePlan &dp) { return 2 * static_cast<size_t>(dp.stage_elems) * 16; }
 
 // fp32: Q, x and y are complex64; otherwise complex128.
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,

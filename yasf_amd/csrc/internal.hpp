@@ -67,14 +67,14 @@ struct PairJob {
 constexpr int kWavesPerJob = 4;
 constexpr int kWaveMaxTiles = 8;                                // C tiles per wave
 constexpr int kJobMaxTiles = kWavesPerJob * kWaveMaxTiles;      // C tiles per job
-constexpr int kJobMaxColTiles = 8;                              // column tiles per job
+constexpr int kJobMaxColTiles = 2;                              // column tiles per job
 
 // Everything the coupling kernels need for one lmax.
 struct CouplingPlan {
     int lmax = 0, nmax = 0, ndm = 0, maxcols = 0, nq = 0;
     std::vector<int> colmap;          // [ndm][maxcols] -> source mode n2 or -1
     std::vector<PairJob> jobs;        // per 16-particle target tile, sorted by cost
-    int max_job_cols = 0;             // widest job in columns (ntile*16)
+    int stage_elems = 0;              // complex elements of one pair-kernel LDS stage
     std::vector<int> crow;            // CSR row offsets per target mode n1 (nmax+1)
     std::vector<int> cidx;            // Q column index q*16 + col
     std::vector<double> cval;         // ab5 entry (re, im) interleaved
@@ -85,5 +85,28 @@ struct CouplingPlan {
     long long nnz_ab5 = 0;
 };
 CouplingPlan build_coupling_plan(int lmax);
+
+// Device-side views of a CouplingPlan (coupling.hip, pairtable.hip).
+struct DevicePlan {
+    int lmax, nmax, maxcols, nq, njobs, stage_elems;
+    PairJob *jobs;
+    int *colmap, *crow, *cidx;
+    double2 *cval;
+    double *la, *lb, *ld;
+    int *dmoff;  // [ndm] first order slot of each dm in a pair-table target tile
+};
+
+// On-the-fly pair GEMM (pair factors generated in the kernel).
+void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
+                 void *Q, hipStream_t s);
+// ab5 contraction + optional master-operator epilogue.
+void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
+                     int mode, void *y, bool fp32, hipStream_t s);
+// Resident pair-factor table: bytes, build, and the streaming pair GEMM.
+size_t pair_table_bytes(const DevicePlan &dp, int N, bool fp32);
+void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,
+                             void *table, hipStream_t s);
+void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
+                            void *Q, hipStream_t s);
 
 }  // namespace yasfhip

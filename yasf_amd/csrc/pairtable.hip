@@ -1,0 +1,202 @@
+// Resident pair-factor table and the streaming pair GEMM.
+//
+// The pair factors G_{s1 s2}[p, dm] depend only on the geometry and the wave
+// number, not on x.  When HBM allows (a 2,000-sphere lmax-6 cluster needs
+// 10.8 GB in complex128), they are generated once per session and channel, in
+// FP64, directly in the 16x16x4 MFMA A-operand layout:
+//   table[target tile t][dm d][K-step ks][order ip][lane]      (complex T)
+//   lane = (source - 4 ks) * 16 + target-in-tile,  ip = p - |dm|
+// so every matvec is a pure streaming complex GEMM with no special functions:
+// each wave loads its A fragments straight from HBM (1 KB coalesced per
+// fragment), gathers its B fragments from x, and issues the MFMAs, with the
+// next K-step's operands in flight.  No LDS, no barriers.
+#include "device_common.hpp"
+
+namespace yasfhip {
+
+namespace {
+
+// One block per (target tile, dm, 16-source chunk); thread = one pair.
+template <typename T2>
+__global__ __launch_bounds__(256) void pair_table_build_kernel(
+    const double4 *__restrict__ pos, int N, int N4, double k, int lmax, int ndm,
+    const int *__restrict__ dmoff, const double *__restrict__ la, const double *__restrict__ lb,
+    const double *__restrict__ ld, T2 *__restrict__ table) {
+    const int nchunk = (N + 15) / 16;
+    const int c = blockIdx.x % nchunk;
+    const int d = (blockIdx.x / nchunk) % ndm;
+    const int t = blockIdx.x / (nchunk * ndm);
+    const int P = 2 * lmax + 1;
+    const int SR = P * P;
+    const int dm = d - 2 * lmax;
+    const int R = P - (dm < 0 ? -dm : dm);
+    const int r = threadIdx.x & 15, kk = threadIdx.x >> 4;
+    const int s1 = t * 16 + r, s2 = c * 16 + kk;
+    const int ks = s2 >> 2;
+    if (ks >= N4)
+        return;
+    const int lane = (s2 & 3) * 16 + r;
+    const bool valid = (s1 < N) && (s2 < N) && (s1 != s2);
+    const double4 p1 = pos[s1];  // pos is padded to a multiple of 16
+    const double4 p2 = pos[s2 < N ? s2 : 0];
+    T2 *dst = table + ((static_cast<size_t>(t) * SR + dmoff[d]) * N4 + static_cast<size_t>(ks) * R) * 64 +
+              lane;
+    pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld, dst, 64);
+}
+
+// One block of 4 waves per (job, target tile).  Wave w owns the orders
+// ip = w, w+4, ... (< R) of the job and all of its column tiles; its C tile
+// (ip, it) goes to Q slot qbase + ip * ntile + it (the same slots as the
+// on-the-fly kernel, so the contraction lists are shared).
+template <typename T, int IPW>
+__global__ __launch_bounds__(256) void pair_table_gemm_kernel(
+    const typename Cplx<T>::type *__restrict__ table, const typename Cplx<T>::type *__restrict__ x, int N,
+    int N4, int nmax, const PairJob *__restrict__ jobs, int ntiles1, const int *__restrict__ colmap,
+    int maxcols, int lmax, const int *__restrict__ dmoff, T *__restrict__ Q, int nq) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    typedef typename Cplx<T>::type T2;
+    constexpr int NT = kJobMaxColTiles;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int jid = blockIdx.x / ntiles1;
+    const int t = blockIdx.x - jid * ntiles1;
+    const int dm = uniform(jobs[jid].dm);
+    const int tile0 = uniform(jobs[jid].tile0);
+    const int ntile = uniform(jobs[jid].ntile);
+    const int qbase = uniform(jobs[jid].qbase);
+    const int P = 2 * lmax + 1;
+    const int R = P - (dm < 0 ? -dm : dm);
+    const int d = dm + 2 * lmax;
+    const int lo = lane & 15, hi = lane >> 4;
+
+    const T2 *tab = table + (static_cast<size_t>(t) * P * P + dmoff[d]) * N4 * 64 + lane;
+    int n2c[NT];
+#pragma unroll
+    for (int it = 0; it < NT; ++it)
+        n2c[it] = (it < ntile) ? colmap[d * maxcols + (tile0 + it) * 16 + lo] : -1;
+
+    acc_t acc[IPW][NT][2];
+#pragma unroll
+    for (int j = 0; j < IPW; ++j)
+#pragma unroll
+        for (int it = 0; it < NT; ++it) {
+            acc[j][it][0] = acc_t{0, 0, 0, 0};
+            acc[j][it][1] = acc_t{0, 0, 0, 0};
+        }
+
+    T2 an[IPW], bn[NT];
+    auto load = [&](int ks) {
+#pragma unroll
+        for (int j = 0; j < IPW; ++j) {
+            const int ip = wave + 4 * j;
+            an[j].x = T(0);
+            an[j].y = T(0);
+            if (ip < R)
+                an[j] = tab[(static_cast<size_t>(ks) * R + ip) * 64];
+        }
+        const int s2 = ks * 4 + hi;
+#pragma unroll
+        for (int it = 0; it < NT; ++it) {
+            bn[it].x = T(0);
+            bn[it].y = T(0);
+            if (n2c[it] >= 0 && s2 < N)
+                bn[it] = x[static_cast<size_t>(s2) * nmax + n2c[it]];
+        }
+    };
+
+    load(0);
+    for (int ks = 0; ks < N4; ++ks) {
+        T2 a[IPW], b[NT];
+#pragma unroll
+        for (int j = 0; j < IPW; ++j)
+            a[j] = an[j];
+#pragma unroll
+        for (int it = 0; it < NT; ++it)
+            b[it] = bn[it];
+        if (ks + 1 < N4)
+            load(ks + 1);
+#pragma unroll
+        for (int j = 0; j < IPW; ++j) {
+            if (wave + 4 * j < R) {
+#pragma unroll
+                for (int it = 0; it < NT; ++it) {
+                    if (it < ntile) {
+                        acc[j][it][0] = Mfma<T>::mac(a[j].x, b[it].x, acc[j][it][0]);
+                        acc[j][it][0] = Mfma<T>::mac(-a[j].y, b[it].y, acc[j][it][0]);
+                        acc[j][it][1] = Mfma<T>::mac(a[j].x, b[it].y, acc[j][it][1]);
+                        acc[j][it][1] = Mfma<T>::mac(a[j].y, b[it].x, acc[j][it][1]);
+                    }
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int j = 0; j < IPW; ++j) {
+        const int ip = wave + 4 * j;
+        if (ip >= R)
+            continue;
+#pragma unroll
+        for (int it = 0; it < NT; ++it) {
+            if (it >= ntile)
+                continue;
+            T *dst = Q + 2 * (((static_cast<size_t>(t) * nq + qbase + ip * ntile + it) * 16 + lo) * 16);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = Mfma<T>::row(i, hi);
+                dst[2 * r] = acc[j][it][0][i];
+                dst[2 * r + 1] = acc[j][it][1][i];
+            }
+        }
+    }
+}
+
+template <typename T>
+void gemm_t(const DevicePlan &dp, int N, const void *table, const void *x, void *Q, hipStream_t s) {
+    typedef typename Cplx<T>::type T2;
+    const int ntiles1 = (N + 15) / 16;
+    const int N4 = (N + 3) / 4;
+    const dim3 grid(dp.njobs * ntiles1), block(256);
+    const T2 *tb = static_cast<const T2 *>(table);
+    const T2 *xv = static_cast<const T2 *>(x);
+    T *q = static_cast<T *>(Q);
+    // orders per wave: ceil(R/4) <= ceil((2L+1)/4)
+    if (2 * dp.lmax + 1 <= 16)
+        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 4>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
+                           dp.jobs, ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
+    else
+        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 5>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
+                           dp.jobs, ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
+    YH_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+size_t pair_table_bytes(const DevicePlan &dp, int N, bool fp32) {
+    const size_t P = 2 * dp.lmax + 1;
+    const size_t ntiles1 = (N + 15) / 16, N4 = (N + 3) / 4;
+    return ntiles1 * P * P * N4 * 64 * (fp32 ? sizeof(float2) : sizeof(double2));
+}
+
+void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,
+                             void *table, hipStream_t s) {
+    const int ntiles1 = (N + 15) / 16, N4 = (N + 3) / 4, nchunk = (N + 15) / 16;
+    const int ndm = 4 * dp.lmax + 1;
+    const dim3 grid(ntiles1 * ndm * nchunk), block(256);
+    if (fp32)
+        hipLaunchKernelGGL(pair_table_build_kernel<float2>, grid, block, 0, s, pos, N, N4, k, dp.lmax,
+                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<float2 *>(table));
+    else
+        hipLaunchKernelGGL(pair_table_build_kernel<double2>, grid, block, 0, s, pos, N, N4, k, dp.lmax,
+                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<double2 *>(table));
+    YH_CHECK(hipGetLastError());
+}
+
+void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
+                            void *Q, hipStream_t s) {
+    if (fp32)
+        gemm_t<float>(dp, N, table, x, Q, s);
+    else
+        gemm_t<double>(dp, N, table, x, Q, s);
+}
+
+}  // namespace yasfhip

@@ -161,7 +161,7 @@ CouplingPlan build_coupling_plan(int lmax) {
                         slot[{dmi, std::abs(dm) + ip, (t0 + it) * 16 + c}] = (q + ip * nt + it) * 16 + c;
             q += R * nt;
             macs += static_cast<long long>(R) * nt * 16;
-            plan.max_job_cols = std::max(plan.max_job_cols, nt * 16);
+            plan.stage_elems = std::max(plan.stage_elems, R * 256 + 16 * nt * 16);
             plan.jobs.push_back(job);
         }
     }
