@@ -89,6 +89,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-precond", action="store_true")
     ap.add_argument("--per-box", type=float, default=8.0)
+    ap.add_argument("--box-cap", type=int, default=12, help="bisect boxes above this many spheres")
     ap.add_argument("--restart", type=int, default=100)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=48, help="target particles in the CPU sample")
@@ -117,7 +118,7 @@ def main():
     boxes = None
     if use_pc:
         part = split_oversized(make_particle_partition(pos, edge_size_for(pos, args.per_box)), pos,
-                               MAX_BLOCK_ROWS // 96)
+                               min(MAX_BLOCK_ROWS // 96, args.box_cap))
         boxes = np.empty(N, dtype=np.int32)
         for i, members in enumerate(part):
             boxes[members] = i
@@ -170,7 +171,8 @@ def main():
         "config": {"workload": "dense FP64 solve: 2000 spheres, lmax 6, vf 0.2, lambda 550 nm, "
                                "GMRES(100) to 1e-6" + (", block-diag precond ~8/box" if use_pc else ""),
                    "particles": N, "lmax": lmax, "unknowns": N * 96,
-                   "preconditioner": use_pc, "restart": args.restart},
+                   "preconditioner": use_pc, "restart": args.restart,
+                   "box_cap": args.box_cap if use_pc else None},
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "gmres_iterations": iters,

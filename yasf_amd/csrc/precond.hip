@@ -14,7 +14,7 @@
 #include <algorithm>
 #include <numeric>
 
-#include "internal.hpp"
+#include "device_common.hpp"
 
 namespace yasfhip {
 
@@ -25,8 +25,10 @@ struct BlockPrecond {
     int *particles = nullptr;    // particle ids ordered by box
     long long *blk_off = nullptr;  // [nbox] element offset of each block
     int *piv_off = nullptr;      // [nbox] offset of each block's pivots
+    int *dinv_off = nullptr;     // [nbox] first 16-row diagonal block of each box
     double2 *blocks = nullptr;
     int *pivots = nullptr;
+    double2 *dinv = nullptr;     // per 16-row diagonal block: L11^-1 then U11^-1 (2 x 16 x 16)
     long long total_elems = 0;
     int max_m = 0;  // largest padded block size
 };
@@ -36,111 +38,76 @@ constexpr int kMaxBlock = 3840;
 
 namespace {
 
-// Coupling entry W[(s1,n1),(s2,n2)] (reference cpu_numba.py:74-91 with the
-// tables of misc.py:149-229 evaluated for this one pair).
-__device__ double2 coupling_entry(double dx, double dy, double dz, double k, int l1, int m1, int tau1,
-                                  int n1, int l2, int m2, int tau2, int n2, int nmax, int P,
-                                  const double2 *__restrict__ ab5, const double *__restrict__ la,
-                                  const double *__restrict__ lb, const double *__restrict__ ld) {
-    const double d = sqrt(dx * dx + dy * dy + dz * dz);
-    double ct = d > 0.0 ? dz / d : 0.0;
-    ct = fmin(1.0, fmax(-1.0, ct));
-    const double st = sqrt(1.0 - ct * ct);
-    const double rho2 = dx * dx + dy * dy;
-    double c1 = 1.0, s1 = 0.0;
-    if (rho2 > 0.0) {
-        const double ir = 1.0 / sqrt(rho2);
-        c1 = dx * ir;
-        s1 = dy * ir;
-    }
-    const int dm = m2 - m1;
-    const int adm = dm < 0 ? -dm : dm;
-    double er = 1.0, ei = 0.0, br = c1, bi = s1;
-    for (int q = adm; q; q >>= 1) {
-        if (q & 1) {
-            const double t = er * br - ei * bi;
-            ei = er * bi + ei * br;
-            er = t;
-        }
-        const double t = br * br - bi * bi;
-        bi = 2.0 * br * bi;
-        br = t;
-    }
-    if (dm < 0)
-        ei = -ei;
-    const double x = k * d;
-    double sx, cx;
-    sincos(x, &sx, &cx);
-    const double inv = 1.0 / x;
-    double ar = sx * inv, ai = -cx * inv;          // h_p
-    double hr = (ar - cx) * inv, hi = (ai - sx) * inv;  // h_{p+1}
-    double cur = ld[adm], prv = 0.0;                // P_p^adm from p = adm
-    for (int i = 0; i < adm; ++i)
-        cur *= st;
-    const int dl = l1 > l2 ? l1 - l2 : l2 - l1;
-    const int dt = tau1 > tau2 ? tau1 - tau2 : tau2 - tau1;
-    const int p_start = max(adm, dl + dt);
-    double sr = 0.0, si = 0.0;
-    for (int p = 0; p <= l1 + l2; ++p) {
-        if (p >= p_start) {
-            const double2 a = ab5[(static_cast<long>(n2) * nmax + n1) * P + p];
-            const double phr = cur * ar, phi = cur * ai;
-            sr += a.x * phr - a.y * phi;
-            si += a.x * phi + a.y * phr;
-        }
-        const double f = (2 * p + 3) * inv;
-        const double nr = f * hr - ar, ni = f * hi - ai;
-        ar = hr;
-        ai = hi;
-        hr = nr;
-        hi = ni;
-        if (p >= adm && p + 1 < P) {
-            const double nx = la[p * P + adm] * ct * cur - lb[p * P + adm] * prv;
-            prv = cur;
-            cur = nx;
-        }
-    }
-    return make_double2(sr * er - si * ei, sr * ei + si * er);
-}
-
 __device__ __forceinline__ int padded16(int m) { return (m + 15) & ~15; }
 
-// One block per box; threads stride over the box's mp x mp elements
-// (mp = m rounded up to 16; the padding is the identity).
+// One block per (box, target particle a1): for each source particle a2 of the
+// box the pair's factors G[p, dm] (all (2L+1)^2 of them) are generated once
+// into LDS, then the nmax x nmax sub-block
+//   M[(a1,n1),(a2,n2)] = -t[s1,n1] * sum_p ab5[n2,n1,p] G[p, m2-m1]
+// is written through a CSR over (n1, n2) of the nonzero ab5 entries
+// (the reference's coupling formula, cpu_numba.py:74-91).  Diagonal
+// sub-blocks are the identity.
 __global__ __launch_bounds__(256) void build_blocks_kernel(
     const double4 *__restrict__ pos, const int *__restrict__ box_first, const int *__restrict__ particles,
-    const long long *__restrict__ blk_off, const int4 *__restrict__ modes, int nmax, int lmax, double k,
-    const double2 *__restrict__ t, const double2 *__restrict__ ab5, const double *__restrict__ la,
+    const long long *__restrict__ blk_off, const int2 *__restrict__ rowblocks, int nmax, int lmax,
+    double k, const double2 *__restrict__ t, const int *__restrict__ erow, const int *__restrict__ eg,
+    const double2 *__restrict__ ev, const int *__restrict__ dmoff, const double *__restrict__ la,
     const double *__restrict__ lb, const double *__restrict__ ld, double2 *__restrict__ blocks) {
-    const int b = blockIdx.x;
+    __shared__ double2 G[(2 * kMaxLmax + 1) * (2 * kMaxLmax + 1)];
+    const int b = rowblocks[blockIdx.x].x, a1 = rowblocks[blockIdx.x].y;
     const int first = box_first[b];
     const int np_box = box_first[b + 1] - first;
-    const long m = static_cast<long>(np_box) * nmax;
-    const long mp = padded16(static_cast<int>(m));
-    double2 *A = blocks + blk_off[b];
-    const int P = 2 * lmax + 1;
-    for (long e = threadIdx.x; e < mp * mp; e += blockDim.x) {
-        const long i = e / mp, j = e % mp;
-        double2 v;
-        if (i >= m || j >= m) {
-            v = make_double2(i == j ? 1.0 : 0.0, 0.0);
-        } else {
-            const int a1 = static_cast<int>(i / nmax), n1 = static_cast<int>(i % nmax);
-            const int a2 = static_cast<int>(j / nmax), n2 = static_cast<int>(j % nmax);
-            if (a1 == a2) {
-                v = make_double2(i == j ? 1.0 : 0.0, 0.0);
-            } else {
-                const int s1 = particles[first + a1], s2 = particles[first + a2];
-                const double4 p1 = pos[s1], p2 = pos[s2];
-                const int4 md1 = modes[n1], md2 = modes[n2];
-                const double2 w = coupling_entry(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, md1.y, md1.z,
-                                                 md1.x, n1, md2.y, md2.z, md2.x, n2, nmax, P, ab5, la,
-                                                 lb, ld);
-                const double2 tt = t[static_cast<long>(s1) * nmax + n1];
-                v = make_double2(-(tt.x * w.x - tt.y * w.y), -(tt.x * w.y + tt.y * w.x));
+    const int mp = padded16(np_box * nmax);
+    double2 *A = blocks + blk_off[b] + static_cast<long>(a1) * nmax * mp;
+    const int P = 2 * lmax + 1, ndm = 4 * lmax + 1;
+    const int s1 = particles[first + a1];
+    const double4 p1 = pos[s1];
+    const int nn = nmax * nmax;
+    for (int a2 = 0; a2 < np_box; ++a2) {
+        if (a2 == a1) {
+            for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+                const int n1 = e / nmax, n2 = e - n1 * nmax;
+                A[static_cast<long>(n1) * mp + a2 * nmax + n2] = make_double2(n1 == n2 ? 1.0 : 0.0, 0.0);
             }
+            continue;
         }
-        A[e] = v;
+        const double4 p2 = pos[particles[first + a2]];
+        if (threadIdx.x < ndm) {
+            const int dm = static_cast<int>(threadIdx.x) - 2 * lmax;
+            pair_series<double2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, true, la, lb, ld,
+                                 G + dmoff[threadIdx.x], 1);
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+            const int n1 = e / nmax, n2 = e - n1 * nmax;
+            double sr = 0.0, si = 0.0;
+            for (int q = erow[e]; q < erow[e + 1]; ++q) {
+                const double2 a = ev[q], g = G[eg[q]];
+                sr += a.x * g.x - a.y * g.y;
+                si += a.x * g.y + a.y * g.x;
+            }
+            const double2 tt = t[static_cast<long>(s1) * nmax + n1];
+            A[static_cast<long>(n1) * mp + a2 * nmax + n2] =
+                make_double2(-(tt.x * sr - tt.y * si), -(tt.x * si + tt.y * sr));
+        }
+        __syncthreads();
+    }
+}
+
+// Identity in the padding of each box block (rows/columns m..mp-1).
+__global__ __launch_bounds__(256) void pad_blocks_kernel(const int *__restrict__ box_first, int nmax,
+                                                         const long long *__restrict__ blk_off,
+                                                         double2 *__restrict__ blocks) {
+    const int b = blockIdx.x;
+    const int m = (box_first[b + 1] - box_first[b]) * nmax;
+    const int mp = padded16(m);
+    if (mp == m)
+        return;
+    double2 *A = blocks + blk_off[b];
+    for (long e = threadIdx.x; e < static_cast<long>(mp) * mp; e += blockDim.x) {
+        const int i = static_cast<int>(e / mp), j = static_cast<int>(e % mp);
+        if (i >= m || j >= m)
+            A[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
 }
 
@@ -165,7 +132,9 @@ __global__ __launch_bounds__(256) void lu_blocked_kernel(const int *__restrict__
                                                          const long long *__restrict__ blk_off,
                                                          const int *__restrict__ piv_off,
                                                          double2 *__restrict__ blocks,
-                                                         int *__restrict__ pivots) {
+                                                         int *__restrict__ pivots,
+                                                         const int *__restrict__ dinv_off,
+                                                         double2 *__restrict__ dinv) {
     __shared__ double red_v[4];
     __shared__ int red_i[4];
     __shared__ int piv_sh;
@@ -235,9 +204,10 @@ __global__ __launch_bounds__(256) void lu_blocked_kernel(const int *__restrict__
             }
             __syncthreads();
         }
-        if (kb + 16 >= mp)
-            break;
-        // Linv = L11^-1 (unit lower); thread c solves L11 y = e_c.
+        // Inverses of the diagonal 16x16 factors, kept for the substitutions:
+        // thread c < 16 solves L11 y = e_c (unit lower), thread 16 + c solves
+        // U11 z = e_c (upper).
+        double2 *Dinv = dinv + static_cast<long>(dinv_off[b] + kb / 16) * 512;
         if (tid < 16) {
             const int c = tid;
             double2 yv[16];
@@ -248,9 +218,23 @@ __global__ __launch_bounds__(256) void lu_blocked_kernel(const int *__restrict__
                     v = csub(v, cmul(at(kb + r, kb + q), yv[q]));
                 yv[r] = r < c ? make_double2(0.0, 0.0) : v;
                 Lsh[r][c] = yv[r];
+                Dinv[r * 16 + c] = yv[r];
+            }
+        } else if (tid < 32) {
+            const int c = tid - 16;
+            double2 zv[16];
+#pragma unroll
+            for (int r = 15; r >= 0; --r) {
+                double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+                for (int q = r + 1; q <= c; ++q)
+                    v = csub(v, cmul(at(kb + r, kb + q), zv[q]));
+                zv[r] = r > c ? make_double2(0.0, 0.0) : cmul(v, cinv(at(kb + r, kb + r)));
+                Dinv[256 + r * 16 + c] = zv[r];
             }
         }
         __syncthreads();
+        if (kb + 16 >= mp)
+            break;
         const int nt = (mp - kb - 16) >> 4;
         // U12 = Linv A12, 16x16 complex tiles on the MFMA (each wave owns whole
         // column tiles, so reading and overwriting a tile needs no barrier).
@@ -310,24 +294,28 @@ __global__ __launch_bounds__(256) void lu_blocked_kernel(const int *__restrict__
 }
 
 // z[box] = U^-1 L^-1 P r[box] with 16-row blocked substitutions; the
-// right-hand side lives in LDS (mp complex), diagonal 16x16 blocks are staged
-// in LDS and solved by one thread, off-diagonal updates use all threads.
+// right-hand side lives in LDS (mp complex).  Each 16-row diagonal step is a
+// 16x16 product with the stored L11^-1 / U11^-1, the off-diagonal updates use
+// all threads (one row per thread, 16 contiguous entries).
 __global__ __launch_bounds__(256) void lu_apply_kernel(const int *__restrict__ box_first,
                                                        const int *__restrict__ particles, int nmax,
                                                        const long long *__restrict__ blk_off,
                                                        const int *__restrict__ piv_off,
                                                        const double2 *__restrict__ blocks,
                                                        const int *__restrict__ pivots,
+                                                       const int *__restrict__ dinv_off,
+                                                       const double2 *__restrict__ dinv,
                                                        const double2 *__restrict__ r,
                                                        double2 *__restrict__ z) {
     extern __shared__ double2 y[];
-    __shared__ double2 D[16][16];
+    __shared__ double2 yb[16];
     const int b = blockIdx.x;
     const int first = box_first[b];
     const int m = (box_first[b + 1] - first) * nmax;
     const int mp = padded16(m);
     const double2 *A = blocks + blk_off[b];
     const int *piv = pivots + piv_off[b];
+    const double2 *D = dinv + static_cast<long>(dinv_off[b]) * 512;
     const int tid = threadIdx.x;
     for (int i = tid; i < mp; i += blockDim.x)
         y[i] = i < m ? r[static_cast<long>(particles[first + i / nmax]) * nmax + i % nmax]
@@ -346,52 +334,60 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(const int *__restrict__ b
     const int nb = mp >> 4;
     for (int bb = 0; bb < nb; ++bb) {  // unit lower
         const int base = bb * 16;
-        D[tid >> 4][tid & 15] = A[static_cast<long>(base + (tid >> 4)) * mp + base + (tid & 15)];
-        __syncthreads();
-        if (tid == 0)
-            for (int rr = 1; rr < 16; ++rr) {
-                double2 v = y[base + rr];
-                for (int q = 0; q < rr; ++q)
-                    v = csub(v, cmul(D[rr][q], y[base + q]));
-                y[base + rr] = v;
+        if (tid < 16) {
+            const double2 *Li = D + static_cast<long>(bb) * 512 + tid * 16;
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double2 v = cmul(Li[q], y[base + q]);
+                acc.x += v.x;
+                acc.y += v.y;
             }
+            yb[tid] = acc;
+        }
         __syncthreads();
-        double2 yb[16];
+        if (tid < 16)
+            y[base + tid] = yb[tid];
+        double2 yv[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q)
-            yb[q] = y[base + q];
+            yv[q] = yb[q];
         for (int i = base + 16 + tid; i < mp; i += blockDim.x) {
             double2 acc = y[i];
             const double2 *row = A + static_cast<long>(i) * mp + base;
 #pragma unroll
             for (int q = 0; q < 16; ++q)
-                acc = csub(acc, cmul(row[q], yb[q]));
+                acc = csub(acc, cmul(row[q], yv[q]));
             y[i] = acc;
         }
         __syncthreads();
     }
     for (int bb = nb - 1; bb >= 0; --bb) {  // upper with diagonal
         const int base = bb * 16;
-        D[tid >> 4][tid & 15] = A[static_cast<long>(base + (tid >> 4)) * mp + base + (tid & 15)];
-        __syncthreads();
-        if (tid == 0)
-            for (int rr = 15; rr >= 0; --rr) {
-                double2 v = y[base + rr];
-                for (int q = rr + 1; q < 16; ++q)
-                    v = csub(v, cmul(D[rr][q], y[base + q]));
-                y[base + rr] = cmul(v, cinv(D[rr][rr]));
+        if (tid < 16) {
+            const double2 *Ui = D + static_cast<long>(bb) * 512 + 256 + tid * 16;
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double2 v = cmul(Ui[q], y[base + q]);
+                acc.x += v.x;
+                acc.y += v.y;
             }
+            yb[tid] = acc;
+        }
         __syncthreads();
-        double2 yb[16];
+        if (tid < 16)
+            y[base + tid] = yb[tid];
+        double2 yv[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q)
-            yb[q] = y[base + q];
+            yv[q] = yb[q];
         for (int i = tid; i < base; i += blockDim.x) {
             double2 acc = y[i];
             const double2 *row = A + static_cast<long>(i) * mp + base;
 #pragma unroll
             for (int q = 0; q < 16; ++q)
-                acc = csub(acc, cmul(row[q], yb[q]));
+                acc = csub(acc, cmul(row[q], yv[q]));
             y[i] = acc;
         }
         __syncthreads();
@@ -461,30 +457,64 @@ BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &
         bp->blocks = dalloc<double2>(static_cast<size_t>(elems));
         bp->pivots = dalloc<int>(static_cast<size_t>(pv));
 
+        std::vector<int> doff;
+        std::vector<int2> rowblocks;
+        int nbtot = 0;
+        for (int bx = 0; bx < bp->nbox; ++bx) {
+            doff.push_back(nbtot);
+            const int cnt = first[bx + 1] - first[bx];
+            nbtot += (cnt * plan.nmax + 15) / 16;
+            for (int a1 = 0; a1 < cnt; ++a1)
+                rowblocks.push_back(make_int2(bx, a1));
+        }
+        bp->dinv_off = dupload(doff, s);
+        bp->dinv = dalloc<double2>(static_cast<size_t>(nbtot) * 512);
+
+        // CSR over (n1, n2) of the nonzero ab5[n2][n1][p]; the G index of an
+        // entry is dmoff[m2 - m1 + 2L] + p - |m2 - m1|.
+        const int L = plan.lmax, nmax = plan.nmax, P = 2 * L + 1;
+        const auto modes = modes_of(L);
+        std::vector<int> dmoff(4 * L + 1);
+        for (int d = 0, acc = 0; d < 4 * L + 1; ++d) {
+            dmoff[d] = acc;
+            acc += P - std::abs(d - 2 * L);
+        }
+        std::vector<int> erow(1, 0), eg;
+        std::vector<double2> ev;
+        for (int n1 = 0; n1 < nmax; ++n1)
+            for (int n2 = 0; n2 < nmax; ++n2) {
+                const int dm = modes[n2].m - modes[n1].m;
+                for (int p = 0; p < P; ++p) {
+                    const cplx a = ab5[(static_cast<size_t>(n2) * nmax + n1) * P + p];
+                    if (a == cplx(0.0, 0.0))
+                        continue;
+                    eg.push_back(dmoff[dm + 2 * L] + p - std::abs(dm));
+                    ev.push_back(make_double2(a.real(), a.imag()));
+                }
+                erow.push_back(static_cast<int>(eg.size()));
+            }
         std::vector<double4> pos(N);
         for (int i = 0; i < N; ++i)
             pos[i] = make_double4(pos_host[3 * i], pos_host[3 * i + 1], pos_host[3 * i + 2], 0.0);
-        const auto modes = modes_of(plan.lmax);
-        std::vector<int4> md(modes.size());
-        for (size_t i = 0; i < modes.size(); ++i)
-            md[i] = make_int4(modes[i].tau, modes[i].l, modes[i].m, 0);
-        std::vector<double2> ab(ab5.size());
-        for (size_t i = 0; i < ab5.size(); ++i)
-            ab[i] = make_double2(ab5[i].real(), ab5[i].imag());
         double4 *dpos = dupload(pos, s);
-        int4 *dmd = dupload(md, s);
-        double2 *dab = dupload(ab, s);
+        int2 *drb = dupload(rowblocks, s);
+        int *derow = dupload(erow, s), *deg = dupload(eg, s), *ddmoff = dupload(dmoff, s);
+        double2 *dev = dupload(ev, s);
         double *la = dupload(plan.leg_alpha, s), *lb = dupload(plan.leg_beta, s),
                *ld = dupload(plan.leg_diag, s);
-        hipLaunchKernelGGL(build_blocks_kernel, dim3(bp->nbox), dim3(256), 0, s, dpos, bp->box_first,
-                           bp->particles, bp->blk_off, dmd, plan.nmax, plan.lmax, k, t_dev, dab, la, lb,
-                           ld, bp->blocks);
+        hipLaunchKernelGGL(build_blocks_kernel, dim3(static_cast<unsigned>(rowblocks.size())), dim3(256), 0,
+                           s, dpos, bp->box_first, bp->particles, bp->blk_off, drb, nmax, L, k, t_dev,
+                           derow, deg, dev, ddmoff, la, lb, ld, bp->blocks);
         YH_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(lu_blocked_kernel, dim3(bp->nbox), dim3(256), 0, s, bp->box_first, plan.nmax,
-                           bp->blk_off, bp->piv_off, bp->blocks, bp->pivots);
+        hipLaunchKernelGGL(pad_blocks_kernel, dim3(bp->nbox), dim3(256), 0, s, bp->box_first, nmax,
+                           bp->blk_off, bp->blocks);
+        YH_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(lu_blocked_kernel, dim3(bp->nbox), dim3(256), 0, s, bp->box_first, nmax,
+                           bp->blk_off, bp->piv_off, bp->blocks, bp->pivots, bp->dinv_off, bp->dinv);
         YH_CHECK(hipGetLastError());
         YH_CHECK(hipStreamSynchronize(s));
-        for (void *p : {static_cast<void *>(dpos), static_cast<void *>(dmd), static_cast<void *>(dab),
+        for (void *p : {static_cast<void *>(dpos), static_cast<void *>(drb), static_cast<void *>(derow),
+                        static_cast<void *>(deg), static_cast<void *>(ddmoff), static_cast<void *>(dev),
                         static_cast<void *>(la), static_cast<void *>(lb), static_cast<void *>(ld)})
             YH_CHECK(hipFree(p));
     } catch (...) {
@@ -499,7 +529,8 @@ void precond_destroy(BlockPrecond *p) {
         return;
     for (void *b : {static_cast<void *>(p->box_first), static_cast<void *>(p->particles),
                     static_cast<void *>(p->blk_off), static_cast<void *>(p->piv_off),
-                    static_cast<void *>(p->blocks), static_cast<void *>(p->pivots)})
+                    static_cast<void *>(p->blocks), static_cast<void *>(p->pivots),
+                    static_cast<void *>(p->dinv_off), static_cast<void *>(p->dinv)})
         if (b)
             (void)hipFree(b);
     delete p;
@@ -508,7 +539,7 @@ void precond_destroy(BlockPrecond *p) {
 void precond_apply(BlockPrecond *p, const double2 *r, double2 *z, hipStream_t s) {
     hipLaunchKernelGGL(lu_apply_kernel, dim3(p->nbox), dim3(256), p->max_m * sizeof(double2), s,
                        p->box_first, p->particles, p->nmax, p->blk_off, p->piv_off, p->blocks, p->pivots,
-                       r, z);
+                       p->dinv_off, p->dinv, r, z);
     YH_CHECK(hipGetLastError());
 }
 

@@ -62,8 +62,11 @@ def edge_size_for(positions: np.ndarray, particles_per_box: float) -> np.ndarray
 
 
 class BlockDiagonalPreconditioner:
-    def __init__(self, partition_edge_sizes):
+    def __init__(self, partition_edge_sizes, max_box_particles: int | None = None):
+        """``max_box_particles`` (optional) bisects boxes holding more particles,
+        which balances the per-box factorisation work."""
         self.partition_edge_sizes = np.asarray(partition_edge_sizes, dtype=float)
+        self.max_box_particles = max_box_particles
         if self.partition_edge_sizes.shape != (3,):
             raise ValueError("partition_edge_sizes must be array of shape (3,)")
         self.partitioning = None
@@ -78,6 +81,8 @@ class BlockDiagonalPreconditioner:
         t0 = perf_counter()
         pos = simulation.parameters.particles.position
         cap = max(1, MAX_BLOCK_ROWS // simulation.numerics.nmax)
+        if self.max_box_particles is not None:
+            cap = max(1, min(cap, int(self.max_box_particles)))
         self.partitioning = split_oversized(make_particle_partition(pos, self.partition_edge_sizes),
                                             pos, cap)
         box = np.empty(pos.shape[0], dtype=np.int32)
