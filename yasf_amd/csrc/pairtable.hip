@@ -44,18 +44,99 @@ __global__ __launch_bounds__(256) void pair_table_build_kernel(
     pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld, dst, 64);
 }
 
-// One block of 4 waves per (job, target tile).  Wave w owns the orders
-// ip = w, w+4, ... (< R) of the job and all of its column tiles; its C tile
+// One wave's streaming GEMM with NIP orders (ip = wave + 8 j) and NTL column
+// tiles, all compile-time, so the 4*NIP*NTL MFMAs of a K-step form one
+// straight-line block the scheduler can interleave.
+constexpr int kTableWaves = 8;  // waves per (job, target tile) block
+
+template <typename T, int NIP, int NTL>
+__device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__restrict__ tab,
+                                                const typename Cplx<T>::type *__restrict__ x, int N,
+                                                int N4, int nmax, int R, int wave, const int (&n2c)[2],
+                                                T *__restrict__ Qt, int ntile) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    typedef typename Cplx<T>::type T2;
+    const int lane = threadIdx.x & 63, lo = lane & 15, hi = lane >> 4;
+    acc_t acc[NIP][NTL][2];
+#pragma unroll
+    for (int j = 0; j < NIP; ++j)
+#pragma unroll
+        for (int it = 0; it < NTL; ++it) {
+            acc[j][it][0] = acc_t{0, 0, 0, 0};
+            acc[j][it][1] = acc_t{0, 0, 0, 0};
+        }
+    // The next K-step's operands are in flight during the current MFMAs.
+    T2 a0[NIP], b0[NTL];
+    auto load = [&](int ks, T2 (&an)[NIP], T2 (&bn)[NTL]) {
+        const bool kin = ks < N4;
+#pragma unroll
+        for (int j = 0; j < NIP; ++j) {
+            an[j].x = T(0);
+            an[j].y = T(0);
+            if (kin)
+                an[j] = tab[(static_cast<size_t>(ks) * R + wave + kTableWaves * j) * 64];
+        }
+        const int s2 = ks * 4 + hi;
+#pragma unroll
+        for (int it = 0; it < NTL; ++it) {
+            bn[it].x = T(0);
+            bn[it].y = T(0);
+            if (n2c[it] >= 0 && s2 < N)
+                bn[it] = x[static_cast<size_t>(s2) * nmax + n2c[it]];
+        }
+    };
+    load(0, a0, b0);
+    for (int ks = 0; ks < N4; ++ks) {
+        T2 a[NIP], b[NTL];
+#pragma unroll
+        for (int j = 0; j < NIP; ++j)
+            a[j] = a0[j];
+#pragma unroll
+        for (int it = 0; it < NTL; ++it)
+            b[it] = b0[it];
+        load(ks + 1, a0, b0);
+        // real parts first, then imaginary parts: consecutive MFMAs never
+        // update the same accumulator
+#pragma unroll
+        for (int j = 0; j < NIP; ++j)
+#pragma unroll
+            for (int it = 0; it < NTL; ++it) {
+                acc[j][it][0] = Mfma<T>::mac(a[j].x, b[it].x, acc[j][it][0]);
+                acc[j][it][1] = Mfma<T>::mac(a[j].x, b[it].y, acc[j][it][1]);
+            }
+#pragma unroll
+        for (int j = 0; j < NIP; ++j)
+#pragma unroll
+            for (int it = 0; it < NTL; ++it) {
+                acc[j][it][0] = Mfma<T>::mac(-a[j].y, b[it].y, acc[j][it][0]);
+                acc[j][it][1] = Mfma<T>::mac(a[j].y, b[it].x, acc[j][it][1]);
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < NIP; ++j)
+#pragma unroll
+        for (int it = 0; it < NTL; ++it) {
+            T *dst = Qt + 2 * ((static_cast<size_t>((wave + kTableWaves * j) * ntile + it) * 16 + lo) * 16);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = Mfma<T>::row(i, hi);
+                dst[2 * r] = acc[j][it][0][i];
+                dst[2 * r + 1] = acc[j][it][1][i];
+            }
+        }
+}
+
+// One block of 8 waves per (job, target tile).  Wave w owns the orders
+// ip = w, w+8, ... (< R) of the job and all of its column tiles; its C tile
 // (ip, it) goes to Q slot qbase + ip * ntile + it (the same slots as the
 // on-the-fly kernel, so the contraction lists are shared).
-template <typename T, int IPW>
-__global__ __launch_bounds__(256) void pair_table_gemm_kernel(
+template <typename T, int NIPMAX>
+__global__ __launch_bounds__(512) void pair_table_gemm_kernel(
     const typename Cplx<T>::type *__restrict__ table, const typename Cplx<T>::type *__restrict__ x, int N,
     int N4, int nmax, const PairJob *__restrict__ jobs, int ntiles1, const int *__restrict__ colmap,
     int maxcols, int lmax, const int *__restrict__ dmoff, T *__restrict__ Q, int nq) {
-    typedef typename Mfma<T>::acc_t acc_t;
+    static_assert(kJobMaxColTiles == 2, "table GEMM dispatch assumes at most 2 column tiles per job");
     typedef typename Cplx<T>::type T2;
-    constexpr int NT = kJobMaxColTiles;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int jid = blockIdx.x / ntiles1;
     const int t = blockIdx.x - jid * ntiles1;
@@ -66,87 +147,26 @@ __global__ __launch_bounds__(256) void pair_table_gemm_kernel(
     const int P = 2 * lmax + 1;
     const int R = P - (dm < 0 ? -dm : dm);
     const int d = dm + 2 * lmax;
-    const int lo = lane & 15, hi = lane >> 4;
-
+    const int lo = lane & 15;
+    const int nip = R > wave ? (R - wave + kTableWaves - 1) / kTableWaves : 0;  // orders of this wave
+    if (nip == 0)
+        return;
     const T2 *tab = table + (static_cast<size_t>(t) * P * P + dmoff[d]) * N4 * 64 + lane;
-    int n2c[NT];
+    int n2c[2];
 #pragma unroll
-    for (int it = 0; it < NT; ++it)
+    for (int it = 0; it < 2; ++it)
         n2c[it] = (it < ntile) ? colmap[d * maxcols + (tile0 + it) * 16 + lo] : -1;
-
-    acc_t acc[IPW][NT][2];
-#pragma unroll
-    for (int j = 0; j < IPW; ++j)
-#pragma unroll
-        for (int it = 0; it < NT; ++it) {
-            acc[j][it][0] = acc_t{0, 0, 0, 0};
-            acc[j][it][1] = acc_t{0, 0, 0, 0};
-        }
-
-    T2 an[IPW], bn[NT];
-    auto load = [&](int ks) {
-#pragma unroll
-        for (int j = 0; j < IPW; ++j) {
-            const int ip = wave + 4 * j;
-            an[j].x = T(0);
-            an[j].y = T(0);
-            if (ip < R)
-                an[j] = tab[(static_cast<size_t>(ks) * R + ip) * 64];
-        }
-        const int s2 = ks * 4 + hi;
-#pragma unroll
-        for (int it = 0; it < NT; ++it) {
-            bn[it].x = T(0);
-            bn[it].y = T(0);
-            if (n2c[it] >= 0 && s2 < N)
-                bn[it] = x[static_cast<size_t>(s2) * nmax + n2c[it]];
-        }
-    };
-
-    load(0);
-    for (int ks = 0; ks < N4; ++ks) {
-        T2 a[IPW], b[NT];
-#pragma unroll
-        for (int j = 0; j < IPW; ++j)
-            a[j] = an[j];
-#pragma unroll
-        for (int it = 0; it < NT; ++it)
-            b[it] = bn[it];
-        if (ks + 1 < N4)
-            load(ks + 1);
-#pragma unroll
-        for (int j = 0; j < IPW; ++j) {
-            if (wave + 4 * j < R) {
-#pragma unroll
-                for (int it = 0; it < NT; ++it) {
-                    if (it < ntile) {
-                        acc[j][it][0] = Mfma<T>::mac(a[j].x, b[it].x, acc[j][it][0]);
-                        acc[j][it][0] = Mfma<T>::mac(-a[j].y, b[it].y, acc[j][it][0]);
-                        acc[j][it][1] = Mfma<T>::mac(a[j].x, b[it].y, acc[j][it][1]);
-                        acc[j][it][1] = Mfma<T>::mac(a[j].y, b[it].x, acc[j][it][1]);
-                    }
-                }
-            }
-        }
-    }
-
-#pragma unroll
-    for (int j = 0; j < IPW; ++j) {
-        const int ip = wave + 4 * j;
-        if (ip >= R)
-            continue;
-#pragma unroll
-        for (int it = 0; it < NT; ++it) {
-            if (it >= ntile)
-                continue;
-            T *dst = Q + 2 * (((static_cast<size_t>(t) * nq + qbase + ip * ntile + it) * 16 + lo) * 16);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int r = Mfma<T>::row(i, hi);
-                dst[2 * r] = acc[j][it][0][i];
-                dst[2 * r + 1] = acc[j][it][1][i];
-            }
-        }
+    T *Qt = Q + 2 * (static_cast<size_t>(t) * nq + qbase) * 256;
+    switch (nip * 2 + ntile - 1) {
+#define YH_CASE(NIP, NTL)                                                                            \
+    case NIP * 2 + NTL - 1:                                                                          \
+        if constexpr (NIP <= NIPMAX)                                                                 \
+            table_gemm_wave<T, NIP, NTL>(tab, x, N, N4, nmax, R, wave, n2c, Qt, ntile);              \
+        break;
+        YH_CASE(1, 1) YH_CASE(1, 2) YH_CASE(2, 1) YH_CASE(2, 2) YH_CASE(3, 1) YH_CASE(3, 2)
+#undef YH_CASE
+    default:
+        break;
     }
 }
 
@@ -155,16 +175,16 @@ void gemm_t(const DevicePlan &dp, int N, const void *table, const void *x, void 
     typedef typename Cplx<T>::type T2;
     const int ntiles1 = (N + 15) / 16;
     const int N4 = (N + 3) / 4;
-    const dim3 grid(dp.njobs * ntiles1), block(256);
+    const dim3 grid(dp.njobs * ntiles1), block(64 * kTableWaves);
     const T2 *tb = static_cast<const T2 *>(table);
     const T2 *xv = static_cast<const T2 *>(x);
     T *q = static_cast<T *>(Q);
-    // orders per wave: ceil(R/4) <= ceil((2L+1)/4)
+    // orders per wave: ceil(R/8) <= ceil((2L+1)/8), 3 only at lmax 8
     if (2 * dp.lmax + 1 <= 16)
-        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 4>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
+        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 2>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
                            dp.jobs, ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
     else
-        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 5>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
+        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 3>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
                            dp.jobs, ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
     YH_CHECK(hipGetLastError());
 }
