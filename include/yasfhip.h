@@ -40,7 +40,7 @@ enum { YASFHIP_FP64 = 0, YASFHIP_FP32 = 1 };
  * generated inside the matvec kernel, or generated once per session and
  * channel into a resident HBM table (ntiles*(2L+1)^2*ceil(N/4)*64 complex
  * values) that every later matvec streams.  AUTO uses the table when it needs
- * at most half of the free device memory. */
+ * at most three quarters of the free device memory. */
 enum { YASFHIP_MATVEC_AUTO = 0, YASFHIP_MATVEC_ON_THE_FLY = 1, YASFHIP_MATVEC_TABLE = 2 };
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -70,7 +70,8 @@ int yasfhip_set_tmatrix(yasfhip_session *s, const double *t);
 /* y = W x for one channel (CouplingBackend.multiply(x, idx),
  * yasfpy/coupling/backends.py:99-125).  precision: YASFHIP_FP64 computes in
  * complex128; YASFHIP_FP32 runs the pair GEMM in complex64 (geometry and
- * special functions stay FP64) with complex128 in/out. */
+ * special functions are evaluated in FP64 and rounded; the ab5 contraction
+ * accumulates in FP64) with complex128 in/out. */
 int yasfhip_coupling_matvec(yasfhip_session *s, int channel, const double *x, double *y,
                             int precision);
 
@@ -91,6 +92,26 @@ int yasfhip_master_matvec(yasfhip_session *s, int channel, const double *x, doub
 int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double *x_inout,
                         double rtol, int restart, int maxiter, int use_precond, int *info,
                         int *iterations, double *residual, double *history, int history_cap);
+
+/* Mixed-precision solve of (I - T W) x = b: FP64 iterative refinement around
+ * GMRES whose operator runs the pair GEMM in complex64.  Each refinement step
+ * forms the FP64 residual r = b - A x (FP64 matvec), solves A d = r with
+ * SciPy-style GMRES (x0 = 0, the given restart and inner_maxiter cycles, the
+ * preconditioner if use_precond, FP64 Arnoldi/Givens) to the relative
+ * reduction max(inner_rtol, rtol ||b|| / (2 ||r||)), and adds d to x.  It
+ * stops when ||b - A x|| <= rtol ||b|| (FP64 operator), after
+ * max_refinements steps, or when a step does not reduce the residual.
+ * info: 0 converged, 1 not.  refinements = steps taken; iterations = inner
+ * iterations summed over the steps; residual = final FP64 ||b - A x|| / ||b||.
+ * history (may be NULL) receives that relative residual before the first
+ * step and after each step (up to history_cap values).  The reference solves
+ * in FP64 only (yasfpy/solver.py:108-121); this entry point has the same
+ * convergence criterion on the FP64 operator. */
+int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b, double *x_inout,
+                                double rtol, double inner_rtol, int restart, int inner_maxiter,
+                                int max_refinements, int use_precond, int *info, int *refinements,
+                                int *iterations, double *residual, double *history,
+                                int history_cap);
 
 /* Block-diagonal preconditioner for one channel: particles are grouped into
  * boxes by box_of_particle[N] (values 0..num_boxes-1); each box's block of

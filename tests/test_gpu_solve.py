@@ -15,12 +15,14 @@ from yasf_amd.clusters import cluster
 pytestmark = pytest.mark.gpu
 
 
-def build_sim(positions, radii, n, lam, nmed, lmax, tol, restart=100, maxiter=1000, pc=None):
+def build_sim(positions, radii, n, lam, nmed, lmax, tol, restart=100, maxiter=1000, pc=None,
+              mixed=False):
     particles = Particles(positions, radii, n)
     field = InitialField(beam_width=0, focal_point=np.zeros(3), polar_angle=0.0,
                          azimuthal_angle=0.0, polarization="TE")
     params = Parameters(np.atleast_1d(lam), np.atleast_1d(nmed), particles, field)
-    solver = Solver("gmres", tolerance=tol, max_iter=maxiter, restart=restart, preconditioner=pc)
+    solver = Solver("gmres", tolerance=tol, max_iter=maxiter, restart=restart, preconditioner=pc,
+                    mixed_precision=mixed)
     sim = Simulation(params, Numerics(lmax=lmax, solver=solver, coupling_backend="hip"))
     sim.compute_mie_coefficients()
     sim.compute_initial_field_coefficients()
@@ -117,3 +119,50 @@ def test_gmres_errors_and_nonconvergence():
     zero = np.zeros(sim.session.n, dtype=complex)
     x, info, iters, res, _ = sim.session.gmres(zero)
     assert info == 0 and np.all(x == 0)
+
+
+@pytest.mark.parametrize("use_pc", [False, True], ids=["plain", "block_precond"])
+def test_mixed_precision_refinement(use_pc):
+    """FP64 refinement around complex64-operator GMRES reaches the FP64
+    criterion ||b - A x|| <= 1e-6 ||b|| against the oracle's operator and
+    agrees with the all-FP64 solve to the solve tolerance."""
+    pos, radii, n, lam, nmed, L = _cfg1()
+    pc = BlockDiagonalPreconditioner(edge_size_for(pos, 8)) if use_pc else None
+    sim = build_sim(pos, radii, n, lam, nmed, L, 1e-6, restart=50, pc=pc, mixed=True)
+    if pc is not None:
+        pc.prepare(sim)
+    sim.compute_scattered_field_coefficients()
+    assert sim.scattered_field_err_codes[0] == 0
+    x = sim.scattered_field_coefficients[:, :, 0].ravel()
+    b = sim.right_hand_side[:, :, 0].ravel()
+    k = 2 * np.pi / lam
+    t = sim.tmatrix_flat()[0]
+    res = np.linalg.norm(b - R.master_matvec(L, pos, k, t, x)) / np.linalg.norm(b)
+    assert res <= 1e-6
+    xr, info, _ = R.gmres_solve(lambda v: R.master_matvec(L, pos, k, t, v), b, rtol=1e-8, restart=100)
+    assert info == 0
+    # forward error of a 1e-6-residual solution, bounded through the
+    # conditioning of this small cluster (well below 1e-4)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-5
+
+
+def test_mixed_precision_history_and_errors():
+    pos, radii, n, lam, nmed, L = _cfg1()
+    sim = build_sim(pos, radii, n, lam, nmed, L, 1e-6, restart=50)
+    s = sim.session
+    b = sim.right_hand_side[:, :, 0].ravel()
+    x, info, steps, iters, res, hist = s.gmres_refined(b, rtol=1e-10, inner_rtol=1e-5, restart=50)
+    # each step reduces the FP64 residual; 1e-10 needs more than one step
+    assert steps >= 2 and np.all(np.diff(hist) < 0)
+    assert hist[-1] == pytest.approx(res)
+    assert info == 0 and res <= 1e-10
+    # one refinement step with inner_rtol 1e-3 cannot reach 1e-6 from x0 = b
+    x, info, steps, iters, res, hist = s.gmres_refined(b, rtol=1e-6, inner_rtol=1e-3,
+                                                       max_refinements=1)
+    assert steps == 1 and info == 1 and res > 1e-6
+    with pytest.raises(ValueError):
+        s.gmres_refined(b, inner_rtol=0.0)
+    with pytest.raises(ValueError):
+        s.gmres_refined(b, max_refinements=0)
+    x, info, steps, iters, res, hist = s.gmres_refined(np.zeros_like(b))
+    assert info == 0 and steps == 0 and np.all(x == 0)

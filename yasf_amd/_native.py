@@ -59,6 +59,7 @@ def _load():
             "yasfhip_coupling_matvec": ([P, i, P, P, i], i),
             "yasfhip_master_matvec": ([P, i, P, P, i], i),
             "yasfhip_gmres_solve": ([P, i, P, P, d, i, i, i, ip, ip, dp, P, i], i),
+            "yasfhip_gmres_solve_refined": ([P, i, P, P, d, d, i, i, i, i, ip, ip, ip, dp, P, i], i),
             "yasfhip_precond_build": ([P, i, P, i], i),
             "yasfhip_precond_apply": ([P, P, P], i),
             "yasfhip_time_matvec": ([P, i, i, i, dp, dp], i),
@@ -81,7 +82,7 @@ def exported_symbols() -> list[str]:
     return [
         "yasfhip_last_error", "yasfhip_translation_table", "yasfhip_plan_stats", "yasfhip_create",
         "yasfhip_destroy", "yasfhip_set_tmatrix", "yasfhip_coupling_matvec", "yasfhip_master_matvec",
-        "yasfhip_gmres_solve", "yasfhip_precond_build", "yasfhip_precond_apply",
+        "yasfhip_gmres_solve", "yasfhip_gmres_solve_refined", "yasfhip_precond_build", "yasfhip_precond_apply",
         "yasfhip_time_matvec", "yasfhip_set_matvec_mode",
     ]
 
@@ -195,6 +196,26 @@ class Session:
             int(bool(use_precond)), ctypes.byref(info), ctypes.byref(iters), ctypes.byref(res),
             _ptr(hist), hist.size))
         return x, info.value, iters.value, res.value, hist[: min(iters.value, hist.size)].copy()
+
+    def gmres_refined(self, b, x0=None, channel: int = 0, rtol: float = 1e-6, inner_rtol: float = 1e-5,
+                      restart: int = 50, inner_maxiter: int = 1000, max_refinements: int = 10,
+                      use_precond: bool = False):
+        """FP64 iterative refinement around complex64-operator GMRES.
+
+        Returns (x, info, refinements, inner_iterations, relative_residual,
+        relative residual before the first and after each refinement)."""
+        b = self._vec(b)
+        x = self._vec(b if x0 is None else x0).copy()
+        info = ctypes.c_int()
+        steps = ctypes.c_int()
+        iters = ctypes.c_int()
+        res = ctypes.c_double()
+        hist = np.zeros(int(max_refinements) + 1, dtype=np.float64)
+        _check(self._lib.yasfhip_gmres_solve_refined(
+            self._h, int(channel), _ptr(b), _ptr(x), float(rtol), float(inner_rtol), int(restart),
+            int(inner_maxiter), int(max_refinements), int(bool(use_precond)), ctypes.byref(info),
+            ctypes.byref(steps), ctypes.byref(iters), ctypes.byref(res), _ptr(hist), hist.size))
+        return x, info.value, steps.value, iters.value, res.value, hist[: steps.value + 1].copy()
 
     def precond_build(self, box_of_particle, channel: int = 0):
         box = np.ascontiguousarray(box_of_particle, dtype=np.int32)

@@ -23,6 +23,7 @@ struct Reducer {
 void launch_mgs_step(double2 *w, const double2 *v_prev, const double2 *h_prev, const double2 *v_next,
                      long n, Reducer red, double *out, hipStream_t s);
 void launch_scale(double2 *y, const double2 *x, double a, long n, hipStream_t s);
+void launch_add(double2 *x, const double2 *d, long n, hipStream_t s);
 void launch_update(double2 *x, const double2 *V, const double2 *y, int m, long n, long ld,
                    hipStream_t s);
 void launch_residual(double2 *r, const double2 *b, const double2 *ax, long n, Reducer red, double *out,
@@ -59,6 +60,29 @@ T *upload(const std::vector<T> &v, hipStream_t s) {
     return d;
 }
 
+// LAPACK zlartg semantics: [c s; -conj(s) c] [f; g] = [r; 0], c real.
+void lartg(cplx f, cplx g, double &c, cplx &s, cplx &r) {
+    if (g == cplx(0, 0)) {
+        c = 1.0;
+        s = cplx(0, 0);
+        r = f;
+        return;
+    }
+    if (f == cplx(0, 0)) {
+        c = 0.0;
+        const double ag = std::abs(g);
+        s = std::conj(g) / ag;
+        r = cplx(ag, 0.0);
+        return;
+    }
+    const double af = std::abs(f), ag = std::abs(g);
+    const double d = std::sqrt(af * af + ag * ag);
+    c = af / d;
+    const cplx sgn = f / af;
+    s = sgn * std::conj(g) / d;
+    r = sgn * d;
+}
+
 }  // namespace
 
 }  // namespace yasfhip
@@ -78,11 +102,12 @@ struct yasfhip_session {
     double2 *t = nullptr;
     bool t_set = false;
     double2 *xin = nullptr, *yout = nullptr;
-    float2 *x32 = nullptr, *y32 = nullptr;
+    float2 *x32 = nullptr;  // complex64 copy of the pair GEMM's x
     // GMRES workspace
     int vcap = 0;
     double2 *V = nullptr, *bvec = nullptr, *xvec = nullptr, *rvec = nullptr, *avec = nullptr,
             *ycoef = nullptr;
+    double2 *rb = nullptr, *rx = nullptr, *rr = nullptr;  // iterative refinement: b, x, residual
     double *scal = nullptr, *partials = nullptr;
     unsigned *counter = nullptr;
     double *scal_host = nullptr;
@@ -99,8 +124,9 @@ struct yasfhip_session {
     ~yasfhip_session() {
         if (precond)
             precond_destroy(precond);
-        void *bufs[] = {pos,  Q,    t,    xin,   yout, x32,      y32,     V,    bvec,
-                        xvec, rvec, avec, ycoef, scal, partials, counter, hdev};
+        void *bufs[] = {pos,  Q,    t,    xin,   yout,    x32,     V,    bvec,
+                        xvec, rvec, avec, ycoef, scal, partials, counter, hdev, rb,
+                        rx,   rr};
         for (void *b : bufs)
             if (b)
                 (void)hipFree(b);
@@ -137,7 +163,7 @@ struct yasfhip_session {
                     return false;
                 size_t free_b = 0, total_b = 0;
                 YH_CHECK(hipMemGetInfo(&free_b, &total_b));
-                if (bytes > free_b / 2) {  // leave room for other sessions and solver state
+                if (bytes > free_b / 4 * 3) {  // leave room for solver state and other sessions
                     table_refused[ix] = true;
                     return false;
                 }
@@ -157,18 +183,18 @@ struct yasfhip_session {
     }
 
     // y = W x (mode 0) or x - t W x (mode 1) for device vectors of complex128.
+    // fp32: the pair GEMM runs in complex64 on a complex64 copy of x; the ab5
+    // contraction and the epilogue stay FP64.
     void matvec_dev(int channel, const double2 *x, double2 *y, int mode, bool fp32) {
         const double2 *tt = t + static_cast<size_t>(channel) * n;
         if (!fp32) {
             pair_gemm(channel, x, false);
-            launch_contract(dp, N, Q, x, tt, mode, y, false, stream);
+            launch_contract(dp, N, Q, x, tt, mode, y, false, false, stream);
         } else {
             launch_to_fp32(reinterpret_cast<float *>(x32), reinterpret_cast<const double *>(x), 2 * n,
                            stream);
             pair_gemm(channel, x32, true);
-            launch_contract(dp, N, Q, x32, tt, mode, y32, true, stream);
-            launch_to_fp64(reinterpret_cast<double *>(y), reinterpret_cast<const float *>(y32), 2 * n,
-                           stream);
+            launch_contract(dp, N, Q, x, tt, mode, y, true, false, stream);
         }
     }
 
@@ -209,6 +235,171 @@ struct yasfhip_session {
         else if (z != r)
             YH_CHECK(hipMemcpyAsync(z, r, n * sizeof(double2), hipMemcpyDeviceToDevice, stream));
     }
+
+    struct GmresOutcome {
+        int info = 0;         // 0 converged, else maxiter (SciPy)
+        int iterations = 0;   // inner (Arnoldi) iterations
+        double rnorm = 0.0;   // final ||b - A x|| with the operator used
+        double bnorm = 0.0;
+    };
+
+    // SciPy's gmres(A, b, x0, M, restart, rtol, atol=0, maxiter) on device
+    // vectors (scipy/sparse/linalg/_isolve/iterative.py:692-840): bb is the
+    // right-hand side, x holds x0 on entry and the solution on exit; neither
+    // may alias rvec, avec or V.  fp32 selects the complex64 pair GEMM for
+    // every operator application (the Arnoldi process stays FP64).
+    // history: presid/||b|| per inner iteration, up to history_cap values.
+    GmresOutcome gmres_run(int channel, const double2 *bb, double2 *x, double rtol, int restart,
+                           int maxiter, bool pc, bool fp32, double *history, int history_cap) {
+        GmresOutcome out;
+        hipStream_t st = stream;
+        double2 *r = rvec, *av = avec;
+        int nhist = 0;
+        double rnorm = 0.0;
+        const double bnrm2 = norm_dev(bb);
+        out.bnorm = bnrm2;
+        if (bnrm2 == 0.0) {  // SciPy returns b
+            YH_CHECK(hipMemcpyAsync(x, bb, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+            return out;
+        }
+        const double atol = rtol * bnrm2;
+        const double eps = DBL_EPSILON;
+        psolve(bb, av, pc);
+        const double Mb_nrm2 = norm_dev(av);
+        double ptol_max_factor = 1.0;
+        double ptol = Mb_nrm2 * std::min(ptol_max_factor, atol / bnrm2);
+        double presid = 0.0;
+        std::vector<cplx> h(static_cast<size_t>(restart) * (restart + 1));
+        std::vector<double> gc(restart);
+        std::vector<cplx> gs(restart), S(restart + 1), y(restart);
+        auto H = [&](int col, int k) -> cplx & { return h[static_cast<size_t>(col) * (restart + 1) + k]; };
+
+        for (int iteration = 0; iteration < maxiter; ++iteration) {
+            if (iteration == 0) {
+                // r = b - A x if x.any() else b
+                launch_norm2(x, n, reducer(), scal, st);
+                fetch_scalars(1);
+                if (scal_host[0] != 0.0) {
+                    matvec_dev(channel, x, av, 1, fp32);
+                    launch_residual(r, bb, av, n, reducer(), scal, st);
+                    fetch_scalars(1);
+                } else {
+                    YH_CHECK(hipMemcpyAsync(r, bb, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+                    launch_norm2(r, n, reducer(), scal, st);
+                    fetch_scalars(1);
+                }
+                rnorm = std::sqrt(scal_host[0]);
+                if (rnorm < atol) {
+                    out.rnorm = rnorm;
+                    return out;
+                }
+            }
+            psolve(r, V, pc);
+            const double tmp0 = norm_dev(V);
+            launch_scale(V, V, 1.0 / tmp0, n, st);
+            std::fill(S.begin(), S.end(), cplx(0, 0));
+            S[0] = tmp0;
+            bool breakdown = false;
+            int col = 0;
+            for (col = 0; col < restart; ++col) {
+                double2 *vc = V + static_cast<size_t>(col) * n;
+                double2 *w = V + static_cast<size_t>(col + 1) * n;
+                if (pc) {
+                    matvec_dev(channel, vc, av, 1, fp32);
+                    psolve(av, w, true);
+                } else {
+                    matvec_dev(channel, vc, w, 1, fp32);
+                }
+                // Modified Gram-Schmidt, one fused launch per basis vector:
+                // launch kq subtracts h[kq-1] v[kq-1] and forms <v[kq], w>;
+                // slot kq of hdev holds (Re, Im, ||w||^2) of launch kq.
+                for (int kq = 0; kq <= col; ++kq) {
+                    const double2 *vk = V + static_cast<size_t>(kq) * n;
+                    const double2 *vp = kq > 0 ? V + static_cast<size_t>(kq - 1) * n : nullptr;
+                    const double2 *hp =
+                        kq > 0 ? reinterpret_cast<const double2 *>(hdev + 4 * (kq - 1)) : nullptr;
+                    launch_mgs_step(w, vp, hp, vk, n, reducer(), hdev + 4 * kq, st);
+                }
+                launch_mgs_step(w, vc, reinterpret_cast<const double2 *>(hdev + 4 * col), nullptr, n,
+                                reducer(), hdev + 4 * (col + 1), st);
+                YH_CHECK(hipMemcpyAsync(hhost, hdev, 4 * (col + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                                        st));
+                YH_CHECK(hipStreamSynchronize(st));
+                const double h0 = std::sqrt(hhost[2]);
+                for (int kq = 0; kq <= col; ++kq)
+                    H(col, kq) = cplx(hhost[4 * kq], hhost[4 * kq + 1]);
+                const double h1 = std::sqrt(hhost[4 * (col + 1) + 2]);
+                H(col, col + 1) = h1;
+                if (h1 <= eps * h0) {
+                    H(col, col + 1) = 0.0;
+                    breakdown = true;
+                } else {
+                    launch_scale(w, w, 1.0 / h1, n, st);
+                }
+                for (int kq = 0; kq < col; ++kq) {
+                    const double c = gc[kq];
+                    const cplx sn = gs[kq];
+                    const cplx n0 = H(col, kq), n1 = H(col, kq + 1);
+                    H(col, kq) = c * n0 + sn * n1;
+                    H(col, kq + 1) = -std::conj(sn) * n0 + c * n1;
+                }
+                double c;
+                cplx sn, mag;
+                lartg(H(col, col), H(col, col + 1), c, sn, mag);
+                gc[col] = c;
+                gs[col] = sn;
+                H(col, col) = mag;
+                H(col, col + 1) = 0.0;
+                const cplx tmp = -std::conj(sn) * S[col];
+                S[col] = c * S[col];
+                S[col + 1] = tmp;
+                presid = std::abs(tmp);
+                ++out.iterations;
+                if (history && nhist < history_cap)
+                    history[nhist++] = presid / bnrm2;
+                if (presid <= ptol || breakdown)
+                    break;
+            }
+            if (col == restart)
+                col = restart - 1;
+            if (H(col, col) == cplx(0, 0))
+                S[col] = 0.0;
+            for (int kq = 0; kq <= col; ++kq)
+                y[kq] = S[kq];
+            for (int kq = col; kq > 0; --kq) {
+                if (y[kq] != cplx(0, 0)) {
+                    y[kq] /= H(kq, kq);
+                    const cplx tk = y[kq];
+                    for (int i = 0; i < kq; ++i)
+                        y[i] -= tk * H(kq, i);
+                }
+            }
+            if (y[0] != cplx(0, 0))
+                y[0] /= H(0, 0);
+            std::vector<double2> ydev(col + 1);
+            for (int kq = 0; kq <= col; ++kq)
+                ydev[kq] = make_double2(y[kq].real(), y[kq].imag());
+            YH_CHECK(hipMemcpyAsync(ycoef, ydev.data(), (col + 1) * sizeof(double2), hipMemcpyHostToDevice,
+                                    st));
+            launch_update(x, V, ycoef, col + 1, n, n, st);
+            matvec_dev(channel, x, av, 1, fp32);
+            launch_residual(r, bb, av, n, reducer(), scal, st);
+            fetch_scalars(1);
+            rnorm = std::sqrt(scal_host[0]);
+            if (rnorm <= atol)
+                break;
+            else if (breakdown)
+                break;
+            else if (presid <= ptol)
+                ptol_max_factor = std::max(eps, 0.25 * ptol_max_factor);
+            else
+                ptol_max_factor = std::min(1.0, 1.5 * ptol_max_factor);
+            ptol = presid * std::min(ptol_max_factor, atol / rnorm);
+        }
+        out.rnorm = rnorm;
+        out.info = (rnorm <= atol) ? 0 : maxiter;
+        return out;
+    }
 };
 
 namespace {
@@ -230,29 +421,6 @@ int guarded(F &&f) {
         g_last_error = e.what();
         return kInternal;
     }
-}
-
-// LAPACK zlartg semantics: [c s; -conj(s) c] [f; g] = [r; 0], c real.
-void lartg(cplx f, cplx g, double &c, cplx &s, cplx &r) {
-    if (g == cplx(0, 0)) {
-        c = 1.0;
-        s = cplx(0, 0);
-        r = f;
-        return;
-    }
-    if (f == cplx(0, 0)) {
-        c = 0.0;
-        const double ag = std::abs(g);
-        s = std::conj(g) / ag;
-        r = cplx(ag, 0.0);
-        return;
-    }
-    const double af = std::abs(f), ag = std::abs(g);
-    const double d = std::sqrt(af * af + ag * ag);
-    c = af / d;
-    const cplx sgn = f / af;
-    s = sgn * std::conj(g) / d;
-    r = sgn * d;
 }
 
 }  // namespace
@@ -341,7 +509,6 @@ int yasfhip_create(const double *positions, int particle_number, int lmax, const
         s->xin = dmalloc<double2>(s->n);
         s->yout = dmalloc<double2>(s->n);
         s->x32 = dmalloc<float2>(s->n);
-        s->y32 = dmalloc<float2>(s->n);
         s->bvec = dmalloc<double2>(s->n);
         s->xvec = dmalloc<double2>(s->n);
         s->rvec = dmalloc<double2>(s->n);
@@ -412,165 +579,93 @@ int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double
         restart = static_cast<int>(std::min<long>(restart, n));
         s->ensure_gmres(restart);
         hipStream_t st = s->stream;
-        const bool pc = use_precond != 0;
-        double2 *x = s->xvec, *bb = s->bvec, *r = s->rvec, *av = s->avec, *V = s->V;
-        YH_CHECK(hipMemcpyAsync(bb, b, n * sizeof(double2), hipMemcpyHostToDevice, st));
-        YH_CHECK(hipMemcpyAsync(x, x_inout, n * sizeof(double2), hipMemcpyHostToDevice, st));
-        int inner_iter = 0, nhist = 0, out_info = 0;
-        double rnorm = 0.0;
-        const double bnrm2 = s->norm_dev(bb);
-        auto finish = [&](double rn) {
-            YH_CHECK(hipMemcpyAsync(x_inout, x, n * sizeof(double2), hipMemcpyDeviceToHost, st));
-            YH_CHECK(hipStreamSynchronize(st));
-            if (info)
-                *info = out_info;
-            if (iterations)
-                *iterations = inner_iter;
-            if (residual)
-                *residual = bnrm2 > 0 ? rn / bnrm2 : 0.0;
-        };
-        if (bnrm2 == 0.0) {  // SciPy returns b
-            YH_CHECK(hipMemcpyAsync(x, bb, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
-            finish(0.0);
-            return;
-        }
-        const double atol = rtol * bnrm2;
-        const double eps = DBL_EPSILON;
-        s->psolve(bb, av, pc);
-        const double Mb_nrm2 = s->norm_dev(av);
-        double ptol_max_factor = 1.0;
-        double ptol = Mb_nrm2 * std::min(ptol_max_factor, atol / bnrm2);
-        double presid = 0.0;
-        std::vector<cplx> h(static_cast<size_t>(restart) * (restart + 1));
-        std::vector<double> gc(restart);
-        std::vector<cplx> gs(restart), S(restart + 1), y(restart);
-        auto H = [&](int col, int k) -> cplx & { return h[static_cast<size_t>(col) * (restart + 1) + k]; };
+        YH_CHECK(hipMemcpyAsync(s->bvec, b, n * sizeof(double2), hipMemcpyHostToDevice, st));
+        YH_CHECK(hipMemcpyAsync(s->xvec, x_inout, n * sizeof(double2), hipMemcpyHostToDevice, st));
+        const auto o = s->gmres_run(channel, s->bvec, s->xvec, rtol, restart, maxiter, use_precond != 0,
+                                    false, history, history ? history_cap : 0);
+        YH_CHECK(hipMemcpyAsync(x_inout, s->xvec, n * sizeof(double2), hipMemcpyDeviceToHost, st));
+        YH_CHECK(hipStreamSynchronize(st));
+        if (info)
+            *info = o.info;
+        if (iterations)
+            *iterations = o.iterations;
+        if (residual)
+            *residual = o.bnorm > 0 ? o.rnorm / o.bnorm : 0.0;
+    });
+}
 
-        for (int iteration = 0; iteration < maxiter; ++iteration) {
-            if (iteration == 0) {
-                // r = b - A x if x.any() else b
-                launch_norm2(x, n, s->reducer(), s->scal, st);
-                s->fetch_scalars(1);
-                if (s->scal_host[0] != 0.0) {
-                    s->matvec_dev(channel, x, av, 1, false);
-                    launch_residual(r, bb, av, n, s->reducer(), s->scal, st);
-                    s->fetch_scalars(1);
-                } else {
-                    YH_CHECK(hipMemcpyAsync(r, bb, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
-                    launch_norm2(r, n, s->reducer(), s->scal, st);
-                    s->fetch_scalars(1);
-                }
-                rnorm = std::sqrt(s->scal_host[0]);
-                if (rnorm < atol) {
-                    out_info = 0;
-                    finish(rnorm);
-                    return;
-                }
-            }
-            s->psolve(r, V, pc);
-            const double tmp0 = s->norm_dev(V);
-            launch_scale(V, V, 1.0 / tmp0, n, st);
-            std::fill(S.begin(), S.end(), cplx(0, 0));
-            S[0] = tmp0;
-            bool breakdown = false;
-            int col = 0;
-            for (col = 0; col < restart; ++col) {
-                double2 *vc = V + static_cast<size_t>(col) * n;
-                double2 *w = V + static_cast<size_t>(col + 1) * n;
-                if (pc) {
-                    s->matvec_dev(channel, vc, av, 1, false);
-                    s->psolve(av, w, true);
-                } else {
-                    s->matvec_dev(channel, vc, w, 1, false);
-                }
-                // Modified Gram-Schmidt, one fused launch per basis vector:
-                // launch kq subtracts h[kq-1] v[kq-1] and forms <v[kq], w>;
-                // slot kq of hdev holds (Re, Im, ||w||^2) of launch kq.
-                for (int kq = 0; kq <= col; ++kq) {
-                    const double2 *vk = V + static_cast<size_t>(kq) * n;
-                    const double2 *vp = kq > 0 ? V + static_cast<size_t>(kq - 1) * n : nullptr;
-                    const double2 *hp =
-                        kq > 0 ? reinterpret_cast<const double2 *>(s->hdev + 4 * (kq - 1)) : nullptr;
-                    launch_mgs_step(w, vp, hp, vk, n, s->reducer(), s->hdev + 4 * kq, st);
-                }
-                launch_mgs_step(w, vc, reinterpret_cast<const double2 *>(s->hdev + 4 * col), nullptr, n,
-                                s->reducer(), s->hdev + 4 * (col + 1), st);
-                YH_CHECK(hipMemcpyAsync(s->hhost, s->hdev, 4 * (col + 2) * sizeof(double),
-                                        hipMemcpyDeviceToHost, st));
-                YH_CHECK(hipStreamSynchronize(st));
-                const double h0 = std::sqrt(s->hhost[2]);
-                for (int kq = 0; kq <= col; ++kq)
-                    H(col, kq) = cplx(s->hhost[4 * kq], s->hhost[4 * kq + 1]);
-                const double h1 = std::sqrt(s->hhost[4 * (col + 1) + 2]);
-                H(col, col + 1) = h1;
-                if (h1 <= eps * h0) {
-                    H(col, col + 1) = 0.0;
-                    breakdown = true;
-                } else {
-                    launch_scale(w, w, 1.0 / h1, n, st);
-                }
-                for (int kq = 0; kq < col; ++kq) {
-                    const double c = gc[kq];
-                    const cplx sn = gs[kq];
-                    const cplx n0 = H(col, kq), n1 = H(col, kq + 1);
-                    H(col, kq) = c * n0 + sn * n1;
-                    H(col, kq + 1) = -std::conj(sn) * n0 + c * n1;
-                }
-                double c;
-                cplx sn, mag;
-                lartg(H(col, col), H(col, col + 1), c, sn, mag);
-                gc[col] = c;
-                gs[col] = sn;
-                H(col, col) = mag;
-                H(col, col + 1) = 0.0;
-                const cplx tmp = -std::conj(sn) * S[col];
-                S[col] = c * S[col];
-                S[col + 1] = tmp;
-                presid = std::abs(tmp);
-                ++inner_iter;
-                if (history && nhist < history_cap)
-                    history[nhist++] = presid / bnrm2;
-                if (presid <= ptol || breakdown)
-                    break;
-            }
-            if (col == restart)
-                col = restart - 1;
-            if (H(col, col) == cplx(0, 0))
-                S[col] = 0.0;
-            for (int kq = 0; kq <= col; ++kq)
-                y[kq] = S[kq];
-            for (int kq = col; kq > 0; --kq) {
-                if (y[kq] != cplx(0, 0)) {
-                    y[kq] /= H(kq, kq);
-                    const cplx tk = y[kq];
-                    for (int i = 0; i < kq; ++i)
-                        y[i] -= tk * H(kq, i);
-                }
-            }
-            if (y[0] != cplx(0, 0))
-                y[0] /= H(0, 0);
-            std::vector<double2> ydev(col + 1);
-            for (int kq = 0; kq <= col; ++kq)
-                ydev[kq] = make_double2(y[kq].real(), y[kq].imag());
-            YH_CHECK(hipMemcpyAsync(s->ycoef, ydev.data(), (col + 1) * sizeof(double2),
-                                    hipMemcpyHostToDevice, st));
-            launch_update(x, V, s->ycoef, col + 1, n, n, st);
-            s->matvec_dev(channel, x, av, 1, false);
-            launch_residual(r, bb, av, n, s->reducer(), s->scal, st);
+int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b, double *x_inout,
+                                double rtol, double inner_rtol, int restart, int inner_maxiter,
+                                int max_refinements, int use_precond, int *info, int *refinements,
+                                int *iterations, double *residual, double *history,
+                                int history_cap) {
+    return guarded([&] {
+        YH_REQUIRE(s && b && x_inout, "NULL argument");
+        YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
+        YH_REQUIRE(s->t_set, "T-matrix not set (yasfhip_set_tmatrix)");
+        YH_REQUIRE(rtol >= 0.0, "rtol must be >= 0");
+        YH_REQUIRE(inner_rtol > 0.0 && inner_rtol < 1.0, "inner_rtol must be in (0, 1)");
+        YH_REQUIRE(inner_maxiter >= 1, "inner_maxiter must be >= 1");
+        YH_REQUIRE(max_refinements >= 1, "max_refinements must be >= 1");
+        if (use_precond)
+            YH_REQUIRE(s->precond && s->precond_channel == channel,
+                       "preconditioner not prepared for this channel");
+        const long n = s->n;
+        if (restart <= 0)
+            restart = 20;
+        restart = static_cast<int>(std::min<long>(restart, n));
+        s->ensure_gmres(restart);
+        if (!s->rb) {
+            s->rb = dmalloc<double2>(n);
+            s->rx = dmalloc<double2>(n);
+            s->rr = dmalloc<double2>(n);
+        }
+        hipStream_t st = s->stream;
+        s->table_for(channel, true);  // the inner solves' table first: they make most matvecs
+        YH_CHECK(hipMemcpyAsync(s->rb, b, n * sizeof(double2), hipMemcpyHostToDevice, st));
+        YH_CHECK(hipMemcpyAsync(s->rx, x_inout, n * sizeof(double2), hipMemcpyHostToDevice, st));
+        const double bnorm = s->norm_dev(s->rb);
+        int steps = 0, inner = 0, nhist = 0;
+        double rnorm = 0.0;
+        // FP64 residual r = b - A x of the current iterate
+        auto residual64 = [&] {
+            s->matvec_dev(channel, s->rx, s->avec, 1, false);
+            launch_residual(s->rr, s->rb, s->avec, n, s->reducer(), s->scal, st);
             s->fetch_scalars(1);
             rnorm = std::sqrt(s->scal_host[0]);
-            if (rnorm <= atol)
-                break;
-            else if (breakdown)
-                break;
-            else if (presid <= ptol)
-                ptol_max_factor = std::max(eps, 0.25 * ptol_max_factor);
-            else
-                ptol_max_factor = std::min(1.0, 1.5 * ptol_max_factor);
-            ptol = presid * std::min(ptol_max_factor, atol / rnorm);
+            if (history && nhist < history_cap)
+                history[nhist++] = bnorm > 0 ? rnorm / bnorm : 0.0;
+        };
+        if (bnorm == 0.0) {
+            YH_CHECK(hipMemsetAsync(s->rx, 0, n * sizeof(double2), st));
+        } else {
+            residual64();
+            while (rnorm > rtol * bnorm && steps < max_refinements) {
+                // correction A d = r by GMRES with the complex64 pair GEMM, asked
+                // for the reduction still missing, but not below inner_rtol
+                const double target = std::max(inner_rtol, 0.5 * rtol * bnorm / rnorm);
+                YH_CHECK(hipMemsetAsync(s->xvec, 0, n * sizeof(double2), st));
+                const auto o = s->gmres_run(channel, s->rr, s->xvec, target, restart, inner_maxiter,
+                                            use_precond != 0, true, nullptr, 0);
+                inner += o.iterations;
+                ++steps;
+                launch_add(s->rx, s->xvec, n, st);
+                const double before = rnorm;
+                residual64();
+                if (o.iterations == 0 || rnorm >= before)
+                    break;  // no progress: the complex64 operator cannot reduce r further
+            }
         }
-        out_info = (rnorm <= atol) ? 0 : maxiter;
-        finish(rnorm);
+        YH_CHECK(hipMemcpyAsync(x_inout, s->rx, n * sizeof(double2), hipMemcpyDeviceToHost, st));
+        YH_CHECK(hipStreamSynchronize(st));
+        if (info)
+            *info = (rnorm <= rtol * bnorm) ? 0 : 1;
+        if (refinements)
+            *refinements = steps;
+        if (iterations)
+            *iterations = inner;
+        if (residual)
+            *residual = bnorm > 0 ? rnorm / bnorm : 0.0;
     });
 }
 
@@ -643,8 +738,8 @@ int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps
             YH_CHECK(hipEventRecord(s->ev[0], st));
             s->pair_gemm(channel, xv, fp32);
             YH_CHECK(hipEventRecord(s->ev[1], st));
-            launch_contract(s->dp, s->N, s->Q, xv, s->t + static_cast<size_t>(channel) * s->n, 0,
-                            fp32 ? static_cast<void *>(s->y32) : static_cast<void *>(s->yout), fp32, st);
+            launch_contract(s->dp, s->N, s->Q, s->xin, s->t + static_cast<size_t>(channel) * s->n, 0,
+                            s->yout, fp32, false, st);
             YH_CHECK(hipEventRecord(s->ev[2], st));
             YH_CHECK(hipEventSynchronize(s->ev[2]));
             float a = 0.f, b = 0.f;

@@ -246,14 +246,22 @@ void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, cons
 // Largest LDS the pair kernel asks for, bytes (2 stages of complex128).
 size_t pair_lds_bytes(const DevicePlan &dp) { return 2 * static_cast<size_t>(dp.stage_elems) * 16; }
 
-// fp32: Q, x and y are complex64; otherwise complex128.
+// q_fp32: Q is complex64 (the FP32 pair GEMM), otherwise complex128; the
+// contraction always accumulates in FP64.  io_fp32: x and y are complex64,
+// otherwise complex128 (the mixed-precision operator reads FP32 Q with FP64
+// x and y, so x - t W x is formed in FP64).
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
-                     int mode, void *y, bool fp32, hipStream_t s) {
+                     int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s) {
     const int ntiles1 = (N + 15) / 16;
-    if (fp32)
+    YH_REQUIRE(q_fp32 || !io_fp32, "complex64 vectors need the complex64 pair GEMM");
+    if (io_fp32)
         hipLaunchKernelGGL((contract_kernel<float, float, float>), dim3(ntiles1), dim3(256), 0, s,
                            static_cast<const float *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N, dp.nmax,
                            static_cast<const float *>(x), t, mode, static_cast<float *>(y));
+    else if (q_fp32)
+        hipLaunchKernelGGL((contract_kernel<float, double, double>), dim3(ntiles1), dim3(256), 0, s,
+                           static_cast<const float *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N, dp.nmax,
+                           static_cast<const double *>(x), t, mode, static_cast<double *>(y));
     else
         hipLaunchKernelGGL((contract_kernel<double, double, double>), dim3(ntiles1), dim3(256), 0, s,
                            static_cast<const double *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N,

@@ -99,9 +99,9 @@ struct DevicePlan {
 // On-the-fly pair GEMM (pair factors generated in the kernel).
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
                  void *Q, hipStream_t s);
-// ab5 contraction + optional master-operator epilogue.
+// ab5 contraction (FP64 accumulation) + optional master-operator epilogue.
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
-                     int mode, void *y, bool fp32, hipStream_t s);
+                     int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s);
 // Resident pair-factor table: bytes, build, and the streaming pair GEMM.
 size_t pair_table_bytes(const DevicePlan &dp, int N, bool fp32);
 void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,

@@ -117,6 +117,15 @@ __global__ void scale_kernel(double2 *__restrict__ y, const double2 *__restrict_
     }
 }
 
+// x += d
+__global__ void add_kernel(double2 *__restrict__ x, const double2 *__restrict__ d, long n) {
+    for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x) {
+        const double2 a = x[i], b = d[i];
+        x[i] = make_double2(a.x + b.x, a.y + b.y);
+    }
+}
+
 // x += sum_k y_k V[k]  (k < m)
 __global__ void update_kernel(double2 *__restrict__ x, const double2 *__restrict__ V,
                               const double2 *__restrict__ y, int m, long n, long ld) {
@@ -182,6 +191,11 @@ void launch_scale(double2 *y, const double2 *x, double a, long n, hipStream_t s)
 void launch_update(double2 *x, const double2 *V, const double2 *y, int m, long n, long ld,
                    hipStream_t s) {
     hipLaunchKernelGGL(update_kernel, dim3(grid_for(n)), dim3(kRedThreads), 0, s, x, V, y, m, n, ld);
+    YH_CHECK(hipGetLastError());
+}
+
+void launch_add(double2 *x, const double2 *d, long n, hipStream_t s) {
+    hipLaunchKernelGGL(add_kernel, dim3(grid_for(n)), dim3(kRedThreads), 0, s, x, d, n);
     YH_CHECK(hipGetLastError());
 }
 
