@@ -113,6 +113,27 @@ int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b
                                 int *iterations, double *residual, double *history,
                                 int history_cap);
 
+/* Extinction and scattering cross sections of one channel from the incident
+ * (a_inc) and scattered (f) field coefficients, particle-major complex128
+ * (Optics.compute_cross_sections, yasfpy/optics.py:55-156):
+ *   c_ext = -Re sum conj(a) f * pi / k^2,
+ *   c_sca = |sum_{j1,j2} conj(f_j1) W_J[j1,j2] f_j2| * pi / k^2,
+ * W_J being the coupling operator with spherical Bessel j_p in place of h_p and
+ * the coincident pairs s1 = s2 included (the CPU sum of
+ * yasfpy/functions/cpu_numba.py:410-457), evaluated matrix-free in FP64. */
+int yasfhip_cross_sections(yasfhip_session *s, int channel, const double *a_inc, const double *f,
+                           double *c_ext, double *c_sca);
+
+/* Far-field angle components of the scattered electric field for
+ * num_directions directions (polar[a], azimuthal[a]) in radians
+ * (Optics.compute_scattered_e_field_angle_components, yasfpy/optics.py:177-266,
+ * and compute_electric_field_angle_components,
+ * yasfpy/functions/cpu_numba.py:535-606); e_theta and e_phi receive
+ * num_directions complex128 values.  The phase function and polarisation
+ * components follow from these on the host (optics.py:269-551). */
+int yasfhip_far_field(yasfhip_session *s, int channel, const double *f, int num_directions,
+                      const double *polar, const double *azimuthal, double *e_theta, double *e_phi);
+
 /* Block-diagonal preconditioner for one channel: particles are grouped into
  * boxes by box_of_particle[N] (values 0..num_boxes-1); each box's block of
  * I - T W restricted to intra-box pairs is assembled and LU-factored on the

@@ -261,8 +261,9 @@ def mie_coefficients(lmax: int, radius: np.ndarray, n_particle: np.ndarray,
 # ---------------------------------------------------------------- plane wave
 
 
-def spherical_functions_trigon(lmax: int, ct: float, st: float):
-    """pi_lm, tau_lm angular functions (yasfpy/functions/spherical_functions_trigon.py:39-93)."""
+def spherical_functions_trigon(lmax: int, ct, st, all_angles: bool = False):
+    """pi_lm, tau_lm angular functions (yasfpy/functions/spherical_functions_trigon.py:39-93);
+    ``[lmax+1, lmax+1]`` for the first angle, or ``[lmax+1, lmax+1, A]`` with all_angles."""
     ct = np.atleast_1d(ct).astype(float)
     st = np.atleast_1d(st).astype(float)
     shape = (lmax + 1, lmax + 1, ct.size)
@@ -293,6 +294,8 @@ def spherical_functions_trigon(lmax: int, ct: float, st: float):
             taulm[l + 1, m] = ((l + 1) * ct * pilm[l + 1, m]
                                - (l + 1 + m) * np.sqrt((2 * l + 3) * (l + 1 - m) / (2 * l + 1) / (l + 1 + m))
                                * pilm[l, m])
+    if all_angles:
+        return pilm, taulm
     return pilm[:, :, 0], taulm[:, :, 0]
 
 
@@ -443,3 +446,58 @@ def cross_sections(lmax, positions, k_medium, a_inc, f):
     c_ext = float(np.real(-np.sum(np.conj(a_inc) * f)) / k2 * np.pi)
     c_sca = float(abs(out[0]) / k2 * np.pi)
     return c_ext, c_sca
+
+
+def far_field(lmax, positions, k_medium, f, polar, azimuthal):
+    """(E_theta, E_phi) at directions (polar[a], azimuthal[a]) for one channel:
+    compute_electric_field_angle_components (yasfpy/functions/cpu_numba.py:535-606)
+    with e_r of numerics.py:314-330; the sum over particles is vectorised."""
+    polar = np.atleast_1d(np.asarray(polar, dtype=float))
+    azimuthal = np.atleast_1d(np.asarray(azimuthal, dtype=float))
+    N = positions.shape[0]
+    nmax = nmax_of(lmax)
+    f = np.asarray(f).reshape(N, nmax)
+    pilm, taulm = spherical_functions_trigon(lmax, np.cos(polar), np.sin(polar), all_angles=True)
+    e_r = np.stack([np.sin(polar) * np.cos(azimuthal), np.sin(polar) * np.sin(azimuthal),
+                    np.cos(polar)], axis=1)
+    phase = np.exp(-1j * k_medium * (positions @ e_r.T))  # [N, A]
+    e_theta = np.zeros(polar.size, dtype=complex)
+    e_phi = np.zeros(polar.size, dtype=complex)
+    for tau in (1, 2):
+        for l in range(1, lmax + 1):
+            for m in range(-l, l + 1):
+                n = multi2single_index(0, tau, l, m, lmax)
+                t = (1j ** (tau - l - 2) * (f[:, n] @ phase) / np.sqrt(2 * l * (l + 1))
+                     * np.exp(1j * m * azimuthal))
+                if tau == 1:
+                    e_theta += t * pilm[l, abs(m)] * 1j * m
+                    e_phi -= t * taulm[l, abs(m)]
+                else:
+                    e_theta += t * taulm[l, abs(m)]
+                    e_phi += t * pilm[l, abs(m)] * 1j * m
+    return e_theta, e_phi
+
+
+def polarization_components(e_theta, e_phi):
+    """I, DOP, DOLP, DOLQ, DOLU, DOCP (yasfpy/functions/cpu_numba.py:609-671)."""
+    et2 = np.abs(e_theta) ** 2
+    ep2 = np.abs(e_phi) ** 2
+    cross = e_theta * np.conj(e_phi)
+    I = et2 + ep2
+    Q = et2 - ep2
+    U = -2 * cross.real
+    V = 2 * cross.imag
+    return (I, np.sqrt(Q ** 2 + U ** 2 + V ** 2) / I, np.sqrt(Q ** 2 + U ** 2) / I, -Q / I, U / I, V / I)
+
+
+def phase_function(lmax, positions, k_medium, f, c_sca, sampling_points_number):
+    """Azimuthally averaged phase function on the [n_phi, n_theta] grid of
+    numerics.py:91-108 (optics.py:437-482): I 4 pi / |k|^2 / C_sca."""
+    n_phi, n_theta = (int(v) for v in sampling_points_number)
+    th = np.pi * np.linspace(0, 1, n_theta)
+    ph = 2 * np.pi * np.linspace(0, 1, n_phi + 1)[:-1]
+    pa, aa = np.meshgrid(th, ph, indexing="xy")
+    et, ep = far_field(lmax, positions, k_medium, f, pa.ravel(), aa.ravel())
+    I = polarization_components(et, ep)[0]
+    p3 = I * 4 * np.pi / abs(k_medium) ** 2 / c_sca
+    return th, p3.reshape(n_phi, n_theta).mean(axis=0)

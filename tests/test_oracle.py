@@ -200,3 +200,85 @@ def test_block_preconditioner_single_box_is_exact():
     b = rng.normal(size=t.size) + 1j * rng.normal(size=t.size)
     z = apply(b)
     npt.assert_allclose(R.master_matvec(L, pos, k, t, z), b, rtol=1e-12, atol=1e-12)
+
+
+def _bh_mie(lmax, x, m):
+    """Textbook Mie a_n, b_n (Bohren & Huffman eq. 4.53), independent of the oracle."""
+    n = np.arange(1, lmax + 1)
+    def psi(z):
+        return z * spherical_jn(n, z)
+    def dpsi(z):
+        return spherical_jn(n, z) + z * spherical_jn(n, z, derivative=True)
+    h = spherical_jn(n, x) + 1j * spherical_yn(n, x)
+    dh = (spherical_jn(n, x, derivative=True) + 1j * spherical_yn(n, x, derivative=True))
+    xi, dxi = x * h, h + x * dh
+    mx = m * x
+    a = (m * psi(mx) * dpsi(x) - psi(x) * dpsi(mx)) / (m * psi(mx) * dxi - xi * dpsi(mx))
+    b = (psi(mx) * dpsi(x) - m * psi(x) * dpsi(mx)) / (psi(mx) * dxi - m * xi * dpsi(mx))
+    return a, b
+
+
+def test_phase_function_single_sphere_textbook():
+    """Far field (cpu_numba.py:535-606) + C_sca (cpu_numba.py:410-457) of one sphere
+    reproduce the textbook Mie phase function (|S1|^2 + |S2|^2) / sum (2n+1)(|a|^2+|b|^2),
+    S1, S2 from the pi_n, tau_n recurrences (Bohren & Huffman 4.47)."""
+    lmax, r, m, lam = 8, 180.0, 1.6 + 0.05j, 550.0
+    k = 2 * np.pi / lam
+    pos = np.zeros((1, 3))
+    mie = R.mie_coefficients(lmax, np.array([r]), np.array([m]), np.array([lam]), 1.0)[0, :, 0]
+    a_inc = R.planewave_coefficients(lmax, pos, np.array([k]), 0.0, 0.0, 1)[0, :, 0]
+    f = mie * a_inc
+    _, c_sca = R.cross_sections(lmax, pos, k, a_inc, f)
+    th, p = R.phase_function(lmax, pos, k, f, c_sca, (8, 37))
+    an, bn = _bh_mie(lmax, k * r, m)
+    mu = np.cos(th)
+    pi_prev, pi_cur = np.zeros_like(mu), np.ones_like(mu)
+    s1 = np.zeros_like(mu, dtype=complex)
+    s2 = np.zeros_like(mu, dtype=complex)
+    for nn in range(1, lmax + 1):
+        if nn > 1:
+            pi_prev, pi_cur = pi_cur, ((2 * nn - 1) * mu * pi_cur - nn * pi_prev) / (nn - 1)
+        tau = nn * mu * pi_cur - (nn + 1) * pi_prev
+        c = (2 * nn + 1) / (nn * (nn + 1))
+        s1 += c * (an[nn - 1] * pi_cur + bn[nn - 1] * tau)
+        s2 += c * (an[nn - 1] * tau + bn[nn - 1] * pi_cur)
+    norm = np.sum((2 * np.arange(1, lmax + 1) + 1) * (np.abs(an) ** 2 + np.abs(bn) ** 2))
+    npt.assert_allclose(p, (np.abs(s1) ** 2 + np.abs(s2) ** 2) / norm, rtol=1e-9)
+    # C_sca of one sphere equals the textbook 2 pi / k^2 sum (2n+1)(|a|^2 + |b|^2)
+    assert c_sca == pytest.approx(2 * np.pi / k ** 2 * norm, rel=1e-10)
+
+
+def _solved_cluster(n_index, seed=21, N=7, lmax=3, lam=500.0):
+    from yasf_amd.clusters import cluster
+    pos, radii = cluster(seed, N, 0.1)
+    k = 2 * np.pi / lam
+    mie = R.mie_coefficients(lmax, radii, np.full(N, n_index), np.array([lam]), 1.0)[:, :, 0]
+    t = mie.ravel()
+    a_inc = R.planewave_coefficients(lmax, pos, np.array([k]), 0.3, 1.1, 1)[:, :, 0].ravel()
+    f, info, _ = R.gmres_solve(lambda v: R.master_matvec(lmax, pos, k, t, v), t * a_inc, rtol=1e-12,
+                               restart=200)
+    assert info == 0
+    return pos, k, lmax, a_inc, f
+
+
+def test_cross_section_equals_far_field_integral():
+    """C_sca of a 7-sphere cluster (J-translation pair sum, self terms included) equals
+    the far-field integral 1/k^2 int |E|^2 dOmega (Gauss-Legendre in cos theta,
+    uniform in phi; exact for this band limit): int p dOmega = 4 pi (test_compare_mstm4.py:37-47)."""
+    pos, k, lmax, a_inc, f = _solved_cluster(1.5 + 0.02j)
+    _, c_sca = R.cross_sections(lmax, pos, k, a_inc, f)
+    x, wx = np.polynomial.legendre.leggauss(48)
+    nphi = 96
+    ph = 2 * np.pi * np.arange(nphi) / nphi
+    TH, PH = np.meshgrid(np.arccos(x), ph, indexing="ij")
+    et, ep = R.far_field(lmax, pos, k, f, TH.ravel(), PH.ravel())
+    I = (np.abs(et) ** 2 + np.abs(ep) ** 2).reshape(TH.shape)
+    integral = np.sum(wx[:, None] * I) * (2 * np.pi / nphi)
+    assert integral / k ** 2 == pytest.approx(c_sca, rel=1e-9)
+
+
+def test_lossless_cluster_conserves_energy():
+    """Real refractive index: C_ext = C_sca (optical theorem) to solver tolerance."""
+    pos, k, lmax, a_inc, f = _solved_cluster(1.45 + 0.0j, seed=22)
+    c_ext, c_sca = R.cross_sections(lmax, pos, k, a_inc, f)
+    assert c_ext == pytest.approx(c_sca, rel=1e-9)

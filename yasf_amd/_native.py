@@ -60,6 +60,8 @@ def _load():
             "yasfhip_master_matvec": ([P, i, P, P, i], i),
             "yasfhip_gmres_solve": ([P, i, P, P, d, i, i, i, ip, ip, dp, P, i], i),
             "yasfhip_gmres_solve_refined": ([P, i, P, P, d, d, i, i, i, i, ip, ip, ip, dp, P, i], i),
+            "yasfhip_cross_sections": ([P, i, P, P, dp, dp], i),
+            "yasfhip_far_field": ([P, i, P, i, P, P, P, P], i),
             "yasfhip_precond_build": ([P, i, P, i], i),
             "yasfhip_precond_apply": ([P, P, P], i),
             "yasfhip_time_matvec": ([P, i, i, i, dp, dp], i),
@@ -82,7 +84,8 @@ def exported_symbols() -> list[str]:
     return [
         "yasfhip_last_error", "yasfhip_translation_table", "yasfhip_plan_stats", "yasfhip_create",
         "yasfhip_destroy", "yasfhip_set_tmatrix", "yasfhip_coupling_matvec", "yasfhip_master_matvec",
-        "yasfhip_gmres_solve", "yasfhip_gmres_solve_refined", "yasfhip_precond_build", "yasfhip_precond_apply",
+        "yasfhip_gmres_solve", "yasfhip_gmres_solve_refined", "yasfhip_cross_sections",
+        "yasfhip_far_field", "yasfhip_precond_build", "yasfhip_precond_apply",
         "yasfhip_time_matvec", "yasfhip_set_matvec_mode",
     ]
 
@@ -216,6 +219,29 @@ class Session:
             int(inner_maxiter), int(max_refinements), int(bool(use_precond)), ctypes.byref(info),
             ctypes.byref(steps), ctypes.byref(iters), ctypes.byref(res), _ptr(hist), hist.size))
         return x, info.value, steps.value, iters.value, res.value, hist[: steps.value + 1].copy()
+
+    def cross_sections(self, a_inc, f, channel: int = 0) -> tuple[float, float]:
+        """(C_ext, C_sca) of one channel from incident and scattered coefficients."""
+        a = self._vec(a_inc)
+        fv = self._vec(f)
+        ce = ctypes.c_double()
+        cs = ctypes.c_double()
+        _check(self._lib.yasfhip_cross_sections(self._h, int(channel), _ptr(a), _ptr(fv),
+                                                ctypes.byref(ce), ctypes.byref(cs)))
+        return ce.value, cs.value
+
+    def far_field(self, f, polar, azimuthal, channel: int = 0) -> tuple[np.ndarray, np.ndarray]:
+        """(E_theta, E_phi) of the scattered far field at directions (polar[a], azimuthal[a])."""
+        fv = self._vec(f)
+        th = np.ascontiguousarray(np.ravel(polar), dtype=np.float64)
+        ph = np.ascontiguousarray(np.ravel(azimuthal), dtype=np.float64)
+        if th.size != ph.size or th.size == 0:
+            raise ValueError("polar and azimuthal must be non-empty and of equal size")
+        et = np.empty(th.size, dtype=np.complex128)
+        ep = np.empty(th.size, dtype=np.complex128)
+        _check(self._lib.yasfhip_far_field(self._h, int(channel), _ptr(fv), th.size, _ptr(th), _ptr(ph),
+                                           _ptr(et), _ptr(ep)))
+        return et, ep
 
     def precond_build(self, box_of_particle, channel: int = 0):
         box = np.ascontiguousarray(box_of_particle, dtype=np.int32)

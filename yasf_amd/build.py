@@ -14,7 +14,10 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libyasfhip.so"
-SOURCES = ["tables.cpp", "coupling.hip", "pairtable.hip", "krylov.hip", "precond.hip", "capi.hip"]
+SOURCES = [
+    "tables.cpp", "coupling.hip", "pairtable.hip", "krylov.hip", "precond.hip", "observables.hip",
+    "capi.hip",
+]
 ARCH = os.environ.get("YASFHIP_ARCH", "gfx950")
 
 FLAGS = [

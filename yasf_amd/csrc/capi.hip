@@ -31,6 +31,10 @@ void launch_residual(double2 *r, const double2 *b, const double2 *ax, long n, Re
 void launch_norm2(const double2 *x, long n, Reducer red, double *out, hipStream_t s);
 void launch_to_fp32(float *dst, const double *src, long count, hipStream_t s);
 void launch_to_fp64(double *dst, const float *src, long count, hipStream_t s);
+// observables.hip
+void launch_far_field(const double4 *pos, const double2 *f, int N, int lmax, double k,
+                      const double4 *dir, const double *polar, const double *azimuthal, int A,
+                      double2 *F, double2 *e_theta, double2 *e_phi, hipStream_t s);
 // precond.hip
 struct BlockPrecond;
 BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &ab5,
@@ -666,6 +670,73 @@ int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b
             *iterations = inner;
         if (residual)
             *residual = bnorm > 0 ? rnorm / bnorm : 0.0;
+    });
+}
+
+int yasfhip_cross_sections(yasfhip_session *s, int channel, const double *a_inc, const double *f,
+                           double *c_ext, double *c_sca) {
+    return guarded([&] {
+        YH_REQUIRE(s && a_inc && f && c_ext && c_sca, "NULL argument");
+        YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
+        const long n = s->n;
+        hipStream_t st = s->stream;
+        YH_CHECK(hipMemcpyAsync(s->xin, f, n * sizeof(double2), hipMemcpyHostToDevice, st));
+        YH_CHECK(hipMemcpyAsync(s->avec, a_inc, n * sizeof(double2), hipMemcpyHostToDevice, st));
+        // y = W_J f (regular translation, coincident pairs included), then
+        // <f, y> and <a, f> (conj on the left operand)
+        launch_pair_regular(s->dp, s->pos, s->N, s->k[channel], s->xin, s->Q, st);
+        launch_contract(s->dp, s->N, s->Q, s->xin, s->t, 0, s->yout, false, false, st);
+        launch_mgs_step(s->yout, nullptr, nullptr, s->xin, n, s->reducer(), s->scal, st);
+        launch_mgs_step(s->xin, nullptr, nullptr, s->avec, n, s->reducer(), s->scal + 4, st);
+        s->fetch_scalars(8);
+        const double k2 = s->k[channel] * s->k[channel];
+        *c_ext = -s->scal_host[4] / k2 * M_PI;
+        *c_sca = std::hypot(s->scal_host[0], s->scal_host[1]) / k2 * M_PI;
+    });
+}
+
+int yasfhip_far_field(yasfhip_session *s, int channel, const double *f, int num_directions,
+                      const double *polar, const double *azimuthal, double *e_theta, double *e_phi) {
+    return guarded([&] {
+        YH_REQUIRE(s && f && polar && azimuthal && e_theta && e_phi, "NULL argument");
+        YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
+        YH_REQUIRE(num_directions >= 1, "num_directions must be >= 1");
+        const int A = num_directions;
+        const int Apad = (A + 15) / 16 * 16;
+        std::vector<double4> dir(Apad, make_double4(0, 0, 1, 0));
+        for (int a = 0; a < A; ++a) {
+            YH_REQUIRE(std::isfinite(polar[a]) && std::isfinite(azimuthal[a]), "angles must be finite");
+            const double sth = std::sin(polar[a]);
+            dir[a] = make_double4(sth * std::cos(azimuthal[a]), sth * std::sin(azimuthal[a]),
+                                  std::cos(polar[a]), 0.0);
+        }
+        hipStream_t st = s->stream;
+        // per-call buffers: the far field is evaluated once per solution
+        double4 *ddir = upload(dir, st);
+        std::vector<double> ang(2 * static_cast<size_t>(A));
+        std::copy(polar, polar + A, ang.begin());
+        std::copy(azimuthal, azimuthal + A, ang.begin() + A);
+        double *dang = upload(ang, st);
+        double2 *F = dmalloc<double2>(static_cast<size_t>(A) * s->nmax);
+        double2 *eout = dmalloc<double2>(2 * static_cast<size_t>(A));
+        auto release = [&] {
+            (void)hipStreamSynchronize(st);
+            for (void *p : {static_cast<void *>(ddir), static_cast<void *>(dang), static_cast<void *>(F),
+                            static_cast<void *>(eout)})
+                (void)hipFree(p);
+        };
+        try {
+            YH_CHECK(hipMemcpyAsync(s->xin, f, s->n * sizeof(double2), hipMemcpyHostToDevice, st));
+            launch_far_field(s->pos, s->xin, s->N, s->L, s->k[channel], ddir, dang, dang + A, A, F, eout,
+                             eout + A, st);
+            YH_CHECK(hipMemcpyAsync(e_theta, eout, A * sizeof(double2), hipMemcpyDeviceToHost, st));
+            YH_CHECK(hipMemcpyAsync(e_phi, eout + A, A * sizeof(double2), hipMemcpyDeviceToHost, st));
+            YH_CHECK(hipStreamSynchronize(st));
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
     });
 }
 

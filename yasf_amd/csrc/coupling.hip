@@ -30,7 +30,9 @@ namespace yasfhip {
 // One barrier per chunk hands the buffers over, so generation runs on the
 // VALU while the matrix cores work.
 // Q layout: [target tile][q][col 16][row 16] complex T (interleaved).
-template <typename T>
+// Regular: the J-translation variant (j_p in place of h_p, coincident pairs
+// included) used by the scattering cross section.
+template <typename T, bool Regular>
 __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ pos,
                                                    const typename Cplx<T>::type *__restrict__ x, int N,
                                                    int nmax, double k, const PairJob *__restrict__ jobs,
@@ -103,9 +105,13 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
         const double4 p2 = p2next;
         const int s2n = s2 + SRC;
         p2next = pos[s2n < N ? s2n : 0];
-        const bool valid = (s2 < N) && (s1 < N) && (s2 != s1);
-        pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld,
-                        buf + ptid, 256);
+        const bool valid = (s2 < N) && (s1 < N) && (Regular || s2 != s1);
+        if constexpr (Regular)
+            pair_series_regular<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld,
+                                    buf + ptid, 256);
+        else
+            pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld,
+                            buf + ptid, 256);
     };
 
     acc_t acc[kWaveMaxTiles][2];
@@ -217,18 +223,18 @@ __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q,
     }
 }
 
-template <typename T>
+template <typename T, bool Regular>
 static void launch_pair_t(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
                           void *Q, hipStream_t s) {
     const int ntiles1 = (N + 15) / 16;
     const size_t lds = 2 * static_cast<size_t>(dp.stage_elems) * sizeof(typename Cplx<T>::type);
     static bool lds_opt_in = false;  // gfx950: up to 160 KB of LDS per workgroup
     if (!lds_opt_in) {
-        YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pair_kernel<T>),
+        YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&pair_kernel<T, Regular>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         lds_opt_in = true;
     }
-    hipLaunchKernelGGL((pair_kernel<T>), dim3(dp.njobs * ntiles1), dim3(2 * kWavesPerJob * 64), lds, s,
+    hipLaunchKernelGGL((pair_kernel<T, Regular>), dim3(dp.njobs * ntiles1), dim3(2 * kWavesPerJob * 64), lds, s,
                        pos, static_cast<const typename Cplx<T>::type *>(x), N, dp.nmax, k, dp.jobs,
                        ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.la, dp.lb, dp.ld, static_cast<T *>(Q),
                        dp.nq, dp.stage_elems);
@@ -238,9 +244,14 @@ static void launch_pair_t(const DevicePlan &dp, const double4 *pos, int N, doubl
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
                  void *Q, hipStream_t s) {
     if (fp32)
-        launch_pair_t<float>(dp, pos, N, k, x, Q, s);
+        launch_pair_t<float, false>(dp, pos, N, k, x, Q, s);
     else
-        launch_pair_t<double>(dp, pos, N, k, x, Q, s);
+        launch_pair_t<double, false>(dp, pos, N, k, x, Q, s);
+}
+
+void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
+                         void *Q, hipStream_t s) {
+    launch_pair_t<double, true>(dp, pos, N, k, x, Q, s);
 }
 
 // Largest LDS the pair kernel asks for, bytes (2 stages of complex128).

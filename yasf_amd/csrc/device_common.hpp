@@ -45,23 +45,22 @@ struct Cplx<float> {
     typedef float2 type;
 };
 
-// G_{s1 s2}[p, dm] for every p in [|dm|, 2L], stored to dst[(p - |dm|) * stride].
-// Geometry follows the reference's mutual_lookup (misc.py:149-229):
-// d = r_s1 - r_s2, cos th = d_z/|d| clipped, sin th = sqrt(1 - cos^2),
-// phi = atan2(d_y, d_x), h_p = j_p + i y_p; Legendre per
-// legendre_normalized_trigon.py:59-91.  h_p comes from the upward recurrence
-// h_{p+1} = (2p+1)/x h_p - h_{p-1} seeded with h_0, h_1 (stable for the y_p
-// part that dominates |h_p|).  All arithmetic is FP64; invalid pairs store 0.
-template <typename T2>
-__device__ __forceinline__ void pair_series(double dx, double dy, double dz, double k, int dm, int P,
-                                            bool valid, const double *__restrict__ la,
-                                            const double *__restrict__ lb,
-                                            const double *__restrict__ ld, T2 *dst, int stride) {
+// Pair geometry of the reference's mutual_lookup (misc.py:149-229):
+// d = r_s1 - r_s2, cos th = d_z/|d| clipped (0 at d = 0), sin th =
+// sqrt(1 - cos^2), phi = atan2(d_y, d_x); (er, ei) = e^{i dm phi} by binary
+// powering of e^{i phi}; x = k |d| (1 at d = 0, unused there).
+struct PairGeometry {
+    double ct, st, er, ei, x;
+    bool zero_dist;
+};
+
+__device__ __forceinline__ PairGeometry pair_geometry(double dx, double dy, double dz, double k, int dm) {
+    PairGeometry g;
     const double d = sqrt(dx * dx + dy * dy + dz * dz);
-    const bool zero_dist = !(d > 0.0);
-    double ct = zero_dist ? 0.0 : dz / d;
-    ct = fmin(1.0, fmax(-1.0, ct));
-    const double st = sqrt(1.0 - ct * ct);
+    g.zero_dist = !(d > 0.0);
+    double ct = g.zero_dist ? 0.0 : dz / d;
+    g.ct = fmin(1.0, fmax(-1.0, ct));
+    g.st = sqrt(1.0 - g.ct * g.ct);
     const double rho2 = dx * dx + dy * dy;
     double c1 = 1.0, s1 = 0.0;
     if (rho2 > 0.0) {
@@ -70,7 +69,6 @@ __device__ __forceinline__ void pair_series(double dx, double dy, double dz, dou
         s1 = dy * ir;
     }
     const int adm = dm < 0 ? -dm : dm;
-    // e^{i dm phi} by binary powering of e^{i phi}.
     double er = 1.0, ei = 0.0, br = c1, bi = s1;
     for (int n = adm; n; n >>= 1) {
         if (n & 1) {
@@ -82,25 +80,41 @@ __device__ __forceinline__ void pair_series(double dx, double dy, double dz, dou
         bi = 2.0 * br * bi;
         br = t;
     }
-    if (dm < 0)
-        ei = -ei;
+    g.er = er;
+    g.ei = dm < 0 ? -ei : ei;
+    g.x = g.zero_dist ? 1.0 : k * d;
+    return g;
+}
+
+// G_{s1 s2}[p, dm] for every p in [|dm|, 2L], stored to dst[(p - |dm|) * stride]:
+// Ptilde_p^{|dm|}(cos th) h_p(k d) e^{i dm phi} with h_p = j_p + i y_p and the
+// Legendre functions of legendre_normalized_trigon.py:59-91.  h_p comes from
+// the upward recurrence h_{p+1} = (2p+1)/x h_p - h_{p-1} seeded with h_0, h_1
+// (stable for the y_p part that dominates |h_p|).  All arithmetic is FP64;
+// invalid pairs store 0.
+template <typename T2>
+__device__ __forceinline__ void pair_series(double dx, double dy, double dz, double k, int dm, int P,
+                                            bool valid, const double *__restrict__ la,
+                                            const double *__restrict__ lb,
+                                            const double *__restrict__ ld, T2 *dst, int stride) {
+    const PairGeometry g = pair_geometry(dx, dy, dz, k, dm);
+    const int adm = dm < 0 ? -dm : dm;
     // Hankel: (ar, ai) = h_p, (hr, hi) = h_{p+1}.
-    const double x = zero_dist ? 1.0 : k * d;
     double sx, cx;
-    sincos(x, &sx, &cx);
-    const double inv = 1.0 / x;
+    sincos(g.x, &sx, &cx);
+    const double inv = 1.0 / g.x;
     double ar = sx * inv, ai = -cx * inv;
     double hr = (ar - cx) * inv, hi = (ai - sx) * inv;
     // Legendre P_l^adm: cur = P_l, prv = P_{l-1}, starting at l = adm.
     double cur = ld[adm], prv = 0.0;
     for (int i = 0; i < adm; ++i)
-        cur *= st;
+        cur *= g.st;
     for (int p = 0; p < P; ++p) {
         if (p >= adm) {
             const double phr = cur * ar, phi = cur * ai;
-            double gr = phr * er - phi * ei;
-            double gi = phr * ei + phi * er;
-            if (zero_dist) {  // h_p(0) = j_p(0) = delta_p0, phi = 0, cos th = 0
+            double gr = phr * g.er - phi * g.ei;
+            double gi = phr * g.ei + phi * g.er;
+            if (g.zero_dist) {  // h_p(0) = j_p(0) = delta_p0, phi = 0, cos th = 0
                 gr = (p == 0) ? cur : 0.0;
                 gi = 0.0;
             }
@@ -108,11 +122,11 @@ __device__ __forceinline__ void pair_series(double dx, double dy, double dz, dou
                 gr = 0.0;
                 gi = 0.0;
             }
-            T2 g;
-            g.x = gr;
-            g.y = gi;
-            dst[(p - adm) * stride] = g;
-            const double nx = la[p * P + adm] * ct * cur - lb[p * P + adm] * prv;
+            T2 v;
+            v.x = gr;
+            v.y = gi;
+            dst[(p - adm) * stride] = v;
+            const double nx = la[p * P + adm] * g.ct * cur - lb[p * P + adm] * prv;
             prv = cur;
             cur = nx;
         }
@@ -122,6 +136,89 @@ __device__ __forceinline__ void pair_series(double dx, double dy, double dz, dou
         ai = hi;
         hr = nr;
         hi = ni;
+    }
+}
+
+constexpr int kMaxOrders = 2 * kMaxLmax + 1;  // p = 0 .. 2 lmax
+
+// Spherical Bessel j_p(x), p < kMaxOrders, x > 0.  The upward recurrence is
+// stable while p < x, so it serves x >= kMaxOrders; below that Miller's
+// downward recurrence from order kMaxOrders + 24 (where j_p/y_p is below
+// 1e-18 of its value at p <= 2 lmax for x < kMaxOrders) is normalised to the
+// closed form of j_0 or j_1, whichever is larger; x < 1e-3 uses the series
+// j_p = x^p/(2p+1)!! (1 - x^2/(2(2p+3))) (relative error below 1e-13).
+__device__ __forceinline__ void sph_bessel_j_all(double x, double (&jv)[kMaxOrders]) {
+    double sx, cx;
+    sincos(x, &sx, &cx);
+    const double inv = 1.0 / x;
+    const double j0 = sx * inv, j1 = (sx * inv - cx) * inv;
+    if (x >= kMaxOrders) {
+        jv[0] = j0;
+        jv[1] = j1;
+#pragma unroll
+        for (int p = 1; p + 1 < kMaxOrders; ++p)
+            jv[p + 1] = (2 * p + 1) * inv * jv[p] - jv[p - 1];
+        return;
+    }
+    if (x < 1e-3) {
+        double term = 1.0;  // x^p / (2p+1)!!
+#pragma unroll
+        for (int p = 0; p < kMaxOrders; ++p) {
+            jv[p] = term * (1.0 - x * x / (2.0 * (2 * p + 3)));
+            term *= x / (2 * p + 3);
+        }
+        return;
+    }
+    constexpr int kStart = kMaxOrders + 24;
+    double jp1 = 0.0, jc = 1e-280;  // j_{p+1}, j_p up to a common factor, p = kStart
+#pragma unroll
+    for (int p = kStart; p >= 1; --p) {
+        const double jm = (2 * p + 1) * inv * jc - jp1;
+        jp1 = jc;
+        jc = jm;
+        if (p - 1 < kMaxOrders)
+            jv[p - 1] = jm;
+    }
+    const double scale = fabs(j0) >= fabs(j1) ? j0 / jv[0] : j1 / jv[1];
+#pragma unroll
+    for (int p = 0; p < kMaxOrders; ++p)
+        jv[p] *= scale;
+}
+
+// As pair_series with the regular radial function j_p(k d) in place of h_p:
+// the J-translation factors of the scattering cross section
+// (cpu_numba.py:410-457 with the sph_j lookup).  Coincident positions give
+// j_p(0) = delta_p0.
+template <typename T2>
+__device__ __forceinline__ void pair_series_regular(double dx, double dy, double dz, double k, int dm,
+                                                    int P, bool valid, const double *__restrict__ la,
+                                                    const double *__restrict__ lb,
+                                                    const double *__restrict__ ld, T2 *dst, int stride) {
+    const PairGeometry g = pair_geometry(dx, dy, dz, k, dm);
+    const int adm = dm < 0 ? -dm : dm;
+    double jv[kMaxOrders];
+    if (g.zero_dist) {
+#pragma unroll
+        for (int p = 0; p < kMaxOrders; ++p)
+            jv[p] = p == 0 ? 1.0 : 0.0;
+    } else {
+        sph_bessel_j_all(g.x, jv);
+    }
+    double cur = ld[adm], prv = 0.0;
+    for (int i = 0; i < adm; ++i)
+        cur *= g.st;
+#pragma unroll
+    for (int p = 0; p < kMaxOrders; ++p) {
+        if (p < P && p >= adm) {
+            const double ph = valid ? cur * jv[p] : 0.0;
+            T2 v;
+            v.x = ph * g.er;
+            v.y = ph * g.ei;
+            dst[(p - adm) * stride] = v;
+            const double nx = la[p * P + adm] * g.ct * cur - lb[p * P + adm] * prv;
+            prv = cur;
+            cur = nx;
+        }
     }
 }
 

@@ -99,6 +99,9 @@ struct DevicePlan {
 // On-the-fly pair GEMM (pair factors generated in the kernel).
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
                  void *Q, hipStream_t s);
+// Q of the regular (J-translation) operator, coincident pairs included, FP64.
+void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
+                         void *Q, hipStream_t s);
 // ab5 contraction (FP64 accumulation) + optional master-operator epilogue.
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
                      int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s);
