@@ -20,6 +20,8 @@
 #ifndef YASFHIP_H
 #define YASFHIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -157,6 +159,63 @@ int yasfhip_set_matvec_mode(yasfhip_session *s, int mode);
  * ms_total = mean duration of the whole matvec (pair + contraction). */
 int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps, double *ms_pair,
                         double *ms_total);
+
+/* ------------------------------------------------------------------------
+ * Batched serving (no counterpart in the reference, which solves one
+ * Simulation at a time, yasfpy/scattered_field_coefficients.py:95-156).
+ *
+ * A server owns one HIP stream and one device-memory arena of memory_bytes
+ * (0: 70% of the free memory), allocated at creation.  Submitted requests are
+ * admitted first-in first-out while their resident state fits in the arena
+ * (complex64 pair-factor table, Krylov basis of restart+1 vectors, vectors
+ * and Hessenberg state); a request that cannot fit even alone completes with
+ * info = -1.  Each admitted request is solved by FP64 iterative refinement
+ * around restarted GMRES with the complex64 pair GEMM (x0 = b = t a_inc; one
+ * refinement per restart cycle; converged when the FP64 ||b - A x|| <=
+ * rtol ||b||; each cycle asked for max(inner_rtol, rtol ||b|| / (2 ||r||)));
+ * all requests inside a cycle advance together, one Arnoldi iteration per
+ * batched launch sequence (a HIP graph when use_graph != 0).
+ * Calls on one server are host-synchronous and not thread-safe.
+ * ------------------------------------------------------------------------ */
+typedef struct yasfhip_server yasfhip_server;
+
+int yasfhip_server_create(size_t memory_bytes, int restart, double inner_rtol, int use_graph,
+                          yasfhip_server **out);
+int yasfhip_server_destroy(yasfhip_server *s);
+
+/* Queue one solve: positions (N x 3), lmax, real medium wave number,
+ * T-matrix diagonal t[N*nmax] and incident coefficients a_inc[N*nmax]
+ * (complex128, particle-major), relative tolerance, at most max_cycles
+ * refinement cycles, and whether C_ext / C_sca are computed at completion
+ * (as yasfhip_cross_sections).  The inputs are copied. */
+int yasfhip_server_submit(yasfhip_server *s, const double *positions, int particle_number, int lmax,
+                          double k_medium, const double *t, const double *a_inc, double rtol,
+                          int max_cycles, int observables, long long *request_id);
+
+/* Admission, FP64 residuals of requests between cycles, completions, and one
+ * batched Arnoldi iteration of every request inside a cycle.  Reports the
+ * active and queued counts and the number of requests completed so far. */
+int yasfhip_server_step(yasfhip_server *s, int *active, int *queued, long long *completed);
+
+/* Next completed request (request_id = -1 when none): solution into x_out
+ * when x_capacity >= N*nmax complex values, info (0 converged, 1 not within
+ * max_cycles, -1 never fitted), refinement cycles, inner iterations, final
+ * FP64 relative residual, C_ext, C_sca and the seconds from submission to
+ * completion. */
+int yasfhip_server_poll(yasfhip_server *s, long long *request_id, double *x_out, long long x_capacity,
+                        int *info, int *cycles, int *iterations, double *residual, double *c_ext,
+                        double *c_sca, double *latency_s);
+
+/* Tile-shape autotuner of the batched complex64 pair GEMM: times 2, 4 and 8
+ * waves per (job, target tile) block (each where ceil((2 lmax + 1)/waves)
+ * orders per wave are supported) on a synthetic cluster of particle_number
+ * spheres, keeps the fastest for lmax and returns it; ms[3] receives the mean
+ * times (-1 for shapes not valid at this lmax).  Needs an idle server. */
+int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, int reps, int *waves,
+                            double *ms);
+
+/* Fixes the tile shape (waves per block: 2, 4 or 8) used for lmax. */
+int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves);
 
 #ifdef __cplusplus
 }

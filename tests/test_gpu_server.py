@@ -1,0 +1,129 @@
+"""Batched serving engine (yasfhip_server_*) on the GPU.
+
+Each served solution is checked independently through the single-session
+FP64 operator (itself pinned to the oracle at 1e-12): FP64 relative residual
+<= 1e-6, the reported residual and cross sections consistent with that
+operator, and C_ext / C_sca within 1e-6 of a tight (1e-10) all-FP64 solve.
+Batch composition must not change any result: runs with and without the HIP
+graph and with a small arena (forcing queueing) agree bit for bit."""
+
+import numpy as np
+import pytest
+
+from yasf_amd import _native
+from yasf_amd.serving import make_request, serve, solve_sequential
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(count, seed, n_range=(20, 160)):
+    rng = np.random.default_rng(seed)
+    return [make_request(rng, i, n_range=n_range) for i in range(count)]
+
+
+@pytest.fixture(scope="module")
+def mixed_requests():
+    return _requests(10, 7)
+
+
+@pytest.fixture(scope="module")
+def served(mixed_requests):
+    srv = _native.Server(memory_bytes=4 << 30, restart=50, inner_rtol=1e-4)
+    report = serve(srv, mixed_requests, keep_solutions=True)
+    srv.close()
+    return report
+
+
+def test_served_solutions_meet_fp64_criterion(mixed_requests, served):
+    assert served.completed == len(mixed_requests)
+    assert {r.lmax for r in mixed_requests} >= {3, 8} or len({r.lmax for r in mixed_requests}) >= 4
+    for r in mixed_requests:
+        res = served.results[r.index]
+        assert res["info"] == 0, (r.index, r.particles, r.lmax)
+        s = _native.Session(r.positions, r.lmax, [r.k])
+        s.set_tmatrix(r.t[np.newaxis, :])
+        b = r.t * r.a_inc
+        x = res["x"]
+        rel = np.linalg.norm(b - s.master_matvec(x)) / np.linalg.norm(b)
+        assert rel <= 1e-6
+        assert res["residual"] == pytest.approx(rel, rel=1e-6, abs=1e-12)
+        ce, cs = s.cross_sections(r.a_inc, x)
+        assert res["c_ext"] == pytest.approx(ce, rel=1e-10)
+        assert res["c_sca"] == pytest.approx(cs, rel=1e-10)
+        xt, info, _, _, _ = s.gmres(b, b, rtol=1e-10, restart=100, maxiter=100)
+        assert info == 0
+        ce64, cs64 = s.cross_sections(r.a_inc, xt)
+        assert abs(res["c_ext"] - ce64) <= 1e-6 * abs(ce64)
+        assert abs(res["c_sca"] - cs64) <= 1e-6 * abs(cs64)
+        s.close()
+
+
+def test_graph_eager_and_small_arena_agree_bitwise(mixed_requests, served):
+    eager = _native.Server(memory_bytes=4 << 30, restart=50, inner_rtol=1e-4, use_graph=False)
+    rep_eager = serve(eager, mixed_requests, keep_solutions=True)
+    eager.close()
+    # an arena that holds about two of these requests at a time
+    small = _native.Server(memory_bytes=160 << 20, restart=50, inner_rtol=1e-4)
+    rep_small = serve(small, mixed_requests, keep_solutions=True)
+    small.close()
+    for r in mixed_requests:
+        a = served.results[r.index]
+        for other in (rep_eager.results[r.index], rep_small.results[r.index]):
+            assert other["info"] == 0
+            assert np.array_equal(a["x"], other["x"])
+            assert a["iterations"] == other["iterations"] and a["cycles"] == other["cycles"]
+            assert a["c_ext"] == other["c_ext"] and a["c_sca"] == other["c_sca"]
+
+
+def test_sequential_reference_agrees(mixed_requests, served):
+    seq = solve_sequential(mixed_requests[:4])
+    for r in mixed_requests[:4]:
+        a, b = served.results[r.index], seq.results[r.index]
+        assert b["info"] == 0 and b["residual"] <= 1e-6
+        assert abs(a["c_ext"] - b["c_ext"]) <= 2e-6 * abs(b["c_ext"])
+        assert abs(a["c_sca"] - b["c_sca"]) <= 2e-6 * abs(b["c_sca"])
+
+
+def test_autotune_and_tile_shapes(mixed_requests):
+    srv = _native.Server(memory_bytes=2 << 30)
+    for L, valid in ((3, (2, 4, 8)), (5, (4, 8)), (8, (4, 8))):
+        w, ms = srv.autotune(L, particles=96, reps=2)
+        assert w in valid
+        for i, cand in enumerate((2, 4, 8)):
+            assert (ms[i] > 0) == (cand in valid)
+        assert ms[(2, 4, 8).index(w)] == pytest.approx(ms[ms > 0].min())
+    with pytest.raises(ValueError):
+        srv.set_tile_shape(8, 2)   # 17 orders over 2 waves exceeds the kernel's range
+    # every valid fixed shape gives the same solution bit for bit
+    reqs = mixed_requests[:3]
+    outs = []
+    for waves in (4, 8):
+        for L in range(3, 9):
+            srv.set_tile_shape(L, waves)
+        outs.append(serve(srv, reqs, keep_solutions=True))
+    for r in reqs:
+        assert np.array_equal(outs[0].results[r.index]["x"], outs[1].results[r.index]["x"])
+    srv.close()
+
+
+def test_limits_and_errors():
+    req = _requests(1, 11, n_range=(40, 40))[0]
+    srv = _native.Server(memory_bytes=1 << 20)   # far too small for any request
+    rid = srv.submit(req.positions, req.lmax, req.k, req.t, req.a_inc)
+    srv.step()
+    res = srv.poll()
+    assert res["id"] == rid and res["info"] == -1
+    srv.close()
+    srv = _native.Server(memory_bytes=1 << 30, restart=5)
+    srv.submit(req.positions, req.lmax, req.k, req.t, req.a_inc, rtol=1e-15, max_cycles=2)
+    while True:
+        srv.step()
+        res = srv.poll()
+        if res is not None:
+            break
+    assert res["info"] == 1 and res["cycles"] == 2 and res["iterations"] <= 10
+    with pytest.raises(ValueError):
+        srv.submit(req.positions, 9, req.k, req.t, req.a_inc)
+    with pytest.raises(ValueError):
+        _native.Server(restart=0)
+    srv.close()

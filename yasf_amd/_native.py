@@ -65,6 +65,13 @@ def _load():
             "yasfhip_precond_build": ([P, i, P, i], i),
             "yasfhip_precond_apply": ([P, P, P], i),
             "yasfhip_time_matvec": ([P, i, i, i, dp, dp], i),
+            "yasfhip_server_create": ([ctypes.c_size_t, i, d, i, ctypes.POINTER(P)], i),
+            "yasfhip_server_destroy": ([P], i),
+            "yasfhip_server_submit": ([P, P, i, i, d, P, P, d, i, i, llp], i),
+            "yasfhip_server_step": ([P, ip, ip, llp], i),
+            "yasfhip_server_poll": ([P, llp, P, ctypes.c_longlong, ip, ip, ip, dp, dp, dp, dp], i),
+            "yasfhip_server_autotune": ([P, i, i, i, ip, P], i),
+            "yasfhip_server_set_tile_shape": ([P, i, i], i),
             "yasfhip_set_matvec_mode": ([P, i], i),
         }
         for name, (args, res) in sig.items():
@@ -86,7 +93,9 @@ def exported_symbols() -> list[str]:
         "yasfhip_destroy", "yasfhip_set_tmatrix", "yasfhip_coupling_matvec", "yasfhip_master_matvec",
         "yasfhip_gmres_solve", "yasfhip_gmres_solve_refined", "yasfhip_cross_sections",
         "yasfhip_far_field", "yasfhip_precond_build", "yasfhip_precond_apply",
-        "yasfhip_time_matvec", "yasfhip_set_matvec_mode",
+        "yasfhip_time_matvec", "yasfhip_set_matvec_mode", "yasfhip_server_create",
+        "yasfhip_server_destroy", "yasfhip_server_submit", "yasfhip_server_step", "yasfhip_server_poll",
+        "yasfhip_server_autotune", "yasfhip_server_set_tile_shape",
     ]
 
 
@@ -267,3 +276,87 @@ class Session:
         _check(self._lib.yasfhip_time_matvec(self._h, int(channel), int(precision), int(reps),
                                              ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+class Server:
+    """Batched serving engine (yasfhip_server_*): many independent solves
+    advanced together, memory-admitted into a device arena."""
+
+    def __init__(self, memory_bytes: int = 0, restart: int = 50, inner_rtol: float = 1e-4,
+                 use_graph: bool = True):
+        lib = _load()
+        h = ctypes.c_void_p()
+        _check(lib.yasfhip_server_create(int(memory_bytes), int(restart), float(inner_rtol),
+                                         int(bool(use_graph)), ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+        self._sizes: dict[int, int] = {}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.yasfhip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, positions, lmax: int, k_medium: float, t, a_inc, rtol: float = 1e-6,
+               max_cycles: int = 100, observables: bool = True) -> int:
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        n = pos.shape[0] * 2 * lmax * (lmax + 2)
+        t = np.ascontiguousarray(t, dtype=np.complex128).ravel()
+        a = np.ascontiguousarray(a_inc, dtype=np.complex128).ravel()
+        if t.size != n or a.size != n:
+            raise ValueError(f"t and a_inc need {n} entries")
+        rid = ctypes.c_longlong()
+        _check(self._lib.yasfhip_server_submit(self._h, _ptr(pos), pos.shape[0], int(lmax), float(k_medium),
+                                               _ptr(t), _ptr(a), float(rtol), int(max_cycles),
+                                               int(bool(observables)), ctypes.byref(rid)))
+        self._sizes[rid.value] = n
+        return rid.value
+
+    def step(self) -> tuple[int, int, int]:
+        """(active, queued, completed so far) after one engine step."""
+        a = ctypes.c_int()
+        q = ctypes.c_int()
+        c = ctypes.c_longlong()
+        _check(self._lib.yasfhip_server_step(self._h, ctypes.byref(a), ctypes.byref(q), ctypes.byref(c)))
+        return a.value, q.value, c.value
+
+    def poll(self, with_solution: bool = True):
+        """Next completed request as a dict, or None."""
+        rid = ctypes.c_longlong()
+        info = ctypes.c_int()
+        cycles = ctypes.c_int()
+        iters = ctypes.c_int()
+        res = ctypes.c_double()
+        ce = ctypes.c_double()
+        cs = ctypes.c_double()
+        lat = ctypes.c_double()
+        # the next completion's size is unknown here: pass the largest pending buffer
+        cap = max(self._sizes.values()) if (with_solution and self._sizes) else 0
+        x = np.empty(max(cap, 1), dtype=np.complex128)
+        _check(self._lib.yasfhip_server_poll(self._h, ctypes.byref(rid), _ptr(x) if cap else None, cap,
+                                             ctypes.byref(info), ctypes.byref(cycles), ctypes.byref(iters),
+                                             ctypes.byref(res), ctypes.byref(ce), ctypes.byref(cs),
+                                             ctypes.byref(lat)))
+        if rid.value < 0:
+            return None
+        n = self._sizes.pop(rid.value, 0)
+        return {"id": rid.value, "x": x[:n].copy() if cap else None, "info": info.value,
+                "cycles": cycles.value, "iterations": iters.value, "residual": res.value,
+                "c_ext": ce.value, "c_sca": cs.value, "latency_s": lat.value}
+
+    def autotune(self, lmax: int, particles: int = 600, reps: int = 5):
+        """(chosen waves per block, ms for 2/4/8 waves; -1 where not valid)."""
+        w = ctypes.c_int()
+        ms = np.zeros(3)
+        _check(self._lib.yasfhip_server_autotune(self._h, int(lmax), int(particles), int(reps),
+                                                 ctypes.byref(w), _ptr(ms)))
+        return w.value, ms
+
+    def set_tile_shape(self, lmax: int, waves: int):
+        _check(self._lib.yasfhip_server_set_tile_shape(self._h, int(lmax), int(waves)))

@@ -17,18 +17,23 @@ def lognormal_radii(rng: np.random.Generator, n: int, median: float, sigma_ln: f
 
 def rsa_in_sphere(rng: np.random.Generator, radii: np.ndarray, volume_fraction: float,
                   max_attempts: int = 200000) -> np.ndarray:
-    """Non-overlapping centres (|r_i - r_j| >= r_i + r_j), largest spheres first."""
+    """Non-overlapping centres (|r_i - r_j| >= r_i + r_j), largest spheres first.
+
+    Each sphere draws uniform candidates in the ball of radius R - r until one
+    overlaps no placed sphere.  Overlap candidates come from a k-d tree of the
+    placed spheres (rebuilt every 128 placements) plus a direct check of the
+    ones placed since the last rebuild."""
+    from scipy.spatial import cKDTree
+
     n = radii.size
     R = (np.sum(radii ** 3) / volume_fraction) ** (1.0 / 3.0)
     order = np.argsort(-radii)
     pos = np.zeros((n, 3))
-    placed = np.zeros(n, dtype=bool)
-    cell = 2.0 * radii.max()
-    grid: dict = {}
-
-    def key(p):
-        return tuple(np.floor(p / cell).astype(int))
-
+    placed = np.zeros((n, 3))
+    placed_r = np.zeros(n)
+    rmax = radii.max()
+    count = 0
+    tree, in_tree = None, 0
     for idx in order:
         r = radii[idx]
         lim = R - r
@@ -38,26 +43,22 @@ def rsa_in_sphere(rng: np.random.Generator, radii: np.ndarray, volume_fraction: 
             c = rng.uniform(-lim, lim, 3)
             if c @ c > lim * lim:
                 continue
-            k = key(c)
             ok = True
-            for dx in (-1, 0, 1):
-                for dy in (-1, 0, 1):
-                    for dz in (-1, 0, 1):
-                        for j in grid.get((k[0] + dx, k[1] + dy, k[2] + dz), ()):
-                            d = c - pos[j]
-                            if d @ d < (r + radii[j]) ** 2:
-                                ok = False
-                                break
-                        if not ok:
-                            break
-                    if not ok:
-                        break
-                if not ok:
-                    break
+            if tree is not None:
+                near = tree.query_ball_point(c, r + rmax)
+                if near:
+                    d = placed[near] - c
+                    ok = not np.any(np.einsum("ij,ij->i", d, d) < (r + placed_r[near]) ** 2)
+            if ok and count > in_tree:
+                d = placed[in_tree:count] - c
+                ok = not np.any(np.einsum("ij,ij->i", d, d) < (r + placed_r[in_tree:count]) ** 2)
             if ok:
                 pos[idx] = c
-                placed[idx] = True
-                grid.setdefault(k, []).append(idx)
+                placed[count] = c
+                placed_r[count] = r
+                count += 1
+                if count - in_tree >= 128:
+                    tree, in_tree = cKDTree(placed[:count]), count
                 break
         else:
             raise RuntimeError("RSA failed to place a sphere; lower the volume fraction")

@@ -27,31 +27,34 @@ def mode_index(tau: int, l: int, m: int, lmax: int) -> int:
 
 def mie_coefficients(lmax: int, unique_pairs: np.ndarray, omega: np.ndarray,
                      k_medium: np.ndarray) -> np.ndarray:
-    """``[U, nmax, W]`` T-matrix diagonal for unique (radius, index) pairs."""
+    """``[U, nmax, W]`` T-matrix diagonal for unique (radius, index) pairs
+    (vectorised over pairs, orders and wavelengths)."""
     U = unique_pairs.shape[0]
-    W = omega.size
+    W = np.size(omega)
     out = np.zeros((U, nmax_of(lmax), W), dtype=complex)
-    ls = np.arange(1, lmax + 1)[:, None]
-    for u in range(U):
-        radius = float(np.real(unique_pairs[u, 0]))
-        k_sphere = omega * unique_pairs[u, 1]
-        rel = k_sphere / k_medium
-        x = (k_medium * radius)[None, :]
-        mx = (k_sphere * radius)[None, :]
-        jx, djx_ = spherical_jn(ls, x), spherical_jn(ls, x, derivative=True)
-        yx, dyx_ = spherical_yn(ls, x), spherical_yn(ls, x, derivative=True)
-        jmx, djmx_ = spherical_jn(ls, mx), spherical_jn(ls, mx, derivative=True)
-        hx = jx + 1j * yx
-        dhx = hx + x * (djx_ + 1j * dyx_)          # d/dx [x h_l(x)]
-        djx = jx + x * djx_                        # d/dx [x j_l(x)]
-        djmx = jmx + mx * djmx_
-        b_l = -(jmx * djx - jx * djmx) / (jmx * dhx - hx * djmx)
-        a_l = -(rel ** 2 * jmx * djx - jx * djmx) / (rel ** 2 * jmx * dhx - hx * djmx)
-        for li, l in enumerate(range(1, lmax + 1)):
-            s1 = l * l - 1
-            out[u, s1:s1 + 2 * l + 1, :] = b_l[li]
-            s2 = lmax * (lmax + 2) + l * l - 1
-            out[u, s2:s2 + 2 * l + 1, :] = a_l[li]
+    if U == 0:
+        return out
+    ls = np.arange(1, lmax + 1)[:, None, None]                          # [L, 1, 1]
+    radius = np.real(unique_pairs[:, 0]).astype(float)[None, :, None]   # [1, U, 1]
+    k_sphere = np.asarray(omega)[None, None, :] * unique_pairs[:, 1][None, :, None]
+    km = np.asarray(k_medium)[None, None, :]
+    rel = k_sphere / km
+    x = km * radius
+    mx = k_sphere * radius
+    jx, djx_ = spherical_jn(ls, x), spherical_jn(ls, x, derivative=True)
+    yx, dyx_ = spherical_yn(ls, x), spherical_yn(ls, x, derivative=True)
+    jmx, djmx_ = spherical_jn(ls, mx), spherical_jn(ls, mx, derivative=True)
+    hx = jx + 1j * yx
+    dhx = hx + x * (djx_ + 1j * dyx_)          # d/dx [x h_l(x)]
+    djx = jx + x * djx_                        # d/dx [x j_l(x)]
+    djmx = jmx + mx * djmx_
+    b_l = -(jmx * djx - jx * djmx) / (jmx * dhx - hx * djmx)                                  # [L, U, W]
+    a_l = -(rel ** 2 * jmx * djx - jx * djmx) / (rel ** 2 * jmx * dhx - hx * djmx)
+    for li, l in enumerate(range(1, lmax + 1)):
+        s1 = l * l - 1
+        out[:, s1:s1 + 2 * l + 1, :] = b_l[li][:, None, :]
+        s2 = lmax * (lmax + 2) + l * l - 1
+        out[:, s2:s2 + 2 * l + 1, :] = a_l[li][:, None, :]
     return out
 
 

@@ -134,9 +134,8 @@ struct yasfhip_session {
         for (void *b : bufs)
             if (b)
                 (void)hipFree(b);
-        void *plan_bufs[] = {dp.jobs, dp.colmap, dp.crow, dp.cidx, dp.cval, dp.la, dp.lb, dp.ld, dp.dmoff,
-                             table[0], table[1]};
-        for (void *b : plan_bufs)
+        free_device_plan(dp);
+        for (void *b : table)
             if (b)
                 (void)hipFree(b);
         if (scal_host)
@@ -413,6 +412,15 @@ int fail(const yasfhip::Error &e) {
     return e.code;
 }
 
+}  // namespace
+
+int yasfhip::set_last_error(const std::string &msg, int code) {
+    g_last_error = msg;
+    return code;
+}
+
+namespace {
+
 template <typename F>
 int guarded(F &&f) {
     try {
@@ -484,29 +492,7 @@ int yasfhip_create(const double *positions, int particle_number, int lmax, const
         s->pos = upload(pos, st);
 
         const auto &pl = s->plan;
-        s->dp.lmax = pl.lmax;
-        s->dp.nmax = pl.nmax;
-        s->dp.maxcols = pl.maxcols;
-        s->dp.nq = pl.nq;
-        s->dp.njobs = static_cast<int>(pl.jobs.size());
-        s->dp.stage_elems = pl.stage_elems;
-        s->dp.jobs = upload(pl.jobs, st);
-        s->dp.colmap = upload(pl.colmap, st);
-        s->dp.crow = upload(pl.crow, st);
-        s->dp.cidx = upload(pl.cidx, st);
-        std::vector<double2> cval(pl.cval.size() / 2);
-        for (size_t i = 0; i < cval.size(); ++i)
-            cval[i] = make_double2(pl.cval[2 * i], pl.cval[2 * i + 1]);
-        s->dp.cval = upload(cval, st);
-        s->dp.la = upload(pl.leg_alpha, st);
-        s->dp.lb = upload(pl.leg_beta, st);
-        s->dp.ld = upload(pl.leg_diag, st);
-        std::vector<int> dmoff(4 * lmax + 1);
-        for (int d = 0, acc = 0; d < 4 * lmax + 1; ++d) {
-            dmoff[d] = acc;
-            acc += 2 * lmax + 1 - std::abs(d - 2 * lmax);
-        }
-        s->dp.dmoff = upload(dmoff, st);
+        s->dp = upload_device_plan(pl, st);
 
         s->Q = dmalloc<double2>(static_cast<size_t>(s->ntiles1) * pl.nq * 256);
         s->t = dmalloc<double2>(static_cast<size_t>(s->W) * s->n);

@@ -183,7 +183,7 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
 
 // (W x)[s1, n1] = sum_e ab5_e Q[e], then either y = W x (mode 0) or the
 // master operator y = x - t (W x) (mode 1, matrix_apply.py:104-110).
-// One block per 16-particle target tile; lane&15 = target in tile.
+// One block per 16-particle target tile.
 template <typename TQ, typename TX, typename TY>
 __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q, int nq,
                                                        const int *__restrict__ crow,
@@ -192,35 +192,7 @@ __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q,
                                                        const TX *__restrict__ x,
                                                        const double2 *__restrict__ t, int mode,
                                                        TY *__restrict__ y) {
-    const int s1t = blockIdx.x;
-    const int r = threadIdx.x & 15;
-    const int s1 = s1t * 16 + r;
-    const size_t qrow = static_cast<size_t>(s1t) * nq * 16;
-    for (int n1 = threadIdx.x >> 4; n1 < nmax; n1 += blockDim.x >> 4) {
-        double sr = 0.0, si = 0.0;
-        const int e1 = crow[n1 + 1];
-        for (int e = crow[n1]; e < e1; ++e) {
-            const double2 a = cval[e];
-            const size_t qi = 2 * ((qrow + cidx[e]) * 16 + r);
-            const double qr = static_cast<double>(Q[qi]);
-            const double qim = static_cast<double>(Q[qi + 1]);
-            sr += a.x * qr - a.y * qim;
-            si += a.x * qim + a.y * qr;
-        }
-        if (s1 < N) {
-            const size_t j = static_cast<size_t>(s1) * nmax + n1;
-            if (mode == 1) {
-                const double2 tj = t[j];
-                const double xr = static_cast<double>(x[2 * j]);
-                const double xi = static_cast<double>(x[2 * j + 1]);
-                y[2 * j] = static_cast<TY>(xr - (tj.x * sr - tj.y * si));
-                y[2 * j + 1] = static_cast<TY>(xi - (tj.x * si + tj.y * sr));
-            } else {
-                y[2 * j] = static_cast<TY>(sr);
-                y[2 * j + 1] = static_cast<TY>(si);
-            }
-        }
-    }
+    contract_tile<TQ, TX, TY>(Q, nq, crow, cidx, cval, N, nmax, x, t, mode, y, blockIdx.x);
 }
 
 template <typename T, bool Regular>

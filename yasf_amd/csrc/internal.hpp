@@ -96,6 +96,14 @@ struct DevicePlan {
     int *dmoff;  // [ndm] first order slot of each dm in a pair-table target tile
 };
 
+// Records the message of the calling thread's last failure (yasfhip_last_error)
+// and returns code.
+int set_last_error(const std::string &msg, int code);
+
+// Device copies of a plan's tables (asynchronous on s) and their release.
+DevicePlan upload_device_plan(const CouplingPlan &plan, hipStream_t s);
+void free_device_plan(DevicePlan &dp);
+
 // On-the-fly pair GEMM (pair factors generated in the kernel).
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
                  void *Q, hipStream_t s);

@@ -1,0 +1,1287 @@
+// Batched serving of many independent solves on one device.
+//
+// Requests (geometry, lmax, wave number, T-matrix diagonal, incident
+// coefficients) are admitted first-in first-out while they fit in a device
+// memory arena (one allocation made at start-up, carved per request; nothing
+// is allocated per request).  An admitted request keeps its complex64
+// pair-factor table, Krylov basis, Hessenberg/Givens state and vectors
+// resident until it completes.
+//
+// Each request runs GMRES-IR: restart cycles of GMRES whose operator applies
+// the complex64 pair GEMM (contraction, epilogue, Arnoldi and Givens in FP64),
+// and after every cycle the FP64 residual b - A x (FP64 on-the-fly matvec)
+// decides convergence (||r|| <= rtol ||b||, the reference's criterion,
+// yasfpy/solver.py:108-121) and seeds the next cycle, which is asked for the
+// reduction max(inner_rtol, rtol ||b|| / (2 ||r||)).  x0 = b as in
+// yasfpy/scattered_field_coefficients.py:95-156.
+//
+// One engine step advances every request that is inside a cycle by one
+// Arnoldi iteration with a fixed sequence of batched launches (one grid over
+// all such requests each): normalise + complex64 copy, pair GEMM over the
+// resident tables (one launch per tile shape), contraction + master epilogue,
+// classical Gram-Schmidt with one reorthogonalisation (CGS2: dots, reduce,
+// update + dots, reduce, update + norm), and the Givens update.  The
+// sequence is captured once per set of active requests in a HIP graph and
+// replayed; requests outside a cycle skip every kernel through their device
+// state word.  Cycle ends, FP64 residuals, admission and completion run
+// between replays.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+
+#include "../../include/yasfhip.h"
+#include "table_gemm.hpp"
+
+namespace yasfhip {
+
+// krylov.hip / observables-side helpers reused per request
+struct Reducer {
+    double *partials;
+    unsigned *counter;
+};
+void launch_update(double2 *x, const double2 *V, const double2 *y, int m, long n, long ld,
+                   hipStream_t s);
+void launch_residual(double2 *r, const double2 *b, const double2 *ax, long n, Reducer red, double *out,
+                     hipStream_t s);
+void launch_norm2(const double2 *x, long n, Reducer red, double *out, hipStream_t s);
+void launch_mgs_step(double2 *w, const double2 *v_prev, const double2 *h_prev, const double2 *v_next,
+                     long n, Reducer red, double *out, hipStream_t s);
+
+namespace {
+
+constexpr int kChunk = 2048;        // vector elements per block of the Arnoldi kernels
+constexpr int kVecThreads = 256;
+constexpr int kPerThread = kChunk / kVecThreads;
+constexpr int kMaxRestart = 128;
+constexpr int kMaxGemmGroups = 3;   // tile shapes: 2, 4 or 8 waves per block
+
+enum : int { kIdle = 0, kArnoldi = 1, kCycleDone = 2 };
+
+// Static description of an active request (device copy rebuilt when the set
+// of active requests changes).
+struct BatchReq {
+    const double4 *pos;
+    const float2 *table;  // complex64 pair factors in MFMA A layout
+    const double2 *t;     // T-matrix diagonal
+    double2 *V;           // [restart+1][n] Krylov basis (FP64)
+    float2 *x32;          // complex64 copy of the current basis vector
+    float *Q;             // pair-GEMM output [ntiles1][nq][16][16] complex64
+    double2 *partial;     // [chunks][restart+1] dot-product partials
+    double *npartial;     // [2][chunks] squared-norm partials (before, after)
+    double2 *H;           // [restart][restart+1] Hessenberg, column-major
+    double2 *h1, *h2;     // [restart+1] Gram-Schmidt coefficients of the two passes
+    double2 *g;           // [restart+1] rotated residual vector
+    double *gc;           // [restart] Givens cosines
+    double2 *gs;          // [restart] Givens sines
+    const PairJob *jobs;
+    const int *colmap, *crow, *cidx, *dmoff;
+    const double2 *cval;
+    long n;
+    int N, lmax, nmax, ntiles1, N4, nq, maxcols, chunks, restart;
+};
+
+// Dynamic state, written by the device inside a cycle and by the host between.
+struct ReqState {
+    int state;
+    int j;          // columns done in this cycle; V[j] is the next basis vector
+    int breakdown;
+    int pad;
+    double inv_norm;  // pending scale of V[j] (the norm found by the Givens step)
+    double target;    // absolute presid target of the cycle
+    double presid;
+};
+
+// Largest r with start[r] <= b (start[0] = 0).
+__device__ __forceinline__ int find_owner(const int *__restrict__ start, int count, int b) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return uniform(lo);
+}
+
+__device__ __forceinline__ double2 wave_sum2(double2 v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_down(v.x, off, 64);
+        v.y += __shfl_down(v.y, off, 64);
+    }
+    return v;
+}
+
+// V[j] *= inv_norm (normalising the vector the previous Givens step sized)
+// and its complex64 copy for the pair GEMM.  Grid: chunks of every request.
+__global__ __launch_bounds__(kVecThreads) void serve_prep_kernel(const BatchReq *__restrict__ reqs,
+                                                                 const ReqState *__restrict__ st,
+                                                                 const int *__restrict__ cstart, int nreq) {
+    const int r = find_owner(cstart, nreq, blockIdx.x);
+    if (st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    const long c = blockIdx.x - cstart[r];
+    const double s = st[r].inv_norm;
+    double2 *v = q.V + static_cast<long>(st[r].j) * q.n;
+    const long e0 = c * kChunk, e1 = min(e0 + kChunk, q.n);
+    for (long e = e0 + threadIdx.x; e < e1; e += kVecThreads) {
+        double2 a = v[e];
+        a.x *= s;
+        a.y *= s;
+        v[e] = a;
+        q.x32[e] = make_float2(static_cast<float>(a.x), static_cast<float>(a.y));
+    }
+}
+
+// Complex64 pair GEMM over the resident tables of a group of requests that
+// share one tile shape (WAVES waves per (job, target tile) block).
+template <int WAVES, int NIPMAX>
+__global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *__restrict__ reqs,
+                                                                const ReqState *__restrict__ st,
+                                                                const int *__restrict__ gstart,
+                                                                const int *__restrict__ gidx, int ngrp) {
+    const int gi = find_owner(gstart, ngrp, blockIdx.x);
+    const int r = uniform(gidx[gi]);
+    if (st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = blockIdx.x - gstart[gi];
+    const int ntiles1 = q.ntiles1;
+    const int jid = b / ntiles1;
+    const int t = b - jid * ntiles1;
+    const int dm = uniform(q.jobs[jid].dm);
+    const int tile0 = uniform(q.jobs[jid].tile0);
+    const int ntile = uniform(q.jobs[jid].ntile);
+    const int qbase = uniform(q.jobs[jid].qbase);
+    const int lmax = q.lmax;
+    const int P = 2 * lmax + 1;
+    const int R = P - (dm < 0 ? -dm : dm);
+    const int d = dm + 2 * lmax;
+    const int nip = R > wave ? (R - wave + WAVES - 1) / WAVES : 0;
+    if (nip == 0)
+        return;
+    const float2 *tab = q.table + (static_cast<size_t>(t) * P * P + q.dmoff[d]) * q.N4 * 64 + lane;
+    int n2c[2];
+#pragma unroll
+    for (int it = 0; it < 2; ++it)
+        n2c[it] = (it < ntile) ? q.colmap[d * q.maxcols + (tile0 + it) * 16 + (lane & 15)] : -1;
+    float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
+    table_gemm_dispatch<float, NIPMAX, WAVES>(tab, q.x32, q.N, q.N4, q.nmax, R, wave, nip, ntile, n2c, Qt);
+}
+
+// V[j+1] = V[j] - t (W V[j]) from the complex64 Q, FP64 accumulation.
+// Grid: 16-particle target tiles of every request.
+__global__ __launch_bounds__(256) void serve_contract_kernel(const BatchReq *__restrict__ reqs,
+                                                             const ReqState *__restrict__ st,
+                                                             const int *__restrict__ tstart, int nreq) {
+    const int r = find_owner(tstart, nreq, blockIdx.x);
+    if (st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    const int j = st[r].j;
+    const double2 *v = q.V + static_cast<long>(j) * q.n;
+    double2 *w = q.V + static_cast<long>(j + 1) * q.n;
+    contract_tile<float, double, double>(q.Q, q.nq, q.crow, q.cidx, q.cval, q.N, q.nmax,
+                                         reinterpret_cast<const double *>(v), q.t, 1,
+                                         reinterpret_cast<double *>(w), blockIdx.x - tstart[r]);
+}
+
+// Classical Gram-Schmidt passes over w = V[j+1], one block per chunk:
+//   mode 0: partial[c][k] = <V_k, w>_c (k <= j), npartial[0][c] = ||w||_c^2
+//   mode 1: w -= sum_k h1[k] V_k, then partial[c][k] = <V_k, w>_c
+//   mode 2: w -= sum_k h2[k] V_k, then npartial[1][c] = ||w||_c^2
+// (<a, b> = sum conj(a) b as np.vdot).
+__global__ __launch_bounds__(kVecThreads) void serve_gs_kernel(const BatchReq *__restrict__ reqs,
+                                                               const ReqState *__restrict__ st,
+                                                               const int *__restrict__ cstart, int nreq,
+                                                               int mode) {
+    __shared__ double2 sh[kMaxRestart + 1][kVecThreads / 64];
+    const int r = find_owner(cstart, nreq, blockIdx.x);
+    if (st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    const int c = blockIdx.x - cstart[r];
+    const int j = st[r].j;
+    const long n = q.n;
+    const int R1 = q.restart + 1;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double2 *w = q.V + static_cast<long>(j + 1) * n;
+    const long e0 = static_cast<long>(c) * kChunk;
+    double2 wl[kPerThread];
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+        const long e = e0 + i * kVecThreads + threadIdx.x;
+        wl[i] = e < n ? w[e] : make_double2(0.0, 0.0);
+    }
+    if (mode != 0) {
+        const double2 *h = mode == 1 ? q.h1 : q.h2;
+        for (int k = 0; k <= j; ++k) {
+            const double2 hk = h[k];
+            const double2 *vk = q.V + static_cast<long>(k) * n;
+#pragma unroll
+            for (int i = 0; i < kPerThread; ++i) {
+                const long e = e0 + i * kVecThreads + threadIdx.x;
+                if (e < n) {
+                    const double2 a = vk[e];
+                    wl[i].x -= hk.x * a.x - hk.y * a.y;
+                    wl[i].y -= hk.x * a.y + hk.y * a.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i) {
+            const long e = e0 + i * kVecThreads + threadIdx.x;
+            if (e < n)
+                w[e] = wl[i];
+        }
+    }
+    if (mode != 2) {
+        for (int k = 0; k <= j; ++k) {
+            const double2 *vk = q.V + static_cast<long>(k) * n;
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < kPerThread; ++i) {
+                const long e = e0 + i * kVecThreads + threadIdx.x;
+                if (e < n) {
+                    const double2 a = vk[e];
+                    acc.x += a.x * wl[i].x + a.y * wl[i].y;
+                    acc.y += a.x * wl[i].y - a.y * wl[i].x;
+                }
+            }
+            acc = wave_sum2(acc);
+            if (lane == 0)
+                sh[k][wv] = acc;
+        }
+    }
+    if (mode != 1) {
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i)
+            acc.x += wl[i].x * wl[i].x + wl[i].y * wl[i].y;
+        acc = wave_sum2(acc);
+        if (lane == 0)
+            sh[kMaxRestart][wv] = acc;
+    }
+    __syncthreads();
+    if (mode != 2) {
+        for (int k = threadIdx.x; k <= j; k += kVecThreads) {
+            double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < kVecThreads / 64; ++i) {
+                s.x += sh[k][i].x;
+                s.y += sh[k][i].y;
+            }
+            q.partial[static_cast<long>(c) * R1 + k] = s;
+        }
+    }
+    if (mode != 1 && threadIdx.x == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < kVecThreads / 64; ++i)
+            s += sh[kMaxRestart][i].x;
+        q.npartial[(mode == 2 ? q.chunks : 0) + c] = s;
+    }
+}
+
+// h[k] = sum_c partial[c][k] in chunk order (deterministic); one block per request.
+__global__ __launch_bounds__(kMaxRestart + 1) void serve_reduce_kernel(const BatchReq *__restrict__ reqs,
+                                                                      const ReqState *__restrict__ st,
+                                                                      int nreq, int pass) {
+    const int r = blockIdx.x;
+    if (r >= nreq || st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    const int j = st[r].j, R1 = q.restart + 1;
+    for (int k = threadIdx.x; k <= j; k += blockDim.x) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int c = 0; c < q.chunks; ++c) {
+            const double2 p = q.partial[static_cast<long>(c) * R1 + k];
+            s.x += p.x;
+            s.y += p.y;
+        }
+        (pass == 0 ? q.h1 : q.h2)[k] = s;
+    }
+}
+
+__device__ __forceinline__ double cabs2(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
+
+// LAPACK zlartg semantics: [c s; -conj(s) c] [f; g] = [r; 0], c real.
+__device__ void lartg_dev(double2 f, double2 g, double &c, double2 &s, double2 &r) {
+    if (g.x == 0.0 && g.y == 0.0) {
+        c = 1.0;
+        s = make_double2(0.0, 0.0);
+        r = f;
+        return;
+    }
+    if (f.x == 0.0 && f.y == 0.0) {
+        c = 0.0;
+        const double ag = cabs2(g);
+        s = make_double2(g.x / ag, -g.y / ag);
+        r = make_double2(ag, 0.0);
+        return;
+    }
+    const double af = cabs2(f), ag = cabs2(g);
+    const double d = sqrt(af * af + ag * ag);
+    c = af / d;
+    const double2 sg = make_double2(f.x / af, f.y / af);  // f / |f|
+    // s = sg * conj(g) / d
+    s = make_double2((sg.x * g.x + sg.y * g.y) / d, (sg.y * g.x - sg.x * g.y) / d);
+    r = make_double2(sg.x * d, sg.y * d);
+}
+
+// Hessenberg column j, Givens rotation, residual estimate and the cycle's
+// stopping decision (SciPy's inner loop, iterative.py:755-795).  One thread
+// per request.
+__global__ void serve_givens_kernel(const BatchReq *__restrict__ reqs, ReqState *__restrict__ st,
+                                    int nreq) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nreq || st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    ReqState s = st[r];
+    const int j = s.j, R1 = q.restart + 1;
+    double h0sq = 0.0, nsq = 0.0;
+    for (int c = 0; c < q.chunks; ++c) {
+        h0sq += q.npartial[c];
+        nsq += q.npartial[q.chunks + c];
+    }
+    double2 *col = q.H + static_cast<long>(j) * R1;
+    for (int k = 0; k <= j; ++k)
+        col[k] = make_double2(q.h1[k].x + q.h2[k].x, q.h1[k].y + q.h2[k].y);
+    const double hn = sqrt(nsq);
+    bool breakdown = false;
+    col[j + 1] = make_double2(hn, 0.0);
+    if (hn <= 2.220446049250313e-16 * sqrt(h0sq)) {
+        col[j + 1] = make_double2(0.0, 0.0);
+        breakdown = true;
+    }
+    for (int k = 0; k < j; ++k) {
+        const double c = q.gc[k];
+        const double2 sn = q.gs[k];
+        const double2 n0 = col[k], n1 = col[k + 1];
+        // col[k] = c n0 + sn n1 ; col[k+1] = -conj(sn) n0 + c n1
+        col[k] = make_double2(c * n0.x + sn.x * n1.x - sn.y * n1.y, c * n0.y + sn.x * n1.y + sn.y * n1.x);
+        col[k + 1] = make_double2(-(sn.x * n0.x + sn.y * n0.y) + c * n1.x,
+                                  -(sn.x * n0.y - sn.y * n0.x) + c * n1.y);
+    }
+    double c;
+    double2 sn, mag;
+    lartg_dev(col[j], col[j + 1], c, sn, mag);
+    q.gc[j] = c;
+    q.gs[j] = sn;
+    col[j] = mag;
+    col[j + 1] = make_double2(0.0, 0.0);
+    const double2 gj = q.g[j];
+    // tmp = -conj(sn) g[j]
+    const double2 tmp = make_double2(-(sn.x * gj.x + sn.y * gj.y), -(sn.x * gj.y - sn.y * gj.x));
+    q.g[j] = make_double2(c * gj.x, c * gj.y);
+    q.g[j + 1] = tmp;
+    s.presid = cabs2(tmp);
+    s.inv_norm = breakdown ? 1.0 : 1.0 / hn;
+    s.breakdown = breakdown ? 1 : 0;
+    s.j = j + 1;
+    if (s.presid <= s.target || s.j == q.restart || breakdown)
+        s.state = kCycleDone;
+    st[r] = s;
+}
+
+// Cycle end: y = H^-1 g (upper triangular, m columns); one thread.
+__global__ void serve_backsolve_kernel(const double2 *__restrict__ H, const double2 *__restrict__ g, int m,
+                                       int R1, double2 *__restrict__ y) {
+    for (int k = m - 1; k >= 0; --k) {
+        double2 v = g[k];
+        for (int i = k + 1; i < m; ++i) {
+            const double2 hik = H[static_cast<long>(i) * R1 + k];
+            const double2 yi = y[i];
+            v.x -= hik.x * yi.x - hik.y * yi.y;
+            v.y -= hik.x * yi.y + hik.y * yi.x;
+        }
+        const double2 d = H[static_cast<long>(k) * R1 + k];
+        const double dd = d.x * d.x + d.y * d.y;
+        y[k] = dd > 0.0 ? make_double2((v.x * d.x + v.y * d.y) / dd, (v.y * d.x - v.x * d.y) / dd)
+                        : make_double2(0.0, 0.0);
+    }
+}
+
+// b = t a, x = b (x0 = b), at admission.
+__global__ void serve_rhs_kernel(const double2 *__restrict__ t, const double2 *__restrict__ a,
+                                 double2 *__restrict__ b, double2 *__restrict__ x, long n) {
+    for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x) {
+        const double2 tt = t[i], aa = a[i];
+        const double2 v = make_double2(tt.x * aa.x - tt.y * aa.y, tt.x * aa.y + tt.y * aa.x);
+        b[i] = v;
+        x[i] = v;
+    }
+}
+
+// g = [rnorm, 0, ...] at a cycle start.
+__global__ void serve_cycle_init_kernel(double2 *__restrict__ g, int R1, double rnorm) {
+    for (int k = threadIdx.x; k < R1; k += blockDim.x)
+        g[k] = make_double2(k == 0 ? rnorm : 0.0, 0.0);
+}
+
+// ---------------------------------------------------------------- host side
+
+// First-fit device arena over one allocation, offsets aligned to 256 bytes.
+class Arena {
+  public:
+    void init(size_t bytes) {
+        YH_CHECK(hipMalloc(&base_, bytes));
+        cap_ = bytes;
+        free_.clear();
+        free_[0] = bytes;
+    }
+    ~Arena() {
+        if (base_)
+            (void)hipFree(base_);
+    }
+    void *alloc(size_t bytes) {
+        bytes = (bytes + 255) / 256 * 256;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                const size_t off = it->first, rest = it->second - bytes;
+                free_.erase(it);
+                if (rest)
+                    free_[off + bytes] = rest;
+                used_[off] = bytes;
+                return static_cast<char *>(base_) + off;
+            }
+        }
+        return nullptr;
+    }
+    void release(void *p) {
+        const size_t off = static_cast<char *>(p) - static_cast<char *>(base_);
+        auto u = used_.find(off);
+        if (u == used_.end())
+            return;
+        size_t start = off, len = u->second;
+        used_.erase(u);
+        auto nx = free_.lower_bound(start);
+        if (nx != free_.end() && nx->first == start + len) {
+            len += nx->second;
+            nx = free_.erase(nx);
+        }
+        if (nx != free_.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == start) {
+                start = pv->first;
+                len += pv->second;
+                free_.erase(pv);
+            }
+        }
+        free_[start] = len;
+    }
+    size_t capacity() const { return cap_; }
+    size_t in_use() const {
+        size_t s = 0;
+        for (auto &u : used_)
+            s += u.second;
+        return s;
+    }
+
+  private:
+    void *base_ = nullptr;
+    size_t cap_ = 0;
+    std::map<size_t, size_t> free_, used_;
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+enum Phase { kQueued, kRefresh, kInCycle, kFinished };
+
+struct Request {
+    long long id = 0;
+    int N = 0, L = 0, nmax = 0, ntiles1 = 0, chunks = 0;
+    long n = 0;
+    double k = 0.0, rtol = 1e-6;
+    int max_cycles = 100;
+    bool observables = true;
+    std::vector<double4> pos;
+    std::vector<double2> t, a;
+    // device (one arena block)
+    void *block = nullptr;
+    double4 *pos_d = nullptr;
+    double2 *t_d = nullptr, *a_d = nullptr, *b_d = nullptr, *x_d = nullptr, *y_d = nullptr;
+    float2 *table = nullptr, *x32 = nullptr;
+    void *Q = nullptr;
+    double2 *V = nullptr, *partial = nullptr, *H = nullptr, *h1 = nullptr, *h2 = nullptr, *g = nullptr,
+            *gs = nullptr;
+    double *npartial = nullptr, *gc = nullptr;
+    // progress
+    Phase phase = kQueued;
+    bool cycle_ended = false, first_refresh = true;
+    int cycles = 0, iterations = 0, waves = 8, slot = -1;
+    double bnorm = 0.0, rnorm = 0.0;
+    double t_submit = 0.0, t_done = 0.0;
+    // results
+    std::vector<double2> x_out;
+    int info = 0;
+    double c_ext = 0.0, c_sca = 0.0;
+};
+
+struct PlanEntry {
+    CouplingPlan plan;
+    DevicePlan dp{};
+};
+
+}  // namespace
+
+}  // namespace yasfhip
+
+using namespace yasfhip;
+
+struct yasfhip_server {
+    hipStream_t stream = nullptr;
+    Arena arena;
+    int restart = 50;
+    double inner_rtol = 1e-4;
+    bool use_graph = true;
+    long long next_id = 1;
+    std::map<int, PlanEntry> plans;
+    int waves_for_lmax[kMaxLmax + 1];
+    std::deque<Request *> queue;
+    std::vector<Request *> active;
+    std::deque<Request *> finished;
+    long long completed = 0;
+    // batched launch data
+    BatchReq *d_reqs = nullptr;
+    ReqState *d_state = nullptr, *h_state = nullptr;
+    int *d_index = nullptr;  // [cstart | tstart | gstart x groups | gidx x groups]
+    int cap_reqs = 0, cap_index = 0;
+    std::vector<int> h_index;
+    std::vector<BatchReq> h_reqs;  // host copies, kept alive across their async uploads
+    int chunk_blocks = 0, tile_blocks = 0;
+    int grp_blocks[kMaxGemmGroups] = {0, 0, 0}, grp_count[kMaxGemmGroups] = {0, 0, 0};
+    int grp_waves[kMaxGemmGroups] = {2, 4, 8};
+    bool set_changed = true;
+    hipGraphExec_t graph = nullptr;
+    // per-request scalar results of the refresh / completion syncs
+    double *scal = nullptr, *scal_host = nullptr;
+    int cap_scal = 0;
+    double *partials = nullptr;
+    unsigned *counter = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+
+    ~yasfhip_server() {
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        if (graph)
+            (void)hipGraphExecDestroy(graph);
+        for (auto *r : queue)
+            delete r;
+        for (auto *r : active)
+            delete r;
+        for (auto *r : finished)
+            delete r;
+        for (auto &p : plans)
+            free_device_plan(p.second.dp);
+        void *bufs[] = {d_reqs, d_state, d_index, scal, partials, counter};
+        for (void *b : bufs)
+            if (b)
+                (void)hipFree(b);
+        if (h_state)
+            (void)hipHostFree(h_state);
+        if (scal_host)
+            (void)hipHostFree(scal_host);
+        for (auto &e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    Reducer reducer() const { return Reducer{partials, counter}; }
+
+    PlanEntry &plan_for(int L) {
+        auto it = plans.find(L);
+        if (it != plans.end())
+            return it->second;
+        PlanEntry &e = plans[L];
+        e.plan = build_coupling_plan(L);
+        e.dp = upload_device_plan(e.plan, stream);
+        return e;
+    }
+
+    // Bytes of one request's arena block, with the carving of its buffers.
+    size_t layout(Request &q, bool carve) {
+        const DevicePlan &dp = plan_for(q.L).dp;
+        const size_t R1 = restart + 1;
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            const size_t o = off;
+            off += (bytes + 255) / 256 * 256;
+            return o;
+        };
+        const size_t o_pos = take(sizeof(double4) * q.ntiles1 * 16);
+        const size_t o_t = take(sizeof(double2) * q.n), o_a = take(sizeof(double2) * q.n);
+        const size_t o_b = take(sizeof(double2) * q.n), o_x = take(sizeof(double2) * q.n);
+        const size_t o_y = take(sizeof(double2) * R1);
+        const size_t o_tab = take(pair_table_bytes(dp, q.N, true));
+        const size_t o_x32 = take(sizeof(float2) * q.n);
+        const size_t o_Q = take(sizeof(double2) * q.ntiles1 * dp.nq * 256);
+        const size_t o_V = take(sizeof(double2) * R1 * q.n);
+        const size_t o_part = take(sizeof(double2) * R1 * q.chunks);
+        const size_t o_np = take(sizeof(double) * 2 * q.chunks);
+        const size_t o_H = take(sizeof(double2) * R1 * restart);
+        const size_t o_h1 = take(sizeof(double2) * R1), o_h2 = take(sizeof(double2) * R1);
+        const size_t o_g = take(sizeof(double2) * R1);
+        const size_t o_gc = take(sizeof(double) * R1), o_gs = take(sizeof(double2) * R1);
+        if (carve) {
+            char *b = static_cast<char *>(q.block);
+            q.pos_d = reinterpret_cast<double4 *>(b + o_pos);
+            q.t_d = reinterpret_cast<double2 *>(b + o_t);
+            q.a_d = reinterpret_cast<double2 *>(b + o_a);
+            q.b_d = reinterpret_cast<double2 *>(b + o_b);
+            q.x_d = reinterpret_cast<double2 *>(b + o_x);
+            q.y_d = reinterpret_cast<double2 *>(b + o_y);
+            q.table = reinterpret_cast<float2 *>(b + o_tab);
+            q.x32 = reinterpret_cast<float2 *>(b + o_x32);
+            q.Q = b + o_Q;
+            q.V = reinterpret_cast<double2 *>(b + o_V);
+            q.partial = reinterpret_cast<double2 *>(b + o_part);
+            q.npartial = reinterpret_cast<double *>(b + o_np);
+            q.H = reinterpret_cast<double2 *>(b + o_H);
+            q.h1 = reinterpret_cast<double2 *>(b + o_h1);
+            q.h2 = reinterpret_cast<double2 *>(b + o_h2);
+            q.g = reinterpret_cast<double2 *>(b + o_g);
+            q.gc = reinterpret_cast<double *>(b + o_gc);
+            q.gs = reinterpret_cast<double2 *>(b + o_gs);
+        }
+        return off;
+    }
+
+    void ensure_scal(int count) {
+        if (cap_scal >= count)
+            return;
+        if (scal)
+            YH_CHECK(hipFree(scal));
+        if (scal_host)
+            YH_CHECK(hipHostFree(scal_host));
+        cap_scal = std::max(count, 2 * cap_scal);
+        YH_CHECK(hipMalloc(&scal, 8 * sizeof(double) * cap_scal));
+        YH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&scal_host), 8 * sizeof(double) * cap_scal,
+                               hipHostMallocDefault));
+    }
+
+    // Orders per wave, ceil((2L+1)/waves), must not exceed the kernel's NIPMAX.
+    static bool waves_valid(int L, int waves) {
+        const int nipmax = waves == 8 ? 3 : 5;
+        return waves >= 2 && (2 * L + 1 + waves - 1) / waves <= nipmax;
+    }
+
+    int waves_of(int L) const {
+        const int w = waves_for_lmax[L];
+        YH_REQUIRE(waves_valid(L, w), "invalid tile shape for this lmax");
+        return w;
+    }
+
+    // Admission: first-in first-out while the head request fits.
+    void admit() {
+        while (!queue.empty()) {
+            Request &q = *queue.front();
+            const size_t bytes = layout(q, false);
+            q.block = arena.alloc(bytes);
+            if (!q.block) {
+                if (active.empty()) {  // can never fit: fail it
+                    q.info = -1;
+                    q.phase = kFinished;
+                    q.t_done = now_s();
+                    finished.push_back(&q);
+                    queue.pop_front();
+                    ++completed;
+                    continue;
+                }
+                return;
+            }
+            layout(q, true);
+            queue.pop_front();
+            hipStream_t st = stream;
+            YH_CHECK(hipMemcpyAsync(q.pos_d, q.pos.data(), sizeof(double4) * q.pos.size(),
+                                    hipMemcpyHostToDevice, st));
+            YH_CHECK(hipMemcpyAsync(q.t_d, q.t.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
+            YH_CHECK(hipMemcpyAsync(q.a_d, q.a.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(serve_rhs_kernel, dim3(std::min<long>(1024, (q.n + 255) / 256)), dim3(256), 0,
+                               st, q.t_d, q.a_d, q.b_d, q.x_d, q.n);
+            YH_CHECK(hipGetLastError());
+            const DevicePlan &dp = plan_for(q.L).dp;
+            launch_pair_table_build(dp, q.pos_d, q.N, q.k, true, q.table, st);
+            q.waves = waves_of(q.L);
+            q.phase = kRefresh;
+            q.first_refresh = true;
+            q.slot = static_cast<int>(active.size());
+            active.push_back(&q);
+            ensure_state(static_cast<int>(active.size()));
+            h_state[q.slot] = ReqState{kIdle, 0, 0, 0, 1.0, 0.0, 0.0};
+            set_changed = true;
+        }
+    }
+
+    // FP64 residual of every request waiting for one (after admission or a
+    // finished cycle), convergence decisions and cycle starts.
+    void refresh() {
+        std::vector<Request *> todo;
+        for (auto *q : active)
+            if (q->phase == kRefresh)
+                todo.push_back(q);
+        if (todo.empty())
+            return;
+        ensure_scal(static_cast<int>(active.size()));
+        hipStream_t st = stream;
+        for (auto *q : todo) {
+            const DevicePlan &dp = plan_for(q->L).dp;
+            if (q->cycle_ended) {
+                const int m = h_state[q->slot].j;
+                hipLaunchKernelGGL(serve_backsolve_kernel, dim3(1), dim3(1), 0, st, q->H, q->g, m, restart + 1,
+                                   q->y_d);
+                YH_CHECK(hipGetLastError());
+                launch_update(q->x_d, q->V, q->y_d, m, q->n, q->n, st);
+            }
+            // r = b - A x (FP64) into V[0]; V[1] holds A x
+            double2 *ax = q->V + q->n;
+            launch_pair(dp, q->pos_d, q->N, q->k, q->x_d, false, q->Q, st);
+            launch_contract(dp, q->N, q->Q, q->x_d, q->t_d, 1, ax, false, false, st);
+            launch_residual(q->V, q->b_d, ax, q->n, reducer(), scal + 8 * q->slot, st);
+            if (q->first_refresh)
+                launch_norm2(q->b_d, q->n, reducer(), scal + 8 * q->slot + 1, st);
+        }
+        YH_CHECK(hipMemcpyAsync(scal_host, scal, 8 * sizeof(double) * active.size(), hipMemcpyDeviceToHost,
+                                st));
+        YH_CHECK(hipStreamSynchronize(st));
+        std::vector<Request *> done;
+        for (auto *q : todo) {
+            q->rnorm = std::sqrt(scal_host[8 * q->slot]);
+            if (q->first_refresh) {
+                q->bnorm = std::sqrt(scal_host[8 * q->slot + 1]);
+                q->first_refresh = false;
+            }
+            if (q->cycle_ended) {
+                q->iterations += h_state[q->slot].j;
+                ++q->cycles;
+                q->cycle_ended = false;
+            }
+            const bool converged = q->rnorm <= q->rtol * q->bnorm;
+            if (converged || q->cycles >= q->max_cycles || q->bnorm == 0.0) {
+                q->info = converged || q->bnorm == 0.0 ? 0 : 1;
+                done.push_back(q);
+                continue;
+            }
+            // next cycle: V[0] = r / ||r|| (applied by the prep kernel)
+            hipLaunchKernelGGL(serve_cycle_init_kernel, dim3(1), dim3(128), 0, st, q->g, restart + 1, q->rnorm);
+            YH_CHECK(hipGetLastError());
+            ReqState &s = h_state[q->slot];
+            s.state = kArnoldi;
+            s.j = 0;
+            s.breakdown = 0;
+            s.inv_norm = 1.0 / q->rnorm;
+            s.target = std::max(inner_rtol, 0.5 * q->rtol * q->bnorm / q->rnorm) * q->rnorm;
+            s.presid = q->rnorm;
+            q->phase = kInCycle;
+        }
+        if (!done.empty())
+            complete(done);
+    }
+
+    // Observables, results to the host and release of the arena blocks.
+    void complete(std::vector<Request *> &done) {
+        hipStream_t st = stream;
+        for (auto *q : done) {
+            if (!q->observables)
+                continue;
+            const DevicePlan &dp = plan_for(q->L).dp;
+            double2 *y = q->V + q->n;
+            launch_pair_regular(dp, q->pos_d, q->N, q->k, q->x_d, q->Q, st);
+            launch_contract(dp, q->N, q->Q, q->x_d, q->t_d, 0, y, false, false, st);
+            launch_mgs_step(y, nullptr, nullptr, q->x_d, q->n, reducer(), scal + 8 * q->slot + 2, st);
+            launch_mgs_step(q->x_d, nullptr, nullptr, q->a_d, q->n, reducer(), scal + 8 * q->slot + 5, st);
+        }
+        for (auto *q : done) {
+            q->x_out.resize(q->n);
+            YH_CHECK(hipMemcpyAsync(q->x_out.data(), q->x_d, sizeof(double2) * q->n, hipMemcpyDeviceToHost, st));
+        }
+        YH_CHECK(hipMemcpyAsync(scal_host, scal, 8 * sizeof(double) * active.size(), hipMemcpyDeviceToHost,
+                                st));
+        YH_CHECK(hipStreamSynchronize(st));
+        const double t = now_s();
+        for (auto *q : done) {
+            if (q->observables) {
+                const double k2 = q->k * q->k;
+                q->c_sca = std::hypot(scal_host[8 * q->slot + 2], scal_host[8 * q->slot + 3]) / k2 * M_PI;
+                q->c_ext = -scal_host[8 * q->slot + 5] / k2 * M_PI;
+            }
+            q->phase = kFinished;
+            q->t_done = t;
+            arena.release(q->block);
+            q->block = nullptr;
+            finished.push_back(q);
+            ++completed;
+        }
+        // compact the active list and its host states
+        std::vector<Request *> keep;
+        std::vector<ReqState> keep_state;
+        for (auto *q : active)
+            if (q->phase != kFinished) {
+                keep_state.push_back(h_state[q->slot]);
+                q->slot = static_cast<int>(keep.size());
+                keep.push_back(q);
+            }
+        active.swap(keep);
+        for (size_t i = 0; i < keep_state.size(); ++i)
+            h_state[i] = keep_state[i];
+        set_changed = true;
+    }
+
+    // Room for nreq descriptors and state words (device and pinned host).
+    void ensure_state(int nreq) {
+        if (nreq <= cap_reqs)
+            return;
+        YH_CHECK(hipStreamSynchronize(stream));
+        if (graph) {
+            YH_CHECK(hipGraphExecDestroy(graph));
+            graph = nullptr;
+        }
+        if (d_reqs)
+            YH_CHECK(hipFree(d_reqs));
+        if (d_state)
+            YH_CHECK(hipFree(d_state));
+        ReqState *old = h_state;
+        const int cap = std::max(nreq, 2 * cap_reqs);
+        YH_CHECK(hipMalloc(&d_reqs, sizeof(BatchReq) * cap));
+        YH_CHECK(hipMalloc(&d_state, sizeof(ReqState) * cap));
+        YH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h_state), sizeof(ReqState) * cap,
+                               hipHostMallocDefault));
+        std::memset(h_state, 0, sizeof(ReqState) * cap);
+        if (old) {
+            std::memcpy(h_state, old, sizeof(ReqState) * cap_reqs);
+            YH_CHECK(hipHostFree(old));
+        }
+        cap_reqs = cap;
+        set_changed = true;
+    }
+
+    // Device descriptors and block partitions of the current active set.
+    void rebuild_batch() {
+        const int nreq = static_cast<int>(active.size());
+        ensure_state(nreq);
+        h_reqs.assign(nreq, BatchReq{});
+        std::vector<BatchReq> &br = h_reqs;
+        h_index.assign(2 * (nreq + 1) + kMaxGemmGroups * 2 * (nreq + 1), 0);
+        int *cstart = h_index.data(), *tstart = cstart + nreq + 1;
+        int cb = 0, tb = 0;
+        for (int r = 0; r < nreq; ++r) {
+            Request &q = *active[r];
+            const DevicePlan &dp = plan_for(q.L).dp;
+            BatchReq &b = br[r];
+            b.pos = q.pos_d;
+            b.table = q.table;
+            b.t = q.t_d;
+            b.V = q.V;
+            b.x32 = q.x32;
+            b.Q = static_cast<float *>(q.Q);
+            b.partial = q.partial;
+            b.npartial = q.npartial;
+            b.H = q.H;
+            b.h1 = q.h1;
+            b.h2 = q.h2;
+            b.g = q.g;
+            b.gc = q.gc;
+            b.gs = q.gs;
+            b.jobs = dp.jobs;
+            b.colmap = dp.colmap;
+            b.crow = dp.crow;
+            b.cidx = dp.cidx;
+            b.dmoff = dp.dmoff;
+            b.cval = dp.cval;
+            b.n = q.n;
+            b.N = q.N;
+            b.lmax = q.L;
+            b.nmax = q.nmax;
+            b.ntiles1 = q.ntiles1;
+            b.N4 = (q.N + 3) / 4;
+            b.nq = dp.nq;
+            b.maxcols = dp.maxcols;
+            b.chunks = q.chunks;
+            b.restart = restart;
+            cstart[r] = cb;
+            tstart[r] = tb;
+            cb += q.chunks;
+            tb += q.ntiles1;
+        }
+        cstart[nreq] = cb;
+        tstart[nreq] = tb;
+        chunk_blocks = cb;
+        tile_blocks = tb;
+        for (int gi = 0; gi < kMaxGemmGroups; ++gi) {
+            int *gstart = h_index.data() + 2 * (nreq + 1) + gi * 2 * (nreq + 1);
+            int *gidx = gstart + nreq + 1;
+            int cnt = 0, blocks = 0;
+            for (int r = 0; r < nreq; ++r) {
+                Request &q = *active[r];
+                if (q.waves != grp_waves[gi])
+                    continue;
+                gstart[cnt] = blocks;
+                gidx[cnt] = r;
+                blocks += plan_for(q.L).dp.njobs * q.ntiles1;
+                ++cnt;
+            }
+            gstart[cnt] = blocks;
+            grp_count[gi] = cnt;
+            grp_blocks[gi] = blocks;
+        }
+        if (static_cast<int>(h_index.size()) > cap_index) {
+            if (d_index)
+                YH_CHECK(hipFree(d_index));
+            cap_index = static_cast<int>(2 * h_index.size());
+            YH_CHECK(hipMalloc(&d_index, sizeof(int) * cap_index));
+        }
+        YH_CHECK(hipMemcpyAsync(d_reqs, br.data(), sizeof(BatchReq) * nreq, hipMemcpyHostToDevice, stream));
+        YH_CHECK(hipMemcpyAsync(d_index, h_index.data(), sizeof(int) * h_index.size(), hipMemcpyHostToDevice,
+                                stream));
+        if (graph) {
+            YH_CHECK(hipGraphExecDestroy(graph));
+            graph = nullptr;
+        }
+        set_changed = false;
+    }
+
+    // The batched Arnoldi iteration (see the file comment).
+    void launch_iteration() {
+        const int nreq = static_cast<int>(active.size());
+        const int *cstart = d_index, *tstart = d_index + nreq + 1;
+        hipStream_t st = stream;
+        hipLaunchKernelGGL(serve_prep_kernel, dim3(chunk_blocks), dim3(kVecThreads), 0, st, d_reqs, d_state,
+                           cstart, nreq);
+        for (int gi = 0; gi < kMaxGemmGroups; ++gi) {
+            if (!grp_count[gi])
+                continue;
+            const int *gstart = d_index + 2 * (nreq + 1) + gi * 2 * (nreq + 1);
+            const int *gidx = gstart + nreq + 1;
+            switch (grp_waves[gi]) {
+            case 2:
+                hipLaunchKernelGGL((serve_gemm_kernel<2, 5>), dim3(grp_blocks[gi]), dim3(128), 0, st, d_reqs,
+                                   d_state, gstart, gidx, grp_count[gi]);
+                break;
+            case 4:
+                hipLaunchKernelGGL((serve_gemm_kernel<4, 5>), dim3(grp_blocks[gi]), dim3(256), 0, st, d_reqs,
+                                   d_state, gstart, gidx, grp_count[gi]);
+                break;
+            default:
+                hipLaunchKernelGGL((serve_gemm_kernel<8, 3>), dim3(grp_blocks[gi]), dim3(512), 0, st, d_reqs,
+                                   d_state, gstart, gidx, grp_count[gi]);
+                break;
+            }
+        }
+        hipLaunchKernelGGL(serve_contract_kernel, dim3(tile_blocks), dim3(256), 0, st, d_reqs, d_state, tstart,
+                           nreq);
+        hipLaunchKernelGGL(serve_gs_kernel, dim3(chunk_blocks), dim3(kVecThreads), 0, st, d_reqs, d_state,
+                           cstart, nreq, 0);
+        hipLaunchKernelGGL(serve_reduce_kernel, dim3(nreq), dim3(kMaxRestart + 1), 0, st, d_reqs, d_state, nreq,
+                           0);
+        hipLaunchKernelGGL(serve_gs_kernel, dim3(chunk_blocks), dim3(kVecThreads), 0, st, d_reqs, d_state,
+                           cstart, nreq, 1);
+        hipLaunchKernelGGL(serve_reduce_kernel, dim3(nreq), dim3(kMaxRestart + 1), 0, st, d_reqs, d_state, nreq,
+                           1);
+        hipLaunchKernelGGL(serve_gs_kernel, dim3(chunk_blocks), dim3(kVecThreads), 0, st, d_reqs, d_state,
+                           cstart, nreq, 2);
+        hipLaunchKernelGGL(serve_givens_kernel, dim3((nreq + 63) / 64), dim3(64), 0, st, d_reqs, d_state, nreq);
+        YH_CHECK(hipGetLastError());
+    }
+
+    // Times the batched complex64 pair GEMM of one synthetic request (N spheres
+    // on a jittered cubic lattice, lmax L) for every valid tile shape and
+    // keeps the fastest for L.  Needs an idle server.
+    int autotune(int L, int N, int reps, double *ms) {
+        YH_REQUIRE(active.empty() && queue.empty(), "autotune needs an idle server");
+        Request q;
+        q.N = N;
+        q.L = L;
+        q.nmax = 2 * L * (L + 2);
+        q.n = static_cast<long>(N) * q.nmax;
+        q.ntiles1 = (N + 15) / 16;
+        q.chunks = static_cast<int>((q.n + kChunk - 1) / kChunk);
+        q.k = 2.0 * M_PI / 500.0;
+        const int side = static_cast<int>(std::ceil(std::cbrt(static_cast<double>(N))));
+        q.pos.assign(q.ntiles1 * 16, make_double4(0, 0, 0, 0));
+        unsigned rng = 12345u;
+        auto jitter = [&] {
+            rng = rng * 1664525u + 1013904223u;
+            return 20.0 * ((rng >> 8) / 16777216.0 - 0.5);
+        };
+        for (int i = 0; i < N; ++i)
+            q.pos[i] = make_double4(250.0 * (i % side) + jitter(), 250.0 * ((i / side) % side) + jitter(),
+                                    250.0 * (i / (side * side)) + jitter(), 0.0);
+        q.block = arena.alloc(layout(q, false));
+        YH_REQUIRE(q.block, "autotune request does not fit in the arena");
+        layout(q, true);
+        hipStream_t st = stream;
+        std::vector<float2> xh(q.n);
+        for (long i = 0; i < q.n; ++i)
+            xh[i] = make_float2(static_cast<float>(std::sin(0.37 * i)), static_cast<float>(std::cos(0.21 * i)));
+        YH_CHECK(hipMemcpyAsync(q.pos_d, q.pos.data(), sizeof(double4) * q.pos.size(), hipMemcpyHostToDevice,
+                                st));
+        YH_CHECK(hipMemcpyAsync(q.x32, xh.data(), sizeof(float2) * q.n, hipMemcpyHostToDevice, st));
+        const DevicePlan &dp = plan_for(L).dp;
+        launch_pair_table_build(dp, q.pos_d, N, q.k, true, q.table, st);
+        const int cand[kMaxGemmGroups] = {2, 4, 8};
+        int best = 8;
+        double best_ms = 1e300;
+        q.slot = 0;
+        active.push_back(&q);
+        try {
+            for (int gi = 0; gi < kMaxGemmGroups; ++gi) {
+                ms[gi] = -1.0;
+                if (!waves_valid(L, cand[gi]))
+                    continue;
+                q.waves = cand[gi];
+                set_changed = true;
+                rebuild_batch();
+                h_state[0] = ReqState{kArnoldi, 0, 0, 0, 1.0, 0.0, 0.0};
+                YH_CHECK(hipMemcpyAsync(d_state, h_state, sizeof(ReqState), hipMemcpyHostToDevice, st));
+                const int *gstart = d_index + 2 * 2 + gi * 2 * 2;
+                auto run = [&] {
+                    switch (cand[gi]) {
+                    case 2:
+                        hipLaunchKernelGGL((serve_gemm_kernel<2, 5>), dim3(grp_blocks[gi]), dim3(128), 0, st,
+                                           d_reqs, d_state, gstart, gstart + 2, 1);
+                        break;
+                    case 4:
+                        hipLaunchKernelGGL((serve_gemm_kernel<4, 5>), dim3(grp_blocks[gi]), dim3(256), 0, st,
+                                           d_reqs, d_state, gstart, gstart + 2, 1);
+                        break;
+                    default:
+                        hipLaunchKernelGGL((serve_gemm_kernel<8, 3>), dim3(grp_blocks[gi]), dim3(512), 0, st,
+                                           d_reqs, d_state, gstart, gstart + 2, 1);
+                        break;
+                    }
+                    YH_CHECK(hipGetLastError());
+                };
+                run();  // warm-up
+                YH_CHECK(hipEventRecord(ev[0], st));
+                for (int i = 0; i < reps; ++i)
+                    run();
+                YH_CHECK(hipEventRecord(ev[1], st));
+                YH_CHECK(hipEventSynchronize(ev[1]));
+                float t = 0.f;
+                YH_CHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
+                ms[gi] = t / reps;
+                if (ms[gi] < best_ms) {
+                    best_ms = ms[gi];
+                    best = cand[gi];
+                }
+            }
+        } catch (...) {
+            active.clear();
+            arena.release(q.block);
+            set_changed = true;
+            throw;
+        }
+        active.clear();
+        arena.release(q.block);
+        set_changed = true;
+        waves_for_lmax[L] = best;
+        return best;
+    }
+
+    // One engine step; returns the number of requests inside a cycle.
+    int step() {
+        admit();
+        refresh();
+        admit();  // completions may have freed memory
+        refresh();
+        if (set_changed)
+            rebuild_batch();
+        int in_cycle = 0;
+        for (auto *q : active)
+            if (q->phase == kInCycle)
+                ++in_cycle;
+        if (!in_cycle)
+            return 0;
+        const int nreq = static_cast<int>(active.size());
+        YH_CHECK(hipMemcpyAsync(d_state, h_state, sizeof(ReqState) * nreq, hipMemcpyHostToDevice, stream));
+        if (use_graph) {
+            if (!graph) {
+                hipGraph_t g = nullptr;
+                YH_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                launch_iteration();
+                YH_CHECK(hipStreamEndCapture(stream, &g));
+                YH_CHECK(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
+                YH_CHECK(hipGraphDestroy(g));
+            }
+            YH_CHECK(hipGraphLaunch(graph, stream));
+        } else {
+            launch_iteration();
+        }
+        YH_CHECK(hipMemcpyAsync(h_state, d_state, sizeof(ReqState) * nreq, hipMemcpyDeviceToHost, stream));
+        YH_CHECK(hipStreamSynchronize(stream));
+        for (auto *q : active)
+            if (q->phase == kInCycle && h_state[q->slot].state == kCycleDone) {
+                q->phase = kRefresh;
+                q->cycle_ended = true;
+                h_state[q->slot].state = kIdle;
+            }
+        return in_cycle;
+    }
+};
+
+namespace {
+
+template <typename F>
+int sguard(F &&f) {
+    try {
+        f();
+        set_last_error("", kOk);
+        return kOk;
+    } catch (const yasfhip::Error &e) {
+        return set_last_error(e.what(), e.code);
+    } catch (const std::exception &e) {
+        return set_last_error(e.what(), kInternal);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int yasfhip_server_create(size_t memory_bytes, int restart, double inner_rtol, int use_graph,
+                          yasfhip_server **out) {
+    return sguard([&] {
+        YH_REQUIRE(out, "out is NULL");
+        YH_REQUIRE(restart >= 1 && restart <= kMaxRestart, "restart must be in [1, 128]");
+        YH_REQUIRE(inner_rtol > 0.0 && inner_rtol < 1.0, "inner_rtol must be in (0, 1)");
+        auto s = std::make_unique<yasfhip_server>();
+        s->restart = restart;
+        s->inner_rtol = inner_rtol;
+        s->use_graph = use_graph != 0;
+        for (int L = 0; L <= kMaxLmax; ++L)
+            s->waves_for_lmax[L] = 8;
+        YH_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        for (auto &e : s->ev)
+            YH_CHECK(hipEventCreate(&e));
+        if (memory_bytes == 0) {
+            size_t free_b = 0, total_b = 0;
+            YH_CHECK(hipMemGetInfo(&free_b, &total_b));
+            memory_bytes = free_b / 10 * 7;
+        }
+        s->arena.init(memory_bytes);
+        YH_CHECK(hipMalloc(&s->partials, 512 * 4 * sizeof(double)));
+        YH_CHECK(hipMalloc(&s->counter, sizeof(unsigned)));
+        YH_CHECK(hipMemsetAsync(s->counter, 0, sizeof(unsigned), s->stream));
+        s->ensure_scal(64);
+        YH_CHECK(hipStreamSynchronize(s->stream));
+        *out = s.release();
+    });
+}
+
+int yasfhip_server_destroy(yasfhip_server *s) {
+    return sguard([&] { delete s; });
+}
+
+int yasfhip_server_submit(yasfhip_server *s, const double *positions, int particle_number, int lmax,
+                          double k_medium, const double *t, const double *a_inc, double rtol, int max_cycles,
+                          int observables, long long *request_id) {
+    return sguard([&] {
+        YH_REQUIRE(s && positions && t && a_inc && request_id, "NULL argument");
+        YH_REQUIRE(particle_number >= 1, "particle_number must be >= 1");
+        YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
+        YH_REQUIRE(std::isfinite(k_medium) && k_medium > 0.0, "k_medium must be real, finite and positive");
+        YH_REQUIRE(rtol > 0.0, "rtol must be > 0");
+        YH_REQUIRE(max_cycles >= 1, "max_cycles must be >= 1");
+        auto q = std::make_unique<Request>();
+        q->id = s->next_id++;
+        q->N = particle_number;
+        q->L = lmax;
+        q->nmax = 2 * lmax * (lmax + 2);
+        q->n = static_cast<long>(q->N) * q->nmax;
+        q->ntiles1 = (q->N + 15) / 16;
+        q->chunks = static_cast<int>((q->n + kChunk - 1) / kChunk);
+        q->k = k_medium;
+        q->rtol = rtol;
+        q->max_cycles = max_cycles;
+        q->observables = observables != 0;
+        q->pos.assign(q->ntiles1 * 16, make_double4(0, 0, 0, 0));
+        for (int i = 0; i < q->N; ++i)
+            q->pos[i] = make_double4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 0.0);
+        q->t.resize(q->n);
+        q->a.resize(q->n);
+        std::memcpy(q->t.data(), t, sizeof(double2) * q->n);
+        std::memcpy(q->a.data(), a_inc, sizeof(double2) * q->n);
+        q->t_submit = now_s();
+        *request_id = q->id;
+        s->queue.push_back(q.release());
+    });
+}
+
+int yasfhip_server_step(yasfhip_server *s, int *active, int *queued, long long *completed) {
+    return sguard([&] {
+        YH_REQUIRE(s, "NULL argument");
+        s->step();
+        if (active)
+            *active = static_cast<int>(s->active.size());
+        if (queued)
+            *queued = static_cast<int>(s->queue.size());
+        if (completed)
+            *completed = s->completed;
+    });
+}
+
+int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, int reps, int *waves,
+                            double *ms) {
+    return sguard([&] {
+        YH_REQUIRE(s && waves && ms, "NULL argument");
+        YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
+        YH_REQUIRE(particle_number >= 16, "particle_number must be >= 16");
+        YH_REQUIRE(reps >= 1, "reps must be >= 1");
+        *waves = s->autotune(lmax, particle_number, reps, ms);
+    });
+}
+
+int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves) {
+    return sguard([&] {
+        YH_REQUIRE(s, "NULL argument");
+        YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
+        YH_REQUIRE(yasfhip_server::waves_valid(lmax, waves) && (waves == 2 || waves == 4 || waves == 8),
+                   "waves must be 2, 4 or 8 with ceil((2 lmax + 1) / waves) orders per wave within range");
+        s->waves_for_lmax[lmax] = waves;
+    });
+}
+
+int yasfhip_server_poll(yasfhip_server *s, long long *request_id, double *x_out, long long x_capacity,
+                        int *info, int *cycles, int *iterations, double *residual, double *c_ext,
+                        double *c_sca, double *latency_s) {
+    return sguard([&] {
+        YH_REQUIRE(s && request_id, "NULL argument");
+        *request_id = -1;
+        if (s->finished.empty())
+            return;
+        std::unique_ptr<Request> q(s->finished.front());
+        s->finished.pop_front();
+        *request_id = q->id;
+        if (x_out && x_capacity >= q->n && !q->x_out.empty())
+            std::memcpy(x_out, q->x_out.data(), sizeof(double2) * q->n);
+        if (info)
+            *info = q->info;
+        if (cycles)
+            *cycles = q->cycles;
+        if (iterations)
+            *iterations = q->iterations;
+        if (residual)
+            *residual = q->bnorm > 0 ? q->rnorm / q->bnorm : 0.0;
+        if (c_ext)
+            *c_ext = q->c_ext;
+        if (c_sca)
+            *c_sca = q->c_sca;
+        if (latency_s)
+            *latency_s = q->t_done - q->t_submit;
+    });
+}
+
+}  // extern "C"

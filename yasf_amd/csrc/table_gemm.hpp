@@ -1,0 +1,113 @@
+// The streaming complex GEMM of one wave over a resident pair-factor table
+// (layout table[target tile][dm][K-step][order][lane], MFMA A operand), shared
+// by the single-session kernel (pairtable.hip) and the batched serving kernel
+// (server.hip).
+#pragma once
+
+#include "device_common.hpp"
+
+namespace yasfhip {
+
+// One wave's GEMM with NIP orders (ip = wave + WAVES j) and NTL column tiles,
+// all compile-time, so the 4*NIP*NTL MFMAs of a K-step form one straight-line
+// block the scheduler can interleave.  tab points at the wave's target tile
+// and dm (plus the lane); its C tile (ip, it) goes to Qt + slot ip*ntile + it.
+template <typename T, int NIP, int NTL, int WAVES>
+__device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__restrict__ tab,
+                                                const typename Cplx<T>::type *__restrict__ x, int N,
+                                                int N4, int nmax, int R, int wave, const int (&n2c)[2],
+                                                T *__restrict__ Qt, int ntile) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    typedef typename Cplx<T>::type T2;
+    const int lane = threadIdx.x & 63, lo = lane & 15, hi = lane >> 4;
+    acc_t acc[NIP][NTL][2];
+#pragma unroll
+    for (int j = 0; j < NIP; ++j)
+#pragma unroll
+        for (int it = 0; it < NTL; ++it) {
+            acc[j][it][0] = acc_t{0, 0, 0, 0};
+            acc[j][it][1] = acc_t{0, 0, 0, 0};
+        }
+    // The next K-step's operands are in flight during the current MFMAs.
+    T2 a0[NIP], b0[NTL];
+    auto load = [&](int ks, T2 (&an)[NIP], T2 (&bn)[NTL]) {
+        const bool kin = ks < N4;
+#pragma unroll
+        for (int j = 0; j < NIP; ++j) {
+            an[j].x = T(0);
+            an[j].y = T(0);
+            if (kin)
+                an[j] = tab[(static_cast<size_t>(ks) * R + wave + WAVES * j) * 64];
+        }
+        const int s2 = ks * 4 + hi;
+#pragma unroll
+        for (int it = 0; it < NTL; ++it) {
+            bn[it].x = T(0);
+            bn[it].y = T(0);
+            if (n2c[it] >= 0 && s2 < N)
+                bn[it] = x[static_cast<size_t>(s2) * nmax + n2c[it]];
+        }
+    };
+    load(0, a0, b0);
+    for (int ks = 0; ks < N4; ++ks) {
+        T2 a[NIP], b[NTL];
+#pragma unroll
+        for (int j = 0; j < NIP; ++j)
+            a[j] = a0[j];
+#pragma unroll
+        for (int it = 0; it < NTL; ++it)
+            b[it] = b0[it];
+        load(ks + 1, a0, b0);
+        // real parts first, then imaginary parts: consecutive MFMAs never
+        // update the same accumulator
+#pragma unroll
+        for (int j = 0; j < NIP; ++j)
+#pragma unroll
+            for (int it = 0; it < NTL; ++it) {
+                acc[j][it][0] = Mfma<T>::mac(a[j].x, b[it].x, acc[j][it][0]);
+                acc[j][it][1] = Mfma<T>::mac(a[j].x, b[it].y, acc[j][it][1]);
+            }
+#pragma unroll
+        for (int j = 0; j < NIP; ++j)
+#pragma unroll
+            for (int it = 0; it < NTL; ++it) {
+                acc[j][it][0] = Mfma<T>::mac(-a[j].y, b[it].y, acc[j][it][0]);
+                acc[j][it][1] = Mfma<T>::mac(a[j].y, b[it].x, acc[j][it][1]);
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < NIP; ++j)
+#pragma unroll
+        for (int it = 0; it < NTL; ++it) {
+            T *dst = Qt + 2 * ((static_cast<size_t>((wave + WAVES * j) * ntile + it) * 16 + lo) * 16);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = Mfma<T>::row(i, hi);
+                dst[2 * r] = acc[j][it][0][i];
+                dst[2 * r + 1] = acc[j][it][1][i];
+            }
+        }
+}
+
+// Dispatches a wave's (nip, ntile) to its compile-time instance; nip <= NIPMAX.
+template <typename T, int NIPMAX, int WAVES>
+__device__ __forceinline__ void table_gemm_dispatch(const typename Cplx<T>::type *__restrict__ tab,
+                                                    const typename Cplx<T>::type *__restrict__ x, int N,
+                                                    int N4, int nmax, int R, int wave, int nip, int ntile,
+                                                    const int (&n2c)[2], T *__restrict__ Qt) {
+    static_assert(kJobMaxColTiles == 2, "table GEMM dispatch assumes at most 2 column tiles per job");
+    switch (nip * 2 + ntile - 1) {
+#define YH_CASE(NIP, NTL)                                                                            \
+    case NIP * 2 + NTL - 1:                                                                          \
+        if constexpr (NIP <= NIPMAX)                                                                 \
+            table_gemm_wave<T, NIP, NTL, WAVES>(tab, x, N, N4, nmax, R, wave, n2c, Qt, ntile);       \
+        break;
+        YH_CASE(1, 1) YH_CASE(1, 2) YH_CASE(2, 1) YH_CASE(2, 2) YH_CASE(3, 1) YH_CASE(3, 2)
+        YH_CASE(4, 1) YH_CASE(4, 2) YH_CASE(5, 1) YH_CASE(5, 2)
+#undef YH_CASE
+    default:
+        break;
+    }
+}
+
+}  // namespace yasfhip

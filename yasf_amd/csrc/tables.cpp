@@ -220,4 +220,54 @@ CouplingPlan build_coupling_plan(int lmax) {
     return plan;
 }
 
+namespace {
+
+template <typename T>
+T *upload_vec(const std::vector<T> &v, hipStream_t s) {
+    void *d = nullptr;
+    YH_CHECK(hipMalloc(&d, std::max<size_t>(1, v.size()) * sizeof(T)));
+    if (!v.empty())
+        YH_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return static_cast<T *>(d);
+}
+
+}  // namespace
+
+DevicePlan upload_device_plan(const CouplingPlan &pl, hipStream_t s) {
+    DevicePlan dp{};
+    dp.lmax = pl.lmax;
+    dp.nmax = pl.nmax;
+    dp.maxcols = pl.maxcols;
+    dp.nq = pl.nq;
+    dp.njobs = static_cast<int>(pl.jobs.size());
+    dp.stage_elems = pl.stage_elems;
+    dp.jobs = upload_vec(pl.jobs, s);
+    dp.colmap = upload_vec(pl.colmap, s);
+    dp.crow = upload_vec(pl.crow, s);
+    dp.cidx = upload_vec(pl.cidx, s);
+    std::vector<double2> cval(pl.cval.size() / 2);
+    for (size_t i = 0; i < cval.size(); ++i)
+        cval[i] = make_double2(pl.cval[2 * i], pl.cval[2 * i + 1]);
+    dp.cval = upload_vec(cval, s);
+    dp.la = upload_vec(pl.leg_alpha, s);
+    dp.lb = upload_vec(pl.leg_beta, s);
+    dp.ld = upload_vec(pl.leg_diag, s);
+    const int L = pl.lmax;
+    std::vector<int> dmoff(4 * L + 1);
+    for (int d = 0, acc = 0; d < 4 * L + 1; ++d) {
+        dmoff[d] = acc;
+        acc += 2 * L + 1 - std::abs(d - 2 * L);
+    }
+    dp.dmoff = upload_vec(dmoff, s);
+    return dp;
+}
+
+void free_device_plan(DevicePlan &dp) {
+    void *bufs[] = {dp.jobs, dp.colmap, dp.crow, dp.cidx, dp.cval, dp.la, dp.lb, dp.ld, dp.dmoff};
+    for (void *b : bufs)
+        if (b)
+            (void)hipFree(b);
+    dp = DevicePlan{};
+}
+
 }  // namespace yasfhip
