@@ -1,21 +1,33 @@
-"""Benchmark: dense FP64 GMRES solves of a 2,000-sphere cluster on one MI355X.
+"""Benchmarks of the multiple-sphere T-matrix solve on one MI355X.
 
-Workload (seeded): 2,000 spheres, radii log-normal (median 100 nm, sigma_ln
-0.2), random sequential addition in a sphere at volume fraction 0.2, n = 1.5 +
-0.01i in vacuum, lambda = 550 nm, plane wave along +z (TE), lmax = 6 (96
-coefficients per sphere, 192,000 unknowns), complex128 matrix-free coupling,
-block-diagonal preconditioner with about 8 spheres per box, GMRES (restart
-100) to 1e-6 relative residual.
+Default workload (``--workload solve``, seeded): 2,000 spheres, radii
+log-normal (median 100 nm, sigma_ln 0.2), random sequential addition in a
+sphere at volume fraction 0.2, n = 1.5 + 0.01i in vacuum, lambda = 550 nm,
+plane wave along +z (TE), lmax = 6 (96 coefficients per sphere, 192,000
+unknowns), complex128 matrix-free coupling over a resident FP64 pair-factor
+table, block-diagonal preconditioner (about 8 spheres per box, boxes bisected
+above 12), GMRES (restart 100) to 1e-6 relative residual.
 
 One step = one solve: preconditioner build + LU on the device, then device
-GMRES from x0 = b.  Inputs (geometry, T-matrix, right-hand side) are resident
-on the device before timing.  Prints one JSON line.
+GMRES from x0 = b.  Geometry, T-matrix and pair table are resident on the
+device before timing; each solve uploads b (3 MB) and downloads x.  The same
+system is then solved with the mixed-precision path (FP64 iterative
+refinement around GMRES with the complex64 pair GEMM) and reported beside it.
+Prints one JSON line.
+
+``--workload serving``: 512 seeded solve requests of uneven size (N
+log-uniform in [20, 3000], lmax 3..8, lambda 300-1000 nm, random index and
+incidence) through the batched mixed-precision server, all queued at the start
+(or Poisson arrivals with ``--rate``), against the all-FP64 single-solve path
+one request at a time; prints one JSON line with solves/s and latency
+percentiles of both.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -26,7 +38,10 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 matrix (= vector)
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
+METRIC = "solves/s + p50/p99 latency at 1e-6 rel. residual; matvec TFLOP/s vs gfx950 peak"
+TRAFFIC_FILE = ROOT / "profiles" / "pair_table_gemm_fp64_2000spheres_lmax6_traffic.json"
 
 
 def workload(seed=2, N=2000, lmax=6, lam=550.0, vf=0.2):
@@ -82,38 +97,42 @@ def cpu_baseline(pos, k, lmax, x, sample_particles, threads):
     return dt * N / sample_particles, used
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-precond", action="store_true")
-    ap.add_argument("--per-box", type=float, default=8.0)
-    ap.add_argument("--box-cap", type=int, default=12, help="bisect boxes above this many spheres")
-    ap.add_argument("--restart", type=int, default=100)
-    ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=48, help="target particles in the CPU sample")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: every run here uses a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
-        sys.exit(2)
-
+def pmc_calibration():
+    """Known-size kernels for the HBM-traffic counters: a 1 GiB fill (writes
+    1 GiB) and a 1 GiB device copy (reads and writes 1 GiB)."""
     import torch
 
-    if not torch.cuda.is_available():
-        print("bench.py needs a GPU", file=sys.stderr)
-        sys.exit(1)
-    torch.cuda.set_device(0)
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def traffic_record():
+    """HBM bytes per launch of the FP64 table GEMM on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (null when absent)."""
+    try:
+        rec = json.loads(TRAFFIC_FILE.read_text())
+        return float(rec["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def run_solve(args):
+    import torch
 
     from yasf_amd import _native
     from yasf_amd.preconditioner import (MAX_BLOCK_ROWS, edge_size_for, make_particle_partition,
                                          split_oversized)
 
+    if args.pmc_calibration:
+        pmc_calibration()
     lmax, N = 6, 2000
     pos, radii, k, t, b = workload()
     s = _native.Session(pos, lmax, [k])
     s.set_tmatrix(t)
+    s.set_matvec_mode(_native.MATVEC_TABLE)
     use_pc = not args.no_precond
     boxes = None
     if use_pc:
@@ -128,8 +147,15 @@ def main():
             s.precond_build(boxes)
         return s.gmres(b, b, rtol=1e-6, restart=args.restart, maxiter=1000, use_precond=use_pc)
 
+    def step_mixed():
+        if use_pc:
+            s.precond_build(boxes)
+        return s.gmres_refined(b, b, rtol=1e-6, inner_rtol=1e-4, restart=args.restart,
+                               inner_maxiter=1000, max_refinements=10, use_precond=use_pc)
+
     for _ in range(args.warmup):
         step()
+        step_mixed()
         print("warmup step done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     lat = []
@@ -145,8 +171,22 @@ def main():
     total = time.perf_counter() - t_start
     x, info, iters, res, _ = results[-1]
 
-    # Matvec kernel timing with HIP events on the session stream.
+    # mixed precision on the same system, same preconditioner
+    lat_m = []
+    mixed = None
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        mixed = step_mixed()
+        lat_m.append(time.perf_counter() - t0)
+    torch.cuda.synchronize()
+    xm, info_m, steps_m, iters_m, res_m, _ = mixed
+    a_inc = b / t
+    c64 = s.cross_sections(a_inc, x)
+    cm = s.cross_sections(a_inc, xm)
+
+    # Pair-kernel timing with HIP events on the session stream.
     pair_ms, mv_ms = s.time_matvec(reps=5)
+    pair32_ms, mv32_ms = s.time_matvec(precision=_native.FP32, reps=5)
     pairs = N * (N - 1)
     f_fact = factorised_flops_per_pair(lmax) * pairs
     f_direct = direct_flops_per_pair(lmax) * pairs
@@ -154,9 +194,10 @@ def main():
     ntiles = (N + 15) // 16
     f_issued = macs_issued * 8 * (ntiles * 16) * N  # MFMA flops incl. tile padding
     achieved = f_fact / (pair_ms * 1e-3) / 1e12
+    achieved32 = f_fact / (pair32_ms * 1e-3) / 1e12
 
     line = {
-        "metric": "solves/s + p50/p99 latency at 1e-6 rel. residual; matvec TFLOP/s vs gfx950 peak",
+        "metric": METRIC,
         "value": args.steps / total,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -172,7 +213,8 @@ def main():
                                "GMRES(100) to 1e-6" + (", block-diag precond ~8/box" if use_pc else ""),
                    "particles": N, "lmax": lmax, "unknowns": N * 96,
                    "preconditioner": use_pc, "restart": args.restart,
-                   "box_cap": args.box_cap if use_pc else None},
+                   "box_cap": args.box_cap if use_pc else None,
+                   "matvec": "resident FP64 pair-factor table + ab5 contraction"},
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "gmres_iterations": iters,
@@ -186,11 +228,25 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
-            "kernel": "pair_kernel<double> (FP64 MFMA 16x16x4)",
+            "traffic": traffic_record(),
+            "kernel": "pair_table_gemm_kernel<double> (FP64 MFMA 16x16x4 over the resident pair table)",
             "flops_per_launch": f_fact,
             "issued_mfma_tflops": f_issued / (pair_ms * 1e-3) / 1e12,
             "direct_count_tflops_per_matvec": f_direct / (mv_ms * 1e-3) / 1e12,
+        },
+        "mixed_precision": {
+            "solves_per_s": args.steps / sum(lat_m),
+            "p50_latency_s": float(np.percentile(lat_m, 50)),
+            "speedup_vs_fp64": float(np.percentile(lat, 50) / np.percentile(lat_m, 50)),
+            "refinements": steps_m,
+            "inner_iterations": iters_m,
+            "relative_residual_fp64": res_m,
+            "info": info_m,
+            "c_ext_rel_diff_vs_fp64": abs(cm[0] - c64[0]) / abs(c64[0]),
+            "c_sca_rel_diff_vs_fp64": abs(cm[1] - c64[1]) / abs(c64[1]),
+            "pair_kernel_ms_fp32": pair32_ms,
+            "pair_kernel_tflops_fp32": achieved32,
+            "pair_kernel_frac_fp32_peak": achieved32 / PEAK_FP32_TFLOPS,
         },
     }
     if not args.skip_cpu:
@@ -207,6 +263,107 @@ def main():
                       f"{N} sources, scaled to the full matvec ({sec:.2f} s/matvec)",
         }
     print(json.dumps(line), flush=True)
+
+
+def run_serving(args):
+    import torch
+
+    from yasf_amd import _native
+    from yasf_amd.serving import make_workload, serve, solve_sequential
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    reqs = make_workload(args.requests, seed=4, rate=args.rate)
+    log(f"workload: {len(reqs)} requests prepared on the host in {time.perf_counter() - t0:.1f} s")
+    srv = _native.Server(restart=50, inner_rtol=1e-4)
+    tiles = {}
+    for L in range(3, 9):
+        w, ms = srv.autotune(L, particles=600, reps=3)
+        tiles[L] = {"waves": w, "ms": [round(float(v), 4) for v in ms]}
+    log(f"autotuned tile shapes: {tiles}")
+    if args.warmup:
+        serve(srv, make_workload(8, seed=99, n_range=(20, 200)))
+    torch.cuda.synchronize()
+    rep = serve(srv, reqs, progress=log)
+    log(f"served: {rep.as_dict()}")
+    base = None
+    if not args.no_baseline:
+        base = solve_sequential(reqs, progress=log)
+        log(f"sequential FP64: {base.as_dict()}")
+    infos = [r["info"] for r in rep.results.values()]
+    line = {
+        "metric": METRIC,
+        "value": rep.solves_per_s,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * rep.makespan_s / len(reqs),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 pair GEMM inside FP64 refinement (fp32/fp64)",
+        "data": "synthetic (seeded request stream, seed 4)",
+        "config": {"workload": "serving: 512 requests, N log-uniform [20, 3000], lmax 3..8, "
+                               "lambda 300-1000 nm, random index and incidence, rtol 1e-6",
+                   "requests": len(reqs),
+                   "arrival_rate_per_s": args.rate if math.isfinite(args.rate) else None,
+                   "arrivals": "poisson" if math.isfinite(args.rate) else "all queued at start (closed)",
+                   "restart": 50, "inner_rtol": 1e-4, "tile_shapes": tiles},
+        "p50_latency_s": rep.p50_s,
+        "p99_latency_s": rep.p99_s,
+        "makespan_s": rep.makespan_s,
+        "converged": int(sum(1 for i in infos if i == 0)),
+        "mean_inner_iterations": float(np.mean([r["iterations"] for r in rep.results.values()])),
+    }
+    if base is not None:
+        d_ext = max(abs(rep.results[i]["c_ext"] - base.results[i]["c_ext"]) / abs(base.results[i]["c_ext"])
+                    for i in rep.results)
+        d_sca = max(abs(rep.results[i]["c_sca"] - base.results[i]["c_sca"]) / abs(base.results[i]["c_sca"])
+                    for i in rep.results)
+        line["fp64_single_solve"] = {"solves_per_s": base.solves_per_s, "p50_latency_s": base.p50_s,
+                                     "p99_latency_s": base.p99_s, "makespan_s": base.makespan_s}
+        line["speedup_vs_fp64_single_solve"] = rep.solves_per_s / base.solves_per_s
+        line["max_rel_diff_c_ext"] = d_ext
+        line["max_rel_diff_c_sca"] = d_sca
+    print(json.dumps(line), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["solve", "serving"], default="solve")
+    ap.add_argument("--no-precond", action="store_true")
+    ap.add_argument("--per-box", type=float, default=8.0)
+    ap.add_argument("--box-cap", type=int, default=12, help="bisect boxes above this many spheres")
+    ap.add_argument("--restart", type=int, default=100)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=48, help="target particles in the CPU sample")
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="run a 1 GiB fill and copy first (reference kernels for rocprofv3 --pmc)")
+    ap.add_argument("--requests", type=int, default=512, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=math.inf, help="serving: Poisson arrivals per second")
+    ap.add_argument("--no-baseline", action="store_true", help="serving: skip the sequential FP64 run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every run here uses a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    if args.workload == "serving":
+        run_serving(args)
+    else:
+        run_solve(args)
 
 
 if __name__ == "__main__":
