@@ -154,6 +154,17 @@ int yasfhip_precond_apply(yasfhip_session *s, const double *r, double *z);
  * yasfpy/lookups.py:124-171). */
 int yasfhip_set_matvec_mode(yasfhip_session *s, int mode);
 
+/* Tile-shape autotuner of the resident-table GEMM for one precision: times
+ * `reps` launches with each column-tile grouping (jobs of at most 2, 4 or 8
+ * 16-column tiles per dm -- 8 in FP32 only; wider groups stream each table
+ * slice fewer times but hold more accumulators), keeps the fastest for this
+ * session and returns its index in *grouping; ms[3] receives the mean times
+ * (-1 for groupings not offered; all -1 when the session keeps no resident
+ * table, in which case nothing changes).  The table is
+ * built first if needed.  Results do not depend on the grouping. */
+int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps, int *grouping,
+                        double *ms);
+
 /* Timing of `reps` device-resident matvecs of one channel on the session
  * stream with HIP events: ms_pair = mean duration of the pair kernel,
  * ms_total = mean duration of the whole matvec (pair + contraction). */
@@ -206,16 +217,18 @@ int yasfhip_server_poll(yasfhip_server *s, long long *request_id, double *x_out,
                         int *info, int *cycles, int *iterations, double *residual, double *c_ext,
                         double *c_sca, double *latency_s);
 
-/* Tile-shape autotuner of the batched complex64 pair GEMM: times 2, 4 and 8
- * waves per (job, target tile) block (each where ceil((2 lmax + 1)/waves)
- * orders per wave are supported) on a synthetic cluster of particle_number
- * spheres, keeps the fastest for lmax and returns it; ms[3] receives the mean
- * times (-1 for shapes not valid at this lmax).  Needs an idle server. */
+/* Tile-shape autotuner of the batched complex64 pair GEMM.  A shape is
+ * (waves per (job, target tile) block, column tiles per job); the candidates
+ * are, in this order, (2, 2), (4, 2), (8, 2), (4, 4), (8, 4), (8, 8), each
+ * usable where ceil((2 lmax + 1) / waves) orders per wave stay within 5 (2 or
+ * 4 waves) or 3 (8 waves).  Times every usable shape on a synthetic cluster of
+ * particle_number spheres, keeps the fastest for lmax and returns it; ms[6]
+ * receives the mean times (-1 where not usable).  Needs an idle server. */
 int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, int reps, int *waves,
-                            double *ms);
+                            int *column_tiles, double *ms);
 
-/* Fixes the tile shape (waves per block: 2, 4 or 8) used for lmax. */
-int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves);
+/* Fixes the tile shape used for lmax (one of the candidates above). */
+int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int column_tiles);
 
 #ifdef __cplusplus
 }

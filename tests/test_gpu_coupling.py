@@ -118,3 +118,20 @@ def test_master_and_multichannel():
     for w in range(2):
         x = rand_vec(rng, n)
         assert rel(s.master_matvec(x, channel=w), R.master_matvec(L, pos, k[w], t[w], x)) < 1e-12
+
+
+@pytest.mark.parametrize("lmax", [3, 6, 8])
+def test_table_groupings_agree(lmax):
+    """The table-GEMM tuner times every offered column-tile grouping (2/4 tiles
+    in FP64, 2/4/8 in FP32) and the matvec with its choice keeps oracle parity."""
+    pos, _ = cluster(20 + lmax, 45, 0.1)
+    k = 2 * np.pi / 500.0
+    x = rand_vec(np.random.default_rng(lmax), 45 * 2 * lmax * (lmax + 2))
+    s = _native.Session(pos, lmax, [k])
+    s.set_matvec_mode(_native.MATVEC_TABLE)
+    for prec, offered in ((_native.FP64, (0, 1)), (_native.FP32, (0, 1, 2))):
+        g, ms = s.tune_matvec(precision=prec, reps=1)
+        assert g in offered and all((ms[i] > 0) == (i in offered) for i in range(3))
+        y = s.coupling_matvec(x, precision=prec)
+        oracle_y = R.coupling_matvec(lmax, pos, k, x)
+        assert rel(y, oracle_y) < (1e-12 if prec == _native.FP64 else 2e-5)

@@ -86,23 +86,28 @@ def test_sequential_reference_agrees(mixed_requests, served):
 
 def test_autotune_and_tile_shapes(mixed_requests):
     srv = _native.Server(memory_bytes=2 << 30)
-    for L, valid in ((3, (2, 4, 8)), (5, (4, 8)), (8, (4, 8))):
-        w, ms = srv.autotune(L, particles=96, reps=2)
-        assert w in valid
-        for i, cand in enumerate((2, 4, 8)):
-            assert (ms[i] > 0) == (cand in valid)
-        assert ms[(2, 4, 8).index(w)] == pytest.approx(ms[ms > 0].min())
+    shapes = _native.Server.TILE_SHAPES
+    for L in (3, 5, 8):
+        usable = [(w, ct) for w, ct in shapes if -(-(2 * L + 1) // w) <= (3 if w == 8 else 5)]
+        best, ms = srv.autotune(L, particles=96, reps=2)
+        assert best in usable
+        for i, shape in enumerate(shapes):
+            assert (ms[i] > 0) == (shape in usable)
+        assert ms[shapes.index(best)] == pytest.approx(ms[ms > 0].min())
     with pytest.raises(ValueError):
-        srv.set_tile_shape(8, 2)   # 17 orders over 2 waves exceeds the kernel's range
-    # every valid fixed shape gives the same solution bit for bit
+        srv.set_tile_shape(8, 2, 2)   # 17 orders over 2 waves exceeds the kernel's range
+    with pytest.raises(ValueError):
+        srv.set_tile_shape(4, 2, 8)   # not an instantiated shape
+    # every usable fixed shape gives the same solution bit for bit
     reqs = mixed_requests[:3]
     outs = []
-    for waves in (4, 8):
+    for waves, ct in ((4, 2), (8, 2), (4, 4), (8, 4), (8, 8)):
         for L in range(3, 9):
-            srv.set_tile_shape(L, waves)
+            srv.set_tile_shape(L, waves, ct)
         outs.append(serve(srv, reqs, keep_solutions=True))
     for r in reqs:
-        assert np.array_equal(outs[0].results[r.index]["x"], outs[1].results[r.index]["x"])
+        for o in outs[1:]:
+            assert np.array_equal(outs[0].results[r.index]["x"], o.results[r.index]["x"])
     srv.close()
 
 

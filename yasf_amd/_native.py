@@ -70,9 +70,10 @@ def _load():
             "yasfhip_server_submit": ([P, P, i, i, d, P, P, d, i, i, llp], i),
             "yasfhip_server_step": ([P, ip, ip, llp], i),
             "yasfhip_server_poll": ([P, llp, P, ctypes.c_longlong, ip, ip, ip, dp, dp, dp, dp], i),
-            "yasfhip_server_autotune": ([P, i, i, i, ip, P], i),
-            "yasfhip_server_set_tile_shape": ([P, i, i], i),
+            "yasfhip_server_autotune": ([P, i, i, i, ip, ip, P], i),
+            "yasfhip_server_set_tile_shape": ([P, i, i, i], i),
             "yasfhip_set_matvec_mode": ([P, i], i),
+            "yasfhip_tune_matvec": ([P, i, i, i, ip, P], i),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -93,7 +94,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_destroy", "yasfhip_set_tmatrix", "yasfhip_coupling_matvec", "yasfhip_master_matvec",
         "yasfhip_gmres_solve", "yasfhip_gmres_solve_refined", "yasfhip_cross_sections",
         "yasfhip_far_field", "yasfhip_precond_build", "yasfhip_precond_apply",
-        "yasfhip_time_matvec", "yasfhip_set_matvec_mode", "yasfhip_server_create",
+        "yasfhip_time_matvec", "yasfhip_set_matvec_mode", "yasfhip_tune_matvec", "yasfhip_server_create",
         "yasfhip_server_destroy", "yasfhip_server_submit", "yasfhip_server_step", "yasfhip_server_poll",
         "yasfhip_server_autotune", "yasfhip_server_set_tile_shape",
     ]
@@ -269,6 +270,14 @@ class Session:
         """MATVEC_AUTO, MATVEC_ON_THE_FLY or MATVEC_TABLE (resident pair-factor table)."""
         _check(self._lib.yasfhip_set_matvec_mode(self._h, int(mode)))
 
+    def tune_matvec(self, channel: int = 0, precision: int = FP64, reps: int = 3):
+        """Picks the table GEMM's column-tile grouping; returns (index, ms per grouping 2/4/8)."""
+        g = ctypes.c_int()
+        ms = np.zeros(3)
+        _check(self._lib.yasfhip_tune_matvec(self._h, int(channel), int(precision), int(reps),
+                                             ctypes.byref(g), _ptr(ms)))
+        return g.value, ms
+
     def time_matvec(self, channel: int = 0, precision: int = FP64, reps: int = 5) -> tuple[float, float]:
         """Mean (pair-kernel ms, whole-matvec ms) over `reps` device-resident matvecs."""
         a = ctypes.c_double()
@@ -350,13 +359,16 @@ class Server:
                 "cycles": cycles.value, "iterations": iters.value, "residual": res.value,
                 "c_ext": ce.value, "c_sca": cs.value, "latency_s": lat.value}
 
-    def autotune(self, lmax: int, particles: int = 600, reps: int = 5):
-        """(chosen waves per block, ms for 2/4/8 waves; -1 where not valid)."""
-        w = ctypes.c_int()
-        ms = np.zeros(3)
-        _check(self._lib.yasfhip_server_autotune(self._h, int(lmax), int(particles), int(reps),
-                                                 ctypes.byref(w), _ptr(ms)))
-        return w.value, ms
+    TILE_SHAPES = ((2, 2), (4, 2), (8, 2), (4, 4), (8, 4), (8, 8))
 
-    def set_tile_shape(self, lmax: int, waves: int):
-        _check(self._lib.yasfhip_server_set_tile_shape(self._h, int(lmax), int(waves)))
+    def autotune(self, lmax: int, particles: int = 600, reps: int = 5):
+        """((waves, column tiles) chosen, ms per candidate of TILE_SHAPES; -1 where not usable)."""
+        w = ctypes.c_int()
+        ct = ctypes.c_int()
+        ms = np.zeros(len(self.TILE_SHAPES))
+        _check(self._lib.yasfhip_server_autotune(self._h, int(lmax), int(particles), int(reps),
+                                                 ctypes.byref(w), ctypes.byref(ct), _ptr(ms)))
+        return (w.value, ct.value), ms
+
+    def set_tile_shape(self, lmax: int, waves: int, column_tiles: int = 2):
+        _check(self._lib.yasfhip_server_set_tile_shape(self._h, int(lmax), int(waves), int(column_tiles)))

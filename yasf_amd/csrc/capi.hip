@@ -123,6 +123,7 @@ struct yasfhip_session {
     void *table[2] = {nullptr, nullptr};
     int table_channel[2] = {-1, -1};
     bool table_refused[2] = {false, false};
+    int table_grouping[2] = {0, 0};  // column-tile grouping of the table GEMM per precision
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 
     ~yasfhip_session() {
@@ -180,7 +181,8 @@ struct yasfhip_session {
 
     void pair_gemm(int channel, const void *xv, bool fp32) {
         if (table_for(channel, fp32))
-            launch_pair_table_gemm(dp, N, table[fp32 ? 1 : 0], xv, fp32, Q, stream);
+            launch_pair_table_gemm(dp, N, table[fp32 ? 1 : 0], xv, fp32, table_grouping[fp32 ? 1 : 0], Q,
+                                   stream);
         else
             launch_pair(dp, pos, N, k[channel], xv, fp32, Q, stream);
     }
@@ -774,6 +776,49 @@ int yasfhip_set_matvec_mode(yasfhip_session *s, int mode) {
                     s->table_channel[ix] = -1;
                 }
         s->table_refused[0] = s->table_refused[1] = false;
+    });
+}
+
+int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps, int *grouping,
+                        double *ms) {
+    return guarded([&] {
+        YH_REQUIRE(s && grouping && ms, "NULL argument");
+        YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
+        YH_REQUIRE(precision == YASFHIP_FP64 || precision == YASFHIP_FP32, "unknown precision");
+        YH_REQUIRE(reps >= 1, "reps must be >= 1");
+        const bool fp32 = precision == YASFHIP_FP32;
+        const int ix = fp32 ? 1 : 0;
+        for (int g = 0; g < kTableGroupings; ++g)
+            ms[g] = -1.0;
+        *grouping = s->table_grouping[ix];
+        hipStream_t st = s->stream;
+        if (!s->table_for(channel, fp32))
+            return;  // no resident table for this session: nothing to tune
+        const void *xv = s->xin;
+        if (fp32) {
+            launch_to_fp32(reinterpret_cast<float *>(s->x32), reinterpret_cast<const double *>(s->xin),
+                           2 * s->n, st);
+            xv = s->x32;
+        }
+        double best = 1e300;
+        for (int g = 0; g < kTableGroupings; ++g) {
+            if (!fp32 && kTableGrouping[g] > 4)
+                continue;  // not offered in FP64 (register capacity)
+            launch_pair_table_gemm(s->dp, s->N, s->table[ix], xv, fp32, g, s->Q, st);  // warm-up
+            YH_CHECK(hipEventRecord(s->ev[0], st));
+            for (int r = 0; r < reps; ++r)
+                launch_pair_table_gemm(s->dp, s->N, s->table[ix], xv, fp32, g, s->Q, st);
+            YH_CHECK(hipEventRecord(s->ev[1], st));
+            YH_CHECK(hipEventSynchronize(s->ev[1]));
+            float t = 0.f;
+            YH_CHECK(hipEventElapsedTime(&t, s->ev[0], s->ev[1]));
+            ms[g] = t / reps;
+            if (ms[g] < best) {
+                best = ms[g];
+                *grouping = g;
+            }
+        }
+        s->table_grouping[ix] = *grouping;
     });
 }
 

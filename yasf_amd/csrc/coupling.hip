@@ -29,7 +29,8 @@ namespace yasfhip {
 //              on chunk i from buffer i&1.
 // One barrier per chunk hands the buffers over, so generation runs on the
 // VALU while the matrix cores work.
-// Q layout: [target tile][q][col 16][row 16] complex T (interleaved).
+// Q layout: [target tile][slot][col 16][row 16] complex T (interleaved); C tile
+// (ip, column tile it) of a dm goes to slot qbase + ip * tstride + it.
 // Regular: the J-translation variant (j_p in place of h_p, coincident pairs
 // included) used by the scattering cross section.
 template <typename T, bool Regular>
@@ -55,6 +56,7 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
     const int tile0 = uniform(jobs[jid].tile0);
     const int ntile = uniform(jobs[jid].ntile);
     const int qbase = uniform(jobs[jid].qbase);
+    const int tstride = uniform(jobs[jid].tstride);
     const int P = 2 * lmax + 1;
     const int R = P - (dm < 0 ? -dm : dm);
     const int ncols = ntile * 16;
@@ -170,7 +172,9 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
     for (int j = 0; j < kWaveMaxTiles; ++j) {
         const int c = wave + kWavesPerJob * j;
         if (c < total) {
-            T *dst = Q + 2 * (((static_cast<size_t>(s1t) * nq + qbase + c) * 16 + lo) * 16);
+            const int ip = c / ntile, it = c - ip * ntile;
+            const size_t slot = static_cast<size_t>(s1t) * nq + qbase + ip * tstride + tile0 + it;
+            T *dst = Q + 2 * ((slot * 16 + lo) * 16);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = Mfma<T>::row(i, hi);

@@ -59,10 +59,21 @@ std::vector<cplx> translation_table(int lmax);
 // One workgroup's share of the pair GEMM for one 16-particle target tile:
 // a signed dm, all orders p in [|dm|, 2L] (R = 2L+1-|dm| of them) and the
 // column tiles [tile0, tile0+ntile) of the dm's compatible source columns.
-// Its R*ntile C tiles, flattened as c = ip*ntile + it, go to Q slots qbase + c.
+// Q slots are laid out per dm independently of how its column tiles are
+// grouped into jobs: C tile (ip, column tile it) of the dm goes to slot
+// qbase + ip * tstride + it, qbase being the dm's first slot and tstride its
+// number of column tiles.
 struct PairJob {
-    int dm, tile0, ntile, qbase, cost, pad0, pad1, pad2;
+    int dm, tile0, ntile, qbase, cost, tstride, pad1, pad2;
 };
+
+// Column-tile groupings of the resident-table GEMM's job lists: a dm's column
+// tiles are split evenly into jobs of at most kTableGrouping[g] tiles.  Wider
+// groups stream each table slice from HBM fewer times but hold more
+// accumulators per wave.
+constexpr int kTableGroupings = 3;
+constexpr int kTableGrouping[kTableGroupings] = {2, 4, 8};
+constexpr int kTableMaxColTiles = 8;
 
 constexpr int kWavesPerJob = 4;
 constexpr int kWaveMaxTiles = 8;                                // C tiles per wave
@@ -74,6 +85,7 @@ struct CouplingPlan {
     int lmax = 0, nmax = 0, ndm = 0, maxcols = 0, nq = 0;
     std::vector<int> colmap;          // [ndm][maxcols] -> source mode n2 or -1
     std::vector<PairJob> jobs;        // per 16-particle target tile, sorted by cost
+    std::vector<PairJob> table_jobs[kTableGroupings];  // resident-table GEMM jobs, sorted by cost
     int stage_elems = 0;              // complex elements of one pair-kernel LDS stage
     std::vector<int> crow;            // CSR row offsets per target mode n1 (nmax+1)
     std::vector<int> cidx;            // Q column index q*16 + col
@@ -90,6 +102,8 @@ CouplingPlan build_coupling_plan(int lmax);
 struct DevicePlan {
     int lmax, nmax, maxcols, nq, njobs, stage_elems;
     PairJob *jobs;
+    PairJob *tjobs[kTableGroupings];
+    int ntjobs[kTableGroupings];
     int *colmap, *crow, *cidx;
     double2 *cval;
     double *la, *lb, *ld;
@@ -117,7 +131,8 @@ void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, 
 size_t pair_table_bytes(const DevicePlan &dp, int N, bool fp32);
 void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,
                              void *table, hipStream_t s);
+// grouping: index into kTableGrouping (column tiles per job).
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
-                            void *Q, hipStream_t s);
+                            int grouping, void *Q, hipStream_t s);
 
 }  // namespace yasfhip

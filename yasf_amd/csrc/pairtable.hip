@@ -47,10 +47,9 @@ __global__ __launch_bounds__(256) void pair_table_build_kernel(
 constexpr int kTableWaves = 8;  // waves per (job, target tile) block
 
 // One block of 8 waves per (job, target tile).  Wave w owns the orders
-// ip = w, w+8, ... (< R) of the job and all of its column tiles; its C tile
-// (ip, it) goes to Q slot qbase + ip * ntile + it (the same slots as the
-// on-the-fly kernel, so the contraction lists are shared).
-template <typename T, int NIPMAX>
+// ip = w, w+8, ... (< R) of the job and all of its (up to NTLMAX) column
+// tiles, so each table fragment it loads feeds all of the job's columns.
+template <typename T, int NIPMAX, int NTLMAX>
 __global__ __launch_bounds__(512) void pair_table_gemm_kernel(
     const typename Cplx<T>::type *__restrict__ table, const typename Cplx<T>::type *__restrict__ x, int N,
     int N4, int nmax, const PairJob *__restrict__ jobs, int ntiles1, const int *__restrict__ colmap,
@@ -63,6 +62,7 @@ __global__ __launch_bounds__(512) void pair_table_gemm_kernel(
     const int tile0 = uniform(jobs[jid].tile0);
     const int ntile = uniform(jobs[jid].ntile);
     const int qbase = uniform(jobs[jid].qbase);
+    const int tstride = uniform(jobs[jid].tstride);
     const int P = 2 * lmax + 1;
     const int R = P - (dm < 0 ? -dm : dm);
     const int d = dm + 2 * lmax;
@@ -71,31 +71,53 @@ __global__ __launch_bounds__(512) void pair_table_gemm_kernel(
     if (nip == 0)
         return;
     const T2 *tab = table + (static_cast<size_t>(t) * P * P + dmoff[d]) * N4 * 64 + lane;
-    int n2c[2];
+    int n2c[kTableMaxColTiles];
 #pragma unroll
-    for (int it = 0; it < 2; ++it)
+    for (int it = 0; it < kTableMaxColTiles; ++it)
         n2c[it] = (it < ntile) ? colmap[d * maxcols + (tile0 + it) * 16 + lo] : -1;
     T *Qt = Q + 2 * (static_cast<size_t>(t) * nq + qbase) * 256;
-    table_gemm_dispatch<T, NIPMAX, kTableWaves>(tab, x, N, N4, nmax, R, wave, nip, ntile, n2c, Qt);
+    table_gemm_dispatch<T, 1, NIPMAX, NTLMAX, kTableWaves>(tab, x, N, N4, nmax, R, wave, nip, ntile, n2c, Qt,
+                                                           tstride, tile0);
 }
 
-template <typename T>
-void gemm_t(const DevicePlan &dp, int N, const void *table, const void *x, void *Q, hipStream_t s) {
+template <typename T, int NTLMAX>
+void gemm_g(const DevicePlan &dp, int g, int N, const void *table, const void *x, void *Q, hipStream_t s) {
     typedef typename Cplx<T>::type T2;
     const int ntiles1 = (N + 15) / 16;
     const int N4 = (N + 3) / 4;
-    const dim3 grid(dp.njobs * ntiles1), block(64 * kTableWaves);
+    const dim3 grid(dp.ntjobs[g] * ntiles1), block(64 * kTableWaves);
     const T2 *tb = static_cast<const T2 *>(table);
     const T2 *xv = static_cast<const T2 *>(x);
     T *q = static_cast<T *>(Q);
     // orders per wave: ceil(R/8) <= ceil((2L+1)/8), 3 only at lmax 8
     if (2 * dp.lmax + 1 <= 16)
-        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 2>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
-                           dp.jobs, ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
+        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 2, NTLMAX>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
+                           dp.tjobs[g], ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
     else
-        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 3>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
-                           dp.jobs, ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
+        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 3, NTLMAX>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
+                           dp.tjobs[g], ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
     YH_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void gemm_t(const DevicePlan &dp, int g, int N, const void *table, const void *x, void *Q, hipStream_t s) {
+    static_assert(kTableGrouping[0] == 2 && kTableGrouping[1] == 4 && kTableGrouping[2] == 8,
+                  "instances follow the column-tile groupings");
+    switch (g) {
+    case 0:
+        gemm_g<T, 2>(dp, g, N, table, x, Q, s);
+        break;
+    case 1:
+        gemm_g<T, 4>(dp, g, N, table, x, Q, s);
+        break;
+    default:
+        // complex128 accumulators of 8 column tiles do not fit a wave's registers
+        if constexpr (sizeof(T) == sizeof(float))
+            gemm_g<T, 8>(dp, 2, N, table, x, Q, s);
+        else
+            YH_REQUIRE(false, "the FP64 table GEMM supports jobs of at most 4 column tiles");
+        break;
+    }
 }
 
 }  // namespace
@@ -121,11 +143,12 @@ void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, do
 }
 
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
-                            void *Q, hipStream_t s) {
+                            int grouping, void *Q, hipStream_t s) {
+    YH_REQUIRE(grouping >= 0 && grouping < kTableGroupings, "unknown column-tile grouping");
     if (fp32)
-        gemm_t<float>(dp, N, table, x, Q, s);
+        gemm_t<float>(dp, grouping, N, table, x, Q, s);
     else
-        gemm_t<double>(dp, N, table, x, Q, s);
+        gemm_t<double>(dp, grouping, N, table, x, Q, s);
 }
 
 }  // namespace yasfhip

@@ -57,7 +57,16 @@ constexpr int kChunk = 2048;        // vector elements per block of the Arnoldi 
 constexpr int kVecThreads = 256;
 constexpr int kPerThread = kChunk / kVecThreads;
 constexpr int kMaxRestart = 128;
-constexpr int kMaxGemmGroups = 3;   // tile shapes: 2, 4 or 8 waves per block
+// Tile shapes of the batched complex64 pair GEMM: waves per (job, target
+// tile) block and the column-tile grouping of the job list (index into
+// kTableGrouping).  Orders per wave, ceil((2L+1)/waves), must not exceed
+// kShapeNipmax (5 with 2 or 4 waves, 3 with 8).
+struct TileShape {
+    int waves, grouping;
+};
+constexpr int kShapes = 6;
+constexpr TileShape kShape[kShapes] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}, {8, 2}};
+constexpr int kMaxGemmGroups = kShapes;
 
 enum : int { kIdle = 0, kArnoldi = 1, kCycleDone = 2 };
 
@@ -140,8 +149,9 @@ __global__ __launch_bounds__(kVecThreads) void serve_prep_kernel(const BatchReq 
 }
 
 // Complex64 pair GEMM over the resident tables of a group of requests that
-// share one tile shape (WAVES waves per (job, target tile) block).
-template <int WAVES, int NIPMAX>
+// share one tile shape (WAVES waves per (job, target tile) block, jobs of at
+// most NTLMAX column tiles).
+template <int WAVES, int NIPMAX, int NTLMAX>
 __global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *__restrict__ reqs,
                                                                 const ReqState *__restrict__ st,
                                                                 const int *__restrict__ gstart,
@@ -160,6 +170,7 @@ __global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *
     const int tile0 = uniform(q.jobs[jid].tile0);
     const int ntile = uniform(q.jobs[jid].ntile);
     const int qbase = uniform(q.jobs[jid].qbase);
+    const int tstride = uniform(q.jobs[jid].tstride);
     const int lmax = q.lmax;
     const int P = 2 * lmax + 1;
     const int R = P - (dm < 0 ? -dm : dm);
@@ -168,12 +179,13 @@ __global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *
     if (nip == 0)
         return;
     const float2 *tab = q.table + (static_cast<size_t>(t) * P * P + q.dmoff[d]) * q.N4 * 64 + lane;
-    int n2c[2];
+    int n2c[kTableMaxColTiles];
 #pragma unroll
-    for (int it = 0; it < 2; ++it)
+    for (int it = 0; it < kTableMaxColTiles; ++it)
         n2c[it] = (it < ntile) ? q.colmap[d * q.maxcols + (tile0 + it) * 16 + (lane & 15)] : -1;
     float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
-    table_gemm_dispatch<float, NIPMAX, WAVES>(tab, q.x32, q.N, q.N4, q.nmax, R, wave, nip, ntile, n2c, Qt);
+    table_gemm_dispatch<float, 1, NIPMAX, NTLMAX, WAVES>(tab, q.x32, q.N, q.N4, q.nmax, R, wave, nip, ntile, n2c,
+                                                         Qt, tstride, tile0);
 }
 
 // V[j+1] = V[j] - t (W V[j]) from the complex64 Q, FP64 accumulation.
@@ -520,7 +532,7 @@ struct Request {
     // progress
     Phase phase = kQueued;
     bool cycle_ended = false, first_refresh = true;
-    int cycles = 0, iterations = 0, waves = 8, slot = -1;
+    int cycles = 0, iterations = 0, shape = 2, slot = -1;
     double bnorm = 0.0, rnorm = 0.0;
     double t_submit = 0.0, t_done = 0.0;
     // results
@@ -548,7 +560,7 @@ struct yasfhip_server {
     bool use_graph = true;
     long long next_id = 1;
     std::map<int, PlanEntry> plans;
-    int waves_for_lmax[kMaxLmax + 1];
+    int shape_for_lmax[kMaxLmax + 1];  // index into kShape
     std::deque<Request *> queue;
     std::vector<Request *> active;
     std::deque<Request *> finished;
@@ -561,8 +573,7 @@ struct yasfhip_server {
     std::vector<int> h_index;
     std::vector<BatchReq> h_reqs;  // host copies, kept alive across their async uploads
     int chunk_blocks = 0, tile_blocks = 0;
-    int grp_blocks[kMaxGemmGroups] = {0, 0, 0}, grp_count[kMaxGemmGroups] = {0, 0, 0};
-    int grp_waves[kMaxGemmGroups] = {2, 4, 8};
+    int grp_blocks[kMaxGemmGroups] = {}, grp_count[kMaxGemmGroups] = {};
     bool set_changed = true;
     hipGraphExec_t graph = nullptr;
     // per-request scalar results of the refresh / completion syncs
@@ -674,15 +685,18 @@ struct yasfhip_server {
     }
 
     // Orders per wave, ceil((2L+1)/waves), must not exceed the kernel's NIPMAX.
-    static bool waves_valid(int L, int waves) {
+    static bool shape_valid(int L, int shape) {
+        if (shape < 0 || shape >= kShapes)
+            return false;
+        const int waves = kShape[shape].waves;
         const int nipmax = waves == 8 ? 3 : 5;
-        return waves >= 2 && (2 * L + 1 + waves - 1) / waves <= nipmax;
+        return (2 * L + 1 + waves - 1) / waves <= nipmax;
     }
 
-    int waves_of(int L) const {
-        const int w = waves_for_lmax[L];
-        YH_REQUIRE(waves_valid(L, w), "invalid tile shape for this lmax");
-        return w;
+    int shape_of(int L) const {
+        const int sh = shape_for_lmax[L];
+        YH_REQUIRE(shape_valid(L, sh), "invalid tile shape for this lmax");
+        return sh;
     }
 
     // Admission: first-in first-out while the head request fits.
@@ -715,7 +729,7 @@ struct yasfhip_server {
             YH_CHECK(hipGetLastError());
             const DevicePlan &dp = plan_for(q.L).dp;
             launch_pair_table_build(dp, q.pos_d, q.N, q.k, true, q.table, st);
-            q.waves = waves_of(q.L);
+            q.shape = shape_of(q.L);
             q.phase = kRefresh;
             q.first_refresh = true;
             q.slot = static_cast<int>(active.size());
@@ -895,7 +909,7 @@ struct yasfhip_server {
             b.g = q.g;
             b.gc = q.gc;
             b.gs = q.gs;
-            b.jobs = dp.jobs;
+            b.jobs = dp.tjobs[kShape[q.shape].grouping];
             b.colmap = dp.colmap;
             b.crow = dp.crow;
             b.cidx = dp.cidx;
@@ -926,11 +940,11 @@ struct yasfhip_server {
             int cnt = 0, blocks = 0;
             for (int r = 0; r < nreq; ++r) {
                 Request &q = *active[r];
-                if (q.waves != grp_waves[gi])
+                if (q.shape != gi)
                     continue;
                 gstart[cnt] = blocks;
                 gidx[cnt] = r;
-                blocks += plan_for(q.L).dp.njobs * q.ntiles1;
+                blocks += plan_for(q.L).dp.ntjobs[kShape[gi].grouping] * q.ntiles1;
                 ++cnt;
             }
             gstart[cnt] = blocks;
@@ -953,6 +967,39 @@ struct yasfhip_server {
         set_changed = false;
     }
 
+    // The batched complex64 pair GEMM of one tile-shape group.
+    void launch_gemm(int shape, const int *gstart, const int *gidx, int count, int blocks) {
+        hipStream_t st = stream;
+        static_assert(kShapes == 6, "one instance per entry of kShape");
+        switch (shape) {
+        case 0:
+            hipLaunchKernelGGL((serve_gemm_kernel<2, 5, 2>), dim3(blocks), dim3(128), 0, st, d_reqs, d_state,
+                               gstart, gidx, count);
+            break;
+        case 1:
+            hipLaunchKernelGGL((serve_gemm_kernel<4, 5, 2>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
+                               gstart, gidx, count);
+            break;
+        case 2:
+            hipLaunchKernelGGL((serve_gemm_kernel<8, 3, 2>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+                               gstart, gidx, count);
+            break;
+        case 3:
+            hipLaunchKernelGGL((serve_gemm_kernel<4, 5, 4>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
+                               gstart, gidx, count);
+            break;
+        case 4:
+            hipLaunchKernelGGL((serve_gemm_kernel<8, 3, 4>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+                               gstart, gidx, count);
+            break;
+        default:
+            hipLaunchKernelGGL((serve_gemm_kernel<8, 3, 8>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+                               gstart, gidx, count);
+            break;
+        }
+        YH_CHECK(hipGetLastError());
+    }
+
     // The batched Arnoldi iteration (see the file comment).
     void launch_iteration() {
         const int nreq = static_cast<int>(active.size());
@@ -965,20 +1012,7 @@ struct yasfhip_server {
                 continue;
             const int *gstart = d_index + 2 * (nreq + 1) + gi * 2 * (nreq + 1);
             const int *gidx = gstart + nreq + 1;
-            switch (grp_waves[gi]) {
-            case 2:
-                hipLaunchKernelGGL((serve_gemm_kernel<2, 5>), dim3(grp_blocks[gi]), dim3(128), 0, st, d_reqs,
-                                   d_state, gstart, gidx, grp_count[gi]);
-                break;
-            case 4:
-                hipLaunchKernelGGL((serve_gemm_kernel<4, 5>), dim3(grp_blocks[gi]), dim3(256), 0, st, d_reqs,
-                                   d_state, gstart, gidx, grp_count[gi]);
-                break;
-            default:
-                hipLaunchKernelGGL((serve_gemm_kernel<8, 3>), dim3(grp_blocks[gi]), dim3(512), 0, st, d_reqs,
-                                   d_state, gstart, gidx, grp_count[gi]);
-                break;
-            }
+            launch_gemm(gi, gstart, gidx, grp_count[gi], grp_blocks[gi]);
         }
         hipLaunchKernelGGL(serve_contract_kernel, dim3(tile_blocks), dim3(256), 0, st, d_reqs, d_state, tstart,
                            nreq);
@@ -1031,43 +1065,25 @@ struct yasfhip_server {
         YH_CHECK(hipMemcpyAsync(q.x32, xh.data(), sizeof(float2) * q.n, hipMemcpyHostToDevice, st));
         const DevicePlan &dp = plan_for(L).dp;
         launch_pair_table_build(dp, q.pos_d, N, q.k, true, q.table, st);
-        const int cand[kMaxGemmGroups] = {2, 4, 8};
-        int best = 8;
+        int best = -1;
         double best_ms = 1e300;
         q.slot = 0;
         active.push_back(&q);
         try {
-            for (int gi = 0; gi < kMaxGemmGroups; ++gi) {
+            for (int gi = 0; gi < kShapes; ++gi) {
                 ms[gi] = -1.0;
-                if (!waves_valid(L, cand[gi]))
+                if (!shape_valid(L, gi))
                     continue;
-                q.waves = cand[gi];
+                q.shape = gi;
                 set_changed = true;
                 rebuild_batch();
                 h_state[0] = ReqState{kArnoldi, 0, 0, 0, 1.0, 0.0, 0.0};
                 YH_CHECK(hipMemcpyAsync(d_state, h_state, sizeof(ReqState), hipMemcpyHostToDevice, st));
                 const int *gstart = d_index + 2 * 2 + gi * 2 * 2;
-                auto run = [&] {
-                    switch (cand[gi]) {
-                    case 2:
-                        hipLaunchKernelGGL((serve_gemm_kernel<2, 5>), dim3(grp_blocks[gi]), dim3(128), 0, st,
-                                           d_reqs, d_state, gstart, gstart + 2, 1);
-                        break;
-                    case 4:
-                        hipLaunchKernelGGL((serve_gemm_kernel<4, 5>), dim3(grp_blocks[gi]), dim3(256), 0, st,
-                                           d_reqs, d_state, gstart, gstart + 2, 1);
-                        break;
-                    default:
-                        hipLaunchKernelGGL((serve_gemm_kernel<8, 3>), dim3(grp_blocks[gi]), dim3(512), 0, st,
-                                           d_reqs, d_state, gstart, gstart + 2, 1);
-                        break;
-                    }
-                    YH_CHECK(hipGetLastError());
-                };
-                run();  // warm-up
+                launch_gemm(gi, gstart, gstart + 2, 1, grp_blocks[gi]);  // warm-up
                 YH_CHECK(hipEventRecord(ev[0], st));
                 for (int i = 0; i < reps; ++i)
-                    run();
+                    launch_gemm(gi, gstart, gstart + 2, 1, grp_blocks[gi]);
                 YH_CHECK(hipEventRecord(ev[1], st));
                 YH_CHECK(hipEventSynchronize(ev[1]));
                 float t = 0.f;
@@ -1075,7 +1091,7 @@ struct yasfhip_server {
                 ms[gi] = t / reps;
                 if (ms[gi] < best_ms) {
                     best_ms = ms[gi];
-                    best = cand[gi];
+                    best = gi;
                 }
             }
         } catch (...) {
@@ -1087,7 +1103,7 @@ struct yasfhip_server {
         active.clear();
         arena.release(q.block);
         set_changed = true;
-        waves_for_lmax[L] = best;
+        shape_for_lmax[L] = best;
         return best;
     }
 
@@ -1162,7 +1178,7 @@ int yasfhip_server_create(size_t memory_bytes, int restart, double inner_rtol, i
         s->inner_rtol = inner_rtol;
         s->use_graph = use_graph != 0;
         for (int L = 0; L <= kMaxLmax; ++L)
-            s->waves_for_lmax[L] = 8;
+            s->shape_for_lmax[L] = 2;  // 8 waves, jobs of up to 2 column tiles
         YH_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         for (auto &e : s->ev)
             YH_CHECK(hipEventCreate(&e));
@@ -1234,23 +1250,30 @@ int yasfhip_server_step(yasfhip_server *s, int *active, int *queued, long long *
 }
 
 int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, int reps, int *waves,
-                            double *ms) {
+                            int *column_tiles, double *ms) {
     return sguard([&] {
-        YH_REQUIRE(s && waves && ms, "NULL argument");
+        YH_REQUIRE(s && waves && column_tiles && ms, "NULL argument");
         YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
         YH_REQUIRE(particle_number >= 16, "particle_number must be >= 16");
         YH_REQUIRE(reps >= 1, "reps must be >= 1");
-        *waves = s->autotune(lmax, particle_number, reps, ms);
+        const int sh = s->autotune(lmax, particle_number, reps, ms);
+        *waves = kShape[sh].waves;
+        *column_tiles = kTableGrouping[kShape[sh].grouping];
     });
 }
 
-int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves) {
+int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int column_tiles) {
     return sguard([&] {
         YH_REQUIRE(s, "NULL argument");
         YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
-        YH_REQUIRE(yasfhip_server::waves_valid(lmax, waves) && (waves == 2 || waves == 4 || waves == 8),
-                   "waves must be 2, 4 or 8 with ceil((2 lmax + 1) / waves) orders per wave within range");
-        s->waves_for_lmax[lmax] = waves;
+        int found = -1;
+        for (int i = 0; i < kShapes; ++i)
+            if (kShape[i].waves == waves && kTableGrouping[kShape[i].grouping] == column_tiles)
+                found = i;
+        YH_REQUIRE(found >= 0 && yasfhip_server::shape_valid(lmax, found),
+                   "unsupported tile shape for this lmax (waves 2/4/8 x column tiles 2/4/8 as listed in "
+                   "yasfhip.h, with ceil((2 lmax + 1) / waves) orders per wave within range)");
+        s->shape_for_lmax[lmax] = found;
     });
 }
 

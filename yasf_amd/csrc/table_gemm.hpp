@@ -11,12 +11,15 @@ namespace yasfhip {
 // One wave's GEMM with NIP orders (ip = wave + WAVES j) and NTL column tiles,
 // all compile-time, so the 4*NIP*NTL MFMAs of a K-step form one straight-line
 // block the scheduler can interleave.  tab points at the wave's target tile
-// and dm (plus the lane); its C tile (ip, it) goes to Qt + slot ip*ntile + it.
+// and dm (plus the lane); n2c[it] is the source mode of the lane's column in
+// column tile it (-1: padding).  C tile (ip, tile0 + it) goes to slot
+// ip * tstride + tile0 + it past Qt.
 template <typename T, int NIP, int NTL, int WAVES>
 __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__restrict__ tab,
                                                 const typename Cplx<T>::type *__restrict__ x, int N,
-                                                int N4, int nmax, int R, int wave, const int (&n2c)[2],
-                                                T *__restrict__ Qt, int ntile) {
+                                                int N4, int nmax, int R, int wave,
+                                                const int (&n2c)[kTableMaxColTiles], T *__restrict__ Qt,
+                                                int tstride, int tile0) {
     typedef typename Mfma<T>::acc_t acc_t;
     typedef typename Cplx<T>::type T2;
     const int lane = threadIdx.x & 63, lo = lane & 15, hi = lane >> 4;
@@ -79,7 +82,8 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
     for (int j = 0; j < NIP; ++j)
 #pragma unroll
         for (int it = 0; it < NTL; ++it) {
-            T *dst = Qt + 2 * ((static_cast<size_t>((wave + WAVES * j) * ntile + it) * 16 + lo) * 16);
+            const size_t slot = static_cast<size_t>(wave + WAVES * j) * tstride + tile0 + it;
+            T *dst = Qt + 2 * ((slot * 16 + lo) * 16);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = Mfma<T>::row(i, hi);
@@ -89,24 +93,38 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
         }
 }
 
-// Dispatches a wave's (nip, ntile) to its compile-time instance; nip <= NIPMAX.
-template <typename T, int NIPMAX, int WAVES>
+// Runs the compile-time instance for a wave's (nip, ntile), nip <= NIPMAX and
+// ntile <= NTLMAX (uniform values: scalar branches).
+template <typename T, int NIP, int NTL, int NTLMAX, int WAVES>
+__device__ __forceinline__ void table_gemm_ntl(const typename Cplx<T>::type *__restrict__ tab,
+                                               const typename Cplx<T>::type *__restrict__ x, int N, int N4,
+                                               int nmax, int R, int wave, int ntile,
+                                               const int (&n2c)[kTableMaxColTiles], T *__restrict__ Qt,
+                                               int tstride, int tile0) {
+    if constexpr (NTL <= NTLMAX) {
+        if (ntile == NTL) {
+            table_gemm_wave<T, NIP, NTL, WAVES>(tab, x, N, N4, nmax, R, wave, n2c, Qt, tstride, tile0);
+            return;
+        }
+        table_gemm_ntl<T, NIP, NTL + 1, NTLMAX, WAVES>(tab, x, N, N4, nmax, R, wave, ntile, n2c, Qt, tstride,
+                                                       tile0);
+    }
+}
+
+template <typename T, int NIP, int NIPMAX, int NTLMAX, int WAVES>
 __device__ __forceinline__ void table_gemm_dispatch(const typename Cplx<T>::type *__restrict__ tab,
                                                     const typename Cplx<T>::type *__restrict__ x, int N,
                                                     int N4, int nmax, int R, int wave, int nip, int ntile,
-                                                    const int (&n2c)[2], T *__restrict__ Qt) {
-    static_assert(kJobMaxColTiles == 2, "table GEMM dispatch assumes at most 2 column tiles per job");
-    switch (nip * 2 + ntile - 1) {
-#define YH_CASE(NIP, NTL)                                                                            \
-    case NIP * 2 + NTL - 1:                                                                          \
-        if constexpr (NIP <= NIPMAX)                                                                 \
-            table_gemm_wave<T, NIP, NTL, WAVES>(tab, x, N, N4, nmax, R, wave, n2c, Qt, ntile);       \
-        break;
-        YH_CASE(1, 1) YH_CASE(1, 2) YH_CASE(2, 1) YH_CASE(2, 2) YH_CASE(3, 1) YH_CASE(3, 2)
-        YH_CASE(4, 1) YH_CASE(4, 2) YH_CASE(5, 1) YH_CASE(5, 2)
-#undef YH_CASE
-    default:
-        break;
+                                                    const int (&n2c)[kTableMaxColTiles], T *__restrict__ Qt,
+                                                    int tstride, int tile0) {
+    if constexpr (NIP <= NIPMAX) {
+        if (nip == NIP) {
+            table_gemm_ntl<T, NIP, 1, NTLMAX, WAVES>(tab, x, N, N4, nmax, R, wave, ntile, n2c, Qt, tstride,
+                                                     tile0);
+            return;
+        }
+        table_gemm_dispatch<T, NIP + 1, NIPMAX, NTLMAX, WAVES>(tab, x, N, N4, nmax, R, wave, nip, ntile, n2c,
+                                                               Qt, tstride, tile0);
     }
 }
 

@@ -142,8 +142,11 @@ CouplingPlan build_coupling_plan(int lmax) {
         for (size_t c = 0; c < cols[dmi].size(); ++c)
             plan.colmap[static_cast<size_t>(dmi) * maxcols + c] = cols[dmi][c];
 
-    // Jobs: (dm, all its orders, a group of column tiles) with at most
-    // kJobMaxTiles C tiles.  slot[(dmi, p, col)] -> Q column index q*16 + col%16.
+    // Q slots: per dm, R orders x all of its column tiles, slot(ip, it) =
+    // qbase(dm) + ip * ntiles(dm) + it; slot[(dmi, p, col)] -> Q column index
+    // slot * 16 + col % 16.  Jobs: (dm, all its orders, a group of column
+    // tiles) with at most kJobMaxTiles C tiles for the on-the-fly kernel; the
+    // table GEMM's lists group the same column tiles more widely.
     std::map<std::tuple<int, int, int>, int> slot;
     int q = 0;
     long long macs = 0;
@@ -151,24 +154,33 @@ CouplingPlan build_coupling_plan(int lmax) {
         const int dm = dmi - 2 * lmax;
         const int R = P - std::abs(dm);
         const int ntiles = (static_cast<int>(cols[dmi].size()) + 15) / 16;
+        for (int ip = 0; ip < R; ++ip)
+            for (int it = 0; it < ntiles; ++it)
+                for (int c = 0; c < 16; ++c)
+                    slot[{dmi, std::abs(dm) + ip, it * 16 + c}] = (q + ip * ntiles + it) * 16 + c;
         const int group = std::max(1, std::min(kJobMaxColTiles, kJobMaxTiles / R));
         for (int t0 = 0; t0 < ntiles; t0 += group) {
             const int nt = std::min(group, ntiles - t0);
-            PairJob job{dm, t0, nt, q, R * nt, 0, 0, 0};
-            for (int ip = 0; ip < R; ++ip)
-                for (int it = 0; it < nt; ++it)
-                    for (int c = 0; c < 16; ++c)
-                        slot[{dmi, std::abs(dm) + ip, (t0 + it) * 16 + c}] = (q + ip * nt + it) * 16 + c;
-            q += R * nt;
-            macs += static_cast<long long>(R) * nt * 16;
+            plan.jobs.push_back(PairJob{dm, t0, nt, q, R * nt, ntiles, 0, 0});
             plan.stage_elems = std::max(plan.stage_elems, R * 256 + 16 * nt * 16);
-            plan.jobs.push_back(job);
         }
+        for (int g = 0; g < kTableGroupings; ++g) {
+            const int njob = (ntiles + kTableGrouping[g] - 1) / kTableGrouping[g];
+            for (int j = 0, t0 = 0; j < njob; ++j) {  // even split: sizes differ by at most one
+                const int nt = ntiles / njob + (j < ntiles % njob ? 1 : 0);
+                plan.table_jobs[g].push_back(PairJob{dm, t0, nt, q, R * nt, ntiles, 0, 0});
+                t0 += nt;
+            }
+        }
+        q += R * ntiles;
+        macs += static_cast<long long>(R) * ntiles * 16;
     }
     plan.nq = q;
     plan.pair_macs_per_pair = macs;
-    std::stable_sort(plan.jobs.begin(), plan.jobs.end(),
-                     [](const PairJob &a, const PairJob &b) { return a.cost > b.cost; });
+    auto by_cost = [](const PairJob &a, const PairJob &b) { return a.cost > b.cost; };
+    std::stable_sort(plan.jobs.begin(), plan.jobs.end(), by_cost);
+    for (auto &tj : plan.table_jobs)
+        std::stable_sort(tj.begin(), tj.end(), by_cost);
 
     // Contraction lists: for each n1, the nonzero ab5[n2][n1][p].
     const auto ab5 = translation_table(lmax);
@@ -242,6 +254,10 @@ DevicePlan upload_device_plan(const CouplingPlan &pl, hipStream_t s) {
     dp.njobs = static_cast<int>(pl.jobs.size());
     dp.stage_elems = pl.stage_elems;
     dp.jobs = upload_vec(pl.jobs, s);
+    for (int g = 0; g < kTableGroupings; ++g) {
+        dp.tjobs[g] = upload_vec(pl.table_jobs[g], s);
+        dp.ntjobs[g] = static_cast<int>(pl.table_jobs[g].size());
+    }
     dp.colmap = upload_vec(pl.colmap, s);
     dp.crow = upload_vec(pl.crow, s);
     dp.cidx = upload_vec(pl.cidx, s);
@@ -263,7 +279,8 @@ DevicePlan upload_device_plan(const CouplingPlan &pl, hipStream_t s) {
 }
 
 void free_device_plan(DevicePlan &dp) {
-    void *bufs[] = {dp.jobs, dp.colmap, dp.crow, dp.cidx, dp.cval, dp.la, dp.lb, dp.ld, dp.dmoff};
+    void *bufs[] = {dp.jobs,  dp.tjobs[0], dp.tjobs[1], dp.tjobs[2], dp.colmap, dp.crow,
+                    dp.cidx,  dp.cval,     dp.la,       dp.lb,       dp.ld,     dp.dmoff};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
