@@ -31,36 +31,36 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
             acc[j][it][0] = acc_t{0, 0, 0, 0};
             acc[j][it][1] = acc_t{0, 0, 0, 0};
         }
-    // The next K-step's operands are in flight during the current MFMAs.
-    T2 a0[NIP], b0[NTL];
+    // Two operand buffers in a ping-pong: the loads of K-step ks+1 are issued
+    // before the MFMAs of step ks and only the older batch is waited for.
+    // Loads are unconditional (clamped to valid addresses), so the wait
+    // counts are static; B lanes outside the job or past the last source are
+    // zeroed at use (their table entries are zero as well).
+    T2 aA[NIP], bA[NTL], aB[NIP], bB[NTL];
+    size_t boff[NTL];
+#pragma unroll
+    for (int it = 0; it < NTL; ++it)
+        boff[it] = n2c[it] >= 0 ? static_cast<size_t>(n2c[it]) : 0;
     auto load = [&](int ks, T2 (&an)[NIP], T2 (&bn)[NTL]) {
-        const bool kin = ks < N4;
-#pragma unroll
-        for (int j = 0; j < NIP; ++j) {
-            an[j].x = T(0);
-            an[j].y = T(0);
-            if (kin)
-                an[j] = tab[(static_cast<size_t>(ks) * R + wave + WAVES * j) * 64];
-        }
-        const int s2 = ks * 4 + hi;
-#pragma unroll
-        for (int it = 0; it < NTL; ++it) {
-            bn[it].x = T(0);
-            bn[it].y = T(0);
-            if (n2c[it] >= 0 && s2 < N)
-                bn[it] = x[static_cast<size_t>(s2) * nmax + n2c[it]];
-        }
-    };
-    load(0, a0, b0);
-    for (int ks = 0; ks < N4; ++ks) {
-        T2 a[NIP], b[NTL];
+        const int kc = ks < N4 ? ks : N4 - 1;
 #pragma unroll
         for (int j = 0; j < NIP; ++j)
-            a[j] = a0[j];
+            an[j] = tab[(static_cast<size_t>(kc) * R + wave + WAVES * j) * 64];
+        const int s2 = kc * 4 + hi;
+        const size_t row = static_cast<size_t>(s2 < N ? s2 : 0) * nmax;
 #pragma unroll
         for (int it = 0; it < NTL; ++it)
-            b[it] = b0[it];
-        load(ks + 1, a0, b0);
+            bn[it] = x[row + boff[it]];
+    };
+    auto mma = [&](int ks, const T2 (&a)[NIP], const T2 (&bl)[NTL]) {
+        const bool srcv = ks * 4 + hi < N;
+        T2 b[NTL];
+#pragma unroll
+        for (int it = 0; it < NTL; ++it) {
+            const bool v = srcv && n2c[it] >= 0;
+            b[it].x = v ? bl[it].x : T(0);
+            b[it].y = v ? bl[it].y : T(0);
+        }
         // real parts first, then imaginary parts: consecutive MFMAs never
         // update the same accumulator
 #pragma unroll
@@ -77,7 +77,17 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
                 acc[j][it][0] = Mfma<T>::mac(-a[j].y, b[it].y, acc[j][it][0]);
                 acc[j][it][1] = Mfma<T>::mac(a[j].y, b[it].x, acc[j][it][1]);
             }
+    };
+    load(0, aA, bA);
+    int ks = 0;
+    for (; ks + 1 < N4; ks += 2) {
+        load(ks + 1, aB, bB);
+        mma(ks, aA, bA);
+        load(ks + 2, aA, bA);
+        mma(ks + 1, aB, bB);
     }
+    if (ks < N4)
+        mma(ks, aA, bA);
 #pragma unroll
     for (int j = 0; j < NIP; ++j)
 #pragma unroll
