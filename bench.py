@@ -140,6 +140,8 @@ def run_solve(args):
     s = _native.Session(pos, lmax, [k])
     s.set_tmatrix(t)
     s.set_matvec_mode(_native.MATVEC_TABLE)
+    tuned = {name: s.tune_matvec(precision=prec, reps=3)[0]
+             for name, prec in (("fp64", _native.FP64), ("fp32", _native.FP32))}
     use_pc = not args.no_precond
     boxes = None
     if use_pc:
@@ -221,7 +223,8 @@ def run_solve(args):
                    "particles": N, "lmax": lmax, "unknowns": N * 96,
                    "preconditioner": use_pc, "restart": args.restart,
                    "box_cap": args.box_cap if use_pc else None,
-                   "matvec": "resident FP64 pair-factor table + ab5 contraction"},
+                   "matvec": "resident FP64 pair-factor table + ab5 contraction",
+                   "table_gemm_column_tiles": {k: [2, 4, 8][v] for k, v in tuned.items()}},
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "gmres_iterations": iters,

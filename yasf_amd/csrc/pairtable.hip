@@ -76,8 +76,10 @@ __global__ __launch_bounds__(512) void pair_table_gemm_kernel(
     for (int it = 0; it < kTableMaxColTiles; ++it)
         n2c[it] = (it < ntile) ? colmap[d * maxcols + (tile0 + it) * 16 + lo] : -1;
     T *Qt = Q + 2 * (static_cast<size_t>(t) * nq + qbase) * 256;
-    table_gemm_dispatch<T, 1, NIPMAX, NTLMAX, kTableWaves>(tab, x, N, N4, nmax, R, wave, nip, ntile, n2c, Qt,
-                                                           tstride, tile0);
+    // complex128 operands: a 3-deep pipeline; complex64: 2-deep keeps more waves resident
+    constexpr int kDepth = sizeof(T) == sizeof(double) ? 3 : 2;
+    table_gemm_dispatch<T, 1, NIPMAX, NTLMAX, kTableWaves, kDepth>(tab, x, N, N4, nmax, R, wave, nip, ntile,
+                                                                   n2c, Qt, tstride, tile0);
 }
 
 template <typename T, int NTLMAX>

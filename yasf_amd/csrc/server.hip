@@ -184,8 +184,8 @@ __global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *
     for (int it = 0; it < kTableMaxColTiles; ++it)
         n2c[it] = (it < ntile) ? q.colmap[d * q.maxcols + (tile0 + it) * 16 + (lane & 15)] : -1;
     float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
-    table_gemm_dispatch<float, 1, NIPMAX, NTLMAX, WAVES>(tab, q.x32, q.N, q.N4, q.nmax, R, wave, nip, ntile, n2c,
-                                                         Qt, tstride, tile0);
+    table_gemm_dispatch<float, 1, NIPMAX, NTLMAX, WAVES, 3>(tab, q.x32, q.N, q.N4, q.nmax, R, wave, nip, ntile,
+                                                            n2c, Qt, tstride, tile0);
 }
 
 // V[j+1] = V[j] - t (W V[j]) from the complex64 Q, FP64 accumulation.

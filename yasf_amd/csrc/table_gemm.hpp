@@ -14,7 +14,7 @@ namespace yasfhip {
 // and dm (plus the lane); n2c[it] is the source mode of the lane's column in
 // column tile it (-1: padding).  C tile (ip, tile0 + it) goes to slot
 // ip * tstride + tile0 + it past Qt.
-template <typename T, int NIP, int NTL, int WAVES>
+template <typename T, int NIP, int NTL, int WAVES, int DEPTH>
 __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__restrict__ tab,
                                                 const typename Cplx<T>::type *__restrict__ x, int N,
                                                 int N4, int nmax, int R, int wave,
@@ -31,8 +31,9 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
             acc[j][it][0] = acc_t{0, 0, 0, 0};
             acc[j][it][1] = acc_t{0, 0, 0, 0};
         }
-    // Two operand buffers in a ping-pong: the loads of K-step ks+1 are issued
-    // before the MFMAs of step ks and only the older batch is waited for.
+    // DEPTH operand buffers in rotation: the loads of K-step ks+DEPTH-1 are
+    // issued before the MFMAs of step ks and only the oldest batch is waited
+    // for (a deeper pipeline hides more latency but costs registers).
     // Loads are unconditional (clamped to valid addresses), so the wait
     // counts are static; B lanes outside the job or past the last source are
     // zeroed at use (their table entries are zero as well).
@@ -78,16 +79,36 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
                 acc[j][it][1] = Mfma<T>::mac(a[j].y, b[it].x, acc[j][it][1]);
             }
     };
-    load(0, aA, bA);
-    int ks = 0;
-    for (; ks + 1 < N4; ks += 2) {
-        load(ks + 1, aB, bB);
-        mma(ks, aA, bA);
-        load(ks + 2, aA, bA);
-        mma(ks + 1, aB, bB);
+    if constexpr (DEPTH == 2) {
+        load(0, aA, bA);
+        int ks = 0;
+        for (; ks + 1 < N4; ks += 2) {
+            load(ks + 1, aB, bB);
+            mma(ks, aA, bA);
+            load(ks + 2, aA, bA);
+            mma(ks + 1, aB, bB);
+        }
+        if (ks < N4)
+            mma(ks, aA, bA);
+    } else {
+        static_assert(DEPTH == 3, "pipeline depth 2 or 3");
+        T2 aC[NIP], bC[NTL];
+        load(0, aA, bA);
+        load(1, aB, bB);
+        int ks = 0;
+        for (; ks + 2 < N4; ks += 3) {
+            load(ks + 2, aC, bC);
+            mma(ks, aA, bA);
+            load(ks + 3, aA, bA);
+            mma(ks + 1, aB, bB);
+            load(ks + 4, aB, bB);
+            mma(ks + 2, aC, bC);
+        }
+        if (ks < N4)
+            mma(ks, aA, bA);
+        if (ks + 1 < N4)
+            mma(ks + 1, aB, bB);
     }
-    if (ks < N4)
-        mma(ks, aA, bA);
 #pragma unroll
     for (int j = 0; j < NIP; ++j)
 #pragma unroll
@@ -105,7 +126,7 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
 
 // Runs the compile-time instance for a wave's (nip, ntile), nip <= NIPMAX and
 // ntile <= NTLMAX (uniform values: scalar branches).
-template <typename T, int NIP, int NTL, int NTLMAX, int WAVES>
+template <typename T, int NIP, int NTL, int NTLMAX, int WAVES, int DEPTH>
 __device__ __forceinline__ void table_gemm_ntl(const typename Cplx<T>::type *__restrict__ tab,
                                                const typename Cplx<T>::type *__restrict__ x, int N, int N4,
                                                int nmax, int R, int wave, int ntile,
@@ -113,15 +134,15 @@ __device__ __forceinline__ void table_gemm_ntl(const typename Cplx<T>::type *__r
                                                int tstride, int tile0) {
     if constexpr (NTL <= NTLMAX) {
         if (ntile == NTL) {
-            table_gemm_wave<T, NIP, NTL, WAVES>(tab, x, N, N4, nmax, R, wave, n2c, Qt, tstride, tile0);
+            table_gemm_wave<T, NIP, NTL, WAVES, DEPTH>(tab, x, N, N4, nmax, R, wave, n2c, Qt, tstride, tile0);
             return;
         }
-        table_gemm_ntl<T, NIP, NTL + 1, NTLMAX, WAVES>(tab, x, N, N4, nmax, R, wave, ntile, n2c, Qt, tstride,
-                                                       tile0);
+        table_gemm_ntl<T, NIP, NTL + 1, NTLMAX, WAVES, DEPTH>(tab, x, N, N4, nmax, R, wave, ntile, n2c, Qt,
+                                                              tstride, tile0);
     }
 }
 
-template <typename T, int NIP, int NIPMAX, int NTLMAX, int WAVES>
+template <typename T, int NIP, int NIPMAX, int NTLMAX, int WAVES, int DEPTH>
 __device__ __forceinline__ void table_gemm_dispatch(const typename Cplx<T>::type *__restrict__ tab,
                                                     const typename Cplx<T>::type *__restrict__ x, int N,
                                                     int N4, int nmax, int R, int wave, int nip, int ntile,
@@ -129,12 +150,12 @@ __device__ __forceinline__ void table_gemm_dispatch(const typename Cplx<T>::type
                                                     int tstride, int tile0) {
     if constexpr (NIP <= NIPMAX) {
         if (nip == NIP) {
-            table_gemm_ntl<T, NIP, 1, NTLMAX, WAVES>(tab, x, N, N4, nmax, R, wave, ntile, n2c, Qt, tstride,
-                                                     tile0);
+            table_gemm_ntl<T, NIP, 1, NTLMAX, WAVES, DEPTH>(tab, x, N, N4, nmax, R, wave, ntile, n2c, Qt,
+                                                            tstride, tile0);
             return;
         }
-        table_gemm_dispatch<T, NIP + 1, NIPMAX, NTLMAX, WAVES>(tab, x, N, N4, nmax, R, wave, nip, ntile, n2c,
-                                                               Qt, tstride, tile0);
+        table_gemm_dispatch<T, NIP + 1, NIPMAX, NTLMAX, WAVES, DEPTH>(tab, x, N, N4, nmax, R, wave, nip, ntile,
+                                                                      n2c, Qt, tstride, tile0);
     }
 }
 
