@@ -60,7 +60,8 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
     const int P = 2 * lmax + 1;
     const int R = P - (dm < 0 ? -dm : dm);
     const int ncols = ntile * 16;
-    const int total = R * ntile;
+    const int nord = uniform(jobs[jid].nord);
+    const int total = nord * ntile;  // C tiles: the first nord orders x the job's column tiles
     const int boff = R * 256;
     const int *cmap = colmap + (dm + 2 * lmax) * maxcols + tile0 * 16;
     const int nchunks = (N + SRC - 1) / SRC;

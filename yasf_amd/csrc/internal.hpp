@@ -63,8 +63,10 @@ std::vector<cplx> translation_table(int lmax);
 // grouped into jobs: C tile (ip, column tile it) of the dm goes to slot
 // qbase + ip * tstride + it, qbase being the dm's first slot and tstride its
 // number of column tiles.
+// Only the first nord orders (p = |dm| .. |dm|+nord-1) are computed: higher
+// orders have no nonzero ab5 term on the job's column tiles.
 struct PairJob {
-    int dm, tile0, ntile, qbase, cost, tstride, pad1, pad2;
+    int dm, tile0, ntile, qbase, cost, tstride, nord, pad;
 };
 
 // Column-tile groupings of the resident-table GEMM's job lists: a dm's column
@@ -93,7 +95,7 @@ struct CouplingPlan {
     std::vector<double> leg_alpha;    // [(2L+1)^2] Legendre recurrence alpha(l, m)
     std::vector<double> leg_beta;     // [(2L+1)^2] Legendre recurrence beta(l, m)
     std::vector<double> leg_diag;     // [2L+1] P_m^m = leg_diag[m] * sin^m
-    long long pair_macs_per_pair = 0; // complex MACs the pair kernel issues per pair
+    long long pair_macs_per_pair = 0; // complex MACs the on-the-fly pair kernel issues per pair
     long long nnz_ab5 = 0;
 };
 CouplingPlan build_coupling_plan(int lmax);

@@ -67,7 +67,8 @@ __global__ __launch_bounds__(512) void pair_table_gemm_kernel(
     const int R = P - (dm < 0 ? -dm : dm);
     const int d = dm + 2 * lmax;
     const int lo = lane & 15;
-    const int nip = R > wave ? (R - wave + kTableWaves - 1) / kTableWaves : 0;  // orders of this wave
+    const int nord = uniform(jobs[jid].nord);  // orders computed (R is the table's order stride)
+    const int nip = nord > wave ? (nord - wave + kTableWaves - 1) / kTableWaves : 0;  // orders of this wave
     if (nip == 0)
         return;
     const T2 *tab = table + (static_cast<size_t>(t) * P * P + dmoff[d]) * N4 * 64 + lane;

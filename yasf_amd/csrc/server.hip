@@ -175,7 +175,8 @@ __global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *
     const int P = 2 * lmax + 1;
     const int R = P - (dm < 0 ? -dm : dm);
     const int d = dm + 2 * lmax;
-    const int nip = R > wave ? (R - wave + WAVES - 1) / WAVES : 0;
+    const int nord = uniform(q.jobs[jid].nord);  // orders computed (R is the table's order stride)
+    const int nip = nord > wave ? (nord - wave + WAVES - 1) / WAVES : 0;
     if (nip == 0)
         return;
     const float2 *tab = q.table + (static_cast<size_t>(t) * P * P + q.dmoff[d]) * q.N4 * 64 + lane;

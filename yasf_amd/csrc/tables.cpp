@@ -125,14 +125,46 @@ CouplingPlan build_coupling_plan(int lmax) {
     const auto modes = modes_of(lmax);
     const int P = 2 * lmax + 1;
 
-    // Source columns compatible with each signed dm: |m2 - dm| <= L.
+    // Source columns compatible with each signed dm: |m2 - dm| <= L, ordered
+    // by decreasing l2.  Order p needs only columns with l2 >= p - L
+    // (triangle rule of the 3j symbols), so with this order the columns an
+    // order needs form a prefix that shrinks as p grows.
     std::vector<std::vector<int>> cols(plan.ndm);
     for (int dmi = 0; dmi < plan.ndm; ++dmi) {
         const int dm = dmi - 2 * lmax;
         for (int n2 = 0; n2 < plan.nmax; ++n2)
             if (std::abs(modes[n2].m - dm) <= lmax)
                 cols[dmi].push_back(n2);
+        std::stable_sort(cols[dmi].begin(), cols[dmi].end(),
+                         [&](int a, int b) { return modes[a].l > modes[b].l; });
     }
+    // need[dmi][p]: column tiles of the dm that order p contributes to (one
+    // past the last tile holding a column with a nonzero ab5 term at p).
+    const auto ab5 = translation_table(lmax);
+    std::vector<std::vector<int>> need(plan.ndm, std::vector<int>(P, 0));
+    for (int dmi = 0; dmi < plan.ndm; ++dmi) {
+        const int dm = dmi - 2 * lmax;
+        for (size_t c = 0; c < cols[dmi].size(); ++c) {
+            const int n2 = cols[dmi][c];
+            for (int n1 = 0; n1 < plan.nmax; ++n1) {
+                if (modes[n2].m - modes[n1].m != dm)
+                    continue;
+                for (int p = 0; p < P; ++p)
+                    if (ab5[(static_cast<size_t>(n2) * plan.nmax + n1) * P + p] != cplx(0.0, 0.0))
+                        need[dmi][p] = std::max(need[dmi][p], static_cast<int>(c) / 16 + 1);
+            }
+        }
+    }
+    // orders a job over column tiles [t0, ...) must compute: the prefix of
+    // p = |dm|, |dm|+1, ... up to the last order that needs a tile >= t0
+    auto orders_for = [&](int dmi, int t0) {
+        const int adm = std::abs(dmi - 2 * lmax);
+        int nord = 0;
+        for (int p = adm; p < P; ++p)
+            if (need[dmi][p] > t0)
+                nord = p - adm + 1;
+        return nord;
+    };
     int maxcols = 0;
     for (auto &c : cols)
         maxcols = std::max<int>(maxcols, ((static_cast<int>(c.size()) + 15) / 16) * 16);
@@ -161,19 +193,24 @@ CouplingPlan build_coupling_plan(int lmax) {
         const int group = std::max(1, std::min(kJobMaxColTiles, kJobMaxTiles / R));
         for (int t0 = 0; t0 < ntiles; t0 += group) {
             const int nt = std::min(group, ntiles - t0);
-            plan.jobs.push_back(PairJob{dm, t0, nt, q, R * nt, ntiles, 0, 0});
+            const int nord = orders_for(dmi, t0);
+            if (nord == 0)
+                continue;
+            plan.jobs.push_back(PairJob{dm, t0, nt, q, nord * nt, ntiles, nord, 0});
             plan.stage_elems = std::max(plan.stage_elems, R * 256 + 16 * nt * 16);
+            macs += static_cast<long long>(nord) * nt * 16;
         }
         for (int g = 0; g < kTableGroupings; ++g) {
             const int njob = (ntiles + kTableGrouping[g] - 1) / kTableGrouping[g];
             for (int j = 0, t0 = 0; j < njob; ++j) {  // even split: sizes differ by at most one
                 const int nt = ntiles / njob + (j < ntiles % njob ? 1 : 0);
-                plan.table_jobs[g].push_back(PairJob{dm, t0, nt, q, R * nt, ntiles, 0, 0});
+                const int nord = orders_for(dmi, t0);
+                if (nord > 0)
+                    plan.table_jobs[g].push_back(PairJob{dm, t0, nt, q, nord * nt, ntiles, nord, 0});
                 t0 += nt;
             }
         }
         q += R * ntiles;
-        macs += static_cast<long long>(R) * ntiles * 16;
     }
     plan.nq = q;
     plan.pair_macs_per_pair = macs;
@@ -183,7 +220,6 @@ CouplingPlan build_coupling_plan(int lmax) {
         std::stable_sort(tj.begin(), tj.end(), by_cost);
 
     // Contraction lists: for each n1, the nonzero ab5[n2][n1][p].
-    const auto ab5 = translation_table(lmax);
     plan.crow.assign(plan.nmax + 1, 0);
     for (int n1 = 0; n1 < plan.nmax; ++n1) {
         const int m1 = modes[n1].m;
