@@ -188,7 +188,7 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
 
 // (W x)[s1, n1] = sum_e ab5_e Q[e], then either y = W x (mode 0) or the
 // master operator y = x - t (W x) (mode 1, matrix_apply.py:104-110).
-// One block per 16-particle target tile.
+// One block per (16-particle target tile, group of 16 target modes).
 template <typename TQ, typename TX, typename TY>
 __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q, int nq,
                                                        const int *__restrict__ crow,
@@ -197,7 +197,9 @@ __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q,
                                                        const TX *__restrict__ x,
                                                        const double2 *__restrict__ t, int mode,
                                                        TY *__restrict__ y) {
-    contract_tile<TQ, TX, TY>(Q, nq, crow, cidx, cval, N, nmax, x, t, mode, y, blockIdx.x);
+    const int groups = contract_groups(nmax);
+    const int s1t = blockIdx.x / groups;
+    contract_tile<TQ, TX, TY>(Q, nq, crow, cidx, cval, N, nmax, x, t, mode, y, s1t, blockIdx.x - s1t * groups);
 }
 
 template <typename T, bool Regular>
@@ -240,18 +242,18 @@ size_t pair_lds_bytes(const DevicePlan &dp) { return 2 * static_cast<size_t>(dp.
 // x and y, so x - t W x is formed in FP64).
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
                      int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s) {
-    const int ntiles1 = (N + 15) / 16;
+    const int nblocks = (N + 15) / 16 * contract_groups(dp.nmax);
     YH_REQUIRE(q_fp32 || !io_fp32, "complex64 vectors need the complex64 pair GEMM");
     if (io_fp32)
-        hipLaunchKernelGGL((contract_kernel<float, float, float>), dim3(ntiles1), dim3(256), 0, s,
+        hipLaunchKernelGGL((contract_kernel<float, float, float>), dim3(nblocks), dim3(256), 0, s,
                            static_cast<const float *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N, dp.nmax,
                            static_cast<const float *>(x), t, mode, static_cast<float *>(y));
     else if (q_fp32)
-        hipLaunchKernelGGL((contract_kernel<float, double, double>), dim3(ntiles1), dim3(256), 0, s,
+        hipLaunchKernelGGL((contract_kernel<float, double, double>), dim3(nblocks), dim3(256), 0, s,
                            static_cast<const float *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N, dp.nmax,
                            static_cast<const double *>(x), t, mode, static_cast<double *>(y));
     else
-        hipLaunchKernelGGL((contract_kernel<double, double, double>), dim3(ntiles1), dim3(256), 0, s,
+        hipLaunchKernelGGL((contract_kernel<double, double, double>), dim3(nblocks), dim3(256), 0, s,
                            static_cast<const double *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N,
                            dp.nmax, static_cast<const double *>(x), t, mode, static_cast<double *>(y));
     YH_CHECK(hipGetLastError());

@@ -222,42 +222,50 @@ __device__ __forceinline__ void pair_series_regular(double dx, double dy, double
     }
 }
 
-// (W x)[s1, n1] = sum_e ab5_e Q[e] for the 16 targets of tile s1t, then
-// either y = W x (mode 0) or the master operator y = x - t (W x) (mode 1,
-// matrix_apply.py:104-110); FP64 accumulation.  256 threads; lane&15 = target
-// in tile, the thread's n1 = threadIdx.x>>4, +16, ...
+// (W x)[s1, n1] = sum_e ab5_e Q[e] for the 16 targets of tile s1t and the 16
+// target modes n1 = 16 g + threadIdx.x/16 of mode group g, then either
+// y = W x (mode 0) or the master operator y = x - t (W x) (mode 1,
+// matrix_apply.py:104-110); FP64 accumulation.  256 threads: lane&15 =
+// target in tile, so the 16 targets of one Q slot are read as one coalesced
+// row.  A launch covers (target tile, mode group) pairs: ceil(nmax/16) blocks
+// per tile.
+constexpr int kContractModes = 16;  // target modes per block
+
+__host__ __device__ inline int contract_groups(int nmax) { return (nmax + kContractModes - 1) / kContractModes; }
+
 template <typename TQ, typename TX, typename TY>
 __device__ __forceinline__ void contract_tile(const TQ *__restrict__ Q, int nq, const int *__restrict__ crow,
                                               const int *__restrict__ cidx, const double2 *__restrict__ cval,
                                               int N, int nmax, const TX *__restrict__ x,
                                               const double2 *__restrict__ t, int mode, TY *__restrict__ y,
-                                              int s1t) {
+                                              int s1t, int group) {
     const int r = threadIdx.x & 15;
     const int s1 = s1t * 16 + r;
+    const int n1 = group * kContractModes + (threadIdx.x >> 4);
+    if (n1 >= nmax)
+        return;
     const size_t qrow = static_cast<size_t>(s1t) * nq * 16;
-    for (int n1 = threadIdx.x >> 4; n1 < nmax; n1 += blockDim.x >> 4) {
-        double sr = 0.0, si = 0.0;
-        const int e1 = crow[n1 + 1];
-        for (int e = crow[n1]; e < e1; ++e) {
-            const double2 a = cval[e];
-            const size_t qi = 2 * ((qrow + cidx[e]) * 16 + r);
-            const double qr = static_cast<double>(Q[qi]);
-            const double qim = static_cast<double>(Q[qi + 1]);
-            sr += a.x * qr - a.y * qim;
-            si += a.x * qim + a.y * qr;
-        }
-        if (s1 < N) {
-            const size_t j = static_cast<size_t>(s1) * nmax + n1;
-            if (mode == 1) {
-                const double2 tj = t[j];
-                const double xr = static_cast<double>(x[2 * j]);
-                const double xi = static_cast<double>(x[2 * j + 1]);
-                y[2 * j] = static_cast<TY>(xr - (tj.x * sr - tj.y * si));
-                y[2 * j + 1] = static_cast<TY>(xi - (tj.x * si + tj.y * sr));
-            } else {
-                y[2 * j] = static_cast<TY>(sr);
-                y[2 * j + 1] = static_cast<TY>(si);
-            }
+    double sr = 0.0, si = 0.0;
+    const int e1 = crow[n1 + 1];
+    for (int e = crow[n1]; e < e1; ++e) {
+        const double2 a = cval[e];
+        const size_t qi = 2 * ((qrow + cidx[e]) * 16 + r);
+        const double qr = static_cast<double>(Q[qi]);
+        const double qim = static_cast<double>(Q[qi + 1]);
+        sr += a.x * qr - a.y * qim;
+        si += a.x * qim + a.y * qr;
+    }
+    if (s1 < N) {
+        const size_t j = static_cast<size_t>(s1) * nmax + n1;
+        if (mode == 1) {
+            const double2 tj = t[j];
+            const double xr = static_cast<double>(x[2 * j]);
+            const double xi = static_cast<double>(x[2 * j + 1]);
+            y[2 * j] = static_cast<TY>(xr - (tj.x * sr - tj.y * si));
+            y[2 * j + 1] = static_cast<TY>(xi - (tj.x * si + tj.y * sr));
+        } else {
+            y[2 * j] = static_cast<TY>(sr);
+            y[2 * j + 1] = static_cast<TY>(si);
         }
     }
 }

@@ -190,7 +190,7 @@ __global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *
 }
 
 // V[j+1] = V[j] - t (W V[j]) from the complex64 Q, FP64 accumulation.
-// Grid: 16-particle target tiles of every request.
+// Grid: (16-particle target tile, group of 16 target modes) of every request.
 __global__ __launch_bounds__(256) void serve_contract_kernel(const BatchReq *__restrict__ reqs,
                                                              const ReqState *__restrict__ st,
                                                              const int *__restrict__ tstart, int nreq) {
@@ -201,9 +201,12 @@ __global__ __launch_bounds__(256) void serve_contract_kernel(const BatchReq *__r
     const int j = st[r].j;
     const double2 *v = q.V + static_cast<long>(j) * q.n;
     double2 *w = q.V + static_cast<long>(j + 1) * q.n;
+    const int groups = contract_groups(q.nmax);
+    const int b = blockIdx.x - tstart[r];
+    const int s1t = b / groups;
     contract_tile<float, double, double>(q.Q, q.nq, q.crow, q.cidx, q.cval, q.N, q.nmax,
                                          reinterpret_cast<const double *>(v), q.t, 1,
-                                         reinterpret_cast<double *>(w), blockIdx.x - tstart[r]);
+                                         reinterpret_cast<double *>(w), s1t, b - s1t * groups);
 }
 
 // Classical Gram-Schmidt passes over w = V[j+1], one block per chunk:
@@ -929,7 +932,7 @@ struct yasfhip_server {
             cstart[r] = cb;
             tstart[r] = tb;
             cb += q.chunks;
-            tb += q.ntiles1;
+            tb += q.ntiles1 * contract_groups(q.nmax);
         }
         cstart[nreq] = cb;
         tstart[nreq] = tb;
