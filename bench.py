@@ -6,7 +6,7 @@ sphere at volume fraction 0.2, n = 1.5 + 0.01i in vacuum, lambda = 550 nm,
 plane wave along +z (TE), lmax = 6 (96 coefficients per sphere, 192,000
 unknowns), complex128 matrix-free coupling over a resident FP64 pair-factor
 table, block-diagonal preconditioner (about 8 spheres per box, boxes bisected
-above 12), GMRES (restart 100) to 1e-6 relative residual.
+above 8), GMRES (restart 100) to 1e-6 relative residual.
 
 One step = one solve: preconditioner build + LU on the device, then device
 GMRES from x0 = b.  Geometry, T-matrix and pair table are resident on the
@@ -441,7 +441,7 @@ def main():
     ap.add_argument("--workload", choices=["solve", "precision", "serving"], default="solve")
     ap.add_argument("--no-precond", action="store_true")
     ap.add_argument("--per-box", type=float, default=8.0)
-    ap.add_argument("--box-cap", type=int, default=12, help="bisect boxes above this many spheres")
+    ap.add_argument("--box-cap", type=int, default=8, help="bisect boxes above this many spheres")
     ap.add_argument("--restart", type=int, default=100)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=48, help="target particles in the CPU sample")
