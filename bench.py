@@ -382,8 +382,8 @@ def run_serving(args):
     srv = _native.Server(restart=50, inner_rtol=1e-4)
     tiles = {}
     for L in range(3, 9):
-        w, ms = srv.autotune(L, particles=600, reps=3)
-        tiles[L] = {"waves": w, "ms": [round(float(v), 4) for v in ms]}
+        (waves, ctiles), ms = srv.autotune(L, particles=600, reps=3)
+        tiles[L] = {"waves": waves, "column_tiles": ctiles, "ms": [round(float(v), 4) for v in ms]}
     log(f"autotuned tile shapes: {tiles}")
     if args.warmup:
         serve(srv, make_workload(8, seed=99, n_range=(20, 200)))

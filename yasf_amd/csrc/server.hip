@@ -9,7 +9,9 @@
 //
 // Each request runs GMRES-IR: restart cycles of GMRES whose operator applies
 // the complex64 pair GEMM (contraction, epilogue, Arnoldi and Givens in FP64),
-// and after every cycle the FP64 residual b - A x (FP64 on-the-fly matvec)
+// and after every cycle the FP64 residual b - A x (FP64 matvec: over a
+// resident FP64 table when the request is at most a quarter of the arena,
+// otherwise with the pair factors generated on the fly)
 // decides convergence (||r|| <= rtol ||b||, the reference's criterion,
 // yasfpy/solver.py:108-121) and seeds the next cycle, which is asked for the
 // reduction max(inner_rtol, rtol ||b|| / (2 ||r||)).  x0 = b as in
@@ -529,6 +531,8 @@ struct Request {
     double4 *pos_d = nullptr;
     double2 *t_d = nullptr, *a_d = nullptr, *b_d = nullptr, *x_d = nullptr, *y_d = nullptr;
     float2 *table = nullptr, *x32 = nullptr;
+    bool table64 = false;          // resident FP64 table for the residual matvec
+    double2 *tab64 = nullptr;
     void *Q = nullptr;
     double2 *V = nullptr, *partial = nullptr, *H = nullptr, *h1 = nullptr, *h2 = nullptr, *g = nullptr,
             *gs = nullptr;
@@ -642,6 +646,7 @@ struct yasfhip_server {
         const size_t o_b = take(sizeof(double2) * q.n), o_x = take(sizeof(double2) * q.n);
         const size_t o_y = take(sizeof(double2) * R1);
         const size_t o_tab = take(pair_table_bytes(dp, q.N, true));
+        const size_t o_tab64 = q.table64 ? take(pair_table_bytes(dp, q.N, false)) : 0;
         const size_t o_x32 = take(sizeof(float2) * q.n);
         const size_t o_Q = take(sizeof(double2) * q.ntiles1 * dp.nq * 256);
         const size_t o_V = take(sizeof(double2) * R1 * q.n);
@@ -660,6 +665,7 @@ struct yasfhip_server {
             q.x_d = reinterpret_cast<double2 *>(b + o_x);
             q.y_d = reinterpret_cast<double2 *>(b + o_y);
             q.table = reinterpret_cast<float2 *>(b + o_tab);
+            q.tab64 = q.table64 ? reinterpret_cast<double2 *>(b + o_tab64) : nullptr;
             q.x32 = reinterpret_cast<float2 *>(b + o_x32);
             q.Q = b + o_Q;
             q.V = reinterpret_cast<double2 *>(b + o_V);
@@ -707,8 +713,15 @@ struct yasfhip_server {
     void admit() {
         while (!queue.empty()) {
             Request &q = *queue.front();
-            const size_t bytes = layout(q, false);
-            q.block = arena.alloc(bytes);
+            // an FP64 table speeds up the FP64 residual of every cycle; it is
+            // kept only for requests that leave most of the arena to others
+            q.table64 = true;
+            const size_t with64 = layout(q, false);
+            q.block = with64 <= arena.capacity() / 4 ? arena.alloc(with64) : nullptr;
+            if (!q.block) {
+                q.table64 = false;
+                q.block = arena.alloc(layout(q, false));
+            }
             if (!q.block) {
                 if (active.empty()) {  // can never fit: fail it
                     q.info = -1;
@@ -733,6 +746,8 @@ struct yasfhip_server {
             YH_CHECK(hipGetLastError());
             const DevicePlan &dp = plan_for(q.L).dp;
             launch_pair_table_build(dp, q.pos_d, q.N, q.k, true, q.table, st);
+            if (q.table64)
+                launch_pair_table_build(dp, q.pos_d, q.N, q.k, false, q.tab64, st);
             q.shape = shape_of(q.L);
             q.phase = kRefresh;
             q.first_refresh = true;
@@ -766,7 +781,10 @@ struct yasfhip_server {
             }
             // r = b - A x (FP64) into V[0]; V[1] holds A x
             double2 *ax = q->V + q->n;
-            launch_pair(dp, q->pos_d, q->N, q->k, q->x_d, false, q->Q, st);
+            if (q->table64)
+                launch_pair_table_gemm(dp, q->N, q->tab64, q->x_d, false, 0, q->Q, st);
+            else
+                launch_pair(dp, q->pos_d, q->N, q->k, q->x_d, false, q->Q, st);
             launch_contract(dp, q->N, q->Q, q->x_d, q->t_d, 1, ax, false, false, st);
             launch_residual(q->V, q->b_d, ax, q->n, reducer(), scal + 8 * q->slot, st);
             if (q->first_refresh)
