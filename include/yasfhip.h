@@ -198,10 +198,18 @@ int yasfhip_server_destroy(yasfhip_server *s);
  * T-matrix diagonal t[N*nmax] and incident coefficients a_inc[N*nmax]
  * (complex128, particle-major), relative tolerance, at most max_cycles
  * refinement cycles, and whether C_ext / C_sca are computed at completion
- * (as yasfhip_cross_sections).  The inputs are copied. */
+ * (as yasfhip_cross_sections).  The inputs are copied.
+ * operator_key != 0 declares that the operator I - T W (positions, lmax,
+ * wave number, T) is the same for every request with this key, e.g. the
+ * incidence directions of one cluster at one wavelength: they share one
+ * device copy of the geometry, T-matrix and pair-factor tables; only a_inc
+ * differs.  A key already in use with another N, lmax or wave number is an
+ * error; positions and T of later requests with the key are not read.
+ * A request whose complex64 table does not fit runs the on-the-fly pair GEMM. */
 int yasfhip_server_submit(yasfhip_server *s, const double *positions, int particle_number, int lmax,
                           double k_medium, const double *t, const double *a_inc, double rtol,
-                          int max_cycles, int observables, long long *request_id);
+                          int max_cycles, int observables, long long operator_key,
+                          long long *request_id);
 
 /* Admission, FP64 residuals of requests between cycles, completions, and one
  * batched Arnoldi iteration of every request inside a cycle.  Reports the
@@ -226,6 +234,12 @@ int yasfhip_server_poll(yasfhip_server *s, long long *request_id, double *x_out,
  * receives the mean times (-1 where not usable).  Needs an idle server. */
 int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, int reps, int *waves,
                             int *column_tiles, double *ms);
+
+/* keep_tables != 0 (default): operators admitted from now on keep resident
+ * pair-factor tables where they fit (complex64 and FP64 within a quarter of
+ * the arena, complex64 within half); 0: none, every GEMM generates the pair
+ * factors on the fly (for callers that leave most device memory to others). */
+int yasfhip_server_set_table_policy(yasfhip_server *s, int keep_tables);
 
 /* Fixes the tile shape used for lmax (one of the candidates above). */
 int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int column_tiles);

@@ -6,13 +6,15 @@ FP64 operator (itself pinned to the oracle at 1e-12): FP64 relative residual
 operator, and C_ext / C_sca within 1e-6 of a tight (1e-10) all-FP64 solve.
 Batch composition must not change any result: runs with and without the HIP
 graph agree bit for bit, and a small arena (forcing queueing and the on-the-fly
-FP64 residual) agrees to rounding."""
+FP64 residual) agrees to rounding.  Requests that share an operator by key give
+the same solutions bit for bit as unshared ones; without pair tables (on-the-fly
+complex64 GEMMs) the solutions still meet the FP64 criterion."""
 
 import numpy as np
 import pytest
 
 from yasf_amd import _native
-from yasf_amd.serving import make_request, serve, solve_sequential
+from yasf_amd.serving import make_aggregate_workload, make_request, serve, solve_sequential
 
 pytestmark = pytest.mark.gpu
 
@@ -139,3 +141,86 @@ def test_limits_and_errors():
     with pytest.raises(ValueError):
         _native.Server(restart=0)
     srv.close()
+
+
+@pytest.fixture(scope="module")
+def aggregate_requests():
+    # two aggregates x two wavelengths x three incidence directions
+    return make_aggregate_workload([48, 90], [450.0, 700.0], 3, seed=6)
+
+
+def _fp64_check(r, res, tol=1e-6):
+    s = _native.Session(r.positions, r.lmax, [r.k])
+    s.set_tmatrix(r.t[np.newaxis, :])
+    b = r.t * r.a_inc
+    rel = np.linalg.norm(b - s.master_matvec(res["x"])) / np.linalg.norm(b)
+    ce, cs = s.cross_sections(r.a_inc, res["x"])
+    s.close()
+    assert res["info"] == 0 and rel <= tol
+    assert res["c_ext"] == pytest.approx(ce, rel=1e-10)
+    assert res["c_sca"] == pytest.approx(cs, rel=1e-10)
+
+
+def test_shared_operators_match_unshared(aggregate_requests):
+    reqs = aggregate_requests
+    assert len({r.operator_key for r in reqs}) == 4 and len(reqs) == 12
+    srv = _native.Server(memory_bytes=2 << 30, restart=50, inner_rtol=1e-4)
+    shared = serve(srv, reqs, keep_solutions=True)
+    srv.close()
+    import copy
+    plain = [copy.copy(r) for r in reqs]
+    for r in plain:
+        r.operator_key = 0
+    srv = _native.Server(memory_bytes=2 << 30, restart=50, inner_rtol=1e-4)
+    unshared = serve(srv, plain, keep_solutions=True)
+    srv.close()
+    for r in reqs:
+        a, b = shared.results[r.index], unshared.results[r.index]
+        assert np.array_equal(a["x"], b["x"])
+        assert a["iterations"] == b["iterations"] and a["c_ext"] == b["c_ext"]
+        _fp64_check(r, a)
+    # directions of one (aggregate, wavelength) differ in solution, not operator
+    k0 = [r for r in reqs if r.operator_key == reqs[0].operator_key]
+    assert not np.array_equal(shared.results[k0[0].index]["x"], shared.results[k0[1].index]["x"])
+
+
+def test_operator_key_rules(aggregate_requests):
+    r0 = aggregate_requests[0]
+    other = next(r for r in aggregate_requests if r.particles != r0.particles)
+    srv = _native.Server(memory_bytes=1 << 30)
+    srv.submit(r0.positions, r0.lmax, r0.k, r0.t, r0.a_inc, operator_key=77)
+    with pytest.raises(ValueError):   # same key, different size
+        srv.submit(other.positions, other.lmax, other.k, other.t, other.a_inc, operator_key=77)
+    with pytest.raises(ValueError):   # same key, different wave number
+        srv.submit(r0.positions, r0.lmax, 1.01 * r0.k, r0.t, r0.a_inc, operator_key=77)
+    while srv.poll() is None:
+        srv.step()
+    # the key is free again once its requests have completed
+    rid = srv.submit(other.positions, other.lmax, other.k, other.t, other.a_inc, operator_key=77)
+    while True:
+        srv.step()
+        res = srv.poll()
+        if res is not None:
+            break
+    assert res["id"] == rid and res["info"] == 0
+    srv.close()
+
+
+def test_on_the_fly_pair_gemm(aggregate_requests, mixed_requests, served):
+    # no resident pair tables: every complex64 GEMM and FP64 residual
+    # generates its pair factors on the fly; keyed and unkeyed requests mixed
+    import copy
+    extra = [copy.copy(r) for r in mixed_requests[:3]]
+    for i, r in enumerate(extra):
+        r.index = 1000 + i
+    reqs = aggregate_requests[:6] + extra
+    srv = _native.Server(memory_bytes=2 << 30, restart=50, inner_rtol=1e-4)
+    srv.set_table_policy(False)
+    rep = serve(srv, reqs, keep_solutions=True)
+    srv.close()
+    for r in reqs:
+        _fp64_check(r, rep.results[r.index])
+    for r, e in zip(mixed_requests[:3], extra):
+        a, b = rep.results[e.index], served.results[r.index]
+        assert abs(a["c_ext"] - b["c_ext"]) <= 2e-6 * abs(b["c_ext"])
+        assert abs(a["c_sca"] - b["c_sca"]) <= 2e-6 * abs(b["c_sca"])

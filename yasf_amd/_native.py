@@ -67,11 +67,12 @@ def _load():
             "yasfhip_time_matvec": ([P, i, i, i, dp, dp], i),
             "yasfhip_server_create": ([ctypes.c_size_t, i, d, i, ctypes.POINTER(P)], i),
             "yasfhip_server_destroy": ([P], i),
-            "yasfhip_server_submit": ([P, P, i, i, d, P, P, d, i, i, llp], i),
+            "yasfhip_server_submit": ([P, P, i, i, d, P, P, d, i, i, ctypes.c_longlong, llp], i),
             "yasfhip_server_step": ([P, ip, ip, llp], i),
             "yasfhip_server_poll": ([P, llp, P, ctypes.c_longlong, ip, ip, ip, dp, dp, dp, dp], i),
             "yasfhip_server_autotune": ([P, i, i, i, ip, ip, P], i),
             "yasfhip_server_set_tile_shape": ([P, i, i, i], i),
+            "yasfhip_server_set_table_policy": ([P, i], i),
             "yasfhip_set_matvec_mode": ([P, i], i),
             "yasfhip_tune_matvec": ([P, i, i, i, ip, P], i),
         }
@@ -96,7 +97,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_far_field", "yasfhip_precond_build", "yasfhip_precond_apply",
         "yasfhip_time_matvec", "yasfhip_set_matvec_mode", "yasfhip_tune_matvec", "yasfhip_server_create",
         "yasfhip_server_destroy", "yasfhip_server_submit", "yasfhip_server_step", "yasfhip_server_poll",
-        "yasfhip_server_autotune", "yasfhip_server_set_tile_shape",
+        "yasfhip_server_autotune", "yasfhip_server_set_tile_shape", "yasfhip_server_set_table_policy",
     ]
 
 
@@ -313,7 +314,9 @@ class Server:
             pass
 
     def submit(self, positions, lmax: int, k_medium: float, t, a_inc, rtol: float = 1e-6,
-               max_cycles: int = 100, observables: bool = True) -> int:
+               max_cycles: int = 100, observables: bool = True, operator_key: int = 0) -> int:
+        """Queues one solve; requests with the same nonzero operator_key share
+        the operator (geometry, lmax, wave number, T-matrix) on the device."""
         pos = np.ascontiguousarray(positions, dtype=np.float64)
         n = pos.shape[0] * 2 * lmax * (lmax + 2)
         t = np.ascontiguousarray(t, dtype=np.complex128).ravel()
@@ -323,7 +326,7 @@ class Server:
         rid = ctypes.c_longlong()
         _check(self._lib.yasfhip_server_submit(self._h, _ptr(pos), pos.shape[0], int(lmax), float(k_medium),
                                                _ptr(t), _ptr(a), float(rtol), int(max_cycles),
-                                               int(bool(observables)), ctypes.byref(rid)))
+                                               int(bool(observables)), int(operator_key), ctypes.byref(rid)))
         self._sizes[rid.value] = n
         return rid.value
 
@@ -372,3 +375,7 @@ class Server:
 
     def set_tile_shape(self, lmax: int, waves: int, column_tiles: int = 2):
         _check(self._lib.yasfhip_server_set_tile_shape(self._h, int(lmax), int(waves), int(column_tiles)))
+
+    def set_table_policy(self, keep_tables: bool):
+        """Keep resident pair tables where they fit (default) or generate on the fly."""
+        _check(self._lib.yasfhip_server_set_table_policy(self._h, int(bool(keep_tables))))

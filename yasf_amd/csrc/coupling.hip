@@ -41,13 +41,15 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
                                                    int maxcols, int lmax, const double *__restrict__ la,
                                                    const double *__restrict__ lb,
                                                    const double *__restrict__ ld, T *__restrict__ Q, int nq,
-                                                   int stage_elems) {
+                                                   int stage_elems, const int *__restrict__ gate) {
     typedef typename Mfma<T>::acc_t acc_t;
     typedef typename Cplx<T>::type T2;
     constexpr int SRC = 16;  // sources per chunk (4 MFMA K-steps)
     extern __shared__ unsigned char lds_raw[];
     T2 *stage = reinterpret_cast<T2 *>(lds_raw);  // [2][stage_elems]: A [R][256] then B [16][ncols]
 
+    if (gate && *gate != 1)  // batched callers: run only while the request's state word is 1
+        return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool producer = wave >= kWavesPerJob;
     const int jid = blockIdx.x / ntiles1;
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q,
 
 template <typename T, bool Regular>
 static void launch_pair_t(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
-                          void *Q, hipStream_t s) {
+                          void *Q, hipStream_t s, const int *gate = nullptr) {
     const int ntiles1 = (N + 15) / 16;
     const size_t lds = 2 * static_cast<size_t>(dp.stage_elems) * sizeof(typename Cplx<T>::type);
     static bool lds_opt_in = false;  // gfx950: up to 160 KB of LDS per workgroup
@@ -216,16 +218,16 @@ static void launch_pair_t(const DevicePlan &dp, const double4 *pos, int N, doubl
     hipLaunchKernelGGL((pair_kernel<T, Regular>), dim3(dp.njobs * ntiles1), dim3(2 * kWavesPerJob * 64), lds, s,
                        pos, static_cast<const typename Cplx<T>::type *>(x), N, dp.nmax, k, dp.jobs,
                        ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.la, dp.lb, dp.ld, static_cast<T *>(Q),
-                       dp.nq, dp.stage_elems);
+                       dp.nq, dp.stage_elems, gate);
     YH_CHECK(hipGetLastError());
 }
 
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
-                 void *Q, hipStream_t s) {
+                 void *Q, hipStream_t s, const int *gate) {
     if (fp32)
-        launch_pair_t<float, false>(dp, pos, N, k, x, Q, s);
+        launch_pair_t<float, false>(dp, pos, N, k, x, Q, s, gate);
     else
-        launch_pair_t<double, false>(dp, pos, N, k, x, Q, s);
+        launch_pair_t<double, false>(dp, pos, N, k, x, Q, s, gate);
 }
 
 void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,

@@ -120,9 +120,10 @@ int set_last_error(const std::string &msg, int code);
 DevicePlan upload_device_plan(const CouplingPlan &plan, hipStream_t s);
 void free_device_plan(DevicePlan &dp);
 
-// On-the-fly pair GEMM (pair factors generated in the kernel).
+// On-the-fly pair GEMM (pair factors generated in the kernel).  gate (device
+// int, may be null): the kernel does nothing unless *gate == 1 at launch.
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
-                 void *Q, hipStream_t s);
+                 void *Q, hipStream_t s, const int *gate = nullptr);
 // Q of the regular (J-translation) operator, coincident pairs included, FP64.
 void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
                          void *Q, hipStream_t s);

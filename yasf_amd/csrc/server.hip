@@ -517,8 +517,26 @@ double now_s() {
 
 enum Phase { kQueued, kRefresh, kInCycle, kFinished };
 
+// Geometry, T-matrix and pair tables of one operator I - T W (one arena
+// block), shared by the requests submitted with the same operator key, e.g.
+// the incidence directions of one cluster at one wavelength.  The complex64
+// table feeds the batched GEMM (without it the request runs the on-the-fly
+// pair GEMM); the FP64 table, when present, the FP64 residuals.
+struct SharedOp {
+    long long key = 0;
+    int N = 0, L = 0, ntiles1 = 0, refs = 0;
+    long n = 0;
+    double k = 0.0;
+    void *block = nullptr;
+    double4 *pos_d = nullptr;
+    double2 *t_d = nullptr;
+    bool table32 = false, table64 = false;
+    float2 *table = nullptr;
+    double2 *tab64 = nullptr;
+};
+
 struct Request {
-    long long id = 0;
+    long long id = 0, key = 0;
     int N = 0, L = 0, nmax = 0, ntiles1 = 0, chunks = 0;
     long n = 0;
     double k = 0.0, rtol = 1e-6;
@@ -526,13 +544,11 @@ struct Request {
     bool observables = true;
     std::vector<double4> pos;
     std::vector<double2> t, a;
-    // device (one arena block)
+    SharedOp *op = nullptr;
+    // device (one arena block per request besides the shared operator)
     void *block = nullptr;
-    double4 *pos_d = nullptr;
-    double2 *t_d = nullptr, *a_d = nullptr, *b_d = nullptr, *x_d = nullptr, *y_d = nullptr;
-    float2 *table = nullptr, *x32 = nullptr;
-    bool table64 = false;          // resident FP64 table for the residual matvec
-    double2 *tab64 = nullptr;
+    double2 *a_d = nullptr, *b_d = nullptr, *x_d = nullptr, *y_d = nullptr;
+    float2 *x32 = nullptr;
     void *Q = nullptr;
     double2 *V = nullptr, *partial = nullptr, *H = nullptr, *h1 = nullptr, *h2 = nullptr, *g = nullptr,
             *gs = nullptr;
@@ -571,6 +587,15 @@ struct yasfhip_server {
     int shape_for_lmax[kMaxLmax + 1];  // index into kShape
     std::deque<Request *> queue;
     std::vector<Request *> active;
+    std::map<long long, SharedOp *> ops;  // operators by key, while referenced
+    struct KeyUse {
+        int N, L;
+        double k;
+        int uses;
+    };
+    std::map<long long, KeyUse> key_uses;  // keys of queued or active requests
+    std::vector<int> otf;                  // active requests without a complex64 table
+    bool tables_off = false;               // never keep pair tables (on-the-fly pair GEMMs)
     std::deque<Request *> finished;
     long long completed = 0;
     // batched launch data
@@ -596,6 +621,15 @@ struct yasfhip_server {
             (void)hipStreamSynchronize(stream);
         if (graph)
             (void)hipGraphExecDestroy(graph);
+        std::vector<SharedOp *> live;  // operators still referenced (their memory is the arena's)
+        for (auto *r : active)
+            if (r->op && std::find(live.begin(), live.end(), r->op) == live.end())
+                live.push_back(r->op);
+        for (auto &kv : ops)
+            if (std::find(live.begin(), live.end(), kv.second) == live.end())
+                live.push_back(kv.second);
+        for (auto *o : live)
+            delete o;
         for (auto *r : queue)
             delete r;
         for (auto *r : active)
@@ -631,22 +665,39 @@ struct yasfhip_server {
         return e;
     }
 
-    // Bytes of one request's arena block, with the carving of its buffers.
+    static size_t aligned(size_t &off, size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    }
+
+    // Bytes of an operator's arena block, with the carving of its buffers.
+    size_t layout_op(SharedOp &o, bool carve) {
+        const DevicePlan &dp = plan_for(o.L).dp;
+        size_t off = 0;
+        const size_t o_pos = aligned(off, sizeof(double4) * o.ntiles1 * 16);
+        const size_t o_t = aligned(off, sizeof(double2) * o.n);
+        const size_t o_tab = o.table32 ? aligned(off, pair_table_bytes(dp, o.N, true)) : 0;
+        const size_t o_tab64 = o.table64 ? aligned(off, pair_table_bytes(dp, o.N, false)) : 0;
+        if (carve) {
+            char *b = static_cast<char *>(o.block);
+            o.pos_d = reinterpret_cast<double4 *>(b + o_pos);
+            o.t_d = reinterpret_cast<double2 *>(b + o_t);
+            o.table = o.table32 ? reinterpret_cast<float2 *>(b + o_tab) : nullptr;
+            o.tab64 = o.table64 ? reinterpret_cast<double2 *>(b + o_tab64) : nullptr;
+        }
+        return off;
+    }
+
+    // Bytes of one request's own arena block, with the carving of its buffers.
     size_t layout(Request &q, bool carve) {
         const DevicePlan &dp = plan_for(q.L).dp;
         const size_t R1 = restart + 1;
         size_t off = 0;
-        auto take = [&](size_t bytes) {
-            const size_t o = off;
-            off += (bytes + 255) / 256 * 256;
-            return o;
-        };
-        const size_t o_pos = take(sizeof(double4) * q.ntiles1 * 16);
-        const size_t o_t = take(sizeof(double2) * q.n), o_a = take(sizeof(double2) * q.n);
+        auto take = [&](size_t bytes) { return aligned(off, bytes); };
+        const size_t o_a = take(sizeof(double2) * q.n);
         const size_t o_b = take(sizeof(double2) * q.n), o_x = take(sizeof(double2) * q.n);
         const size_t o_y = take(sizeof(double2) * R1);
-        const size_t o_tab = take(pair_table_bytes(dp, q.N, true));
-        const size_t o_tab64 = q.table64 ? take(pair_table_bytes(dp, q.N, false)) : 0;
         const size_t o_x32 = take(sizeof(float2) * q.n);
         const size_t o_Q = take(sizeof(double2) * q.ntiles1 * dp.nq * 256);
         const size_t o_V = take(sizeof(double2) * R1 * q.n);
@@ -658,14 +709,10 @@ struct yasfhip_server {
         const size_t o_gc = take(sizeof(double) * R1), o_gs = take(sizeof(double2) * R1);
         if (carve) {
             char *b = static_cast<char *>(q.block);
-            q.pos_d = reinterpret_cast<double4 *>(b + o_pos);
-            q.t_d = reinterpret_cast<double2 *>(b + o_t);
             q.a_d = reinterpret_cast<double2 *>(b + o_a);
             q.b_d = reinterpret_cast<double2 *>(b + o_b);
             q.x_d = reinterpret_cast<double2 *>(b + o_x);
             q.y_d = reinterpret_cast<double2 *>(b + o_y);
-            q.table = reinterpret_cast<float2 *>(b + o_tab);
-            q.tab64 = q.table64 ? reinterpret_cast<double2 *>(b + o_tab64) : nullptr;
             q.x32 = reinterpret_cast<float2 *>(b + o_x32);
             q.Q = b + o_Q;
             q.V = reinterpret_cast<double2 *>(b + o_V);
@@ -679,6 +726,54 @@ struct yasfhip_server {
             q.gs = reinterpret_cast<double2 *>(b + o_gs);
         }
         return off;
+    }
+
+    // A new operator for request q with the largest table set that fits:
+    // complex64 + FP64 tables when within a quarter of the arena, complex64
+    // within half of it, else none (on-the-fly pair GEMMs).  Null when not
+    // even the tableless block can be allocated now.
+    SharedOp *create_op(const Request &q) {
+        auto o = std::make_unique<SharedOp>();
+        o->key = q.key;
+        o->N = q.N;
+        o->L = q.L;
+        o->ntiles1 = q.ntiles1;
+        o->n = q.n;
+        o->k = q.k;
+        const size_t cap = arena.capacity();
+        const bool modes[3][2] = {{true, true}, {true, false}, {false, false}};
+        const size_t limit[3] = {cap / 4, cap / 2, cap};
+        for (int i = tables_off ? 2 : 0; i < 3 && !o->block; ++i) {
+            o->table32 = modes[i][0];
+            o->table64 = modes[i][1];
+            const size_t bytes = layout_op(*o, false);
+            if (bytes <= limit[i])
+                o->block = arena.alloc(bytes);
+        }
+        if (!o->block)
+            return nullptr;
+        layout_op(*o, true);
+        hipStream_t st = stream;
+        YH_CHECK(hipMemcpyAsync(o->pos_d, q.pos.data(), sizeof(double4) * q.pos.size(), hipMemcpyHostToDevice,
+                                st));
+        YH_CHECK(hipMemcpyAsync(o->t_d, q.t.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
+        const DevicePlan &dp = plan_for(q.L).dp;
+        if (o->table32)
+            launch_pair_table_build(dp, o->pos_d, o->N, o->k, true, o->table, st);
+        if (o->table64)
+            launch_pair_table_build(dp, o->pos_d, o->N, o->k, false, o->tab64, st);
+        if (o->key != 0)
+            ops[o->key] = o.get();
+        return o.release();
+    }
+
+    void release_op(SharedOp *o) {
+        if (--o->refs > 0)
+            return;
+        if (o->key != 0)
+            ops.erase(o->key);
+        arena.release(o->block);
+        delete o;
     }
 
     void ensure_scal(int count) {
@@ -713,41 +808,45 @@ struct yasfhip_server {
     void admit() {
         while (!queue.empty()) {
             Request &q = *queue.front();
-            // an FP64 table speeds up the FP64 residual of every cycle; it is
-            // kept only for requests that leave most of the arena to others
-            q.table64 = true;
-            const size_t with64 = layout(q, false);
-            q.block = with64 <= arena.capacity() / 4 ? arena.alloc(with64) : nullptr;
-            if (!q.block) {
-                q.table64 = false;
-                q.block = arena.alloc(layout(q, false));
+            SharedOp *op = nullptr;
+            bool created = false;
+            if (q.key != 0) {
+                auto it = ops.find(q.key);
+                if (it != ops.end())
+                    op = it->second;
             }
-            if (!q.block) {
+            if (!op) {
+                op = create_op(q);
+                created = op != nullptr;
+            }
+            if (op)
+                q.block = arena.alloc(layout(q, false));
+            if (!op || !q.block) {
+                if (created) {
+                    ++op->refs;
+                    release_op(op);
+                }
                 if (active.empty()) {  // can never fit: fail it
                     q.info = -1;
                     q.phase = kFinished;
                     q.t_done = now_s();
                     finished.push_back(&q);
                     queue.pop_front();
+                    drop_key(q.key);
                     ++completed;
                     continue;
                 }
                 return;
             }
+            ++op->refs;
+            q.op = op;
             layout(q, true);
             queue.pop_front();
             hipStream_t st = stream;
-            YH_CHECK(hipMemcpyAsync(q.pos_d, q.pos.data(), sizeof(double4) * q.pos.size(),
-                                    hipMemcpyHostToDevice, st));
-            YH_CHECK(hipMemcpyAsync(q.t_d, q.t.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
             YH_CHECK(hipMemcpyAsync(q.a_d, q.a.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL(serve_rhs_kernel, dim3(std::min<long>(1024, (q.n + 255) / 256)), dim3(256), 0,
-                               st, q.t_d, q.a_d, q.b_d, q.x_d, q.n);
+                               st, op->t_d, q.a_d, q.b_d, q.x_d, q.n);
             YH_CHECK(hipGetLastError());
-            const DevicePlan &dp = plan_for(q.L).dp;
-            launch_pair_table_build(dp, q.pos_d, q.N, q.k, true, q.table, st);
-            if (q.table64)
-                launch_pair_table_build(dp, q.pos_d, q.N, q.k, false, q.tab64, st);
             q.shape = shape_of(q.L);
             q.phase = kRefresh;
             q.first_refresh = true;
@@ -757,6 +856,14 @@ struct yasfhip_server {
             h_state[q.slot] = ReqState{kIdle, 0, 0, 0, 1.0, 0.0, 0.0};
             set_changed = true;
         }
+    }
+
+    void drop_key(long long key) {
+        if (key == 0)
+            return;
+        auto it = key_uses.find(key);
+        if (it != key_uses.end() && --it->second.uses == 0)
+            key_uses.erase(it);
     }
 
     // FP64 residual of every request waiting for one (after admission or a
@@ -781,11 +888,12 @@ struct yasfhip_server {
             }
             // r = b - A x (FP64) into V[0]; V[1] holds A x
             double2 *ax = q->V + q->n;
-            if (q->table64)
-                launch_pair_table_gemm(dp, q->N, q->tab64, q->x_d, false, 0, q->Q, st);
+            const SharedOp &o = *q->op;
+            if (o.table64)
+                launch_pair_table_gemm(dp, q->N, o.tab64, q->x_d, false, 0, q->Q, st);
             else
-                launch_pair(dp, q->pos_d, q->N, q->k, q->x_d, false, q->Q, st);
-            launch_contract(dp, q->N, q->Q, q->x_d, q->t_d, 1, ax, false, false, st);
+                launch_pair(dp, o.pos_d, q->N, q->k, q->x_d, false, q->Q, st);
+            launch_contract(dp, q->N, q->Q, q->x_d, o.t_d, 1, ax, false, false, st);
             launch_residual(q->V, q->b_d, ax, q->n, reducer(), scal + 8 * q->slot, st);
             if (q->first_refresh)
                 launch_norm2(q->b_d, q->n, reducer(), scal + 8 * q->slot + 1, st);
@@ -835,8 +943,8 @@ struct yasfhip_server {
                 continue;
             const DevicePlan &dp = plan_for(q->L).dp;
             double2 *y = q->V + q->n;
-            launch_pair_regular(dp, q->pos_d, q->N, q->k, q->x_d, q->Q, st);
-            launch_contract(dp, q->N, q->Q, q->x_d, q->t_d, 0, y, false, false, st);
+            launch_pair_regular(dp, q->op->pos_d, q->N, q->k, q->x_d, q->Q, st);
+            launch_contract(dp, q->N, q->Q, q->x_d, q->op->t_d, 0, y, false, false, st);
             launch_mgs_step(y, nullptr, nullptr, q->x_d, q->n, reducer(), scal + 8 * q->slot + 2, st);
             launch_mgs_step(q->x_d, nullptr, nullptr, q->a_d, q->n, reducer(), scal + 8 * q->slot + 5, st);
         }
@@ -858,6 +966,9 @@ struct yasfhip_server {
             q->t_done = t;
             arena.release(q->block);
             q->block = nullptr;
+            release_op(q->op);
+            q->op = nullptr;
+            drop_key(q->key);
             finished.push_back(q);
             ++completed;
         }
@@ -917,9 +1028,9 @@ struct yasfhip_server {
             Request &q = *active[r];
             const DevicePlan &dp = plan_for(q.L).dp;
             BatchReq &b = br[r];
-            b.pos = q.pos_d;
-            b.table = q.table;
-            b.t = q.t_d;
+            b.pos = q.op->pos_d;
+            b.table = q.op->table;
+            b.t = q.op->t_d;
             b.V = q.V;
             b.x32 = q.x32;
             b.Q = static_cast<float *>(q.Q);
@@ -962,7 +1073,7 @@ struct yasfhip_server {
             int cnt = 0, blocks = 0;
             for (int r = 0; r < nreq; ++r) {
                 Request &q = *active[r];
-                if (q.shape != gi)
+                if (q.shape != gi || !q.op->table32)
                     continue;
                 gstart[cnt] = blocks;
                 gidx[cnt] = r;
@@ -973,6 +1084,10 @@ struct yasfhip_server {
             grp_count[gi] = cnt;
             grp_blocks[gi] = blocks;
         }
+        otf.clear();
+        for (int r = 0; r < nreq; ++r)
+            if (!active[r]->op->table32)
+                otf.push_back(r);
         if (static_cast<int>(h_index.size()) > cap_index) {
             if (d_index)
                 YH_CHECK(hipFree(d_index));
@@ -1036,6 +1151,11 @@ struct yasfhip_server {
             const int *gidx = gstart + nreq + 1;
             launch_gemm(gi, gstart, gidx, grp_count[gi], grp_blocks[gi]);
         }
+        for (int r : otf) {  // requests without a complex64 table: on-the-fly pair GEMM, gated
+            const Request &q = *active[r];
+            launch_pair(plan_for(q.L).dp, q.op->pos_d, q.N, q.k, q.x32, true, q.Q, st,
+                        reinterpret_cast<const int *>(d_state + r));
+        }
         hipLaunchKernelGGL(serve_contract_kernel, dim3(tile_blocks), dim3(256), 0, st, d_reqs, d_state, tstart,
                            nreq);
         hipLaunchKernelGGL(serve_gs_kernel, dim3(chunk_blocks), dim3(kVecThreads), 0, st, d_reqs, d_state,
@@ -1075,18 +1195,22 @@ struct yasfhip_server {
         for (int i = 0; i < N; ++i)
             q.pos[i] = make_double4(250.0 * (i % side) + jitter(), 250.0 * ((i / side) % side) + jitter(),
                                     250.0 * (i / (side * side)) + jitter(), 0.0);
+        q.t.assign(q.n, make_double2(0.0, 0.0));
+        SharedOp *op = create_op(q);
+        YH_REQUIRE(op && op->table32, "autotune request does not fit in the arena");
+        ++op->refs;
+        q.op = op;
         q.block = arena.alloc(layout(q, false));
-        YH_REQUIRE(q.block, "autotune request does not fit in the arena");
+        if (!q.block) {
+            release_op(op);
+            YH_REQUIRE(false, "autotune request does not fit in the arena");
+        }
         layout(q, true);
         hipStream_t st = stream;
         std::vector<float2> xh(q.n);
         for (long i = 0; i < q.n; ++i)
             xh[i] = make_float2(static_cast<float>(std::sin(0.37 * i)), static_cast<float>(std::cos(0.21 * i)));
-        YH_CHECK(hipMemcpyAsync(q.pos_d, q.pos.data(), sizeof(double4) * q.pos.size(), hipMemcpyHostToDevice,
-                                st));
         YH_CHECK(hipMemcpyAsync(q.x32, xh.data(), sizeof(float2) * q.n, hipMemcpyHostToDevice, st));
-        const DevicePlan &dp = plan_for(L).dp;
-        launch_pair_table_build(dp, q.pos_d, N, q.k, true, q.table, st);
         int best = -1;
         double best_ms = 1e300;
         q.slot = 0;
@@ -1119,11 +1243,14 @@ struct yasfhip_server {
         } catch (...) {
             active.clear();
             arena.release(q.block);
+            release_op(op);
             set_changed = true;
             throw;
         }
+        YH_CHECK(hipStreamSynchronize(st));
         active.clear();
         arena.release(q.block);
+        release_op(op);
         set_changed = true;
         shape_for_lmax[L] = best;
         return best;
@@ -1225,7 +1352,7 @@ int yasfhip_server_destroy(yasfhip_server *s) {
 
 int yasfhip_server_submit(yasfhip_server *s, const double *positions, int particle_number, int lmax,
                           double k_medium, const double *t, const double *a_inc, double rtol, int max_cycles,
-                          int observables, long long *request_id) {
+                          int observables, long long operator_key, long long *request_id) {
     return sguard([&] {
         YH_REQUIRE(s && positions && t && a_inc && request_id, "NULL argument");
         YH_REQUIRE(particle_number >= 1, "particle_number must be >= 1");
@@ -1233,8 +1360,15 @@ int yasfhip_server_submit(yasfhip_server *s, const double *positions, int partic
         YH_REQUIRE(std::isfinite(k_medium) && k_medium > 0.0, "k_medium must be real, finite and positive");
         YH_REQUIRE(rtol > 0.0, "rtol must be > 0");
         YH_REQUIRE(max_cycles >= 1, "max_cycles must be >= 1");
+        if (operator_key != 0) {
+            auto it = s->key_uses.find(operator_key);
+            if (it != s->key_uses.end())
+                YH_REQUIRE(it->second.N == particle_number && it->second.L == lmax && it->second.k == k_medium,
+                           "operator_key already names an operator with a different size, lmax or wave number");
+        }
         auto q = std::make_unique<Request>();
         q->id = s->next_id++;
+        q->key = operator_key;
         q->N = particle_number;
         q->L = lmax;
         q->nmax = 2 * lmax * (lmax + 2);
@@ -1254,6 +1388,12 @@ int yasfhip_server_submit(yasfhip_server *s, const double *positions, int partic
         std::memcpy(q->a.data(), a_inc, sizeof(double2) * q->n);
         q->t_submit = now_s();
         *request_id = q->id;
+        if (operator_key != 0) {
+            auto &u = s->key_uses[operator_key];
+            if (u.uses == 0)
+                u = yasfhip_server::KeyUse{particle_number, lmax, k_medium, 0};
+            ++u.uses;
+        }
         s->queue.push_back(q.release());
     });
 }
@@ -1296,6 +1436,13 @@ int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int co
                    "unsupported tile shape for this lmax (waves 2/4/8 x column tiles 2/4/8 as listed in "
                    "yasfhip.h, with ceil((2 lmax + 1) / waves) orders per wave within range)");
         s->shape_for_lmax[lmax] = found;
+    });
+}
+
+int yasfhip_server_set_table_policy(yasfhip_server *s, int keep_tables) {
+    return sguard([&] {
+        YH_REQUIRE(s, "NULL argument");
+        s->tables_off = keep_tables == 0;
     });
 }
 

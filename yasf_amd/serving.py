@@ -43,6 +43,7 @@ class ServeRequest:
     polar: float
     azimuthal: float
     arrival_s: float = 0.0
+    operator_key: int = 0   # requests with the same nonzero key share one operator
 
     @property
     def particles(self) -> int:
@@ -87,6 +88,35 @@ def make_workload(count: int = 512, seed: int = 4, rate: float = math.inf, **kw)
     return reqs
 
 
+def make_aggregate_workload(particle_counts, wavelengths, directions: int, seed: int = 5,
+                            lmax: int = 4, m: complex = 1.7 + 0.1j, monomer_radius: float = 40.0,
+                            Df: float = 1.8, kf: float = 1.3) -> list[ServeRequest]:
+    """Fractal-aggregate requests: for every aggregate (one per entry of
+    particle_counts) and wavelength, `directions` plane waves (TE) from seeded
+    random directions; the requests of one (aggregate, wavelength) share their
+    operator key.  All arrive at the start."""
+    from yasf_amd.aggregates import fractal_aggregate
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for ai, N in enumerate(particle_counts):
+        pos, radii = fractal_aggregate(seed * 1000 + ai, int(N), Df, kf, monomer_radius)
+        p = Particles(pos, radii, np.full(int(N), m))
+        for wi, lam in enumerate(wavelengths):
+            omega = np.array([2 * math.pi / lam])
+            k = omega * 1.0
+            mie = mie_coefficients(lmax, p.unique_radius_index_pairs, omega, k)
+            t = np.ascontiguousarray(mie[p.single_unique_array_idx, :, 0]).ravel()
+            key = (ai + 1) * 100000 + wi + 1
+            for _ in range(directions):
+                beta = float(math.acos(rng.uniform(-1.0, 1.0)))
+                alpha = float(rng.uniform(0.0, 2 * math.pi))
+                a = np.ascontiguousarray(planewave_coefficients(lmax, pos, k, beta, alpha, 1)[:, :, 0]).ravel()
+                reqs.append(ServeRequest(len(reqs), pos, lmax, float(lam), float(k[0]), t, a, m, beta, alpha,
+                                         operator_key=key))
+    return reqs
+
+
 @dataclass
 class ServeReport:
     completed: int
@@ -118,7 +148,8 @@ def serve(server, requests: list[ServeRequest], rtol: float = 1e-6, keep_solutio
         now = time.perf_counter() - t0
         while nxt < total and order[nxt].arrival_s <= now:
             r = order[nxt]
-            ids[server.submit(r.positions, r.lmax, r.k, r.t, r.a_inc, rtol=rtol)] = r
+            ids[server.submit(r.positions, r.lmax, r.k, r.t, r.a_inc, rtol=rtol,
+                              operator_key=r.operator_key)] = r
             nxt += 1
         active, queued, _ = server.step()
         done_t = time.perf_counter() - t0
