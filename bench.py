@@ -28,6 +28,16 @@ incidence) through the batched mixed-precision server, all queued at the start
 (or Poisson arrivals with ``--rate``), against the all-FP64 single-solve path
 one request at a time; prints one JSON line with solves/s and latency
 percentiles of both.
+
+``--workload aggregates``: serving at scale on fractal aggregates (seeded
+cluster-cluster aggregation, Df 1.8, kf 1.3, monomer radius 40 nm, n = 1.7 +
+0.1i, N uniform in [1000, 20000] from seed 5), lmax 4; per aggregate a sample
+of the 64 wavelengths of linspace(350, 900) nm (``--wavelengths``, evenly
+spread) x ``--directions`` random incidence directions, each a request; the
+requests of one (aggregate, wavelength) share one operator on the device.
+Solves/s and latency percentiles of the batch, all queued at the start; a few
+requests are re-solved on the all-FP64 single-solve path for the error and a
+per-request FP64 time.
 """
 
 from __future__ import annotations
@@ -433,12 +443,83 @@ def run_serving(args):
     print(json.dumps(line), flush=True)
 
 
+def run_aggregates(args):
+    import torch
+
+    from yasf_amd import _native
+    from yasf_amd.serving import make_aggregate_workload, serve, solve_sequential
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    rng = np.random.default_rng(5)
+    counts = [int(rng.uniform(1000, 20000)) for _ in range(args.aggregates)]
+    if args.particles:
+        counts = [args.particles] * args.aggregates
+    grid = np.linspace(350.0, 900.0, 64)
+    pick = np.unique(np.round(np.linspace(0, 63, args.wavelengths)).astype(int))
+    lams = [float(grid[i]) for i in pick]
+    t0 = time.perf_counter()
+    reqs = make_aggregate_workload(counts, lams, args.directions, seed=5)
+    log(f"workload: aggregates N={counts}, wavelengths {[round(v, 1) for v in lams]} nm, "
+        f"{args.directions} directions: {len(reqs)} requests prepared in {time.perf_counter() - t0:.1f} s")
+    srv = _native.Server(restart=50, inner_rtol=1e-4)
+    (waves, ctiles), ms = srv.autotune(4, particles=600, reps=3)
+    log(f"autotuned lmax-4 tile shape: {waves} waves, {ctiles} column tiles ({ms})")
+    if args.warmup:
+        serve(srv, make_aggregate_workload([200], [500.0], 2, seed=99))
+    torch.cuda.synchronize()
+    rep = serve(srv, reqs, progress=log)
+    srv.close()
+    log(f"served: {rep.as_dict()}")
+    infos = [r["info"] for r in rep.results.values()]
+    line = {
+        "metric": METRIC,
+        "value": rep.solves_per_s,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * rep.makespan_s / len(reqs),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 pair GEMM inside FP64 refinement (fp32/fp64)",
+        "data": "synthetic (seeded fractal aggregates, seed 5)",
+        "config": {"workload": "serving at scale: fractal aggregates Df 1.8, kf 1.3, monomer 40 nm, "
+                               "n 1.7+0.1i, lmax 4; sampled wavelengths x incidence directions, rtol 1e-6",
+                   "particles": counts, "wavelengths_nm": lams, "directions": args.directions,
+                   "requests": len(reqs), "operators": len(counts) * len(lams),
+                   "restart": 50, "inner_rtol": 1e-4, "tile_shape": [waves, ctiles]},
+        "p50_latency_s": rep.p50_s,
+        "p99_latency_s": rep.p99_s,
+        "makespan_s": rep.makespan_s,
+        "converged": int(sum(1 for i in infos if i == 0)),
+        "mean_inner_iterations": float(np.mean([r["iterations"] for r in rep.results.values()])),
+    }
+    if args.fp64_check:
+        # first request of each operator on the all-FP64 single-solve path
+        firsts = {}
+        for r in reqs:
+            firsts.setdefault(r.operator_key, r)
+        sample = list(firsts.values())[: args.fp64_check]
+        base = solve_sequential(sample, progress=log)
+        line["fp64_single_solve"] = {
+            "requests": len(sample), "seconds_per_solve": base.makespan_s / len(sample),
+            "max_rel_diff_c_ext": max(abs(rep.results[r.index]["c_ext"] - base.results[r.index]["c_ext"])
+                                      / abs(base.results[r.index]["c_ext"]) for r in sample),
+            "max_rel_diff_c_sca": max(abs(rep.results[r.index]["c_sca"] - base.results[r.index]["c_sca"])
+                                      / abs(base.results[r.index]["c_sca"]) for r in sample)}
+        line["speedup_vs_fp64_single_solve"] = rep.solves_per_s * base.makespan_s / len(sample)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["solve", "precision", "serving"], default="solve")
+    ap.add_argument("--workload", choices=["solve", "precision", "serving", "aggregates"], default="solve")
     ap.add_argument("--no-precond", action="store_true")
     ap.add_argument("--per-box", type=float, default=8.0)
     ap.add_argument("--box-cap", type=int, default=8, help="bisect boxes above this many spheres")
@@ -450,10 +531,16 @@ def main():
     ap.add_argument("--requests", type=int, default=512, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=math.inf, help="serving: Poisson arrivals per second")
     ap.add_argument("--no-baseline", action="store_true", help="serving: skip the sequential FP64 run")
+    ap.add_argument("--aggregates", type=int, default=1, help="aggregates: number of aggregates")
+    ap.add_argument("--particles", type=int, default=0, help="aggregates: fixed N instead of the seeded draw")
+    ap.add_argument("--wavelengths", type=int, default=2, help="aggregates: wavelengths sampled of 64")
+    ap.add_argument("--directions", type=int, default=16, help="aggregates: incidence directions")
+    ap.add_argument("--fp64-check", type=int, default=1,
+                    help="aggregates: operators whose first request is re-solved in FP64 (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every run here uses a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print("bench.py measures one GPU: runs on more than one GPU are not supported "
+              "(the sharding plan is documented, not implemented).", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -464,6 +551,8 @@ def main():
     torch.cuda.set_device(0)
     if args.workload == "serving":
         run_serving(args)
+    elif args.workload == "aggregates":
+        run_aggregates(args)
     elif args.workload == "precision":
         run_precision(args)
     else:

@@ -39,11 +39,19 @@ enum {
 enum { YASFHIP_FP64 = 0, YASFHIP_FP32 = 1 };
 
 /* How the coupling matvec obtains the pair factors P_p^{|dm|} h_p e^{i dm phi}:
- * generated inside the matvec kernel, or generated once per session and
- * channel into a resident HBM table (ntiles*(2L+1)^2*ceil(N/4)*64 complex
- * values) that every later matvec streams.  AUTO uses the table when it needs
- * at most three quarters of the free device memory. */
-enum { YASFHIP_MATVEC_AUTO = 0, YASFHIP_MATVEC_ON_THE_FLY = 1, YASFHIP_MATVEC_TABLE = 2 };
+ * generated inside the matvec kernel (ON_THE_FLY); generated once per session
+ * and channel into a resident HBM table (ntiles*(2L+1)^2*ceil(N/4)*64 complex
+ * values) that every later matvec streams (TABLE); or, inside every matvec, a
+ * slab of that table (a range of 16-particle target tiles) built and streamed
+ * at a time (SLAB: same kernels as TABLE, memory of a slab only).  AUTO uses
+ * the table when it needs at most three quarters of the free device memory,
+ * else slabs of up to a quarter of it. */
+enum {
+    YASFHIP_MATVEC_AUTO = 0,
+    YASFHIP_MATVEC_ON_THE_FLY = 1,
+    YASFHIP_MATVEC_TABLE = 2,
+    YASFHIP_MATVEC_SLAB = 3
+};
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *yasfhip_last_error(void);
@@ -154,6 +162,10 @@ int yasfhip_precond_apply(yasfhip_session *s, const double *r, double *z);
  * yasfpy/lookups.py:124-171). */
 int yasfhip_set_matvec_mode(yasfhip_session *s, int mode);
 
+/* Target tiles per slab (SLAB and AUTO modes); 0 (default): as many as fit in
+ * a quarter of the free memory.  Frees the current slab. */
+int yasfhip_set_slab_tiles(yasfhip_session *s, int tiles);
+
 /* Tile-shape autotuner of the resident-table GEMM for one precision: times
  * `reps` launches with each column-tile grouping (jobs of at most 2, 4 or 8
  * 16-column tiles per dm -- 8 in FP32 only; wider groups stream each table
@@ -235,11 +247,15 @@ int yasfhip_server_poll(yasfhip_server *s, long long *request_id, double *x_out,
 int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, int reps, int *waves,
                             int *column_tiles, double *ms);
 
-/* keep_tables != 0 (default): operators admitted from now on keep resident
- * pair-factor tables where they fit (complex64 and FP64 within a quarter of
- * the arena, complex64 within half); 0: none, every GEMM generates the pair
- * factors on the fly (for callers that leave most device memory to others). */
-int yasfhip_server_set_table_policy(yasfhip_server *s, int keep_tables);
+/* Pair-factor memory of operators admitted from now on.  policy 1 (default):
+ * resident tables where they fit (complex64 and FP64 within a quarter of the
+ * arena, complex64 within half), else a slab of the table (a range of target
+ * tiles, at most an eighth of the arena) rebuilt range by range in every
+ * pair GEMM, once for all requests of the operator; 2: slabs only; 0: no
+ * tables or slabs, every GEMM generates the pair factors on the fly.
+ * slab_tiles > 0 fixes the FP64 16-particle target tiles per slab (complex64
+ * GEMMs use twice as many), 0 sizes slabs to the arena. */
+int yasfhip_server_set_table_policy(yasfhip_server *s, int policy, int slab_tiles);
 
 /* Fixes the tile shape used for lmax (one of the candidates above). */
 int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int column_tiles);

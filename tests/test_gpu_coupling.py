@@ -33,10 +33,17 @@ def test_fixture_matvec(golden):
     assert rel(y, R.coupling_matvec(L, d["positions"], k, d["x"])) < 1e-12
 
 
-MODES = [_native.MATVEC_ON_THE_FLY, _native.MATVEC_TABLE]
+MODES = [_native.MATVEC_ON_THE_FLY, _native.MATVEC_TABLE, _native.MATVEC_SLAB]
+MODE_IDS = ["on_the_fly", "table", "slab"]
 
 
-@pytest.mark.parametrize("mode", MODES, ids=["on_the_fly", "table"])
+def set_mode(s, mode, slab_tiles=1):
+    s.set_matvec_mode(mode)
+    if mode == _native.MATVEC_SLAB:
+        s.set_slab_tiles(slab_tiles)  # several slabs per matvec
+
+
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
 @pytest.mark.parametrize("lmax", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_matvec_all_orders(lmax, mode):
     rng = np.random.default_rng(100 + lmax)
@@ -45,7 +52,7 @@ def test_matvec_all_orders(lmax, mode):
     k = 2 * np.pi / 550.0
     x = rand_vec(rng, N * 2 * lmax * (lmax + 2))
     s = _native.Session(pos, lmax, [k])
-    s.set_matvec_mode(mode)
+    set_mode(s, mode)
     y = s.coupling_matvec(x)
     ref = R.coupling_matvec(lmax, pos, k, x)
     assert rel(y, ref) < 1e-12
@@ -68,7 +75,7 @@ def test_small_cluster_config():
     assert rel(s.coupling_matvec(x), R.coupling_matvec(4, pos, k, x)) < 1e-12
 
 
-@pytest.mark.parametrize("mode", MODES, ids=["on_the_fly", "table"])
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
 def test_full_size_sampled_rows(mode):
     """2000 spheres, lmax 6: full matvec on the GPU, oracle on sampled target rows."""
     pos, _ = cluster(2, 2000, 0.2)
@@ -76,7 +83,7 @@ def test_full_size_sampled_rows(mode):
     nmax = 96
     x = rand_vec(np.random.default_rng(2), 2000 * nmax)
     s = _native.Session(pos, 6, [k])
-    s.set_matvec_mode(mode)
+    set_mode(s, mode, slab_tiles=7)  # 125 target tiles: 18 slabs, the last one short
     y = s.coupling_matvec(x)
     for s1 in (0, 777, 1999):
         ref = R.coupling_matvec(6, pos, k, x, s1_lo=s1, n_s1=1)
@@ -95,10 +102,11 @@ def test_modes_agree_and_table_follows_channel():
     x = rand_vec(np.random.default_rng(8), 40 * 70)
     out = {}
     for mode in MODES:
-        s.set_matvec_mode(mode)
+        set_mode(s, mode)
         out[mode] = [s.coupling_matvec(x, channel=w) for w in (0, 1, 0)]
     for w in range(3):
         assert rel(out[MODES[1]][w], out[MODES[0]][w]) < 1e-13
+        assert np.array_equal(out[MODES[2]][w], out[MODES[1]][w])  # slab = table, bit for bit
     assert rel(out[MODES[1]][0], out[MODES[1]][1]) > 1e-3  # the channels differ
     with pytest.raises(ValueError):
         s.set_matvec_mode(7)
@@ -135,3 +143,30 @@ def test_table_groupings_agree(lmax):
         y = s.coupling_matvec(x, precision=prec)
         oracle_y = R.coupling_matvec(lmax, pos, k, x)
         assert rel(y, oracle_y) < (1e-12 if prec == _native.FP64 else 2e-5)
+
+
+def test_slab_matches_table_and_tunes():
+    """Slabs of any size give the resident table's result bit for bit in both
+    precisions (same build and GEMM per target tile); the tuner times the
+    first slab when no table is kept; AUTO sizes its own slab."""
+    pos, _ = cluster(31, 70, 0.15)   # 5 target tiles
+    k = 2 * np.pi / 480.0
+    L = 4
+    x = rand_vec(np.random.default_rng(31), 70 * 48)
+    s = _native.Session(pos, L, [k])
+    s.set_matvec_mode(_native.MATVEC_TABLE)
+    ref = {p: s.coupling_matvec(x, precision=p) for p in (_native.FP64, _native.FP32)}
+    assert rel(ref[_native.FP64], R.coupling_matvec(L, pos, k, x)) < 1e-12
+    s.set_matvec_mode(_native.MATVEC_SLAB)
+    for tiles in (1, 2, 4, 5, 0):
+        s.set_slab_tiles(tiles)
+        for p in (_native.FP64, _native.FP32):
+            assert np.array_equal(s.coupling_matvec(x, precision=p), ref[p]), (tiles, p)
+    s.set_slab_tiles(2)
+    for p, offered in ((_native.FP64, (0, 1)), (_native.FP32, (0, 1, 2))):
+        g, ms = s.tune_matvec(precision=p, reps=1)
+        assert g in offered and all((ms[i] > 0) == (i in offered) for i in range(3))
+        assert np.array_equal(s.coupling_matvec(x, precision=p), ref[p])
+    with pytest.raises(ValueError):
+        s.set_slab_tiles(-1)
+    s.close()

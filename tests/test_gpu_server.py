@@ -215,7 +215,7 @@ def test_on_the_fly_pair_gemm(aggregate_requests, mixed_requests, served):
         r.index = 1000 + i
     reqs = aggregate_requests[:6] + extra
     srv = _native.Server(memory_bytes=2 << 30, restart=50, inner_rtol=1e-4)
-    srv.set_table_policy(False)
+    srv.set_table_policy(_native.Server.TABLES_ON_THE_FLY)
     rep = serve(srv, reqs, keep_solutions=True)
     srv.close()
     for r in reqs:
@@ -224,3 +224,30 @@ def test_on_the_fly_pair_gemm(aggregate_requests, mixed_requests, served):
         a, b = rep.results[e.index], served.results[r.index]
         assert abs(a["c_ext"] - b["c_ext"]) <= 2e-6 * abs(b["c_ext"])
         assert abs(a["c_sca"] - b["c_sca"]) <= 2e-6 * abs(b["c_sca"])
+
+
+def test_slab_operators_match_tables(aggregate_requests, mixed_requests, served):
+    # slabs of two FP64 (four complex64) target tiles: several slabs per GEMM,
+    # one build per slab for all requests of an operator; same kernels per
+    # tile as the resident tables, so the same solutions bit for bit
+    import copy
+    extra = [copy.copy(r) for r in mixed_requests[:4]]
+    for i, r in enumerate(extra):
+        r.index = 2000 + i
+    reqs = aggregate_requests + extra
+    srv = _native.Server(memory_bytes=2 << 30, restart=50, inner_rtol=1e-4)
+    tab = serve(srv, reqs, keep_solutions=True)
+    srv.close()
+    srv = _native.Server(memory_bytes=2 << 30, restart=50, inner_rtol=1e-4)
+    srv.set_table_policy(_native.Server.TABLES_SLABS, slab_tiles=2)
+    slab = serve(srv, reqs, keep_solutions=True)
+    srv.close()
+    for r in reqs:
+        a, b = tab.results[r.index], slab.results[r.index]
+        assert b["info"] == 0
+        assert np.array_equal(a["x"], b["x"]) and a["iterations"] == b["iterations"]
+        assert a["c_ext"] == b["c_ext"] and a["c_sca"] == b["c_sca"]
+    for r, e in zip(mixed_requests[:4], extra):
+        assert np.array_equal(slab.results[e.index]["x"], served.results[r.index]["x"])
+    with pytest.raises(ValueError):
+        _native.Server(memory_bytes=1 << 30).set_table_policy(3)

@@ -21,6 +21,7 @@ FP32 = 1
 MATVEC_AUTO = 0
 MATVEC_ON_THE_FLY = 1
 MATVEC_TABLE = 2
+MATVEC_SLAB = 3
 
 _lib = None
 _lock = threading.Lock()
@@ -72,7 +73,8 @@ def _load():
             "yasfhip_server_poll": ([P, llp, P, ctypes.c_longlong, ip, ip, ip, dp, dp, dp, dp], i),
             "yasfhip_server_autotune": ([P, i, i, i, ip, ip, P], i),
             "yasfhip_server_set_tile_shape": ([P, i, i, i], i),
-            "yasfhip_server_set_table_policy": ([P, i], i),
+            "yasfhip_server_set_table_policy": ([P, i, i], i),
+            "yasfhip_set_slab_tiles": ([P, i], i),
             "yasfhip_set_matvec_mode": ([P, i], i),
             "yasfhip_tune_matvec": ([P, i, i, i, ip, P], i),
         }
@@ -97,7 +99,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_far_field", "yasfhip_precond_build", "yasfhip_precond_apply",
         "yasfhip_time_matvec", "yasfhip_set_matvec_mode", "yasfhip_tune_matvec", "yasfhip_server_create",
         "yasfhip_server_destroy", "yasfhip_server_submit", "yasfhip_server_step", "yasfhip_server_poll",
-        "yasfhip_server_autotune", "yasfhip_server_set_tile_shape", "yasfhip_server_set_table_policy",
+        "yasfhip_server_autotune", "yasfhip_server_set_tile_shape", "yasfhip_server_set_table_policy", "yasfhip_set_slab_tiles",
     ]
 
 
@@ -268,8 +270,14 @@ class Session:
         return z
 
     def set_matvec_mode(self, mode: int):
-        """MATVEC_AUTO, MATVEC_ON_THE_FLY or MATVEC_TABLE (resident pair-factor table)."""
+        """MATVEC_AUTO, MATVEC_ON_THE_FLY, MATVEC_TABLE (resident pair-factor
+        table) or MATVEC_SLAB (the table built and streamed a slab of target
+        tiles at a time inside every matvec)."""
         _check(self._lib.yasfhip_set_matvec_mode(self._h, int(mode)))
+
+    def set_slab_tiles(self, tiles: int):
+        """Target tiles per slab (0: as many as fit in a quarter of the free memory)."""
+        _check(self._lib.yasfhip_set_slab_tiles(self._h, int(tiles)))
 
     def tune_matvec(self, channel: int = 0, precision: int = FP64, reps: int = 3):
         """Picks the table GEMM's column-tile grouping; returns (index, ms per grouping 2/4/8)."""
@@ -376,6 +384,10 @@ class Server:
     def set_tile_shape(self, lmax: int, waves: int, column_tiles: int = 2):
         _check(self._lib.yasfhip_server_set_tile_shape(self._h, int(lmax), int(waves), int(column_tiles)))
 
-    def set_table_policy(self, keep_tables: bool):
-        """Keep resident pair tables where they fit (default) or generate on the fly."""
-        _check(self._lib.yasfhip_server_set_table_policy(self._h, int(bool(keep_tables))))
+    TABLES_ON_THE_FLY, TABLES_AUTO, TABLES_SLABS = 0, 1, 2
+
+    def set_table_policy(self, policy: int, slab_tiles: int = 0):
+        """TABLES_AUTO (default): resident pair tables where they fit, else
+        slabs; TABLES_SLABS: slabs only; TABLES_ON_THE_FLY: neither.
+        slab_tiles > 0 fixes the FP64 target tiles per slab."""
+        _check(self._lib.yasfhip_server_set_table_policy(self._h, int(policy), int(slab_tiles)))

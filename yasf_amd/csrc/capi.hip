@@ -124,6 +124,12 @@ struct yasfhip_session {
     int table_channel[2] = {-1, -1};
     bool table_refused[2] = {false, false};
     int table_grouping[2] = {0, 0};  // column-tile grouping of the table GEMM per precision
+    // Slab of the pair table (a few target tiles), rebuilt tile range by tile
+    // range inside every matvec when the full table does not fit.
+    void *slab = nullptr;
+    size_t slab_bytes = 0;
+    int slab_tiles_req = 0;  // tiles per slab fixed by the caller (0: what fits)
+    bool slab_refused = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 
     ~yasfhip_session() {
@@ -139,6 +145,8 @@ struct yasfhip_session {
         for (void *b : table)
             if (b)
                 (void)hipFree(b);
+        if (slab)
+            (void)hipFree(slab);
         if (scal_host)
             (void)hipHostFree(scal_host);
         if (hhost)
@@ -179,12 +187,57 @@ struct yasfhip_session {
         return true;
     }
 
+    // Target tiles per slab for this precision once the slab buffer exists;
+    // 0 when the matvec should generate the pair factors on the fly.  AUTO
+    // and SLAB size the buffer to a quarter of the free memory (capped at the
+    // full FP64 table); a caller-fixed tile count sizes it for FP64 tiles.
+    int slab_tiles(bool fp32) {
+        if (matvec_mode == YASFHIP_MATVEC_ON_THE_FLY || matvec_mode == YASFHIP_MATVEC_TABLE)
+            return 0;
+        const int ntiles1 = (N + 15) / 16;
+        if (!slab) {
+            if (slab_refused)
+                return 0;
+            const size_t tile64 = pair_table_tile_bytes(dp, N, false);
+            size_t bytes = 0;
+            if (slab_tiles_req > 0) {
+                bytes = tile64 * std::min(slab_tiles_req, ntiles1);
+            } else {
+                size_t free_b = 0, total_b = 0;
+                YH_CHECK(hipMemGetInfo(&free_b, &total_b));
+                bytes = std::min(free_b / 4, tile64 * ntiles1);
+                bytes = bytes / tile64 * tile64;
+            }
+            if (bytes < tile64) {
+                slab_refused = true;
+                return 0;
+            }
+            YH_CHECK(hipMalloc(&slab, bytes));
+            slab_bytes = bytes;
+        }
+        int tiles = static_cast<int>(slab_bytes / pair_table_tile_bytes(dp, N, fp32));
+        if (slab_tiles_req > 0)
+            tiles = std::min(tiles, slab_tiles_req);
+        return std::min(tiles, ntiles1);
+    }
+
     void pair_gemm(int channel, const void *xv, bool fp32) {
-        if (table_for(channel, fp32))
-            launch_pair_table_gemm(dp, N, table[fp32 ? 1 : 0], xv, fp32, table_grouping[fp32 ? 1 : 0], Q,
-                                   stream);
-        else
+        const int ix = fp32 ? 1 : 0;
+        if (matvec_mode != YASFHIP_MATVEC_SLAB && table_for(channel, fp32)) {
+            launch_pair_table_gemm(dp, N, table[ix], xv, fp32, table_grouping[ix], Q, stream);
+            return;
+        }
+        const int tiles = slab_tiles(fp32);
+        if (tiles == 0) {
             launch_pair(dp, pos, N, k[channel], xv, fp32, Q, stream);
+            return;
+        }
+        const int ntiles1 = (N + 15) / 16;
+        for (int t0 = 0; t0 < ntiles1; t0 += tiles) {
+            const int ntl = std::min(tiles, ntiles1 - t0);
+            launch_pair_table_build(dp, pos, N, k[channel], fp32, slab, stream, t0, ntl);
+            launch_pair_table_gemm(dp, N, slab, xv, fp32, table_grouping[ix], Q, stream, t0, ntl);
+        }
     }
 
     // y = W x (mode 0) or x - t W x (mode 1) for device vectors of complex128.
@@ -765,17 +818,37 @@ int yasfhip_set_matvec_mode(yasfhip_session *s, int mode) {
     return guarded([&] {
         YH_REQUIRE(s, "NULL argument");
         YH_REQUIRE(mode == YASFHIP_MATVEC_AUTO || mode == YASFHIP_MATVEC_ON_THE_FLY ||
-                       mode == YASFHIP_MATVEC_TABLE,
+                       mode == YASFHIP_MATVEC_TABLE || mode == YASFHIP_MATVEC_SLAB,
                    "unknown matvec mode");
         s->matvec_mode = mode;
-        if (mode == YASFHIP_MATVEC_ON_THE_FLY)
+        if (mode == YASFHIP_MATVEC_ON_THE_FLY || mode == YASFHIP_MATVEC_SLAB)
             for (int ix = 0; ix < 2; ++ix)
                 if (s->table[ix]) {
                     YH_CHECK(hipFree(s->table[ix]));
                     s->table[ix] = nullptr;
                     s->table_channel[ix] = -1;
                 }
+        if ((mode == YASFHIP_MATVEC_ON_THE_FLY || mode == YASFHIP_MATVEC_TABLE) && s->slab) {
+            YH_CHECK(hipFree(s->slab));
+            s->slab = nullptr;
+            s->slab_bytes = 0;
+        }
         s->table_refused[0] = s->table_refused[1] = false;
+        s->slab_refused = false;
+    });
+}
+
+int yasfhip_set_slab_tiles(yasfhip_session *s, int tiles) {
+    return guarded([&] {
+        YH_REQUIRE(s, "NULL argument");
+        YH_REQUIRE(tiles >= 0, "tiles must be >= 0");
+        YH_CHECK(hipStreamSynchronize(s->stream));
+        if (s->slab)
+            YH_CHECK(hipFree(s->slab));
+        s->slab = nullptr;
+        s->slab_bytes = 0;
+        s->slab_refused = false;
+        s->slab_tiles_req = tiles;
     });
 }
 
@@ -792,8 +865,18 @@ int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps
             ms[g] = -1.0;
         *grouping = s->table_grouping[ix];
         hipStream_t st = s->stream;
-        if (!s->table_for(channel, fp32))
-            return;  // no resident table for this session: nothing to tune
+        // the resident table, else the first slab of one (tile range [0, ntl))
+        const void *tab = nullptr;
+        int ntl = -1;
+        if (s->matvec_mode != YASFHIP_MATVEC_SLAB && s->table_for(channel, fp32)) {
+            tab = s->table[ix];
+        } else if (const int tiles = s->slab_tiles(fp32)) {
+            ntl = tiles;
+            launch_pair_table_build(s->dp, s->pos, s->N, s->k[channel], fp32, s->slab, st, 0, ntl);
+            tab = s->slab;
+        } else {
+            return;  // pair factors on the fly: nothing to tune
+        }
         const void *xv = s->xin;
         if (fp32) {
             launch_to_fp32(reinterpret_cast<float *>(s->x32), reinterpret_cast<const double *>(s->xin),
@@ -804,10 +887,10 @@ int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps
         for (int g = 0; g < kTableGroupings; ++g) {
             if (!fp32 && kTableGrouping[g] > 4)
                 continue;  // not offered in FP64 (register capacity)
-            launch_pair_table_gemm(s->dp, s->N, s->table[ix], xv, fp32, g, s->Q, st);  // warm-up
+            launch_pair_table_gemm(s->dp, s->N, tab, xv, fp32, g, s->Q, st, 0, ntl);  // warm-up
             YH_CHECK(hipEventRecord(s->ev[0], st));
             for (int r = 0; r < reps; ++r)
-                launch_pair_table_gemm(s->dp, s->N, s->table[ix], xv, fp32, g, s->Q, st);
+                launch_pair_table_gemm(s->dp, s->N, tab, xv, fp32, g, s->Q, st, 0, ntl);
             YH_CHECK(hipEventRecord(s->ev[1], st));
             YH_CHECK(hipEventSynchronize(s->ev[1]));
             float t = 0.f;

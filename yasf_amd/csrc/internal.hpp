@@ -131,11 +131,16 @@ void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
                      int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s);
 // Resident pair-factor table: bytes, build, and the streaming pair GEMM.
+// Both work on target tiles [t0, t0 + ntl) (ntl < 0: to the last tile) of a
+// table whose pointer is at tile t0, so a slab of a few tiles can stand in
+// for a table too large to keep: build the slab, multiply, next slab.
 size_t pair_table_bytes(const DevicePlan &dp, int N, bool fp32);
+size_t pair_table_tile_bytes(const DevicePlan &dp, int N, bool fp32);  // one target tile
 void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,
-                             void *table, hipStream_t s);
+                             void *table, hipStream_t s, int t0 = 0, int ntl = -1,
+                             const int *gate = nullptr);
 // grouping: index into kTableGrouping (column tiles per job).
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
-                            int grouping, void *Q, hipStream_t s);
+                            int grouping, void *Q, hipStream_t s, int t0 = 0, int ntl = -1);
 
 }  // namespace yasfhip

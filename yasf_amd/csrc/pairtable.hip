@@ -17,11 +17,16 @@ namespace yasfhip {
 namespace {
 
 // One block per (target tile, dm, 16-source chunk); thread = one pair.
+// Builds target tiles t0.. of the table (table points at tile t0: a slab of
+// the full table when t0 > 0 or fewer tiles are launched).  gate: run only
+// while *gate == 1 (batched callers), null: always.
 template <typename T2>
 __global__ __launch_bounds__(256) void pair_table_build_kernel(
     const double4 *__restrict__ pos, int N, int N4, double k, int lmax, int ndm,
     const int *__restrict__ dmoff, const double *__restrict__ la, const double *__restrict__ lb,
-    const double *__restrict__ ld, T2 *__restrict__ table) {
+    const double *__restrict__ ld, T2 *__restrict__ table, int t0, const int *__restrict__ gate) {
+    if (gate && *gate != 1)
+        return;
     const int nchunk = (N + 15) / 16;
     const int c = blockIdx.x % nchunk;
     const int d = (blockIdx.x / nchunk) % ndm;
@@ -31,7 +36,7 @@ __global__ __launch_bounds__(256) void pair_table_build_kernel(
     const int dm = d - 2 * lmax;
     const int R = P - (dm < 0 ? -dm : dm);
     const int r = threadIdx.x & 15, kk = threadIdx.x >> 4;
-    const int s1 = t * 16 + r, s2 = c * 16 + kk;
+    const int s1 = (t0 + t) * 16 + r, s2 = c * 16 + kk;
     const int ks = s2 >> 2;
     if (ks >= N4)
         return;
@@ -49,15 +54,16 @@ constexpr int kTableWaves = 8;  // waves per (job, target tile) block
 // One block of 8 waves per (job, target tile).  Wave w owns the orders
 // ip = w, w+8, ... (< R) of the job and all of its (up to NTLMAX) column
 // tiles, so each table fragment it loads feeds all of the job's columns.
+// The grid covers ntl target tiles from t0 (table points at tile t0).
 template <typename T, int NIPMAX, int NTLMAX>
 __global__ __launch_bounds__(512) void pair_table_gemm_kernel(
     const typename Cplx<T>::type *__restrict__ table, const typename Cplx<T>::type *__restrict__ x, int N,
-    int N4, int nmax, const PairJob *__restrict__ jobs, int ntiles1, const int *__restrict__ colmap,
+    int N4, int nmax, const PairJob *__restrict__ jobs, int ntl, int t0, const int *__restrict__ colmap,
     int maxcols, int lmax, const int *__restrict__ dmoff, T *__restrict__ Q, int nq) {
     typedef typename Cplx<T>::type T2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int jid = blockIdx.x / ntiles1;
-    const int t = blockIdx.x - jid * ntiles1;
+    const int jid = blockIdx.x / ntl;
+    const int t = blockIdx.x - jid * ntl;  // tile within the slab
     const int dm = uniform(jobs[jid].dm);
     const int tile0 = uniform(jobs[jid].tile0);
     const int ntile = uniform(jobs[jid].ntile);
@@ -76,7 +82,7 @@ __global__ __launch_bounds__(512) void pair_table_gemm_kernel(
 #pragma unroll
     for (int it = 0; it < kTableMaxColTiles; ++it)
         n2c[it] = (it < ntile) ? colmap[d * maxcols + (tile0 + it) * 16 + lo] : -1;
-    T *Qt = Q + 2 * (static_cast<size_t>(t) * nq + qbase) * 256;
+    T *Qt = Q + 2 * (static_cast<size_t>(t0 + t) * nq + qbase) * 256;
     // complex128 operands: a 3-deep pipeline; complex64: 2-deep keeps more waves resident
     constexpr int kDepth = sizeof(T) == sizeof(double) ? 3 : 2;
     table_gemm_dispatch<T, 1, NIPMAX, NTLMAX, kTableWaves, kDepth>(tab, x, N, N4, nmax, R, wave, nip, ntile,
@@ -84,39 +90,40 @@ __global__ __launch_bounds__(512) void pair_table_gemm_kernel(
 }
 
 template <typename T, int NTLMAX>
-void gemm_g(const DevicePlan &dp, int g, int N, const void *table, const void *x, void *Q, hipStream_t s) {
+void gemm_g(const DevicePlan &dp, int g, int N, const void *table, const void *x, void *Q, hipStream_t s,
+            int t0, int ntl) {
     typedef typename Cplx<T>::type T2;
-    const int ntiles1 = (N + 15) / 16;
     const int N4 = (N + 3) / 4;
-    const dim3 grid(dp.ntjobs[g] * ntiles1), block(64 * kTableWaves);
+    const dim3 grid(dp.ntjobs[g] * ntl), block(64 * kTableWaves);
     const T2 *tb = static_cast<const T2 *>(table);
     const T2 *xv = static_cast<const T2 *>(x);
     T *q = static_cast<T *>(Q);
     // orders per wave: ceil(R/8) <= ceil((2L+1)/8), 3 only at lmax 8
     if (2 * dp.lmax + 1 <= 16)
         hipLaunchKernelGGL((pair_table_gemm_kernel<T, 2, NTLMAX>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
-                           dp.tjobs[g], ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
+                           dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
     else
         hipLaunchKernelGGL((pair_table_gemm_kernel<T, 3, NTLMAX>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
-                           dp.tjobs[g], ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
+                           dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
     YH_CHECK(hipGetLastError());
 }
 
 template <typename T>
-void gemm_t(const DevicePlan &dp, int g, int N, const void *table, const void *x, void *Q, hipStream_t s) {
+void gemm_t(const DevicePlan &dp, int g, int N, const void *table, const void *x, void *Q, hipStream_t s,
+            int t0, int ntl) {
     static_assert(kTableGrouping[0] == 2 && kTableGrouping[1] == 4 && kTableGrouping[2] == 8,
                   "instances follow the column-tile groupings");
     switch (g) {
     case 0:
-        gemm_g<T, 2>(dp, g, N, table, x, Q, s);
+        gemm_g<T, 2>(dp, g, N, table, x, Q, s, t0, ntl);
         break;
     case 1:
-        gemm_g<T, 4>(dp, g, N, table, x, Q, s);
+        gemm_g<T, 4>(dp, g, N, table, x, Q, s, t0, ntl);
         break;
     default:
         // complex128 accumulators of 8 column tiles do not fit a wave's registers
         if constexpr (sizeof(T) == sizeof(float))
-            gemm_g<T, 8>(dp, 2, N, table, x, Q, s);
+            gemm_g<T, 8>(dp, 2, N, table, x, Q, s, t0, ntl);
         else
             YH_REQUIRE(false, "the FP64 table GEMM supports jobs of at most 4 column tiles");
         break;
@@ -131,27 +138,40 @@ size_t pair_table_bytes(const DevicePlan &dp, int N, bool fp32) {
     return ntiles1 * P * P * N4 * 64 * (fp32 ? sizeof(float2) : sizeof(double2));
 }
 
+size_t pair_table_tile_bytes(const DevicePlan &dp, int N, bool fp32) {
+    const size_t P = 2 * dp.lmax + 1;
+    const size_t N4 = (N + 3) / 4;
+    return P * P * N4 * 64 * (fp32 ? sizeof(float2) : sizeof(double2));
+}
+
 void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,
-                             void *table, hipStream_t s) {
+                             void *table, hipStream_t s, int t0, int ntl, const int *gate) {
     const int ntiles1 = (N + 15) / 16, N4 = (N + 3) / 4, nchunk = (N + 15) / 16;
+    if (ntl < 0)
+        ntl = ntiles1 - t0;
+    YH_REQUIRE(t0 >= 0 && ntl >= 1 && t0 + ntl <= ntiles1, "table tile range outside the targets");
     const int ndm = 4 * dp.lmax + 1;
-    const dim3 grid(ntiles1 * ndm * nchunk), block(256);
+    const dim3 grid(ntl * ndm * nchunk), block(256);
     if (fp32)
         hipLaunchKernelGGL(pair_table_build_kernel<float2>, grid, block, 0, s, pos, N, N4, k, dp.lmax,
-                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<float2 *>(table));
+                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<float2 *>(table), t0, gate);
     else
         hipLaunchKernelGGL(pair_table_build_kernel<double2>, grid, block, 0, s, pos, N, N4, k, dp.lmax,
-                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<double2 *>(table));
+                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<double2 *>(table), t0, gate);
     YH_CHECK(hipGetLastError());
 }
 
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
-                            int grouping, void *Q, hipStream_t s) {
+                            int grouping, void *Q, hipStream_t s, int t0, int ntl) {
     YH_REQUIRE(grouping >= 0 && grouping < kTableGroupings, "unknown column-tile grouping");
+    const int ntiles1 = (N + 15) / 16;
+    if (ntl < 0)
+        ntl = ntiles1 - t0;
+    YH_REQUIRE(t0 >= 0 && ntl >= 1 && t0 + ntl <= ntiles1, "table tile range outside the targets");
     if (fp32)
-        gemm_t<float>(dp, grouping, N, table, x, Q, s);
+        gemm_t<float>(dp, grouping, N, table, x, Q, s, t0, ntl);
     else
-        gemm_t<double>(dp, grouping, N, table, x, Q, s);
+        gemm_t<double>(dp, grouping, N, table, x, Q, s, t0, ntl);
 }
 
 }  // namespace yasfhip

@@ -150,24 +150,11 @@ __global__ __launch_bounds__(kVecThreads) void serve_prep_kernel(const BatchReq 
     }
 }
 
-// Complex64 pair GEMM over the resident tables of a group of requests that
-// share one tile shape (WAVES waves per (job, target tile) block, jobs of at
-// most NTLMAX column tiles).
+// One (job, target tile t) block of a request's complex64 pair GEMM; tab
+// points at target tile t of the request's pair table (or of a slab of it).
 template <int WAVES, int NIPMAX, int NTLMAX>
-__global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *__restrict__ reqs,
-                                                                const ReqState *__restrict__ st,
-                                                                const int *__restrict__ gstart,
-                                                                const int *__restrict__ gidx, int ngrp) {
-    const int gi = find_owner(gstart, ngrp, blockIdx.x);
-    const int r = uniform(gidx[gi]);
-    if (st[r].state != kArnoldi)
-        return;
-    const BatchReq &q = reqs[r];
+__device__ __forceinline__ void serve_gemm_block(const BatchReq &q, int jid, int t, const float2 *tabtile) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int b = blockIdx.x - gstart[gi];
-    const int ntiles1 = q.ntiles1;
-    const int jid = b / ntiles1;
-    const int t = b - jid * ntiles1;
     const int dm = uniform(q.jobs[jid].dm);
     const int tile0 = uniform(q.jobs[jid].tile0);
     const int ntile = uniform(q.jobs[jid].ntile);
@@ -181,7 +168,7 @@ __global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *
     const int nip = nord > wave ? (nord - wave + WAVES - 1) / WAVES : 0;
     if (nip == 0)
         return;
-    const float2 *tab = q.table + (static_cast<size_t>(t) * P * P + q.dmoff[d]) * q.N4 * 64 + lane;
+    const float2 *tab = tabtile + static_cast<size_t>(q.dmoff[d]) * q.N4 * 64 + lane;
     int n2c[kTableMaxColTiles];
 #pragma unroll
     for (int it = 0; it < kTableMaxColTiles; ++it)
@@ -189,6 +176,60 @@ __global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *
     float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
     table_gemm_dispatch<float, 1, NIPMAX, NTLMAX, WAVES, 3>(tab, q.x32, q.N, q.N4, q.nmax, R, wave, nip, ntile,
                                                             n2c, Qt, tstride, tile0);
+}
+
+// Complex64 pair GEMM over the resident tables of a group of requests that
+// share one tile shape (WAVES waves per (job, target tile) block, jobs of at
+// most NTLMAX column tiles).
+template <int WAVES, int NIPMAX, int NTLMAX>
+__global__ __launch_bounds__(64 * WAVES) void serve_gemm_kernel(const BatchReq *__restrict__ reqs,
+                                                                const ReqState *__restrict__ st,
+                                                                const int *__restrict__ gstart,
+                                                                const int *__restrict__ gidx, int ngrp) {
+    const int gi = find_owner(gstart, ngrp, blockIdx.x);
+    const int r = uniform(gidx[gi]);
+    if (st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    const int b = blockIdx.x - gstart[gi];
+    const int jid = b / q.ntiles1;
+    const int t = b - jid * q.ntiles1;
+    const int P = 2 * q.lmax + 1;
+    serve_gemm_block<WAVES, NIPMAX, NTLMAX>(q, jid, t, q.table + static_cast<size_t>(t) * P * P * q.N4 * 64);
+}
+
+// The same GEMM for the nr requests ridx[] of one operator over a slab of its
+// pair table (target tiles t0 .. t0+ntl-1, slab at tile t0).  Block order:
+// request fastest, so the requests' blocks of one (job, tile) run together
+// and share that slice of the slab in the caches.
+template <int WAVES, int NIPMAX, int NTLMAX>
+__global__ __launch_bounds__(64 * WAVES) void serve_slab_gemm_kernel(const BatchReq *__restrict__ reqs,
+                                                                     const ReqState *__restrict__ st,
+                                                                     const int *__restrict__ ridx, int nr,
+                                                                     const float2 *__restrict__ slab, int t0,
+                                                                     int ntl) {
+    const int ri = blockIdx.x % nr;
+    const int rest = blockIdx.x / nr;
+    const int tl = rest % ntl;
+    const int jid = rest / ntl;
+    const int r = uniform(ridx[ri]);
+    if (st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    const int P = 2 * q.lmax + 1;
+    serve_gemm_block<WAVES, NIPMAX, NTLMAX>(q, jid, t0 + tl, slab + static_cast<size_t>(tl) * P * P * q.N4 * 64);
+}
+
+// flag = 1 when any of the nr requests ridx[] is inside a cycle, else 0
+// (gates the slab builds of an operator).
+__global__ void serve_any_active_kernel(const ReqState *__restrict__ st, const int *__restrict__ ridx, int nr,
+                                        int *__restrict__ flag) {
+    int any = 0;
+    for (int i = threadIdx.x; i < nr; i += blockDim.x)
+        any |= st[ridx[i]].state == kArnoldi;
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0)
+        *flag = any;
 }
 
 // V[j+1] = V[j] - t (W V[j]) from the complex64 Q, FP64 accumulation.
@@ -533,6 +574,18 @@ struct SharedOp {
     bool table32 = false, table64 = false;
     float2 *table = nullptr;
     double2 *tab64 = nullptr;
+    // without tables: a slab of slab_tiles64 FP64 target tiles (2x as many
+    // complex64 tiles), rebuilt range by range in every pair GEMM
+    void *slab = nullptr;
+    int slab_tiles64 = 0;
+};
+
+// The active requests of one slab operator, as laid out in the index buffer.
+struct SlabGroup {
+    SharedOp *op;
+    int shape;         // tile shape of these requests (index into kShape)
+    int first, count;  // d_index offset of the request indices, their number
+    int flag;          // index of the operator's gate word in d_flags
 };
 
 struct Request {
@@ -594,8 +647,14 @@ struct yasfhip_server {
         int uses;
     };
     std::map<long long, KeyUse> key_uses;  // keys of queued or active requests
-    std::vector<int> otf;                  // active requests without a complex64 table
-    bool tables_off = false;               // never keep pair tables (on-the-fly pair GEMMs)
+    std::vector<int> otf;                  // active requests without a complex64 table or slab
+    std::vector<SlabGroup> slab_groups;    // active requests of slab operators, by operator
+    int *d_flags = nullptr;                // gate words of the slab groups
+    int cap_flags = 0;
+    // 1: resident tables where they fit, else slabs; 2: slabs only; 0: pair
+    // factors on the fly only
+    int table_policy = 1;
+    int slab_tiles_req = 0;  // FP64 target tiles per slab (0: an eighth of the arena)
     std::deque<Request *> finished;
     long long completed = 0;
     // batched launch data
@@ -638,7 +697,7 @@ struct yasfhip_server {
             delete r;
         for (auto &p : plans)
             free_device_plan(p.second.dp);
-        void *bufs[] = {d_reqs, d_state, d_index, scal, partials, counter};
+        void *bufs[] = {d_reqs, d_state, d_index, d_flags, scal, partials, counter};
         for (void *b : bufs)
             if (b)
                 (void)hipFree(b);
@@ -679,12 +738,15 @@ struct yasfhip_server {
         const size_t o_t = aligned(off, sizeof(double2) * o.n);
         const size_t o_tab = o.table32 ? aligned(off, pair_table_bytes(dp, o.N, true)) : 0;
         const size_t o_tab64 = o.table64 ? aligned(off, pair_table_bytes(dp, o.N, false)) : 0;
+        const size_t o_slab =
+            o.slab_tiles64 ? aligned(off, pair_table_tile_bytes(dp, o.N, false) * o.slab_tiles64) : 0;
         if (carve) {
             char *b = static_cast<char *>(o.block);
             o.pos_d = reinterpret_cast<double4 *>(b + o_pos);
             o.t_d = reinterpret_cast<double2 *>(b + o_t);
             o.table = o.table32 ? reinterpret_cast<float2 *>(b + o_tab) : nullptr;
             o.tab64 = o.table64 ? reinterpret_cast<double2 *>(b + o_tab64) : nullptr;
+            o.slab = o.slab_tiles64 ? b + o_slab : nullptr;
         }
         return off;
     }
@@ -730,8 +792,9 @@ struct yasfhip_server {
 
     // A new operator for request q with the largest table set that fits:
     // complex64 + FP64 tables when within a quarter of the arena, complex64
-    // within half of it, else none (on-the-fly pair GEMMs).  Null when not
-    // even the tableless block can be allocated now.
+    // within half of it, else a slab of at most an eighth of it, else none
+    // (on-the-fly pair GEMMs).  Null when not even the tableless block can be
+    // allocated now.
     SharedOp *create_op(const Request &q) {
         auto o = std::make_unique<SharedOp>();
         o->key = q.key;
@@ -741,11 +804,26 @@ struct yasfhip_server {
         o->n = q.n;
         o->k = q.k;
         const size_t cap = arena.capacity();
-        const bool modes[3][2] = {{true, true}, {true, false}, {false, false}};
-        const size_t limit[3] = {cap / 4, cap / 2, cap};
-        for (int i = tables_off ? 2 : 0; i < 3 && !o->block; ++i) {
-            o->table32 = modes[i][0];
-            o->table64 = modes[i][1];
+        const DevicePlan &dpl = plan_for(q.L).dp;
+        const int ntiles1 = (q.N + 15) / 16;
+        const size_t tile64 = pair_table_tile_bytes(dpl, q.N, false);
+        int slab64 = slab_tiles_req > 0 ? slab_tiles_req : static_cast<int>(std::min<size_t>(cap / 8 / tile64, 1 << 20));
+        slab64 = std::min(slab64, (ntiles1 + 1) / 2);  // 2x as many complex64 tiles cover all targets
+        // {complex64 table, FP64 table, FP64 slab tiles}, each within a share of the arena
+        const int modes[4][3] = {{1, 1, 0}, {1, 0, 0}, {0, 0, slab64}, {0, 0, 0}};
+        const size_t limit[4] = {cap / 4, cap / 2, cap, cap};  // (the slab is sized within an eighth)
+        const int first = table_policy == 1 ? 0 : (table_policy == 2 ? 2 : 3);
+        for (int i = first; i < 4 && !o->block; ++i) {
+            o->table32 = modes[i][0] != 0;
+            o->table64 = modes[i][1] != 0;
+            o->slab_tiles64 = modes[i][2];
+            if (i == 2) {  // the largest slab the arena can give now, halving down to one tile
+                for (; o->slab_tiles64 >= 1 && !o->block; o->slab_tiles64 /= 2)
+                    o->block = arena.alloc(layout_op(*o, false));
+                if (o->block)
+                    o->slab_tiles64 *= 2;  // undo the loop's last halving
+                continue;
+            }
             const size_t bytes = layout_op(*o, false);
             if (bytes <= limit[i])
                 o->block = arena.alloc(bytes);
@@ -878,7 +956,6 @@ struct yasfhip_server {
         ensure_scal(static_cast<int>(active.size()));
         hipStream_t st = stream;
         for (auto *q : todo) {
-            const DevicePlan &dp = plan_for(q->L).dp;
             if (q->cycle_ended) {
                 const int m = h_state[q->slot].j;
                 hipLaunchKernelGGL(serve_backsolve_kernel, dim3(1), dim3(1), 0, st, q->H, q->g, m, restart + 1,
@@ -886,13 +963,40 @@ struct yasfhip_server {
                 YH_CHECK(hipGetLastError());
                 launch_update(q->x_d, q->V, q->y_d, m, q->n, q->n, st);
             }
+        }
+        // FP64 pair GEMMs of A x: resident FP64 table, slab (one build per
+        // slab for all of an operator's requests here), or on the fly
+        std::vector<char> gemm_done(todo.size(), 0);
+        for (size_t i = 0; i < todo.size(); ++i) {
+            Request *q = todo[i];
+            const SharedOp &o = *q->op;
+            const DevicePlan &dp = plan_for(q->L).dp;
+            if (gemm_done[i])
+                continue;
+            if (o.table64) {
+                launch_pair_table_gemm(dp, q->N, o.tab64, q->x_d, false, 0, q->Q, st);
+            } else if (o.slab) {
+                for (int t0 = 0; t0 < o.ntiles1; t0 += o.slab_tiles64) {
+                    const int ntl = std::min(o.slab_tiles64, o.ntiles1 - t0);
+                    launch_pair_table_build(dp, o.pos_d, o.N, o.k, false, o.slab, st, t0, ntl);
+                    for (size_t j = i; j < todo.size(); ++j)
+                        if (todo[j]->op == q->op)
+                            launch_pair_table_gemm(dp, q->N, o.slab, todo[j]->x_d, false, 0, todo[j]->Q, st, t0,
+                                                   ntl);
+                }
+                for (size_t j = i; j < todo.size(); ++j)
+                    if (todo[j]->op == q->op)
+                        gemm_done[j] = 1;
+            } else {
+                launch_pair(dp, o.pos_d, q->N, q->k, q->x_d, false, q->Q, st);
+            }
+            gemm_done[i] = 1;
+        }
+        for (auto *q : todo) {
+            const DevicePlan &dp = plan_for(q->L).dp;
             // r = b - A x (FP64) into V[0]; V[1] holds A x
             double2 *ax = q->V + q->n;
             const SharedOp &o = *q->op;
-            if (o.table64)
-                launch_pair_table_gemm(dp, q->N, o.tab64, q->x_d, false, 0, q->Q, st);
-            else
-                launch_pair(dp, o.pos_d, q->N, q->k, q->x_d, false, q->Q, st);
             launch_contract(dp, q->N, q->Q, q->x_d, o.t_d, 1, ax, false, false, st);
             launch_residual(q->V, q->b_d, ax, q->n, reducer(), scal + 8 * q->slot, st);
             if (q->first_refresh)
@@ -1085,9 +1189,35 @@ struct yasfhip_server {
             grp_blocks[gi] = blocks;
         }
         otf.clear();
-        for (int r = 0; r < nreq; ++r)
-            if (!active[r]->op->table32)
+        slab_groups.clear();
+        for (int r = 0; r < nreq; ++r) {
+            SharedOp *o = active[r]->op;
+            if (o->table32)
+                continue;
+            if (!o->slab) {
                 otf.push_back(r);
+                continue;
+            }
+            const int shape = active[r]->shape;
+            bool seen = false;
+            for (const auto &g : slab_groups)
+                seen |= g.op == o && g.shape == shape;
+            if (seen)
+                continue;
+            SlabGroup g{o, shape, static_cast<int>(h_index.size()), 0, static_cast<int>(slab_groups.size())};
+            for (int r2 = r; r2 < nreq; ++r2)
+                if (active[r2]->op == o && active[r2]->shape == shape) {
+                    h_index.push_back(r2);
+                    ++g.count;
+                }
+            slab_groups.push_back(g);
+        }
+        if (static_cast<int>(slab_groups.size()) > cap_flags) {
+            if (d_flags)
+                YH_CHECK(hipFree(d_flags));
+            cap_flags = 2 * static_cast<int>(slab_groups.size());
+            YH_CHECK(hipMalloc(&d_flags, sizeof(int) * cap_flags));
+        }
         if (static_cast<int>(h_index.size()) > cap_index) {
             if (d_index)
                 YH_CHECK(hipFree(d_index));
@@ -1137,6 +1267,44 @@ struct yasfhip_server {
         YH_CHECK(hipGetLastError());
     }
 
+    // The complex64 GEMM of a slab group's requests over target tiles
+    // [t0, t0 + ntl) of the slab.
+    void launch_slab_gemm(const SlabGroup &g, int t0, int ntl) {
+        hipStream_t st = stream;
+        const SharedOp &o = *g.op;
+        const int shape = g.shape;
+        const int blocks = plan_for(o.L).dp.ntjobs[kShape[shape].grouping] * ntl * g.count;
+        const int *ridx = d_index + g.first;
+        const float2 *slab = static_cast<const float2 *>(o.slab);
+        switch (shape) {
+        case 0:
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<2, 5, 2>), dim3(blocks), dim3(128), 0, st, d_reqs, d_state,
+                               ridx, g.count, slab, t0, ntl);
+            break;
+        case 1:
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<4, 5, 2>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
+                               ridx, g.count, slab, t0, ntl);
+            break;
+        case 2:
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<8, 3, 2>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+                               ridx, g.count, slab, t0, ntl);
+            break;
+        case 3:
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<4, 5, 4>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
+                               ridx, g.count, slab, t0, ntl);
+            break;
+        case 4:
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<8, 3, 4>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+                               ridx, g.count, slab, t0, ntl);
+            break;
+        default:
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<8, 3, 8>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+                               ridx, g.count, slab, t0, ntl);
+            break;
+        }
+        YH_CHECK(hipGetLastError());
+    }
+
     // The batched Arnoldi iteration (see the file comment).
     void launch_iteration() {
         const int nreq = static_cast<int>(active.size());
@@ -1150,6 +1318,20 @@ struct yasfhip_server {
             const int *gstart = d_index + 2 * (nreq + 1) + gi * 2 * (nreq + 1);
             const int *gidx = gstart + nreq + 1;
             launch_gemm(gi, gstart, gidx, grp_count[gi], grp_blocks[gi]);
+        }
+        for (const auto &g : slab_groups) {  // slab operators: build a slab once, multiply for all requests
+            const SharedOp &o = *g.op;
+            const DevicePlan &dp = plan_for(o.L).dp;
+            int *flag = d_flags + g.flag;
+            hipLaunchKernelGGL(serve_any_active_kernel, dim3(1), dim3(64), 0, st, d_state, d_index + g.first,
+                               g.count, flag);
+            YH_CHECK(hipGetLastError());
+            const int tiles = 2 * o.slab_tiles64;
+            for (int t0 = 0; t0 < o.ntiles1; t0 += tiles) {
+                const int ntl = std::min(tiles, o.ntiles1 - t0);
+                launch_pair_table_build(dp, o.pos_d, o.N, o.k, true, o.slab, st, t0, ntl, flag);
+                launch_slab_gemm(g, t0, ntl);
+            }
         }
         for (int r : otf) {  // requests without a complex64 table: on-the-fly pair GEMM, gated
             const Request &q = *active[r];
@@ -1196,8 +1378,22 @@ struct yasfhip_server {
             q.pos[i] = make_double4(250.0 * (i % side) + jitter(), 250.0 * ((i / side) % side) + jitter(),
                                     250.0 * (i / (side * side)) + jitter(), 0.0);
         q.t.assign(q.n, make_double2(0.0, 0.0));
-        SharedOp *op = create_op(q);
-        YH_REQUIRE(op && op->table32, "autotune request does not fit in the arena");
+        const int policy = table_policy;
+        table_policy = 1;  // the tuner times the resident-table GEMM
+        SharedOp *op = nullptr;
+        try {
+            op = create_op(q);
+        } catch (...) {
+            table_policy = policy;
+            throw;
+        }
+        table_policy = policy;
+        if (op && !op->table32) {
+            ++op->refs;
+            release_op(op);
+            op = nullptr;
+        }
+        YH_REQUIRE(op, "autotune request does not fit in the arena");
         ++op->refs;
         q.op = op;
         q.block = arena.alloc(layout(q, false));
@@ -1439,10 +1635,13 @@ int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int co
     });
 }
 
-int yasfhip_server_set_table_policy(yasfhip_server *s, int keep_tables) {
+int yasfhip_server_set_table_policy(yasfhip_server *s, int policy, int slab_tiles) {
     return sguard([&] {
         YH_REQUIRE(s, "NULL argument");
-        s->tables_off = keep_tables == 0;
+        YH_REQUIRE(policy >= 0 && policy <= 2, "unknown table policy");
+        YH_REQUIRE(slab_tiles >= 0, "slab_tiles must be >= 0");
+        s->table_policy = policy;
+        s->slab_tiles_req = slab_tiles;
     });
 }
 
