@@ -392,8 +392,10 @@ def run_serving(args):
     srv = _native.Server(restart=50, inner_rtol=1e-4)
     tiles = {}
     for L in range(3, 9):
-        (waves, ctiles), ms = srv.autotune(L, particles=600, reps=3)
-        tiles[L] = {"waves": waves, "column_tiles": ctiles, "ms": [round(float(v), 4) for v in ms]}
+        tiles[L] = {}
+        for cls, particles in (("small", 400), ("large", 2000)):
+            (waves, ctiles), ms = srv.autotune(L, particles=particles, reps=3)
+            tiles[L][cls] = {"waves": waves, "column_tiles": ctiles, "ms": [round(float(v), 4) for v in ms]}
     log(f"autotuned tile shapes: {tiles}")
     if args.warmup:
         serve(srv, make_workload(8, seed=99, n_range=(20, 200)))
@@ -464,8 +466,8 @@ def run_aggregates(args):
     log(f"workload: aggregates N={counts}, wavelengths {[round(v, 1) for v in lams]} nm, "
         f"{args.directions} directions: {len(reqs)} requests prepared in {time.perf_counter() - t0:.1f} s")
     srv = _native.Server(restart=50, inner_rtol=1e-4)
-    (waves, ctiles), ms = srv.autotune(4, particles=600, reps=3)
-    log(f"autotuned lmax-4 tile shape: {waves} waves, {ctiles} column tiles ({ms})")
+    (waves, ctiles), ms = srv.autotune(4, particles=3000, reps=3)
+    log(f"autotuned lmax-4 tile shape (large requests): {waves} waves, {ctiles} column tiles ({ms})")
     if args.warmup:
         serve(srv, make_aggregate_workload([200], [500.0], 2, seed=99))
     torch.cuda.synchronize()

@@ -241,23 +241,30 @@ int yasfhip_server_poll(yasfhip_server *s, long long *request_id, double *x_out,
  * (waves per (job, target tile) block, column tiles per job); the candidates
  * are, in this order, (2, 2), (4, 2), (8, 2), (4, 4), (8, 4), (8, 8), each
  * usable where ceil((2 lmax + 1) / waves) orders per wave stay within 5 (2 or
- * 4 waves) or 3 (8 waves).  Times every usable shape on a synthetic cluster of
- * particle_number spheres, keeps the fastest for lmax and returns it; ms[6]
- * receives the mean times (-1 where not usable).  Needs an idle server. */
+ * 4 waves) or 3 (8 waves).  Shapes are kept per lmax and size class
+ * (requests of fewer than 1000 particles, and of at least 1000).  Times every
+ * usable shape on a synthetic cluster of particle_number spheres, keeps the
+ * fastest for lmax and that size class and returns it; ms[6] receives the
+ * mean times (-1 where not usable).  Needs an idle server. */
 int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, int reps, int *waves,
                             int *column_tiles, double *ms);
 
 /* Pair-factor memory of operators admitted from now on.  policy 1 (default):
  * resident tables where they fit (complex64 and FP64 within a quarter of the
- * arena, complex64 within half), else a slab of the table (a range of target
+ * arena; complex64 within half, with an FP64 slab of up to a sixteenth for
+ * the FP64 residual and C_sca), else a slab of the table (a range of target
  * tiles, at most an eighth of the arena) rebuilt range by range in every
  * pair GEMM, once for all requests of the operator; 2: slabs only; 0: no
- * tables or slabs, every GEMM generates the pair factors on the fly.
+ * tables or slabs, every GEMM generates the pair factors on the fly.  The
+ * complex64 GEMMs of requests sharing an operator run as one launch whose
+ * blocks interleave the requests, so each table slice is streamed once for
+ * all of them.
  * slab_tiles > 0 fixes the FP64 16-particle target tiles per slab (complex64
  * GEMMs use twice as many), 0 sizes slabs to the arena. */
 int yasfhip_server_set_table_policy(yasfhip_server *s, int policy, int slab_tiles);
 
-/* Fixes the tile shape used for lmax (one of the candidates above). */
+/* Fixes the tile shape used for lmax, both size classes (one of the
+ * candidates above). */
 int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int column_tiles);
 
 #ifdef __cplusplus

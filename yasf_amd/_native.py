@@ -373,7 +373,10 @@ class Server:
     TILE_SHAPES = ((2, 2), (4, 2), (8, 2), (4, 4), (8, 4), (8, 8))
 
     def autotune(self, lmax: int, particles: int = 600, reps: int = 5):
-        """((waves, column tiles) chosen, ms per candidate of TILE_SHAPES; -1 where not usable)."""
+        """Times every tile shape on a synthetic request of `particles` spheres
+        and keeps the fastest for lmax and that request size class (fewer than
+        1000 particles, or at least 1000).  Returns ((waves, column tiles)
+        chosen, ms per candidate of TILE_SHAPES; -1 where not usable)."""
         w = ctypes.c_int()
         ct = ctypes.c_int()
         ms = np.zeros(len(self.TILE_SHAPES))

@@ -136,9 +136,10 @@ void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, 
 // for a table too large to keep: build the slab, multiply, next slab.
 size_t pair_table_bytes(const DevicePlan &dp, int N, bool fp32);
 size_t pair_table_tile_bytes(const DevicePlan &dp, int N, bool fp32);  // one target tile
+// regular: factors of the regular (J) translation (FP64 only).
 void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,
                              void *table, hipStream_t s, int t0 = 0, int ntl = -1,
-                             const int *gate = nullptr);
+                             const int *gate = nullptr, bool regular = false);
 // grouping: index into kTableGrouping (column tiles per job).
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
                             int grouping, void *Q, hipStream_t s, int t0 = 0, int ntl = -1);

@@ -20,7 +20,9 @@ namespace {
 // Builds target tiles t0.. of the table (table points at tile t0: a slab of
 // the full table when t0 > 0 or fewer tiles are launched).  gate: run only
 // while *gate == 1 (batched callers), null: always.
-template <typename T2>
+// Regular: factors of the regular (J) translation, j_p in place of h_p and
+// coincident pairs included.
+template <typename T2, bool Regular>
 __global__ __launch_bounds__(256) void pair_table_build_kernel(
     const double4 *__restrict__ pos, int N, int N4, double k, int lmax, int ndm,
     const int *__restrict__ dmoff, const double *__restrict__ la, const double *__restrict__ lb,
@@ -41,12 +43,15 @@ __global__ __launch_bounds__(256) void pair_table_build_kernel(
     if (ks >= N4)
         return;
     const int lane = (s2 & 3) * 16 + r;
-    const bool valid = (s1 < N) && (s2 < N) && (s1 != s2);
+    const bool valid = (s1 < N) && (s2 < N) && (Regular || s1 != s2);
     const double4 p1 = pos[s1];  // pos is padded to a multiple of 16
     const double4 p2 = pos[s2 < N ? s2 : 0];
     T2 *dst = table + ((static_cast<size_t>(t) * SR + dmoff[d]) * N4 + static_cast<size_t>(ks) * R) * 64 +
               lane;
-    pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld, dst, 64);
+    if constexpr (Regular)
+        pair_series_regular<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld, dst, 64);
+    else
+        pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld, dst, 64);
 }
 
 constexpr int kTableWaves = 8;  // waves per (job, target tile) block
@@ -145,18 +150,22 @@ size_t pair_table_tile_bytes(const DevicePlan &dp, int N, bool fp32) {
 }
 
 void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,
-                             void *table, hipStream_t s, int t0, int ntl, const int *gate) {
+                             void *table, hipStream_t s, int t0, int ntl, const int *gate, bool regular) {
     const int ntiles1 = (N + 15) / 16, N4 = (N + 3) / 4, nchunk = (N + 15) / 16;
     if (ntl < 0)
         ntl = ntiles1 - t0;
     YH_REQUIRE(t0 >= 0 && ntl >= 1 && t0 + ntl <= ntiles1, "table tile range outside the targets");
     const int ndm = 4 * dp.lmax + 1;
     const dim3 grid(ntl * ndm * nchunk), block(256);
-    if (fp32)
-        hipLaunchKernelGGL(pair_table_build_kernel<float2>, grid, block, 0, s, pos, N, N4, k, dp.lmax,
+    YH_REQUIRE(!(regular && fp32), "the regular table is FP64 only");
+    if (regular)
+        hipLaunchKernelGGL((pair_table_build_kernel<double2, true>), grid, block, 0, s, pos, N, N4, k, dp.lmax,
+                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<double2 *>(table), t0, gate);
+    else if (fp32)
+        hipLaunchKernelGGL((pair_table_build_kernel<float2, false>), grid, block, 0, s, pos, N, N4, k, dp.lmax,
                            ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<float2 *>(table), t0, gate);
     else
-        hipLaunchKernelGGL(pair_table_build_kernel<double2>, grid, block, 0, s, pos, N, N4, k, dp.lmax,
+        hipLaunchKernelGGL((pair_table_build_kernel<double2, false>), grid, block, 0, s, pos, N, N4, k, dp.lmax,
                            ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<double2 *>(table), t0, gate);
     YH_CHECK(hipGetLastError());
 }

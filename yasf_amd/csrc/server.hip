@@ -69,6 +69,7 @@ struct TileShape {
 constexpr int kShapes = 6;
 constexpr TileShape kShape[kShapes] = {{2, 0}, {4, 0}, {8, 0}, {4, 1}, {8, 1}, {8, 2}};
 constexpr int kMaxGemmGroups = kShapes;
+constexpr int kLargeRequest = 1000;  // particles from which a request uses the large-size tile shape
 
 enum : int { kIdle = 0, kArnoldi = 1, kCycleDone = 2 };
 
@@ -580,7 +581,8 @@ struct SharedOp {
     int slab_tiles64 = 0;
 };
 
-// The active requests of one slab operator, as laid out in the index buffer.
+// The active requests of one operator (and tile shape) whose GEMMs run as one
+// launch: a slab operator, or a table operator shared by several requests.
 struct SlabGroup {
     SharedOp *op;
     int shape;         // tile shape of these requests (index into kShape)
@@ -637,7 +639,10 @@ struct yasfhip_server {
     bool use_graph = true;
     long long next_id = 1;
     std::map<int, PlanEntry> plans;
-    int shape_for_lmax[kMaxLmax + 1];  // index into kShape
+    // index into kShape per lmax and request size class (fewer / at least
+    // kLargeRequest particles: small GEMMs want more waves per block, large
+    // ones fewer)
+    int shape_for_lmax[kMaxLmax + 1][2];
     std::deque<Request *> queue;
     std::vector<Request *> active;
     std::map<long long, SharedOp *> ops;  // operators by key, while referenced
@@ -648,7 +653,7 @@ struct yasfhip_server {
     };
     std::map<long long, KeyUse> key_uses;  // keys of queued or active requests
     std::vector<int> otf;                  // active requests without a complex64 table or slab
-    std::vector<SlabGroup> slab_groups;    // active requests of slab operators, by operator
+    std::vector<SlabGroup> slab_groups;    // requests sharing a slab operator, or a table one (>= 2)
     int *d_flags = nullptr;                // gate words of the slab groups
     int cap_flags = 0;
     // 1: resident tables where they fit, else slabs; 2: slabs only; 0: pair
@@ -807,21 +812,35 @@ struct yasfhip_server {
         const DevicePlan &dpl = plan_for(q.L).dp;
         const int ntiles1 = (q.N + 15) / 16;
         const size_t tile64 = pair_table_tile_bytes(dpl, q.N, false);
-        int slab64 = slab_tiles_req > 0 ? slab_tiles_req : static_cast<int>(std::min<size_t>(cap / 8 / tile64, 1 << 20));
+        int slab64 =
+            slab_tiles_req > 0 ? slab_tiles_req : static_cast<int>(std::min<size_t>(cap / 8 / tile64, 1 << 20));
         slab64 = std::min(slab64, (ntiles1 + 1) / 2);  // 2x as many complex64 tiles cover all targets
-        // {complex64 table, FP64 table, FP64 slab tiles}, each within a share of the arena
-        const int modes[4][3] = {{1, 1, 0}, {1, 0, 0}, {0, 0, slab64}, {0, 0, 0}};
-        const size_t limit[4] = {cap / 4, cap / 2, cap, cap};  // (the slab is sized within an eighth)
+        // {complex64 table, FP64 table, FP64 slab tiles}, each within a share
+        // of the arena; with a complex64 table but no FP64 one, a slab of up
+        // to a sixteenth of the arena serves the FP64 residual and C_sca
+        const int slab64s = slab_tiles_req > 0
+                                ? slab64
+                                : std::min(slab64, static_cast<int>(std::min<size_t>(cap / 16 / tile64, 1 << 20)));
+        const int modes[4][3] = {{1, 1, 0}, {1, 0, slab64s}, {0, 0, slab64}, {0, 0, 0}};
+        const size_t limit[4] = {cap / 4, cap / 2 + tile64 * slab64s, cap, cap};
         const int first = table_policy == 1 ? 0 : (table_policy == 2 ? 2 : 3);
         for (int i = first; i < 4 && !o->block; ++i) {
             o->table32 = modes[i][0] != 0;
             o->table64 = modes[i][1] != 0;
             o->slab_tiles64 = modes[i][2];
-            if (i == 2) {  // the largest slab the arena can give now, halving down to one tile
-                for (; o->slab_tiles64 >= 1 && !o->block; o->slab_tiles64 /= 2)
+            if (i == 1 && o->slab_tiles64 < 1)
+                o->slab_tiles64 = 0;
+            if (i == 1 || i == 2) {  // the largest slab the arena can give now, halving down to one tile
+                if (i == 1 && layout_op(*o, false) > limit[1])
+                    continue;
+                for (int tiles = o->slab_tiles64; tiles >= 1 && !o->block; tiles /= 2) {
+                    o->slab_tiles64 = tiles;
                     o->block = arena.alloc(layout_op(*o, false));
-                if (o->block)
-                    o->slab_tiles64 *= 2;  // undo the loop's last halving
+                }
+                if (!o->block && i == 1) {  // complex64 table alone
+                    o->slab_tiles64 = 0;
+                    o->block = arena.alloc(layout_op(*o, false));
+                }
                 continue;
             }
             const size_t bytes = layout_op(*o, false);
@@ -876,8 +895,10 @@ struct yasfhip_server {
         return (2 * L + 1 + waves - 1) / waves <= nipmax;
     }
 
-    int shape_of(int L) const {
-        const int sh = shape_for_lmax[L];
+    static int size_class(int N) { return N >= kLargeRequest ? 1 : 0; }
+
+    int shape_of(int L, int N) const {
+        const int sh = shape_for_lmax[L][size_class(N)];
         YH_REQUIRE(shape_valid(L, sh), "invalid tile shape for this lmax");
         return sh;
     }
@@ -925,7 +946,7 @@ struct yasfhip_server {
             hipLaunchKernelGGL(serve_rhs_kernel, dim3(std::min<long>(1024, (q.n + 255) / 256)), dim3(256), 0,
                                st, op->t_d, q.a_d, q.b_d, q.x_d, q.n);
             YH_CHECK(hipGetLastError());
-            q.shape = shape_of(q.L);
+            q.shape = shape_of(q.L, q.N);
             q.phase = kRefresh;
             q.first_refresh = true;
             q.slot = static_cast<int>(active.size());
@@ -1042,12 +1063,38 @@ struct yasfhip_server {
     // Observables, results to the host and release of the arena blocks.
     void complete(std::vector<Request *> &done) {
         hipStream_t st = stream;
+        // regular (J) pair GEMMs of C_sca: over slabs of the regular table
+        // (built once per slab for an operator's completing requests), else
+        // on the fly
+        std::vector<char> regular_done(done.size(), 0);
+        for (size_t i = 0; i < done.size(); ++i) {
+            Request *q = done[i];
+            if (!q->observables || regular_done[i])
+                continue;
+            const SharedOp &o = *q->op;
+            const DevicePlan &dp = plan_for(q->L).dp;
+            if (o.slab) {
+                for (int t0 = 0; t0 < o.ntiles1; t0 += o.slab_tiles64) {
+                    const int ntl = std::min(o.slab_tiles64, o.ntiles1 - t0);
+                    launch_pair_table_build(dp, o.pos_d, o.N, o.k, false, o.slab, st, t0, ntl, nullptr, true);
+                    for (size_t j = i; j < done.size(); ++j)
+                        if (done[j]->op == q->op && done[j]->observables)
+                            launch_pair_table_gemm(dp, q->N, o.slab, done[j]->x_d, false, 0, done[j]->Q, st, t0,
+                                                   ntl);
+                }
+                for (size_t j = i; j < done.size(); ++j)
+                    if (done[j]->op == q->op)
+                        regular_done[j] = 1;
+            } else {
+                launch_pair_regular(dp, o.pos_d, q->N, q->k, q->x_d, q->Q, st);
+            }
+            regular_done[i] = 1;
+        }
         for (auto *q : done) {
             if (!q->observables)
                 continue;
             const DevicePlan &dp = plan_for(q->L).dp;
             double2 *y = q->V + q->n;
-            launch_pair_regular(dp, q->op->pos_d, q->N, q->k, q->x_d, q->Q, st);
             launch_contract(dp, q->N, q->Q, q->x_d, q->op->t_d, 0, y, false, false, st);
             launch_mgs_step(y, nullptr, nullptr, q->x_d, q->n, reducer(), scal + 8 * q->slot + 2, st);
             launch_mgs_step(q->x_d, nullptr, nullptr, q->a_d, q->n, reducer(), scal + 8 * q->slot + 5, st);
@@ -1177,7 +1224,7 @@ struct yasfhip_server {
             int cnt = 0, blocks = 0;
             for (int r = 0; r < nreq; ++r) {
                 Request &q = *active[r];
-                if (q.shape != gi || !q.op->table32)
+                if (q.shape != gi || !q.op->table32 || op_group_size(r) > 1)
                     continue;
                 gstart[cnt] = blocks;
                 gidx[cnt] = r;
@@ -1192,9 +1239,9 @@ struct yasfhip_server {
         slab_groups.clear();
         for (int r = 0; r < nreq; ++r) {
             SharedOp *o = active[r]->op;
-            if (o->table32)
-                continue;
-            if (!o->slab) {
+            if (o->table32 && op_group_size(r) < 2)
+                continue;  // in the batched GEMM above
+            if (!o->table32 && !o->slab) {
                 otf.push_back(r);
                 continue;
             }
@@ -1267,15 +1314,23 @@ struct yasfhip_server {
         YH_CHECK(hipGetLastError());
     }
 
-    // The complex64 GEMM of a slab group's requests over target tiles
-    // [t0, t0 + ntl) of the slab.
+    // Active requests that share active request r's operator and tile shape.
+    int op_group_size(int r) const {
+        int c = 0;
+        for (const Request *q : active)
+            c += q->op == active[r]->op && q->shape == active[r]->shape;
+        return c;
+    }
+
+    // The complex64 GEMM of an operator group's requests over target tiles
+    // [t0, t0 + ntl) of its resident table (all tiles) or of its slab.
     void launch_slab_gemm(const SlabGroup &g, int t0, int ntl) {
         hipStream_t st = stream;
         const SharedOp &o = *g.op;
         const int shape = g.shape;
         const int blocks = plan_for(o.L).dp.ntjobs[kShape[shape].grouping] * ntl * g.count;
         const int *ridx = d_index + g.first;
-        const float2 *slab = static_cast<const float2 *>(o.slab);
+        const float2 *slab = o.table32 ? o.table : static_cast<const float2 *>(o.slab);
         switch (shape) {
         case 0:
             hipLaunchKernelGGL((serve_slab_gemm_kernel<2, 5, 2>), dim3(blocks), dim3(128), 0, st, d_reqs, d_state,
@@ -1319,9 +1374,15 @@ struct yasfhip_server {
             const int *gidx = gstart + nreq + 1;
             launch_gemm(gi, gstart, gidx, grp_count[gi], grp_blocks[gi]);
         }
-        for (const auto &g : slab_groups) {  // slab operators: build a slab once, multiply for all requests
+        // operator groups: the requests' GEMMs block-interleaved over the
+        // shared table, or over each slab, built once for all of them
+        for (const auto &g : slab_groups) {
             const SharedOp &o = *g.op;
             const DevicePlan &dp = plan_for(o.L).dp;
+            if (o.table32) {
+                launch_slab_gemm(g, 0, o.ntiles1);
+                continue;
+            }
             int *flag = d_flags + g.flag;
             hipLaunchKernelGGL(serve_any_active_kernel, dim3(1), dim3(64), 0, st, d_state, d_index + g.first,
                                g.count, flag);
@@ -1448,7 +1509,7 @@ struct yasfhip_server {
         arena.release(q.block);
         release_op(op);
         set_changed = true;
-        shape_for_lmax[L] = best;
+        shape_for_lmax[L][size_class(N)] = best;
         return best;
     }
 
@@ -1523,7 +1584,7 @@ int yasfhip_server_create(size_t memory_bytes, int restart, double inner_rtol, i
         s->inner_rtol = inner_rtol;
         s->use_graph = use_graph != 0;
         for (int L = 0; L <= kMaxLmax; ++L)
-            s->shape_for_lmax[L] = 2;  // 8 waves, jobs of up to 2 column tiles
+            s->shape_for_lmax[L][0] = s->shape_for_lmax[L][1] = 2;  // 8 waves, jobs of up to 2 column tiles
         YH_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         for (auto &e : s->ev)
             YH_CHECK(hipEventCreate(&e));
@@ -1631,7 +1692,7 @@ int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int co
         YH_REQUIRE(found >= 0 && yasfhip_server::shape_valid(lmax, found),
                    "unsupported tile shape for this lmax (waves 2/4/8 x column tiles 2/4/8 as listed in "
                    "yasfhip.h, with ceil((2 lmax + 1) / waves) orders per wave within range)");
-        s->shape_for_lmax[lmax] = found;
+        s->shape_for_lmax[lmax][0] = s->shape_for_lmax[lmax][1] = found;
     });
 }
 
