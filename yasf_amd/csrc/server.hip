@@ -795,11 +795,14 @@ struct yasfhip_server {
         return off;
     }
 
-    // A new operator for request q with the largest table set that fits:
-    // complex64 + FP64 tables when within a quarter of the arena, complex64
-    // within half of it, else a slab of at most an eighth of it, else none
-    // (on-the-fly pair GEMMs).  Null when not even the tableless block can be
-    // allocated now.
+    // A new operator for request q.  Policy 1 takes the first that fits the
+    // arena's share: complex64 + FP64 tables within a quarter of the arena;
+    // complex64 within half of it, with an FP64 slab of up to a sixteenth
+    // (else alone, FP64 on the fly); else a slab of at most an eighth.  Null
+    // when the chosen form cannot be allocated now: the request waits for
+    // memory rather than running in a slower form -- except in an idle
+    // arena, where smaller slabs and at last the tableless form (on-the-fly
+    // pair GEMMs) are tried.
     SharedOp *create_op(const Request &q) {
         auto o = std::make_unique<SharedOp>();
         o->key = q.key;
@@ -809,44 +812,46 @@ struct yasfhip_server {
         o->n = q.n;
         o->k = q.k;
         const size_t cap = arena.capacity();
+        const bool idle = active.empty();
         const DevicePlan &dpl = plan_for(q.L).dp;
         const int ntiles1 = (q.N + 15) / 16;
         const size_t tile64 = pair_table_tile_bytes(dpl, q.N, false);
-        int slab64 =
-            slab_tiles_req > 0 ? slab_tiles_req : static_cast<int>(std::min<size_t>(cap / 8 / tile64, 1 << 20));
-        slab64 = std::min(slab64, (ntiles1 + 1) / 2);  // 2x as many complex64 tiles cover all targets
-        // {complex64 table, FP64 table, FP64 slab tiles}, each within a share
-        // of the arena; with a complex64 table but no FP64 one, a slab of up
-        // to a sixteenth of the arena serves the FP64 residual and C_sca
+        const int half = (ntiles1 + 1) / 2;  // FP64 tiles whose complex64 double covers all targets
+        int slab64 = slab_tiles_req > 0
+                         ? slab_tiles_req
+                         : static_cast<int>(std::min<size_t>(cap / 8 / tile64, 1 << 20));
+        slab64 = std::min(slab64, half);
         const int slab64s = slab_tiles_req > 0
                                 ? slab64
                                 : std::min(slab64, static_cast<int>(std::min<size_t>(cap / 16 / tile64, 1 << 20)));
-        const int modes[4][3] = {{1, 1, 0}, {1, 0, slab64s}, {0, 0, slab64}, {0, 0, 0}};
-        const size_t limit[4] = {cap / 4, cap / 2 + tile64 * slab64s, cap, cap};
-        const int first = table_policy == 1 ? 0 : (table_policy == 2 ? 2 : 3);
-        for (int i = first; i < 4 && !o->block; ++i) {
-            o->table32 = modes[i][0] != 0;
-            o->table64 = modes[i][1] != 0;
-            o->slab_tiles64 = modes[i][2];
-            if (i == 1 && o->slab_tiles64 < 1)
-                o->slab_tiles64 = 0;
-            if (i == 1 || i == 2) {  // the largest slab the arena can give now, halving down to one tile
-                if (i == 1 && layout_op(*o, false) > limit[1])
-                    continue;
-                for (int tiles = o->slab_tiles64; tiles >= 1 && !o->block; tiles /= 2) {
-                    o->slab_tiles64 = tiles;
-                    o->block = arena.alloc(layout_op(*o, false));
-                }
-                if (!o->block && i == 1) {  // complex64 table alone
-                    o->slab_tiles64 = 0;
-                    o->block = arena.alloc(layout_op(*o, false));
-                }
-                continue;
-            }
+        const int min_slab = idle ? 1 : std::min(16, slab64);  // smaller slabs starve the GEMM of blocks
+        auto attempt = [&](bool t32, bool t64, int slab, size_t limit) {
+            o->table32 = t32;
+            o->table64 = t64;
+            o->slab_tiles64 = slab;
             const size_t bytes = layout_op(*o, false);
-            if (bytes <= limit[i])
+            if (bytes <= limit)
                 o->block = arena.alloc(bytes);
+            return o->block != nullptr;
+        };
+        bool table_form = false;  // policy 1 and the complex64 table fits the arena's share
+        if (table_policy == 1) {
+            table_form = true;
+            if (!attempt(true, true, 0, cap / 4) && !(slab64s >= 1 && attempt(true, false, slab64s, cap / 2 + tile64 * slab64s)) &&
+                !attempt(true, false, 0, cap / 2)) {
+                o->table32 = true;
+                o->table64 = false;
+                o->slab_tiles64 = 0;
+                table_form = layout_op(*o, false) <= cap / 2;
+            }
         }
+        if (!o->block && table_form && !idle)
+            return nullptr;  // wait for memory: the table fits once others complete
+        if (!o->block && table_policy != 0)
+            for (int tiles = slab64; tiles >= min_slab && !o->block; tiles /= 2)
+                attempt(false, false, tiles, cap);
+        if (!o->block && (table_policy == 0 || idle))
+            attempt(false, false, 0, cap);
         if (!o->block)
             return nullptr;
         layout_op(*o, true);
