@@ -162,6 +162,15 @@ int yasfhip_precond_apply(yasfhip_session *s, const double *r, double *z);
  * yasfpy/lookups.py:124-171). */
 int yasfhip_set_matvec_mode(yasfhip_session *s, int mode);
 
+/* Near-field approximation of W: pairs of spheres farther apart than radius
+ * (same length unit as the positions) do not couple (radius 0: W = 0); a
+ * negative or infinite radius restores the exact operator (the default).  Applies to the coupling and
+ * master matvecs and the GMRES solves; the preconditioner and the cross
+ * sections keep every pair.  Replaces NearFarCouplingBackend
+ * (yasfpy/coupling/backends.py:259-459: near = |r_s1 - r_s2|^2 <= cutoff^2,
+ * self-coupling excluded, far pairs dropped). */
+int yasfhip_set_near_field_radius(yasfhip_session *s, double radius);
+
 /* Target tiles per slab (SLAB and AUTO modes); 0 (default): as many as fit in
  * a quarter of the free memory.  Frees the current slab. */
 int yasfhip_set_slab_tiles(yasfhip_session *s, int tiles);

@@ -334,9 +334,15 @@ def planewave_coefficients(lmax: int, positions: np.ndarray, k_medium: np.ndarra
 
 
 def coupling_matvec(lmax: int, positions: np.ndarray, k_medium: float, x: np.ndarray,
-                    s1_lo: int = 0, n_s1: int | None = None) -> np.ndarray:
+                    s1_lo: int = 0, n_s1: int | None = None,
+                    near_field_radius: float | None = None) -> np.ndarray:
     """``(W x)`` rows of particles ``[s1_lo, s1_lo+n_s1)`` for one channel
-    (yasfpy/coupling/cpu_dense.py:92-104 -> cpu_numba.py:21-95)."""
+    (yasfpy/coupling/cpu_dense.py:92-104 -> cpu_numba.py:21-95).
+
+    near_field_radius: the near/far backend's approximation
+    (yasfpy/coupling/backends.py:300-345, 402-457): only pairs with
+    ``|r_s1 - r_s2|^2 <= radius^2`` couple; the far pairs' Hankel factors are
+    zeroed, which drops exactly their terms from the sum."""
     from oracle import load_native
 
     lib = load_native()
@@ -345,6 +351,10 @@ def coupling_matvec(lmax: int, positions: np.ndarray, k_medium: float, x: np.nda
     nmax = nmax_of(lmax)
     _, sph_h, ephi, plm = mutual_lookup(lmax, positions[s1_lo:s1_lo + n_s1], positions,
                                         np.array([k_medium]))
+    if near_field_radius is not None:
+        d = positions[s1_lo:s1_lo + n_s1, None, :] - positions[None, :, :]
+        far = np.sum(d * d, axis=2) > float(near_field_radius) ** 2
+        sph_h = np.where(far[None, :, :, None], 0.0, sph_h)
     idx = idx_lookup(lmax, N)
     ab5 = translation_table(lmax)
     x = np.ascontiguousarray(x, dtype=complex)

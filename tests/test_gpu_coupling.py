@@ -170,3 +170,57 @@ def test_slab_matches_table_and_tunes():
     with pytest.raises(ValueError):
         s.set_slab_tiles(-1)
     s.close()
+
+
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_near_field_cutoff(mode):
+    """Near/far approximation (backends.py:259-459) against the oracle's
+    restatement at 1e-12 in every pair-factor mode; radius 0 removes all
+    coupling; a radius beyond every pair, or none, gives the exact W."""
+    pos, _ = cluster(40, 40, 0.15)
+    L, k = 3, 2 * np.pi / 530.0
+    x = rand_vec(np.random.default_rng(40), 40 * 30)
+    d = np.linalg.norm(pos[:, None] - pos[None], axis=2)
+    radius = float(np.median(d[np.triu_indices(40, 1)]))
+    s = _native.Session(pos, L, [k])
+    set_mode(s, mode)
+    exact = s.coupling_matvec(x)
+    s.set_near_field_radius(radius)
+    for prec, tol in ((_native.FP64, 1e-12), (_native.FP32, 2e-5)):
+        y = s.coupling_matvec(x, precision=prec)
+        assert rel(y, R.coupling_matvec(L, pos, k, x, near_field_radius=radius)) < tol
+    assert rel(y, exact) > 1e-2
+    s.set_near_field_radius(0.0)
+    assert np.all(s.coupling_matvec(x) == 0)
+    s.set_near_field_radius(10 * d.max())
+    assert np.array_equal(s.coupling_matvec(x), exact)
+    s.set_near_field_radius(None)
+    assert np.array_equal(s.coupling_matvec(x), exact)
+    s.close()
+
+
+def test_reference_backend_names_on_device():
+    """The reference's backend tests through the Simulation mirror: nearfar
+    with a radius beyond every pair equals dense (idx=None with one x gives
+    every channel's W x, and idx=1 the slice), radius 0 gives W x = 0
+    (test_nearfar_backend.py, test_tiled_dense_backend.py)."""
+    from yasf_amd import InitialField, Numerics, Parameters, Simulation
+    from yasf_amd.model import Particles
+
+    def make(backend, radius):
+        p = Particles(np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.25, 0.5, 0.0]]), np.full(3, 0.1),
+                      np.full(3, 1.5 + 0j))
+        params = Parameters(np.array([1.0, 1.3]), np.array([1.0 + 0j, 1.0 + 0j]), p, InitialField(beam_width=0))
+        return Simulation(params, Numerics(lmax=1, coupling_backend=backend, coupling_near_field_radius=radius))
+
+    dense, tiled, nf, nf0 = make("dense", None), make("tiled_dense", None), make("nearfar", 10.0), make("nearfar", 0.0)
+    x = rand_vec(np.random.default_rng(0), 3 * 6)
+    wd = dense.coupling_matrix_multiply(x)
+    assert wd.shape == (18, 2)
+    npt.assert_allclose(tiled.coupling_matrix_multiply(x), wd, rtol=1e-12, atol=1e-12)
+    npt.assert_allclose(nf.coupling_matrix_multiply(x), wd, rtol=1e-12, atol=1e-12)
+    npt.assert_allclose(nf.coupling_matrix_multiply(x, idx=1), wd[:, 1], rtol=1e-12, atol=1e-12)
+    assert np.all(nf0.coupling_matrix_multiply(x) == 0)
+    for w in range(2):
+        k = 2 * np.pi / [1.0, 1.3][w]
+        assert rel(wd[:, w], R.coupling_matvec(1, dense.parameters.particles.position, k, x)) < 1e-12

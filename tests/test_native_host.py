@@ -87,9 +87,16 @@ def test_backend_selection():
     p = Particles(np.zeros((1, 3)), np.array([50.0]), np.array([1.5 + 0j]))
     params = Parameters(np.array([500.0]), np.array([1.0]), p, InitialField(beam_width=0))
     with pytest.raises(ValueError):
-        Simulation(params, Numerics(lmax=2, coupling_backend="dense"))
-    with pytest.raises(ValueError):
         Simulation(params, Numerics(lmax=2, coupling_backend="nope"))
-    sim = Simulation(params, Numerics(lmax=2, coupling_backend="HIP"))
-    assert sim.coupling_backend.requires_pairwise_lookups is False
-    assert sim.plm is None and sim.sph_h is None
+    # every reference name selects the device operator, without pair tables
+    # (test_coupling_backend_selection.py); the near/far one needs a radius
+    # when used (test_coupling_backend_selection.py:66-74)
+    for name, near in (("HIP", None), ("dense", None), ("tiled_dense", None), ("Tiled-Dense", None),
+                       ("nearfar", 250.0), ("near_far", 250.0)):
+        sim = Simulation(params, Numerics(lmax=2, coupling_backend=name, coupling_near_field_radius=near))
+        assert sim.coupling_backend.requires_pairwise_lookups is False
+        assert sim.plm is None and sim.sph_h is None and sim.e_j_dm_phi is None
+        assert sim.coupling_backend.near_field_radius == near
+    sim = Simulation(params, Numerics(lmax=2, coupling_backend="nearfar"))
+    with pytest.raises(ValueError):
+        sim.coupling_matrix_multiply(np.zeros(16, dtype=complex))

@@ -172,6 +172,34 @@ def test_coupling_axis_selection_rule():
     assert np.abs(W[m[:, None] == m[None, :]]).max() > 1e-3
 
 
+def test_near_field_cutoff():
+    """The near/far restriction (backends.py:300-345): the reference's own
+    cases -- a radius beyond every pair equals the exact W (1e-12), radius 0
+    gives W = 0 -- and, for a radius between the pair distances, the sum of
+    the exact two-sphere operators of the near pairs alone."""
+    lmax = 2
+    pos = np.array([[0.0, 0.0, 0.0], [400.0, 0.0, 0.0], [100.0, 200.0, 0.0], [900.0, 300.0, 50.0]])
+    k = 2 * np.pi / 520.0
+    N, nmax = pos.shape[0], R.nmax_of(lmax)
+    x = np.random.default_rng(5).normal(size=N * nmax) + 1j * np.random.default_rng(6).normal(size=N * nmax)
+    exact = R.coupling_matvec(lmax, pos, k, x)
+    npt.assert_allclose(R.coupling_matvec(lmax, pos, k, x, near_field_radius=1e4), exact, rtol=1e-12, atol=0)
+    assert np.all(R.coupling_matvec(lmax, pos, k, x, near_field_radius=0.0) == 0)
+    radius = 450.0
+    ref = np.zeros_like(exact)
+    xs = x.reshape(N, nmax)
+    for a in range(N):
+        for b in range(a + 1, N):
+            if np.linalg.norm(pos[a] - pos[b]) > radius:
+                continue
+            y = R.coupling_matvec(lmax, pos[[a, b]], k, np.concatenate([xs[a], xs[b]])).reshape(2, nmax)
+            ref.reshape(N, nmax)[a] += y[0]
+            ref.reshape(N, nmax)[b] += y[1]
+    got = R.coupling_matvec(lmax, pos, k, x, near_field_radius=radius)
+    npt.assert_allclose(got, ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
+    assert np.linalg.norm(got - exact) > 1e-3 * np.linalg.norm(exact)  # some pairs were dropped
+
+
 def test_full_simulation_solve(golden):
     d = golden("full_simulation")
     L = int(d["lmax"])

@@ -75,6 +75,7 @@ def _load():
             "yasfhip_server_set_tile_shape": ([P, i, i, i], i),
             "yasfhip_server_set_table_policy": ([P, i, i], i),
             "yasfhip_set_slab_tiles": ([P, i], i),
+            "yasfhip_set_near_field_radius": ([P, d], i),
             "yasfhip_set_matvec_mode": ([P, i], i),
             "yasfhip_tune_matvec": ([P, i, i, i, ip, P], i),
         }
@@ -99,7 +100,8 @@ def exported_symbols() -> list[str]:
         "yasfhip_far_field", "yasfhip_precond_build", "yasfhip_precond_apply",
         "yasfhip_time_matvec", "yasfhip_set_matvec_mode", "yasfhip_tune_matvec", "yasfhip_server_create",
         "yasfhip_server_destroy", "yasfhip_server_submit", "yasfhip_server_step", "yasfhip_server_poll",
-        "yasfhip_server_autotune", "yasfhip_server_set_tile_shape", "yasfhip_server_set_table_policy", "yasfhip_set_slab_tiles",
+        "yasfhip_server_autotune", "yasfhip_server_set_tile_shape", "yasfhip_server_set_table_policy",
+        "yasfhip_set_slab_tiles", "yasfhip_set_near_field_radius",
     ]
 
 
@@ -274,6 +276,11 @@ class Session:
         table) or MATVEC_SLAB (the table built and streamed a slab of target
         tiles at a time inside every matvec)."""
         _check(self._lib.yasfhip_set_matvec_mode(self._h, int(mode)))
+
+    def set_near_field_radius(self, radius):
+        """Pairs farther apart than radius do not couple (0: none; None / inf: exact W)."""
+        r = -1.0 if radius is None else float(radius)
+        _check(self._lib.yasfhip_set_near_field_radius(self._h, r))
 
     def set_slab_tiles(self, tiles: int):
         """Target tiles per slab (0: as many as fit in a quarter of the free memory)."""

@@ -124,6 +124,7 @@ struct yasfhip_session {
     int table_channel[2] = {-1, -1};
     bool table_refused[2] = {false, false};
     int table_grouping[2] = {0, 0};  // column-tile grouping of the table GEMM per precision
+    double near2 = kAllPairs;          // squared near-field radius of W (kAllPairs: every pair)
     // Slab of the pair table (a few target tiles), rebuilt tile range by tile
     // range inside every matvec when the full table does not fit.
     void *slab = nullptr;
@@ -182,7 +183,7 @@ struct yasfhip_session {
             }
             YH_CHECK(hipMalloc(&table[ix], bytes));
         }
-        launch_pair_table_build(dp, pos, N, k[channel], fp32, table[ix], stream);
+        launch_pair_table_build(dp, pos, N, k[channel], fp32, table[ix], stream, 0, -1, nullptr, false, near2);
         table_channel[ix] = channel;
         return true;
     }
@@ -229,13 +230,13 @@ struct yasfhip_session {
         }
         const int tiles = slab_tiles(fp32);
         if (tiles == 0) {
-            launch_pair(dp, pos, N, k[channel], xv, fp32, Q, stream);
+            launch_pair(dp, pos, N, k[channel], xv, fp32, Q, stream, nullptr, near2);
             return;
         }
         const int ntiles1 = (N + 15) / 16;
         for (int t0 = 0; t0 < ntiles1; t0 += tiles) {
             const int ntl = std::min(tiles, ntiles1 - t0);
-            launch_pair_table_build(dp, pos, N, k[channel], fp32, slab, stream, t0, ntl);
+            launch_pair_table_build(dp, pos, N, k[channel], fp32, slab, stream, t0, ntl, nullptr, false, near2);
             launch_pair_table_gemm(dp, N, slab, xv, fp32, table_grouping[ix], Q, stream, t0, ntl);
         }
     }
@@ -838,6 +839,16 @@ int yasfhip_set_matvec_mode(yasfhip_session *s, int mode) {
     });
 }
 
+int yasfhip_set_near_field_radius(yasfhip_session *s, double radius) {
+    return guarded([&] {
+        YH_REQUIRE(s, "NULL argument");
+        YH_REQUIRE(!std::isnan(radius), "radius must not be NaN");
+        YH_CHECK(hipStreamSynchronize(s->stream));
+        s->near2 = (radius >= 0.0 && std::isfinite(radius)) ? radius * radius : kAllPairs;
+        s->table_channel[0] = s->table_channel[1] = -1;  // tables are rebuilt on their next use
+    });
+}
+
 int yasfhip_set_slab_tiles(yasfhip_session *s, int tiles) {
     return guarded([&] {
         YH_REQUIRE(s, "NULL argument");
@@ -872,7 +883,8 @@ int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps
             tab = s->table[ix];
         } else if (const int tiles = s->slab_tiles(fp32)) {
             ntl = tiles;
-            launch_pair_table_build(s->dp, s->pos, s->N, s->k[channel], fp32, s->slab, st, 0, ntl);
+            launch_pair_table_build(s->dp, s->pos, s->N, s->k[channel], fp32, s->slab, st, 0, ntl, nullptr, false,
+                                    s->near2);
             tab = s->slab;
         } else {
             return;  // pair factors on the fly: nothing to tune

@@ -41,7 +41,8 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
                                                    int maxcols, int lmax, const double *__restrict__ la,
                                                    const double *__restrict__ lb,
                                                    const double *__restrict__ ld, T *__restrict__ Q, int nq,
-                                                   int stage_elems, const int *__restrict__ gate) {
+                                                   int stage_elems, const int *__restrict__ gate,
+                                                   double near2) {
     typedef typename Mfma<T>::acc_t acc_t;
     typedef typename Cplx<T>::type T2;
     constexpr int SRC = 16;  // sources per chunk (4 MFMA K-steps)
@@ -110,13 +111,13 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
         const double4 p2 = p2next;
         const int s2n = s2 + SRC;
         p2next = pos[s2n < N ? s2n : 0];
-        const bool valid = (s2 < N) && (s1 < N) && (Regular || s2 != s1);
+        const double dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
+        // near2: pairs farther apart than the near-field radius do not couple
+        const bool valid = (s2 < N) && (s1 < N) && (Regular || (s2 != s1 && dx * dx + dy * dy + dz * dz <= near2));
         if constexpr (Regular)
-            pair_series_regular<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld,
-                                    buf + ptid, 256);
+            pair_series_regular<T2>(dx, dy, dz, k, dm, P, valid, la, lb, ld, buf + ptid, 256);
         else
-            pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld,
-                            buf + ptid, 256);
+            pair_series<T2>(dx, dy, dz, k, dm, P, valid, la, lb, ld, buf + ptid, 256);
     };
 
     acc_t acc[kWaveMaxTiles][2];
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q,
 
 template <typename T, bool Regular>
 static void launch_pair_t(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
-                          void *Q, hipStream_t s, const int *gate = nullptr) {
+                          void *Q, hipStream_t s, const int *gate = nullptr, double near2 = kAllPairs) {
     const int ntiles1 = (N + 15) / 16;
     const size_t lds = 2 * static_cast<size_t>(dp.stage_elems) * sizeof(typename Cplx<T>::type);
     static bool lds_opt_in = false;  // gfx950: up to 160 KB of LDS per workgroup
@@ -218,16 +219,16 @@ static void launch_pair_t(const DevicePlan &dp, const double4 *pos, int N, doubl
     hipLaunchKernelGGL((pair_kernel<T, Regular>), dim3(dp.njobs * ntiles1), dim3(2 * kWavesPerJob * 64), lds, s,
                        pos, static_cast<const typename Cplx<T>::type *>(x), N, dp.nmax, k, dp.jobs,
                        ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.la, dp.lb, dp.ld, static_cast<T *>(Q),
-                       dp.nq, dp.stage_elems, gate);
+                       dp.nq, dp.stage_elems, gate, near2);
     YH_CHECK(hipGetLastError());
 }
 
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
-                 void *Q, hipStream_t s, const int *gate) {
+                 void *Q, hipStream_t s, const int *gate, double near2) {
     if (fp32)
-        launch_pair_t<float, false>(dp, pos, N, k, x, Q, s, gate);
+        launch_pair_t<float, false>(dp, pos, N, k, x, Q, s, gate, near2);
     else
-        launch_pair_t<double, false>(dp, pos, N, k, x, Q, s, gate);
+        launch_pair_t<double, false>(dp, pos, N, k, x, Q, s, gate, near2);
 }
 
 void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <complex>
 #include <cstdint>
 #include <stdexcept>
@@ -122,8 +123,11 @@ void free_device_plan(DevicePlan &dp);
 
 // On-the-fly pair GEMM (pair factors generated in the kernel).  gate (device
 // int, may be null): the kernel does nothing unless *gate == 1 at launch.
+// Pairs farther apart than sqrt(near2) are left out of W (the near-field
+// approximation); kAllPairs keeps every pair.
+constexpr double kAllPairs = HUGE_VAL;
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
-                 void *Q, hipStream_t s, const int *gate = nullptr);
+                 void *Q, hipStream_t s, const int *gate = nullptr, double near2 = kAllPairs);
 // Q of the regular (J-translation) operator, coincident pairs included, FP64.
 void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
                          void *Q, hipStream_t s);
@@ -139,7 +143,7 @@ size_t pair_table_tile_bytes(const DevicePlan &dp, int N, bool fp32);  // one ta
 // regular: factors of the regular (J) translation (FP64 only).
 void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,
                              void *table, hipStream_t s, int t0 = 0, int ntl = -1,
-                             const int *gate = nullptr, bool regular = false);
+                             const int *gate = nullptr, bool regular = false, double near2 = kAllPairs);
 // grouping: index into kTableGrouping (column tiles per job).
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
                             int grouping, void *Q, hipStream_t s, int t0 = 0, int ntl = -1);

@@ -26,7 +26,8 @@ template <typename T2, bool Regular>
 __global__ __launch_bounds__(256) void pair_table_build_kernel(
     const double4 *__restrict__ pos, int N, int N4, double k, int lmax, int ndm,
     const int *__restrict__ dmoff, const double *__restrict__ la, const double *__restrict__ lb,
-    const double *__restrict__ ld, T2 *__restrict__ table, int t0, const int *__restrict__ gate) {
+    const double *__restrict__ ld, T2 *__restrict__ table, int t0, const int *__restrict__ gate,
+    double near2) {
     if (gate && *gate != 1)
         return;
     const int nchunk = (N + 15) / 16;
@@ -43,15 +44,17 @@ __global__ __launch_bounds__(256) void pair_table_build_kernel(
     if (ks >= N4)
         return;
     const int lane = (s2 & 3) * 16 + r;
-    const bool valid = (s1 < N) && (s2 < N) && (Regular || s1 != s2);
     const double4 p1 = pos[s1];  // pos is padded to a multiple of 16
     const double4 p2 = pos[s2 < N ? s2 : 0];
+    const double dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
+    // near2: pairs farther apart than the near-field radius do not couple
+    const bool valid = (s1 < N) && (s2 < N) && (Regular || (s1 != s2 && dx * dx + dy * dy + dz * dz <= near2));
     T2 *dst = table + ((static_cast<size_t>(t) * SR + dmoff[d]) * N4 + static_cast<size_t>(ks) * R) * 64 +
               lane;
     if constexpr (Regular)
-        pair_series_regular<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld, dst, 64);
+        pair_series_regular<T2>(dx, dy, dz, k, dm, P, valid, la, lb, ld, dst, 64);
     else
-        pair_series<T2>(p1.x - p2.x, p1.y - p2.y, p1.z - p2.z, k, dm, P, valid, la, lb, ld, dst, 64);
+        pair_series<T2>(dx, dy, dz, k, dm, P, valid, la, lb, ld, dst, 64);
 }
 
 constexpr int kTableWaves = 8;  // waves per (job, target tile) block
@@ -150,7 +153,8 @@ size_t pair_table_tile_bytes(const DevicePlan &dp, int N, bool fp32) {
 }
 
 void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,
-                             void *table, hipStream_t s, int t0, int ntl, const int *gate, bool regular) {
+                             void *table, hipStream_t s, int t0, int ntl, const int *gate, bool regular,
+                             double near2) {
     const int ntiles1 = (N + 15) / 16, N4 = (N + 3) / 4, nchunk = (N + 15) / 16;
     if (ntl < 0)
         ntl = ntiles1 - t0;
@@ -160,13 +164,13 @@ void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, do
     YH_REQUIRE(!(regular && fp32), "the regular table is FP64 only");
     if (regular)
         hipLaunchKernelGGL((pair_table_build_kernel<double2, true>), grid, block, 0, s, pos, N, N4, k, dp.lmax,
-                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<double2 *>(table), t0, gate);
+                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<double2 *>(table), t0, gate, near2);
     else if (fp32)
         hipLaunchKernelGGL((pair_table_build_kernel<float2, false>), grid, block, 0, s, pos, N, N4, k, dp.lmax,
-                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<float2 *>(table), t0, gate);
+                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<float2 *>(table), t0, gate, near2);
     else
         hipLaunchKernelGGL((pair_table_build_kernel<double2, false>), grid, block, 0, s, pos, N, N4, k, dp.lmax,
-                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<double2 *>(table), t0, gate);
+                           ndm, dp.dmoff, dp.la, dp.lb, dp.ld, static_cast<double2 *>(table), t0, gate, near2);
     YH_CHECK(hipGetLastError());
 }
 

@@ -28,6 +28,7 @@ class Simulation:
         self.coupling_backend = make_backend(getattr(numerics, "coupling_backend", "hip"), self)
         self.sph_j = self.sph_h = self.e_j_dm_phi = self.plm = None
         self._session: _native.Session | None = None
+        self._near_field_radius: float | None = None
         self._tmatrix_uploaded = False
         self.mie_coefficients = None
         self.initial_field_coefficients = None
@@ -39,6 +40,12 @@ class Simulation:
         if self._session is None:
             self._session = _native.Session(self.parameters.particles.position, self.numerics.lmax,
                                             self.parameters.k_medium)
+            self._near_field_radius = None
+        # the near/far cut-off as numerics holds it now (a change rebuilds the pair tables)
+        radius = getattr(self.coupling_backend, "near_field_radius", None)
+        if radius != self._near_field_radius:
+            self._session.set_near_field_radius(radius)
+            self._near_field_radius = radius
         if not self._tmatrix_uploaded and self.mie_coefficients is not None:
             self._session.set_tmatrix(self.tmatrix_flat())
             self._tmatrix_uploaded = True
