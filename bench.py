@@ -126,12 +126,13 @@ def pmc_calibration():
     del a, b
 
 
-def traffic_record():
-    """HBM bytes per launch of the FP64 table GEMM on this workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (null when absent)."""
+def traffic_record(kernel: str):
+    """HBM bytes per launch of one FP64 table-GEMM instance on this workload,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (null when
+    absent)."""
     try:
         rec = json.loads(TRAFFIC_FILE.read_text())
-        return float(rec["hbm_bytes_per_launch"])
+        return float(rec["kernels"][kernel]["hbm_bytes_per_launch"])
     except (OSError, ValueError, KeyError):
         return None
 
@@ -213,6 +214,7 @@ def run_solve(args):
     ntiles = (N + 15) // 16
     f_issued = macs_issued * 8 * (ntiles * 16) * N  # MFMA flops incl. tile padding
     achieved = f_fact / (pair_ms * 1e-3) / 1e12
+    kernel64 = f"pair_table_gemm_kernel<double, 2, {[2, 4, 8][tuned['fp64']]}>"
     achieved32 = f_fact / (pair32_ms * 1e-3) / 1e12
 
     line = {
@@ -248,8 +250,8 @@ def run_solve(args):
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": traffic_record(),
-            "kernel": "pair_table_gemm_kernel<double> (FP64 MFMA 16x16x4 over the resident pair table)",
+            "traffic": traffic_record(kernel64),
+            "kernel": kernel64 + " (FP64 MFMA 16x16x4 over the resident pair table)",
             "flops_per_launch": f_fact,
             "issued_mfma_tflops": f_issued / (pair_ms * 1e-3) / 1e12,
             "direct_count_tflops_per_matvec": f_direct / (mv_ms * 1e-3) / 1e12,

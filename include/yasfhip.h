@@ -162,6 +162,24 @@ int yasfhip_precond_apply(yasfhip_session *s, const double *r, double *z);
  * yasfpy/lookups.py:124-171). */
 int yasfhip_set_matvec_mode(yasfhip_session *s, int mode);
 
+/* Mie T-matrix diagonal on the device: t[s][n] (N x nmax complex, the
+ * reference's mode order) of spheres with radius[s] and relative refractive
+ * index relative_index[s] (complex, sphere index / medium index) in a medium
+ * of real wave number k_medium: -b_l on tau = 1, -a_l on tau = 2 in Bohren &
+ * Huffman's notation, replicated over m.  Replaces mie_coefficients
+ * (yasfpy/mie_coefficients.py:29-161) for one wavelength. */
+int yasfhip_mie_coefficients(int lmax, int particle_number, const double *radius, const double *relative_index,
+                             double k_medium, double *t);
+
+/* Plane-wave incident coefficients a[s][n] (N x nmax complex) on the device:
+ * propagation direction (polar, azimuthal), polarization 1 (TE) or 2 (TM),
+ * amplitude, focal point (NULL: origin).  Replaces
+ * compute_initial_field_coefficients_planewave
+ * (yasfpy/initial_field_coefficients.py:79-142) for one wavelength. */
+int yasfhip_planewave_coefficients(int lmax, int particle_number, const double *positions, double k_medium,
+                                   double polar, double azimuthal, int polarization, double amplitude,
+                                   const double *focal_point, double *a);
+
 /* Near-field approximation of W: pairs of spheres farther apart than radius
  * (same length unit as the positions) do not couple (radius 0: W = 0); a
  * negative or infinite radius restores the exact operator (the default).  Applies to the coupling and
@@ -226,11 +244,26 @@ int yasfhip_server_destroy(yasfhip_server *s);
  * device copy of the geometry, T-matrix and pair-factor tables; only a_inc
  * differs.  A key already in use with another N, lmax or wave number is an
  * error; positions and T of later requests with the key are not read.
- * A request whose complex64 table does not fit runs the on-the-fly pair GEMM. */
+ * Pair-factor memory per operator: see yasfhip_server_set_table_policy. */
 int yasfhip_server_submit(yasfhip_server *s, const double *positions, int particle_number, int lmax,
                           double k_medium, const double *t, const double *a_inc, double rtol,
                           int max_cycles, int observables, long long operator_key,
                           long long *request_id);
+
+/* As yasfhip_server_submit, with the system set up on the device at
+ * admission: the T-matrix diagonal from the spheres' radii and relative
+ * refractive indices (complex, sphere / medium; as yasfhip_mie_coefficients)
+ * and a_inc of a plane wave (as yasfhip_planewave_coefficients: direction
+ * (polar, azimuthal), polarization 1 TE / 2 TM, amplitude, focal point or
+ * NULL for the origin).  With an operator_key, radii and indices of later
+ * requests with the key are not read.  Replaces the host set-up of
+ * yasfpy/mie_coefficients.py:29-161 and initial_field_coefficients.py:79-142
+ * per request. */
+int yasfhip_server_submit_spheres(yasfhip_server *s, const double *positions, const double *radius,
+                                  const double *relative_index, int particle_number, int lmax, double k_medium,
+                                  double polar, double azimuthal, int polarization, double amplitude,
+                                  const double *focal_point, double rtol, int max_cycles, int observables,
+                                  long long operator_key, long long *request_id);
 
 /* Admission, FP64 residuals of requests between cycles, completions, and one
  * batched Arnoldi iteration of every request inside a cycle.  Reports the

@@ -76,6 +76,10 @@ def _load():
             "yasfhip_server_set_table_policy": ([P, i, i], i),
             "yasfhip_set_slab_tiles": ([P, i], i),
             "yasfhip_set_near_field_radius": ([P, d], i),
+            "yasfhip_mie_coefficients": ([i, i, P, P, d, P], i),
+            "yasfhip_server_submit_spheres": ([P, P, P, P, i, i, d, d, d, i, d, P, d, i, i, ctypes.c_longlong,
+                                               llp], i),
+            "yasfhip_planewave_coefficients": ([i, i, P, d, d, d, i, d, P, P], i),
             "yasfhip_set_matvec_mode": ([P, i], i),
             "yasfhip_tune_matvec": ([P, i, i, i, ip, P], i),
         }
@@ -101,7 +105,8 @@ def exported_symbols() -> list[str]:
         "yasfhip_time_matvec", "yasfhip_set_matvec_mode", "yasfhip_tune_matvec", "yasfhip_server_create",
         "yasfhip_server_destroy", "yasfhip_server_submit", "yasfhip_server_step", "yasfhip_server_poll",
         "yasfhip_server_autotune", "yasfhip_server_set_tile_shape", "yasfhip_server_set_table_policy",
-        "yasfhip_set_slab_tiles", "yasfhip_set_near_field_radius",
+        "yasfhip_set_slab_tiles", "yasfhip_set_near_field_radius", "yasfhip_mie_coefficients",
+        "yasfhip_planewave_coefficients", "yasfhip_server_submit_spheres",
     ]
 
 
@@ -138,6 +143,30 @@ def plan_stats(lmax: int) -> tuple[int, int]:
     b = ctypes.c_longlong()
     _check(_load().yasfhip_plan_stats(int(lmax), ctypes.byref(a), ctypes.byref(b)))
     return a.value, b.value
+
+
+def mie_coefficients(lmax: int, radius, relative_index, k_medium: float) -> np.ndarray:
+    """Mie T-matrix diagonal ``[N * nmax]`` on the device (one wavelength)."""
+    r = np.ascontiguousarray(radius, dtype=np.float64).ravel()
+    m = np.ascontiguousarray(np.broadcast_to(relative_index, r.shape), dtype=np.complex128)
+    out = np.empty(r.size * 2 * lmax * (lmax + 2), dtype=np.complex128)
+    _check(_load().yasfhip_mie_coefficients(int(lmax), r.size, _ptr(r), _ptr(m), float(k_medium), _ptr(out)))
+    return out
+
+
+def planewave_coefficients(lmax: int, positions, k_medium: float, polar: float, azimuthal: float,
+                           polarization: int = 1, amplitude: float = 1.0, focal_point=None) -> np.ndarray:
+    """Plane-wave incident coefficients ``[N * nmax]`` on the device (one wavelength)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float64)
+    if pos.ndim != 2 or pos.shape[1] != 3:
+        raise ValueError("positions must have shape (N, 3)")
+    f = None if focal_point is None else np.ascontiguousarray(focal_point, dtype=np.float64)
+    out = np.empty(pos.shape[0] * 2 * lmax * (lmax + 2), dtype=np.complex128)
+    _check(_load().yasfhip_planewave_coefficients(int(lmax), pos.shape[0], _ptr(pos), float(k_medium),
+                                                  float(polar), float(azimuthal), int(polarization),
+                                                  float(amplitude), _ptr(f) if f is not None else None,
+                                                  _ptr(out)))
+    return out
 
 
 class Session:
@@ -343,6 +372,28 @@ class Server:
                                                _ptr(t), _ptr(a), float(rtol), int(max_cycles),
                                                int(bool(observables)), int(operator_key), ctypes.byref(rid)))
         self._sizes[rid.value] = n
+        return rid.value
+
+    def submit_spheres(self, positions, radius, relative_index, lmax: int, k_medium: float, polar: float,
+                       azimuthal: float, polarization: int = 1, amplitude: float = 1.0, focal_point=None,
+                       rtol: float = 1e-6, max_cycles: int = 100, observables: bool = True,
+                       operator_key: int = 0) -> int:
+        """Queues one solve whose T-matrix (Mie, from radius and relative
+        refractive index per sphere) and plane-wave coefficients are set up on
+        the device at admission."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        if pos.ndim != 2 or pos.shape[1] != 3:
+            raise ValueError("positions must have shape (N, 3)")
+        N = pos.shape[0]
+        r = np.ascontiguousarray(np.broadcast_to(radius, (N,)), dtype=np.float64)
+        m = np.ascontiguousarray(np.broadcast_to(relative_index, (N,)), dtype=np.complex128)
+        f = None if focal_point is None else np.ascontiguousarray(focal_point, dtype=np.float64)
+        rid = ctypes.c_longlong()
+        _check(self._lib.yasfhip_server_submit_spheres(
+            self._h, _ptr(pos), _ptr(r), _ptr(m), N, int(lmax), float(k_medium), float(polar), float(azimuthal),
+            int(polarization), float(amplitude), _ptr(f) if f is not None else None, float(rtol),
+            int(max_cycles), int(bool(observables)), int(operator_key), ctypes.byref(rid)))
+        self._sizes[rid.value] = N * 2 * lmax * (lmax + 2)
         return rid.value
 
     def step(self) -> tuple[int, int, int]:

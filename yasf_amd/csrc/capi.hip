@@ -839,6 +839,54 @@ int yasfhip_set_matvec_mode(yasfhip_session *s, int mode) {
     });
 }
 
+int yasfhip_mie_coefficients(int lmax, int particle_number, const double *radius, const double *relative_index,
+                             double k_medium, double *t) {
+    return guarded([&] {
+        YH_REQUIRE(radius && relative_index && t, "NULL argument");
+        YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
+        YH_REQUIRE(particle_number >= 1, "particle_number must be >= 1");
+        YH_REQUIRE(std::isfinite(k_medium) && k_medium > 0.0, "k_medium must be real, finite and positive");
+        const int N = particle_number;
+        const size_t n = static_cast<size_t>(N) * 2 * lmax * (lmax + 2);
+        std::vector<double4> sph(N);
+        for (int i = 0; i < N; ++i) {
+            YH_REQUIRE(radius[i] > 0.0 && std::isfinite(radius[i]), "radii must be positive and finite");
+            sph[i] = make_double4(radius[i], relative_index[2 * i], relative_index[2 * i + 1], 0.0);
+        }
+        DeviceBuffer<double4> d_sph(N);
+        DeviceBuffer<double2> d_t(n);
+        YH_CHECK(hipMemcpy(d_sph.get(), sph.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
+        launch_mie(d_sph.get(), N, lmax, k_medium, d_t.get(), nullptr);
+        YH_CHECK(hipMemcpy(t, d_t.get(), sizeof(double2) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int yasfhip_planewave_coefficients(int lmax, int particle_number, const double *positions, double k_medium,
+                                   double polar, double azimuthal, int polarization, double amplitude,
+                                   const double *focal_point, double *a) {
+    return guarded([&] {
+        YH_REQUIRE(positions && a, "NULL argument");
+        YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
+        YH_REQUIRE(particle_number >= 1, "particle_number must be >= 1");
+        YH_REQUIRE(std::isfinite(k_medium) && k_medium > 0.0, "k_medium must be real, finite and positive");
+        const int N = particle_number, nmax = 2 * lmax * (lmax + 2);
+        const size_t n = static_cast<size_t>(N) * nmax;
+        std::vector<double2> c(nmax);
+        planewave_angular(lmax, polar, azimuthal, polarization, amplitude, c.data());
+        std::vector<double4> pos(N);
+        for (int i = 0; i < N; ++i)
+            pos[i] = make_double4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 0.0);
+        const double zero[3] = {0.0, 0.0, 0.0};
+        DeviceBuffer<double4> d_pos(N);
+        DeviceBuffer<double2> d_c(nmax), d_a(n);
+        YH_CHECK(hipMemcpy(d_pos.get(), pos.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
+        YH_CHECK(hipMemcpy(d_c.get(), c.data(), sizeof(double2) * nmax, hipMemcpyHostToDevice));
+        launch_planewave(d_pos.get(), N, lmax, k_medium, polar, azimuthal, focal_point ? focal_point : zero,
+                         d_c.get(), d_a.get(), nullptr);
+        YH_CHECK(hipMemcpy(a, d_a.get(), sizeof(double2) * n, hipMemcpyDeviceToHost));
+    });
+}
+
 int yasfhip_set_near_field_radius(yasfhip_session *s, double radius) {
     return guarded([&] {
         YH_REQUIRE(s, "NULL argument");

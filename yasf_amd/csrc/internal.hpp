@@ -148,4 +148,30 @@ void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, do
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
                             int grouping, void *Q, hipStream_t s, int t0 = 0, int ntl = -1);
 
+// Device allocation owned for a scope (freed on every exit path).
+template <typename T>
+class DeviceBuffer {
+  public:
+    explicit DeviceBuffer(size_t count) { YH_CHECK(hipMalloc(&p_, sizeof(T) * (count ? count : 1))); }
+    ~DeviceBuffer() {
+        if (p_)
+            (void)hipFree(p_);
+    }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    T *get() const { return p_; }
+
+  private:
+    T *p_ = nullptr;
+};
+
+// Per-request set-up on the device.  sph[s] = (radius, Re rel, Im rel, -) with
+// rel = sphere index / medium index; t: [N][nmax] Mie T-matrix diagonal.
+void launch_mie(const double4 *sph, int N, int lmax, double k, double2 *t, hipStream_t s);
+// a[s][n] = c[n] e^{i k (r_s - r_f) . d(polar, azimuthal)}; focal: host double[3].
+void launch_planewave(const double4 *pos, int N, int lmax, double k, double polar, double azimuthal,
+                      const double *focal, const double2 *c, double2 *a, hipStream_t s);
+// c[n] = 4 A e^{-i m azimuthal} B^dagger_n(polar) on the host (nmax entries).
+void planewave_angular(int lmax, double polar, double azimuthal, int pol, double amplitude, double2 *c);
+
 }  // namespace yasfhip

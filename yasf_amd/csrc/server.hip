@@ -572,6 +572,8 @@ struct SharedOp {
     void *block = nullptr;
     double4 *pos_d = nullptr;
     double2 *t_d = nullptr;
+    double4 *sph_d = nullptr;  // (radius, relative index) per sphere, for the device Mie set-up
+    bool device_setup = false;
     bool table32 = false, table64 = false;
     float2 *table = nullptr;
     double2 *tab64 = nullptr;
@@ -599,10 +601,16 @@ struct Request {
     bool observables = true;
     std::vector<double4> pos;
     std::vector<double2> t, a;
+    // device set-up instead of host t and a: sph[s] = (radius, relative
+    // index), the plane wave's angular coefficients and its geometry
+    bool device_setup = false;
+    std::vector<double4> sph;
+    std::vector<double2> acoef;
+    double polar = 0.0, azimuthal = 0.0, focal[3] = {0.0, 0.0, 0.0};
     SharedOp *op = nullptr;
     // device (one arena block per request besides the shared operator)
     void *block = nullptr;
-    double2 *a_d = nullptr, *b_d = nullptr, *x_d = nullptr, *y_d = nullptr;
+    double2 *a_d = nullptr, *b_d = nullptr, *x_d = nullptr, *y_d = nullptr, *acoef_d = nullptr;
     float2 *x32 = nullptr;
     void *Q = nullptr;
     double2 *V = nullptr, *partial = nullptr, *H = nullptr, *h1 = nullptr, *h2 = nullptr, *g = nullptr,
@@ -741,6 +749,7 @@ struct yasfhip_server {
         size_t off = 0;
         const size_t o_pos = aligned(off, sizeof(double4) * o.ntiles1 * 16);
         const size_t o_t = aligned(off, sizeof(double2) * o.n);
+        const size_t o_sph = o.device_setup ? aligned(off, sizeof(double4) * o.N) : 0;
         const size_t o_tab = o.table32 ? aligned(off, pair_table_bytes(dp, o.N, true)) : 0;
         const size_t o_tab64 = o.table64 ? aligned(off, pair_table_bytes(dp, o.N, false)) : 0;
         const size_t o_slab =
@@ -749,6 +758,7 @@ struct yasfhip_server {
             char *b = static_cast<char *>(o.block);
             o.pos_d = reinterpret_cast<double4 *>(b + o_pos);
             o.t_d = reinterpret_cast<double2 *>(b + o_t);
+            o.sph_d = o.device_setup ? reinterpret_cast<double4 *>(b + o_sph) : nullptr;
             o.table = o.table32 ? reinterpret_cast<float2 *>(b + o_tab) : nullptr;
             o.tab64 = o.table64 ? reinterpret_cast<double2 *>(b + o_tab64) : nullptr;
             o.slab = o.slab_tiles64 ? b + o_slab : nullptr;
@@ -765,6 +775,7 @@ struct yasfhip_server {
         const size_t o_a = take(sizeof(double2) * q.n);
         const size_t o_b = take(sizeof(double2) * q.n), o_x = take(sizeof(double2) * q.n);
         const size_t o_y = take(sizeof(double2) * R1);
+        const size_t o_ac = q.device_setup ? take(sizeof(double2) * q.nmax) : 0;
         const size_t o_x32 = take(sizeof(float2) * q.n);
         const size_t o_Q = take(sizeof(double2) * q.ntiles1 * dp.nq * 256);
         const size_t o_V = take(sizeof(double2) * R1 * q.n);
@@ -780,6 +791,7 @@ struct yasfhip_server {
             q.b_d = reinterpret_cast<double2 *>(b + o_b);
             q.x_d = reinterpret_cast<double2 *>(b + o_x);
             q.y_d = reinterpret_cast<double2 *>(b + o_y);
+            q.acoef_d = q.device_setup ? reinterpret_cast<double2 *>(b + o_ac) : nullptr;
             q.x32 = reinterpret_cast<float2 *>(b + o_x32);
             q.Q = b + o_Q;
             q.V = reinterpret_cast<double2 *>(b + o_V);
@@ -811,6 +823,7 @@ struct yasfhip_server {
         o->ntiles1 = q.ntiles1;
         o->n = q.n;
         o->k = q.k;
+        o->device_setup = q.device_setup;
         const size_t cap = arena.capacity();
         const bool idle = active.empty();
         const DevicePlan &dpl = plan_for(q.L).dp;
@@ -858,8 +871,13 @@ struct yasfhip_server {
         hipStream_t st = stream;
         YH_CHECK(hipMemcpyAsync(o->pos_d, q.pos.data(), sizeof(double4) * q.pos.size(), hipMemcpyHostToDevice,
                                 st));
-        YH_CHECK(hipMemcpyAsync(o->t_d, q.t.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
         const DevicePlan &dp = plan_for(q.L).dp;
+        if (q.device_setup) {  // Mie T-matrix entries on the device
+            YH_CHECK(hipMemcpyAsync(o->sph_d, q.sph.data(), sizeof(double4) * q.N, hipMemcpyHostToDevice, st));
+            launch_mie(o->sph_d, q.N, q.L, q.k, o->t_d, st);
+        } else {
+            YH_CHECK(hipMemcpyAsync(o->t_d, q.t.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
+        }
         if (o->table32)
             launch_pair_table_build(dp, o->pos_d, o->N, o->k, true, o->table, st);
         if (o->table64)
@@ -947,7 +965,13 @@ struct yasfhip_server {
             layout(q, true);
             queue.pop_front();
             hipStream_t st = stream;
-            YH_CHECK(hipMemcpyAsync(q.a_d, q.a.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
+            if (q.device_setup) {  // plane-wave coefficients on the device
+                YH_CHECK(hipMemcpyAsync(q.acoef_d, q.acoef.data(), sizeof(double2) * q.nmax,
+                                        hipMemcpyHostToDevice, st));
+                launch_planewave(op->pos_d, q.N, q.L, q.k, q.polar, q.azimuthal, q.focal, q.acoef_d, q.a_d, st);
+            } else {
+                YH_CHECK(hipMemcpyAsync(q.a_d, q.a.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
+            }
             hipLaunchKernelGGL(serve_rhs_kernel, dim3(std::min<long>(1024, (q.n + 255) / 256)), dim3(256), 0,
                                st, op->t_d, q.a_d, q.b_d, q.x_d, q.n);
             YH_CHECK(hipGetLastError());
@@ -1576,6 +1600,56 @@ int sguard(F &&f) {
 
 }  // namespace
 
+namespace {
+
+// Validation, key bookkeeping and queueing shared by the submit entry points;
+// fill() sets the request's operator and incidence data.
+template <typename F>
+void submit_request(yasfhip_server *s, const double *positions, int particle_number, int lmax, double k_medium,
+                    double rtol, int max_cycles, int observables, long long operator_key, long long *request_id,
+                    F &&fill) {
+    YH_REQUIRE(s && positions && request_id, "NULL argument");
+    YH_REQUIRE(particle_number >= 1, "particle_number must be >= 1");
+    YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
+    YH_REQUIRE(std::isfinite(k_medium) && k_medium > 0.0, "k_medium must be real, finite and positive");
+    YH_REQUIRE(rtol > 0.0, "rtol must be > 0");
+    YH_REQUIRE(max_cycles >= 1, "max_cycles must be >= 1");
+    if (operator_key != 0) {
+        auto it = s->key_uses.find(operator_key);
+        if (it != s->key_uses.end())
+            YH_REQUIRE(it->second.N == particle_number && it->second.L == lmax && it->second.k == k_medium,
+                       "operator_key already names an operator with a different size, lmax or wave number");
+    }
+    auto q = std::make_unique<Request>();
+    q->key = operator_key;
+    q->N = particle_number;
+    q->L = lmax;
+    q->nmax = 2 * lmax * (lmax + 2);
+    q->n = static_cast<long>(q->N) * q->nmax;
+    q->ntiles1 = (q->N + 15) / 16;
+    q->chunks = static_cast<int>((q->n + kChunk - 1) / kChunk);
+    q->k = k_medium;
+    q->rtol = rtol;
+    q->max_cycles = max_cycles;
+    q->observables = observables != 0;
+    q->pos.assign(q->ntiles1 * 16, make_double4(0, 0, 0, 0));
+    for (int i = 0; i < q->N; ++i)
+        q->pos[i] = make_double4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 0.0);
+    fill(*q);
+    q->id = s->next_id++;
+    q->t_submit = now_s();
+    *request_id = q->id;
+    if (operator_key != 0) {
+        auto &u = s->key_uses[operator_key];
+        if (u.uses == 0)
+            u = yasfhip_server::KeyUse{particle_number, lmax, k_medium, 0};
+        ++u.uses;
+    }
+    s->queue.push_back(q.release());
+}
+
+}  // namespace
+
 extern "C" {
 
 int yasfhip_server_create(size_t memory_bytes, int restart, double inner_rtol, int use_graph,
@@ -1616,47 +1690,44 @@ int yasfhip_server_submit(yasfhip_server *s, const double *positions, int partic
                           double k_medium, const double *t, const double *a_inc, double rtol, int max_cycles,
                           int observables, long long operator_key, long long *request_id) {
     return sguard([&] {
-        YH_REQUIRE(s && positions && t && a_inc && request_id, "NULL argument");
-        YH_REQUIRE(particle_number >= 1, "particle_number must be >= 1");
-        YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
-        YH_REQUIRE(std::isfinite(k_medium) && k_medium > 0.0, "k_medium must be real, finite and positive");
-        YH_REQUIRE(rtol > 0.0, "rtol must be > 0");
-        YH_REQUIRE(max_cycles >= 1, "max_cycles must be >= 1");
-        if (operator_key != 0) {
-            auto it = s->key_uses.find(operator_key);
-            if (it != s->key_uses.end())
-                YH_REQUIRE(it->second.N == particle_number && it->second.L == lmax && it->second.k == k_medium,
-                           "operator_key already names an operator with a different size, lmax or wave number");
-        }
-        auto q = std::make_unique<Request>();
-        q->id = s->next_id++;
-        q->key = operator_key;
-        q->N = particle_number;
-        q->L = lmax;
-        q->nmax = 2 * lmax * (lmax + 2);
-        q->n = static_cast<long>(q->N) * q->nmax;
-        q->ntiles1 = (q->N + 15) / 16;
-        q->chunks = static_cast<int>((q->n + kChunk - 1) / kChunk);
-        q->k = k_medium;
-        q->rtol = rtol;
-        q->max_cycles = max_cycles;
-        q->observables = observables != 0;
-        q->pos.assign(q->ntiles1 * 16, make_double4(0, 0, 0, 0));
-        for (int i = 0; i < q->N; ++i)
-            q->pos[i] = make_double4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 0.0);
-        q->t.resize(q->n);
-        q->a.resize(q->n);
-        std::memcpy(q->t.data(), t, sizeof(double2) * q->n);
-        std::memcpy(q->a.data(), a_inc, sizeof(double2) * q->n);
-        q->t_submit = now_s();
-        *request_id = q->id;
-        if (operator_key != 0) {
-            auto &u = s->key_uses[operator_key];
-            if (u.uses == 0)
-                u = yasfhip_server::KeyUse{particle_number, lmax, k_medium, 0};
-            ++u.uses;
-        }
-        s->queue.push_back(q.release());
+        YH_REQUIRE(t && a_inc, "NULL argument");
+        submit_request(s, positions, particle_number, lmax, k_medium, rtol, max_cycles, observables, operator_key,
+                       request_id, [&](Request &q) {
+                           q.t.resize(q.n);
+                           q.a.resize(q.n);
+                           std::memcpy(q.t.data(), t, sizeof(double2) * q.n);
+                           std::memcpy(q.a.data(), a_inc, sizeof(double2) * q.n);
+                       });
+    });
+}
+
+int yasfhip_server_submit_spheres(yasfhip_server *s, const double *positions, const double *radius,
+                                  const double *relative_index, int particle_number, int lmax, double k_medium,
+                                  double polar, double azimuthal, int polarization, double amplitude,
+                                  const double *focal_point, double rtol, int max_cycles, int observables,
+                                  long long operator_key, long long *request_id) {
+    return sguard([&] {
+        YH_REQUIRE(radius && relative_index, "NULL argument");
+        YH_REQUIRE(std::isfinite(polar) && std::isfinite(azimuthal) && std::isfinite(amplitude),
+                   "incidence angles and amplitude must be finite");
+        submit_request(s, positions, particle_number, lmax, k_medium, rtol, max_cycles, observables, operator_key,
+                       request_id, [&](Request &q) {
+                           q.device_setup = true;
+                           q.sph.resize(q.N);
+                           for (int i = 0; i < q.N; ++i) {
+                               YH_REQUIRE(radius[i] > 0.0 && std::isfinite(radius[i]),
+                                          "radii must be positive and finite");
+                               q.sph[i] = make_double4(radius[i], relative_index[2 * i],
+                                                       relative_index[2 * i + 1], 0.0);
+                           }
+                           q.acoef.resize(q.nmax);
+                           planewave_angular(lmax, polar, azimuthal, polarization, amplitude, q.acoef.data());
+                           q.polar = polar;
+                           q.azimuthal = azimuthal;
+                           if (focal_point)
+                               for (int c = 0; c < 3; ++c)
+                                   q.focal[c] = focal_point[c];
+                       });
     });
 }
 

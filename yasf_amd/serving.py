@@ -44,6 +44,7 @@ class ServeRequest:
     azimuthal: float
     arrival_s: float = 0.0
     operator_key: int = 0   # requests with the same nonzero key share one operator
+    radii: np.ndarray | None = None  # sphere radii (for the device set-up of t)
 
     @property
     def particles(self) -> int:
@@ -71,7 +72,7 @@ def make_request(rng: np.random.Generator, index: int, n_range=(20, 3000), lmax_
     mie = mie_coefficients(lmax, p.unique_radius_index_pairs, omega, k)
     t = np.ascontiguousarray(mie[p.single_unique_array_idx, :, 0]).ravel()
     a = np.ascontiguousarray(planewave_coefficients(lmax, pos, k, beta, alpha, 1)[:, :, 0]).ravel()
-    return ServeRequest(index, pos, lmax, lam, float(k[0]), t, a, m, beta, alpha)
+    return ServeRequest(index, pos, lmax, lam, float(k[0]), t, a, m, beta, alpha, radii=radii)
 
 
 def make_workload(count: int = 512, seed: int = 4, rate: float = math.inf, **kw) -> list[ServeRequest]:
@@ -113,7 +114,7 @@ def make_aggregate_workload(particle_counts, wavelengths, directions: int, seed:
                 alpha = float(rng.uniform(0.0, 2 * math.pi))
                 a = np.ascontiguousarray(planewave_coefficients(lmax, pos, k, beta, alpha, 1)[:, :, 0]).ravel()
                 reqs.append(ServeRequest(len(reqs), pos, lmax, float(lam), float(k[0]), t, a, m, beta, alpha,
-                                         operator_key=key))
+                                         operator_key=key, radii=radii))
     return reqs
 
 
@@ -133,10 +134,12 @@ class ServeReport:
 
 
 def serve(server, requests: list[ServeRequest], rtol: float = 1e-6, keep_solutions: bool = False,
-          progress=None) -> ServeReport:
+          progress=None, device_setup: bool = False) -> ServeReport:
     """Submits each request at its arrival time (relative to the start) and
     steps the server until all have completed.  Latency = completion -
-    arrival; solves/s = requests / (last completion - start)."""
+    arrival; solves/s = requests / (last completion - start).  device_setup:
+    submit radii, refractive index and incidence, and let the server set up t
+    and a_inc on the device (TE plane wave in vacuum, as the workloads here)."""
     order = sorted(requests, key=lambda r: r.arrival_s)
     ids = {}
     results = {}
@@ -148,8 +151,13 @@ def serve(server, requests: list[ServeRequest], rtol: float = 1e-6, keep_solutio
         now = time.perf_counter() - t0
         while nxt < total and order[nxt].arrival_s <= now:
             r = order[nxt]
-            ids[server.submit(r.positions, r.lmax, r.k, r.t, r.a_inc, rtol=rtol,
-                              operator_key=r.operator_key)] = r
+            if device_setup:
+                rid = server.submit_spheres(r.positions, r.radii, r.refractive_index, r.lmax, r.k, r.polar,
+                                            r.azimuthal, polarization=1, rtol=rtol, operator_key=r.operator_key)
+            else:
+                rid = server.submit(r.positions, r.lmax, r.k, r.t, r.a_inc, rtol=rtol,
+                                    operator_key=r.operator_key)
+            ids[rid] = r
             nxt += 1
         active, queued, _ = server.step()
         done_t = time.perf_counter() - t0
