@@ -172,7 +172,7 @@ def run_solve(args):
     def step_mixed():
         if use_pc:
             s.precond_build(boxes)
-        return s.gmres_refined(b, b, rtol=1e-6, inner_rtol=1e-4, restart=args.restart,
+        return s.gmres_refined(b, b, rtol=1e-6, inner_rtol=args.inner_rtol, restart=args.restart,
                                inner_maxiter=1000, max_refinements=10, use_precond=use_pc)
 
     for _ in range(args.warmup):
@@ -333,7 +333,7 @@ def run_precision(args):
     for i in range(max(1, args.steps)):
         t0 = time.perf_counter()
         xm, info_m, steps_m, iters_m, res_m, hist = s.gmres_refined(
-            b, b, rtol=1e-6, inner_rtol=1e-4, restart=50, inner_maxiter=1000, max_refinements=10,
+            b, b, rtol=1e-6, inner_rtol=args.inner_rtol, restart=50, inner_maxiter=1000, max_refinements=10,
             use_precond=True)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0)
@@ -400,7 +400,7 @@ def run_serving(args):
     t0 = time.perf_counter()
     reqs = make_workload(args.requests, seed=4, rate=args.rate)
     log(f"workload: {len(reqs)} requests prepared on the host in {time.perf_counter() - t0:.1f} s")
-    srv = _native.Server(restart=50, inner_rtol=1e-4)
+    srv = _native.Server(restart=50, inner_rtol=args.inner_rtol)
     tiles = {}
     for L in range(3, 9):
         tiles[L] = {}
@@ -436,7 +436,7 @@ def run_serving(args):
                    "requests": len(reqs),
                    "arrival_rate_per_s": args.rate if math.isfinite(args.rate) else None,
                    "arrivals": "poisson" if math.isfinite(args.rate) else "all queued at start (closed)",
-                   "restart": 50, "inner_rtol": 1e-4, "tile_shapes": tiles,
+                   "restart": 50, "inner_rtol": args.inner_rtol, "tile_shapes": tiles,
                    "setup": "host" if args.host_setup else "device (Mie + plane wave at admission)"},
         "p50_latency_s": rep.p50_s,
         "p99_latency_s": rep.p99_s,
@@ -477,7 +477,7 @@ def run_aggregates(args):
     reqs = make_aggregate_workload(counts, lams, args.directions, seed=5)
     log(f"workload: aggregates N={counts}, wavelengths {[round(v, 1) for v in lams]} nm, "
         f"{args.directions} directions: {len(reqs)} requests prepared in {time.perf_counter() - t0:.1f} s")
-    srv = _native.Server(restart=50, inner_rtol=1e-4)
+    srv = _native.Server(restart=50, inner_rtol=args.inner_rtol)
     (waves, ctiles), ms = srv.autotune(4, particles=3000, reps=3)
     log(f"autotuned lmax-4 tile shape (large requests): {waves} waves, {ctiles} column tiles ({ms})")
     if args.warmup:
@@ -504,7 +504,7 @@ def run_aggregates(args):
                                "n 1.7+0.1i, lmax 4; sampled wavelengths x incidence directions, rtol 1e-6",
                    "particles": counts, "wavelengths_nm": lams, "directions": args.directions,
                    "requests": len(reqs), "operators": len(counts) * len(lams),
-                   "restart": 50, "inner_rtol": 1e-4, "tile_shape": [waves, ctiles],
+                   "restart": 50, "inner_rtol": args.inner_rtol, "tile_shape": [waves, ctiles],
                    "setup": "host" if args.host_setup else "device (Mie + plane wave at admission)"},
         "p50_latency_s": rep.p50_s,
         "p99_latency_s": rep.p99_s,
@@ -546,6 +546,8 @@ def main():
     ap.add_argument("--requests", type=int, default=512, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=math.inf, help="serving: Poisson arrivals per second")
     ap.add_argument("--no-baseline", action="store_true", help="serving: skip the sequential FP64 run")
+    ap.add_argument("--inner-rtol", type=float, default=1e-4,
+                    help="mixed precision: least relative residual reduction of each inner GMRES cycle")
     ap.add_argument("--host-setup", action="store_true",
                     help="serving/aggregates: submit host-computed t and a_inc instead of spheres + incidence "
                          "(default: the server sets them up on the device)")
