@@ -216,7 +216,9 @@ def run_solve(args):
     ntiles = (N + 15) // 16
     f_issued = macs_issued * 8 * (ntiles * 16) * N  # MFMA flops incl. tile padding
     achieved = f_fact / (pair_ms * 1e-3) / 1e12
-    kernel64 = f"pair_table_gemm_kernel<double, 2, {[2, 4, 8][tuned['fp64']]}>"
+    waves64, tiles64 = _native.Session.TABLE_SHAPES[tuned["fp64"]]
+    # instance name as rocprofv3 reports it: <T, waves, orders per wave (lmax <= 7), column tiles>
+    kernel64 = f"pair_table_gemm_kernel<double, {waves64}, {-(-15 // waves64)}, {tiles64}>"
     achieved32 = f_fact / (pair32_ms * 1e-3) / 1e12
 
     line = {
@@ -238,7 +240,9 @@ def run_solve(args):
                    "preconditioner": use_pc, "restart": args.restart,
                    "box_cap": args.box_cap if use_pc else None,
                    "matvec": "resident FP64 pair-factor table + ab5 contraction",
-                   "table_gemm_column_tiles": {k: [2, 4, 8][v] for k, v in tuned.items()}},
+                   "table_gemm_shape": {k: {"waves": _native.Session.TABLE_SHAPES[v][0],
+                                            "column_tiles": _native.Session.TABLE_SHAPES[v][1]}
+                                        for k, v in tuned.items()}},
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "gmres_iterations": iters,

@@ -193,20 +193,29 @@ int yasfhip_set_near_field_radius(yasfhip_session *s, double radius);
  * a quarter of the free memory.  Frees the current slab. */
 int yasfhip_set_slab_tiles(yasfhip_session *s, int tiles);
 
-/* Tile-shape autotuner of the resident-table GEMM for one precision: times
- * `reps` launches with each column-tile grouping (jobs of at most 2, 4 or 8
- * 16-column tiles per dm -- 8 in FP32 only; wider groups stream each table
- * slice fewer times but hold more accumulators), keeps the fastest for this
- * session and returns its index in *grouping; ms[3] receives the mean times
- * (-1 for groupings not offered; all -1 when the session keeps no resident
- * table, in which case nothing changes).  The table is
- * built first if needed.  Results do not depend on the grouping. */
+/* Tile-shape autotuner of the resident-table (or slab) GEMM for one
+ * precision: times `reps` launches of each shape and keeps the fastest for
+ * this session.  Shapes 0, 1, 2: 8 waves per block with jobs of at most 2, 4
+ * or 8 16-column tiles per dm; 3, 4: 4 waves with 2 or 4 (fewer waves give
+ * each more orders).  FP64 offers 0, 1 and 3, FP32 0-4: wider jobs stream
+ * each table slice fewer times but hold more accumulators.  Returns the
+ * index in *grouping; ms[6] receives the mean times (-1 for shapes not
+ * offered; all -1 when the session generates the pair factors on the fly, in
+ * which case nothing changes).  The table (or first slab) is built first if
+ * needed.  Results do not depend on the shape. */
 int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps, int *grouping,
                         double *ms);
 
+/* Fixes the table GEMM shape of one precision (an index offered above)
+ * without timing.  No counterpart in the reference. */
+int yasfhip_set_table_shape(yasfhip_session *s, int precision, int shape);
+
 /* Timing of `reps` device-resident matvecs of one channel on the session
  * stream with HIP events: ms_pair = mean duration of the pair kernel,
- * ms_total = mean duration of the whole matvec (pair + contraction). */
+ * ms_total = mean duration of the whole matvec (pair + contraction).  The
+ * input is a deterministic pseudo-random vector (the GEMMs run measurably
+ * faster on all-zero operands, which draw less power); the tuner above
+ * times the same input. */
 int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps, double *ms_pair,
                         double *ms_total);
 

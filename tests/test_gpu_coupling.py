@@ -137,9 +137,18 @@ def test_table_groupings_agree(lmax):
     x = rand_vec(np.random.default_rng(lmax), 45 * 2 * lmax * (lmax + 2))
     s = _native.Session(pos, lmax, [k])
     s.set_matvec_mode(_native.MATVEC_TABLE)
-    for prec, offered in ((_native.FP64, (0, 1)), (_native.FP32, (0, 1, 2))):
+    for prec, offered in ((_native.FP64, (0, 1, 3)), (_native.FP32, (0, 1, 2, 3, 4))):
         g, ms = s.tune_matvec(precision=prec, reps=1)
-        assert g in offered and all((ms[i] > 0) == (i in offered) for i in range(3))
+        assert g in offered and all((ms[i] > 0) == (i in offered) for i in range(6))
+        # every offered shape computes the same C tiles in the same K order: equal bit for bit
+        outs = []
+        for shape in offered:
+            s.set_table_shape(shape, precision=prec)
+            outs.append(s.coupling_matvec(x, precision=prec))
+        assert all(np.array_equal(o, outs[0]) for o in outs[1:])
+        with pytest.raises(ValueError):
+            s.set_table_shape(5, precision=prec)
+        s.set_table_shape(g, precision=prec)
         y = s.coupling_matvec(x, precision=prec)
         oracle_y = R.coupling_matvec(lmax, pos, k, x)
         assert rel(y, oracle_y) < (1e-12 if prec == _native.FP64 else 2e-5)
@@ -163,9 +172,9 @@ def test_slab_matches_table_and_tunes():
         for p in (_native.FP64, _native.FP32):
             assert np.array_equal(s.coupling_matvec(x, precision=p), ref[p]), (tiles, p)
     s.set_slab_tiles(2)
-    for p, offered in ((_native.FP64, (0, 1)), (_native.FP32, (0, 1, 2))):
+    for p, offered in ((_native.FP64, (0, 1, 3)), (_native.FP32, (0, 1, 2, 3, 4))):
         g, ms = s.tune_matvec(precision=p, reps=1)
-        assert g in offered and all((ms[i] > 0) == (i in offered) for i in range(3))
+        assert g in offered and all((ms[i] > 0) == (i in offered) for i in range(6))
         assert np.array_equal(s.coupling_matvec(x, precision=p), ref[p])
     with pytest.raises(ValueError):
         s.set_slab_tiles(-1)

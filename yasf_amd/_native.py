@@ -76,6 +76,7 @@ def _load():
             "yasfhip_server_set_table_policy": ([P, i, i], i),
             "yasfhip_set_slab_tiles": ([P, i], i),
             "yasfhip_set_near_field_radius": ([P, d], i),
+            "yasfhip_set_table_shape": ([P, i, i], i),
             "yasfhip_mie_coefficients": ([i, i, P, P, d, P], i),
             "yasfhip_server_submit_spheres": ([P, P, P, P, i, i, d, d, d, i, d, P, d, i, i, ctypes.c_longlong,
                                                llp], i),
@@ -106,7 +107,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_server_destroy", "yasfhip_server_submit", "yasfhip_server_step", "yasfhip_server_poll",
         "yasfhip_server_autotune", "yasfhip_server_set_tile_shape", "yasfhip_server_set_table_policy",
         "yasfhip_set_slab_tiles", "yasfhip_set_near_field_radius", "yasfhip_mie_coefficients",
-        "yasfhip_planewave_coefficients", "yasfhip_server_submit_spheres",
+        "yasfhip_planewave_coefficients", "yasfhip_server_submit_spheres", "yasfhip_set_table_shape",
     ]
 
 
@@ -316,12 +317,19 @@ class Session:
         _check(self._lib.yasfhip_set_slab_tiles(self._h, int(tiles)))
 
     def tune_matvec(self, channel: int = 0, precision: int = FP64, reps: int = 3):
-        """Picks the table GEMM's column-tile grouping; returns (index, ms per grouping 2/4/8)."""
+        """Picks the table GEMM's shape; returns (index, ms per shape: 8 waves with
+        2/4/8 column tiles, then 4 waves with 2/4/8; -1 where not offered)."""
         g = ctypes.c_int()
-        ms = np.zeros(3)
+        ms = np.zeros(6)
         _check(self._lib.yasfhip_tune_matvec(self._h, int(channel), int(precision), int(reps),
                                              ctypes.byref(g), _ptr(ms)))
         return g.value, ms
+
+    TABLE_SHAPES = ((8, 2), (8, 4), (8, 8), (4, 2), (4, 4), (4, 8))  # (waves, column tiles) per index
+
+    def set_table_shape(self, shape: int, precision: int = FP64):
+        """Fixes the table GEMM shape (index into TABLE_SHAPES) without timing."""
+        _check(self._lib.yasfhip_set_table_shape(self._h, int(precision), int(shape)))
 
     def time_matvec(self, channel: int = 0, precision: int = FP64, reps: int = 5) -> tuple[float, float]:
         """Mean (pair-kernel ms, whole-matvec ms) over `reps` device-resident matvecs."""

@@ -31,6 +31,7 @@ void launch_residual(double2 *r, const double2 *b, const double2 *ax, long n, Re
 void launch_norm2(const double2 *x, long n, Reducer red, double *out, hipStream_t s);
 void launch_to_fp32(float *dst, const double *src, long count, hipStream_t s);
 void launch_to_fp64(double *dst, const float *src, long count, hipStream_t s);
+void launch_fill_test(double *dst, long count, hipStream_t s);  // timing input in [-1, 1)
 // observables.hip
 void launch_far_field(const double4 *pos, const double2 *f, int N, int lmax, double k,
                       const double4 *dir, const double *polar, const double *azimuthal, int A,
@@ -123,7 +124,7 @@ struct yasfhip_session {
     void *table[2] = {nullptr, nullptr};
     int table_channel[2] = {-1, -1};
     bool table_refused[2] = {false, false};
-    int table_grouping[2] = {0, 0};  // column-tile grouping of the table GEMM per precision
+    int table_grouping[2] = {0, 0};  // table GEMM shape per precision (waves x column tiles)
     double near2 = kAllPairs;          // squared near-field radius of W (kAllPairs: every pair)
     // Slab of the pair table (a few target tiles), rebuilt tile range by tile
     // range inside every matvec when the full table does not fit.
@@ -920,7 +921,7 @@ int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps
         YH_REQUIRE(reps >= 1, "reps must be >= 1");
         const bool fp32 = precision == YASFHIP_FP32;
         const int ix = fp32 ? 1 : 0;
-        for (int g = 0; g < kTableGroupings; ++g)
+        for (int g = 0; g < kTableShapes; ++g)
             ms[g] = -1.0;
         *grouping = s->table_grouping[ix];
         hipStream_t st = s->stream;
@@ -937,6 +938,8 @@ int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps
         } else {
             return;  // pair factors on the fly: nothing to tune
         }
+        // realistic operands: the GEMM's speed depends on its data
+        launch_fill_test(reinterpret_cast<double *>(s->xin), 2 * s->n, st);
         const void *xv = s->xin;
         if (fp32) {
             launch_to_fp32(reinterpret_cast<float *>(s->x32), reinterpret_cast<const double *>(s->xin),
@@ -944,9 +947,9 @@ int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps
             xv = s->x32;
         }
         double best = 1e300;
-        for (int g = 0; g < kTableGroupings; ++g) {
-            if (!fp32 && kTableGrouping[g] > 4)
-                continue;  // not offered in FP64 (register capacity)
+        for (int g = 0; g < kTableShapes; ++g) {
+            if (!table_shape_offered(g, fp32))
+                continue;  // wider FP64 jobs exceed a wave's registers
             launch_pair_table_gemm(s->dp, s->N, tab, xv, fp32, g, s->Q, st, 0, ntl);  // warm-up
             YH_CHECK(hipEventRecord(s->ev[0], st));
             for (int r = 0; r < reps; ++r)
@@ -965,6 +968,16 @@ int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps
     });
 }
 
+int yasfhip_set_table_shape(yasfhip_session *s, int precision, int shape) {
+    return guarded([&] {
+        YH_REQUIRE(s, "NULL argument");
+        YH_REQUIRE(precision == YASFHIP_FP64 || precision == YASFHIP_FP32, "unknown precision");
+        YH_REQUIRE(table_shape_offered(shape, precision == YASFHIP_FP32),
+                   "table GEMM shape not offered in this precision");
+        s->table_grouping[precision == YASFHIP_FP32 ? 1 : 0] = shape;
+    });
+}
+
 int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps, double *ms_pair,
                         double *ms_total) {
     return guarded([&] {
@@ -974,6 +987,7 @@ int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps
         const bool fp32 = precision == YASFHIP_FP32;
         hipStream_t st = s->stream;
         double pair_ms = 0.0, total_ms = 0.0;
+        launch_fill_test(reinterpret_cast<double *>(s->xin), 2 * s->n, st);  // realistic operands
         if (fp32)
             launch_to_fp32(reinterpret_cast<float *>(s->x32), reinterpret_cast<const double *>(s->xin),
                            2 * s->n, st);

@@ -144,7 +144,16 @@ size_t pair_table_tile_bytes(const DevicePlan &dp, int N, bool fp32);  // one ta
 void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, bool fp32,
                              void *table, hipStream_t s, int t0 = 0, int ntl = -1,
                              const int *gate = nullptr, bool regular = false, double near2 = kAllPairs);
-// grouping: index into kTableGrouping (column tiles per job).
+// Shapes of the resident-table GEMM: shape % kTableGroupings indexes
+// kTableGrouping (column tiles per job), shape / kTableGroupings picks 8 or
+// 4 waves per block.  Complex128 offers 8 waves with 2 or 4 column tiles and
+// 4 waves with 2 (wider jobs do not fit a wave's registers).
+constexpr int kTableShapes = 2 * kTableGroupings;
+constexpr bool table_shape_offered(int shape, bool fp32) {
+    return shape >= 0 && shape < kTableShapes &&
+           (fp32 ? shape != 5 : (shape == 0 || shape == 1 || shape == 3));
+}
+// grouping: a table GEMM shape (above).
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
                             int grouping, void *Q, hipStream_t s, int t0 = 0, int ntl = -1);
 

@@ -228,6 +228,25 @@ void launch_to_fp32(float *dst, const double *src, long count, hipStream_t s) {
     YH_CHECK(hipGetLastError());
 }
 
+// dst[i] = a deterministic pseudo-random value in [-1, 1) (a hash of i):
+// timing inputs, since the GEMMs' speed depends on their data (all-zero
+// operands draw less power and run faster than real ones).
+__global__ void fill_test_kernel(double *__restrict__ dst, long count) {
+    for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < count;
+         i += static_cast<long>(gridDim.x) * blockDim.x) {
+        unsigned long long h = static_cast<unsigned long long>(i) * 0x9E3779B97F4A7C15ull;
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 29;
+        dst[i] = static_cast<double>(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+}
+
+void launch_fill_test(double *dst, long count, hipStream_t s) {
+    hipLaunchKernelGGL(fill_test_kernel, dim3(grid_for(count)), dim3(kRedThreads), 0, s, dst, count);
+    YH_CHECK(hipGetLastError());
+}
+
 void launch_to_fp64(double *dst, const float *src, long count, hipStream_t s) {
     hipLaunchKernelGGL(to_fp64_kernel, dim3(grid_for(count)), dim3(kRedThreads), 0, s, dst, src, count);
     YH_CHECK(hipGetLastError());

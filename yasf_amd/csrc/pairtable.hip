@@ -57,14 +57,14 @@ __global__ __launch_bounds__(256) void pair_table_build_kernel(
         pair_series<T2>(dx, dy, dz, k, dm, P, valid, la, lb, ld, dst, 64);
 }
 
-constexpr int kTableWaves = 8;  // waves per (job, target tile) block
-
-// One block of 8 waves per (job, target tile).  Wave w owns the orders
-// ip = w, w+8, ... (< R) of the job and all of its (up to NTLMAX) column
+// One block of WAVES waves per (job, target tile).  Wave w owns the orders
+// ip = w, w+WAVES, ... (< R) of the job and all of its (up to NTLMAX) column
 // tiles, so each table fragment it loads feeds all of the job's columns.
+// Fewer waves give each wave more orders: more MFMAs per K-step to hide the
+// operand loads behind, at more accumulator registers.
 // The grid covers ntl target tiles from t0 (table points at tile t0).
-template <typename T, int NIPMAX, int NTLMAX>
-__global__ __launch_bounds__(512) void pair_table_gemm_kernel(
+template <typename T, int WAVES, int NIPMAX, int NTLMAX>
+__global__ __launch_bounds__(64 * WAVES) void pair_table_gemm_kernel(
     const typename Cplx<T>::type *__restrict__ table, const typename Cplx<T>::type *__restrict__ x, int N,
     int N4, int nmax, const PairJob *__restrict__ jobs, int ntl, int t0, const int *__restrict__ colmap,
     int maxcols, int lmax, const int *__restrict__ dmoff, T *__restrict__ Q, int nq) {
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(512) void pair_table_gemm_kernel(
     const int d = dm + 2 * lmax;
     const int lo = lane & 15;
     const int nord = uniform(jobs[jid].nord);  // orders computed (R is the table's order stride)
-    const int nip = nord > wave ? (nord - wave + kTableWaves - 1) / kTableWaves : 0;  // orders of this wave
+    const int nip = nord > wave ? (nord - wave + WAVES - 1) / WAVES : 0;  // orders of this wave
     if (nip == 0)
         return;
     const T2 *tab = table + (static_cast<size_t>(t) * P * P + dmoff[d]) * N4 * 64 + lane;
@@ -93,48 +93,50 @@ __global__ __launch_bounds__(512) void pair_table_gemm_kernel(
     T *Qt = Q + 2 * (static_cast<size_t>(t0 + t) * nq + qbase) * 256;
     // complex128 operands: a 3-deep pipeline; complex64: 2-deep keeps more waves resident
     constexpr int kDepth = sizeof(T) == sizeof(double) ? 3 : 2;
-    table_gemm_dispatch<T, 1, NIPMAX, NTLMAX, kTableWaves, kDepth>(tab, x, N, N4, nmax, R, wave, nip, ntile,
-                                                                   n2c, Qt, tstride, tile0);
+    table_gemm_dispatch<T, 1, NIPMAX, NTLMAX, WAVES, kDepth>(tab, x, N, N4, nmax, R, wave, nip, ntile, n2c, Qt,
+                                                             tstride, tile0);
 }
 
-template <typename T, int NTLMAX>
+template <typename T, int WAVES, int NTLMAX>
 void gemm_g(const DevicePlan &dp, int g, int N, const void *table, const void *x, void *Q, hipStream_t s,
             int t0, int ntl) {
     typedef typename Cplx<T>::type T2;
     const int N4 = (N + 3) / 4;
-    const dim3 grid(dp.ntjobs[g] * ntl), block(64 * kTableWaves);
+    const dim3 grid(dp.ntjobs[g] * ntl), block(64 * WAVES);
     const T2 *tb = static_cast<const T2 *>(table);
     const T2 *xv = static_cast<const T2 *>(x);
     T *q = static_cast<T *>(Q);
-    // orders per wave: ceil(R/8) <= ceil((2L+1)/8), 3 only at lmax 8
-    if (2 * dp.lmax + 1 <= 16)
-        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 2, NTLMAX>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
-                           dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
+    // orders per wave: ceil(R / WAVES) <= ceil((2L+1) / WAVES) (2L+1 <= 17)
+    constexpr int kNipLo = (15 + WAVES - 1) / WAVES, kNipHi = (17 + WAVES - 1) / WAVES;
+    if (2 * dp.lmax + 1 <= 15)
+        hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipLo, NTLMAX>), grid, block, 0, s, tb, xv, N, N4,
+                           dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
     else
-        hipLaunchKernelGGL((pair_table_gemm_kernel<T, 3, NTLMAX>), grid, block, 0, s, tb, xv, N, N4, dp.nmax,
-                           dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
+        hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipHi, NTLMAX>), grid, block, 0, s, tb, xv, N, N4,
+                           dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
     YH_CHECK(hipGetLastError());
 }
 
 template <typename T>
-void gemm_t(const DevicePlan &dp, int g, int N, const void *table, const void *x, void *Q, hipStream_t s,
+void gemm_t(const DevicePlan &dp, int shape, int N, const void *table, const void *x, void *Q, hipStream_t s,
             int t0, int ntl) {
     static_assert(kTableGrouping[0] == 2 && kTableGrouping[1] == 4 && kTableGrouping[2] == 8,
                   "instances follow the column-tile groupings");
-    switch (g) {
-    case 0:
-        gemm_g<T, 2>(dp, g, N, table, x, Q, s, t0, ntl);
-        break;
-    case 1:
-        gemm_g<T, 4>(dp, g, N, table, x, Q, s, t0, ntl);
-        break;
-    default:
-        // complex128 accumulators of 8 column tiles do not fit a wave's registers
-        if constexpr (sizeof(T) == sizeof(float))
-            gemm_g<T, 8>(dp, 2, N, table, x, Q, s, t0, ntl);
-        else
-            YH_REQUIRE(false, "the FP64 table GEMM supports jobs of at most 4 column tiles");
-        break;
+    YH_REQUIRE(table_shape_offered(shape, sizeof(T) == sizeof(float)),
+               "table GEMM shape not offered in this precision");
+    const int g = shape % kTableGroupings;
+    if (shape < kTableGroupings) {  // 8 waves
+        if (g == 0)
+            gemm_g<T, 8, 2>(dp, g, N, table, x, Q, s, t0, ntl);
+        else if (g == 1)
+            gemm_g<T, 8, 4>(dp, g, N, table, x, Q, s, t0, ntl);
+        else if constexpr (sizeof(T) == sizeof(float))
+            gemm_g<T, 8, 8>(dp, g, N, table, x, Q, s, t0, ntl);
+    } else {  // 4 waves
+        if (g == 0)
+            gemm_g<T, 4, 2>(dp, g, N, table, x, Q, s, t0, ntl);
+        else if constexpr (sizeof(T) == sizeof(float))
+            gemm_g<T, 4, 4>(dp, g, N, table, x, Q, s, t0, ntl);
     }
 }
 
@@ -176,7 +178,7 @@ void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, do
 
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
                             int grouping, void *Q, hipStream_t s, int t0, int ntl) {
-    YH_REQUIRE(grouping >= 0 && grouping < kTableGroupings, "unknown column-tile grouping");
+    YH_REQUIRE(grouping >= 0 && grouping < kTableShapes, "unknown table GEMM shape");
     const int ntiles1 = (N + 15) / 16;
     if (ntl < 0)
         ntl = ntiles1 - t0;
