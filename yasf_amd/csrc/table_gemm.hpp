@@ -38,9 +38,9 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
 #pragma unroll
             for (int c = 0; c < NACC; ++c)
                 acc[j][it][c] = acc_t{0, 0, 0, 0};
-    // DEPTH operand buffers in rotation: the loads of K-step ks+DEPTH-1 are
+    // DEPTH (3) operand buffers in rotation: the loads of K-step ks+2 are
     // issued before the MFMAs of step ks and only the oldest batch is waited
-    // for (a deeper pipeline hides more latency but costs registers).
+    // for (measured: two buffers are up to 7% slower, four no faster).
     // Loads are unconditional (clamped to valid addresses), so the wait
     // counts are static; B lanes outside the job or past the last source are
     // zeroed at use (their table entries are zero as well).
@@ -111,20 +111,9 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
                 }
         }
     };
-    if constexpr (DEPTH == 2) {
-        load(0, aA, bA);
-        int ks = 0;
-        for (; ks + 1 < N4; ks += 2) {
-            load(ks + 1, aB, bB);
-            mma(ks, aA, bA);
-            load(ks + 2, aA, bA);
-            mma(ks + 1, aB, bB);
-        }
-        if (ks < N4)
-            mma(ks, aA, bA);
-    } else {
-        static_assert(DEPTH == 3, "pipeline depth 2 or 3");
-        T2 aC[NIP], bC[NTL];
+    static_assert(DEPTH == 3, "a three-deep operand ring");
+    T2 aC[NIP], bC[NTL];
+    {
         load(0, aA, bA);
         load(1, aB, bB);
         int ks = 0;
