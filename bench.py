@@ -214,7 +214,9 @@ def run_solve(args):
     f_direct = direct_flops_per_pair(lmax) * pairs
     macs_issued, _ = _native.plan_stats(lmax)
     ntiles = (N + 15) // 16
-    f_issued = macs_issued * 8 * (ntiles * 16) * N  # MFMA flops incl. tile padding
+    # real MFMA flops issued, tile padding included: 3 real products (6 flop)
+    # per complex MAC in the FP64 GEMM (Gauss's form)
+    f_issued = macs_issued * 6 * (ntiles * 16) * N
     achieved = f_fact / (pair_ms * 1e-3) / 1e12
     waves64, tiles64 = _native.Session.TABLE_SHAPES[tuned["fp64"]]
     # instance name as rocprofv3 reports it: <T, waves, orders per wave (lmax <= 7), column tiles>
