@@ -96,6 +96,7 @@ struct CouplingPlan {
     std::vector<double> leg_alpha;    // [(2L+1)^2] Legendre recurrence alpha(l, m)
     std::vector<double> leg_beta;     // [(2L+1)^2] Legendre recurrence beta(l, m)
     std::vector<double> leg_diag;     // [2L+1] P_m^m = leg_diag[m] * sin^m
+    std::vector<int> order_tiles;     // [ndm][2L+1]: column tiles (prefix) of the dm that order p feeds
     long long pair_macs_per_pair = 0; // complex MACs the on-the-fly pair kernel issues per pair
     long long nnz_ab5 = 0;
 };
@@ -111,6 +112,7 @@ struct DevicePlan {
     double2 *cval;
     double *la, *lb, *ld;
     int *dmoff;  // [ndm] first order slot of each dm in a pair-table target tile
+    int *order_tiles;  // [ndm][2L+1] column tiles that order p feeds (a prefix of the dm's tiles)
 };
 
 // Records the message of the calling thread's last failure (yasfhip_last_error)

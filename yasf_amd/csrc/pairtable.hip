@@ -67,7 +67,8 @@ template <typename T, int WAVES, int NIPMAX, int NTLMAX>
 __global__ __launch_bounds__(64 * WAVES) void pair_table_gemm_kernel(
     const typename Cplx<T>::type *__restrict__ table, const typename Cplx<T>::type *__restrict__ x, int N,
     int N4, int nmax, const PairJob *__restrict__ jobs, int ntl, int t0, const int *__restrict__ colmap,
-    int maxcols, int lmax, const int *__restrict__ dmoff, T *__restrict__ Q, int nq) {
+    int maxcols, int lmax, const int *__restrict__ dmoff, const int *__restrict__ order_tiles,
+    T *__restrict__ Q, int nq) {
     typedef typename Cplx<T>::type T2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int jid = blockIdx.x / ntl;
@@ -94,7 +95,8 @@ __global__ __launch_bounds__(64 * WAVES) void pair_table_gemm_kernel(
     // a 3-deep operand pipeline (complex64 measured: 2-deep is 7% slower at
     // lmax 4 and 8, 4-deep no faster than 3)
     table_gemm_dispatch<T, 1, NIPMAX, NTLMAX, WAVES, 3>(tab, x, N, N4, nmax, R, wave, nip, ntile, n2c, Qt,
-                                                             tstride, tile0);
+                                                        tstride, tile0,
+                                                        order_tiles + d * P + (dm < 0 ? -dm : dm));
 }
 
 template <typename T, int WAVES, int NTLMAX>
@@ -110,10 +112,12 @@ void gemm_g(const DevicePlan &dp, int g, int N, const void *table, const void *x
     constexpr int kNipLo = (15 + WAVES - 1) / WAVES, kNipHi = (17 + WAVES - 1) / WAVES;
     if (2 * dp.lmax + 1 <= 15)
         hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipLo, NTLMAX>), grid, block, 0, s, tb, xv, N, N4,
-                           dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
+                           dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, dp.order_tiles, q,
+                           dp.nq);
     else
         hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipHi, NTLMAX>), grid, block, 0, s, tb, xv, N, N4,
-                           dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, q, dp.nq);
+                           dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, dp.order_tiles, q,
+                           dp.nq);
     YH_CHECK(hipGetLastError());
 }
 

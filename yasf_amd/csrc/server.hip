@@ -90,7 +90,7 @@ struct BatchReq {
     double *gc;           // [restart] Givens cosines
     double2 *gs;          // [restart] Givens sines
     const PairJob *jobs;
-    const int *colmap, *crow, *cidx, *dmoff;
+    const int *colmap, *crow, *cidx, *dmoff, *order_tiles;
     const double2 *cval;
     long n;
     int N, lmax, nmax, ntiles1, N4, nq, maxcols, chunks, restart;
@@ -176,7 +176,8 @@ __device__ __forceinline__ void serve_gemm_block(const BatchReq &q, int jid, int
         n2c[it] = (it < ntile) ? q.colmap[d * q.maxcols + (tile0 + it) * 16 + (lane & 15)] : -1;
     float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
     table_gemm_dispatch<float, 1, NIPMAX, NTLMAX, WAVES, 3>(tab, q.x32, q.N, q.N4, q.nmax, R, wave, nip, ntile,
-                                                            n2c, Qt, tstride, tile0);
+                                                            n2c, Qt, tstride, tile0,
+                                                            q.order_tiles + d * P + (dm < 0 ? -dm : dm));
 }
 
 // Complex64 pair GEMM over the resident tables of a group of requests that
@@ -1227,6 +1228,7 @@ struct yasfhip_server {
             b.crow = dp.crow;
             b.cidx = dp.cidx;
             b.dmoff = dp.dmoff;
+            b.order_tiles = dp.order_tiles;
             b.cval = dp.cval;
             b.n = q.n;
             b.N = q.N;

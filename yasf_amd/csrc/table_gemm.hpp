@@ -9,26 +9,38 @@
 namespace yasfhip {
 
 // One wave's GEMM with NIP orders (ip = wave + WAVES j) and NTL column tiles,
-// all compile-time, so the 4*NIP*NTL MFMAs of a K-step form one straight-line
-// block the scheduler can interleave.  tab points at the wave's target tile
-// and dm (plus the lane); n2c[it] is the source mode of the lane's column in
-// column tile it (-1: padding).  C tile (ip, tile0 + it) goes to slot
-// ip * tstride + tile0 + it past Qt.
+// all compile-time, so the MFMAs of a K-step form one straight-line block the
+// scheduler can interleave.  tab points at the wave's target tile and dm
+// (plus the lane); n2c[it] is the source mode of the lane's column in column
+// tile it (-1: padding).  otile[ip]: column tiles (from tile 0 of the dm)
+// that order ip feeds -- a prefix, since the dm's columns are sorted by l2 --
+// so the C tiles (ip, it) past it are skipped (left zero).  C tile
+// (ip, tile0 + it) goes to slot ip * tstride + tile0 + it past Qt.
 template <typename T, int NIP, int NTL, int WAVES, int DEPTH>
 __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__restrict__ tab,
                                                 const typename Cplx<T>::type *__restrict__ x, int N,
                                                 int N4, int nmax, int R, int wave,
                                                 const int (&n2c)[kTableMaxColTiles], T *__restrict__ Qt,
-                                                int tstride, int tile0) {
+                                                int tstride, int tile0, const int *__restrict__ otile) {
     typedef typename Mfma<T>::acc_t acc_t;
     typedef typename Cplx<T>::type T2;
     const int lane = threadIdx.x & 63, lo = lane & 15, hi = lane >> 4;
+    int tn[NIP];  // active column tiles of each of the wave's orders (uniform)
+#pragma unroll
+    for (int j = 0; j < NIP; ++j) {
+        const int t = uniform(otile[wave + WAVES * j]) - tile0;
+        tn[j] = t < 0 ? 0 : (t > NTL ? NTL : t);
+    }
     // Complex128: the complex product with three real MFMAs (Gauss): P1 =
     // Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 -
     // P2 at the end -- a quarter fewer FP64 MFMAs (measured: 7.04 -> 6.07 ms
     // at 2,000 spheres, lmax 6).  Complex64 keeps four (the third accumulator
     // costs it more occupancy than the saved MFMAs gain).
     constexpr bool kGaussComplex = sizeof(T) == sizeof(double);
+    // Complex64 skips the C tiles past each order's column prefix (the
+    // padding of the job's order range: -12% to -19% time at lmax 4..8);
+    // in complex128 the branches cost what the skipped MFMAs save.
+    constexpr bool kTrimTiles = sizeof(T) == sizeof(float);
     constexpr int NACC = kGaussComplex ? 3 : 2;
     acc_t acc[NIP][NTL][NACC];
 #pragma unroll
@@ -98,17 +110,19 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
 #pragma unroll
             for (int j = 0; j < NIP; ++j)
 #pragma unroll
-                for (int it = 0; it < NTL; ++it) {
-                    acc[j][it][0] = Mfma<T>::mac(a[j].x, b[it].x, acc[j][it][0]);
-                    acc[j][it][1] = Mfma<T>::mac(a[j].x, b[it].y, acc[j][it][1]);
-                }
+                for (int it = 0; it < NTL; ++it)
+                    if (!kTrimTiles || it < tn[j]) {
+                        acc[j][it][0] = Mfma<T>::mac(a[j].x, b[it].x, acc[j][it][0]);
+                        acc[j][it][1] = Mfma<T>::mac(a[j].x, b[it].y, acc[j][it][1]);
+                    }
 #pragma unroll
             for (int j = 0; j < NIP; ++j)
 #pragma unroll
-                for (int it = 0; it < NTL; ++it) {
-                    acc[j][it][0] = Mfma<T>::mac(-a[j].y, b[it].y, acc[j][it][0]);
-                    acc[j][it][1] = Mfma<T>::mac(a[j].y, b[it].x, acc[j][it][1]);
-                }
+                for (int it = 0; it < NTL; ++it)
+                    if (!kTrimTiles || it < tn[j]) {
+                        acc[j][it][0] = Mfma<T>::mac(-a[j].y, b[it].y, acc[j][it][0]);
+                        acc[j][it][1] = Mfma<T>::mac(a[j].y, b[it].x, acc[j][it][1]);
+                    }
         }
     };
     static_assert(DEPTH == 3, "a three-deep operand ring");
@@ -158,14 +172,15 @@ __device__ __forceinline__ void table_gemm_ntl(const typename Cplx<T>::type *__r
                                                const typename Cplx<T>::type *__restrict__ x, int N, int N4,
                                                int nmax, int R, int wave, int ntile,
                                                const int (&n2c)[kTableMaxColTiles], T *__restrict__ Qt,
-                                               int tstride, int tile0) {
+                                               int tstride, int tile0, const int *__restrict__ otile) {
     if constexpr (NTL <= NTLMAX) {
         if (ntile == NTL) {
-            table_gemm_wave<T, NIP, NTL, WAVES, DEPTH>(tab, x, N, N4, nmax, R, wave, n2c, Qt, tstride, tile0);
+            table_gemm_wave<T, NIP, NTL, WAVES, DEPTH>(tab, x, N, N4, nmax, R, wave, n2c, Qt, tstride, tile0,
+                                                       otile);
             return;
         }
         table_gemm_ntl<T, NIP, NTL + 1, NTLMAX, WAVES, DEPTH>(tab, x, N, N4, nmax, R, wave, ntile, n2c, Qt,
-                                                              tstride, tile0);
+                                                              tstride, tile0, otile);
     }
 }
 
@@ -174,15 +189,15 @@ __device__ __forceinline__ void table_gemm_dispatch(const typename Cplx<T>::type
                                                     const typename Cplx<T>::type *__restrict__ x, int N,
                                                     int N4, int nmax, int R, int wave, int nip, int ntile,
                                                     const int (&n2c)[kTableMaxColTiles], T *__restrict__ Qt,
-                                                    int tstride, int tile0) {
+                                                    int tstride, int tile0, const int *__restrict__ otile) {
     if constexpr (NIP <= NIPMAX) {
         if (nip == NIP) {
             table_gemm_ntl<T, NIP, 1, NTLMAX, WAVES, DEPTH>(tab, x, N, N4, nmax, R, wave, ntile, n2c, Qt,
-                                                            tstride, tile0);
+                                                            tstride, tile0, otile);
             return;
         }
         table_gemm_dispatch<T, NIP + 1, NIPMAX, NTLMAX, WAVES, DEPTH>(tab, x, N, N4, nmax, R, wave, nip, ntile,
-                                                                      n2c, Qt, tstride, tile0);
+                                                                      n2c, Qt, tstride, tile0, otile);
     }
 }
 

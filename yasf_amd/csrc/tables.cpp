@@ -155,6 +155,10 @@ CouplingPlan build_coupling_plan(int lmax) {
             }
         }
     }
+    plan.order_tiles.assign(static_cast<size_t>(plan.ndm) * P, 0);
+    for (int dmi = 0; dmi < plan.ndm; ++dmi)
+        for (int p = 0; p < P; ++p)
+            plan.order_tiles[static_cast<size_t>(dmi) * P + p] = need[dmi][p];
     // orders a job over column tiles [t0, ...) must compute: the prefix of
     // p = |dm|, |dm|+1, ... up to the last order that needs a tile >= t0
     auto orders_for = [&](int dmi, int t0) {
@@ -311,12 +315,13 @@ DevicePlan upload_device_plan(const CouplingPlan &pl, hipStream_t s) {
         acc += 2 * L + 1 - std::abs(d - 2 * L);
     }
     dp.dmoff = upload_vec(dmoff, s);
+    dp.order_tiles = upload_vec(pl.order_tiles, s);
     return dp;
 }
 
 void free_device_plan(DevicePlan &dp) {
-    void *bufs[] = {dp.jobs,  dp.tjobs[0], dp.tjobs[1], dp.tjobs[2], dp.colmap, dp.crow,
-                    dp.cidx,  dp.cval,     dp.la,       dp.lb,       dp.ld,     dp.dmoff};
+    void *bufs[] = {dp.jobs, dp.tjobs[0], dp.tjobs[1], dp.tjobs[2], dp.colmap, dp.crow, dp.cidx,
+                    dp.cval, dp.la,       dp.lb,       dp.ld,       dp.dmoff,  dp.order_tiles};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
