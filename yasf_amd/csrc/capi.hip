@@ -502,14 +502,14 @@ int yasfhip_translation_table(int lmax, double *out) {
     return guarded([&] {
         YH_REQUIRE(out, "out is NULL");
         YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
-        const auto ab5 = translation_table(lmax);
+        const auto &ab5 = translation_table_cached(lmax);
         std::memcpy(out, ab5.data(), ab5.size() * sizeof(cplx));
     });
 }
 
 int yasfhip_plan_stats(int lmax, long long *pair_macs_per_pair, long long *nnz_ab5) {
     return guarded([&] {
-        const auto plan = build_coupling_plan(lmax);
+        const auto &plan = coupling_plan_cached(lmax);
         if (pair_macs_per_pair)
             *pair_macs_per_pair = plan.pair_macs_per_pair;
         if (nnz_ab5)
@@ -531,7 +531,7 @@ int yasfhip_create(const double *positions, int particle_number, int lmax, const
         s->N = particle_number;
         s->L = lmax;
         s->W = channels;
-        s->plan = build_coupling_plan(lmax);
+        s->plan = coupling_plan_cached(lmax);
         s->nmax = s->plan.nmax;
         s->n = static_cast<long>(s->N) * s->nmax;
         s->ntiles1 = (s->N + 15) / 16;
@@ -798,7 +798,7 @@ int yasfhip_precond_build(yasfhip_session *s, int channel, const int *box_of_par
             s->precond = nullptr;
             s->precond_channel = -1;
         }
-        s->precond = precond_create(s->plan, translation_table(s->L), s->pos_host.data(), s->N,
+        s->precond = precond_create(s->plan, translation_table_cached(s->L), s->pos_host.data(), s->N,
                                     box_of_particle, num_boxes, s->k[channel],
                                     s->t + static_cast<size_t>(channel) * s->n, s->stream);
         s->precond_channel = channel;

@@ -56,6 +56,8 @@ std::vector<Mode> modes_of(int lmax);
 
 // ab5[j1][j2][p] (reference yasfpy/numerics.py:224-272), row-major complex.
 std::vector<cplx> translation_table(int lmax);
+// The same, built once per process and lmax (thread-safe).
+const std::vector<cplx> &translation_table_cached(int lmax);
 
 // One workgroup's share of the pair GEMM for one 16-particle target tile:
 // a signed dm, all orders p in [|dm|, 2L] (R = 2L+1-|dm| of them) and the
@@ -101,6 +103,8 @@ struct CouplingPlan {
     long long nnz_ab5 = 0;
 };
 CouplingPlan build_coupling_plan(int lmax);
+// The same, built once per process and lmax (thread-safe).
+const CouplingPlan &coupling_plan_cached(int lmax);
 
 // Device-side views of a CouplingPlan (coupling.hip, pairtable.hip).
 struct DevicePlan {

@@ -733,7 +733,7 @@ struct yasfhip_server {
         if (it != plans.end())
             return it->second;
         PlanEntry &e = plans[L];
-        e.plan = build_coupling_plan(L);
+        e.plan = coupling_plan_cached(L);
         e.dp = upload_device_plan(e.plan, stream);
         return e;
     }

@@ -5,6 +5,8 @@
 #include <cfloat>
 #include <cmath>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <tuple>
 
 #include "internal.hpp"
@@ -116,6 +118,39 @@ std::vector<cplx> translation_table(int lmax) {
 //   (W x)[s1][n1]   = sum_{p, n2} ab5[n2][n1][p] Q[s1][p][m2-m1][n2]
 // which regroups the reference's quadruple sum (cpu_numba.py:50-93) so the
 // pair-independent ab5 contraction leaves the O(N^2) loop.
+namespace {
+
+// Process-wide caches: ab5 and the plan depend on lmax only, and take up to
+// 0.12 s and 0.23 s to build at lmax 8 -- once per process instead of once
+// per session, preconditioner build or server.
+std::mutex g_cache_mutex;
+std::unique_ptr<const std::vector<cplx>> g_ab5[kMaxLmax + 1];
+std::unique_ptr<const CouplingPlan> g_plans[kMaxLmax + 1];
+
+}  // namespace
+
+const std::vector<cplx> &translation_table_cached(int lmax) {
+    YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    if (!g_ab5[lmax])
+        g_ab5[lmax] = std::make_unique<const std::vector<cplx>>(translation_table(lmax));
+    return *g_ab5[lmax];
+}
+
+const CouplingPlan &coupling_plan_cached(int lmax) {
+    YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
+    {
+        std::lock_guard<std::mutex> lock(g_cache_mutex);
+        if (g_plans[lmax])
+            return *g_plans[lmax];
+    }
+    auto plan = std::make_unique<const CouplingPlan>(build_coupling_plan(lmax));  // outside the lock
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    if (!g_plans[lmax])
+        g_plans[lmax] = std::move(plan);
+    return *g_plans[lmax];
+}
+
 CouplingPlan build_coupling_plan(int lmax) {
     YH_REQUIRE(lmax >= 1 && lmax <= kMaxLmax, "lmax must be in [1, 8]");
     CouplingPlan plan;
@@ -140,7 +175,7 @@ CouplingPlan build_coupling_plan(int lmax) {
     }
     // need[dmi][p]: column tiles of the dm that order p contributes to (one
     // past the last tile holding a column with a nonzero ab5 term at p).
-    const auto ab5 = translation_table(lmax);
+    const auto &ab5 = translation_table_cached(lmax);
     std::vector<std::vector<int>> need(plan.ndm, std::vector<int>(P, 0));
     for (int dmi = 0; dmi < plan.ndm; ++dmi) {
         const int dm = dmi - 2 * lmax;
