@@ -27,7 +27,7 @@ struct BlockPrecond {
     int *piv_off = nullptr;      // [nbox] offset of each block's pivots
     int *dinv_off = nullptr;     // [nbox] first 16-row diagonal block of each box
     double2 *blocks = nullptr;
-    int *pivots = nullptr;
+    int *pivots = nullptr;       // after the build: the row permutation of each box (see below)
     double2 *dinv = nullptr;     // per 16-row diagonal block: L11^-1 then U11^-1 (2 x 16 x 16)
     long long total_elems = 0;
     int max_m = 0;  // largest padded block size
@@ -293,11 +293,40 @@ __global__ __launch_bounds__(256) void lu_blocked_kernel(const int *__restrict__
     }
 }
 
+// Replaces each box's pivot sequence (row c swapped with piv[c], c = 0, 1,
+// ...) by the permutation it amounts to: after the kernel, piv[i] is the row
+// of the right-hand side that lands in row i, so the apply gathers in
+// parallel instead of swapping in sequence.  One block per box.
+__global__ __launch_bounds__(256) void pivots_to_permutation_kernel(const int *__restrict__ box_first, int nmax,
+                                                                    const int *__restrict__ piv_off,
+                                                                    int *__restrict__ pivots) {
+    extern __shared__ int idx[];
+    const int b = blockIdx.x;
+    const int mp = padded16((box_first[b + 1] - box_first[b]) * nmax);
+    int *piv = pivots + piv_off[b];
+    for (int i = threadIdx.x; i < mp; i += blockDim.x)
+        idx[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int c = 0; c < mp; ++c) {
+            const int p = piv[c];
+            if (p != c) {
+                const int t0 = idx[c];
+                idx[c] = idx[p];
+                idx[p] = t0;
+            }
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < mp; i += blockDim.x)
+        piv[i] = idx[i];
+}
+
 // z[box] = U^-1 L^-1 P r[box] with 16-row blocked substitutions; the
 // right-hand side lives in LDS (mp complex).  Each 16-row diagonal step is a
 // 16x16 product with the stored L11^-1 / U11^-1, the off-diagonal updates use
 // all threads (one row per thread, 16 contiguous entries).
-__global__ __launch_bounds__(256) void lu_apply_kernel(const int *__restrict__ box_first,
+constexpr int kApplyThreads = 1024;  // one block per box (measured: 6% faster than 256)
+__global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__restrict__ box_first,
                                                        const int *__restrict__ particles, int nmax,
                                                        const long long *__restrict__ blk_off,
                                                        const int *__restrict__ piv_off,
@@ -317,19 +346,11 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(const int *__restrict__ b
     const int *piv = pivots + piv_off[b];
     const double2 *D = dinv + static_cast<long>(dinv_off[b]) * 512;
     const int tid = threadIdx.x;
-    for (int i = tid; i < mp; i += blockDim.x)
-        y[i] = i < m ? r[static_cast<long>(particles[first + i / nmax]) * nmax + i % nmax]
-                     : make_double2(0.0, 0.0);
-    __syncthreads();
-    if (tid == 0)
-        for (int c = 0; c < mp; ++c) {
-            const int p = piv[c];
-            if (p != c) {
-                const double2 t0 = y[c];
-                y[c] = y[p];
-                y[p] = t0;
-            }
-        }
+    for (int i = tid; i < mp; i += blockDim.x) {  // y = P r: row i of y is row piv[i] of r
+        const int src = piv[i];
+        y[i] = src < m ? r[static_cast<long>(particles[first + src / nmax]) * nmax + src % nmax]
+                       : make_double2(0.0, 0.0);
+    }
     __syncthreads();
     const int nb = mp >> 4;
     for (int bb = 0; bb < nb; ++bb) {  // unit lower
@@ -512,6 +533,9 @@ BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &
         hipLaunchKernelGGL(lu_blocked_kernel, dim3(bp->nbox), dim3(256), 0, s, bp->box_first, nmax,
                            bp->blk_off, bp->piv_off, bp->blocks, bp->pivots, bp->dinv_off, bp->dinv);
         YH_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(pivots_to_permutation_kernel, dim3(bp->nbox), dim3(256), bp->max_m * sizeof(int), s,
+                           bp->box_first, nmax, bp->piv_off, bp->pivots);
+        YH_CHECK(hipGetLastError());
         YH_CHECK(hipStreamSynchronize(s));
         for (void *p : {static_cast<void *>(dpos), static_cast<void *>(drb), static_cast<void *>(derow),
                         static_cast<void *>(deg), static_cast<void *>(ddmoff), static_cast<void *>(dev),
@@ -537,7 +561,7 @@ void precond_destroy(BlockPrecond *p) {
 }
 
 void precond_apply(BlockPrecond *p, const double2 *r, double2 *z, hipStream_t s) {
-    hipLaunchKernelGGL(lu_apply_kernel, dim3(p->nbox), dim3(256), p->max_m * sizeof(double2), s,
+    hipLaunchKernelGGL(lu_apply_kernel, dim3(p->nbox), dim3(kApplyThreads), p->max_m * sizeof(double2), s,
                        p->box_first, p->particles, p->nmax, p->blk_off, p->piv_off, p->blocks, p->pivots,
                        p->dinv_off, p->dinv, r, z);
     YH_CHECK(hipGetLastError());
