@@ -34,6 +34,27 @@ struct Mfma<float> {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Raw buffer resource over [base, base + bytes) (bytes < 2^32): a load whose
+// per-lane offset lies at or past `bytes` returns zero instead of touching
+// memory.  Word 3 = 0x00020000: 32-bit data format, the value the CDNA
+// raw-buffer descriptors use.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *base, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, static_cast<int>(bytes), 0x00020000);
+}
+// Offset that is out of range for every resource made by buffer_rsrc.
+constexpr unsigned kBufferNull = 0x80000000u;
+
+// One complex element per lane from byte offset voff (per lane, range-checked)
+// plus soff (wave-uniform, not range-checked).
+__device__ __forceinline__ double2 buffer_load_cplx(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                                    double2 *) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ float2 buffer_load_cplx(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                                   float2 *) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 template <typename T>
 struct Cplx;
 template <>

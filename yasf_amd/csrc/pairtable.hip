@@ -86,7 +86,7 @@ __global__ __launch_bounds__(64 * WAVES) void pair_table_gemm_kernel(
     const int nip = nord > wave ? (nord - wave + WAVES - 1) / WAVES : 0;  // orders of this wave
     if (nip == 0)
         return;
-    const T2 *tab = table + (static_cast<size_t>(t) * P * P + dmoff[d]) * N4 * 64 + lane;
+    const T2 *tab = table + (static_cast<size_t>(t) * P * P + dmoff[d]) * N4 * 64;
     int n2c[kTableMaxColTiles];
 #pragma unroll
     for (int it = 0; it < kTableMaxColTiles; ++it)
@@ -187,6 +187,8 @@ void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, cons
     if (ntl < 0)
         ntl = ntiles1 - t0;
     YH_REQUIRE(t0 >= 0 && ntl >= 1 && t0 + ntl <= ntiles1, "table tile range outside the targets");
+    YH_REQUIRE(table_gemm_extent_ok(N, dp.nmax, dp.lmax, fp32 ? sizeof(float2) : sizeof(double2)),
+               "cluster too large for the table GEMM's 32-bit offsets");
     if (fp32)
         gemm_t<float>(dp, grouping, N, table, x, Q, s, t0, ntl);
     else

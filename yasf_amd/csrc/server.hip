@@ -169,7 +169,7 @@ __device__ __forceinline__ void serve_gemm_block(const BatchReq &q, int jid, int
     const int nip = nord > wave ? (nord - wave + WAVES - 1) / WAVES : 0;
     if (nip == 0)
         return;
-    const float2 *tab = tabtile + static_cast<size_t>(q.dmoff[d]) * q.N4 * 64 + lane;
+    const float2 *tab = tabtile + static_cast<size_t>(q.dmoff[d]) * q.N4 * 64;
     int n2c[kTableMaxColTiles];
 #pragma unroll
     for (int it = 0; it < kTableMaxColTiles; ++it)
@@ -1616,6 +1616,8 @@ void submit_request(yasfhip_server *s, const double *positions, int particle_num
     YH_REQUIRE(std::isfinite(k_medium) && k_medium > 0.0, "k_medium must be real, finite and positive");
     YH_REQUIRE(rtol > 0.0, "rtol must be > 0");
     YH_REQUIRE(max_cycles >= 1, "max_cycles must be >= 1");
+    YH_REQUIRE(table_gemm_extent_ok(particle_number, 2 * lmax * (lmax + 2), lmax, sizeof(double2)),
+               "particle_number too large for the pair GEMM's 32-bit offsets");
     if (operator_key != 0) {
         auto it = s->key_uses.find(operator_key);
         if (it != s->key_uses.end())

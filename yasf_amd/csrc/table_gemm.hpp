@@ -11,11 +11,13 @@ namespace yasfhip {
 // One wave's GEMM with NIP orders (ip = wave + WAVES j) and NTL column tiles,
 // all compile-time, so the MFMAs of a K-step form one straight-line block the
 // scheduler can interleave.  tab points at the wave's target tile and dm
-// (plus the lane); n2c[it] is the source mode of the lane's column in column
-// tile it (-1: padding).  otile[ip]: column tiles (from tile 0 of the dm)
-// that order ip feeds -- a prefix, since the dm's columns are sorted by l2 --
-// so the C tiles (ip, it) past it are skipped (left zero).  C tile
-// (ip, tile0 + it) goes to slot ip * tstride + tile0 + it past Qt.
+// (N4 K-steps x R orders of 64-lane fragments); n2c[it] is the source mode of
+// the lane's column in column tile it (-1: padding).  otile[ip]: column tiles
+// (from tile 0 of the dm) that order ip feeds -- a prefix, since the dm's
+// columns are sorted by l2 -- so the C tiles (ip, it) past it are skipped
+// (left zero).  C tile (ip, tile0 + it) goes to slot ip * tstride + tile0 + it
+// past Qt.  Byte extents: the table slice and x below 2^31 (buffer offsets),
+// checked by the launchers with table_gemm_extent_ok.
 template <typename T, int NIP, int NTL, int WAVES, int DEPTH>
 __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__restrict__ tab,
                                                 const typename Cplx<T>::type *__restrict__ x, int N,
@@ -53,34 +55,32 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
     // DEPTH (3) operand buffers in rotation: the loads of K-step ks+2 are
     // issued before the MFMAs of step ks and only the oldest batch is waited
     // for (measured: two buffers are up to 7% slower, four no faster).
-    // Loads are unconditional (clamped to valid addresses), so the wait
-    // counts are static; B lanes outside the job or past the last source are
-    // zeroed at use (their table entries are zero as well).
+    // Loads are unconditional, so the wait counts are static.  Both operands
+    // come through buffer loads: A at a wave-uniform offset (scalar) plus the
+    // lane's, B at the lane's offset into x, where lanes past the last source
+    // or in padding columns fall outside x and read zero -- no 64-bit
+    // address registers per load, no masking at use.
     T2 aA[NIP], bA[NTL], aB[NIP], bB[NTL];
-    size_t boff[NTL];
+    const int wv = uniform(wave);
+    const __amdgpu_buffer_rsrc_t ra = buffer_rsrc(tab, static_cast<size_t>(R) * N4 * 64 * sizeof(T2));
+    const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(x, static_cast<size_t>(N) * nmax * sizeof(T2));
+    const unsigned va = lane * sizeof(T2);
+    unsigned vb[NTL];
 #pragma unroll
     for (int it = 0; it < NTL; ++it)
-        boff[it] = n2c[it] >= 0 ? static_cast<size_t>(n2c[it]) : 0;
+        vb[it] = n2c[it] >= 0 ? static_cast<unsigned>((hi * nmax + n2c[it]) * sizeof(T2)) : kBufferNull;
     auto load = [&](int ks, T2 (&an)[NIP], T2 (&bn)[NTL]) {
         const int kc = ks < N4 ? ks : N4 - 1;
 #pragma unroll
         for (int j = 0; j < NIP; ++j)
-            an[j] = tab[(static_cast<size_t>(kc) * R + wave + WAVES * j) * 64];
-        const int s2 = kc * 4 + hi;
-        const size_t row = static_cast<size_t>(s2 < N ? s2 : 0) * nmax;
+            an[j] = buffer_load_cplx(ra, va, static_cast<unsigned>((kc * R + wv + WAVES * j) * 64 * sizeof(T2)),
+                                     static_cast<T2 *>(nullptr));
+        const unsigned kb = static_cast<unsigned>(kc * 4 * nmax * sizeof(T2));
 #pragma unroll
         for (int it = 0; it < NTL; ++it)
-            bn[it] = x[row + boff[it]];
+            bn[it] = buffer_load_cplx(rx, vb[it] + kb, 0, static_cast<T2 *>(nullptr));
     };
-    auto mma = [&](int ks, const T2 (&a)[NIP], const T2 (&bl)[NTL]) {
-        const bool srcv = ks * 4 + hi < N;
-        T2 b[NTL];
-#pragma unroll
-        for (int it = 0; it < NTL; ++it) {
-            const bool v = srcv && n2c[it] >= 0;
-            b[it].x = v ? bl[it].x : T(0);
-            b[it].y = v ? bl[it].y : T(0);
-        }
+    auto mma = [&](int ks, const T2 (&a)[NIP], const T2 (&b)[NTL]) {
         // one accumulator after another: consecutive MFMAs never update the
         // same accumulator
         if constexpr (kGaussComplex) {
@@ -163,6 +163,15 @@ __device__ __forceinline__ void table_gemm_wave(const typename Cplx<T>::type *__
                 }
             }
         }
+}
+
+// Host check of the 32-bit buffer offsets table_gemm_wave uses: the
+// (target tile, dm) slice of the table and x must stay below 2^31 bytes.
+inline bool table_gemm_extent_ok(int N, int nmax, int lmax, size_t elem_bytes) {
+    const size_t N4 = (static_cast<size_t>(N) + 3) / 4;
+    const size_t slice = static_cast<size_t>(2 * lmax + 1) * N4 * 64 * elem_bytes;
+    const size_t xb = static_cast<size_t>(N) * nmax * elem_bytes;
+    return slice < (size_t{1} << 31) && xb < (size_t{1} << 31);
 }
 
 // Runs the compile-time instance for a wave's (nip, ntile), nip <= NIPMAX and
