@@ -97,7 +97,8 @@ def test_autotune_and_tile_shapes(mixed_requests):
     srv = _native.Server(memory_bytes=2 << 30)
     shapes = _native.Server.TILE_SHAPES
     for L in (3, 5, 8):
-        usable = [(w, ct) for w, ct in shapes if -(-(2 * L + 1) // w) <= (3 if w == 8 else 5)]
+        # blocks of 4 and 8 waves take any lmax; 2-wave blocks up to 5 orders per wave
+        usable = [(w, ct) for w, ct in shapes if w >= 4 or -(-(2 * L + 1) // w) <= 5]
         best, ms = srv.autotune(L, particles=96, reps=2)
         assert best in usable
         for i, shape in enumerate(shapes):
@@ -107,7 +108,9 @@ def test_autotune_and_tile_shapes(mixed_requests):
         srv.set_tile_shape(8, 2, 2)   # 17 orders over 2 waves exceeds the kernel's range
     with pytest.raises(ValueError):
         srv.set_tile_shape(4, 2, 8)   # not an instantiated shape
-    # every usable fixed shape gives the same solution bit for bit
+    # every usable fixed shape gives the same solution to rounding (blocks cut
+    # leftover orders along K by their job's order count, which the shape's
+    # column-tile grouping sets)
     reqs = mixed_requests[:3]
     outs = []
     for waves, ct in ((4, 2), (8, 2), (4, 4), (8, 4), (8, 8)):
@@ -115,8 +118,11 @@ def test_autotune_and_tile_shapes(mixed_requests):
             srv.set_tile_shape(L, waves, ct)
         outs.append(serve(srv, reqs, keep_solutions=True))
     for r in reqs:
+        x0 = outs[0].results[r.index]["x"]
         for o in outs[1:]:
-            assert np.array_equal(outs[0].results[r.index]["x"], o.results[r.index]["x"])
+            x = o.results[r.index]["x"]
+            assert np.linalg.norm(x - x0) <= 1e-5 * np.linalg.norm(x0)
+            assert o.results[r.index]["c_ext"] == pytest.approx(outs[0].results[r.index]["c_ext"], rel=1e-6)
     srv.close()
 
 

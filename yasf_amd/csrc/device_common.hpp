@@ -37,9 +37,17 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 // Raw buffer resource over [base, base + bytes) (bytes < 2^32): a load whose
 // per-lane offset lies at or past `bytes` returns zero instead of touching
 // memory.  Word 3 = 0x00020000: 32-bit data format, the value the CDNA
-// raw-buffer descriptors use.
+// raw-buffer descriptors use.  base and bytes must be the same in every lane:
+// they are read from the first active lane, so the descriptor lives in
+// scalar registers (a descriptor the compiler cannot prove uniform turns every
+// load into a loop over the distinct values).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *base, size_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, static_cast<int>(bytes), 0x00020000);
+    const unsigned long long p = reinterpret_cast<unsigned long long>(base);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(p));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(p >> 32));
+    void *u = reinterpret_cast<void *>((static_cast<unsigned long long>(hi) << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(u, 0, __builtin_amdgcn_readfirstlane(static_cast<int>(bytes)),
+                                             0x00020000);
 }
 // Offset that is out of range for every resource made by buffer_rsrc.
 constexpr unsigned kBufferNull = 0x80000000u;

@@ -57,20 +57,18 @@ __global__ __launch_bounds__(256) void pair_table_build_kernel(
         pair_series<T2>(dx, dy, dz, k, dm, P, valid, la, lb, ld, dst, 64);
 }
 
-// One block of WAVES waves per (job, target tile).  Wave w owns the orders
-// ip = w, w+WAVES, ... (< R) of the job and all of its (up to NTLMAX) column
-// tiles, so each table fragment it loads feeds all of the job's columns.
-// Fewer waves give each wave more orders: more MFMAs per K-step to hide the
-// operand loads behind, at more accumulator registers.
-// The grid covers ntl target tiles from t0 (table points at tile t0).
+// One block of WAVES waves per (job, target tile), which deals the job's
+// orders out over its waves (table_gemm_block); each table fragment a wave
+// loads feeds all of the job's (up to NTLMAX) column tiles.  Fewer waves give
+// each wave more orders: more MFMAs per K-step to hide the operand loads
+// behind, at more accumulator registers.  The grid covers ntl target tiles
+// from t0 (table points at tile t0).
 template <typename T, int WAVES, int NIPMAX, int NTLMAX>
 __global__ __launch_bounds__(64 * WAVES) void pair_table_gemm_kernel(
     const typename Cplx<T>::type *__restrict__ table, const typename Cplx<T>::type *__restrict__ x, int N,
     int N4, int nmax, const PairJob *__restrict__ jobs, int ntl, int t0, const int *__restrict__ colmap,
     int maxcols, int lmax, const int *__restrict__ dmoff, const int *__restrict__ order_tiles,
     T *__restrict__ Q, int nq) {
-    typedef typename Cplx<T>::type T2;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int jid = blockIdx.x / ntl;
     const int t = blockIdx.x - jid * ntl;  // tile within the slab
     const int dm = uniform(jobs[jid].dm);
@@ -81,22 +79,18 @@ __global__ __launch_bounds__(64 * WAVES) void pair_table_gemm_kernel(
     const int P = 2 * lmax + 1;
     const int R = P - (dm < 0 ? -dm : dm);
     const int d = dm + 2 * lmax;
-    const int lo = lane & 15;
     const int nord = uniform(jobs[jid].nord);  // orders computed (R is the table's order stride)
-    const int nip = nord > wave ? (nord - wave + WAVES - 1) / WAVES : 0;  // orders of this wave
-    if (nip == 0)
-        return;
-    const T2 *tab = table + (static_cast<size_t>(t) * P * P + dmoff[d]) * N4 * 64;
-    int n2c[kTableMaxColTiles];
-#pragma unroll
-    for (int it = 0; it < kTableMaxColTiles; ++it)
-        n2c[it] = (it < ntile) ? colmap[d * maxcols + (tile0 + it) * 16 + lo] : -1;
+    const TableGemmOperands<T> op{table + (static_cast<size_t>(t) * P * P + dmoff[d]) * N4 * 64,
+                                  x,
+                                  N,
+                                  N4,
+                                  nmax,
+                                  R,
+                                  tile0,
+                                  order_tiles + d * P + (dm < 0 ? -dm : dm),
+                                  colmap + d * maxcols + tile0 * 16};
     T *Qt = Q + 2 * (static_cast<size_t>(t0 + t) * nq + qbase) * 256;
-    // a 3-deep operand pipeline (complex64 measured: 2-deep is 7% slower at
-    // lmax 4 and 8, 4-deep no faster than 3)
-    table_gemm_dispatch<T, 1, NIPMAX, NTLMAX, WAVES, 3>(tab, x, N, N4, nmax, R, wave, nip, ntile, n2c, Qt,
-                                                        tstride, tile0,
-                                                        order_tiles + d * P + (dm < 0 ? -dm : dm));
+    table_gemm_block<T, WAVES, NIPMAX, NTLMAX>(op, nord, ntile, Qt, tstride);
 }
 
 template <typename T, int WAVES, int NTLMAX>
@@ -108,9 +102,9 @@ void gemm_g(const DevicePlan &dp, int g, int N, const void *table, const void *x
     const T2 *tb = static_cast<const T2 *>(table);
     const T2 *xv = static_cast<const T2 *>(x);
     T *q = static_cast<T *>(Q);
-    // orders per wave: ceil(R / WAVES) <= ceil((2L+1) / WAVES) (2L+1 <= 17)
-    constexpr int kNipLo = (15 + WAVES - 1) / WAVES, kNipHi = (17 + WAVES - 1) / WAVES;
-    if (2 * dp.lmax + 1 <= 15)
+    // whole orders per wave for 2L+1 <= 15 and <= 17 orders
+    constexpr int kNipLo = table_gemm_nipmax<WAVES>(15), kNipHi = table_gemm_nipmax<WAVES>(17);
+    if (2 * dp.lmax + 1 <= 15 || kNipLo == kNipHi)
         hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipLo, NTLMAX>), grid, block, 0, s, tb, xv, N, N4,
                            dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, dp.order_tiles, q,
                            dp.nq);

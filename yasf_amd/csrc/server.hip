@@ -153,9 +153,14 @@ __global__ __launch_bounds__(kVecThreads) void serve_prep_kernel(const BatchReq 
 
 // One (job, target tile t) block of a request's complex64 pair GEMM; tab
 // points at target tile t of the request's pair table (or of a slab of it).
+// Orders per wave the serving GEMM instances hold: whole orders of blocks of
+// 4 and 8 waves at lmax 8, and of 2-wave blocks up to lmax 4.
+constexpr int kServeNip2 = 5;
+constexpr int kServeNip4 = table_gemm_nipmax<4>(2 * kMaxLmax + 1);
+constexpr int kServeNip8 = table_gemm_nipmax<8>(2 * kMaxLmax + 1);
+
 template <int WAVES, int NIPMAX, int NTLMAX>
 __device__ __forceinline__ void serve_gemm_block(const BatchReq &q, int jid, int t, const float2 *tabtile) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int dm = uniform(q.jobs[jid].dm);
     const int tile0 = uniform(q.jobs[jid].tile0);
     const int ntile = uniform(q.jobs[jid].ntile);
@@ -166,18 +171,17 @@ __device__ __forceinline__ void serve_gemm_block(const BatchReq &q, int jid, int
     const int R = P - (dm < 0 ? -dm : dm);
     const int d = dm + 2 * lmax;
     const int nord = uniform(q.jobs[jid].nord);  // orders computed (R is the table's order stride)
-    const int nip = nord > wave ? (nord - wave + WAVES - 1) / WAVES : 0;
-    if (nip == 0)
-        return;
-    const float2 *tab = tabtile + static_cast<size_t>(q.dmoff[d]) * q.N4 * 64;
-    int n2c[kTableMaxColTiles];
-#pragma unroll
-    for (int it = 0; it < kTableMaxColTiles; ++it)
-        n2c[it] = (it < ntile) ? q.colmap[d * q.maxcols + (tile0 + it) * 16 + (lane & 15)] : -1;
+    const TableGemmOperands<float> op{tabtile + static_cast<size_t>(q.dmoff[d]) * q.N4 * 64,
+                                      q.x32,
+                                      q.N,
+                                      q.N4,
+                                      q.nmax,
+                                      R,
+                                      tile0,
+                                      q.order_tiles + d * P + (dm < 0 ? -dm : dm),
+                                      q.colmap + d * q.maxcols + tile0 * 16};
     float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
-    table_gemm_dispatch<float, 1, NIPMAX, NTLMAX, WAVES, 3>(tab, q.x32, q.N, q.N4, q.nmax, R, wave, nip, ntile,
-                                                            n2c, Qt, tstride, tile0,
-                                                            q.order_tiles + d * P + (dm < 0 ? -dm : dm));
+    table_gemm_block<float, WAVES, NIPMAX, NTLMAX>(op, nord, ntile, Qt, tstride);
 }
 
 // Complex64 pair GEMM over the resident tables of a group of requests that
@@ -910,13 +914,13 @@ struct yasfhip_server {
                                hipHostMallocDefault));
     }
 
-    // Orders per wave, ceil((2L+1)/waves), must not exceed the kernel's NIPMAX.
+    // Whole orders per wave must not exceed the kernel's NIPMAX (blocks of
+    // 4 and 8 waves cut the leftover orders along K, so any lmax fits).
     static bool shape_valid(int L, int shape) {
         if (shape < 0 || shape >= kShapes)
             return false;
         const int waves = kShape[shape].waves;
-        const int nipmax = waves == 8 ? 3 : 5;
-        return (2 * L + 1 + waves - 1) / waves <= nipmax;
+        return waves >= 4 || (2 * L + 1 + waves - 1) / waves <= kServeNip2;
     }
 
     static int size_class(int N) { return N >= kLargeRequest ? 1 : 0; }
@@ -1318,27 +1322,27 @@ struct yasfhip_server {
         static_assert(kShapes == 6, "one instance per entry of kShape");
         switch (shape) {
         case 0:
-            hipLaunchKernelGGL((serve_gemm_kernel<2, 5, 2>), dim3(blocks), dim3(128), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_gemm_kernel<2, kServeNip2, 2>), dim3(blocks), dim3(128), 0, st, d_reqs, d_state,
                                gstart, gidx, count);
             break;
         case 1:
-            hipLaunchKernelGGL((serve_gemm_kernel<4, 5, 2>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_gemm_kernel<4, kServeNip4, 2>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
                                gstart, gidx, count);
             break;
         case 2:
-            hipLaunchKernelGGL((serve_gemm_kernel<8, 3, 2>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_gemm_kernel<8, kServeNip8, 2>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
                                gstart, gidx, count);
             break;
         case 3:
-            hipLaunchKernelGGL((serve_gemm_kernel<4, 5, 4>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_gemm_kernel<4, kServeNip4, 4>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
                                gstart, gidx, count);
             break;
         case 4:
-            hipLaunchKernelGGL((serve_gemm_kernel<8, 3, 4>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_gemm_kernel<8, kServeNip8, 4>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
                                gstart, gidx, count);
             break;
         default:
-            hipLaunchKernelGGL((serve_gemm_kernel<8, 3, 8>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_gemm_kernel<8, kServeNip8, 8>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
                                gstart, gidx, count);
             break;
         }
@@ -1364,27 +1368,27 @@ struct yasfhip_server {
         const float2 *slab = o.table32 ? o.table : static_cast<const float2 *>(o.slab);
         switch (shape) {
         case 0:
-            hipLaunchKernelGGL((serve_slab_gemm_kernel<2, 5, 2>), dim3(blocks), dim3(128), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<2, kServeNip2, 2>), dim3(blocks), dim3(128), 0, st, d_reqs, d_state,
                                ridx, g.count, slab, t0, ntl);
             break;
         case 1:
-            hipLaunchKernelGGL((serve_slab_gemm_kernel<4, 5, 2>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<4, kServeNip4, 2>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
                                ridx, g.count, slab, t0, ntl);
             break;
         case 2:
-            hipLaunchKernelGGL((serve_slab_gemm_kernel<8, 3, 2>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<8, kServeNip8, 2>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
                                ridx, g.count, slab, t0, ntl);
             break;
         case 3:
-            hipLaunchKernelGGL((serve_slab_gemm_kernel<4, 5, 4>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<4, kServeNip4, 4>), dim3(blocks), dim3(256), 0, st, d_reqs, d_state,
                                ridx, g.count, slab, t0, ntl);
             break;
         case 4:
-            hipLaunchKernelGGL((serve_slab_gemm_kernel<8, 3, 4>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<8, kServeNip8, 4>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
                                ridx, g.count, slab, t0, ntl);
             break;
         default:
-            hipLaunchKernelGGL((serve_slab_gemm_kernel<8, 3, 8>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
+            hipLaunchKernelGGL((serve_slab_gemm_kernel<8, kServeNip8, 8>), dim3(blocks), dim3(512), 0, st, d_reqs, d_state,
                                ridx, g.count, slab, t0, ntl);
             break;
         }
