@@ -22,6 +22,10 @@ struct Reducer {
 };
 void launch_mgs_step(double2 *w, const double2 *v_prev, const double2 *h_prev, const double2 *v_next,
                      long n, Reducer red, double *out, hipStream_t s);
+// chained modified-Gram-Schmidt steps; partials buffers of 512 x 4 doubles
+void launch_mgs_chain(double2 *w, const double2 *v_prev, const double *part_prev, const double2 *v_next,
+                      long n, double *part_out, double *h_out, hipStream_t s);
+void launch_partials_sum(const double *part, long n, double *out, hipStream_t s);
 void launch_scale(double2 *y, const double2 *x, double a, long n, hipStream_t s);
 void launch_add(double2 *x, const double2 *d, long n, hipStream_t s);
 void launch_update(double2 *x, const double2 *V, const double2 *y, int m, long n, long ld,
@@ -114,6 +118,8 @@ struct yasfhip_session {
             *ycoef = nullptr;
     double2 *rb = nullptr, *rx = nullptr, *rr = nullptr;  // iterative refinement: b, x, residual
     double *scal = nullptr, *partials = nullptr;
+    static constexpr int kMgsPartials = 512 * 4;  // one launch's partials (krylov.hip grid x 4)
+    double *mgs_part = nullptr;                   // two of them, used in turn
     unsigned *counter = nullptr;
     double *scal_host = nullptr;
     double *hdev = nullptr, *hhost = nullptr;  // per-step MGS results, 4 doubles per slot
@@ -137,9 +143,8 @@ struct yasfhip_session {
     ~yasfhip_session() {
         if (precond)
             precond_destroy(precond);
-        void *bufs[] = {pos,  Q,    t,    xin,   yout,    x32,     V,    bvec,
-                        xvec, rvec, avec, ycoef, scal, partials, counter, hdev, rb,
-                        rx,   rr};
+        void *bufs[] = {pos,  Q,    t,    xin,   yout,    x32,     V,    bvec,     xvec, rvec,
+                        avec, ycoef, scal, partials, counter, hdev, rb, rx, rr, mgs_part};
         for (void *b : bufs)
             if (b)
                 (void)hipFree(b);
@@ -270,6 +275,8 @@ struct yasfhip_session {
         if (hhost)
             YH_CHECK(hipHostFree(hhost));
         V = dmalloc<double2>(static_cast<size_t>(restart + 1) * n);
+        if (!mgs_part)
+            mgs_part = dmalloc<double>(2 * kMgsPartials);
         ycoef = dmalloc<double2>(restart + 1);
         hdev = dmalloc<double>(4 * static_cast<size_t>(restart + 2));
         YH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&hhost), 4 * (restart + 2) * sizeof(double),
@@ -371,17 +378,18 @@ struct yasfhip_session {
                     matvec_dev(channel, vc, w, 1, fp32);
                 }
                 // Modified Gram-Schmidt, one fused launch per basis vector:
-                // launch kq subtracts h[kq-1] v[kq-1] and forms <v[kq], w>;
-                // slot kq of hdev holds (Re, Im, ||w||^2) of launch kq.
+                // launch kq subtracts h[kq-1] v[kq-1] and forms the partials
+                // of <v[kq], w> (buffer kq % 2); the next launch sums them and
+                // stores (Re, Im, ||w||^2) of launch kq to slot kq of hdev.
+                auto part = [&](int launch) { return mgs_part + (launch & 1) * kMgsPartials; };
                 for (int kq = 0; kq <= col; ++kq) {
                     const double2 *vk = V + static_cast<size_t>(kq) * n;
                     const double2 *vp = kq > 0 ? V + static_cast<size_t>(kq - 1) * n : nullptr;
-                    const double2 *hp =
-                        kq > 0 ? reinterpret_cast<const double2 *>(hdev + 4 * (kq - 1)) : nullptr;
-                    launch_mgs_step(w, vp, hp, vk, n, reducer(), hdev + 4 * kq, st);
+                    launch_mgs_chain(w, vp, part(kq + 1), vk, n, part(kq), kq > 0 ? hdev + 4 * (kq - 1) : nullptr,
+                                     st);
                 }
-                launch_mgs_step(w, vc, reinterpret_cast<const double2 *>(hdev + 4 * col), nullptr, n,
-                                reducer(), hdev + 4 * (col + 1), st);
+                launch_mgs_chain(w, vc, part(col), nullptr, n, part(col + 1), hdev + 4 * col, st);
+                launch_partials_sum(part(col + 1), n, hdev + 4 * (col + 1), st);
                 YH_CHECK(hipMemcpyAsync(hhost, hdev, 4 * (col + 2) * sizeof(double), hipMemcpyDeviceToHost,
                                         st));
                 YH_CHECK(hipStreamSynchronize(st));

@@ -108,6 +108,101 @@ __global__ __launch_bounds__(kRedThreads) void mgs_step_kernel(double2 *__restri
     finish_reduction<3>(acc, red, out);
 }
 
+// Sum of the per-block partials of one launch (nblk rows of 4 doubles, the
+// first 3 used) in a fixed order, left in every thread.
+__device__ __forceinline__ void sum_partials3(const double *__restrict__ part, int nblk, double (&tot)[3]) {
+    __shared__ double sh[kRedThreads / 64][3];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            a[i] += part[b * 4 + i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double v = wave_sum(a[i]);
+        if (lane == 0)
+            sh[w][i] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double v = 0.0;
+        for (int j = 0; j < kRedThreads / 64; ++j)
+            v += sh[j][i];
+        tot[i] = v;
+    }
+    __syncthreads();
+}
+
+// One modified-Gram-Schmidt step without a grid-wide reduction inside the
+// launch: the previous step's partials (part_prev, nprev blocks) are summed
+// by every block at the start, giving h = <v_prev, w> of that step (block 0
+// stores it with that step's ||w||^2 to h_out), then w -= h v_prev, and this
+// step's partials of <v_next, w> and ||w||^2 go to part_out (one row per
+// block).  A chain of these needs no fence or atomic: each launch boundary
+// orders the partials (a device-scope release per block would write back
+// the L2 of every XCD).
+__global__ __launch_bounds__(kRedThreads) void mgs_chain_kernel(double2 *__restrict__ w,
+                                                                const double2 *__restrict__ v_prev,
+                                                                const double *__restrict__ part_prev, int nprev,
+                                                                const double2 *__restrict__ v_next, long n,
+                                                                double *__restrict__ part_out,
+                                                                double *__restrict__ h_out) {
+    double2 h = make_double2(0.0, 0.0);
+    if (v_prev) {
+        double tot[3];
+        sum_partials3(part_prev, nprev, tot);
+        h = make_double2(tot[0], tot[1]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            h_out[0] = tot[0];
+            h_out[1] = tot[1];
+            h_out[2] = tot[2];
+        }
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x) {
+        double2 wi = w[i];
+        if (v_prev) {
+            const double2 vp = v_prev[i];
+            wi.x -= h.x * vp.x - h.y * vp.y;
+            wi.y -= h.x * vp.y + h.y * vp.x;
+            w[i] = wi;
+        }
+        if (v_next) {
+            const double2 vn = v_next[i];
+            acc[0] += vn.x * wi.x + vn.y * wi.y;
+            acc[1] += vn.x * wi.y - vn.y * wi.x;
+        }
+        acc[2] += wi.x * wi.x + wi.y * wi.y;
+    }
+    __shared__ double sh[kRedThreads / 64][3];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double v = wave_sum(acc[i]);
+        if (lane == 0)
+            sh[wv][i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double v = 0.0;
+        for (int j = 0; j < kRedThreads / 64; ++j)
+            v += sh[j][threadIdx.x];
+        part_out[blockIdx.x * 4 + threadIdx.x] = v;
+    }
+}
+
+// out[0..3) = the summed partials of a launch (one block).
+__global__ __launch_bounds__(kRedThreads) void partials_sum_kernel(const double *__restrict__ part, int nblk,
+                                                                   double *__restrict__ out) {
+    double tot[3];
+    sum_partials3(part, nblk, tot);
+    if (threadIdx.x < 3)
+        out[threadIdx.x] = tot[threadIdx.x];
+}
+
 // y = a * x (real scale), in place allowed.
 __global__ void scale_kernel(double2 *__restrict__ y, const double2 *__restrict__ x, double a, long n) {
     for (long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; i < n;
@@ -180,6 +275,20 @@ void launch_mgs_step(double2 *w, const double2 *v_prev, const double2 *h_prev, c
                      long n, Reducer red, double *out, hipStream_t s) {
     hipLaunchKernelGGL(mgs_step_kernel, dim3(grid_for(n)), dim3(kRedThreads), 0, s, w, v_prev, h_prev,
                        v_next, n, red, out);
+    YH_CHECK(hipGetLastError());
+}
+
+// Partials buffers of launch_mgs_chain hold kRedBlocks rows of 4 doubles.
+void launch_mgs_chain(double2 *w, const double2 *v_prev, const double *part_prev, const double2 *v_next,
+                      long n, double *part_out, double *h_out, hipStream_t s) {
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(mgs_chain_kernel, dim3(g), dim3(kRedThreads), 0, s, w, v_prev, part_prev, g, v_next, n,
+                       part_out, h_out);
+    YH_CHECK(hipGetLastError());
+}
+
+void launch_partials_sum(const double *part, long n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(partials_sum_kernel, dim3(1), dim3(kRedThreads), 0, s, part, grid_for(n), out);
     YH_CHECK(hipGetLastError());
 }
 
