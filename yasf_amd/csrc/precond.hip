@@ -322,11 +322,21 @@ __global__ __launch_bounds__(256) void pivots_to_permutation_kernel(const int *_
         piv[i] = idx[i];
 }
 
+// Barrier for LDS traffic only: waits for this wave's LDS operations, not
+// for its global loads, so loads issued before it stay in flight across it
+// (__syncthreads also waits for every outstanding global load).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // z[box] = U^-1 L^-1 P r[box] with 16-row blocked substitutions; the
 // right-hand side lives in LDS (mp complex).  Each 16-row diagonal step is a
 // 16x16 product with the stored L11^-1 / U11^-1, the off-diagonal updates use
-// all threads (one row per thread, 16 contiguous entries).
-constexpr int kApplyThreads = 1024;  // one block per box (measured: 6% faster than 256)
+// all threads (one row per thread at a time, 16 contiguous entries).  A
+// thread's first kApplyRows rows of the next step are loaded as soon as the
+// current update is done, so the loads run under the two barriers and the
+// next diagonal product (rows past kApplyRows x blockDim.x of a step are
+// loaded when used).
+constexpr int kApplyThreads = 512;  // one block per box
+constexpr int kApplyRows = 2;       // rows per thread loaded a step ahead
 __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__restrict__ box_first,
                                                        const int *__restrict__ particles, int nmax,
                                                        const long long *__restrict__ blk_off,
@@ -347,34 +357,68 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
     const int *piv = pivots + piv_off[b];
     const double2 *D = dinv + static_cast<long>(dinv_off[b]) * 512;
     const int tid = threadIdx.x;
-    for (int i = tid; i < mp; i += blockDim.x) {  // y = P r: row i of y is row piv[i] of r
+    const int nthr = blockDim.x;
+    for (int i = tid; i < mp; i += nthr) {  // y = P r: row i of y is row piv[i] of r
         const int src = piv[i];
         y[i] = src < m ? r[static_cast<long>(particles[first + src / nmax]) * nmax + src % nmax]
                        : make_double2(0.0, 0.0);
     }
-    __syncthreads();
     const int nb = mp >> 4;
-    for (int bb = 0; bb < nb; ++bb) {  // unit lower
-        const int base = bb * 16;
-        if (tid < 16) {
-            const double2 *Li = D + static_cast<long>(bb) * 512 + tid * 16;
+    // pan[k]: the 16 entries of row lo + tid + k nthr in the step's column
+    // block, loaded one step ahead
+    double2 pan[kApplyRows][16];
+    auto load_rows = [&](int lo, int hi, int base) {
+#pragma unroll
+        for (int k = 0; k < kApplyRows; ++k) {
+            const int i = lo + tid + k * nthr;
+            if (i < hi) {
+                const double2 *row = A + static_cast<long>(i) * mp + base;
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    pan[k][q] = row[q];
+            }
+        }
+    };
+    // one 16-row step: diagonal product into y[base..base+16), then the rows
+    // [lo, hi) less the 16 new values times their columns of A
+    auto step = [&](int base, const double2 *Dblk, int lo, int hi) {
+        if (tid < 64) {  // wave 0: row tid / 4 of the 16 x 16 product, a quarter per lane
+            const int rr = tid >> 2, q0 = (tid & 3) * 4;
+            const double2 *Di = Dblk + rr * 16 + q0;
             double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const double2 v = cmul(Li[q], y[base + q]);
+            for (int q = 0; q < 4; ++q) {
+                const double2 v = cmul(Di[q], y[base + q0 + q]);
                 acc.x += v.x;
                 acc.y += v.y;
             }
-            yb[tid] = acc;
+#pragma unroll
+            for (int off = 1; off < 4; off <<= 1) {
+                acc.x += __shfl_xor(acc.x, off, 64);
+                acc.y += __shfl_xor(acc.y, off, 64);
+            }
+            if ((tid & 3) == 0)
+                yb[rr] = acc;
         }
-        __syncthreads();
+        lds_barrier();
         if (tid < 16)
             y[base + tid] = yb[tid];
         double2 yv[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q)
             yv[q] = yb[q];
-        for (int i = base + 16 + tid; i < mp; i += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < kApplyRows; ++k) {
+            const int i = lo + tid + k * nthr;
+            if (i < hi) {
+                double2 acc = y[i];
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    acc = csub(acc, cmul(pan[k][q], yv[q]));
+                y[i] = acc;
+            }
+        }
+        for (int i = lo + tid + kApplyRows * nthr; i < hi; i += nthr) {
             double2 acc = y[i];
             const double2 *row = A + static_cast<long>(i) * mp + base;
 #pragma unroll
@@ -382,39 +426,27 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
                 acc = csub(acc, cmul(row[q], yv[q]));
             y[i] = acc;
         }
-        __syncthreads();
-    }
-    for (int bb = nb - 1; bb >= 0; --bb) {  // upper with diagonal
+        asm volatile("" ::: "memory");  // the next step's row loads stay below this step's rows
+    };
+    load_rows(16, mp, 0);  // unit lower: step bb updates rows [base + 16, mp)
+    __syncthreads();
+    for (int bb = 0; bb < nb; ++bb) {
         const int base = bb * 16;
-        if (tid < 16) {
-            const double2 *Ui = D + static_cast<long>(bb) * 512 + 256 + tid * 16;
-            double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const double2 v = cmul(Ui[q], y[base + q]);
-                acc.x += v.x;
-                acc.y += v.y;
-            }
-            yb[tid] = acc;
-        }
-        __syncthreads();
-        if (tid < 16)
-            y[base + tid] = yb[tid];
-        double2 yv[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q)
-            yv[q] = yb[q];
-        for (int i = tid; i < base; i += blockDim.x) {
-            double2 acc = y[i];
-            const double2 *row = A + static_cast<long>(i) * mp + base;
-#pragma unroll
-            for (int q = 0; q < 16; ++q)
-                acc = csub(acc, cmul(row[q], yv[q]));
-            y[i] = acc;
-        }
-        __syncthreads();
+        step(base, D + static_cast<long>(bb) * 512, base + 16, mp);
+        if (bb + 1 < nb)
+            load_rows(base + 32, mp, base + 16);
+        else  // the upper pass starts at the last block
+            load_rows(0, base, base);
+        lds_barrier();
     }
-    for (int i = tid; i < m; i += blockDim.x)
+    for (int bb = nb - 1; bb >= 0; --bb) {  // upper with diagonal: step bb updates rows [0, base)
+        const int base = bb * 16;
+        step(base, D + static_cast<long>(bb) * 512 + 256, 0, base);
+        if (bb > 0)
+            load_rows(0, base - 16, base - 16);
+        lds_barrier();
+    }
+    for (int i = tid; i < m; i += nthr)
         z[static_cast<long>(particles[first + i / nmax]) * nmax + i % nmax] = y[i];
 }
 
