@@ -191,7 +191,11 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
 
 // (W x)[s1, n1] = sum_e ab5_e Q[e], then either y = W x (mode 0) or the
 // master operator y = x - t (W x) (mode 1, matrix_apply.py:104-110).
-// One block per (16-particle target tile, group of 16 target modes).
+// One block per (16-particle target tile, group of 16 target modes).  The
+// mode groups of a tile read the same Q rows, so they share an XCD (and its
+// L2): blocks are handed to the 8 XCDs in turn, and block 8 q + x runs tile
+// 8 (q / groups) + x, group q % groups (the grid is padded to a multiple of
+// 8 tiles; blocks past the last tile return).
 template <typename TQ, typename TX, typename TY>
 __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q, int nq,
                                                        const int *__restrict__ crow,
@@ -201,8 +205,11 @@ __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q,
                                                        const double2 *__restrict__ t, int mode,
                                                        TY *__restrict__ y) {
     const int groups = contract_groups(nmax);
-    const int s1t = blockIdx.x / groups;
-    contract_tile<TQ, TX, TY>(Q, nq, crow, cidx, cval, N, nmax, x, t, mode, y, s1t, blockIdx.x - s1t * groups);
+    const int q = blockIdx.x >> 3, xcd = blockIdx.x & 7;
+    const int s1t = (q / groups) * 8 + xcd;
+    if (s1t * 16 >= N)
+        return;
+    contract_tile<TQ, TX, TY>(Q, nq, crow, cidx, cval, N, nmax, x, t, mode, y, s1t, q % groups);
 }
 
 template <typename T, bool Regular>
@@ -245,7 +252,7 @@ size_t pair_lds_bytes(const DevicePlan &dp) { return 2 * static_cast<size_t>(dp.
 // x and y, so x - t W x is formed in FP64).
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
                      int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s) {
-    const int nblocks = (N + 15) / 16 * contract_groups(dp.nmax);
+    const int nblocks = ((N + 15) / 16 + 7) / 8 * 8 * contract_groups(dp.nmax);
     YH_REQUIRE(q_fp32 || !io_fp32, "complex64 vectors need the complex64 pair GEMM");
     if (io_fp32)
         hipLaunchKernelGGL((contract_kernel<float, float, float>), dim3(nblocks), dim3(256), 0, s,

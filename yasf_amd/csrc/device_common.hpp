@@ -276,13 +276,34 @@ __device__ __forceinline__ void contract_tile(const TQ *__restrict__ Q, int nq, 
     const size_t qrow = static_cast<size_t>(s1t) * nq * 16;
     double sr = 0.0, si = 0.0;
     const int e1 = crow[n1 + 1];
-    for (int e = crow[n1]; e < e1; ++e) {
-        const double2 a = cval[e];
-        const size_t qi = 2 * ((qrow + cidx[e]) * 16 + r);
-        const double qr = static_cast<double>(Q[qi]);
-        const double qim = static_cast<double>(Q[qi + 1]);
-        sr += a.x * qr - a.y * qim;
-        si += a.x * qim + a.y * qr;
+    // batches of kBatch entries: all their index, coefficient and Q loads are
+    // in flight together (one entry at a time waits two memory round trips
+    // per entry); entries past the row's end repeat its first with a zero
+    // coefficient, and the sum runs in entry order as before
+    constexpr int kBatch = 8;
+    for (int e = crow[n1]; e < e1; e += kBatch) {
+        const int cnt = min(kBatch, e1 - e);
+        int ci[kBatch];
+        double2 a[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int ee = u < cnt ? e + u : e;
+            ci[u] = cidx[ee];
+            a[u] = cval[ee];
+        }
+        double qr[kBatch], qim[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const size_t qi = 2 * ((qrow + ci[u]) * 16 + r);
+            qr[u] = static_cast<double>(Q[qi]);
+            qim[u] = static_cast<double>(Q[qi + 1]);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (u < cnt) {
+                sr += a[u].x * qr[u] - a[u].y * qim[u];
+                si += a[u].x * qim[u] + a[u].y * qr[u];
+            }
     }
     if (s1 < N) {
         const size_t j = static_cast<size_t>(s1) * nmax + n1;
