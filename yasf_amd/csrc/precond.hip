@@ -335,7 +335,7 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // current update is done, so the loads run under the two barriers and the
 // next diagonal product (rows past kApplyRows x blockDim.x of a step are
 // loaded when used).
-constexpr int kApplyThreads = 512;  // one block per box
+constexpr int kApplyThreads = 512;  // one block per box (at least 320: the diagonal staging)
 constexpr int kApplyRows = 2;       // rows per thread loaded a step ahead
 __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__restrict__ box_first,
                                                        const int *__restrict__ particles, int nmax,
@@ -349,6 +349,7 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
                                                        double2 *__restrict__ z) {
     extern __shared__ double2 y[];
     __shared__ double2 yb[16];
+    __shared__ double2 Dsh[256];
     const int b = blockIdx.x;
     const int first = box_first[b];
     const int m = (box_first[b + 1] - first) * nmax;
@@ -381,10 +382,22 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
     };
     // one 16-row step: diagonal product into y[base..base+16), then the rows
     // [lo, hi) less the 16 new values times their columns of A
-    auto step = [&](int base, const double2 *Dblk, int lo, int hi) {
+    // The step's 16 x 16 diagonal inverse goes through LDS (Dsh): threads
+    // 64..319 load it a step ahead, before their row loads, so the wait for it
+    // does not wait for those.
+    double2 dv = make_double2(0.0, 0.0);
+    auto stage_load = [&](const double2 *Dblk) {
+        if (tid >= 64 && tid < 64 + 256)
+            dv = Dblk[tid - 64];
+    };
+    auto stage_store = [&]() {
+        if (tid >= 64 && tid < 64 + 256)
+            Dsh[tid - 64] = dv;
+    };
+    auto step = [&](int base, int lo, int hi) {
         if (tid < 64) {  // wave 0: row tid / 4 of the 16 x 16 product, a quarter per lane
             const int rr = tid >> 2, q0 = (tid & 3) * 4;
-            const double2 *Di = Dblk + rr * 16 + q0;
+            const double2 *Di = Dsh + rr * 16 + q0;
             double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -428,22 +441,31 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
         }
         asm volatile("" ::: "memory");  // the next step's row loads stay below this step's rows
     };
+    stage_load(D);
     load_rows(16, mp, 0);  // unit lower: step bb updates rows [base + 16, mp)
+    stage_store();
     __syncthreads();
     for (int bb = 0; bb < nb; ++bb) {
         const int base = bb * 16;
-        step(base, D + static_cast<long>(bb) * 512, base + 16, mp);
-        if (bb + 1 < nb)
+        step(base, base + 16, mp);
+        if (bb + 1 < nb) {
+            stage_load(D + static_cast<long>(bb + 1) * 512);
             load_rows(base + 32, mp, base + 16);
-        else  // the upper pass starts at the last block
+        } else {  // the upper pass starts at the last block
+            stage_load(D + static_cast<long>(bb) * 512 + 256);
             load_rows(0, base, base);
+        }
+        stage_store();
         lds_barrier();
     }
     for (int bb = nb - 1; bb >= 0; --bb) {  // upper with diagonal: step bb updates rows [0, base)
         const int base = bb * 16;
-        step(base, D + static_cast<long>(bb) * 512 + 256, 0, base);
-        if (bb > 0)
+        step(base, 0, base);
+        if (bb > 0) {
+            stage_load(D + static_cast<long>(bb - 1) * 512 + 256);
             load_rows(0, base - 16, base - 16);
+            stage_store();
+        }
         lds_barrier();
     }
     for (int i = tid; i < m; i += nthr)
