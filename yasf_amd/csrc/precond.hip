@@ -124,11 +124,24 @@ __device__ __forceinline__ double2 cinv(double2 d) {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 // Blocked right-looking LU with partial pivoting (pivot = largest |re|+|im|
-// as LAPACK's izamax), one block of kLuWaves waves per box, in place on the padded
-// mp x mp block.  Per 16-column panel: unblocked panel factorisation, the
-// 16-row U12 = L11^-1 A12 solve, then A22 -= L21 U12 as 16x16 complex tiles on
-// the FP64 MFMA (16x16x4, four real products per complex product).
-constexpr int kLuWaves = 16;  // waves per box (measured: 9% faster than 4)
+// as LAPACK's izamax), one block of kLuWaves waves per box, in place on the
+// padded mp x mp block, with two levels of blocking:
+// * outer panels of kLuOuter columns; inside one, 16-column blocks: unblocked
+//   factorisation (pivot search, full-row swap, rank-1 updates within the 16
+//   columns), the 16x16 diagonal inverses, then the 16-row U = L^-1 A and
+//   the rank-16 update restricted to the outer panel's columns;
+// * then, for the columns right of the outer panel, U12 = L11^-1 A12 by block
+//   forward substitution (16-row blocks, stored inverses) and one rank-64
+//   update A22 -= L21 U12, with U12 staged in LDS kLuChunk columns at a time.
+// The trailing matrix is read and written once per outer panel instead of
+// once per 16 columns.  Complex products are four real FP64 MFMAs
+// (16x16x4); the C tile layout of the MFMA (rows 4 i + (lane >> 4), column
+// lane & 15) is also its B operand layout, so solved tiles feed the next
+// products from registers.
+constexpr int kLuWaves = 8;     // waves per box
+constexpr int kLuOuter = 64;    // columns per outer panel
+constexpr int kLuChunk = 128;   // trailing columns per LDS stage of U12
+constexpr size_t kLuLds = static_cast<size_t>(kLuOuter) * kLuChunk * sizeof(double2);  // 128 KB
 __global__ __launch_bounds__(64 * kLuWaves) void lu_blocked_kernel(const int *__restrict__ box_first, int nmax,
                                                          const long long *__restrict__ blk_off,
                                                          const int *__restrict__ piv_off,
@@ -140,6 +153,7 @@ __global__ __launch_bounds__(64 * kLuWaves) void lu_blocked_kernel(const int *__
     __shared__ int red_i[kLuWaves];
     __shared__ int piv_sh;
     __shared__ double2 Lsh[16][16];
+    extern __shared__ double2 Ush[];  // [kLuOuter][kLuChunk]
     const int b = blockIdx.x;
     const int m = (box_first[b + 1] - box_first[b]) * nmax;
     const int mp = padded16(m);
@@ -148,149 +162,262 @@ __global__ __launch_bounds__(64 * kLuWaves) void lu_blocked_kernel(const int *__
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lo = lane & 15, hi = lane >> 4;
     auto at = [&](int i, int j) -> double2 & { return A[static_cast<long>(i) * mp + j]; };
+    double2 *Dbox = dinv + static_cast<long>(dinv_off[b]) * 512;
 
-    for (int kb = 0; kb < mp; kb += 16) {
-        for (int j = 0; j < 16; ++j) {
-            const int c = kb + j;
-            double bv = -1.0;
-            int bi = c;
-            for (int i = c + tid; i < mp; i += blockDim.x) {
-                const double v = cabs1(at(i, c));
-                if (v > bv) {
-                    bv = v;
-                    bi = i;
-                }
-            }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ov = __shfl_down(bv, off, 64);
-                const int oi = __shfl_down(bi, off, 64);
-                if (ov > bv || (ov == bv && oi < bi)) {
-                    bv = ov;
-                    bi = oi;
-                }
-            }
-            if (lane == 0) {
-                red_v[wave] = bv;
-                red_i[wave] = bi;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                double v = red_v[0];
-                int ii = red_i[0];
-                for (int q = 1; q < kLuWaves; ++q)
-                    if (red_v[q] > v || (red_v[q] == v && red_i[q] < ii)) {
-                        v = red_v[q];
-                        ii = red_i[q];
+    for (int K0 = 0; K0 < mp; K0 += kLuOuter) {
+        const int pend = min(K0 + kLuOuter, mp);  // outer panel columns [K0, pend)
+        for (int kb = K0; kb < pend; kb += 16) {
+            for (int j = 0; j < 16; ++j) {
+                const int c = kb + j;
+                double bv = -1.0;
+                int bi = c;
+                for (int i = c + tid; i < mp; i += blockDim.x) {
+                    const double v = cabs1(at(i, c));
+                    if (v > bv) {
+                        bv = v;
+                        bi = i;
                     }
-                piv_sh = ii;
-                piv[c] = ii;
-            }
-            __syncthreads();
-            const int p = piv_sh;
-            if (p != c)
-                for (int jj = tid; jj < mp; jj += blockDim.x) {
-                    const double2 t0 = at(c, jj);
-                    at(c, jj) = at(p, jj);
-                    at(p, jj) = t0;
                 }
-            __syncthreads();
-            const double2 inv = cinv(at(c, c));
-            const int pend = kb + 16;
-            for (int i = c + 1 + tid; i < mp; i += blockDim.x) {
-                const double2 l = cmul(at(i, c), inv);
-                at(i, c) = l;
-                for (int jj = c + 1; jj < pend; ++jj)
-                    at(i, jj) = csub(at(i, jj), cmul(l, at(c, jj)));
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ov = __shfl_down(bv, off, 64);
+                    const int oi = __shfl_down(bi, off, 64);
+                    if (ov > bv || (ov == bv && oi < bi)) {
+                        bv = ov;
+                        bi = oi;
+                    }
+                }
+                if (lane == 0) {
+                    red_v[wave] = bv;
+                    red_i[wave] = bi;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    double v = red_v[0];
+                    int ii = red_i[0];
+                    for (int q = 1; q < kLuWaves; ++q)
+                        if (red_v[q] > v || (red_v[q] == v && red_i[q] < ii)) {
+                            v = red_v[q];
+                            ii = red_i[q];
+                        }
+                    piv_sh = ii;
+                    piv[c] = ii;
+                }
+                __syncthreads();
+                const int p = piv_sh;
+                if (p != c)
+                    for (int jj = tid; jj < mp; jj += blockDim.x) {
+                        const double2 t0 = at(c, jj);
+                        at(c, jj) = at(p, jj);
+                        at(p, jj) = t0;
+                    }
+                __syncthreads();
+                const double2 inv = cinv(at(c, c));
+                const int bend = kb + 16;
+                for (int i = c + 1 + tid; i < mp; i += blockDim.x) {
+                    const double2 l = cmul(at(i, c), inv);
+                    at(i, c) = l;
+                    for (int jj = c + 1; jj < bend; ++jj)
+                        at(i, jj) = csub(at(i, jj), cmul(l, at(c, jj)));
+                }
+                __syncthreads();
+            }
+            // Inverses of the diagonal 16x16 factors, kept for the
+            // substitutions: thread c < 16 solves L11 y = e_c (unit lower),
+            // thread 16 + c solves U11 z = e_c (upper).
+            double2 *Dinv = Dbox + static_cast<long>(kb / 16) * 512;
+            if (tid < 16) {
+                const int c = tid;
+                double2 yv[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+                    for (int q = c; q < r; ++q)
+                        v = csub(v, cmul(at(kb + r, kb + q), yv[q]));
+                    yv[r] = r < c ? make_double2(0.0, 0.0) : v;
+                    Lsh[r][c] = yv[r];
+                    Dinv[r * 16 + c] = yv[r];
+                }
+            } else if (tid < 32) {
+                const int c = tid - 16;
+                double2 zv[16];
+#pragma unroll
+                for (int r = 15; r >= 0; --r) {
+                    double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+                    for (int q = r + 1; q <= c; ++q)
+                        v = csub(v, cmul(at(kb + r, kb + q), zv[q]));
+                    zv[r] = r > c ? make_double2(0.0, 0.0) : cmul(v, cinv(at(kb + r, kb + r)));
+                    Dinv[256 + r * 16 + c] = zv[r];
+                }
             }
             __syncthreads();
-        }
-        // Inverses of the diagonal 16x16 factors, kept for the substitutions:
-        // thread c < 16 solves L11 y = e_c (unit lower), thread 16 + c solves
-        // U11 z = e_c (upper).
-        double2 *Dinv = dinv + static_cast<long>(dinv_off[b] + kb / 16) * 512;
-        if (tid < 16) {
-            const int c = tid;
-            double2 yv[16];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
-                for (int q = c; q < r; ++q)
-                    v = csub(v, cmul(at(kb + r, kb + q), yv[q]));
-                yv[r] = r < c ? make_double2(0.0, 0.0) : v;
-                Lsh[r][c] = yv[r];
-                Dinv[r * 16 + c] = yv[r];
-            }
-        } else if (tid < 32) {
-            const int c = tid - 16;
-            double2 zv[16];
-#pragma unroll
-            for (int r = 15; r >= 0; --r) {
-                double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
-                for (int q = r + 1; q <= c; ++q)
-                    v = csub(v, cmul(at(kb + r, kb + q), zv[q]));
-                zv[r] = r > c ? make_double2(0.0, 0.0) : cmul(v, cinv(at(kb + r, kb + r)));
-                Dinv[256 + r * 16 + c] = zv[r];
-            }
-        }
-        __syncthreads();
-        if (kb + 16 >= mp)
-            break;
-        const int nt = (mp - kb - 16) >> 4;
-        // U12 = Linv A12, 16x16 complex tiles on the MFMA (each wave owns whole
-        // column tiles, so reading and overwriting a tile needs no barrier).
-        for (int cb = wave; cb < nt; cb += kLuWaves) {
-            const int col0 = kb + 16 + cb * 16;
-            double2 bv[4];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk)
-                bv[kk] = at(kb + 4 * kk + hi, col0 + lo);
-            v4d cre = {0.0, 0.0, 0.0, 0.0}, cim = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const double2 a = Lsh[lo][4 * kk + hi];
-                cre = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bv[kk].x, cre, 0, 0, 0);
-                cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bv[kk].y, cre, 0, 0, 0);
-                cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bv[kk].y, cim, 0, 0, 0);
-                cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bv[kk].x, cim, 0, 0, 0);
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                at(kb + 4 * i + hi, col0 + lo) = make_double2(cre[i], cim[i]);
-        }
-        __syncthreads();
-        for (int rb = wave; rb < nt; rb += kLuWaves) {
-            const int row0 = kb + 16 + rb * 16;
-            double ar[4], ai[4];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const double2 v = at(row0 + lo, kb + 4 * kk + hi);
-                ar[kk] = v.x;
-                ai[kk] = v.y;
-            }
-            for (int cb = 0; cb < nt; ++cb) {
+            const int ntp = (pend - kb - 16) >> 4;  // column tiles right of the block, inside the panel
+            if (ntp == 0)
+                continue;
+            // U = Linv A for the block's 16 rows (each wave owns whole column
+            // tiles, so reading and overwriting a tile needs no barrier).
+            for (int cb = wave; cb < ntp; cb += kLuWaves) {
                 const int col0 = kb + 16 + cb * 16;
-                v4d cre, cim;
+                double2 bv[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const double2 v = at(row0 + 4 * i + hi, col0 + lo);
-                    cre[i] = v.x;
-                    cim[i] = v.y;
-                }
+                for (int kk = 0; kk < 4; ++kk)
+                    bv[kk] = at(kb + 4 * kk + hi, col0 + lo);
+                v4d cre = {0.0, 0.0, 0.0, 0.0}, cim = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
-                    const double2 bv2 = at(kb + 4 * kk + hi, col0 + lo);
-                    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.x, cre, 0, 0, 0);
-                    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[kk], bv2.y, cre, 0, 0, 0);
-                    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.y, cim, 0, 0, 0);
-                    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[kk], bv2.x, cim, 0, 0, 0);
+                    const double2 a = Lsh[lo][4 * kk + hi];
+                    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bv[kk].x, cre, 0, 0, 0);
+                    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bv[kk].y, cre, 0, 0, 0);
+                    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bv[kk].y, cim, 0, 0, 0);
+                    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bv[kk].x, cim, 0, 0, 0);
                 }
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    at(row0 + 4 * i + hi, col0 + lo) = make_double2(cre[i], cim[i]);
+                    at(kb + 4 * i + hi, col0 + lo) = make_double2(cre[i], cim[i]);
             }
+            __syncthreads();
+            // rank-16 update of the panel's remaining columns, all rows below
+            const int nrb = (mp - kb - 16) >> 4;
+            for (int rb = wave; rb < nrb; rb += kLuWaves) {
+                const int row0 = kb + 16 + rb * 16;
+                double ar[4], ai[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double2 v = at(row0 + lo, kb + 4 * kk + hi);
+                    ar[kk] = v.x;
+                    ai[kk] = v.y;
+                }
+                for (int cb = 0; cb < ntp; ++cb) {
+                    const int col0 = kb + 16 + cb * 16;
+                    v4d cre, cim;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const double2 v = at(row0 + 4 * i + hi, col0 + lo);
+                        cre[i] = v.x;
+                        cim[i] = v.y;
+                    }
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const double2 bv2 = at(kb + 4 * kk + hi, col0 + lo);
+                        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.x, cre, 0, 0, 0);
+                        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[kk], bv2.y, cre, 0, 0, 0);
+                        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.y, cim, 0, 0, 0);
+                        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[kk], bv2.x, cim, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        at(row0 + 4 * i + hi, col0 + lo) = make_double2(cre[i], cim[i]);
+                }
+            }
+            __syncthreads();
+        }
+        if (pend >= mp)
+            break;
+        const int nbi = (pend - K0) >> 4;  // 16-row blocks of the outer panel (<= 4)
+        // U12 = L11^-1 A12 for the columns right of the panel: per column
+        // tile, block forward substitution t_i = Linv_i (A12_i - sum_{j<i}
+        // L_ij t_j), the tiles held in registers (element kk of a tile is
+        // row 4 kk + hi: also the B operand of K-substep kk).
+        const int ntc = (mp - pend) >> 4;
+        for (int cb = wave; cb < ntc; cb += kLuWaves) {
+            const int col0 = pend + cb * 16;
+            double2 tv[kLuOuter / 16][4];
+#pragma unroll
+            for (int i = 0; i < kLuOuter / 16; ++i)
+                if (i < nbi)
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk)
+                        tv[i][kk] = at(K0 + 16 * i + 4 * kk + hi, col0 + lo);
+#pragma unroll
+            for (int i = 0; i < kLuOuter / 16; ++i) {
+                if (i >= nbi)
+                    break;
+                v4d cre, cim;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    cre[kk] = tv[i][kk].x;
+                    cim[kk] = tv[i][kk].y;
+                }
+#pragma unroll
+                for (int j = 0; j < i; ++j)
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const double2 a = at(K0 + 16 * i + lo, K0 + 16 * j + 4 * kk + hi);
+                        const double2 bt = tv[j][kk];
+                        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.x, bt.x, cre, 0, 0, 0);
+                        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bt.y, cre, 0, 0, 0);
+                        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.x, bt.y, cim, 0, 0, 0);
+                        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bt.x, cim, 0, 0, 0);
+                    }
+                const double2 *Li = Dbox + static_cast<long>(K0 / 16 + i) * 512;  // unit-lower inverse
+                v4d ure = {0.0, 0.0, 0.0, 0.0}, uim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double2 a = Li[lo * 16 + 4 * kk + hi];
+                    ure = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, cre[kk], ure, 0, 0, 0);
+                    ure = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, cim[kk], ure, 0, 0, 0);
+                    uim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, cim[kk], uim, 0, 0, 0);
+                    uim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, cre[kk], uim, 0, 0, 0);
+                }
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+                    tv[i][kk] = make_double2(ure[kk], uim[kk]);
+            }
+#pragma unroll
+            for (int i = 0; i < kLuOuter / 16; ++i)
+                if (i < nbi)
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk)
+                        at(K0 + 16 * i + 4 * kk + hi, col0 + lo) = tv[i][kk];
         }
         __syncthreads();
+        // A22 -= L21 U12 (K = pend - K0), U12 staged in LDS kLuChunk columns at
+        // a time; each wave holds the L21 fragments of one row block.
+        const int nks = (pend - K0) >> 2;  // K-substeps of 4
+        const int nrb = (mp - pend) >> 4;
+        for (int c0 = pend; c0 < mp; c0 += kLuChunk) {
+            const int cw = min(kLuChunk, mp - c0);
+            for (int e = tid; e < (pend - K0) * cw; e += blockDim.x) {
+                const int r = e / cw, c = e - r * cw;
+                Ush[r * kLuChunk + c] = at(K0 + r, c0 + c);
+            }
+            __syncthreads();
+            const int nct = cw >> 4;
+            for (int rb = wave; rb < nrb; rb += kLuWaves) {
+                const int row0 = pend + rb * 16;
+                double ar[kLuOuter / 4], ai[kLuOuter / 4];
+#pragma unroll
+                for (int kk = 0; kk < kLuOuter / 4; ++kk)
+                    if (kk < nks) {
+                        const double2 v = at(row0 + lo, K0 + 4 * kk + hi);
+                        ar[kk] = v.x;
+                        ai[kk] = v.y;
+                    }
+                for (int ct = 0; ct < nct; ++ct) {
+                    const int col0 = c0 + ct * 16;
+                    v4d cre, cim;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const double2 v = at(row0 + 4 * i + hi, col0 + lo);
+                        cre[i] = v.x;
+                        cim[i] = v.y;
+                    }
+#pragma unroll
+                    for (int kk = 0; kk < kLuOuter / 4; ++kk)
+                        if (kk < nks) {
+                            const double2 bv2 = Ush[(4 * kk + hi) * kLuChunk + ct * 16 + lo];
+                            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.x, cre, 0, 0, 0);
+                            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[kk], bv2.y, cre, 0, 0, 0);
+                            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.y, cim, 0, 0, 0);
+                            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[kk], bv2.x, cim, 0, 0, 0);
+                        }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        at(row0 + 4 * i + hi, col0 + lo) = make_double2(cre[i], cim[i]);
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -585,7 +712,13 @@ BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &
         hipLaunchKernelGGL(pad_blocks_kernel, dim3(bp->nbox), dim3(256), 0, s, bp->box_first, nmax,
                            bp->blk_off, bp->blocks);
         YH_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(lu_blocked_kernel, dim3(bp->nbox), dim3(64 * kLuWaves), 0, s, bp->box_first, nmax,
+        static bool lu_lds_opt_in = false;  // gfx950: up to 160 KB of LDS per workgroup
+        if (!lu_lds_opt_in) {
+            YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&lu_blocked_kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLuLds)));
+            lu_lds_opt_in = true;
+        }
+        hipLaunchKernelGGL(lu_blocked_kernel, dim3(bp->nbox), dim3(64 * kLuWaves), kLuLds, s, bp->box_first, nmax,
                            bp->blk_off, bp->piv_off, bp->blocks, bp->pivots, bp->dinv_off, bp->dinv);
         YH_CHECK(hipGetLastError());
         hipLaunchKernelGGL(pivots_to_permutation_kernel, dim3(bp->nbox), dim3(256), bp->max_m * sizeof(int), s,
