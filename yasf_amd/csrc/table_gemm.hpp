@@ -28,7 +28,7 @@ struct TableGemmOperands {
 // straight-line block the scheduler can interleave.  The result goes out
 // through emit(j, it, i, re, im): element i of the lane's accumulator
 // (row Mfma<T>::row(i, lane >> 4), column lane & 15) of C tile (j, it).
-template <typename T, int NIP, int NTL, int DEPTH, typename Emit>
+template <typename T, int NIP, int NTL, int DEPTH, bool GAUSS, typename Emit>
 __device__ __forceinline__ void table_gemm_wave(const TableGemmOperands<T> &op, int ip0, int ipstep, int k0,
                                                 int k1, Emit &emit) {
     typedef typename Mfma<T>::acc_t acc_t;
@@ -41,12 +41,13 @@ __device__ __forceinline__ void table_gemm_wave(const TableGemmOperands<T> &op, 
         const int t = uniform(op.otile[ip0 + ipstep * j]) - op.tile0;
         tn[j] = t < 0 ? 0 : (t > NTL ? NTL : t);
     }
-    // Complex128: the complex product with three real MFMAs (Gauss): P1 =
-    // Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 -
-    // P2 at the end -- a quarter fewer FP64 MFMAs (measured: 7.04 -> 6.07 ms
-    // at 2,000 spheres, lmax 6).  Complex64 keeps four (the third accumulator
-    // costs it more occupancy than the saved MFMAs gain).
-    constexpr bool kGaussComplex = sizeof(T) == sizeof(double);
+    // GAUSS: the complex product with three real MFMAs (Gauss): P1 = Ar Br,
+    // P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2 at
+    // the end -- a quarter fewer MFMAs for a third accumulator.  Complex128
+    // always (measured: 7.04 -> 6.07 ms at 2,000 spheres, lmax 6); complex64
+    // only in the widest jobs (8 column tiles: -4% at lmax 6, -6% at lmax 8),
+    // where the MFMAs saved outweigh the occupancy the accumulator costs.
+    constexpr bool kGaussComplex = GAUSS;
     // Complex64 skips the C tiles past each order's column prefix (the
     // padding of the job's order range: -12% to -19% time at lmax 4..8);
     // in complex128 the branches cost what the skipped MFMAs save.
@@ -104,17 +105,20 @@ __device__ __forceinline__ void table_gemm_wave(const TableGemmOperands<T> &op, 
             for (int j = 0; j < NIP; ++j)
 #pragma unroll
                 for (int it = 0; it < NTL; ++it)
-                    acc[j][it][0] = Mfma<T>::mac(a[j].x, b[it].x, acc[j][it][0]);
+                    if (!kTrimTiles || it < tn[j])
+                        acc[j][it][0] = Mfma<T>::mac(a[j].x, b[it].x, acc[j][it][0]);
 #pragma unroll
             for (int j = 0; j < NIP; ++j)
 #pragma unroll
                 for (int it = 0; it < NTL; ++it)
-                    acc[j][it][1] = Mfma<T>::mac(a[j].y, b[it].y, acc[j][it][1]);
+                    if (!kTrimTiles || it < tn[j])
+                        acc[j][it][1] = Mfma<T>::mac(a[j].y, b[it].y, acc[j][it][1]);
 #pragma unroll
             for (int j = 0; j < NIP; ++j)
 #pragma unroll
                 for (int it = 0; it < NTL; ++it)
-                    acc[j][it][2] = Mfma<T>::mac(as[j], bs[it], acc[j][it][2]);
+                    if (!kTrimTiles || it < tn[j])
+                        acc[j][it][2] = Mfma<T>::mac(as[j], bs[it], acc[j][it][2]);
         } else {
 #pragma unroll
             for (int j = 0; j < NIP; ++j)
@@ -198,7 +202,8 @@ __device__ __forceinline__ void table_gemm_ntl(const TableGemmOperands<T> &op, i
             // three operand stages unless the accumulators leave no room (a
             // complex128 wave with 2 x 4 C tiles holds 192 accumulator VGPRs)
             constexpr int kDepth = (sizeof(T) == sizeof(double) && NIP * NTL >= 8) ? 2 : 3;
-            table_gemm_wave<T, NIP, NTL, kDepth>(op, ip0, ipstep, k0, k1, emit);
+            constexpr bool kGauss = sizeof(T) == sizeof(double) || NTLMAX >= 8;
+            table_gemm_wave<T, NIP, NTL, kDepth, kGauss>(op, ip0, ipstep, k0, k1, emit);
             return;
         }
         table_gemm_ntl<T, NIP, NTL + 1, NTLMAX>(op, ip0, ipstep, k0, k1, ntile, emit);
