@@ -202,7 +202,9 @@ int yasfhip_set_slab_tiles(yasfhip_session *s, int tiles);
  * index in *grouping; ms[6] receives the mean times (-1 for shapes not
  * offered; all -1 when the session generates the pair factors on the fly, in
  * which case nothing changes).  The table (or first slab) is built first if
- * needed.  Results do not depend on the shape. */
+ * needed.  Until tuned (or set), a session uses shape 1 in FP64 and 2 in
+ * FP32.  Results agree between shapes to rounding (the orders a block cuts
+ * along K depend on the job's order count). */
 int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps, int *grouping,
                         double *ms);
 

@@ -130,7 +130,10 @@ struct yasfhip_session {
     void *table[2] = {nullptr, nullptr};
     int table_channel[2] = {-1, -1};
     bool table_refused[2] = {false, false};
-    int table_grouping[2] = {0, 0};  // table GEMM shape per precision (waves x column tiles)
+    // table GEMM shape per precision (waves x column tiles) until the tuner
+    // picks one: 8 waves with jobs of 4 tiles (FP64) and of 8 (FP32), the
+    // choices at 2,000 spheres, lmax 6
+    int table_grouping[2] = {1, 2};
     double near2 = kAllPairs;          // squared near-field radius of W (kAllPairs: every pair)
     // Slab of the pair table (a few target tiles), rebuilt tile range by tile
     // range inside every matvec when the full table does not fit.
