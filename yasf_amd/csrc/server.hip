@@ -153,6 +153,10 @@ __global__ __launch_bounds__(kVecThreads) void serve_prep_kernel(const BatchReq 
 
 // One (job, target tile t) block of a request's complex64 pair GEMM; tab
 // points at target tile t of the request's pair table (or of a slab of it).
+// Table GEMM shape of the server's FP64 products (residuals, C_sca): 8 waves
+// with jobs of up to 4 column tiles, 20% faster than jobs of 2 at lmax 4 and 6.
+constexpr int kServeShape64 = 1;
+
 // Orders per wave the serving GEMM instances hold: whole orders of blocks of
 // 4 and 8 waves at lmax 8, and of 2-wave blocks up to lmax 4.
 constexpr int kServeNip2 = 5;
@@ -1029,15 +1033,15 @@ struct yasfhip_server {
             if (gemm_done[i])
                 continue;
             if (o.table64) {
-                launch_pair_table_gemm(dp, q->N, o.tab64, q->x_d, false, 0, q->Q, st);
+                launch_pair_table_gemm(dp, q->N, o.tab64, q->x_d, false, kServeShape64, q->Q, st);
             } else if (o.slab) {
                 for (int t0 = 0; t0 < o.ntiles1; t0 += o.slab_tiles64) {
                     const int ntl = std::min(o.slab_tiles64, o.ntiles1 - t0);
                     launch_pair_table_build(dp, o.pos_d, o.N, o.k, false, o.slab, st, t0, ntl);
                     for (size_t j = i; j < todo.size(); ++j)
                         if (todo[j]->op == q->op)
-                            launch_pair_table_gemm(dp, q->N, o.slab, todo[j]->x_d, false, 0, todo[j]->Q, st, t0,
-                                                   ntl);
+                            launch_pair_table_gemm(dp, q->N, o.slab, todo[j]->x_d, false, kServeShape64, todo[j]->Q,
+                                                   st, t0, ntl);
                 }
                 for (size_t j = i; j < todo.size(); ++j)
                     if (todo[j]->op == q->op)
@@ -1113,8 +1117,8 @@ struct yasfhip_server {
                     launch_pair_table_build(dp, o.pos_d, o.N, o.k, false, o.slab, st, t0, ntl, nullptr, true);
                     for (size_t j = i; j < done.size(); ++j)
                         if (done[j]->op == q->op && done[j]->observables)
-                            launch_pair_table_gemm(dp, q->N, o.slab, done[j]->x_d, false, 0, done[j]->Q, st, t0,
-                                                   ntl);
+                            launch_pair_table_gemm(dp, q->N, o.slab, done[j]->x_d, false, kServeShape64, done[j]->Q,
+                                                   st, t0, ntl);
                 }
                 for (size_t j = i; j < done.size(); ++j)
                     if (done[j]->op == q->op)
