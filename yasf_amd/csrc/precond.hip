@@ -385,35 +385,39 @@ __global__ __launch_bounds__(64 * kLuWaves) void lu_blocked_kernel(const int *__
             const int nct = cw >> 4;
             for (int rb = wave; rb < nrb; rb += kLuWaves) {
                 const int row0 = pend + rb * 16;
-                double ar[kLuOuter / 4], ai[kLuOuter / 4];
+                // L21 fragments (ar, ai) and their sums for the three-MFMA
+                // complex product (Gauss): P1 = Ar Br, P2 = Ai Bi, P3 =
+                // (Ar + Ai)(Br + Bi) from zero, so the MFMAs need not wait for
+                // the C tile, which is loaded first and combined at the end:
+                // C -= (P1 - P2) + i (P3 - P1 - P2)
+                double ar[kLuOuter / 4], ai[kLuOuter / 4], as[kLuOuter / 4];
 #pragma unroll
                 for (int kk = 0; kk < kLuOuter / 4; ++kk)
                     if (kk < nks) {
                         const double2 v = at(row0 + lo, K0 + 4 * kk + hi);
                         ar[kk] = v.x;
                         ai[kk] = v.y;
+                        as[kk] = v.x + v.y;
                     }
                 for (int ct = 0; ct < nct; ++ct) {
                     const int col0 = c0 + ct * 16;
-                    v4d cre, cim;
+                    double2 cv[4];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const double2 v = at(row0 + 4 * i + hi, col0 + lo);
-                        cre[i] = v.x;
-                        cim[i] = v.y;
-                    }
+                    for (int i = 0; i < 4; ++i)
+                        cv[i] = at(row0 + 4 * i + hi, col0 + lo);
+                    v4d p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                     for (int kk = 0; kk < kLuOuter / 4; ++kk)
                         if (kk < nks) {
                             const double2 bv2 = Ush[(4 * kk + hi) * kLuChunk + ct * 16 + lo];
-                            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.x, cre, 0, 0, 0);
-                            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[kk], bv2.y, cre, 0, 0, 0);
-                            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.y, cim, 0, 0, 0);
-                            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[kk], bv2.x, cim, 0, 0, 0);
+                            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[kk], bv2.x, p1, 0, 0, 0);
+                            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[kk], bv2.y, p2, 0, 0, 0);
+                            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(as[kk], bv2.x + bv2.y, p3, 0, 0, 0);
                         }
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        at(row0 + 4 * i + hi, col0 + lo) = make_double2(cre[i], cim[i]);
+                        at(row0 + 4 * i + hi, col0 + lo) =
+                            make_double2(cv[i].x - (p1[i] - p2[i]), cv[i].y - (p3[i] - p1[i] - p2[i]));
                 }
             }
             __syncthreads();
