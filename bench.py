@@ -239,7 +239,8 @@ def run_solve(args):
         "config": {"workload": "dense FP64 solve: 2000 spheres, lmax 6, vf 0.2, lambda 550 nm, "
                                "GMRES(100) to 1e-6" + (", block-diag precond ~8/box" if use_pc else ""),
                    "particles": N, "lmax": lmax, "unknowns": N * 96,
-                   "preconditioner": use_pc, "restart": args.restart,
+                   "preconditioner": use_pc, "precond_factors": "complex64 storage, FP64 LU and substitutions",
+                   "restart": args.restart,
                    "box_cap": args.box_cap if use_pc else None,
                    "matvec": "resident FP64 pair-factor table + ab5 contraction",
                    "table_gemm_shape": {k: {"waves": _native.Session.TABLE_SHAPES[v][0],

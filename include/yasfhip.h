@@ -152,6 +152,16 @@ int yasfhip_far_field(yasfhip_session *s, int channel, const double *f, int num_
 int yasfhip_precond_build(yasfhip_session *s, int channel, const int *box_of_particle,
                           int num_boxes);
 
+/* Storage of the preconditioner's LU factors for the apply, taking effect at
+ * the next yasfhip_precond_build: YASFHIP_FP32 (default) rounds the factors,
+ * computed in FP64, to complex64 -- half the bytes each apply streams, the
+ * reference's own block precision (preconditioner.py builds complex64
+ * blocks) -- while the substitutions run in FP64 with FP64 diagonal
+ * inverses, so the apply stays one fixed linear operator; YASFHIP_FP64 keeps
+ * them in complex128 (M_box^-1 to FP64 rounding).  No counterpart in the
+ * reference. */
+int yasfhip_set_precond_factor_precision(yasfhip_session *s, int precision);
+
 /* z = M^-1 r with the prepared preconditioner (BlockDiagonalPreconditioner.apply,
  * yasfpy/preconditioner.py:177-219). */
 int yasfhip_precond_apply(yasfhip_session *s, const double *r, double *z);

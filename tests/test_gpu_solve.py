@@ -79,12 +79,21 @@ def test_block_preconditioner_apply_and_solve():
     pos, radii, n, lam, nmed, L = _cfg1()
     pc = BlockDiagonalPreconditioner(edge_size_for(pos, 8))
     sim = build_sim(pos, radii, n, lam, nmed, L, 1e-6, restart=100, pc=pc)
+    sim.session.set_precond_factor_precision(_native.FP64)  # M_box^-1 to FP64 rounding
     pc.prepare(sim)
     k = 2 * np.pi / lam
     t = sim.tmatrix_flat()[0]
     ref_apply = R.block_preconditioner(L, pos, k, t, pc.partitioning)
     r = np.random.default_rng(9).normal(size=t.size) + 1j
-    npt.assert_allclose(pc.apply(r), ref_apply(r), rtol=1e-10, atol=1e-10 * np.abs(r).max())
+    z_ref = ref_apply(r)
+    npt.assert_allclose(pc.apply(r), z_ref, rtol=1e-10, atol=1e-10 * np.abs(r).max())
+    # default: factors stored in complex64, substitutions in FP64 -- the apply is
+    # M_box^-1 to about the factors' rounding, and the same operator every time
+    sim.session.set_precond_factor_precision(_native.FP32)
+    pc.prepare(sim)
+    z32 = pc.apply(r)
+    assert np.linalg.norm(z32 - z_ref) / np.linalg.norm(z_ref) < 1e-5
+    assert np.array_equal(pc.apply(r), z32)
     sim.compute_scattered_field_coefficients()
     x_pc = sim.scattered_field_coefficients[:, :, 0].ravel().copy()
     b = sim.right_hand_side[:, :, 0].ravel()
@@ -101,6 +110,7 @@ def test_single_box_preconditioner_is_exact():
     pos, radii = cluster(11, 6, 0.1)
     sim = build_sim(pos, radii, np.full(6, 1.6 + 0.02j), 600.0, 1.0, 3, 1e-8, restart=50)
     s = sim.session
+    s.set_precond_factor_precision(_native.FP64)
     s.precond_build(np.zeros(6, dtype=np.int32))
     b = sim.right_hand_side[:, :, 0].ravel()
     x, info, iters, res, _ = s.gmres(b, b, rtol=1e-8, restart=50, maxiter=10, use_precond=True)

@@ -65,6 +65,7 @@ def _load():
             "yasfhip_far_field": ([P, i, P, i, P, P, P, P], i),
             "yasfhip_precond_build": ([P, i, P, i], i),
             "yasfhip_precond_apply": ([P, P, P], i),
+            "yasfhip_set_precond_factor_precision": ([P, i], i),
             "yasfhip_time_matvec": ([P, i, i, i, dp, dp], i),
             "yasfhip_server_create": ([ctypes.c_size_t, i, d, i, ctypes.POINTER(P)], i),
             "yasfhip_server_destroy": ([P], i),
@@ -108,6 +109,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_server_autotune", "yasfhip_server_set_tile_shape", "yasfhip_server_set_table_policy",
         "yasfhip_set_slab_tiles", "yasfhip_set_near_field_radius", "yasfhip_mie_coefficients",
         "yasfhip_planewave_coefficients", "yasfhip_server_submit_spheres", "yasfhip_set_table_shape",
+        "yasfhip_set_precond_factor_precision",
     ]
 
 
@@ -294,6 +296,12 @@ class Session:
             raise ValueError("box_of_particle must have one entry per particle")
         nbox = int(box.max()) + 1 if box.size else 0
         _check(self._lib.yasfhip_precond_build(self._h, int(channel), _ptr(box), nbox))
+
+    def set_precond_factor_precision(self, precision: int):
+        """FP32 (default): the preconditioner's LU factors are stored in
+        complex64 for the apply (half the bytes; FP64 arithmetic); FP64 keeps
+        them in complex128.  Takes effect at the next precond_build."""
+        _check(self._lib.yasfhip_set_precond_factor_precision(self._h, int(precision)))
 
     def precond_apply(self, r) -> np.ndarray:
         r = self._vec(r)

@@ -44,7 +44,7 @@ void launch_far_field(const double4 *pos, const double2 *f, int N, int lmax, dou
 struct BlockPrecond;
 BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &ab5,
                              const double *pos_host, int N, const int *box_of_particle, int num_boxes,
-                             double k, const double2 *t_dev, hipStream_t s);
+                             double k, const double2 *t_dev, bool fp32_factors, hipStream_t s);
 void precond_destroy(BlockPrecond *p);
 void precond_apply(BlockPrecond *p, const double2 *r, double2 *z, hipStream_t s);
 
@@ -125,6 +125,7 @@ struct yasfhip_session {
     double *hdev = nullptr, *hhost = nullptr;  // per-step MGS results, 4 doubles per slot
     BlockPrecond *precond = nullptr;
     int precond_channel = -1;
+    bool precond_fp32 = true;  // LU factors stored in complex64 for the apply (FP64 arithmetic)
     // Resident pair-factor tables, one per precision (0 = FP64, 1 = FP32).
     int matvec_mode = YASFHIP_MATVEC_AUTO;
     void *table[2] = {nullptr, nullptr};
@@ -811,8 +812,16 @@ int yasfhip_precond_build(yasfhip_session *s, int channel, const int *box_of_par
         }
         s->precond = precond_create(s->plan, translation_table_cached(s->L), s->pos_host.data(), s->N,
                                     box_of_particle, num_boxes, s->k[channel],
-                                    s->t + static_cast<size_t>(channel) * s->n, s->stream);
+                                    s->t + static_cast<size_t>(channel) * s->n, s->precond_fp32, s->stream);
         s->precond_channel = channel;
+    });
+}
+
+int yasfhip_set_precond_factor_precision(yasfhip_session *s, int precision) {
+    return guarded([&] {
+        YH_REQUIRE(s, "NULL argument");
+        YH_REQUIRE(precision == YASFHIP_FP64 || precision == YASFHIP_FP32, "unknown precision");
+        s->precond_fp32 = precision == YASFHIP_FP32;
     });
 }
 

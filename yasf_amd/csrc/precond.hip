@@ -26,7 +26,8 @@ struct BlockPrecond {
     long long *blk_off = nullptr;  // [nbox] element offset of each block
     int *piv_off = nullptr;      // [nbox] offset of each block's pivots
     int *dinv_off = nullptr;     // [nbox] first 16-row diagonal block of each box
-    double2 *blocks = nullptr;
+    double2 *blocks = nullptr;   // FP64 blocks and their factors (freed once blocks32 holds the factors)
+    float2 *blocks32 = nullptr;  // the factors rounded to complex64 for the apply, when chosen
     int *pivots = nullptr;       // after the build: the row permutation of each box (see below)
     double2 *dinv = nullptr;     // per 16-row diagonal block: L11^-1 then U11^-1 (2 x 16 x 16)
     long long total_elems = 0;
@@ -468,11 +469,17 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // loaded when used).
 constexpr int kApplyThreads = 512;  // one block per box (at least 320: the diagonal staging)
 constexpr int kApplyRows = 2;       // rows per thread loaded a step ahead
+__device__ __forceinline__ double2 to_c128(double2 v) { return v; }
+__device__ __forceinline__ double2 to_c128(float2 v) { return make_double2(v.x, v.y); }
+// TF: the type the off-diagonal factors are stored in (complex128, or
+// complex64 for half the bytes); the substitution arithmetic is FP64 either
+// way, with the FP64 diagonal inverses.
+template <typename TF>
 __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__restrict__ box_first,
                                                        const int *__restrict__ particles, int nmax,
                                                        const long long *__restrict__ blk_off,
                                                        const int *__restrict__ piv_off,
-                                                       const double2 *__restrict__ blocks,
+                                                       const TF *__restrict__ blocks,
                                                        const int *__restrict__ pivots,
                                                        const int *__restrict__ dinv_off,
                                                        const double2 *__restrict__ dinv,
@@ -485,7 +492,7 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
     const int first = box_first[b];
     const int m = (box_first[b + 1] - first) * nmax;
     const int mp = padded16(m);
-    const double2 *A = blocks + blk_off[b];
+    const TF *A = blocks + blk_off[b];
     const int *piv = pivots + piv_off[b];
     const double2 *D = dinv + static_cast<long>(dinv_off[b]) * 512;
     const int tid = threadIdx.x;
@@ -498,13 +505,13 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
     const int nb = mp >> 4;
     // pan[k]: the 16 entries of row lo + tid + k nthr in the step's column
     // block, loaded one step ahead
-    double2 pan[kApplyRows][16];
+    TF pan[kApplyRows][16];  // converted at use, so the loads stay in flight
     auto load_rows = [&](int lo, int hi, int base) {
 #pragma unroll
         for (int k = 0; k < kApplyRows; ++k) {
             const int i = lo + tid + k * nthr;
             if (i < hi) {
-                const double2 *row = A + static_cast<long>(i) * mp + base;
+                const TF *row = A + static_cast<long>(i) * mp + base;
 #pragma unroll
                 for (int q = 0; q < 16; ++q)
                     pan[k][q] = row[q];
@@ -558,16 +565,16 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
                 double2 acc = y[i];
 #pragma unroll
                 for (int q = 0; q < 16; ++q)
-                    acc = csub(acc, cmul(pan[k][q], yv[q]));
+                    acc = csub(acc, cmul(to_c128(pan[k][q]), yv[q]));
                 y[i] = acc;
             }
         }
         for (int i = lo + tid + kApplyRows * nthr; i < hi; i += nthr) {
             double2 acc = y[i];
-            const double2 *row = A + static_cast<long>(i) * mp + base;
+            const TF *row = A + static_cast<long>(i) * mp + base;
 #pragma unroll
             for (int q = 0; q < 16; ++q)
-                acc = csub(acc, cmul(row[q], yv[q]));
+                acc = csub(acc, cmul(to_c128(row[q]), yv[q]));
             y[i] = acc;
         }
         asm volatile("" ::: "memory");  // the next step's row loads stay below this step's rows
@@ -603,6 +610,15 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
         z[static_cast<long>(particles[first + i / nmax]) * nmax + i % nmax] = y[i];
 }
 
+// blocks32 = blocks rounded to complex64 (the factors the apply reads).
+__global__ void round_blocks_kernel(const double2 *__restrict__ src, float2 *__restrict__ dst, long long count) {
+    for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < count;
+         i += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const double2 v = src[i];
+        dst[i] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+    }
+}
+
 template <typename T>
 T *dalloc(size_t count) {
     void *p = nullptr;
@@ -624,7 +640,7 @@ void precond_destroy(BlockPrecond *p);
 
 BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &ab5,
                              const double *pos_host, int N, const int *box_of_particle, int num_boxes,
-                             double k, const double2 *t_dev, hipStream_t s) {
+                             double k, const double2 *t_dev, bool fp32_factors, hipStream_t s) {
     auto *bp = new BlockPrecond();
     try {
         bp->nmax = plan.nmax;
@@ -728,7 +744,16 @@ BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &
         hipLaunchKernelGGL(pivots_to_permutation_kernel, dim3(bp->nbox), dim3(256), bp->max_m * sizeof(int), s,
                            bp->box_first, nmax, bp->piv_off, bp->pivots);
         YH_CHECK(hipGetLastError());
+        if (fp32_factors) {  // the factors rounded to complex64; the FP64 copy is not needed after this
+            bp->blocks32 = dalloc<float2>(static_cast<size_t>(elems));
+            hipLaunchKernelGGL(round_blocks_kernel, dim3(4096), dim3(256), 0, s, bp->blocks, bp->blocks32, elems);
+            YH_CHECK(hipGetLastError());
+        }
         YH_CHECK(hipStreamSynchronize(s));
+        if (fp32_factors) {
+            YH_CHECK(hipFree(bp->blocks));
+            bp->blocks = nullptr;
+        }
         for (void *p : {static_cast<void *>(dpos), static_cast<void *>(drb), static_cast<void *>(derow),
                         static_cast<void *>(deg), static_cast<void *>(ddmoff), static_cast<void *>(dev),
                         static_cast<void *>(la), static_cast<void *>(lb), static_cast<void *>(ld)})
@@ -745,17 +770,22 @@ void precond_destroy(BlockPrecond *p) {
         return;
     for (void *b : {static_cast<void *>(p->box_first), static_cast<void *>(p->particles),
                     static_cast<void *>(p->blk_off), static_cast<void *>(p->piv_off),
-                    static_cast<void *>(p->blocks), static_cast<void *>(p->pivots),
-                    static_cast<void *>(p->dinv_off), static_cast<void *>(p->dinv)})
+                    static_cast<void *>(p->blocks), static_cast<void *>(p->blocks32),
+                    static_cast<void *>(p->pivots), static_cast<void *>(p->dinv_off), static_cast<void *>(p->dinv)})
         if (b)
             (void)hipFree(b);
     delete p;
 }
 
 void precond_apply(BlockPrecond *p, const double2 *r, double2 *z, hipStream_t s) {
-    hipLaunchKernelGGL(lu_apply_kernel, dim3(p->nbox), dim3(kApplyThreads), p->max_m * sizeof(double2), s,
-                       p->box_first, p->particles, p->nmax, p->blk_off, p->piv_off, p->blocks, p->pivots,
-                       p->dinv_off, p->dinv, r, z);
+    if (p->blocks32)
+        hipLaunchKernelGGL(lu_apply_kernel<float2>, dim3(p->nbox), dim3(kApplyThreads), p->max_m * sizeof(double2),
+                           s, p->box_first, p->particles, p->nmax, p->blk_off, p->piv_off, p->blocks32, p->pivots,
+                           p->dinv_off, p->dinv, r, z);
+    else
+        hipLaunchKernelGGL(lu_apply_kernel<double2>, dim3(p->nbox), dim3(kApplyThreads),
+                           p->max_m * sizeof(double2), s, p->box_first, p->particles, p->nmax, p->blk_off,
+                           p->piv_off, p->blocks, p->pivots, p->dinv_off, p->dinv, r, z);
     YH_CHECK(hipGetLastError());
 }
 
