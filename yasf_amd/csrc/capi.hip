@@ -118,8 +118,7 @@ struct yasfhip_session {
             *ycoef = nullptr;
     double2 *rb = nullptr, *rx = nullptr, *rr = nullptr;  // iterative refinement: b, x, residual
     double *scal = nullptr, *partials = nullptr;
-    static constexpr int kMgsPartials = 512 * 4;  // one launch's partials (krylov.hip grid x 4)
-    double *mgs_part = nullptr;                   // two of them, used in turn
+    double *mgs_part = nullptr;  // two launches' partials (kRedPartials each), used in turn
     unsigned *counter = nullptr;
     double *scal_host = nullptr;
     double *hdev = nullptr, *hhost = nullptr;  // per-step MGS results, 4 doubles per slot
@@ -280,7 +279,7 @@ struct yasfhip_session {
             YH_CHECK(hipHostFree(hhost));
         V = dmalloc<double2>(static_cast<size_t>(restart + 1) * n);
         if (!mgs_part)
-            mgs_part = dmalloc<double>(2 * kMgsPartials);
+            mgs_part = dmalloc<double>(2 * kRedPartials);
         ycoef = dmalloc<double2>(restart + 1);
         hdev = dmalloc<double>(4 * static_cast<size_t>(restart + 2));
         YH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&hhost), 4 * (restart + 2) * sizeof(double),
@@ -385,7 +384,7 @@ struct yasfhip_session {
                 // launch kq subtracts h[kq-1] v[kq-1] and forms the partials
                 // of <v[kq], w> (buffer kq % 2); the next launch sums them and
                 // stores (Re, Im, ||w||^2) of launch kq to slot kq of hdev.
-                auto part = [&](int launch) { return mgs_part + (launch & 1) * kMgsPartials; };
+                auto part = [&](int launch) { return mgs_part + (launch & 1) * kRedPartials; };
                 for (int kq = 0; kq <= col; ++kq) {
                     const double2 *vk = V + static_cast<size_t>(kq) * n;
                     const double2 *vp = kq > 0 ? V + static_cast<size_t>(kq - 1) * n : nullptr;
@@ -573,7 +572,7 @@ int yasfhip_create(const double *positions, int particle_number, int lmax, const
         s->rvec = dmalloc<double2>(s->n);
         s->avec = dmalloc<double2>(s->n);
         s->scal = dmalloc<double>(8);
-        s->partials = dmalloc<double>(512 * 4);
+        s->partials = dmalloc<double>(kRedPartials);
         s->counter = dmalloc<unsigned>(1);
         YH_CHECK(hipMemsetAsync(s->counter, 0, sizeof(unsigned), st));
         YH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&s->scal_host), 8 * sizeof(double),

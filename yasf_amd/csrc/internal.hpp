@@ -76,6 +76,11 @@ struct PairJob {
 // tiles are split evenly into jobs of at most kTableGrouping[g] tiles.  Wider
 // groups stream each table slice from HBM fewer times but hold more
 // accumulators per wave.
+// Grid of the device GMRES's reductions (krylov.hip): at most this many
+// blocks, each leaving one row of 4 doubles of partial sums.
+constexpr int kRedBlocks = 512;
+constexpr int kRedPartials = kRedBlocks * 4;  // doubles in one launch's partials
+
 constexpr int kTableGroupings = 3;
 constexpr int kTableGrouping[kTableGroupings] = {2, 4, 8};
 constexpr int kTableMaxColTiles = 8;

@@ -6,7 +6,6 @@
 
 namespace yasfhip {
 
-constexpr int kRedBlocks = 512;
 constexpr int kRedThreads = 256;
 
 struct Reducer {

@@ -1685,7 +1685,7 @@ int yasfhip_server_create(size_t memory_bytes, int restart, double inner_rtol, i
             memory_bytes = free_b / 10 * 7;
         }
         s->arena.init(memory_bytes);
-        YH_CHECK(hipMalloc(&s->partials, 512 * 4 * sizeof(double)));
+        YH_CHECK(hipMalloc(&s->partials, kRedPartials * sizeof(double)));
         YH_CHECK(hipMalloc(&s->counter, sizeof(unsigned)));
         YH_CHECK(hipMemsetAsync(s->counter, 0, sizeof(unsigned), s->stream));
         s->ensure_scal(64);
