@@ -130,7 +130,10 @@ int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b
  *   c_sca = |sum_{j1,j2} conj(f_j1) W_J[j1,j2] f_j2| * pi / k^2,
  * W_J being the coupling operator with spherical Bessel j_p in place of h_p and
  * the coincident pairs s1 = s2 included (the CPU sum of
- * yasfpy/functions/cpu_numba.py:410-457), evaluated matrix-free in FP64. */
+ * yasfpy/functions/cpu_numba.py:410-457), evaluated in FP64: over slabs of a
+ * regular pair table in the session's slab buffer where its matvecs stream
+ * slabs (SLAB mode, or AUTO when the table did not fit), otherwise with the
+ * factors generated on the fly. */
 int yasfhip_cross_sections(yasfhip_session *s, int channel, const double *a_inc, const double *f,
                            double *c_ext, double *c_sca);
 

@@ -41,6 +41,28 @@ def test_cross_sections_match_oracle(lmax):
     assert c_sca == pytest.approx(r_sca, rel=1e-10)
 
 
+@pytest.mark.parametrize("mode,slab_tiles", [(_native.MATVEC_ON_THE_FLY, 0), (_native.MATVEC_TABLE, 0),
+                                              (_native.MATVEC_SLAB, 1), (_native.MATVEC_SLAB, 2),
+                                              (_native.MATVEC_AUTO, 0)],
+                         ids=["on_the_fly", "table", "slab1", "slab2", "auto"])
+def test_cross_sections_every_matvec_mode(mode, slab_tiles):
+    # C_sca's regular pair GEMM runs over slabs of the regular table (SLAB)
+    # or on the fly (TABLE, ON_THE_FLY, and AUTO with its table resident);
+    # 37 spheres = 3 target tiles, the last one ragged, so one-tile slabs
+    # make three builds per product
+    lmax = 6
+    pos, k, t, a, f = _coefficients(77, 37, lmax, 560.0, 1.6 + 0.05j)
+    s = _native.Session(pos, lmax, [k])
+    s.set_matvec_mode(mode)
+    if slab_tiles:
+        s.set_slab_tiles(slab_tiles)
+    got = s.cross_sections(a, f)
+    npt.assert_allclose(got, R.cross_sections(lmax, pos, k, a, f), rtol=1e-10)
+    # a matvec in between (the FP64 slab then holds irregular factors) and again
+    s.coupling_matvec(f)
+    npt.assert_allclose(s.cross_sections(a, f), got, rtol=1e-13)
+
+
 def test_cross_sections_single_sphere_and_far_pairs():
     # one sphere: only the coincident (self) term of the J-translation sum
     pos = np.zeros((1, 3))

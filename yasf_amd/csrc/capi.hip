@@ -250,6 +250,24 @@ struct yasfhip_session {
         }
     }
 
+    // Q = the regular (J) pair GEMM of complex128 x for C_sca: over slabs of
+    // the FP64 regular table (build, multiply, next slab) where the session
+    // streams slabs (SLAB mode, or AUTO once its matvecs allocated the slab
+    // buffer), else with the factors generated on the fly.
+    void pair_gemm_regular(int channel, const void *xv) {
+        const int tiles = (slab || matvec_mode == YASFHIP_MATVEC_SLAB) ? slab_tiles(false) : 0;
+        if (tiles == 0) {
+            launch_pair_regular(dp, pos, N, k[channel], xv, Q, stream);
+            return;
+        }
+        const int ntiles1 = (N + 15) / 16;
+        for (int t0 = 0; t0 < ntiles1; t0 += tiles) {
+            const int ntl = std::min(tiles, ntiles1 - t0);
+            launch_pair_table_build(dp, pos, N, k[channel], false, slab, stream, t0, ntl, nullptr, true);
+            launch_pair_table_gemm(dp, N, slab, xv, false, table_grouping[0], Q, stream, t0, ntl);
+        }
+    }
+
     // y = W x (mode 0) or x - t W x (mode 1) for device vectors of complex128.
     // fp32: the pair GEMM runs in complex64 on a complex64 copy of x; the ab5
     // contraction and the epilogue stay FP64.
@@ -738,7 +756,7 @@ int yasfhip_cross_sections(yasfhip_session *s, int channel, const double *a_inc,
         YH_CHECK(hipMemcpyAsync(s->avec, a_inc, n * sizeof(double2), hipMemcpyHostToDevice, st));
         // y = W_J f (regular translation, coincident pairs included), then
         // <f, y> and <a, f> (conj on the left operand)
-        launch_pair_regular(s->dp, s->pos, s->N, s->k[channel], s->xin, s->Q, st);
+        s->pair_gemm_regular(channel, s->xin);
         launch_contract(s->dp, s->N, s->Q, s->xin, s->t, 0, s->yout, false, false, st);
         launch_mgs_step(s->yout, nullptr, nullptr, s->xin, n, s->reducer(), s->scal, st);
         launch_mgs_step(s->xin, nullptr, nullptr, s->avec, n, s->reducer(), s->scal + 4, st);
