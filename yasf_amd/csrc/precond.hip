@@ -13,6 +13,7 @@
 // the identity in the padding).
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 
 #include "device_common.hpp"
 
@@ -27,7 +28,7 @@ struct BlockPrecond {
     int *piv_off = nullptr;      // [nbox] offset of each block's pivots
     int *dinv_off = nullptr;     // [nbox] first 16-row diagonal block of each box
     double2 *blocks = nullptr;   // FP64 blocks and their factors (freed once blocks32 holds the factors)
-    float2 *blocks32 = nullptr;  // the factors rounded to complex64 for the apply, when chosen
+    float2 *blocks32 = nullptr;  // the factors rounded to complex64, column-major per box, when chosen
     int *pivots = nullptr;       // after the build: the row permutation of each box (see below)
     double2 *dinv = nullptr;     // per 16-row diagonal block: L11^-1 then U11^-1 (2 x 16 x 16)
     long long total_elems = 0;
@@ -471,9 +472,11 @@ constexpr int kApplyThreads = 512;  // one block per box (at least 320: the diag
 constexpr int kApplyRows = 2;       // rows per thread loaded a step ahead
 __device__ __forceinline__ double2 to_c128(double2 v) { return v; }
 __device__ __forceinline__ double2 to_c128(float2 v) { return make_double2(v.x, v.y); }
-// TF: the type the off-diagonal factors are stored in (complex128, or
-// complex64 for half the bytes); the substitution arithmetic is FP64 either
-// way, with the FP64 diagonal inverses.
+// TF: the type the off-diagonal factors are stored in: complex128 row-major
+// (the LU's own storage), or complex64 column-major for half the bytes and
+// coalesced reads (a wave's 64 rows of one column are 512 contiguous bytes,
+// where row-major storage has every lane on its own cache line).  The
+// substitution arithmetic is FP64 either way, with the FP64 diagonal inverses.
 template <typename TF>
 __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__restrict__ box_first,
                                                        const int *__restrict__ particles, int nmax,
@@ -503,6 +506,12 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
                        : make_double2(0.0, 0.0);
     }
     const int nb = mp >> 4;
+    // entry (i, base + q) of the factors: stride ld between the 16 columns
+    constexpr bool kColMajor = std::is_same<TF, float2>::value;
+    const long ld = kColMajor ? mp : 1;
+    auto entries = [&](int i, int base) {
+        return kColMajor ? A + static_cast<long>(base) * mp + i : A + static_cast<long>(i) * mp + base;
+    };
     // pan[k]: the 16 entries of row lo + tid + k nthr in the step's column
     // block, loaded one step ahead
     TF pan[kApplyRows][16];  // converted at use, so the loads stay in flight
@@ -511,10 +520,10 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
         for (int k = 0; k < kApplyRows; ++k) {
             const int i = lo + tid + k * nthr;
             if (i < hi) {
-                const TF *row = A + static_cast<long>(i) * mp + base;
+                const TF *row = entries(i, base);
 #pragma unroll
                 for (int q = 0; q < 16; ++q)
-                    pan[k][q] = row[q];
+                    pan[k][q] = row[q * ld];
             }
         }
     };
@@ -571,10 +580,10 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
         }
         for (int i = lo + tid + kApplyRows * nthr; i < hi; i += nthr) {
             double2 acc = y[i];
-            const TF *row = A + static_cast<long>(i) * mp + base;
+            const TF *row = entries(i, base);
 #pragma unroll
             for (int q = 0; q < 16; ++q)
-                acc = csub(acc, cmul(to_c128(row[q]), yv[q]));
+                acc = csub(acc, cmul(to_c128(row[q * ld]), yv[q]));
             y[i] = acc;
         }
         asm volatile("" ::: "memory");  // the next step's row loads stay below this step's rows
@@ -610,12 +619,28 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
         z[static_cast<long>(particles[first + i / nmax]) * nmax + i % nmax] = y[i];
 }
 
-// blocks32 = blocks rounded to complex64 (the factors the apply reads).
-__global__ void round_blocks_kernel(const double2 *__restrict__ src, float2 *__restrict__ dst, long long count) {
-    for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < count;
-         i += static_cast<long long>(gridDim.x) * blockDim.x) {
-        const double2 v = src[i];
-        dst[i] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+// blocks32 = blocks rounded to complex64 and transposed per box (the
+// column-major factors the apply reads).  Block (box, y) moves the 16 x 16
+// tiles y, y + gridDim.y, ... of its box through LDS, reading rows and
+// writing columns contiguously; mp is a multiple of 16.
+__global__ __launch_bounds__(256) void round_blocks_kernel(const int *__restrict__ box_first, int nmax,
+                                                           const long long *__restrict__ blk_off,
+                                                           const double2 *__restrict__ src,
+                                                           float2 *__restrict__ dst) {
+    __shared__ float2 tile[16][17];
+    const int b = blockIdx.x;
+    const int mp = padded16((box_first[b + 1] - box_first[b]) * nmax);
+    const int nt = mp >> 4;
+    const double2 *S = src + blk_off[b];
+    float2 *D = dst + blk_off[b];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    for (int t = blockIdx.y; t < nt * nt; t += gridDim.y) {
+        const int r0 = (t / nt) * 16, c0 = (t % nt) * 16;
+        const double2 v = S[static_cast<long>(r0 + ty) * mp + c0 + tx];
+        tile[ty][tx] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+        __syncthreads();
+        D[static_cast<long>(c0 + ty) * mp + r0 + tx] = tile[tx][ty];
+        __syncthreads();
     }
 }
 
@@ -744,9 +769,10 @@ BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &
         hipLaunchKernelGGL(pivots_to_permutation_kernel, dim3(bp->nbox), dim3(256), bp->max_m * sizeof(int), s,
                            bp->box_first, nmax, bp->piv_off, bp->pivots);
         YH_CHECK(hipGetLastError());
-        if (fp32_factors) {  // the factors rounded to complex64; the FP64 copy is not needed after this
+        if (fp32_factors) {  // the factors rounded to complex64 (column-major); the FP64 copy is not needed after this
             bp->blocks32 = dalloc<float2>(static_cast<size_t>(elems));
-            hipLaunchKernelGGL(round_blocks_kernel, dim3(4096), dim3(256), 0, s, bp->blocks, bp->blocks32, elems);
+            hipLaunchKernelGGL(round_blocks_kernel, dim3(bp->nbox, 16), dim3(256), 0, s, bp->box_first, nmax,
+                               bp->blk_off, bp->blocks, bp->blocks32);
             YH_CHECK(hipGetLastError());
         }
         YH_CHECK(hipStreamSynchronize(s));
