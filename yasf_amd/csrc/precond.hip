@@ -468,8 +468,8 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // current update is done, so the loads run under the two barriers and the
 // next diagonal product (rows past kApplyRows x blockDim.x of a step are
 // loaded when used).
-constexpr int kApplyThreads = 512;  // one block per box (at least 320: the diagonal staging)
-constexpr int kApplyRows = 2;       // rows per thread loaded a step ahead
+constexpr int kApplyThreads = 768;  // one block per box (at least 320: the diagonal staging)
+constexpr int kApplyRows = 1;       // rows per thread loaded a step ahead
 __device__ __forceinline__ double2 to_c128(double2 v) { return v; }
 __device__ __forceinline__ double2 to_c128(float2 v) { return make_double2(v.x, v.y); }
 // TF: the type the off-diagonal factors are stored in: complex128 row-major
