@@ -2,8 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
 ``bench.py``'s cpu_baseline leg may import, call or load anything under
-``oracle/``, and only as the checker (or the timed CPU baseline), never as the
-thing measured or shipped.  The product package ``yasf_amd`` never imports it.
+``oracle/``, and only as the reference that results are compared against (or the
+timed CPU baseline), never as the thing measured or shipped.  The product package ``yasf_amd`` never imports it.
 
 ``restate.py`` holds the numpy restatement, ``matvec.c`` the O(N^2) loop nests
 (built by ``Makefile`` into ``liboracle.so``).

@@ -1,8 +1,9 @@
 /*
  * Oracle (test infrastructure only): plain-C restatement of the reference's
  * CPU coupling kernels.  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load this library, and only as the checker / the timed
- * CPU baseline -- never as part of the product path.
+ * cpu_baseline leg may load this library, and only as the reference that
+ * results are compared against, or as the timed CPU baseline -- never as part
+ * of the product path.
  *
  * The loop nests follow the reference line by line so that this file can be
  * checked against it by reading:

@@ -2,8 +2,8 @@
 multiple-sphere T-matrix solve path.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
-leg may import this module, and only as the checker or as the timed CPU
-baseline.  The product package (``yasf_amd``) never imports it.
+leg may import this module, and only as the reference that results are
+compared against or as the timed CPU baseline.  The product package (``yasf_amd``) never imports it.
 
 Every function cites the reference file:line it restates.  The pairwise
 special-function tables use the same library routines the reference calls
