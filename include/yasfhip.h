@@ -103,6 +103,15 @@ int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double
                         double rtol, int restart, int maxiter, int use_precond, int *info,
                         int *iterations, double *residual, double *history, int history_cap);
 
+/* yasfhip_gmres_solve with b and x_inout in device memory of the session's
+ * device (n complex128 values each): the same solve (solver.py:108-121) for
+ * callers whose vectors are already in HBM, with no host transfers.  Host
+ * pointers are rejected (YASFHIP_INVALID_ARGUMENT); history stays a host
+ * array.  Returns when the solve is done, as yasfhip_gmres_solve does. */
+int yasfhip_gmres_solve_device(yasfhip_session *s, int channel, const double *b, double *x_inout,
+                               double rtol, int restart, int maxiter, int use_precond, int *info,
+                               int *iterations, double *residual, double *history, int history_cap);
+
 /* Mixed-precision solve of (I - T W) x = b: FP64 iterative refinement around
  * GMRES whose operator runs the pair GEMM in complex64.  Each refinement step
  * forms the FP64 residual r = b - A x (FP64 matvec), solves A d = r with

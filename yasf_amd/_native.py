@@ -60,6 +60,7 @@ def _load():
             "yasfhip_coupling_matvec": ([P, i, P, P, i], i),
             "yasfhip_master_matvec": ([P, i, P, P, i], i),
             "yasfhip_gmres_solve": ([P, i, P, P, d, i, i, i, ip, ip, dp, P, i], i),
+            "yasfhip_gmres_solve_device": ([P, i, P, P, d, i, i, i, ip, ip, dp, P, i], i),
             "yasfhip_gmres_solve_refined": ([P, i, P, P, d, d, i, i, i, i, ip, ip, ip, dp, P, i], i),
             "yasfhip_cross_sections": ([P, i, P, P, dp, dp], i),
             "yasfhip_far_field": ([P, i, P, i, P, P, P, P], i),
@@ -102,7 +103,7 @@ def exported_symbols() -> list[str]:
     return [
         "yasfhip_last_error", "yasfhip_translation_table", "yasfhip_plan_stats", "yasfhip_create",
         "yasfhip_destroy", "yasfhip_set_tmatrix", "yasfhip_coupling_matvec", "yasfhip_master_matvec",
-        "yasfhip_gmres_solve", "yasfhip_gmres_solve_refined", "yasfhip_cross_sections",
+        "yasfhip_gmres_solve", "yasfhip_gmres_solve_device", "yasfhip_gmres_solve_refined", "yasfhip_cross_sections",
         "yasfhip_far_field", "yasfhip_precond_build", "yasfhip_precond_apply",
         "yasfhip_time_matvec", "yasfhip_set_matvec_mode", "yasfhip_tune_matvec", "yasfhip_server_create",
         "yasfhip_server_destroy", "yasfhip_server_submit", "yasfhip_server_step", "yasfhip_server_poll",
@@ -246,6 +247,29 @@ class Session:
             int(bool(use_precond)), ctypes.byref(info), ctypes.byref(iters), ctypes.byref(res),
             _ptr(hist), hist.size))
         return x, info.value, iters.value, res.value, hist[: min(iters.value, hist.size)].copy()
+
+    def gmres_device(self, b, x, channel: int = 0, rtol: float = 1e-6, restart: int = 20,
+                     maxiter: int = 1000, use_precond: bool = False, history_cap: int = 100000):
+        """gmres on vectors already in device memory: b and x are contiguous
+        complex128 torch tensors of n values on the session's device; x holds
+        x0 on entry and the solution on exit (torch's current stream is
+        synchronised first; the solution is complete on return).  Returns
+        (info, inner_iterations, relative_residual, pr_norm history)."""
+        import torch  # device vectors only; the rest of the module needs no torch
+
+        for name, v in (("b", b), ("x", x)):
+            if v.dtype != torch.complex128 or not v.is_cuda or not v.is_contiguous() or v.numel() != self.n:
+                raise ValueError(f"{name} must be a contiguous complex128 device tensor of {self.n} values")
+        torch.cuda.current_stream(b.device).synchronize()  # torch's writes to b and x are done
+        info = ctypes.c_int()
+        iters = ctypes.c_int()
+        res = ctypes.c_double()
+        hist = np.zeros(max(1, int(history_cap)), dtype=np.float64)
+        _check(self._lib.yasfhip_gmres_solve_device(
+            self._h, int(channel), ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()), float(rtol),
+            int(restart), int(maxiter), int(bool(use_precond)), ctypes.byref(info), ctypes.byref(iters),
+            ctypes.byref(res), _ptr(hist), hist.size))
+        return info.value, iters.value, res.value, hist[: min(iters.value, hist.size)].copy()
 
     def gmres_refined(self, b, x0=None, channel: int = 0, rtol: float = 1e-6, inner_rtol: float = 1e-5,
                       restart: int = 50, inner_maxiter: int = 1000, max_refinements: int = 10,

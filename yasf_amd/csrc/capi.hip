@@ -637,9 +637,13 @@ int yasfhip_master_matvec(yasfhip_session *s, int channel, const double *x, doub
     return guarded([&] { host_matvec(s, channel, x, y, precision, 1); });
 }
 
-int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double *x_inout,
-                        double rtol, int restart, int maxiter, int use_precond, int *info,
-                        int *iterations, double *residual, double *history, int history_cap) {
+namespace {
+
+// device: b and x_inout are device pointers on the session's device, copied
+// within the device instead of over PCIe.
+int gmres_solve(yasfhip_session *s, int channel, const double *b, double *x_inout, double rtol, int restart,
+                int maxiter, int use_precond, int *info, int *iterations, double *residual, double *history,
+                int history_cap, bool device) {
     return guarded([&] {
         YH_REQUIRE(s && b && x_inout, "NULL argument");
         YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
@@ -649,17 +653,28 @@ int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double
         if (use_precond)
             YH_REQUIRE(s->precond && s->precond_channel == channel,
                        "preconditioner not prepared for this channel");
+        if (device) {
+            for (const void *p : {static_cast<const void *>(b), static_cast<const void *>(x_inout)}) {
+                hipPointerAttribute_t attr{};
+                const hipError_t e = hipPointerGetAttributes(&attr, p);
+                if (e != hipSuccess)
+                    (void)hipGetLastError();  // unknown (host) memory: clear the error, refuse below
+                YH_REQUIRE(e == hipSuccess && attr.type == hipMemoryTypeDevice, "b and x_inout must be device memory");
+            }
+        }
+        const hipMemcpyKind in = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        const hipMemcpyKind out = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
         const long n = s->n;
         if (restart <= 0)
             restart = 20;
         restart = static_cast<int>(std::min<long>(restart, n));
         s->ensure_gmres(restart);
         hipStream_t st = s->stream;
-        YH_CHECK(hipMemcpyAsync(s->bvec, b, n * sizeof(double2), hipMemcpyHostToDevice, st));
-        YH_CHECK(hipMemcpyAsync(s->xvec, x_inout, n * sizeof(double2), hipMemcpyHostToDevice, st));
+        YH_CHECK(hipMemcpyAsync(s->bvec, b, n * sizeof(double2), in, st));
+        YH_CHECK(hipMemcpyAsync(s->xvec, x_inout, n * sizeof(double2), in, st));
         const auto o = s->gmres_run(channel, s->bvec, s->xvec, rtol, restart, maxiter, use_precond != 0,
                                     false, history, history ? history_cap : 0);
-        YH_CHECK(hipMemcpyAsync(x_inout, s->xvec, n * sizeof(double2), hipMemcpyDeviceToHost, st));
+        YH_CHECK(hipMemcpyAsync(x_inout, s->xvec, n * sizeof(double2), out, st));
         YH_CHECK(hipStreamSynchronize(st));
         if (info)
             *info = o.info;
@@ -668,6 +683,22 @@ int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double
         if (residual)
             *residual = o.bnorm > 0 ? o.rnorm / o.bnorm : 0.0;
     });
+}
+
+}  // namespace
+
+int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double *x_inout,
+                        double rtol, int restart, int maxiter, int use_precond, int *info,
+                        int *iterations, double *residual, double *history, int history_cap) {
+    return gmres_solve(s, channel, b, x_inout, rtol, restart, maxiter, use_precond, info, iterations, residual,
+                       history, history_cap, false);
+}
+
+int yasfhip_gmres_solve_device(yasfhip_session *s, int channel, const double *b, double *x_inout,
+                               double rtol, int restart, int maxiter, int use_precond, int *info,
+                               int *iterations, double *residual, double *history, int history_cap) {
+    return gmres_solve(s, channel, b, x_inout, rtol, restart, maxiter, use_precond, info, iterations, residual,
+                       history, history_cap, true);
 }
 
 int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b, double *x_inout,
