@@ -10,10 +10,11 @@ above 8), GMRES (restart 100) to 1e-6 relative residual.
 
 One step = one solve: preconditioner build + LU on the device, then device
 GMRES from x0 = b.  Geometry, T-matrix and pair table are resident on the
-device before timing; each solve uploads b (3 MB) and downloads x.  The same
-system is then solved with the mixed-precision path (FP64 iterative
-refinement around GMRES with the complex64 pair GEMM) and reported beside it.
-Prints one JSON line.
+device before timing (the pair-table build, about 2 ms, is not in the timed
+step); b and x are torch tensors in HBM, so no vector crosses PCIe inside the
+timed region.  The same system is then solved with the mixed-precision path
+(FP64 iterative refinement around GMRES with the complex64 pair GEMM) and
+reported beside it.  Prints one JSON line.
 
 ``--workload precision``: 10,000 spheres (radii uniform 50-150 nm, n =
 1.33 + 0.001i, volume fraction 0.15), lmax 5, lambda 600 nm: the
@@ -116,6 +117,77 @@ def cpu_baseline(pos, k, lmax, x, sample_particles, threads):
     return dt * N / sample_particles, used
 
 
+def host_cpu_model() -> str:
+    """CPU model of this host (as lscpu reports it)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def host_threads() -> tuple[int, int]:
+    """(threads used, CPUs this process may run on): every CPU of the
+    process's affinity set, within the thread allotment OMP_NUM_THREADS
+    states where the host sets one."""
+    affinity = len(os.sched_getaffinity(0))
+    try:
+        allotment = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        allotment = 0
+    return (min(affinity, allotment) if allotment > 0 else affinity), affinity
+
+
+def small_solve_comparison(threads: int, reps: int = 20):
+    """The 50-sphere CPU-runnable configuration (seed 1, lmax 4, volume
+    fraction 0.1, n 1.5+0.01i, 550 nm, plane wave +z TE) solved to 1e-6 with
+    the block-diagonal preconditioner (boxes of at most 8 spheres), in full on
+    the host by the oracle (its particle_interaction_scalar restatement under
+    SciPy's GMRES, blocks assembled and LU-factored on the host) with
+    `threads` and with one thread, and on the GPU (preconditioner build + LU +
+    device GMRES per solve, p50 of `reps`)."""
+    import oracle
+    from oracle import restate as R
+    from yasf_amd import _native
+    from yasf_amd.preconditioner import edge_size_for, make_particle_partition, split_oversized
+
+    lmax = 4
+    pos, radii, k, t, b = workload(seed=1, N=50, lmax=lmax, lam=550.0, vf=0.1)
+    part = split_oversized(make_particle_partition(pos, edge_size_for(pos, 8.0)), pos, 8)
+    boxes = np.empty(pos.shape[0], dtype=np.int32)
+    for i, members in enumerate(part):
+        boxes[members] = i
+    s = _native.Session(pos, lmax, [k])
+    s.set_tmatrix(t)
+    lat = []
+    for _ in range(reps + 2):
+        t0 = time.perf_counter()
+        s.precond_build(boxes)
+        x_gpu, info, iters, res, _ = s.gmres(b, b, rtol=1e-6, restart=100, maxiter=1000, use_precond=True)
+        lat.append(time.perf_counter() - t0)
+    s.close()
+    gpu_s = float(np.percentile(lat[2:], 50))
+    out = {"workload": "50 spheres, lmax 4, vf 0.1, 550 nm, block-diag precond (<= 8/box), GMRES(100) to 1e-6",
+           "gpu_seconds_p50": gpu_s, "gpu_iterations": iters, "gpu_info": info}
+    lib = oracle.load_native()
+    for label, th in (("cpu", threads), ("cpu_1_thread", 1)):
+        used = lib.set_threads(th)
+        t0 = time.perf_counter()
+        M = R.block_preconditioner(lmax, pos, k, t, part)
+        x_cpu, info_c, it_c = R.gmres_solve(lambda v: R.master_matvec(lmax, pos, k, t, v), b, rtol=1e-6,
+                                            restart=100, psolve=M)
+        sec = time.perf_counter() - t0
+        out[label] = {"seconds": sec, "threads": used, "iterations": it_c, "info": int(info_c)}
+    out["cpu_over_gpu_time"] = out["cpu"]["seconds"] / gpu_s
+    out["solution_rel_diff_gpu_vs_cpu"] = float(np.linalg.norm(x_gpu - x_cpu) / np.linalg.norm(x_cpu))
+    return out
+
+
 def pmc_calibration():
     """Known-size kernels for the HBM-traffic counters: a 1 GiB fill (writes
     1 GiB) and a 1 GiB device copy (reads and writes 1 GiB)."""
@@ -208,6 +280,19 @@ def run_solve(args):
         lat_m.append(time.perf_counter() - t0)
     torch.cuda.synchronize()
     xm, info_m, steps_m, iters_m, res_m, _ = mixed
+    # the same solve with the preconditioner's factors kept in complex128
+    fp64_factors = None
+    if use_pc:
+        s.set_precond_factor_precision(_native.FP64)
+        lat64f = []
+        for i in range(max(2, args.steps // 4)):
+            t0 = time.perf_counter()
+            r64f = step()
+            lat64f.append(time.perf_counter() - t0)
+        torch.cuda.synchronize()
+        fp64_factors = {"solves_per_s": len(lat64f) / sum(lat64f), "p50_latency_s": float(np.percentile(lat64f, 50)),
+                        "gmres_iterations": r64f[1], "relative_residual": r64f[2], "info": r64f[0]}
+        s.set_precond_factor_precision(_native.FP32)
     a_inc = b / t
     c64 = s.cross_sections(a_inc, x)
     cm = s.cross_sections(a_inc, xm)
@@ -271,6 +356,7 @@ def run_solve(args):
             "issued_mfma_tflops": f_issued / (pair_ms * 1e-3) / 1e12,
             "direct_count_tflops_per_matvec": f_direct / (mv_ms * 1e-3) / 1e12,
         },
+        "fp64_precond_factors": fp64_factors,
         "mixed_precision": {
             "solves_per_s": args.steps / sum(lat_m),
             "p50_latency_s": float(np.percentile(lat_m, 50)),
@@ -287,18 +373,28 @@ def run_solve(args):
         },
     }
     if not args.skip_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads, affinity = host_threads()
         xr = np.random.default_rng(0).normal(size=t.size) + 0j
         sec, threads = cpu_baseline(pos, k, lmax, xr, args.cpu_sample, threads)
+        sample1 = max(1, args.cpu_sample // 6)
+        sec1, _ = cpu_baseline(pos, k, lmax, xr, sample1, 1)
         matvecs = iters + max(1, iters // args.restart + 1) + 1
         line["cpu_baseline"] = {
             "value": 1.0 / (sec * matvecs),
-            "unit": "solves/s (extrapolated: matvec time x GPU matvec count, no preconditioner)",
+            "unit": "solves/s (extrapolated: CPU matvec time x the GPU solve's matvec count; "
+                    "the preconditioner's build and applies not counted)",
             "cores": threads,
             "kind": "port",
-            "sample": f"oracle particle_interaction_scalar, {args.cpu_sample} target spheres x "
-                      f"{N} sources, scaled to the full matvec ({sec:.2f} s/matvec)",
+            "sample": f"oracle particle_interaction_scalar (full p-range loop), {args.cpu_sample} target spheres x "
+                      f"{N} sources, scaled to the full matvec ({sec:.2f} s/matvec on {threads} threads)",
+            "cpu_model": host_cpu_model(),
+            "cpus_in_affinity": affinity,
+            "one_thread": {"value": 1.0 / (sec1 * matvecs), "seconds_per_matvec": sec1,
+                           "sample": f"{sample1} target spheres x {N} sources, scaled"},
+            "small_solve_in_full": small_solve_comparison(threads),
         }
+    if not args.no_serving:
+        line["serving"] = serving_report(args, lambda msg: print(msg, file=sys.stderr, flush=True))
     print(json.dumps(line), flush=True)
 
 
@@ -402,44 +498,73 @@ def run_precision(args):
     print(json.dumps(line), flush=True)
 
 
-def run_serving(args):
+def serving_report(args, log):
+    """The 512-request serving stream (seed 4) through the batched server,
+    closed-loop (all queued at the start) and with Poisson arrivals at
+    args.rate per second, against the all-FP64 single-solve path one request
+    at a time.  Returns a dict."""
     import torch
 
     from yasf_amd import _native
     from yasf_amd.serving import make_workload, serve, solve_sequential
 
-    def log(msg):
-        print(msg, file=sys.stderr, flush=True)
-
     t0 = time.perf_counter()
-    reqs = make_workload(args.requests, seed=4, rate=args.rate)
-    log(f"workload: {len(reqs)} requests prepared on the host in {time.perf_counter() - t0:.1f} s")
+    reqs = make_workload(args.requests, seed=4)
+    log(f"serving workload: {len(reqs)} requests prepared on the host in {time.perf_counter() - t0:.1f} s")
     srv = _native.Server(restart=50, inner_rtol=args.inner_rtol)
     tiles = {}
     for L in range(3, 9):
         tiles[L] = {}
         for cls, particles in (("small", 400), ("large", 2000)):
             (waves, ctiles), ms = srv.autotune(L, particles=particles, reps=3)
-            tiles[L][cls] = {"waves": waves, "column_tiles": ctiles, "ms": [round(float(v), 4) for v in ms]}
+            tiles[L][cls] = {"waves": waves, "column_tiles": ctiles}
     log(f"autotuned tile shapes: {tiles}")
     if args.warmup:
         serve(srv, make_workload(8, seed=99, n_range=(20, 200)))
     torch.cuda.synchronize()
-    rep = serve(srv, reqs, progress=log, device_setup=not args.host_setup)
-    log(f"served: {rep.as_dict()}")
-    base = None
+    device_setup = not args.host_setup
+    closed = serve(srv, reqs, progress=log, device_setup=device_setup)
+    log(f"served closed-loop: {closed.as_dict()}")
+    out = {"requests": len(reqs), "restart": 50, "inner_rtol": args.inner_rtol, "tile_shapes": tiles,
+           "setup": "device (Mie + plane wave at admission)" if device_setup else "host",
+           "closed_loop": {**closed.as_dict(),
+                           "converged": int(sum(1 for r in closed.results.values() if r["info"] == 0)),
+                           "mean_inner_iterations": float(np.mean([r["iterations"] for r in
+                                                                   closed.results.values()]))}}
+    if math.isfinite(args.rate) and args.rate > 0:
+        timed = make_workload(args.requests, seed=4, rate=args.rate)
+        poisson = serve(srv, timed, progress=log, device_setup=device_setup)
+        log(f"served Poisson {args.rate}/s: {poisson.as_dict()}")
+        out["poisson"] = {"arrival_rate_per_s": args.rate, **poisson.as_dict(),
+                          "converged": int(sum(1 for r in poisson.results.values() if r["info"] == 0))}
+    srv.close()
     if not args.no_baseline:
         base = solve_sequential(reqs, progress=log)
         log(f"sequential FP64: {base.as_dict()}")
-    infos = [r["info"] for r in rep.results.values()]
+        out["fp64_single_solve"] = {**base.as_dict(),
+                                    "converged": int(sum(1 for r in base.results.values() if r["info"] == 0))}
+        out["speedup_vs_fp64_single_solve"] = closed.solves_per_s / base.solves_per_s
+        out["max_rel_diff_c_ext"] = max(abs(closed.results[i]["c_ext"] - base.results[i]["c_ext"])
+                                        / abs(base.results[i]["c_ext"]) for i in closed.results)
+        out["max_rel_diff_c_sca"] = max(abs(closed.results[i]["c_sca"] - base.results[i]["c_sca"])
+                                        / abs(base.results[i]["c_sca"]) for i in closed.results)
+    return out
+
+
+def run_serving(args):
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    rep = serving_report(args, log)
+    closed = rep["closed_loop"]
     line = {
         "metric": METRIC,
-        "value": rep.solves_per_s,
+        "value": closed["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": len(reqs),
+        "steps": rep["requests"],
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * rep.makespan_s / len(reqs),
+        "ms_per_step": 1e3 * closed["makespan_s"] / rep["requests"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -447,27 +572,11 @@ def run_serving(args):
         "data": "synthetic (seeded request stream, seed 4)",
         "config": {"workload": "serving: 512 requests, N log-uniform [20, 3000], lmax 3..8, "
                                "lambda 300-1000 nm, random index and incidence, rtol 1e-6",
-                   "requests": len(reqs),
-                   "arrival_rate_per_s": args.rate if math.isfinite(args.rate) else None,
-                   "arrivals": "poisson" if math.isfinite(args.rate) else "all queued at start (closed)",
-                   "restart": 50, "inner_rtol": args.inner_rtol, "tile_shapes": tiles,
-                   "setup": "host" if args.host_setup else "device (Mie + plane wave at admission)"},
-        "p50_latency_s": rep.p50_s,
-        "p99_latency_s": rep.p99_s,
-        "makespan_s": rep.makespan_s,
-        "converged": int(sum(1 for i in infos if i == 0)),
-        "mean_inner_iterations": float(np.mean([r["iterations"] for r in rep.results.values()])),
+                   "requests": rep["requests"], "arrivals": "all queued at start (closed loop)"},
+        "p50_latency_s": closed["p50_latency_s"],
+        "p99_latency_s": closed["p99_latency_s"],
+        "serving": rep,
     }
-    if base is not None:
-        d_ext = max(abs(rep.results[i]["c_ext"] - base.results[i]["c_ext"]) / abs(base.results[i]["c_ext"])
-                    for i in rep.results)
-        d_sca = max(abs(rep.results[i]["c_sca"] - base.results[i]["c_sca"]) / abs(base.results[i]["c_sca"])
-                    for i in rep.results)
-        line["fp64_single_solve"] = {"solves_per_s": base.solves_per_s, "p50_latency_s": base.p50_s,
-                                     "p99_latency_s": base.p99_s, "makespan_s": base.makespan_s}
-        line["speedup_vs_fp64_single_solve"] = rep.solves_per_s / base.solves_per_s
-        line["max_rel_diff_c_ext"] = d_ext
-        line["max_rel_diff_c_sca"] = d_sca
     print(json.dumps(line), flush=True)
 
 
@@ -558,7 +667,10 @@ def main():
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="run a 1 GiB fill and copy first (reference kernels for rocprofv3 --pmc)")
     ap.add_argument("--requests", type=int, default=512, help="serving: number of requests")
-    ap.add_argument("--rate", type=float, default=math.inf, help="serving: Poisson arrivals per second")
+    ap.add_argument("--rate", type=float, default=16.0,
+                    help="serving: Poisson arrivals per second for the timed-arrival run (inf: closed loop only)")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="solve: leave out the serving sub-object (the 512-request stream)")
     ap.add_argument("--no-baseline", action="store_true", help="serving: skip the sequential FP64 run")
     ap.add_argument("--inner-rtol", type=float, default=1e-4,
                     help="mixed precision: least relative residual reduction of each inner GMRES cycle")

@@ -105,9 +105,13 @@ int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double
 
 /* yasfhip_gmres_solve with b and x_inout in device memory of the session's
  * device (n complex128 values each): the same solve (solver.py:108-121) for
- * callers whose vectors are already in HBM, with no host transfers.  Host
- * pointers are rejected (YASFHIP_INVALID_ARGUMENT); history stays a host
- * array.  Returns when the solve is done, as yasfhip_gmres_solve does. */
+ * callers whose vectors are already in HBM.  b and x stay in device memory
+ * (copied within HBM, never over PCIe); the solve's own scalars (Arnoldi
+ * coefficients, residual norms) still reach the host as they do in
+ * yasfhip_gmres_solve.  Host pointers, and buffers that do not hold n values
+ * inside one device allocation, are rejected (YASFHIP_INVALID_ARGUMENT)
+ * before any device work; history stays a host array.  Returns when the
+ * solve is done, as yasfhip_gmres_solve does. */
 int yasfhip_gmres_solve_device(yasfhip_session *s, int channel, const double *b, double *x_inout,
                                double rtol, int restart, int maxiter, int use_precond, int *info,
                                int *iterations, double *residual, double *history, int history_cap);

@@ -1,0 +1,194 @@
+"""Parity at the benchmark workloads' full sizes (GPU).
+
+The device operator is pinned at each size by rows of W x from the oracle
+(particle_interaction_scalar, cpu_numba.py:21-95) for a few target spheres, as
+tests/test_tiled_dense_backend.py:58-86 pins one backend against another
+(rtol 1e-12 in FP64); solutions are then held to the reference's criterion
+||b - A x|| <= 1e-6 ||b|| through that operator, and mixed-precision cross
+sections to 1e-6 of a 1e-10 FP64 solve.  Workloads:
+  * 2,000 spheres, lmax 6, seed 2: the preconditioned FP64 solve;
+  * 10,000 spheres, lmax 5, seed 3: FP64 / complex64 matvec rows, the mixed
+    solve (FP64 refinement) and its cross sections;
+  * the 512-request serving stream, seed 4: its largest requests served;
+  * a 1,200-monomer fractal aggregate at lmax 4: 16 incidence directions
+    sharing one operator in slab form, equal to unshared solves.
+"""
+
+import copy
+
+import numpy as np
+import pytest
+
+from oracle import restate as R
+from yasf_amd import _native
+from yasf_amd.clusters import cluster
+from yasf_amd.coefficients import mie_coefficients, planewave_coefficients
+from yasf_amd.model import Particles
+from yasf_amd.preconditioner import MAX_BLOCK_ROWS, edge_size_for, make_particle_partition, split_oversized
+from yasf_amd.serving import make_aggregate_workload, make_workload, serve
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(pos, radii, n_index, lam, lmax):
+    p = Particles(pos, radii, np.full(pos.shape[0], n_index))
+    omega = np.array([2 * np.pi / lam])
+    t = mie_coefficients(lmax, p.unique_radius_index_pairs, omega, omega)[p.single_unique_array_idx, :, 0].ravel()
+    a = planewave_coefficients(lmax, pos, omega, 0.0, 0.0, 1)[:, :, 0].ravel()
+    return float(omega[0]), t, a
+
+
+def _boxes(pos, nmax, per_box=8.0, cap=8):
+    part = split_oversized(make_particle_partition(pos, edge_size_for(pos, per_box)), pos,
+                           min(MAX_BLOCK_ROWS // nmax, cap))
+    boxes = np.empty(pos.shape[0], dtype=np.int32)
+    for i, members in enumerate(part):
+        boxes[members] = i
+    return boxes
+
+
+def _rows_match_oracle(s, lmax, pos, k, x, targets, precision=_native.FP64, rtol=1e-12):
+    nmax = 2 * lmax * (lmax + 2)
+    y = s.coupling_matvec(x, precision=precision)
+    for s1 in targets:
+        ref = R.coupling_matvec(lmax, pos, k, x, s1_lo=int(s1), n_s1=1)
+        got = y[s1 * nmax:(s1 + 1) * nmax]
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert err <= rtol, (s1, err)
+    return y
+
+
+def _rel_residual(s, t, a, x):
+    b = t * a
+    return np.linalg.norm(b - s.master_matvec(x)) / np.linalg.norm(b)
+
+
+def test_2000_spheres_lmax6_preconditioned_fp64_solve():
+    N, lmax, lam = 2000, 6, 550.0
+    pos, radii = cluster(2, N, 0.2)
+    k, t, a = _system(pos, radii, 1.5 + 0.01j, lam, lmax)
+    b = t * a
+    s = _native.Session(pos, lmax, [k])
+    s.set_tmatrix(t)
+    s.set_matvec_mode(_native.MATVEC_TABLE)
+    s.precond_build(_boxes(pos, 96))
+    x, info, iters, res, _ = s.gmres(b, b, rtol=1e-6, restart=100, maxiter=1000, use_precond=True)
+    assert info == 0 and res <= 1e-6
+    _rows_match_oracle(s, lmax, pos, k, x, [0, 977, N - 1])
+    assert _rel_residual(s, t, a, x) <= 1e-6
+    print(f"2000 spheres lmax 6: {iters} iterations, residual {res:.2e}", flush=True)
+    s.close()
+
+
+def test_10000_spheres_lmax5_mixed_precision():
+    N, lmax, lam = 10000, 5, 600.0
+    pos, radii = cluster(3, N, 0.15, uniform=(50.0, 150.0))
+    k, t, a = _system(pos, radii, 1.33 + 0.001j, lam, lmax)
+    b = t * a
+    boxes = _boxes(pos, 70)
+    x_probe = np.random.default_rng(31).normal(size=t.size) + 1j * np.random.default_rng(32).normal(size=t.size)
+    targets = [0, 5003, N - 1]
+    s = _native.Session(pos, lmax, [k])  # AUTO: the complex64 table resident, FP64 slabs
+    s.set_tmatrix(t)
+    s.precond_build(boxes)
+    _rows_match_oracle(s, lmax, pos, k, x_probe, targets, precision=_native.FP32, rtol=2e-5)
+    xm, info, steps, iters, res, _ = s.gmres_refined(b, b, rtol=1e-6, inner_rtol=1e-4, restart=50,
+                                                      inner_maxiter=1000, max_refinements=10, use_precond=True)
+    assert info == 0 and res <= 1e-6
+    _rows_match_oracle(s, lmax, pos, k, x_probe, targets)
+    assert _rel_residual(s, t, a, xm) <= 1e-6
+    cm = s.cross_sections(a, xm)
+    # the FP64 solve to 1e-10, warm-started from the mixed solution
+    xr, info_r, _, res_r, _ = s.gmres(b, xm, rtol=1e-10, restart=50, maxiter=1000, use_precond=True)
+    assert info_r == 0 and res_r <= 1e-10
+    cr = s.cross_sections(a, xr)
+    assert abs(cm[0] - cr[0]) <= 1e-6 * abs(cr[0])
+    assert abs(cm[1] - cr[1]) <= 1e-6 * abs(cr[1])
+    print(f"10000 spheres lmax 5: {steps} refinements, {iters} inner iterations, residual {res:.2e}, "
+          f"C_ext err {abs(cm[0] - cr[0]) / abs(cr[0]):.1e}", flush=True)
+    s.close()
+
+
+@pytest.fixture(scope="module")
+def serving_stream():
+    return make_workload(512, seed=4)
+
+
+def test_serving_stream_largest_requests(serving_stream):
+    reqs = serving_stream
+    by_n = max(reqs, key=lambda r: r.particles)
+    by_unknowns = max(reqs, key=lambda r: r.unknowns)
+    by_l8 = max((r for r in reqs if r.lmax == 8), key=lambda r: r.particles)
+    picked = {r.index: r for r in (by_n, by_unknowns, by_l8, reqs[0], reqs[1], reqs[2])}
+    assert by_n.particles >= 2900 and by_unknowns.lmax >= 7
+    srv = _native.Server(restart=50, inner_rtol=1e-4)
+    rep = serve(srv, list(picked.values()), keep_solutions=True)
+    srv.close()
+    for r in picked.values():
+        res = rep.results[r.index]
+        assert res["info"] == 0, (r.index, r.particles, r.lmax)
+        s = _native.Session(r.positions, r.lmax, [r.k])
+        s.set_tmatrix(r.t[np.newaxis, :])
+        if r is by_unknowns:
+            _rows_match_oracle(s, r.lmax, r.positions, r.k, res["x"], [0, r.particles - 1])
+        rel = _rel_residual(s, r.t, r.a_inc, res["x"])
+        assert rel <= 1e-6 and res["residual"] == pytest.approx(rel, rel=1e-6, abs=1e-12)
+        b = r.t * r.a_inc
+        xt, info, _, res_t, _ = s.gmres(b, res["x"], rtol=1e-10, restart=100, maxiter=100)
+        assert info == 0
+        ce, cs = s.cross_sections(r.a_inc, xt)
+        assert abs(res["c_ext"] - ce) <= 1e-6 * abs(ce)
+        assert abs(res["c_sca"] - cs) <= 1e-6 * abs(cs)
+        print(f"request {r.index}: N {r.particles} lmax {r.lmax}: {res['cycles']} cycles, "
+              f"{res['iterations']} iterations, residual {rel:.2e}", flush=True)
+        s.close()
+
+
+def test_aggregate_16_directions_shared_slab_operator():
+    reqs = make_aggregate_workload([1200], [500.0], 16, seed=5)
+    assert len(reqs) == 16 and len({r.operator_key for r in reqs}) == 1
+    srv = _native.Server(restart=50, inner_rtol=1e-4)
+    srv.set_table_policy(_native.Server.TABLES_SLABS, slab_tiles=8)
+    shared = serve(srv, reqs, keep_solutions=True)
+    srv.close()
+    plain = [copy.copy(r) for r in reqs]
+    for r in plain:
+        r.operator_key = 0
+    srv = _native.Server(restart=50, inner_rtol=1e-4)
+    srv.set_table_policy(_native.Server.TABLES_SLABS, slab_tiles=8)
+    unshared = serve(srv, plain, keep_solutions=True)
+    srv.close()
+    r0 = reqs[0]
+    s = _native.Session(r0.positions, r0.lmax, [r0.k])
+    s.set_tmatrix(r0.t[np.newaxis, :])
+    _rows_match_oracle(s, r0.lmax, r0.positions, r0.k, shared.results[r0.index]["x"], [0, 600, 1199])
+    for r in reqs:
+        a, u = shared.results[r.index], unshared.results[r.index]
+        assert a["info"] == 0 and np.array_equal(a["x"], u["x"])
+        assert a["iterations"] == u["iterations"] and a["c_ext"] == u["c_ext"] and a["c_sca"] == u["c_sca"]
+        assert _rel_residual(s, r.t, r.a_inc, a["x"]) <= 1e-6
+        ce, cs = s.cross_sections(r.a_inc, a["x"])
+        assert a["c_ext"] == pytest.approx(ce, rel=1e-10) and a["c_sca"] == pytest.approx(cs, rel=1e-10)
+    s.close()
+
+
+def test_preconditioner_box_at_the_row_limit():
+    # one box of 24 spheres at lmax 8: 3,840 rows, the largest block the
+    # device takes (its apply keeps 60 KiB of right-hand side in LDS)
+    N, lmax, lam = 24, 8, 500.0
+    pos, radii = cluster(17, N, 0.1)
+    k, t, a = _system(pos, radii, 1.6 + 0.02j, lam, lmax)
+    assert N * 2 * lmax * (lmax + 2) == MAX_BLOCK_ROWS
+    s = _native.Session(pos, lmax, [k])
+    s.set_tmatrix(t)
+    s.set_precond_factor_precision(_native.FP64)
+    s.precond_build(np.zeros(N, dtype=np.int32))
+    r = np.random.default_rng(5).normal(size=t.size) + 1j * np.random.default_rng(6).normal(size=t.size)
+    z = s.precond_apply(r)
+    # one box holding every sphere: M = I - T W, so M z must give back r
+    back = R.master_matvec(lmax, pos, k, t, z)
+    assert np.linalg.norm(back - r) / np.linalg.norm(r) <= 1e-10
+    b = t * a
+    x, info, iters, res, _ = s.gmres(b, b, rtol=1e-10, restart=20, maxiter=5, use_precond=True)
+    assert info == 0 and iters <= 2
+    s.close()

@@ -117,61 +117,55 @@ def test_single_box_preconditioner_is_exact():
     assert info == 0 and iters == 1 and res < 1e-8
 
 
-_DEVICE_GMRES_CHILD = r"""
-import ctypes, sys
-import numpy as np
-import torch
-torch.cuda.init()  # torch's HIP runtime first, as in bench.py, then the library
-from oracle import restate as R
-from yasf_amd import _native
-from yasf_amd.clusters import cluster
-pos, radii = cluster(12, 30, 0.1)
-lmax, lam = 4, 550.0
-k = 2 * np.pi / lam
-t = R.mie_coefficients(lmax, radii, np.full(30, 1.5 + 0.01j), np.array([lam]), 1.0)[:, :, 0].ravel()
-a = R.planewave_coefficients(lmax, pos, np.array([k]), 0.0, 0.0, 1)[:, :, 0].ravel()
-b = t * a
-s = _native.Session(pos, lmax, [k])
-s.set_tmatrix(t)
-s.precond_build(np.arange(30, dtype=np.int32) // 8)
-x, info, iters, res, hist = s.gmres(b, b, rtol=1e-7, restart=30, maxiter=20, use_precond=True)
-b_dev = torch.from_numpy(b.copy()).to("cuda")
-x_dev = b_dev.clone()
-info_d, iters_d, res_d, hist_d = s.gmres_device(b_dev, x_dev, rtol=1e-7, restart=30, maxiter=20,
-                                               use_precond=True)
-assert info == info_d == 0 and iters == iters_d and res == res_d, (info, info_d, iters, iters_d, res, res_d)
-assert np.array_equal(hist, hist_d)
-assert np.array_equal(x_dev.cpu().numpy(), x)
-try:
-    s.gmres_device(b_dev.cpu(), x_dev)  # host tensors are refused before any device work
-    raise SystemExit("host tensor accepted")
-except ValueError:
-    pass
-ic, itc, rc = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-status = s._lib.yasfhip_gmres_solve_device(  # a host pointer at the C ABI is refused
-    s._h, 0, b.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(x_dev.data_ptr()), 1e-7, 30, 20, 1,
-    ctypes.byref(ic), ctypes.byref(itc), ctypes.byref(rc), None, 0)
-assert status == 1, status  # YASFHIP_INVALID_ARGUMENT
-x_dev.copy_(b_dev)  # the session still works after the refusal
-assert s.gmres_device(b_dev, x_dev, rtol=1e-7, restart=30, maxiter=20, use_precond=True)[1] == iters
-print("device gmres ok", iters, flush=True)
-"""
-
-
 def test_gmres_device_vectors_match_host_vectors():
     # The device-pointer entry point runs the same solve on vectors in HBM:
     # identical solution, iterations, residual and history.  Its vectors are
-    # torch tensors, and torch's GPU runtime must start before the library in
-    # a process (bench.py's order), so the check runs in a child interpreter.
-    import subprocess
-    import sys
-    from pathlib import Path
+    # torch tensors in this process (the library shares torch's HIP runtime).
+    import ctypes
 
-    root = Path(__file__).resolve().parents[1]
-    out = subprocess.run([sys.executable, "-c", _DEVICE_GMRES_CHILD], cwd=root, capture_output=True, text=True,
-                         timeout=300)
-    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-4000:]
-    assert "device gmres ok" in out.stdout
+    import torch
+
+    assert len(_native.hip_runtimes()) == 1
+    pos, radii = cluster(12, 30, 0.1)
+    lmax, lam = 4, 550.0
+    k = 2 * np.pi / lam
+    t = R.mie_coefficients(lmax, radii, np.full(30, 1.5 + 0.01j), np.array([lam]), 1.0)[:, :, 0].ravel()
+    a = R.planewave_coefficients(lmax, pos, np.array([k]), 0.0, 0.0, 1)[:, :, 0].ravel()
+    b = t * a
+    s = _native.Session(pos, lmax, [k])
+    s.set_tmatrix(t)
+    s.precond_build(np.arange(30, dtype=np.int32) // 8)
+    x, info, iters, res, hist = s.gmres(b, b, rtol=1e-7, restart=30, maxiter=20, use_precond=True)
+    b_dev = torch.from_numpy(b.copy()).to("cuda")
+    x_dev = b_dev.clone()
+    info_d, iters_d, res_d, hist_d = s.gmres_device(b_dev, x_dev, rtol=1e-7, restart=30, maxiter=20,
+                                                   use_precond=True)
+    assert info == info_d == 0 and iters == iters_d and res == res_d
+    assert np.array_equal(hist, hist_d)
+    assert np.array_equal(x_dev.cpu().numpy(), x)
+    with pytest.raises(ValueError):
+        s.gmres_device(b_dev.cpu(), x_dev)  # host tensors are refused before any device work
+    ic, itc, rc = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+
+    def raw(bp, xp):
+        return s._lib.yasfhip_gmres_solve_device(s._h, 0, bp, xp, 1e-7, 30, 20, 1, ctypes.byref(ic),
+                                                 ctypes.byref(itc), ctypes.byref(rc), None, 0)
+
+    # a host pointer at the C ABI is refused (YASFHIP_INVALID_ARGUMENT)
+    assert raw(b.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(x_dev.data_ptr())) == 1
+    # so is a device allocation shorter than n complex128 values (a direct
+    # hipMalloc of 256 bytes from the runtime torch and the library share)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    small = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(small), ctypes.c_size_t(256)) == 0
+    try:
+        assert raw(ctypes.c_void_p(b_dev.data_ptr()), small) == 1
+        assert "inside one device allocation" in _native.library().yasfhip_last_error().decode()
+    finally:
+        assert hip.hipFree(small) == 0
+    x_dev.copy_(b_dev)  # the session still works after the refusals
+    assert s.gmres_device(b_dev, x_dev, rtol=1e-7, restart=30, maxiter=20, use_precond=True)[1] == iters
+    s.close()
 
 
 def test_gmres_errors_and_nonconvergence():

@@ -100,3 +100,34 @@ def test_backend_selection():
     sim = Simulation(params, Numerics(lmax=2, coupling_backend="nearfar"))
     with pytest.raises(ValueError):
         sim.coupling_matrix_multiply(np.zeros(16, dtype=complex))
+
+
+def test_hip_runtime_list_from_maps_text():
+    maps = "\n".join([
+        "7f00-7f10 r-xp 00000000 08:01 1 /usr/lib/x/torch/lib/libamdhip64.so",
+        "7f10-7f20 r--p 00010000 08:01 1 /usr/lib/x/torch/lib/libamdhip64.so",
+        "7f20-7f30 r-xp 00000000 08:01 2 /opt/rocm-7.2.0/lib/libamdhip64.so.7.2.0",
+        "7f30-7f40 rw-p 00000000 00:00 0 ",
+        "7f40-7f50 r-xp 00000000 08:01 3 /usr/lib/libc.so.6",
+    ])
+    assert _native.hip_runtimes_in_maps(maps) == ["/usr/lib/x/torch/lib/libamdhip64.so",
+                                                  "/opt/rocm-7.2.0/lib/libamdhip64.so.7.2.0"]
+    assert _native.hip_runtimes_in_maps("") == []
+
+
+def test_library_loads_into_torch_hip_runtime():
+    # Loading the library first must not leave two HIP runtimes in a process
+    # that uses torch: _load imports torch before the library, so the
+    # library's libamdhip64.so.7 resolves to the runtime torch already mapped.
+    import subprocess
+    import sys
+
+    code = ("from yasf_amd import _native\n"
+            "_native.library()\n"
+            "import torch\n"
+            "print('\\n'.join(_native.hip_runtimes()))\n")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-3000:]
+    runtimes = [line for line in out.stdout.splitlines() if line.strip()]
+    assert len(runtimes) == 1, runtimes
+    assert "torch" in runtimes[0]

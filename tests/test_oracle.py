@@ -310,3 +310,84 @@ def test_lossless_cluster_conserves_energy():
     pos, k, lmax, a_inc, f = _solved_cluster(1.45 + 0.0j, seed=22)
     c_ext, c_sca = R.cross_sections(lmax, pos, k, a_inc, f)
     assert c_ext == pytest.approx(c_sca, rel=1e-9)
+
+
+def test_spherical_functions_trigon_fixture(golden):
+    """pi_lm, tau_lm (spherical_functions_trigon.py:39-93) against the
+    reference's MATLAB fixture (tests/test_comparison.py:79-89, rtol 1e-5)."""
+    d = golden("spherical_functions_trigon")
+    lmax = int(d["lmax_sft"][0, 0])
+    theta = d["theta_sft"].ravel()
+    pilm, taulm = R.spherical_functions_trigon(lmax, np.cos(theta), np.sin(theta), all_angles=True)
+    npt.assert_allclose(pilm, d["pilm_sft"], rtol=1e-5, atol=1e-12)
+    npt.assert_allclose(taulm, d["taulm_sft"], rtol=1e-5, atol=1e-12)
+    assert np.count_nonzero(d["pilm_sft"]) > 0 and np.count_nonzero(d["taulm_sft"]) > 0
+
+
+@pytest.mark.parametrize("tau_key, pol_key, dagger, expected_key", [
+    ("tau_tc_1", "pol_tc_1", False, "B_11"), ("tau_tc_1", "pol_tc_2", False, "B_12"),
+    ("tau_tc_2", "pol_tc_1", False, "B_21"), ("tau_tc_2", "pol_tc_2", False, "B_22"),
+    ("tau_tc_1", "pol_tc_1", True, "Bd_11"), ("tau_tc_1", "pol_tc_2", True, "Bd_12"),
+    ("tau_tc_2", "pol_tc_1", True, "Bd_21"), ("tau_tc_2", "pol_tc_2", True, "Bd_22")])
+def test_transformation_coefficients_fixture(golden, tau_key, pol_key, dagger, expected_key):
+    """B_{tau l m}(beta) and its dagger (misc.py:70-114) against the MATLAB
+    fixture (tests/test_comparison.py:157-185, rtol 1e-5)."""
+    d = golden("transformation_coefficients")
+    lmax = int(d["lmax_tc"][0, 0])
+    theta = float(d["theta_tc"][0, 0])
+    pilm, taulm = R.spherical_functions_trigon(lmax, np.cos(theta), np.sin(theta))
+    out = R.transformation_coefficients(pilm, taulm, int(d[tau_key][0, 0]), int(d["l_tc"][0, 0]),
+                                        int(d["m_tc"][0, 0]), int(d[pol_key][0, 0]), dagger=dagger)
+    npt.assert_allclose(out, np.squeeze(d[expected_key]), rtol=1e-5)
+
+
+def test_polarization_degrees_single_sphere_textbook():
+    """Degrees of polarisation (cpu_numba.py:609-671) of one sphere's far field
+    against the textbook Stokes parameters of the scattered light built from
+    S1, S2 (Bohren & Huffman 3.16, 4.74-4.77): for incident light linearly
+    polarised at angle psi to the scattering plane,
+      I ~ S11 + S12 cos 2psi,  Q ~ S12 + S11 cos 2psi,  U ~ S33 sin 2psi,
+      V ~ S34 sin 2psi (up to sign),
+    with S11 = (|S2|^2 + |S1|^2)/2, S12 = (|S2|^2 - |S1|^2)/2, S33 = Re(S2 S1*),
+    S34 = Im(S2 S1*).  The TE plane wave along +z is polarised along y, so
+    psi = phi - pi/2.  A single coherent field is fully polarised (DOP = 1).
+    The product's host function is held to the same values."""
+    from yasf_amd.optics import polarization_components as product_components
+
+    lmax, r, m, lam = 8, 180.0, 1.6 + 0.05j, 550.0
+    k = 2 * np.pi / lam
+    pos = np.zeros((1, 3))
+    mie = R.mie_coefficients(lmax, np.array([r]), np.array([m]), np.array([lam]), 1.0)[0, :, 0]
+    a_inc = R.planewave_coefficients(lmax, pos, np.array([k]), 0.0, 0.0, 1)[0, :, 0]
+    th = np.linspace(0.1, 3.0, 13)
+    ph = np.linspace(0.05, 2 * np.pi - 0.05, 11)
+    TH, PH = np.meshgrid(th, ph, indexing="ij")
+    et, ep = R.far_field(lmax, pos, k, mie * a_inc, TH.ravel(), PH.ravel())
+    an, bn = _bh_mie(lmax, k * r, m)
+    mu = np.cos(TH.ravel())
+    pi_prev, pi_cur = np.zeros_like(mu), np.ones_like(mu)
+    s1 = np.zeros_like(mu, dtype=complex)
+    s2 = np.zeros_like(mu, dtype=complex)
+    for nn in range(1, lmax + 1):
+        if nn > 1:
+            pi_prev, pi_cur = pi_cur, ((2 * nn - 1) * mu * pi_cur - nn * pi_prev) / (nn - 1)
+        tau = nn * mu * pi_cur - (nn + 1) * pi_prev
+        c = (2 * nn + 1) / (nn * (nn + 1))
+        s1 += c * (an[nn - 1] * pi_cur + bn[nn - 1] * tau)
+        s2 += c * (an[nn - 1] * tau + bn[nn - 1] * pi_cur)
+    s11 = (np.abs(s2) ** 2 + np.abs(s1) ** 2) / 2
+    s12 = (np.abs(s2) ** 2 - np.abs(s1) ** 2) / 2
+    s33 = np.real(s2 * np.conj(s1))
+    s34 = np.imag(s2 * np.conj(s1))
+    psi = PH.ravel() - np.pi / 2
+    c2, sn2 = np.cos(2 * psi), np.sin(2 * psi)
+    i_t = s11 + s12 * c2
+    q_t, u_t, v_t = s12 + s11 * c2, s33 * sn2, s34 * sn2
+    for comp in (R.polarization_components, product_components):
+        I, dop, dolp, dolq, dolu, docp = comp(et, ep)
+        npt.assert_allclose(I / I.max(), i_t / i_t.max(), rtol=1e-9, atol=1e-12)
+        npt.assert_allclose(dop, 1.0, rtol=1e-12)
+        npt.assert_allclose(dolp, np.sqrt(q_t ** 2 + u_t ** 2) / i_t, rtol=1e-9, atol=1e-12)
+        npt.assert_allclose(dolq, -q_t / i_t, rtol=1e-9, atol=1e-12)
+        npt.assert_allclose(np.abs(dolu), np.abs(u_t) / i_t, rtol=1e-9, atol=1e-12)
+        npt.assert_allclose(np.abs(docp), np.abs(v_t) / i_t, rtol=1e-9, atol=1e-12)

@@ -35,6 +35,25 @@ class DeviceError(NativeLibraryError):
     """A HIP call inside the library failed (no GPU, out of memory, ...)."""
 
 
+def hip_runtimes_in_maps(maps_text: str) -> list[str]:
+    """Distinct paths of HIP runtime libraries (libamdhip64) in the text of
+    /proc/<pid>/maps."""
+    paths = []
+    for line in maps_text.splitlines():
+        fields = line.split()
+        if len(fields) >= 6 and "libamdhip64" in Path(fields[-1]).name and fields[-1] not in paths:
+            paths.append(fields[-1])
+    return paths
+
+
+def hip_runtimes() -> list[str]:
+    """HIP runtime libraries mapped into this process."""
+    try:
+        return hip_runtimes_in_maps(Path("/proc/self/maps").read_text())
+    except OSError:
+        return []
+
+
 def _load():
     global _lib
     with _lock:
@@ -43,6 +62,15 @@ def _load():
         if not LIB_PATH.exists():
             raise NativeLibraryError(
                 f"{LIB_PATH} is not built; run `python -m yasf_amd.build` (hipcc, gfx950)")
+        # One HIP runtime per process.  torch ships its own libamdhip64 (same
+        # SONAME as ROCm's) and its libraries ask for it by file name: loaded
+        # first, it also serves this library; loaded after this library, it
+        # would come in as a second runtime that cannot see the first one's
+        # device memory.  So torch, where installed, is imported first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(str(LIB_PATH))
         P = ctypes.c_void_p
         i = ctypes.c_int
@@ -257,6 +285,11 @@ class Session:
         (info, inner_iterations, relative_residual, pr_norm history)."""
         import torch  # device vectors only; the rest of the module needs no torch
 
+        runtimes = hip_runtimes()
+        if len(runtimes) > 1:
+            raise NativeLibraryError(
+                "two HIP runtimes are loaded in this process (" + ", ".join(runtimes) + "): torch's device "
+                "memory is not visible to libyasfhip.so; import torch before anything loads ROCm's libamdhip64")
         for name, v in (("b", b), ("x", x)):
             if v.dtype != torch.complex128 or not v.is_cuda or not v.is_contiguous() or v.numel() != self.n:
                 raise ValueError(f"{name} must be a contiguous complex128 device tensor of {self.n} values")

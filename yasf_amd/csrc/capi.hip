@@ -660,6 +660,17 @@ int gmres_solve(yasfhip_session *s, int channel, const double *b, double *x_inou
                 if (e != hipSuccess)
                     (void)hipGetLastError();  // unknown (host) memory: clear the error, refuse below
                 YH_REQUIRE(e == hipSuccess && attr.type == hipMemoryTypeDevice, "b and x_inout must be device memory");
+                // the n complex128 values must lie inside one allocation
+                hipDeviceptr_t base = nullptr;
+                size_t bytes = 0;
+                const hipError_t r = hipMemGetAddressRange(&base, &bytes, const_cast<void *>(p));
+                if (r != hipSuccess)
+                    (void)hipGetLastError();
+                const char *lo = static_cast<const char *>(base);
+                const char *q = static_cast<const char *>(p);
+                YH_REQUIRE(r == hipSuccess && q >= lo &&
+                               static_cast<size_t>(q - lo) + static_cast<size_t>(s->n) * sizeof(double2) <= bytes,
+                           "b and x_inout must each hold n complex128 values inside one device allocation");
             }
         }
         const hipMemcpyKind in = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;

@@ -37,6 +37,11 @@ struct BlockPrecond {
 
 // Largest padded block (rows); the apply keeps one block's right-hand side in LDS.
 constexpr int kMaxBlock = 3840;
+// The apply's LDS: the box's right-hand side (up to kMaxBlock complex128 rows,
+// dynamic) plus yb and Dsh; above 64 KiB per workgroup it needs the
+// dynamic-LDS opt-in, which the build sets.
+static_assert(kMaxBlock * 16 + (16 + 256) * 16 <= 160 * 1024,
+              "lu_apply_kernel exceeds the 160 KiB of LDS per workgroup");
 
 namespace {
 
@@ -758,9 +763,14 @@ BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &
                            bp->blk_off, bp->blocks);
         YH_CHECK(hipGetLastError());
         static bool lu_lds_opt_in = false;  // gfx950: up to 160 KB of LDS per workgroup
-        if (!lu_lds_opt_in) {
+        if (!lu_lds_opt_in) {  // set here, at the build, so that later applies can be graph-captured
             YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&lu_blocked_kernel),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLuLds)));
+            const int dyn = static_cast<int>(kMaxBlock * sizeof(double2));
+            YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&lu_apply_kernel<float2>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+            YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&lu_apply_kernel<double2>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
             lu_lds_opt_in = true;
         }
         hipLaunchKernelGGL(lu_blocked_kernel, dim3(bp->nbox), dim3(64 * kLuWaves), kLuLds, s, bp->box_first, nmax,
