@@ -85,6 +85,22 @@ int yasfhip_set_tmatrix(yasfhip_session *s, const double *t);
 int yasfhip_coupling_matvec(yasfhip_session *s, int channel, const double *x, double *y,
                             int precision);
 
+/* Allocation-free y = W x: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same operator as
+ * yasfhip_coupling_matvec in FP64 with the pair factors generated on the fly
+ * (YASFHIP_MATVEC_ON_THE_FLY), bit for bit, without a session: positions
+ * (N x 3 doubles), x and y (N * nmax complex128 each) are device pointers on
+ * the current device; everything else (coupling plan, padded positions, the
+ * pair GEMM's partial sums) lives in a 64 MiB array of device memory that
+ * the library holds per GPU until the process ends.  Enqueued on stream
+ * (a hipStream_t, NULL: the null stream) without waiting for it; calls no
+ * hipMalloc/hipFree.  At most 32768 particles, lmax 1..8.  Calls on one GPU
+ * share the array: calls that overlap on one GPU (other streams, threads)
+ * race. */
+int yasfhip_coupling_matvec_noalloc(const double *positions, int particle_number, int lmax,
+                                    double k_medium, const double *x, double *y, void *stream);
+
 /* y = x - t * (W x) (master_matrix_multiply, yasfpy/matrix_apply.py:66-120). */
 int yasfhip_master_matvec(yasfhip_session *s, int channel, const double *x, double *y,
                           int precision);

@@ -113,6 +113,7 @@ def _load():
             "yasfhip_planewave_coefficients": ([i, i, P, d, d, d, i, d, P, P], i),
             "yasfhip_set_matvec_mode": ([P, i], i),
             "yasfhip_tune_matvec": ([P, i, i, i, ip, P], i),
+            "yasfhip_coupling_matvec_noalloc": ([P, i, i, d, P, P, P], i),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -138,7 +139,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_server_autotune", "yasfhip_server_set_tile_shape", "yasfhip_server_set_table_policy",
         "yasfhip_set_slab_tiles", "yasfhip_set_near_field_radius", "yasfhip_mie_coefficients",
         "yasfhip_planewave_coefficients", "yasfhip_server_submit_spheres", "yasfhip_set_table_shape",
-        "yasfhip_set_precond_factor_precision",
+        "yasfhip_set_precond_factor_precision", "yasfhip_coupling_matvec_noalloc",
     ]
 
 

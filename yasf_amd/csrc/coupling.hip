@@ -42,7 +42,7 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
                                                    const double *__restrict__ lb,
                                                    const double *__restrict__ ld, T *__restrict__ Q, int nq,
                                                    int stage_elems, const int *__restrict__ gate,
-                                                   double near2) {
+                                                   double near2, int t0) {
     typedef typename Mfma<T>::acc_t acc_t;
     typedef typename Cplx<T>::type T2;
     constexpr int SRC = 16;  // sources per chunk (4 MFMA K-steps)
@@ -53,8 +53,9 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
         return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool producer = wave >= kWavesPerJob;
+    // ntiles1 target tiles from t0 on; Q holds them from its start
     const int jid = blockIdx.x / ntiles1;
-    const int s1t = blockIdx.x - jid * ntiles1;
+    const int s1t = t0 + static_cast<int>(blockIdx.x) - jid * ntiles1;
     const int dm = uniform(jobs[jid].dm);
     const int tile0 = uniform(jobs[jid].tile0);
     const int ntile = uniform(jobs[jid].ntile);
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(512) void pair_kernel(const double4 *__restrict__ p
         const int c = wave + kWavesPerJob * j;
         if (c < total) {
             const int ip = c / ntile, it = c - ip * ntile;
-            const size_t slot = static_cast<size_t>(s1t) * nq + qbase + ip * tstride + tile0 + it;
+            const size_t slot = static_cast<size_t>(s1t - t0) * nq + qbase + ip * tstride + tile0 + it;
             T *dst = Q + 2 * ((slot * 16 + lo) * 16);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -203,19 +204,21 @@ __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q,
                                                        const double2 *__restrict__ cval, int N, int nmax,
                                                        const TX *__restrict__ x,
                                                        const double2 *__restrict__ t, int mode,
-                                                       TY *__restrict__ y) {
+                                                       TY *__restrict__ y, int t0, int t1) {
+    // target tiles [t0, t1); Q holds them from its start
     const int groups = contract_groups(nmax);
     const int q = blockIdx.x >> 3, xcd = blockIdx.x & 7;
-    const int s1t = (q / groups) * 8 + xcd;
-    if (s1t * 16 >= N)
+    const int s1t = t0 + (q / groups) * 8 + xcd;
+    if (s1t >= t1 || s1t * 16 >= N)
         return;
-    contract_tile<TQ, TX, TY>(Q, nq, crow, cidx, cval, N, nmax, x, t, mode, y, s1t, q % groups);
+    contract_tile<TQ, TX, TY>(Q, nq, crow, cidx, cval, N, nmax, x, t, mode, y, s1t, q % groups, t0);
 }
 
 template <typename T, bool Regular>
 static void launch_pair_t(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
-                          void *Q, hipStream_t s, const int *gate = nullptr, double near2 = kAllPairs) {
-    const int ntiles1 = (N + 15) / 16;
+                          void *Q, hipStream_t s, const int *gate = nullptr, double near2 = kAllPairs,
+                          int t0 = 0, int ntl = -1) {
+    const int ntiles1 = ntl < 0 ? (N + 15) / 16 - t0 : ntl;
     const size_t lds = 2 * static_cast<size_t>(dp.stage_elems) * sizeof(typename Cplx<T>::type);
     static bool lds_opt_in = false;  // gfx950: up to 160 KB of LDS per workgroup
     if (!lds_opt_in) {
@@ -226,16 +229,16 @@ static void launch_pair_t(const DevicePlan &dp, const double4 *pos, int N, doubl
     hipLaunchKernelGGL((pair_kernel<T, Regular>), dim3(dp.njobs * ntiles1), dim3(2 * kWavesPerJob * 64), lds, s,
                        pos, static_cast<const typename Cplx<T>::type *>(x), N, dp.nmax, k, dp.jobs,
                        ntiles1, dp.colmap, dp.maxcols, dp.lmax, dp.la, dp.lb, dp.ld, static_cast<T *>(Q),
-                       dp.nq, dp.stage_elems, gate, near2);
+                       dp.nq, dp.stage_elems, gate, near2, t0);
     YH_CHECK(hipGetLastError());
 }
 
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
-                 void *Q, hipStream_t s, const int *gate, double near2) {
+                 void *Q, hipStream_t s, const int *gate, double near2, int t0, int ntl) {
     if (fp32)
-        launch_pair_t<float, false>(dp, pos, N, k, x, Q, s, gate, near2);
+        launch_pair_t<float, false>(dp, pos, N, k, x, Q, s, gate, near2, t0, ntl);
     else
-        launch_pair_t<double, false>(dp, pos, N, k, x, Q, s, gate, near2);
+        launch_pair_t<double, false>(dp, pos, N, k, x, Q, s, gate, near2, t0, ntl);
 }
 
 void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
@@ -251,21 +254,23 @@ size_t pair_lds_bytes(const DevicePlan &dp) { return 2 * static_cast<size_t>(dp.
 // otherwise complex128 (the mixed-precision operator reads FP32 Q with FP64
 // x and y, so x - t W x is formed in FP64).
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
-                     int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s) {
-    const int nblocks = ((N + 15) / 16 + 7) / 8 * 8 * contract_groups(dp.nmax);
+                     int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s, int t0, int ntl) {
+    const int ntiles = ntl < 0 ? (N + 15) / 16 - t0 : ntl;
+    const int t1 = t0 + ntiles;
+    const int nblocks = (ntiles + 7) / 8 * 8 * contract_groups(dp.nmax);
     YH_REQUIRE(q_fp32 || !io_fp32, "complex64 vectors need the complex64 pair GEMM");
     if (io_fp32)
         hipLaunchKernelGGL((contract_kernel<float, float, float>), dim3(nblocks), dim3(256), 0, s,
                            static_cast<const float *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N, dp.nmax,
-                           static_cast<const float *>(x), t, mode, static_cast<float *>(y));
+                           static_cast<const float *>(x), t, mode, static_cast<float *>(y), t0, t1);
     else if (q_fp32)
         hipLaunchKernelGGL((contract_kernel<float, double, double>), dim3(nblocks), dim3(256), 0, s,
                            static_cast<const float *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N, dp.nmax,
-                           static_cast<const double *>(x), t, mode, static_cast<double *>(y));
+                           static_cast<const double *>(x), t, mode, static_cast<double *>(y), t0, t1);
     else
         hipLaunchKernelGGL((contract_kernel<double, double, double>), dim3(nblocks), dim3(256), 0, s,
                            static_cast<const double *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N,
-                           dp.nmax, static_cast<const double *>(x), t, mode, static_cast<double *>(y));
+                           dp.nmax, static_cast<const double *>(x), t, mode, static_cast<double *>(y), t0, t1);
     YH_CHECK(hipGetLastError());
 }
 

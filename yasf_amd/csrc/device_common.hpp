@@ -267,13 +267,14 @@ __device__ __forceinline__ void contract_tile(const TQ *__restrict__ Q, int nq, 
                                               const int *__restrict__ cidx, const double2 *__restrict__ cval,
                                               int N, int nmax, const TX *__restrict__ x,
                                               const double2 *__restrict__ t, int mode, TY *__restrict__ y,
-                                              int s1t, int group) {
+                                              int s1t, int group, int qt0 = 0) {
+    // Q holds the target tiles from qt0 on (qt0 = 0: all of them)
     const int r = threadIdx.x & 15;
     const int s1 = s1t * 16 + r;
     const int n1 = group * kContractModes + (threadIdx.x >> 4);
     if (n1 >= nmax)
         return;
-    const size_t qrow = static_cast<size_t>(s1t) * nq * 16;
+    const size_t qrow = static_cast<size_t>(s1t - qt0) * nq * 16;
     double sr = 0.0, si = 0.0;
     const int e1 = crow[n1 + 1];
     // batches of kBatch entries: all their index, coefficient and Q loads are

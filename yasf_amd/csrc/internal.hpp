@@ -137,14 +137,19 @@ void free_device_plan(DevicePlan &dp);
 // Pairs farther apart than sqrt(near2) are left out of W (the near-field
 // approximation); kAllPairs keeps every pair.
 constexpr double kAllPairs = HUGE_VAL;
+// t0, ntl: only the target tiles [t0, t0 + ntl) (ntl < 0: to the last),
+// written to Q from its start.
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
-                 void *Q, hipStream_t s, const int *gate = nullptr, double near2 = kAllPairs);
+                 void *Q, hipStream_t s, const int *gate = nullptr, double near2 = kAllPairs, int t0 = 0,
+                 int ntl = -1);
 // Q of the regular (J-translation) operator, coincident pairs included, FP64.
 void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
                          void *Q, hipStream_t s);
 // ab5 contraction (FP64 accumulation) + optional master-operator epilogue.
+// t0, ntl: only the target tiles [t0, t0 + ntl) (ntl < 0: to the last),
+// read from Q from its start.
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
-                     int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s);
+                     int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s, int t0 = 0, int ntl = -1);
 // Resident pair-factor table: bytes, build, and the streaming pair GEMM.
 // Both work on target tiles [t0, t0 + ntl) (ntl < 0: to the last tile) of a
 // table whose pointer is at tile t0, so a slab of a few tiles can stand in
