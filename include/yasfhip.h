@@ -159,12 +159,27 @@ int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b
  *   c_sca = |sum_{j1,j2} conj(f_j1) W_J[j1,j2] f_j2| * pi / k^2,
  * W_J being the coupling operator with spherical Bessel j_p in place of h_p and
  * the coincident pairs s1 = s2 included (the CPU sum of
- * yasfpy/functions/cpu_numba.py:410-457), evaluated in FP64: over slabs of a
- * regular pair table in the session's slab buffer where its matvecs stream
- * slabs (SLAB mode, or AUTO when the table did not fit), otherwise with the
- * factors generated on the fly. */
+ * yasfpy/functions/cpu_numba.py:410-457), evaluated in FP64.  By default
+ * C_sca comes from the equal far-field integral (yasfhip_set_scattering_method);
+ * the pair sum runs over slabs of a regular pair table in the session's slab
+ * buffer where its matvecs stream slabs (SLAB mode, or AUTO when the table
+ * did not fit), otherwise with the factors generated on the fly. */
 int yasfhip_cross_sections(yasfhip_session *s, int channel, const double *a_inc, const double *f,
                            double *c_ext, double *c_sca);
+
+/* How yasfhip_cross_sections (and a server's completions) compute C_sca:
+ * YASFHIP_CSCA_FAR_FIELD (default): C_sca k^2 = integral of |E|^2 over the
+ * directions, by the product of an n_theta-point Gauss-Legendre rule in
+ * cos(theta) and an n_phi-point rule in phi with n_theta = M + 1, n_phi = 2M + 2,
+ * M = ceil(kR + lmax + 15^(2/3) (kR + 1)^(1/3)), R the largest distance of a
+ * sphere from the origin: exact for the band-limited far field, it equals
+ * the pair sum to rounding at O(directions x N x nmax) work instead of
+ * O(N^2 nmax^2) (the far field of compute_electric_field_angle_components,
+ * cpu_numba.py:535-606; the integral identity is the reference's own check,
+ * tests/test_compare_mstm4.py:37-47).  YASFHIP_CSCA_PAIR_SUM: the regular
+ * (J) pair sum of cpu_numba.py:410-457 as described above. */
+enum { YASFHIP_CSCA_FAR_FIELD = 0, YASFHIP_CSCA_PAIR_SUM = 1 };
+int yasfhip_set_scattering_method(yasfhip_session *s, int method);
 
 /* Far-field angle components of the scattered electric field for
  * num_directions directions (polar[a], azimuthal[a]) in radians
@@ -357,6 +372,10 @@ int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, in
  * slab_tiles > 0 fixes the FP64 16-particle target tiles per slab (complex64
  * GEMMs use twice as many), 0 sizes slabs to the arena. */
 int yasfhip_server_set_table_policy(yasfhip_server *s, int policy, int slab_tiles);
+
+/* C_sca method of the completions from now on (YASFHIP_CSCA_FAR_FIELD, the
+ * default, or YASFHIP_CSCA_PAIR_SUM; see yasfhip_set_scattering_method). */
+int yasfhip_server_set_scattering_method(yasfhip_server *s, int method);
 
 /* Fixes the tile shape used for lmax, both size classes (one of the
  * candidates above). */

@@ -30,11 +30,18 @@ def _coefficients(seed, N, lmax, lam, n_index):
     return pos, k, t, a, f
 
 
+METHODS = [(_native.CSCA_FAR_FIELD, "far_field"), (_native.CSCA_PAIR_SUM, "pair_sum")]
+
+
+@pytest.mark.parametrize("method", [m for m, _ in METHODS], ids=[i for _, i in METHODS])
 @pytest.mark.parametrize("lmax", [1, 3, 4, 8])
-def test_cross_sections_match_oracle(lmax):
+def test_cross_sections_match_oracle(lmax, method):
+    # C_sca by far-field quadrature (default) and by the reference's regular
+    # pair sum: both within 1e-10 of the oracle's pair sum
     N = 21 if lmax < 8 else 9
     pos, k, t, a, f = _coefficients(30 + lmax, N, lmax, 520.0, 1.5 + 0.02j)
     s = _native.Session(pos, lmax, [k])
+    s.set_scattering_method(method)
     c_ext, c_sca = s.cross_sections(a, f)
     r_ext, r_sca = R.cross_sections(lmax, pos, k, a, f)
     assert c_ext == pytest.approx(r_ext, rel=1e-10)
@@ -53,6 +60,7 @@ def test_cross_sections_every_matvec_mode(mode, slab_tiles):
     lmax = 6
     pos, k, t, a, f = _coefficients(77, 37, lmax, 560.0, 1.6 + 0.05j)
     s = _native.Session(pos, lmax, [k])
+    s.set_scattering_method(_native.CSCA_PAIR_SUM)
     s.set_matvec_mode(mode)
     if slab_tiles:
         s.set_slab_tiles(slab_tiles)
@@ -63,20 +71,24 @@ def test_cross_sections_every_matvec_mode(mode, slab_tiles):
     npt.assert_allclose(s.cross_sections(a, f), got, rtol=1e-13)
 
 
-def test_cross_sections_single_sphere_and_far_pairs():
+@pytest.mark.parametrize("method", [m for m, _ in METHODS], ids=[i for _, i in METHODS])
+def test_cross_sections_single_sphere_and_far_pairs(method):
     # one sphere: only the coincident (self) term of the J-translation sum
     pos = np.zeros((1, 3))
     lmax, lam, k = 6, 600.0, 2 * np.pi / 600.0
     t = R.mie_coefficients(lmax, np.array([200.0]), np.array([1.7 + 0.1j]), np.array([lam]), 1.0)[0, :, 0]
     a = R.planewave_coefficients(lmax, pos, np.array([k]), 0.0, 0.0, 1)[0, :, 0]
     s = _native.Session(pos, lmax, [k])
+    s.set_scattering_method(method)
     got = s.cross_sections(a, t * a)
     ref = R.cross_sections(lmax, pos, k, a, t * a)
     npt.assert_allclose(got, ref, rtol=1e-12)
-    # two spheres far apart (k d ~ 60): the upward j_p recurrence regime
+    # two spheres far apart (k d ~ 60): the upward j_p recurrence regime of
+    # the pair sum; a wide band (k R ~ 55) for the quadrature
     pos2 = np.array([[0.0, 0.0, 0.0], [1500.0, 3000.0, -4000.0]])
     f2 = np.tile(t * a, 2)
     s2 = _native.Session(pos2, lmax, [k])
+    s2.set_scattering_method(method)
     npt.assert_allclose(s2.cross_sections(np.tile(a, 2), f2),
                         R.cross_sections(lmax, pos2, k, np.tile(a, 2), f2), rtol=1e-11)
 

@@ -267,3 +267,18 @@ def test_slab_operators_match_tables(aggregate_requests, mixed_requests, served)
         assert np.array_equal(slab.results[e.index]["x"], served.results[r.index]["x"])
     with pytest.raises(ValueError):
         _native.Server(memory_bytes=1 << 30).set_table_policy(3)
+
+
+def test_scattering_methods_agree(mixed_requests, served):
+    # completions' C_sca by far-field quadrature (default) and by the
+    # regular pair sum agree to rounding; solutions are unaffected
+    srv = _native.Server(memory_bytes=4 << 30, restart=50, inner_rtol=1e-4)
+    srv.set_scattering_method(_native.CSCA_PAIR_SUM)
+    pair = serve(srv, mixed_requests[:5], keep_solutions=True)
+    srv.close()
+    for r in mixed_requests[:5]:
+        a, b = served.results[r.index], pair.results[r.index]
+        assert np.array_equal(a["x"], b["x"]) and a["c_ext"] == b["c_ext"]
+        assert b["c_sca"] == pytest.approx(a["c_sca"], rel=1e-10)
+    with pytest.raises(ValueError):
+        _native.Server(memory_bytes=1 << 30).set_scattering_method(2)

@@ -23,6 +23,9 @@ MATVEC_ON_THE_FLY = 1
 MATVEC_TABLE = 2
 MATVEC_SLAB = 3
 
+CSCA_FAR_FIELD = 0  # C_sca by quadrature of the far field (default)
+CSCA_PAIR_SUM = 1   # C_sca by the regular (J) pair sum of cpu_numba.py:410-457
+
 _lib = None
 _lock = threading.Lock()
 
@@ -114,6 +117,8 @@ def _load():
             "yasfhip_set_matvec_mode": ([P, i], i),
             "yasfhip_tune_matvec": ([P, i, i, i, ip, P], i),
             "yasfhip_coupling_matvec_noalloc": ([P, i, i, d, P, P, P], i),
+            "yasfhip_set_scattering_method": ([P, i], i),
+            "yasfhip_server_set_scattering_method": ([P, i], i),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -140,6 +145,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_set_slab_tiles", "yasfhip_set_near_field_radius", "yasfhip_mie_coefficients",
         "yasfhip_planewave_coefficients", "yasfhip_server_submit_spheres", "yasfhip_set_table_shape",
         "yasfhip_set_precond_factor_precision", "yasfhip_coupling_matvec_noalloc",
+        "yasfhip_set_scattering_method", "yasfhip_server_set_scattering_method",
     ]
 
 
@@ -367,6 +373,10 @@ class Session:
         _check(self._lib.yasfhip_precond_apply(self._h, _ptr(r), _ptr(z)))
         return z
 
+    def set_scattering_method(self, method: int):
+        """CSCA_FAR_FIELD (default) or CSCA_PAIR_SUM for cross_sections."""
+        _check(self._lib.yasfhip_set_scattering_method(self._h, int(method)))
+
     def set_matvec_mode(self, mode: int):
         """MATVEC_AUTO, MATVEC_ON_THE_FLY, MATVEC_TABLE (resident pair-factor
         table) or MATVEC_SLAB (the table built and streamed a slab of target
@@ -520,6 +530,10 @@ class Server:
         _check(self._lib.yasfhip_server_set_tile_shape(self._h, int(lmax), int(waves), int(column_tiles)))
 
     TABLES_ON_THE_FLY, TABLES_AUTO, TABLES_SLABS = 0, 1, 2
+
+    def set_scattering_method(self, method: int):
+        """CSCA_FAR_FIELD (default) or CSCA_PAIR_SUM for the completions' C_sca."""
+        _check(self._lib.yasfhip_server_set_scattering_method(self._h, int(method)))
 
     def set_table_policy(self, policy: int, slab_tiles: int = 0):
         """TABLES_AUTO (default): resident pair tables where they fit, else

@@ -40,6 +40,10 @@ void launch_fill_test(double *dst, long count, hipStream_t s);  // timing input 
 void launch_far_field(const double4 *pos, const double2 *f, int N, int lmax, double k,
                       const double4 *dir, const double *polar, const double *azimuthal, int A,
                       double2 *F, double2 *e_theta, double2 *e_phi, hipStream_t s);
+void csca_quadrature_grid(double kR, int lmax, int *n_theta, int *n_phi);
+size_t csca_quadrature_partials(int n_theta, int n_phi);
+void launch_csca_quadrature(const double4 *pos, const double2 *f, int N, int lmax, double k, int n_theta,
+                            int n_phi, double *partials, double *out, hipStream_t s);
 // precond.hip
 struct BlockPrecond;
 BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &ab5,
@@ -125,6 +129,7 @@ struct yasfhip_session {
     BlockPrecond *precond = nullptr;
     int precond_channel = -1;
     bool precond_fp32 = true;  // LU factors stored in complex64 for the apply (FP64 arithmetic)
+    int csca_method = YASFHIP_CSCA_FAR_FIELD;  // how yasfhip_cross_sections computes C_sca
     // Resident pair-factor tables, one per precision (0 = FP64, 1 = FP32).
     int matvec_mode = YASFHIP_MATVEC_AUTO;
     void *table[2] = {nullptr, nullptr};
@@ -796,16 +801,40 @@ int yasfhip_cross_sections(yasfhip_session *s, int channel, const double *a_inc,
         hipStream_t st = s->stream;
         YH_CHECK(hipMemcpyAsync(s->xin, f, n * sizeof(double2), hipMemcpyHostToDevice, st));
         YH_CHECK(hipMemcpyAsync(s->avec, a_inc, n * sizeof(double2), hipMemcpyHostToDevice, st));
-        // y = W_J f (regular translation, coincident pairs included), then
-        // <f, y> and <a, f> (conj on the left operand)
-        s->pair_gemm_regular(channel, s->xin);
-        launch_contract(s->dp, s->N, s->Q, s->xin, s->t, 0, s->yout, false, false, st);
-        launch_mgs_step(s->yout, nullptr, nullptr, s->xin, n, s->reducer(), s->scal, st);
-        launch_mgs_step(s->xin, nullptr, nullptr, s->avec, n, s->reducer(), s->scal + 4, st);
-        s->fetch_scalars(8);
+        if (s->csca_method == YASFHIP_CSCA_PAIR_SUM) {
+            // y = W_J f (regular translation, coincident pairs included), then
+            // <f, y> and <a, f> (conj on the left operand)
+            s->pair_gemm_regular(channel, s->xin);
+            launch_contract(s->dp, s->N, s->Q, s->xin, s->t, 0, s->yout, false, false, st);
+            launch_mgs_step(s->yout, nullptr, nullptr, s->xin, n, s->reducer(), s->scal, st);
+            launch_mgs_step(s->xin, nullptr, nullptr, s->avec, n, s->reducer(), s->scal + 4, st);
+            s->fetch_scalars(8);
+        } else {
+            // C_sca by quadrature of the far field, C_ext from <a, f>
+            double r2 = 0.0;
+            for (int i = 0; i < s->N; ++i) {
+                const double *p = &s->pos_host[3 * static_cast<size_t>(i)];
+                r2 = std::max(r2, p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+            }
+            int nth = 0, nph = 0;
+            csca_quadrature_grid(s->k[channel] * std::sqrt(r2), s->L, &nth, &nph);
+            DeviceBuffer<double> part(csca_quadrature_partials(nth, nph));
+            launch_csca_quadrature(s->pos, s->xin, s->N, s->L, s->k[channel], nth, nph, part.get(), s->scal, st);
+            launch_mgs_step(s->xin, nullptr, nullptr, s->avec, n, s->reducer(), s->scal + 4, st);
+            s->fetch_scalars(8);
+        }
         const double k2 = s->k[channel] * s->k[channel];
         *c_ext = -s->scal_host[4] / k2 * M_PI;
         *c_sca = std::hypot(s->scal_host[0], s->scal_host[1]) / k2 * M_PI;
+    });
+}
+
+int yasfhip_set_scattering_method(yasfhip_session *s, int method) {
+    return guarded([&] {
+        YH_REQUIRE(s, "session is NULL");
+        YH_REQUIRE(method == YASFHIP_CSCA_FAR_FIELD || method == YASFHIP_CSCA_PAIR_SUM,
+                   "method must be YASFHIP_CSCA_FAR_FIELD or YASFHIP_CSCA_PAIR_SUM");
+        s->csca_method = method;
     });
 }
 

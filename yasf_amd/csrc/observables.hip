@@ -124,21 +124,13 @@ __device__ __forceinline__ void far_terms(const double2 *__restrict__ Fa, int lm
     }
 }
 
-// One thread per direction; pi_l^m, tau_l^m are generated per m by the
-// recurrences of spherical_functions_trigon.py:39-93 while they are consumed.
-__global__ __launch_bounds__(256) void far_field_assemble_kernel(const double2 *__restrict__ F, int A,
-                                                                 int lmax, int nmax,
-                                                                 const double *__restrict__ polar,
-                                                                 const double *__restrict__ azimuthal,
-                                                                 double2 *__restrict__ e_theta,
-                                                                 double2 *__restrict__ e_phi) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= A)
-        return;
-    const double th = polar[a], ph = azimuthal[a];
-    const double ct = cos(th), st = sin(th);
-    const double2 *Fa = F + static_cast<size_t>(a) * nmax;
-    double2 eth = make_double2(0.0, 0.0), eph = make_double2(0.0, 0.0);
+// E_theta, E_phi of one direction (cos th, sin th, ph) from that direction's
+// row Fa of F; pi_l^m, tau_l^m are generated per m by the recurrences of
+// spherical_functions_trigon.py:39-93 while they are consumed.
+__device__ __forceinline__ void assemble_direction(const double2 *Fa, int lmax, double ct, double st, double ph,
+                                                   double2 &eth, double2 &eph) {
+    eth = make_double2(0.0, 0.0);
+    eph = make_double2(0.0, 0.0);
     // m = 0: pi = 0, tau_l^0 = -sin th P'_l with P' by its own recurrence.
     {
         double p_prev = sqrt(0.5), p_cur = sqrt(1.5) * ct;  // P_0^0, P_1^0
@@ -173,11 +165,192 @@ __global__ __launch_bounds__(256) void far_field_assemble_kernel(const double2 *
             pi_cur = pi_next;
         }
     }
+}
+
+// One thread per direction.
+__global__ __launch_bounds__(256) void far_field_assemble_kernel(const double2 *__restrict__ F, int A,
+                                                                 int lmax, int nmax,
+                                                                 const double *__restrict__ polar,
+                                                                 const double *__restrict__ azimuthal,
+                                                                 double2 *__restrict__ e_theta,
+                                                                 double2 *__restrict__ e_phi) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A)
+        return;
+    const double th = polar[a], ph = azimuthal[a];
+    double2 eth, eph;
+    assemble_direction(F + static_cast<size_t>(a) * nmax, lmax, cos(th), sin(th), ph, eth, eph);
     e_theta[a] = eth;
     e_phi[a] = eph;
 }
 
+// Node x_i and weight w_i of the n-point Gauss-Legendre rule on [-1, 1]
+// (Newton's iteration on P_n from x = cos(pi (i + 3/4) / (n + 1/2))).
+__device__ inline void gauss_legendre_node(int i, int n, double &x, double &w) {
+    x = cos(M_PI * (i + 0.75) / (n + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+        double p0 = 1.0, p1 = x;
+        for (int j = 2; j <= n; ++j) {
+            const double p2 = ((2.0 * j - 1.0) * x * p1 - (j - 1.0) * p0) / j;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (x * p1 - p0) / (x * x - 1.0);
+        const double dx = p1 / dp;
+        x -= dx;
+        if (fabs(dx) <= 1e-16)
+            break;
+    }
+    // P_n'(x) at the converged node
+    double p0 = 1.0, p1 = x;
+    for (int j = 2; j <= n; ++j) {
+        const double p2 = ((2.0 * j - 1.0) * x * p1 - (j - 1.0) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+    }
+    dp = n * (x * p1 - p0) / (x * x - 1.0);
+    w = 2.0 / ((1.0 - x * x) * dp * dp);
+}
+
+// Scattering cross section by quadrature of the far field:
+//   C_sca k^2 = int |E|^2 dOmega = sum_{i,j} w_i (2 pi / n_phi) |E(x_i, phi_j)|^2,
+// Gauss-Legendre in cos th (n_theta nodes) and the trapezoid rule in phi
+// (n_phi points), exact for the band-limited far field of the cluster.  One
+// block (4 waves) per 16 directions: the F rows of its directions (the far
+// field GEMM of far_field_gemm_kernel, all modes) go to LDS, then 16 threads
+// assemble E and the block leaves the weighted sum of its 16 directions (in
+// direction order) in partials[block].
+constexpr int kQuadColTiles = (2 * kMaxLmax * (kMaxLmax + 2) + 15) / 16;  // mode tiles at lmax 8
+__global__ __launch_bounds__(256) void csca_quadrature_kernel(const double4 *__restrict__ pos,
+                                                              const double2 *__restrict__ f, int N, int lmax,
+                                                              int nmax, double k, int n_theta, int n_phi,
+                                                              double *__restrict__ partials) {
+    typedef Mfma<double>::acc_t acc_t;
+    constexpr int kPerWave = (kQuadColTiles + 3) / 4;
+    __shared__ double2 Fs[16][kQuadColTiles * 16];
+    __shared__ double Is[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lo = lane & 15, hi = lane >> 4;
+    const int A = n_theta * n_phi;
+    const int a = blockIdx.x * 16 + lo;
+    const int ith = (a < A ? a : 0) / n_phi, jph = (a < A ? a : 0) - ith * n_phi;
+    double x, w;
+    gauss_legendre_node(ith, n_theta, x, w);
+    const double st = sqrt(fmax(0.0, 1.0 - x * x));
+    const double ph = 2.0 * M_PI * jph / n_phi;
+    double sph, cph;
+    sincos(ph, &sph, &cph);
+    const double ex = st * cph, ey = st * sph, ez = x;
+    const int ntiles = (nmax + 15) / 16;
+    acc_t acc[kPerWave][2];
+#pragma unroll
+    for (int t = 0; t < kPerWave; ++t) {
+        acc[t][0] = acc_t{0, 0, 0, 0};
+        acc[t][1] = acc_t{0, 0, 0, 0};
+    }
+    for (int k0 = 0; k0 < N; k0 += 4) {
+        const int s = k0 + hi;
+        double ar = 0.0, ai = 0.0;
+        if (s < N) {
+            const double4 r = pos[s];
+            sincos(-k * (r.x * ex + r.y * ey + r.z * ez), &ai, &ar);
+        }
+#pragma unroll
+        for (int t = 0; t < kPerWave; ++t) {
+            const int ct = wave + 4 * t;
+            if (ct < ntiles) {
+                const int n = ct * 16 + lo;
+                double2 b = make_double2(0.0, 0.0);
+                if (s < N && n < nmax)
+                    b = f[static_cast<size_t>(s) * nmax + n];
+                acc[t][0] = Mfma<double>::mac(ar, b.x, acc[t][0]);
+                acc[t][0] = Mfma<double>::mac(-ai, b.y, acc[t][0]);
+                acc[t][1] = Mfma<double>::mac(ar, b.y, acc[t][1]);
+                acc[t][1] = Mfma<double>::mac(ai, b.x, acc[t][1]);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kPerWave; ++t) {
+        const int ct = wave + 4 * t;
+        if (ct < ntiles) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                Fs[Mfma<double>::row(i, hi)][ct * 16 + lo] = make_double2(acc[t][0][i], acc[t][1][i]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        const int r = threadIdx.x;  // direction r of the tile: its own node (lanes 0..15 hold them)
+        double v = 0.0;
+        if (a < A) {
+            double2 eth, eph;
+            assemble_direction(Fs[r], lmax, x, st, ph, eth, eph);
+            v = w * (2.0 * M_PI / n_phi) * (eth.x * eth.x + eth.y * eth.y + eph.x * eph.x + eph.y * eph.y);
+        }
+        Is[r] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int r = 0; r < 16; ++r)
+            sum += Is[r];
+        partials[blockIdx.x] = sum;
+    }
+}
+
+// out[0] = scale * sum of count partials, summed in a fixed order (strided
+// per thread, then a fixed tree), out[1] = 0.
+__global__ __launch_bounds__(256) void sum_partials_kernel(const double *__restrict__ partials, int count,
+                                                           double scale, double *__restrict__ out) {
+    __shared__ double red[256];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < count; i += 256)
+        v += partials[i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = scale * red[0];
+        out[1] = 0.0;
+    }
+}
+
 }  // namespace
+
+// Quadrature order M for the far field of spheres within radius R of the
+// origin (k R) at expansion order lmax: n_theta = M + 1 Gauss-Legendre nodes
+// and n_phi = 2 M + 2 azimuths integrate |E|^2 (degree <= 2 M in the
+// direction) exactly; M = k R + lmax + 15^(2/3) (k R + 1)^(1/3) leaves the
+// plane-wave expansion's truncation error below 1e-15 relative.
+void csca_quadrature_grid(double kR, int lmax, int *n_theta, int *n_phi) {
+    const int M = static_cast<int>(std::ceil(kR + lmax + std::pow(15.0, 2.0 / 3.0) * std::cbrt(kR + 1.0)));
+    *n_theta = M + 1;
+    *n_phi = 2 * M + 2;
+}
+
+size_t csca_quadrature_partials(int n_theta, int n_phi) {
+    return (static_cast<size_t>(n_theta) * n_phi + 15) / 16;
+}
+
+// out[0] = C_sca k^2 / pi (so that the pair-sum's |out[0] + i out[1]| pi / k^2
+// convention gives C_sca), out[1] = 0.  partials: csca_quadrature_partials
+// doubles of device scratch.
+void launch_csca_quadrature(const double4 *pos, const double2 *f, int N, int lmax, double k, int n_theta,
+                            int n_phi, double *partials, double *out, hipStream_t s) {
+    const int nmax = 2 * lmax * (lmax + 2);
+    const int blocks = static_cast<int>(csca_quadrature_partials(n_theta, n_phi));
+    hipLaunchKernelGGL(csca_quadrature_kernel, dim3(blocks), dim3(256), 0, s, pos, f, N, lmax, nmax, k, n_theta,
+                       n_phi, partials);
+    YH_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, partials, blocks, 1.0 / M_PI, out);
+    YH_CHECK(hipGetLastError());
+}
 
 void launch_far_field(const double4 *pos, const double2 *f, int N, int lmax, double k,
                       const double4 *dir, const double *polar, const double *azimuthal, int A,

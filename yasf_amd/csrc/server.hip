@@ -52,6 +52,12 @@ void launch_residual(double2 *r, const double2 *b, const double2 *ax, long n, Re
 void launch_norm2(const double2 *x, long n, Reducer red, double *out, hipStream_t s);
 void launch_mgs_step(double2 *w, const double2 *v_prev, const double2 *h_prev, const double2 *v_next,
                      long n, Reducer red, double *out, hipStream_t s);
+void launch_to_fp32(float *dst, const double *src, long count, hipStream_t s);
+// observables.hip: C_sca by quadrature of the far field
+void csca_quadrature_grid(double kR, int lmax, int *n_theta, int *n_phi);
+size_t csca_quadrature_partials(int n_theta, int n_phi);
+void launch_csca_quadrature(const double4 *pos, const double2 *f, int N, int lmax, double k, int n_theta,
+                            int n_phi, double *partials, double *out, hipStream_t s);
 
 namespace {
 
@@ -156,6 +162,9 @@ __global__ __launch_bounds__(kVecThreads) void serve_prep_kernel(const BatchReq 
 // Table GEMM shape of the server's FP64 products (residuals, C_sca): 8 waves
 // with jobs of up to 4 column tiles, 20% faster than jobs of 2 at lmax 4 and 6.
 constexpr int kServeShape64 = 1;
+// Table GEMM shape of a request's single complex64 product (the residual of
+// x0 = b): 8 waves with jobs of up to 8 column tiles, the session default.
+constexpr int kServeShape32 = 2;
 
 // Orders per wave the serving GEMM instances hold: whole orders of blocks of
 // 4 and 8 waves at lmax 8, and of 2-wave blocks up to lmax 4.
@@ -590,6 +599,7 @@ struct SharedOp {
     // complex64 tiles), rebuilt range by range in every pair GEMM
     void *slab = nullptr;
     int slab_tiles64 = 0;
+    double kR = 0.0;  // k times the largest distance of a sphere from the origin (far-field quadrature)
 };
 
 // The active requests of one operator (and tile shape) whose GEMMs run as one
@@ -628,6 +638,7 @@ struct Request {
     // progress
     Phase phase = kQueued;
     bool cycle_ended = false, first_refresh = true;
+    bool approx_residual = false;  // the last refresh's residual came from the complex64 operator
     int cycles = 0, iterations = 0, shape = 2, slot = -1;
     double bnorm = 0.0, rnorm = 0.0;
     double t_submit = 0.0, t_done = 0.0;
@@ -677,6 +688,7 @@ struct yasfhip_server {
     // factors on the fly only
     int table_policy = 1;
     int slab_tiles_req = 0;  // FP64 target tiles per slab (0: an eighth of the arena)
+    int csca_method = YASFHIP_CSCA_FAR_FIELD;  // how completions compute C_sca
     std::deque<Request *> finished;
     long long completed = 0;
     // batched launch data
@@ -833,6 +845,10 @@ struct yasfhip_server {
         o->n = q.n;
         o->k = q.k;
         o->device_setup = q.device_setup;
+        double r2 = 0.0;
+        for (const double4 &p : q.pos)
+            r2 = std::max(r2, p.x * p.x + p.y * p.y + p.z * p.z);
+        o->kR = q.k * std::sqrt(r2);
         const size_t cap = arena.capacity();
         const bool idle = active.empty();
         const DevicePlan &dpl = plan_for(q.L).dp;
@@ -1023,9 +1039,44 @@ struct yasfhip_server {
                 launch_update(q->x_d, q->V, q->y_d, m, q->n, q->n, st);
             }
         }
+        // The residual of x0 = b only seeds the first cycle (convergence is
+        // decided on FP64 residuals alone), so where the operator keeps a
+        // complex64 table it is formed with the complex64 pair GEMM
+        std::vector<char> gemm_done(todo.size(), 0);
+        for (auto *q : todo) {
+            q->approx_residual = q->first_refresh;
+            if (q->approx_residual)
+                launch_to_fp32(reinterpret_cast<float *>(q->x32), reinterpret_cast<const double *>(q->x_d), 2 * q->n,
+                               st);
+        }
+        for (size_t i = 0; i < todo.size(); ++i) {
+            Request *q = todo[i];
+            if (!q->approx_residual || gemm_done[i])
+                continue;
+            const SharedOp &o = *q->op;
+            const DevicePlan &dp = plan_for(q->L).dp;
+            if (o.table32) {
+                launch_pair_table_gemm(dp, q->N, o.table, q->x32, true, kServeShape32, q->Q, st);
+            } else if (o.slab) {  // complex64 slabs (twice the FP64 tiles), one build for the operator's requests
+                const int tiles = 2 * o.slab_tiles64;
+                for (int t0 = 0; t0 < o.ntiles1; t0 += tiles) {
+                    const int ntl = std::min(tiles, o.ntiles1 - t0);
+                    launch_pair_table_build(dp, o.pos_d, o.N, o.k, true, o.slab, st, t0, ntl);
+                    for (size_t j = i; j < todo.size(); ++j)
+                        if (todo[j]->op == q->op && todo[j]->approx_residual)
+                            launch_pair_table_gemm(dp, q->N, o.slab, todo[j]->x32, true, kServeShape32, todo[j]->Q,
+                                                   st, t0, ntl);
+                }
+                for (size_t j = i; j < todo.size(); ++j)
+                    if (todo[j]->op == q->op && todo[j]->approx_residual)
+                        gemm_done[j] = 1;
+            } else {
+                launch_pair(dp, o.pos_d, q->N, q->k, q->x32, true, q->Q, st);
+            }
+            gemm_done[i] = 1;
+        }
         // FP64 pair GEMMs of A x: resident FP64 table, slab (one build per
         // slab for all of an operator's requests here), or on the fly
-        std::vector<char> gemm_done(todo.size(), 0);
         for (size_t i = 0; i < todo.size(); ++i) {
             Request *q = todo[i];
             const SharedOp &o = *q->op;
@@ -1039,12 +1090,12 @@ struct yasfhip_server {
                     const int ntl = std::min(o.slab_tiles64, o.ntiles1 - t0);
                     launch_pair_table_build(dp, o.pos_d, o.N, o.k, false, o.slab, st, t0, ntl);
                     for (size_t j = i; j < todo.size(); ++j)
-                        if (todo[j]->op == q->op)
+                        if (todo[j]->op == q->op && !todo[j]->approx_residual)
                             launch_pair_table_gemm(dp, q->N, o.slab, todo[j]->x_d, false, kServeShape64, todo[j]->Q,
                                                    st, t0, ntl);
                 }
                 for (size_t j = i; j < todo.size(); ++j)
-                    if (todo[j]->op == q->op)
+                    if (todo[j]->op == q->op && !todo[j]->approx_residual)
                         gemm_done[j] = 1;
             } else {
                 launch_pair(dp, o.pos_d, q->N, q->k, q->x_d, false, q->Q, st);
@@ -1056,7 +1107,7 @@ struct yasfhip_server {
             // r = b - A x (FP64) into V[0]; V[1] holds A x
             double2 *ax = q->V + q->n;
             const SharedOp &o = *q->op;
-            launch_contract(dp, q->N, q->Q, q->x_d, o.t_d, 1, ax, false, false, st);
+            launch_contract(dp, q->N, q->Q, q->x_d, o.t_d, 1, ax, q->approx_residual, false, st);
             launch_residual(q->V, q->b_d, ax, q->n, reducer(), scal + 8 * q->slot, st);
             if (q->first_refresh)
                 launch_norm2(q->b_d, q->n, reducer(), scal + 8 * q->slot + 1, st);
@@ -1076,7 +1127,7 @@ struct yasfhip_server {
                 ++q->cycles;
                 q->cycle_ended = false;
             }
-            const bool converged = q->rnorm <= q->rtol * q->bnorm;
+            const bool converged = !q->approx_residual && q->rnorm <= q->rtol * q->bnorm;
             if (converged || q->cycles >= q->max_cycles || q->bnorm == 0.0) {
                 q->info = converged || q->bnorm == 0.0 ? 0 : 1;
                 done.push_back(q);
@@ -1101,11 +1152,27 @@ struct yasfhip_server {
     // Observables, results to the host and release of the arena blocks.
     void complete(std::vector<Request *> &done) {
         hipStream_t st = stream;
-        // regular (J) pair GEMMs of C_sca: over slabs of the regular table
-        // (built once per slab for an operator's completing requests), else
-        // on the fly
+        // C_sca by quadrature of the far field (the converged request's
+        // Krylov basis is free and holds the per-block partial sums), or by
+        // the regular (J) pair sum: pair GEMMs over slabs of the regular
+        // table (built once per slab for an operator's completing requests),
+        // else on the fly
         std::vector<char> regular_done(done.size(), 0);
-        for (size_t i = 0; i < done.size(); ++i) {
+        if (csca_method == YASFHIP_CSCA_FAR_FIELD) {
+            for (auto *q : done) {
+                if (!q->observables)
+                    continue;
+                int nth = 0, nph = 0;
+                csca_quadrature_grid(q->op->kR, q->L, &nth, &nph);
+                YH_REQUIRE(csca_quadrature_partials(nth, nph) * sizeof(double) <=
+                               static_cast<size_t>(restart + 1) * q->n * sizeof(double2),
+                           "far-field quadrature partials exceed the request's Krylov basis");
+                launch_csca_quadrature(q->op->pos_d, q->x_d, q->N, q->L, q->k, nth, nph,
+                                       reinterpret_cast<double *>(q->V), scal + 8 * q->slot + 2, st);
+                launch_mgs_step(q->x_d, nullptr, nullptr, q->a_d, q->n, reducer(), scal + 8 * q->slot + 5, st);
+            }
+        }
+        for (size_t i = 0; i < done.size() && csca_method == YASFHIP_CSCA_PAIR_SUM; ++i) {
             Request *q = done[i];
             if (!q->observables || regular_done[i])
                 continue;
@@ -1129,7 +1196,7 @@ struct yasfhip_server {
             regular_done[i] = 1;
         }
         for (auto *q : done) {
-            if (!q->observables)
+            if (!q->observables || csca_method != YASFHIP_CSCA_PAIR_SUM)
                 continue;
             const DevicePlan &dp = plan_for(q->L).dp;
             double2 *y = q->V + q->n;
@@ -1781,6 +1848,15 @@ int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int co
                    "unsupported tile shape for this lmax (waves 2/4/8 x column tiles 2/4/8 as listed in "
                    "yasfhip.h, with ceil((2 lmax + 1) / waves) orders per wave within range)");
         s->shape_for_lmax[lmax][0] = s->shape_for_lmax[lmax][1] = found;
+    });
+}
+
+int yasfhip_server_set_scattering_method(yasfhip_server *s, int method) {
+    return sguard([&] {
+        YH_REQUIRE(s, "server is NULL");
+        YH_REQUIRE(method == YASFHIP_CSCA_FAR_FIELD || method == YASFHIP_CSCA_PAIR_SUM,
+                   "method must be YASFHIP_CSCA_FAR_FIELD or YASFHIP_CSCA_PAIR_SUM");
+        s->csca_method = method;
     });
 }
 
