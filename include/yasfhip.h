@@ -36,7 +36,19 @@ enum {
     YASFHIP_INTERNAL = 4
 };
 
-enum { YASFHIP_FP64 = 0, YASFHIP_FP32 = 1 };
+/* Arithmetic of the pair GEMM (the O(N^2) part of W x):
+ *   YASFHIP_FP64    complex128 on the FP64 matrix cores;
+ *   YASFHIP_FP32    complex64 on the FP32 matrix cores;
+ *   YASFHIP_FP16X2  pair factors and x each carried as two fp16 terms (hi +
+ *                   lo, scaled per order and per mode column by powers of
+ *                   two) on the fp16 matrix cores with FP32 accumulation:
+ *                   the accuracy of complex64 at the fp16 MFMA rate, over a
+ *                   resident table of the complex64 table's size;
+ *   YASFHIP_FP16    the hi terms only (relative error ~1e-4), half that table.
+ * Every precision evaluates the special functions in FP64 and accumulates
+ * the ab5 contraction in FP64.  The two fp16 forms need a resident table
+ * (matvec mode TABLE, or AUTO where it fits). */
+enum { YASFHIP_FP64 = 0, YASFHIP_FP32 = 1, YASFHIP_FP16X2 = 2, YASFHIP_FP16 = 3 };
 
 /* How the coupling matvec obtains the pair factors P_p^{|dm|} h_p e^{i dm phi}:
  * generated inside the matvec kernel (ON_THE_FLY); generated once per session
@@ -79,9 +91,10 @@ int yasfhip_set_tmatrix(yasfhip_session *s, const double *t);
 
 /* y = W x for one channel (CouplingBackend.multiply(x, idx),
  * yasfpy/coupling/backends.py:99-125).  precision: YASFHIP_FP64 computes in
- * complex128; YASFHIP_FP32 runs the pair GEMM in complex64 (geometry and
- * special functions are evaluated in FP64 and rounded; the ab5 contraction
- * accumulates in FP64) with complex128 in/out. */
+ * complex128; YASFHIP_FP32, YASFHIP_FP16X2 and YASFHIP_FP16 run the pair
+ * GEMM as described above (geometry and special functions are evaluated in
+ * FP64 and rounded; the ab5 contraction accumulates in FP64) with
+ * complex128 in/out. */
 int yasfhip_coupling_matvec(yasfhip_session *s, int channel, const double *x, double *y,
                             int precision);
 
@@ -151,6 +164,11 @@ int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b
                                 int max_refinements, int use_precond, int *info, int *refinements,
                                 int *iterations, double *residual, double *history,
                                 int history_cap);
+
+/* Operator of the inner GMRES of yasfhip_gmres_solve_refined:
+ * YASFHIP_FP32 (default), YASFHIP_FP16X2 or YASFHIP_FP16 (see above).  The
+ * refinement's residuals stay FP64.  No counterpart in the reference. */
+int yasfhip_set_inner_precision(yasfhip_session *s, int precision);
 
 /* Extinction and scattering cross sections of one channel from the incident
  * (a_inc) and scattered (f) field coefficients, particle-major complex128
@@ -372,6 +390,15 @@ int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, in
  * slab_tiles > 0 fixes the FP64 16-particle target tiles per slab (complex64
  * GEMMs use twice as many), 0 sizes slabs to the arena. */
 int yasfhip_server_set_table_policy(yasfhip_server *s, int policy, int slab_tiles);
+
+/* Pair GEMM of the inner GMRES iterations of operators admitted from now
+ * on: YASFHIP_FP32 (default: complex64 tables and GEMMs), YASFHIP_FP16X2 or
+ * YASFHIP_FP16 (fp16-term tables and the fp16 pair GEMM; see the precision
+ * enum).  Applies to operators that keep a resident table; slab and
+ * on-the-fly operators keep the complex64 GEMM.  FP64 residuals decide
+ * convergence in every case.  Needs an idle server (nothing queued or
+ * active). */
+int yasfhip_server_set_inner_precision(yasfhip_server *s, int precision);
 
 /* C_sca method of the completions from now on (YASFHIP_CSCA_FAR_FIELD, the
  * default, or YASFHIP_CSCA_PAIR_SUM; see yasfhip_set_scattering_method). */

@@ -282,3 +282,37 @@ def test_scattering_methods_agree(mixed_requests, served):
         assert b["c_sca"] == pytest.approx(a["c_sca"], rel=1e-10)
     with pytest.raises(ValueError):
         _native.Server(memory_bytes=1 << 30).set_scattering_method(2)
+
+
+@pytest.mark.parametrize("precision", [_native.FP16X2, _native.FP16], ids=["fp16x2", "fp16"])
+def test_half_inner_precision(mixed_requests, served, aggregate_requests, precision):
+    # the inner iterations on the fp16 pair GEMM (fp16-term tables); FP64
+    # residuals still decide convergence, so every solution meets the FP64
+    # criterion and its cross sections agree with the complex64 server's
+    import copy
+    extra = [copy.copy(r) for r in aggregate_requests[:6]]
+    for i, r in enumerate(extra):
+        r.index = 3000 + i  # serve() keys results by request index
+    reqs = mixed_requests + extra
+    runs = []
+    for use_graph in (True, False):
+        srv = _native.Server(memory_bytes=4 << 30, restart=50, inner_rtol=1e-4, use_graph=use_graph)
+        srv.set_inner_precision(precision)
+        runs.append(serve(srv, reqs, keep_solutions=True))
+        srv.close()
+    half, eager = runs
+    for r in reqs:
+        a = half.results[r.index]
+        _fp64_check(r, a)
+        e = eager.results[r.index]
+        assert np.array_equal(a["x"], e["x"]) and a["iterations"] == e["iterations"]
+    for r in mixed_requests:
+        a, b = half.results[r.index], served.results[r.index]
+        assert abs(a["c_ext"] - b["c_ext"]) <= 2e-6 * abs(b["c_ext"])
+        assert abs(a["c_sca"] - b["c_sca"]) <= 2e-6 * abs(b["c_sca"])
+    srv = _native.Server(memory_bytes=1 << 30)
+    r0 = mixed_requests[0]
+    srv.submit(r0.positions, r0.lmax, r0.k, r0.t, r0.a_inc)
+    with pytest.raises(ValueError):  # not on a busy server
+        srv.set_inner_precision(precision)
+    srv.close()

@@ -131,3 +131,12 @@ def test_library_loads_into_torch_hip_runtime():
     runtimes = [line for line in out.stdout.splitlines() if line.strip()]
     assert len(runtimes) == 1, runtimes
     assert "torch" in runtimes[0]
+
+
+def test_serve_refuses_duplicate_request_indices():
+    # results are keyed by request index: duplicates would never all arrive
+    from yasf_amd.serving import ServeRequest, serve
+
+    r = ServeRequest(0, np.zeros((1, 3)), 3, 500.0, 0.01, np.zeros(30, complex), np.zeros(30, complex), 1.5, 0.0, 0.0)
+    with pytest.raises(ValueError):
+        serve(None, [r, r])

@@ -17,6 +17,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libyasfhip.so"
 
 FP64 = 0
 FP32 = 1
+FP16X2 = 2  # pair GEMM on the fp16 matrix cores, operands as two fp16 terms (complex64 accuracy)
+FP16 = 3    # one fp16 term (relative error ~1e-4)
 
 MATVEC_AUTO = 0
 MATVEC_ON_THE_FLY = 1
@@ -118,6 +120,8 @@ def _load():
             "yasfhip_tune_matvec": ([P, i, i, i, ip, P], i),
             "yasfhip_coupling_matvec_noalloc": ([P, i, i, d, P, P, P], i),
             "yasfhip_set_scattering_method": ([P, i], i),
+            "yasfhip_set_inner_precision": ([P, i], i),
+            "yasfhip_server_set_inner_precision": ([P, i], i),
             "yasfhip_server_set_scattering_method": ([P, i], i),
         }
         for name, (args, res) in sig.items():
@@ -145,7 +149,8 @@ def exported_symbols() -> list[str]:
         "yasfhip_set_slab_tiles", "yasfhip_set_near_field_radius", "yasfhip_mie_coefficients",
         "yasfhip_planewave_coefficients", "yasfhip_server_submit_spheres", "yasfhip_set_table_shape",
         "yasfhip_set_precond_factor_precision", "yasfhip_coupling_matvec_noalloc",
-        "yasfhip_set_scattering_method", "yasfhip_server_set_scattering_method",
+        "yasfhip_set_scattering_method", "yasfhip_server_set_scattering_method", "yasfhip_set_inner_precision",
+        "yasfhip_server_set_inner_precision",
     ]
 
 
@@ -373,6 +378,10 @@ class Session:
         _check(self._lib.yasfhip_precond_apply(self._h, _ptr(r), _ptr(z)))
         return z
 
+    def set_inner_precision(self, precision: int):
+        """Operator of gmres_refined's inner GMRES: FP32 (default), FP16X2 or FP16."""
+        _check(self._lib.yasfhip_set_inner_precision(self._h, int(precision)))
+
     def set_scattering_method(self, method: int):
         """CSCA_FAR_FIELD (default) or CSCA_PAIR_SUM for cross_sections."""
         _check(self._lib.yasfhip_set_scattering_method(self._h, int(method)))
@@ -530,6 +539,11 @@ class Server:
         _check(self._lib.yasfhip_server_set_tile_shape(self._h, int(lmax), int(waves), int(column_tiles)))
 
     TABLES_ON_THE_FLY, TABLES_AUTO, TABLES_SLABS = 0, 1, 2
+
+    def set_inner_precision(self, precision: int):
+        """Pair GEMM of the inner iterations of operators admitted from now on:
+        FP32 (default), FP16X2 or FP16.  Needs an idle server."""
+        _check(self._lib.yasfhip_server_set_inner_precision(self._h, int(precision)))
 
     def set_scattering_method(self, method: int):
         """CSCA_FAR_FIELD (default) or CSCA_PAIR_SUM for the completions' C_sca."""

@@ -16,7 +16,7 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libyasfhip.so"
 SOURCES = [
     "tables.cpp", "coupling.hip", "pairtable.hip", "krylov.hip", "precond.hip", "observables.hip",
-    "setup.hip", "capi.hip", "server.hip", "noalloc.hip",
+    "setup.hip", "capi.hip", "server.hip", "noalloc.hip", "half.hip",
 ]
 ARCH = os.environ.get("YASFHIP_ARCH", "gfx950")
 

@@ -44,6 +44,15 @@ void csca_quadrature_grid(double kR, int lmax, int *n_theta, int *n_phi);
 size_t csca_quadrature_partials(int n_theta, int n_phi);
 void launch_csca_quadrature(const double4 *pos, const double2 *f, int N, int lmax, double k, int n_theta,
                             int n_phi, double *partials, double *out, hipStream_t s);
+// half.hip: the pair GEMM on the fp16 matrix cores
+size_t half_table_term_bytes(const DevicePlan &dp, int N);
+size_t half_x_bytes(const DevicePlan &dp, int N);
+void launch_half_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, void *table, bool split,
+                             int *pexp, unsigned long long *dmin, hipStream_t s, double near2);
+void launch_half_x(const DevicePlan &dp, int N, const double2 *x, void *xh, bool split, unsigned *colmax, int *xexp,
+                   hipStream_t s);
+void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void *xh, bool split, const int *pexp,
+                      const int *xexp, void *Q, hipStream_t s);
 // precond.hip
 struct BlockPrecond;
 BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &ab5,
@@ -146,13 +155,24 @@ struct yasfhip_session {
     size_t slab_bytes = 0;
     int slab_tiles_req = 0;  // tiles per slab fixed by the caller (0: what fits)
     bool slab_refused = false;
+    // fp16-term pair table (hi term, then lo term when split) and the
+    // working memory of its GEMM
+    void *htable = nullptr;
+    int htable_channel = -1;
+    bool htable_split = false;
+    void *xh = nullptr;
+    unsigned *hcolmax = nullptr;
+    int *hxexp = nullptr, *hpexp = nullptr;
+    unsigned long long *hdmin = nullptr;
+    int inner_precision = YASFHIP_FP32;  // operator of the refined solve's inner GMRES
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 
     ~yasfhip_session() {
         if (precond)
             precond_destroy(precond);
         void *bufs[] = {pos,  Q,    t,    xin,   yout,    x32,     V,    bvec,     xvec, rvec,
-                        avec, ycoef, scal, partials, counter, hdev, rb, rx, rr, mgs_part};
+                        avec, ycoef, scal, partials, counter, hdev, rb, rx, rr, mgs_part,
+                        htable, xh, hcolmax, hxexp, hpexp, hdmin};
         for (void *b : bufs)
             if (b)
                 (void)hipFree(b);
@@ -200,6 +220,40 @@ struct yasfhip_session {
         launch_pair_table_build(dp, pos, N, k[channel], fp32, table[ix], stream, 0, -1, nullptr, false, near2);
         table_channel[ix] = channel;
         return true;
+    }
+
+    // Makes the fp16-term table hold `channel` (with its lo term when split).
+    // It is a resident table only: the matvec mode must allow one (TABLE, or
+    // AUTO with room for it).
+    void half_table_for(int channel, bool split) {
+        YH_REQUIRE(matvec_mode == YASFHIP_MATVEC_TABLE || matvec_mode == YASFHIP_MATVEC_AUTO,
+                   "the fp16 pair GEMM needs a resident table (matvec mode TABLE or AUTO)");
+        if (htable && htable_channel == channel && (htable_split || !split))
+            return;
+        const size_t term = half_table_term_bytes(dp, N);
+        if (htable && split && !htable_split) {
+            YH_CHECK(hipFree(htable));
+            htable = nullptr;
+        }
+        if (!htable) {
+            const size_t bytes = split ? 2 * term : term;
+            if (matvec_mode == YASFHIP_MATVEC_AUTO) {
+                size_t free_b = 0, total_b = 0;
+                YH_CHECK(hipMemGetInfo(&free_b, &total_b));
+                YH_REQUIRE(bytes <= free_b / 4 * 3, "the fp16 pair table does not fit in device memory");
+            }
+            YH_CHECK(hipMalloc(&htable, bytes));
+            htable_split = split;
+        }
+        if (!xh) {
+            xh = dmalloc<unsigned char>(half_x_bytes(dp, N));
+            hcolmax = dmalloc<unsigned>(nmax);
+            hxexp = dmalloc<int>(nmax + 1);
+            hpexp = dmalloc<int>(2 * L + 1);
+            hdmin = dmalloc<unsigned long long>(1);
+        }
+        launch_half_table_build(dp, pos, N, k[channel], htable, htable_split, hpexp, hdmin, stream, near2);
+        htable_channel = channel;
     }
 
     // Target tiles per slab for this precision once the slab buffer exists;
@@ -274,11 +328,19 @@ struct yasfhip_session {
     }
 
     // y = W x (mode 0) or x - t W x (mode 1) for device vectors of complex128.
-    // fp32: the pair GEMM runs in complex64 on a complex64 copy of x; the ab5
-    // contraction and the epilogue stay FP64.
-    void matvec_dev(int channel, const double2 *x, double2 *y, int mode, bool fp32) {
+    // YASFHIP_FP32: the pair GEMM runs in complex64 on a complex64 copy of x;
+    // YASFHIP_FP16X2 / YASFHIP_FP16: on the fp16 matrix cores over the
+    // fp16-term table and an fp16-term copy of x (complex64 partial sums);
+    // the ab5 contraction and the epilogue stay FP64.
+    void matvec_dev(int channel, const double2 *x, double2 *y, int mode, int precision) {
         const double2 *tt = t + static_cast<size_t>(channel) * n;
-        if (!fp32) {
+        if (precision == YASFHIP_FP16X2 || precision == YASFHIP_FP16) {
+            const bool split = precision == YASFHIP_FP16X2;
+            half_table_for(channel, split);
+            launch_half_x(dp, N, x, xh, split, hcolmax, hxexp, stream);
+            launch_half_gemm(dp, N, htable, xh, split, hpexp, hxexp, Q, stream);
+            launch_contract(dp, N, Q, x, tt, mode, y, true, false, stream);
+        } else if (precision == YASFHIP_FP64) {
             pair_gemm(channel, x, false);
             launch_contract(dp, N, Q, x, tt, mode, y, false, false, stream);
         } else {
@@ -339,11 +401,11 @@ struct yasfhip_session {
     // SciPy's gmres(A, b, x0, M, restart, rtol, atol=0, maxiter) on device
     // vectors (scipy/sparse/linalg/_isolve/iterative.py:692-840): bb is the
     // right-hand side, x holds x0 on entry and the solution on exit; neither
-    // may alias rvec, avec or V.  fp32 selects the complex64 pair GEMM for
-    // every operator application (the Arnoldi process stays FP64).
+    // may alias rvec, avec or V.  precision selects the pair GEMM of every
+    // operator application (the Arnoldi process stays FP64).
     // history: presid/||b|| per inner iteration, up to history_cap values.
     GmresOutcome gmres_run(int channel, const double2 *bb, double2 *x, double rtol, int restart,
-                           int maxiter, bool pc, bool fp32, double *history, int history_cap) {
+                           int maxiter, bool pc, int precision, double *history, int history_cap) {
         GmresOutcome out;
         hipStream_t st = stream;
         double2 *r = rvec, *av = avec;
@@ -373,7 +435,7 @@ struct yasfhip_session {
                 launch_norm2(x, n, reducer(), scal, st);
                 fetch_scalars(1);
                 if (scal_host[0] != 0.0) {
-                    matvec_dev(channel, x, av, 1, fp32);
+                    matvec_dev(channel, x, av, 1, precision);
                     launch_residual(r, bb, av, n, reducer(), scal, st);
                     fetch_scalars(1);
                 } else {
@@ -398,10 +460,10 @@ struct yasfhip_session {
                 double2 *vc = V + static_cast<size_t>(col) * n;
                 double2 *w = V + static_cast<size_t>(col + 1) * n;
                 if (pc) {
-                    matvec_dev(channel, vc, av, 1, fp32);
+                    matvec_dev(channel, vc, av, 1, precision);
                     psolve(av, w, true);
                 } else {
-                    matvec_dev(channel, vc, w, 1, fp32);
+                    matvec_dev(channel, vc, w, 1, precision);
                 }
                 // Modified Gram-Schmidt, one fused launch per basis vector:
                 // launch kq subtracts h[kq-1] v[kq-1] and forms the partials
@@ -476,7 +538,7 @@ struct yasfhip_session {
             YH_CHECK(hipMemcpyAsync(ycoef, ydev.data(), (col + 1) * sizeof(double2), hipMemcpyHostToDevice,
                                     st));
             launch_update(x, V, ycoef, col + 1, n, n, st);
-            matvec_dev(channel, x, av, 1, fp32);
+            matvec_dev(channel, x, av, 1, precision);
             launch_residual(r, bb, av, n, reducer(), scal, st);
             fetch_scalars(1);
             rnorm = std::sqrt(scal_host[0]);
@@ -623,11 +685,11 @@ static void host_matvec(yasfhip_session *s, int channel, const double *x, double
                         int mode) {
     YH_REQUIRE(s && x && y, "NULL argument");
     YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
-    YH_REQUIRE(precision == YASFHIP_FP64 || precision == YASFHIP_FP32, "unknown precision");
+    YH_REQUIRE(precision >= YASFHIP_FP64 && precision <= YASFHIP_FP16, "unknown precision");
     if (mode == 1)
         YH_REQUIRE(s->t_set, "T-matrix not set (yasfhip_set_tmatrix)");
     YH_CHECK(hipMemcpyAsync(s->xin, x, s->n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
-    s->matvec_dev(channel, s->xin, s->yout, mode, precision == YASFHIP_FP32);
+    s->matvec_dev(channel, s->xin, s->yout, mode, precision);
     YH_CHECK(hipMemcpyAsync(y, s->yout, s->n * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
     YH_CHECK(hipStreamSynchronize(s->stream));
 }
@@ -689,7 +751,7 @@ int gmres_solve(yasfhip_session *s, int channel, const double *b, double *x_inou
         YH_CHECK(hipMemcpyAsync(s->bvec, b, n * sizeof(double2), in, st));
         YH_CHECK(hipMemcpyAsync(s->xvec, x_inout, n * sizeof(double2), in, st));
         const auto o = s->gmres_run(channel, s->bvec, s->xvec, rtol, restart, maxiter, use_precond != 0,
-                                    false, history, history ? history_cap : 0);
+                                    YASFHIP_FP64, history, history ? history_cap : 0);
         YH_CHECK(hipMemcpyAsync(x_inout, s->xvec, n * sizeof(double2), out, st));
         YH_CHECK(hipStreamSynchronize(st));
         if (info)
@@ -744,7 +806,10 @@ int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b
             s->rr = dmalloc<double2>(n);
         }
         hipStream_t st = s->stream;
-        s->table_for(channel, true);  // the inner solves' table first: they make most matvecs
+        if (s->inner_precision == YASFHIP_FP32)
+            s->table_for(channel, true);  // the inner solves' table first: they make most matvecs
+        else
+            s->half_table_for(channel, s->inner_precision == YASFHIP_FP16X2);
         YH_CHECK(hipMemcpyAsync(s->rb, b, n * sizeof(double2), hipMemcpyHostToDevice, st));
         YH_CHECK(hipMemcpyAsync(s->rx, x_inout, n * sizeof(double2), hipMemcpyHostToDevice, st));
         const double bnorm = s->norm_dev(s->rb);
@@ -752,7 +817,7 @@ int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b
         double rnorm = 0.0;
         // FP64 residual r = b - A x of the current iterate
         auto residual64 = [&] {
-            s->matvec_dev(channel, s->rx, s->avec, 1, false);
+            s->matvec_dev(channel, s->rx, s->avec, 1, YASFHIP_FP64);
             launch_residual(s->rr, s->rb, s->avec, n, s->reducer(), s->scal, st);
             s->fetch_scalars(1);
             rnorm = std::sqrt(s->scal_host[0]);
@@ -769,7 +834,7 @@ int yasfhip_gmres_solve_refined(yasfhip_session *s, int channel, const double *b
                 const double target = std::max(inner_rtol, 0.5 * rtol * bnorm / rnorm);
                 YH_CHECK(hipMemsetAsync(s->xvec, 0, n * sizeof(double2), st));
                 const auto o = s->gmres_run(channel, s->rr, s->xvec, target, restart, inner_maxiter,
-                                            use_precond != 0, true, nullptr, 0);
+                                            use_precond != 0, s->inner_precision, nullptr, 0);
                 inner += o.iterations;
                 ++steps;
                 launch_add(s->rx, s->xvec, n, st);
@@ -835,6 +900,15 @@ int yasfhip_set_scattering_method(yasfhip_session *s, int method) {
         YH_REQUIRE(method == YASFHIP_CSCA_FAR_FIELD || method == YASFHIP_CSCA_PAIR_SUM,
                    "method must be YASFHIP_CSCA_FAR_FIELD or YASFHIP_CSCA_PAIR_SUM");
         s->csca_method = method;
+    });
+}
+
+int yasfhip_set_inner_precision(yasfhip_session *s, int precision) {
+    return guarded([&] {
+        YH_REQUIRE(s, "session is NULL");
+        YH_REQUIRE(precision == YASFHIP_FP32 || precision == YASFHIP_FP16X2 || precision == YASFHIP_FP16,
+                   "inner precision must be YASFHIP_FP32, YASFHIP_FP16X2 or YASFHIP_FP16");
+        s->inner_precision = precision;
     });
 }
 
@@ -1092,21 +1166,34 @@ int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps
         YH_REQUIRE(s && ms_pair && ms_total, "NULL argument");
         YH_REQUIRE(channel >= 0 && channel < s->W, "channel out of range");
         YH_REQUIRE(reps >= 1, "reps must be >= 1");
+        YH_REQUIRE(precision >= YASFHIP_FP64 && precision <= YASFHIP_FP16, "unknown precision");
         const bool fp32 = precision == YASFHIP_FP32;
+        const bool half = precision == YASFHIP_FP16X2 || precision == YASFHIP_FP16;
+        const bool split = precision == YASFHIP_FP16X2;
         hipStream_t st = s->stream;
         double pair_ms = 0.0, total_ms = 0.0;
         launch_fill_test(reinterpret_cast<double *>(s->xin), 2 * s->n, st);  // realistic operands
         if (fp32)
             launch_to_fp32(reinterpret_cast<float *>(s->x32), reinterpret_cast<const double *>(s->xin),
                            2 * s->n, st);
+        if (half) {
+            s->half_table_for(channel, split);
+            launch_half_x(s->dp, s->N, s->xin, s->xh, split, s->hcolmax, s->hxexp, st);
+        }
         for (int r = 0; r < reps; ++r) {
             const void *xv = fp32 ? static_cast<const void *>(s->x32) : static_cast<const void *>(s->xin);
-            s->table_for(channel, fp32);  // a (re)build is not part of the timed matvec
+            if (half)
+                s->half_table_for(channel, split);
+            else
+                s->table_for(channel, fp32);  // a (re)build is not part of the timed matvec
             YH_CHECK(hipEventRecord(s->ev[0], st));
-            s->pair_gemm(channel, xv, fp32);
+            if (half)
+                launch_half_gemm(s->dp, s->N, s->htable, s->xh, split, s->hpexp, s->hxexp, s->Q, st);
+            else
+                s->pair_gemm(channel, xv, fp32);
             YH_CHECK(hipEventRecord(s->ev[1], st));
             launch_contract(s->dp, s->N, s->Q, s->xin, s->t + static_cast<size_t>(channel) * s->n, 0,
-                            s->yout, fp32, false, st);
+                            s->yout, fp32 || half, false, st);
             YH_CHECK(hipEventRecord(s->ev[2], st));
             YH_CHECK(hipEventSynchronize(s->ev[2]));
             float a = 0.f, b = 0.f;

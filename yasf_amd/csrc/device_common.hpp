@@ -115,17 +115,17 @@ __device__ __forceinline__ PairGeometry pair_geometry(double dx, double dy, doub
     return g;
 }
 
-// G_{s1 s2}[p, dm] for every p in [|dm|, 2L], stored to dst[(p - |dm|) * stride]:
+// G_{s1 s2}[p, dm] for every p in [|dm|, 2L], handed to sink(p - |dm|, re, im):
 // Ptilde_p^{|dm|}(cos th) h_p(k d) e^{i dm phi} with h_p = j_p + i y_p and the
 // Legendre functions of legendre_normalized_trigon.py:59-91.  h_p comes from
 // the upward recurrence h_{p+1} = (2p+1)/x h_p - h_{p-1} seeded with h_0, h_1
 // (stable for the y_p part that dominates |h_p|).  All arithmetic is FP64;
-// invalid pairs store 0.
-template <typename T2>
-__device__ __forceinline__ void pair_series(double dx, double dy, double dz, double k, int dm, int P,
-                                            bool valid, const double *__restrict__ la,
-                                            const double *__restrict__ lb,
-                                            const double *__restrict__ ld, T2 *dst, int stride) {
+// invalid pairs give 0.
+template <typename Sink>
+__device__ __forceinline__ void pair_series_to(double dx, double dy, double dz, double k, int dm, int P,
+                                               bool valid, const double *__restrict__ la,
+                                               const double *__restrict__ lb,
+                                               const double *__restrict__ ld, Sink &&sink) {
     const PairGeometry g = pair_geometry(dx, dy, dz, k, dm);
     const int adm = dm < 0 ? -dm : dm;
     // Hankel: (ar, ai) = h_p, (hr, hi) = h_{p+1}.
@@ -151,10 +151,7 @@ __device__ __forceinline__ void pair_series(double dx, double dy, double dz, dou
                 gr = 0.0;
                 gi = 0.0;
             }
-            T2 v;
-            v.x = gr;
-            v.y = gi;
-            dst[(p - adm) * stride] = v;
+            sink(p - adm, gr, gi);
             const double nx = la[p * P + adm] * g.ct * cur - lb[p * P + adm] * prv;
             prv = cur;
             cur = nx;
@@ -166,6 +163,20 @@ __device__ __forceinline__ void pair_series(double dx, double dy, double dz, dou
         hr = nr;
         hi = ni;
     }
+}
+
+// As above, stored to dst[(p - |dm|) * stride] (float2 or double2).
+template <typename T2>
+__device__ __forceinline__ void pair_series(double dx, double dy, double dz, double k, int dm, int P,
+                                            bool valid, const double *__restrict__ la,
+                                            const double *__restrict__ lb,
+                                            const double *__restrict__ ld, T2 *dst, int stride) {
+    pair_series_to(dx, dy, dz, k, dm, P, valid, la, lb, ld, [&](int ip, double gr, double gi) {
+        T2 v;
+        v.x = gr;
+        v.y = gi;
+        dst[ip * stride] = v;
+    });
 }
 
 constexpr int kMaxOrders = 2 * kMaxLmax + 1;  // p = 0 .. 2 lmax

@@ -1,0 +1,293 @@
+// The pair GEMM on the fp16 matrix cores (v_mfma_f32_16x16x32_f16, FP32
+// accumulation), shared by the session kernels (half.hip) and the batched
+// server (server.hip).
+//
+// Each operand value v is carried as v ~ (hi + lo) 2^-e with hi = fp16(v 2^e)
+// and lo = fp16(v 2^e - hi): a 22-bit significand, so the product of two such
+// sums (hi hi + hi lo + lo hi; lo lo is below the accumulator's rounding)
+// keeps the accuracy of the complex64 GEMM while the table stream and the
+// MFMA rate are those of fp16.  The one-term form (hi only) is offered too
+// (relative error ~1e-4).  Scales are powers of two, so scaling is exact:
+//   A (pair factors): per order p, 2^e_p with e_p from a bound of
+//     |P_p^m| |h_p(k d)| <= sqrt((2p+1)/2) |h_p(k d_min)| (|h_p| falls with
+//     its argument; the normalised Legendre functions obey sum_m |Y_p^m|^2 =
+//     (2p+1)/4pi);
+//   B (source coefficients): per mode column n2, 2^f from the column's
+//     largest |Re|, |Im|;
+// so scaled magnitudes stay below 2^kHalfTop and fp16 keeps them in its
+// normal range over 2^-14..2^16.  The GEMM's C tile is scaled back by
+// 2^-(e_p + f_n2) in FP32 and stored as complex64, like the complex64 GEMM's,
+// so the ab5 contraction that follows is unchanged.
+//
+// Table layout (fp16, two arrays for the hi and the lo terms, lo optional):
+//   frag(t, d, ks, ip) = (t * (2L+1)^2 + dmoff[d]) * N32 + ks * R + ip
+//   half index         = (frag * 2 + plane) * 512 + lane * 8 + i
+// plane 0 = Re, 1 = Im; ks = K-step of 32 sources; lane = (source % 32 / 8) *
+// 16 + target % 16 and i = source % 8: the A-operand layout of
+// v_mfma_f32_16x16x32_f16 (lane l holds A[l % 16][8 (l / 16) + i]).
+// Source vectors: xh[term][plane][n2 (0..nmax, row nmax zero)][N32 * 32]
+// halves, so a lane's 8 sources of one column are 16 contiguous bytes.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace yasfhip {
+
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kHalfTop = 14;  // scaled magnitudes stay below 2^kHalfTop
+
+// Exponent e with bound * 2^e < 2^kHalfTop (0 for a zero or non-finite bound).
+__host__ __device__ inline int half_exponent(double bound) {
+    if (!(bound > 0.0) || !(bound < HUGE_VAL))
+        return 0;
+    int e = 0;
+    frexp(bound, &e);  // bound < 2^e
+    return kHalfTop - e;
+}
+
+// Halves per (term, plane) of a source-vector copy: rows 0..nmax, N32 * 32 sources.
+__host__ __device__ inline size_t half_x_plane(int nmax, int N32) {
+    return static_cast<size_t>(nmax + 1) * N32 * 32;
+}
+
+// Halves of one target tile of the table, one term.
+__host__ __device__ inline size_t half_tile_halves(int lmax, int N32) {
+    const size_t P = 2 * lmax + 1;
+    return P * P * N32 * 2 * 512;
+}
+
+__device__ __forceinline__ f4_t mfma_h16(h8_t a, h8_t b, f4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// One (job, target tile) block of WAVES waves.  hi / lo point at the job's
+// (tile, dm) fragment 0 of the two terms (lo unused unless SPLIT); xhi / xlo
+// at the two terms' plane 0 of the source copy (plane 1 = Im follows at
+// xplane halves); cols: the job's 16 ntile source columns (-1: padding);
+// orders p = |dm| + ip, ip < nord, dealt to the waves in turn; pexp: order
+// exponents from p = |dm| on; xexp[n2] (xexp[nmax] = 0).  C tile (ip, it)
+// goes to Q slot ip * tstride + tile0 + it past Qt (complex64).
+//
+// Pipelining: the B fragments of a K-step (the job's column tiles, shared by
+// all waves) are loaded by the whole block, one K-step ahead, into one of two
+// LDS buffers; each wave loads its orders' A fragments one K-step ahead into
+// registers.  Loads are unconditional (the last K-step reloads itself), so
+// the compiler's waits leave the next step's loads in flight during the
+// MFMAs.  One barrier per K-step hands the LDS buffers over.
+template <int WAVES, int NTL, bool SPLIT>
+__device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_tab, const _Float16 *__restrict__ lo_tab,
+                                                const _Float16 *__restrict__ xhi, const _Float16 *__restrict__ xlo,
+                                                size_t xplane, int N32, int nmax, int R, int nord, int ntile,
+                                                const int *__restrict__ cols, const int *__restrict__ pexp,
+                                                const int *__restrict__ xexp, float *__restrict__ Qt, int tstride,
+                                                int tile0) {
+    constexpr int NIP = (2 * kMaxLmax + 1 + WAVES - 1) / WAVES;
+    constexpr int NPL = SPLIT ? 4 : 2;  // planes: hi Re, hi Im, lo Re, lo Im
+    constexpr int THREADS = WAVES * 64;
+    constexpr int CHUNKS = NTL * NPL * 64;  // 16-byte B pieces of one K-step
+    constexpr int PER = (CHUNKS + THREADS - 1) / THREADS;
+    __shared__ h8_t bsh[2][NTL][NPL][64];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, lo = lane & 15, hi = lane >> 4;
+    const int wave = uniform(tid >> 6);
+    const size_t S32 = static_cast<size_t>(N32) * 32;
+    // this thread's B pieces: source pointer (K-step 0) and LDS slot
+    const _Float16 *bsrc[PER];
+    int bslot[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int q = tid + u * THREADS;
+        const int it = q / (NPL * 64), pl = (q / 64) % NPL, ln = q % 64;
+        const int n2 = (q < CHUNKS && it < ntile) ? cols[it * 16 + (ln & 15)] : -1;
+        const _Float16 *plane = (pl < 2 ? xhi : xlo) + (pl & 1) * xplane;
+        bsrc[u] = plane + static_cast<size_t>(n2 < 0 ? nmax : n2) * S32 + (ln >> 4) * 8;
+        bslot[u] = q < CHUNKS ? q : -1;
+    }
+    h8_t breg[PER];
+    auto load_b = [&](int ks) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            breg[u] = *reinterpret_cast<const h8_t *>(bsrc[u] + static_cast<size_t>(ks) * 32);
+    };
+    auto store_b = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            if (bslot[u] >= 0)
+                (&bsh[buf][0][0][0])[bslot[u]] = breg[u];
+    };
+    // this wave's orders (past nord: the last order reloaded, its MFMAs skipped)
+    int ipw[NIP];
+#pragma unroll
+    for (int j = 0; j < NIP; ++j)
+        ipw[j] = min(wave + j * WAVES, nord - 1);
+    const size_t fstride = 2 * 512;  // halves per fragment (Re and Im planes)
+    h8_t acur[NIP][NPL], anext[NIP][NPL];
+    auto load_a = [&](int ks, h8_t (&a)[NIP][NPL]) {
+#pragma unroll
+        for (int j = 0; j < NIP; ++j) {
+            const size_t fo = (static_cast<size_t>(ks) * R + ipw[j]) * fstride + lane * 8;
+            a[j][0] = *reinterpret_cast<const h8_t *>(hi_tab + fo);
+            a[j][1] = *reinterpret_cast<const h8_t *>(hi_tab + fo + 512);
+            if constexpr (SPLIT) {
+                a[j][2] = *reinterpret_cast<const h8_t *>(lo_tab + fo);
+                a[j][3] = *reinterpret_cast<const h8_t *>(lo_tab + fo + 512);
+            }
+        }
+    };
+    f4_t acc[NIP][NTL][2];
+#pragma unroll
+    for (int j = 0; j < NIP; ++j)
+#pragma unroll
+        for (int it = 0; it < NTL; ++it)
+            acc[j][it][0] = acc[j][it][1] = f4_t{0.f, 0.f, 0.f, 0.f};
+    load_b(0);
+    load_a(0, acur);
+    store_b(0);
+    __syncthreads();
+    for (int ks = 0; ks < N32; ++ks) {
+        const int kn = ks + 1 < N32 ? ks + 1 : ks;
+        load_b(kn);
+        load_a(kn, anext);
+        const int buf = ks & 1;
+#pragma unroll
+        for (int it = 0; it < NTL; ++it) {
+            if (it >= ntile)
+                continue;
+            h8_t b[NPL];
+#pragma unroll
+            for (int pl = 0; pl < NPL; ++pl)
+                b[pl] = bsh[buf][it][pl][lane];
+#pragma unroll
+            for (int j = 0; j < NIP; ++j) {
+                if (wave + j * WAVES >= nord)
+                    continue;
+                const h8_t ar = acur[j][0], ai = acur[j][1], nai = -acur[j][1];
+                f4_t cr = acc[j][it][0], ci = acc[j][it][1];
+                cr = mfma_h16(ar, b[0], cr);
+                cr = mfma_h16(nai, b[1], cr);
+                ci = mfma_h16(ar, b[1], ci);
+                ci = mfma_h16(ai, b[0], ci);
+                if constexpr (SPLIT) {
+                    const h8_t lr = acur[j][2], li = acur[j][3], nli = -acur[j][3];
+                    cr = mfma_h16(ar, b[2], cr);
+                    cr = mfma_h16(nai, b[3], cr);
+                    cr = mfma_h16(lr, b[0], cr);
+                    cr = mfma_h16(nli, b[1], cr);
+                    ci = mfma_h16(ar, b[3], ci);
+                    ci = mfma_h16(ai, b[2], ci);
+                    ci = mfma_h16(lr, b[1], ci);
+                    ci = mfma_h16(li, b[0], ci);
+                }
+                acc[j][it][0] = cr;
+                acc[j][it][1] = ci;
+            }
+        }
+        store_b(buf ^ 1);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NIP; ++j)
+#pragma unroll
+            for (int pl = 0; pl < NPL; ++pl)
+                acur[j][pl] = anext[j][pl];
+    }
+#pragma unroll
+    for (int j = 0; j < NIP; ++j) {
+        const int ip = wave + j * WAVES;
+        if (ip >= nord)
+            continue;
+#pragma unroll
+        for (int it = 0; it < NTL; ++it)
+            if (it < ntile) {
+                const int n2 = cols[it * 16 + lo];
+                const int e = pexp[ip] + xexp[n2 < 0 ? nmax : n2];
+                const size_t slot = static_cast<size_t>(ip) * tstride + tile0 + it;
+                float *dst = Qt + 2 * ((slot * 16 + lo) * 16 + 4 * hi);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    dst[2 * i] = ldexpf(acc[j][it][0][i], -e);
+                    dst[2 * i + 1] = ldexpf(acc[j][it][1][i], -e);
+                }
+            }
+    }
+}
+
+// Source-vector copy for the fp16 GEMM, step 1: per column n2 the largest
+// |Re| or |Im| over the N sources (as float bits, atomic max into colmax,
+// which the caller zeroes).  One block per 256 rows.
+__device__ __forceinline__ void half_colmax_block(const double2 *__restrict__ x, int N, int nmax,
+                                                  unsigned *__restrict__ colmax, int block) {
+    __shared__ unsigned sm[2 * kMaxLmax * (kMaxLmax + 2)];
+    for (int c = threadIdx.x; c < nmax; c += blockDim.x)
+        sm[c] = 0u;
+    __syncthreads();
+    const long e0 = static_cast<long>(block) * 256 * nmax;
+    const long e1 = min(static_cast<long>(N) * nmax, e0 + 256L * nmax);
+    for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const double2 v = x[e];
+        const float m = static_cast<float>(fmax(fabs(v.x), fabs(v.y)));
+        atomicMax(&sm[e % nmax], __float_as_uint(m));
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < nmax; c += blockDim.x)
+        if (sm[c])
+            atomicMax(&colmax[c], sm[c]);
+}
+
+// Step 2: xh terms of x 2^f per column (f = half_exponent of the column's
+// max), rows n2 = 0..nmax (row nmax and sources past N zero); xexp[n2] = f.
+// One thread per (row, source), sources fastest.
+template <bool SPLIT>
+__device__ __forceinline__ void half_convert_elem(const double2 *__restrict__ x, int N, int N32, int nmax,
+                                                  const unsigned *__restrict__ colmax, int *__restrict__ xexp,
+                                                  _Float16 *__restrict__ xhi, _Float16 *__restrict__ xlo,
+                                                  long idx) {
+    const long S32 = static_cast<long>(N32) * 32;
+    const int n2 = static_cast<int>(idx / S32);
+    const long s = idx - n2 * S32;
+    if (n2 > nmax)
+        return;
+    const int f = n2 < nmax ? half_exponent(__uint_as_float(colmax[n2])) : 0;
+    if (s == 0)
+        xexp[n2] = f;
+    double re = 0.0, im = 0.0;
+    if (n2 < nmax && s < N) {
+        const double2 v = x[s * nmax + n2];
+        re = ldexp(v.x, f);
+        im = ldexp(v.y, f);
+    }
+    const size_t plane = half_x_plane(nmax, N32);
+    const size_t o = static_cast<size_t>(n2) * S32 + s;
+    const _Float16 hr = static_cast<_Float16>(re), hm = static_cast<_Float16>(im);
+    xhi[o] = hr;
+    xhi[plane + o] = hm;
+    if constexpr (SPLIT) {
+        xlo[o] = static_cast<_Float16>(re - static_cast<double>(hr));
+        xlo[plane + o] = static_cast<_Float16>(im - static_cast<double>(hm));
+    }
+}
+
+// Order exponents e_p (p < P) of a cluster whose closest pair is
+// sqrt(dmin2) apart: |G_p| <= sqrt((2p+1)/2) |h_p(k d_min)|.
+__device__ __forceinline__ int half_order_exponent(int p, double k, double dmin2) {
+    const double x = k * sqrt(dmin2);
+    if (!(x > 0.0))
+        return 0;
+    double sx, cx;
+    sincos(x, &sx, &cx);
+    const double inv = 1.0 / x;
+    double ar = sx * inv, ai = -cx * inv;                      // h_0
+    double hr = (ar - cx) * inv, hi = (ai - sx) * inv;         // h_1
+    for (int q = 0; q < p; ++q) {
+        const double f = (2 * q + 3) * inv;
+        const double nr = f * hr - ar, ni = f * hi - ai;
+        ar = hr;
+        ai = hi;
+        hr = nr;
+        hi = ni;
+    }
+    return half_exponent(sqrt((2.0 * p + 1.0) / 2.0) * sqrt(ar * ar + ai * ai));
+}
+
+}  // namespace yasfhip

@@ -36,6 +36,7 @@
 #include <memory>
 
 #include "../../include/yasfhip.h"
+#include "half_gemm.hpp"
 #include "table_gemm.hpp"
 
 namespace yasfhip {
@@ -53,6 +54,15 @@ void launch_norm2(const double2 *x, long n, Reducer red, double *out, hipStream_
 void launch_mgs_step(double2 *w, const double2 *v_prev, const double2 *h_prev, const double2 *v_next,
                      long n, Reducer red, double *out, hipStream_t s);
 void launch_to_fp32(float *dst, const double *src, long count, hipStream_t s);
+// half.hip: the pair GEMM on the fp16 matrix cores
+size_t half_table_term_bytes(const DevicePlan &dp, int N);
+size_t half_x_bytes(const DevicePlan &dp, int N);
+void launch_half_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, void *table, bool split,
+                             int *pexp, unsigned long long *dmin, hipStream_t s, double near2);
+void launch_half_x(const DevicePlan &dp, int N, const double2 *x, void *xh, bool split, unsigned *colmax, int *xexp,
+                   hipStream_t s);
+void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void *xh, bool split, const int *pexp,
+                      const int *xexp, void *Q, hipStream_t s);
 // observables.hip: C_sca by quadrature of the far field
 void csca_quadrature_grid(double kR, int lmax, int *n_theta, int *n_phi);
 size_t csca_quadrature_partials(int n_theta, int n_phi);
@@ -98,8 +108,17 @@ struct BatchReq {
     const PairJob *jobs;
     const int *colmap, *crow, *cidx, *dmoff, *order_tiles;
     const double2 *cval;
+    // operators with an fp16-term table (the fp16 pair GEMM): the table, its
+    // order exponents, the job list it runs, and the request's fp16 copy of
+    // the basis vector with its column maxima and exponents
+    const _Float16 *htab;
+    const int *pexp;
+    const PairJob *hjobs;
+    _Float16 *xh;
+    unsigned *colmax;
+    int *xexp;
     long n;
-    int N, lmax, nmax, ntiles1, N4, nq, maxcols, chunks, restart;
+    int N, lmax, nmax, ntiles1, N4, nq, maxcols, chunks, restart, N32;
 };
 
 // Dynamic state, written by the device inside a cycle and by the host between.
@@ -148,6 +167,9 @@ __global__ __launch_bounds__(kVecThreads) void serve_prep_kernel(const BatchReq 
     const double s = st[r].inv_norm;
     double2 *v = q.V + static_cast<long>(st[r].j) * q.n;
     const long e0 = c * kChunk, e1 = min(e0 + kChunk, q.n);
+    if (q.colmax && c == 0)  // the column maxima of the fp16 copy start from zero
+        for (int i = threadIdx.x; i < q.nmax; i += kVecThreads)
+            q.colmax[i] = 0u;
     for (long e = e0 + threadIdx.x; e < e1; e += kVecThreads) {
         double2 a = v[e];
         a.x *= s;
@@ -155,6 +177,75 @@ __global__ __launch_bounds__(kVecThreads) void serve_prep_kernel(const BatchReq 
         v[e] = a;
         q.x32[e] = make_float2(static_cast<float>(a.x), static_cast<float>(a.y));
     }
+}
+
+// The fp16 pair GEMM of the requests whose operator holds an fp16-term
+// table, in three batched steps over the normalised V[j]: column maxima,
+// the fp16 terms, and the GEMM (complex64 Q as the complex64 GEMM leaves it).
+// Grids: per request, hs[] block ranges (find_owner), hidx[] its index.
+__global__ __launch_bounds__(256) void serve_half_colmax_kernel(const BatchReq *__restrict__ reqs,
+                                                                const ReqState *__restrict__ st,
+                                                                const int *__restrict__ hs,
+                                                                const int *__restrict__ hidx, int nh) {
+    const int h = find_owner(hs, nh, blockIdx.x);
+    const int r = uniform(hidx[h]);
+    if (st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    half_colmax_block(q.V + static_cast<long>(st[r].j) * q.n, q.N, q.nmax, q.colmax, blockIdx.x - hs[h]);
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void serve_half_convert_kernel(const BatchReq *__restrict__ reqs,
+                                                                 const ReqState *__restrict__ st,
+                                                                 const int *__restrict__ hs,
+                                                                 const int *__restrict__ hidx, int nh) {
+    const int h = find_owner(hs, nh, blockIdx.x);
+    const int r = uniform(hidx[h]);
+    if (st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    const long idx = (blockIdx.x - hs[h]) * 256L + threadIdx.x;
+    if (idx >= static_cast<long>(q.nmax + 1) * q.N32 * 32)
+        return;
+    _Float16 *xlo = q.xh + 2 * half_x_plane(q.nmax, q.N32);
+    half_convert_elem<SPLIT>(q.V + static_cast<long>(st[r].j) * q.n, q.N, q.N32, q.nmax, q.colmax, q.xexp, q.xh,
+                             xlo, idx);
+}
+
+constexpr int kServeHalfWaves = 8;
+constexpr int kServeHalfGrouping = 1;  // jobs of up to 4 column tiles
+constexpr int kServeHalfNtl = 4;
+
+template <bool SPLIT>
+__global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_gemm_kernel(const BatchReq *__restrict__ reqs,
+                                                                               const ReqState *__restrict__ st,
+                                                                               const int *__restrict__ hs,
+                                                                               const int *__restrict__ hidx, int nh) {
+    const int h = find_owner(hs, nh, blockIdx.x);
+    const int r = uniform(hidx[h]);
+    if (st[r].state != kArnoldi)
+        return;
+    const BatchReq &q = reqs[r];
+    const int b = blockIdx.x - hs[h];
+    const int jid = b / q.ntiles1;
+    const int t = b - jid * q.ntiles1;
+    const int dm = uniform(q.hjobs[jid].dm);
+    const int tile0 = uniform(q.hjobs[jid].tile0);
+    const int ntile = uniform(q.hjobs[jid].ntile);
+    const int qbase = uniform(q.hjobs[jid].qbase);
+    const int tstride = uniform(q.hjobs[jid].tstride);
+    const int nord = uniform(q.hjobs[jid].nord);
+    const int P = 2 * q.lmax + 1;
+    const int adm = dm < 0 ? -dm : dm;
+    const int d = dm + 2 * q.lmax;
+    const size_t f0 = (static_cast<size_t>(t) * P * P + q.dmoff[d]) * q.N32 * 1024;
+    const size_t term = static_cast<size_t>(q.ntiles1) * half_tile_halves(q.lmax, q.N32);
+    const size_t xplane = half_x_plane(q.nmax, q.N32);
+    float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
+    half_gemm_block<kServeHalfWaves, kServeHalfNtl, SPLIT>(
+        q.htab + f0, SPLIT ? q.htab + term + f0 : nullptr, q.xh, q.xh + 2 * xplane, xplane, q.N32, q.nmax,
+        P - adm, nord, ntile, q.colmap + d * q.maxcols + tile0 * 16, q.pexp + adm, q.xexp, Qt, tstride, tile0);
 }
 
 // One (job, target tile t) block of a request's complex64 pair GEMM; tab
@@ -600,6 +691,12 @@ struct SharedOp {
     void *slab = nullptr;
     int slab_tiles64 = 0;
     double kR = 0.0;  // k times the largest distance of a sphere from the origin (far-field quadrature)
+    // fp16-term table in place of the complex64 one (inner precision
+    // FP16X2 / FP16): its terms, order exponents and scratch
+    bool half = false, half_split = false;
+    void *htab = nullptr;
+    int *pexp = nullptr;
+    unsigned long long *dmin = nullptr;
 };
 
 // The active requests of one operator (and tile shape) whose GEMMs run as one
@@ -631,6 +728,9 @@ struct Request {
     void *block = nullptr;
     double2 *a_d = nullptr, *b_d = nullptr, *x_d = nullptr, *y_d = nullptr, *acoef_d = nullptr;
     float2 *x32 = nullptr;
+    void *xh = nullptr;  // fp16 terms of the basis vector (operators with an fp16 table)
+    unsigned *colmax = nullptr;
+    int *xexp = nullptr;
     void *Q = nullptr;
     double2 *V = nullptr, *partial = nullptr, *H = nullptr, *h1 = nullptr, *h2 = nullptr, *g = nullptr,
             *gs = nullptr;
@@ -689,6 +789,9 @@ struct yasfhip_server {
     int table_policy = 1;
     int slab_tiles_req = 0;  // FP64 target tiles per slab (0: an eighth of the arena)
     int csca_method = YASFHIP_CSCA_FAR_FIELD;  // how completions compute C_sca
+    int inner_precision = YASFHIP_FP32;         // pair GEMM of the inner GMRES iterations
+    std::vector<int> half_reqs;                 // active requests whose operator has an fp16 table
+    int half_index = 0, half_colmax_blocks = 0, half_convert_blocks = 0, half_gemm_blocks = 0;
     std::deque<Request *> finished;
     long long completed = 0;
     // batched launch data
@@ -771,7 +874,11 @@ struct yasfhip_server {
         const size_t o_pos = aligned(off, sizeof(double4) * o.ntiles1 * 16);
         const size_t o_t = aligned(off, sizeof(double2) * o.n);
         const size_t o_sph = o.device_setup ? aligned(off, sizeof(double4) * o.N) : 0;
-        const size_t o_tab = o.table32 ? aligned(off, pair_table_bytes(dp, o.N, true)) : 0;
+        const size_t tab_bytes = o.half ? half_table_term_bytes(dp, o.N) * (o.half_split ? 2 : 1)
+                                        : pair_table_bytes(dp, o.N, true);
+        const size_t o_tab = o.table32 ? aligned(off, tab_bytes) : 0;
+        const size_t o_pexp = o.half ? aligned(off, sizeof(int) * (2 * o.L + 1)) : 0;
+        const size_t o_dmin = o.half ? aligned(off, sizeof(unsigned long long)) : 0;
         const size_t o_tab64 = o.table64 ? aligned(off, pair_table_bytes(dp, o.N, false)) : 0;
         const size_t o_slab =
             o.slab_tiles64 ? aligned(off, pair_table_tile_bytes(dp, o.N, false) * o.slab_tiles64) : 0;
@@ -780,7 +887,10 @@ struct yasfhip_server {
             o.pos_d = reinterpret_cast<double4 *>(b + o_pos);
             o.t_d = reinterpret_cast<double2 *>(b + o_t);
             o.sph_d = o.device_setup ? reinterpret_cast<double4 *>(b + o_sph) : nullptr;
-            o.table = o.table32 ? reinterpret_cast<float2 *>(b + o_tab) : nullptr;
+            o.table = o.table32 && !o.half ? reinterpret_cast<float2 *>(b + o_tab) : nullptr;
+            o.htab = o.table32 && o.half ? b + o_tab : nullptr;
+            o.pexp = o.half ? reinterpret_cast<int *>(b + o_pexp) : nullptr;
+            o.dmin = o.half ? reinterpret_cast<unsigned long long *>(b + o_dmin) : nullptr;
             o.tab64 = o.table64 ? reinterpret_cast<double2 *>(b + o_tab64) : nullptr;
             o.slab = o.slab_tiles64 ? b + o_slab : nullptr;
         }
@@ -798,6 +908,10 @@ struct yasfhip_server {
         const size_t o_y = take(sizeof(double2) * R1);
         const size_t o_ac = q.device_setup ? take(sizeof(double2) * q.nmax) : 0;
         const size_t o_x32 = take(sizeof(float2) * q.n);
+        const bool half = q.op && q.op->half;
+        const size_t o_xh = half ? take(half_x_bytes(dp, q.N)) : 0;
+        const size_t o_cm = half ? take(sizeof(unsigned) * q.nmax) : 0;
+        const size_t o_xe = half ? take(sizeof(int) * (q.nmax + 1)) : 0;
         const size_t o_Q = take(sizeof(double2) * q.ntiles1 * dp.nq * 256);
         const size_t o_V = take(sizeof(double2) * R1 * q.n);
         const size_t o_part = take(sizeof(double2) * R1 * q.chunks);
@@ -814,6 +928,9 @@ struct yasfhip_server {
             q.y_d = reinterpret_cast<double2 *>(b + o_y);
             q.acoef_d = q.device_setup ? reinterpret_cast<double2 *>(b + o_ac) : nullptr;
             q.x32 = reinterpret_cast<float2 *>(b + o_x32);
+            q.xh = half ? b + o_xh : nullptr;
+            q.colmax = half ? reinterpret_cast<unsigned *>(b + o_cm) : nullptr;
+            q.xexp = half ? reinterpret_cast<int *>(b + o_xe) : nullptr;
             q.Q = b + o_Q;
             q.V = reinterpret_cast<double2 *>(b + o_V);
             q.partial = reinterpret_cast<double2 *>(b + o_part);
@@ -845,6 +962,8 @@ struct yasfhip_server {
         o->n = q.n;
         o->k = q.k;
         o->device_setup = q.device_setup;
+        o->half = inner_precision == YASFHIP_FP16X2 || inner_precision == YASFHIP_FP16;
+        o->half_split = inner_precision == YASFHIP_FP16X2;
         double r2 = 0.0;
         for (const double4 &p : q.pos)
             r2 = std::max(r2, p.x * p.x + p.y * p.y + p.z * p.z);
@@ -892,6 +1011,8 @@ struct yasfhip_server {
             attempt(false, false, 0, cap);
         if (!o->block)
             return nullptr;
+        if (!o->table32)
+            o->half = false;  // the fp16 form replaces the complex64 table only
         layout_op(*o, true);
         hipStream_t st = stream;
         YH_CHECK(hipMemcpyAsync(o->pos_d, q.pos.data(), sizeof(double4) * q.pos.size(), hipMemcpyHostToDevice,
@@ -903,7 +1024,9 @@ struct yasfhip_server {
         } else {
             YH_CHECK(hipMemcpyAsync(o->t_d, q.t.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
         }
-        if (o->table32)
+        if (o->table32 && o->half)
+            launch_half_table_build(dp, o->pos_d, o->N, o->k, o->htab, o->half_split, o->pexp, o->dmin, st, kAllPairs);
+        else if (o->table32)
             launch_pair_table_build(dp, o->pos_d, o->N, o->k, true, o->table, st);
         if (o->table64)
             launch_pair_table_build(dp, o->pos_d, o->N, o->k, false, o->tab64, st);
@@ -966,8 +1089,11 @@ struct yasfhip_server {
                 op = create_op(q);
                 created = op != nullptr;
             }
-            if (op)
+            if (op) {
+                q.op = op;  // layout() reads the operator's form
                 q.block = arena.alloc(layout(q, false));
+                q.op = nullptr;
+            }
             if (!op || !q.block) {
                 if (created) {
                     ++op->refs;
@@ -1055,7 +1181,10 @@ struct yasfhip_server {
                 continue;
             const SharedOp &o = *q->op;
             const DevicePlan &dp = plan_for(q->L).dp;
-            if (o.table32) {
+            if (o.table32 && o.half) {
+                launch_half_x(dp, q->N, q->x_d, q->xh, o.half_split, q->colmax, q->xexp, st);
+                launch_half_gemm(dp, q->N, o.htab, q->xh, o.half_split, o.pexp, q->xexp, q->Q, st);
+            } else if (o.table32) {
                 launch_pair_table_gemm(dp, q->N, o.table, q->x32, true, kServeShape32, q->Q, st);
             } else if (o.slab) {  // complex64 slabs (twice the FP64 tiles), one build for the operator's requests
                 const int tiles = 2 * o.slab_tiles64;
@@ -1305,6 +1434,14 @@ struct yasfhip_server {
             b.dmoff = dp.dmoff;
             b.order_tiles = dp.order_tiles;
             b.cval = dp.cval;
+            const bool half = q.op->half;
+            b.htab = half ? static_cast<const _Float16 *>(q.op->htab) : nullptr;
+            b.pexp = half ? q.op->pexp : nullptr;
+            b.hjobs = dp.tjobs[kServeHalfGrouping];
+            b.xh = half ? static_cast<_Float16 *>(q.xh) : nullptr;
+            b.colmax = half ? q.colmax : nullptr;
+            b.xexp = half ? q.xexp : nullptr;
+            b.N32 = (q.N + 31) / 32;
             b.n = q.n;
             b.N = q.N;
             b.lmax = q.L;
@@ -1330,7 +1467,7 @@ struct yasfhip_server {
             int cnt = 0, blocks = 0;
             for (int r = 0; r < nreq; ++r) {
                 Request &q = *active[r];
-                if (q.shape != gi || !q.op->table32 || op_group_size(r) > 1)
+                if (q.shape != gi || !q.op->table32 || q.op->half || op_group_size(r) > 1)
                     continue;
                 gstart[cnt] = blocks;
                 gidx[cnt] = r;
@@ -1341,12 +1478,42 @@ struct yasfhip_server {
             grp_count[gi] = cnt;
             grp_blocks[gi] = blocks;
         }
+        // requests over an fp16-term table: block ranges of the three
+        // batched steps (column maxima, fp16 terms, GEMM) and their indices
+        half_reqs.clear();
+        for (int r = 0; r < nreq; ++r)
+            if (active[r]->op->half)
+                half_reqs.push_back(r);
+        {
+            const int nh = static_cast<int>(half_reqs.size());
+            half_index = static_cast<int>(h_index.size());
+            h_index.resize(h_index.size() + 4 * (nh + 1), 0);
+            int *hc = h_index.data() + half_index, *hv = hc + nh + 1, *hg = hv + nh + 1, *hi = hg + nh + 1;
+            int bc = 0, bv = 0, bg = 0;
+            for (int h = 0; h < nh; ++h) {
+                const Request &q = *active[half_reqs[h]];
+                const int N32 = (q.N + 31) / 32;
+                hc[h] = bc;
+                hv[h] = bv;
+                hg[h] = bg;
+                hi[h] = half_reqs[h];
+                bc += (q.N + 255) / 256;
+                bv += static_cast<int>((static_cast<long>(q.nmax + 1) * N32 * 32 + 255) / 256);
+                bg += plan_for(q.L).dp.ntjobs[kServeHalfGrouping] * q.ntiles1;
+            }
+            hc[nh] = bc;
+            hv[nh] = bv;
+            hg[nh] = bg;
+            half_colmax_blocks = bc;
+            half_convert_blocks = bv;
+            half_gemm_blocks = bg;
+        }
         otf.clear();
         slab_groups.clear();
         for (int r = 0; r < nreq; ++r) {
             SharedOp *o = active[r]->op;
-            if (o->table32 && op_group_size(r) < 2)
-                continue;  // in the batched GEMM above
+            if (o->half || (o->table32 && op_group_size(r) < 2))
+                continue;  // in the batched GEMMs above
             if (!o->table32 && !o->slab) {
                 otf.push_back(r);
                 continue;
@@ -1479,6 +1646,25 @@ struct yasfhip_server {
             const int *gstart = d_index + 2 * (nreq + 1) + gi * 2 * (nreq + 1);
             const int *gidx = gstart + nreq + 1;
             launch_gemm(gi, gstart, gidx, grp_count[gi], grp_blocks[gi]);
+        }
+        if (!half_reqs.empty()) {  // the fp16 pair GEMM of requests over fp16-term tables
+            const int nh = static_cast<int>(half_reqs.size());
+            const int *hc = d_index + half_index, *hv = hc + nh + 1, *hg = hv + nh + 1, *hi = hg + nh + 1;
+            const bool split = inner_precision == YASFHIP_FP16X2;
+            hipLaunchKernelGGL(serve_half_colmax_kernel, dim3(half_colmax_blocks), dim3(256), 0, st, d_reqs, d_state,
+                               hc, hi, nh);
+            if (split) {
+                hipLaunchKernelGGL(serve_half_convert_kernel<true>, dim3(half_convert_blocks), dim3(256), 0, st,
+                                   d_reqs, d_state, hv, hi, nh);
+                hipLaunchKernelGGL(serve_half_gemm_kernel<true>, dim3(half_gemm_blocks), dim3(64 * kServeHalfWaves),
+                                   0, st, d_reqs, d_state, hg, hi, nh);
+            } else {
+                hipLaunchKernelGGL(serve_half_convert_kernel<false>, dim3(half_convert_blocks), dim3(256), 0, st,
+                                   d_reqs, d_state, hv, hi, nh);
+                hipLaunchKernelGGL(serve_half_gemm_kernel<false>, dim3(half_gemm_blocks), dim3(64 * kServeHalfWaves),
+                                   0, st, d_reqs, d_state, hg, hi, nh);
+            }
+            YH_CHECK(hipGetLastError());
         }
         // operator groups: the requests' GEMMs block-interleaved over the
         // shared table, or over each slab, built once for all of them
@@ -1848,6 +2034,16 @@ int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int co
                    "unsupported tile shape for this lmax (waves 2/4/8 x column tiles 2/4/8 as listed in "
                    "yasfhip.h, with ceil((2 lmax + 1) / waves) orders per wave within range)");
         s->shape_for_lmax[lmax][0] = s->shape_for_lmax[lmax][1] = found;
+    });
+}
+
+int yasfhip_server_set_inner_precision(yasfhip_server *s, int precision) {
+    return sguard([&] {
+        YH_REQUIRE(s, "server is NULL");
+        YH_REQUIRE(precision == YASFHIP_FP32 || precision == YASFHIP_FP16X2 || precision == YASFHIP_FP16,
+                   "inner precision must be YASFHIP_FP32, YASFHIP_FP16X2 or YASFHIP_FP16");
+        YH_REQUIRE(s->queue.empty() && s->active.empty(), "the inner precision changes on an idle server only");
+        s->inner_precision = precision;
     });
 }
 

@@ -140,6 +140,8 @@ def serve(server, requests: list[ServeRequest], rtol: float = 1e-6, keep_solutio
     arrival; solves/s = requests / (last completion - start).  device_setup:
     submit radii, refractive index and incidence, and let the server set up t
     and a_inc on the device (TE plane wave in vacuum, as the workloads here)."""
+    if len({r.index for r in requests}) != len(requests):
+        raise ValueError("request indices must be distinct (results are keyed by index)")
     order = sorted(requests, key=lambda r: r.arrival_s)
     ids = {}
     results = {}
