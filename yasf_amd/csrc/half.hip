@@ -120,6 +120,7 @@ __global__ __launch_bounds__(64 * WAVES) void half_gemm_kernel(
     const int qbase = uniform(jobs[jid].qbase);
     const int tstride = uniform(jobs[jid].tstride);
     const int nord = uniform(jobs[jid].nord);
+    const int ip0 = uniform(jobs[jid].pad);  // first order of the job's order range
     const int P = 2 * lmax + 1;
     const int adm = dm < 0 ? -dm : dm;
     const int R = P - adm;
@@ -128,13 +129,15 @@ __global__ __launch_bounds__(64 * WAVES) void half_gemm_kernel(
     const size_t xplane = half_x_plane(nmax, N32);
     float *Qt = Q + 2 * (static_cast<size_t>(t0 + t) * nq + qbase) * 256;
     half_gemm_block<WAVES, NTL, SPLIT>(hi_tab + f0, SPLIT ? lo_tab + f0 : nullptr, xhi, xlo, xplane, N32, nmax, R,
-                                       nord, ntile, colmap + d * maxcols + tile0 * 16, pexp + adm, xexp, Qt,
+                                       ip0, nord, ntile, colmap + d * maxcols + tile0 * 16, pexp + adm, xexp, Qt,
                                        tstride, tile0);
 }
 
-constexpr int kHalfWaves = 8;
-constexpr int kHalfGrouping = 1;  // jobs of up to 4 column tiles
-constexpr int kHalfNtl = 4;
+constexpr int kHalfWaves = 8;  // 2 orders per wave: kHalfJobOrders
+static_assert(2 * kHalfWaves == kHalfJobOrders, "orders per fp16 job = 2 per wave");
+// Column tiles per job: 4 for the two-term GEMM (its registers hold no
+// more), 8 for the one-term GEMM (each table slice streamed once).
+constexpr int half_job_tiles(bool split) { return split ? 4 : 8; }
 
 }  // namespace
 
@@ -194,14 +197,15 @@ void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void
     const _Float16 *lo = hi + half_table_term_bytes(dp, N) / sizeof(_Float16);
     const _Float16 *xhi = static_cast<const _Float16 *>(xh);
     const _Float16 *xlo = xhi + 2 * half_x_plane(dp.nmax, N32);
-    const dim3 grid(dp.ntjobs[kHalfGrouping] * ntiles1), block(64 * kHalfWaves);
+    const int h = split ? 0 : 1;
+    const dim3 grid(dp.nhjobs[h] * ntiles1), block(64 * kHalfWaves);
     if (split)
-        hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, kHalfNtl, true>), grid, block, 0, s, hi, lo, xhi, xlo, pexp,
-                           xexp, N32, dp.nmax, dp.lmax, dp.tjobs[kHalfGrouping], ntiles1, 0, dp.colmap, dp.maxcols,
+        hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, half_job_tiles(true), true>), grid, block, 0, s, hi, lo, xhi,
+                           xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntiles1, 0, dp.colmap, dp.maxcols,
                            dp.dmoff, static_cast<float *>(Q), dp.nq);
     else
-        hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, kHalfNtl, false>), grid, block, 0, s, hi, lo, xhi, xlo,
-                           pexp, xexp, N32, dp.nmax, dp.lmax, dp.tjobs[kHalfGrouping], ntiles1, 0, dp.colmap,
+        hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, half_job_tiles(false), false>), grid, block, 0, s, hi, lo,
+                           xhi, xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntiles1, 0, dp.colmap,
                            dp.maxcols, dp.dmoff, static_cast<float *>(Q), dp.nq);
     YH_CHECK(hipGetLastError());
 }

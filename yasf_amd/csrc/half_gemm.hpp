@@ -62,28 +62,40 @@ __device__ __forceinline__ f4_t mfma_h16(h8_t a, h8_t b, f4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
+typedef unsigned u4_t __attribute__((ext_vector_type(4)));
+
+// -a by flipping the sign bits (four 32-bit XORs)
+__device__ __forceinline__ h8_t neg_h8(h8_t a) {
+    return __builtin_bit_cast(h8_t, __builtin_bit_cast(u4_t, a) ^ u4_t{0x80008000u, 0x80008000u, 0x80008000u,
+                                                                        0x80008000u});
+}
+
 // One (job, target tile) block of WAVES waves.  hi / lo point at the job's
 // (tile, dm) fragment 0 of the two terms (lo unused unless SPLIT); xhi / xlo
 // at the two terms' plane 0 of the source copy (plane 1 = Im follows at
 // xplane halves); cols: the job's 16 ntile source columns (-1: padding);
-// orders p = |dm| + ip, ip < nord, dealt to the waves in turn; pexp: order
-// exponents from p = |dm| on; xexp[n2] (xexp[nmax] = 0).  C tile (ip, it)
-// goes to Q slot ip * tstride + tile0 + it past Qt (complex64).
+// orders p = |dm| + ip for ip in [ip0, ip0 + nord), nord <= 2 WAVES, dealt
+// to the waves in turn (at most two each); pexp: order exponents from
+// p = |dm| on; xexp[n2] (xexp[nmax] = 0).  C tile (ip, it) goes to Q slot
+// ip * tstride + tile0 + it past Qt (complex64).
 //
-// Pipelining: the B fragments of a K-step (the job's column tiles, shared by
-// all waves) are loaded by the whole block, one K-step ahead, into one of two
-// LDS buffers; each wave loads its orders' A fragments one K-step ahead into
-// registers.  Loads are unconditional (the last K-step reloads itself), so
-// the compiler's waits leave the next step's loads in flight during the
-// MFMAs.  One barrier per K-step hands the LDS buffers over.
+// Pipelining: the table stream (the operand that comes from HBM) is loaded
+// two K-steps ahead into three register sets used in turn; the B fragments of
+// a K-step (the job's column tiles, shared by all waves) are loaded by the
+// whole block one K-step ahead into one of two LDS buffers.  Operands come
+// through raw buffer loads: a wave's second order slot, when it has none,
+// and the B pieces of absent column tiles read out of range, which returns
+// zero without touching memory, so every load is unconditional and the
+// compiler's waits stay static.  One barrier per K-step hands the LDS
+// buffers over.
 template <int WAVES, int NTL, bool SPLIT>
 __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_tab, const _Float16 *__restrict__ lo_tab,
                                                 const _Float16 *__restrict__ xhi, const _Float16 *__restrict__ xlo,
-                                                size_t xplane, int N32, int nmax, int R, int nord, int ntile,
-                                                const int *__restrict__ cols, const int *__restrict__ pexp,
+                                                size_t xplane, int N32, int nmax, int R, int ip0, int nord,
+                                                int ntile, const int *__restrict__ cols, const int *__restrict__ pexp,
                                                 const int *__restrict__ xexp, float *__restrict__ Qt, int tstride,
                                                 int tile0) {
-    constexpr int NIP = (2 * kMaxLmax + 1 + WAVES - 1) / WAVES;
+    constexpr int NIP = 2;
     constexpr int NPL = SPLIT ? 4 : 2;  // planes: hi Re, hi Im, lo Re, lo Im
     constexpr int THREADS = WAVES * 64;
     constexpr int CHUNKS = NTL * NPL * 64;  // 16-byte B pieces of one K-step
@@ -93,23 +105,29 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
     const int lane = tid & 63, lo = lane & 15, hi = lane >> 4;
     const int wave = uniform(tid >> 6);
     const size_t S32 = static_cast<size_t>(N32) * 32;
-    // this thread's B pieces: source pointer (K-step 0) and LDS slot
-    const _Float16 *bsrc[PER];
+    // B pieces of this thread: byte offset (K-step 0) into the fp16 source
+    // copy (xhi .. 4 planes) and LDS slot
+    const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(xhi, 4 * xplane * sizeof(_Float16));
+    unsigned boff[PER];
     int bslot[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int q = tid + u * THREADS;
         const int it = q / (NPL * 64), pl = (q / 64) % NPL, ln = q % 64;
         const int n2 = (q < CHUNKS && it < ntile) ? cols[it * 16 + (ln & 15)] : -1;
-        const _Float16 *plane = (pl < 2 ? xhi : xlo) + (pl & 1) * xplane;
-        bsrc[u] = plane + static_cast<size_t>(n2 < 0 ? nmax : n2) * S32 + (ln >> 4) * 8;
+        const size_t plane = (pl < 2 ? 0 : 2 * xplane) + (pl & 1) * xplane;
+        boff[u] = n2 < 0 ? kBufferNull
+                         : static_cast<unsigned>((plane + static_cast<size_t>(n2) * S32 + (ln >> 4) * 8) *
+                                                 sizeof(_Float16));
         bslot[u] = q < CHUNKS ? q : -1;
     }
+    (void)xlo;  // the lo terms follow the hi terms in one allocation
     h8_t breg[PER];
     auto load_b = [&](int ks) {
+        const unsigned so = static_cast<unsigned>(ks) * 32 * sizeof(_Float16);
 #pragma unroll
         for (int u = 0; u < PER; ++u)
-            breg[u] = *reinterpret_cast<const h8_t *>(bsrc[u] + static_cast<size_t>(ks) * 32);
+            breg[u] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rx, boff[u], so, 0));
     };
     auto store_b = [&](int buf) {
 #pragma unroll
@@ -117,22 +135,27 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
             if (bslot[u] >= 0)
                 (&bsh[buf][0][0][0])[bslot[u]] = breg[u];
     };
-    // this wave's orders (past nord: the last order reloaded, its MFMAs skipped)
-    int ipw[NIP];
+    // this wave's orders: ip0 + wave and ip0 + wave + WAVES (absent: null offset)
+    const size_t slice = static_cast<size_t>(N32) * R * 1024 * sizeof(_Float16);  // bytes of one term's slice
+    const __amdgpu_buffer_rsrc_t rh = buffer_rsrc(hi_tab, slice);
+    const __amdgpu_buffer_rsrc_t rl = buffer_rsrc(SPLIT ? lo_tab : hi_tab, slice);
+    unsigned aoff[NIP];
 #pragma unroll
-    for (int j = 0; j < NIP; ++j)
-        ipw[j] = min(wave + j * WAVES, nord - 1);
-    const size_t fstride = 2 * 512;  // halves per fragment (Re and Im planes)
-    h8_t acur[NIP][NPL], anext[NIP][NPL];
+    for (int j = 0; j < NIP; ++j) {
+        const int ip = ip0 + wave + j * WAVES;
+        aoff[j] = wave + j * WAVES < nord ? static_cast<unsigned>((ip * 1024 + lane * 8) * sizeof(_Float16))
+                                          : kBufferNull;
+    }
+    const unsigned kstride = static_cast<unsigned>(R) * 1024 * sizeof(_Float16);  // bytes per K-step
     auto load_a = [&](int ks, h8_t (&a)[NIP][NPL]) {
+        const unsigned so = static_cast<unsigned>(ks) * kstride;
 #pragma unroll
         for (int j = 0; j < NIP; ++j) {
-            const size_t fo = (static_cast<size_t>(ks) * R + ipw[j]) * fstride + lane * 8;
-            a[j][0] = *reinterpret_cast<const h8_t *>(hi_tab + fo);
-            a[j][1] = *reinterpret_cast<const h8_t *>(hi_tab + fo + 512);
+            a[j][0] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rh, aoff[j], so, 0));
+            a[j][1] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rh, aoff[j] + 1024, so, 0));
             if constexpr (SPLIT) {
-                a[j][2] = *reinterpret_cast<const h8_t *>(lo_tab + fo);
-                a[j][3] = *reinterpret_cast<const h8_t *>(lo_tab + fo + 512);
+                a[j][2] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rl, aoff[j], so, 0));
+                a[j][3] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rl, aoff[j] + 1024, so, 0));
             }
         }
     };
@@ -142,61 +165,102 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
 #pragma unroll
         for (int it = 0; it < NTL; ++it)
             acc[j][it][0] = acc[j][it][1] = f4_t{0.f, 0.f, 0.f, 0.f};
-    load_b(0);
-    load_a(0, acur);
-    store_b(0);
-    __syncthreads();
-    for (int ks = 0; ks < N32; ++ks) {
-        const int kn = ks + 1 < N32 ? ks + 1 : ks;
-        load_b(kn);
-        load_a(kn, anext);
-        const int buf = ks & 1;
+    // MFMAs of one K-step from the A fragments a and buffer buf of the B
+    // fragments (Im planes negated per column tile, for the real parts)
+    auto step = [&](const h8_t (&a)[NIP][NPL], int buf) {
 #pragma unroll
         for (int it = 0; it < NTL; ++it) {
             if (it >= ntile)
                 continue;
-            h8_t b[NPL];
+            h8_t b[NPL], nb[NPL / 2];
 #pragma unroll
             for (int pl = 0; pl < NPL; ++pl)
                 b[pl] = bsh[buf][it][pl][lane];
+            nb[0] = neg_h8(b[1]);
+            if constexpr (SPLIT)
+                nb[1] = neg_h8(b[3]);
 #pragma unroll
             for (int j = 0; j < NIP; ++j) {
                 if (wave + j * WAVES >= nord)
                     continue;
-                const h8_t ar = acur[j][0], ai = acur[j][1], nai = -acur[j][1];
                 f4_t cr = acc[j][it][0], ci = acc[j][it][1];
-                cr = mfma_h16(ar, b[0], cr);
-                cr = mfma_h16(nai, b[1], cr);
-                ci = mfma_h16(ar, b[1], ci);
-                ci = mfma_h16(ai, b[0], ci);
+                cr = mfma_h16(a[j][0], b[0], cr);
+                ci = mfma_h16(a[j][0], b[1], ci);
+                cr = mfma_h16(a[j][1], nb[0], cr);
+                ci = mfma_h16(a[j][1], b[0], ci);
                 if constexpr (SPLIT) {
-                    const h8_t lr = acur[j][2], li = acur[j][3], nli = -acur[j][3];
-                    cr = mfma_h16(ar, b[2], cr);
-                    cr = mfma_h16(nai, b[3], cr);
-                    cr = mfma_h16(lr, b[0], cr);
-                    cr = mfma_h16(nli, b[1], cr);
-                    ci = mfma_h16(ar, b[3], ci);
-                    ci = mfma_h16(ai, b[2], ci);
-                    ci = mfma_h16(lr, b[1], ci);
-                    ci = mfma_h16(li, b[0], ci);
+                    cr = mfma_h16(a[j][0], b[2], cr);
+                    ci = mfma_h16(a[j][0], b[3], ci);
+                    cr = mfma_h16(a[j][1], nb[1], cr);
+                    ci = mfma_h16(a[j][1], b[2], ci);
+                    cr = mfma_h16(a[j][2], b[0], cr);
+                    ci = mfma_h16(a[j][2], b[1], ci);
+                    cr = mfma_h16(a[j][3], nb[0], cr);
+                    ci = mfma_h16(a[j][3], b[0], ci);
                 }
                 acc[j][it][0] = cr;
                 acc[j][it][1] = ci;
             }
         }
-        store_b(buf ^ 1);
+    };
+    // K-steps in threes over three A register sets (loaded two steps ahead);
+    // K-steps past the last reload the last one (the loads stay unconditional)
+    h8_t a0[NIP][NPL], a1[NIP][NPL], a2[NIP][NPL];
+    auto kc = [&](int k) { return k < N32 ? k : N32 - 1; };
+    load_b(0);
+    load_a(0, a0);
+    load_a(kc(1), a1);
+    store_b(0);
+    __syncthreads();
+    for (int ks = 0; ks < N32; ks += 3) {
+        load_b(kc(ks + 1));
+        load_a(kc(ks + 2), a2);
+        step(a0, 0);
+        store_b(1);
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < NIP; ++j)
-#pragma unroll
-            for (int pl = 0; pl < NPL; ++pl)
-                acur[j][pl] = anext[j][pl];
+        if (ks + 1 >= N32)
+            break;
+        load_b(kc(ks + 2));
+        load_a(kc(ks + 3), a0);
+        step(a1, 1);
+        store_b(0);
+        __syncthreads();
+        if (ks + 2 >= N32)
+            break;
+        load_b(kc(ks + 3));
+        load_a(kc(ks + 4), a1);
+        step(a2, 0);
+        store_b(1);
+        __syncthreads();
+        // buffers: the next round starts on LDS buffer 1; swap the roles
+        if (ks + 3 >= N32)
+            break;
+        load_b(kc(ks + 4));
+        load_a(kc(ks + 5), a2);
+        step(a0, 1);
+        store_b(0);
+        __syncthreads();
+        if (ks + 4 >= N32)
+            break;
+        load_b(kc(ks + 5));
+        load_a(kc(ks + 6), a0);
+        step(a1, 0);
+        store_b(1);
+        __syncthreads();
+        if (ks + 5 >= N32)
+            break;
+        load_b(kc(ks + 6));
+        load_a(kc(ks + 7), a1);
+        step(a2, 1);
+        store_b(0);
+        __syncthreads();
+        ks += 3;  // six K-steps per round: LDS buffers and A sets both back to their start
     }
 #pragma unroll
     for (int j = 0; j < NIP; ++j) {
-        const int ip = wave + j * WAVES;
-        if (ip >= nord)
+        if (wave + j * WAVES >= nord)
             continue;
+        const int ip = ip0 + wave + j * WAVES;
 #pragma unroll
         for (int it = 0; it < NTL; ++it)
             if (it < ntile) {

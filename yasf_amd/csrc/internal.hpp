@@ -89,6 +89,7 @@ constexpr int kWavesPerJob = 4;
 constexpr int kWaveMaxTiles = 8;                                // C tiles per wave
 constexpr int kJobMaxTiles = kWavesPerJob * kWaveMaxTiles;      // C tiles per job
 constexpr int kJobMaxColTiles = 2;                              // column tiles per job
+constexpr int kHalfJobOrders = 16;  // orders per fp16 GEMM job (8 waves x 2)
 
 // Everything the coupling kernels need for one lmax.
 struct CouplingPlan {
@@ -104,6 +105,9 @@ struct CouplingPlan {
     std::vector<double> leg_beta;     // [(2L+1)^2] Legendre recurrence beta(l, m)
     std::vector<double> leg_diag;     // [2L+1] P_m^m = leg_diag[m] * sin^m
     std::vector<int> order_tiles;     // [ndm][2L+1]: column tiles (prefix) of the dm that order p feeds
+    // fp16 GEMM jobs of up to 4 and of up to 8 column tiles (pad = first
+    // order of the job's order range, at most kHalfJobOrders orders)
+    std::vector<PairJob> half_jobs[2];
     long long pair_macs_per_pair = 0; // complex MACs the on-the-fly pair kernel issues per pair
     long long nnz_ab5 = 0;
 };
@@ -122,6 +126,8 @@ struct DevicePlan {
     double *la, *lb, *ld;
     int *dmoff;  // [ndm] first order slot of each dm in a pair-table target tile
     int *order_tiles;  // [ndm][2L+1] column tiles that order p feeds (a prefix of the dm's tiles)
+    PairJob *hjobs[2];  // fp16 GEMM jobs of up to 4 / 8 column tiles
+    int nhjobs[2];
 };
 
 // Records the message of the calling thread's last failure (yasfhip_last_error)

@@ -113,7 +113,7 @@ struct BatchReq {
     // the basis vector with its column maxima and exponents
     const _Float16 *htab;
     const int *pexp;
-    const PairJob *hjobs;
+    const PairJob *hjobs[2];
     _Float16 *xh;
     unsigned *colmax;
     int *xexp;
@@ -213,9 +213,9 @@ __global__ __launch_bounds__(256) void serve_half_convert_kernel(const BatchReq 
                              xlo, idx);
 }
 
-constexpr int kServeHalfWaves = 8;
-constexpr int kServeHalfGrouping = 1;  // jobs of up to 4 column tiles
-constexpr int kServeHalfNtl = 4;
+constexpr int kServeHalfWaves = 8;  // 2 orders per wave (the plan's fp16 jobs)
+// column tiles per fp16 job: 4 with two terms, 8 with one (the plan's job lists 0, 1)
+constexpr int serve_half_tiles(bool split) { return split ? 4 : 8; }
 
 template <bool SPLIT>
 __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_gemm_kernel(const BatchReq *__restrict__ reqs,
@@ -230,12 +230,14 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_gemm_kernel(c
     const int b = blockIdx.x - hs[h];
     const int jid = b / q.ntiles1;
     const int t = b - jid * q.ntiles1;
-    const int dm = uniform(q.hjobs[jid].dm);
-    const int tile0 = uniform(q.hjobs[jid].tile0);
-    const int ntile = uniform(q.hjobs[jid].ntile);
-    const int qbase = uniform(q.hjobs[jid].qbase);
-    const int tstride = uniform(q.hjobs[jid].tstride);
-    const int nord = uniform(q.hjobs[jid].nord);
+    const PairJob *hjobs = q.hjobs[SPLIT ? 0 : 1];
+    const int dm = uniform(hjobs[jid].dm);
+    const int tile0 = uniform(hjobs[jid].tile0);
+    const int ntile = uniform(hjobs[jid].ntile);
+    const int qbase = uniform(hjobs[jid].qbase);
+    const int tstride = uniform(hjobs[jid].tstride);
+    const int nord = uniform(hjobs[jid].nord);
+    const int ip0 = uniform(hjobs[jid].pad);
     const int P = 2 * q.lmax + 1;
     const int adm = dm < 0 ? -dm : dm;
     const int d = dm + 2 * q.lmax;
@@ -243,9 +245,9 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_gemm_kernel(c
     const size_t term = static_cast<size_t>(q.ntiles1) * half_tile_halves(q.lmax, q.N32);
     const size_t xplane = half_x_plane(q.nmax, q.N32);
     float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
-    half_gemm_block<kServeHalfWaves, kServeHalfNtl, SPLIT>(
+    half_gemm_block<kServeHalfWaves, serve_half_tiles(SPLIT), SPLIT>(
         q.htab + f0, SPLIT ? q.htab + term + f0 : nullptr, q.xh, q.xh + 2 * xplane, xplane, q.N32, q.nmax,
-        P - adm, nord, ntile, q.colmap + d * q.maxcols + tile0 * 16, q.pexp + adm, q.xexp, Qt, tstride, tile0);
+        P - adm, ip0, nord, ntile, q.colmap + d * q.maxcols + tile0 * 16, q.pexp + adm, q.xexp, Qt, tstride, tile0);
 }
 
 // One (job, target tile t) block of a request's complex64 pair GEMM; tab
@@ -1437,7 +1439,8 @@ struct yasfhip_server {
             const bool half = q.op->half;
             b.htab = half ? static_cast<const _Float16 *>(q.op->htab) : nullptr;
             b.pexp = half ? q.op->pexp : nullptr;
-            b.hjobs = dp.tjobs[kServeHalfGrouping];
+            b.hjobs[0] = dp.hjobs[0];
+            b.hjobs[1] = dp.hjobs[1];
             b.xh = half ? static_cast<_Float16 *>(q.xh) : nullptr;
             b.colmax = half ? q.colmax : nullptr;
             b.xexp = half ? q.xexp : nullptr;
@@ -1499,7 +1502,7 @@ struct yasfhip_server {
                 hi[h] = half_reqs[h];
                 bc += (q.N + 255) / 256;
                 bv += static_cast<int>((static_cast<long>(q.nmax + 1) * N32 * 32 + 255) / 256);
-                bg += plan_for(q.L).dp.ntjobs[kServeHalfGrouping] * q.ntiles1;
+                bg += plan_for(q.L).dp.nhjobs[inner_precision == YASFHIP_FP16X2 ? 0 : 1] * q.ntiles1;
             }
             hc[nh] = bc;
             hv[nh] = bv;

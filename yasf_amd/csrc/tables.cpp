@@ -257,6 +257,17 @@ CouplingPlan build_coupling_plan(int lmax) {
     std::stable_sort(plan.jobs.begin(), plan.jobs.end(), by_cost);
     for (auto &tj : plan.table_jobs)
         std::stable_sort(tj.begin(), tj.end(), by_cost);
+    // fp16 GEMM jobs: the jobs of up to 4 column tiles with at most
+    // kHalfJobOrders orders each; a longer order range is split and the
+    // piece's first order goes to pad
+    for (int h = 0; h < 2; ++h) {  // jobs of up to 4 (h = 0) or 8 (h = 1) column tiles
+        for (const PairJob &j : plan.table_jobs[1 + h])
+            for (int ip0 = 0; ip0 < j.nord; ip0 += kHalfJobOrders) {
+                const int n = std::min(kHalfJobOrders, j.nord - ip0);
+                plan.half_jobs[h].push_back(PairJob{j.dm, j.tile0, j.ntile, j.qbase, n * j.ntile, j.tstride, n, ip0});
+            }
+        std::stable_sort(plan.half_jobs[h].begin(), plan.half_jobs[h].end(), by_cost);
+    }
 
     // Contraction lists: for each n1, the nonzero ab5[n2][n1][p].
     plan.crow.assign(plan.nmax + 1, 0);
@@ -351,12 +362,17 @@ DevicePlan upload_device_plan(const CouplingPlan &pl, hipStream_t s) {
     }
     dp.dmoff = upload_vec(dmoff, s);
     dp.order_tiles = upload_vec(pl.order_tiles, s);
+    for (int h = 0; h < 2; ++h) {
+        dp.hjobs[h] = upload_vec(pl.half_jobs[h], s);
+        dp.nhjobs[h] = static_cast<int>(pl.half_jobs[h].size());
+    }
     return dp;
 }
 
 void free_device_plan(DevicePlan &dp) {
     void *bufs[] = {dp.jobs, dp.tjobs[0], dp.tjobs[1], dp.tjobs[2], dp.colmap, dp.crow, dp.cidx,
-                    dp.cval, dp.la,       dp.lb,       dp.ld,       dp.dmoff,  dp.order_tiles};
+                    dp.cval, dp.la,       dp.lb,       dp.ld,       dp.dmoff,  dp.order_tiles, dp.hjobs[0],
+                    dp.hjobs[1]};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
