@@ -252,6 +252,18 @@ def run_solve(args):
         return s.gmres_refined(b, b, rtol=1e-6, inner_rtol=args.inner_rtol, restart=args.restart,
                                inner_maxiter=1000, max_refinements=10, use_precond=use_pc)
 
+    def timed_mixed(inner):
+        s.set_inner_precision(inner)
+        step_mixed()  # warm: builds the inner operator's table
+        lat_i = []
+        res_i = None
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            res_i = step_mixed()
+            lat_i.append(time.perf_counter() - t0)
+        s.set_inner_precision(_native.FP32)
+        return lat_i, res_i
+
     for _ in range(args.warmup):
         step()
         step_mixed()
@@ -280,6 +292,9 @@ def run_solve(args):
         lat_m.append(time.perf_counter() - t0)
     torch.cuda.synchronize()
     xm, info_m, steps_m, iters_m, res_m, _ = mixed
+    # the same refinement with the inner operator on the fp16 pair GEMM
+    lat_h, mixed_h = timed_mixed(_native.FP16X2)
+    xh, info_h, steps_h, iters_h, res_h, _ = mixed_h
     # the same solve with the preconditioner's factors kept in complex128
     fp64_factors = None
     if use_pc:
@@ -296,10 +311,13 @@ def run_solve(args):
     a_inc = b / t
     c64 = s.cross_sections(a_inc, x)
     cm = s.cross_sections(a_inc, xm)
+    chh = s.cross_sections(a_inc, xh)
 
     # Pair-kernel timing with HIP events on the session stream.
     pair_ms, mv_ms = s.time_matvec(reps=5)
     pair32_ms, mv32_ms = s.time_matvec(precision=_native.FP32, reps=5)
+    pair16x2_ms, _ = s.time_matvec(precision=_native.FP16X2, reps=5)
+    pair16_ms, _ = s.time_matvec(precision=_native.FP16, reps=5)
     pairs = N * (N - 1)
     f_fact = factorised_flops_per_pair(lmax) * pairs
     f_direct = direct_flops_per_pair(lmax) * pairs
@@ -370,6 +388,19 @@ def run_solve(args):
             "pair_kernel_ms_fp32": pair32_ms,
             "pair_kernel_tflops_fp32": achieved32,
             "pair_kernel_frac_fp32_peak": achieved32 / PEAK_FP32_TFLOPS,
+        },
+        "mixed_precision_fp16x2_inner": {
+            "solves_per_s": len(lat_h) / sum(lat_h),
+            "p50_latency_s": float(np.percentile(lat_h, 50)),
+            "speedup_vs_fp64": float(np.percentile(lat, 50) / np.percentile(lat_h, 50)),
+            "refinements": steps_h,
+            "inner_iterations": iters_h,
+            "relative_residual_fp64": res_h,
+            "info": info_h,
+            "c_ext_rel_diff_vs_fp64": abs(chh[0] - c64[0]) / abs(c64[0]),
+            "c_sca_rel_diff_vs_fp64": abs(chh[1] - c64[1]) / abs(c64[1]),
+            "pair_kernel_ms_fp16x2": pair16x2_ms,
+            "pair_kernel_ms_fp16": pair16_ms,
         },
     }
     if not args.skip_cpu:
@@ -451,6 +482,24 @@ def run_precision(args):
     pair32_ms, _ = s.time_matvec(precision=_native.FP32, reps=3)
     cm = s.cross_sections(a, xm)
     s.close()
+    # the same refinement with the inner GMRES on the two-term fp16 pair GEMM
+    s = session()
+    s.set_inner_precision(_native.FP16X2)
+    s.time_matvec(precision=_native.FP16X2, reps=1)  # the fp16-term table first, as the complex64 one above
+    s.tune_matvec(precision=_native.FP64, reps=3)  # the FP64 residuals' slabs
+    lat_h = []
+    for i in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        xh, info_h, steps_h, iters_h, res_h, _ = s.gmres_refined(
+            b, b, rtol=1e-6, inner_rtol=args.inner_rtol, restart=50, inner_maxiter=1000, max_refinements=10,
+            use_precond=True)
+        torch.cuda.synchronize()
+        lat_h.append(time.perf_counter() - t0)
+        log(f"mixed fp16x2 {i}: {lat_h[-1]:.2f} s, {steps_h} refinements, {iters_h} inner iterations, "
+            f"residual {res_h:.2e}")
+    pair16x2_ms, _ = s.time_matvec(precision=_native.FP16X2, reps=3)
+    ch = s.cross_sections(a, xh)
+    s.close()
     s = session()
     s.tune_matvec(precision=_native.FP64, reps=3)
     lat64 = []
@@ -488,10 +537,17 @@ def run_precision(args):
                   "relative_residual_fp64": res_m, "info": info_m, "pair_kernel_ms_fp32": pair32_ms},
         "fp64": {"seconds": t64, "iterations": it64, "relative_residual": res64, "info": info64,
                  "pair_kernel_ms": pair64_ms},
+        "mixed_fp16x2_inner": {"seconds": float(np.percentile(lat_h, 50)), "refinements": steps_h,
+                               "inner_iterations": iters_h, "relative_residual_fp64": res_h, "info": info_h,
+                               "pair_kernel_ms_fp16x2": pair16x2_ms,
+                               "speedup_vs_fp64": t64 / float(np.percentile(lat_h, 50))},
         "speedup_vs_fp64": t64 / float(np.percentile(lat, 50)),
         "error_vs_fp64_1e-10": {
             "mixed_coefficients": rel(xm, xr), "fp64_coefficients": rel(x64, xr),
             "mixed_c_ext": abs(cm[0] - cr[0]) / abs(cr[0]), "mixed_c_sca": abs(cm[1] - cr[1]) / abs(cr[1]),
+            "mixed_fp16x2_coefficients": rel(xh, xr),
+            "mixed_fp16x2_c_ext": abs(ch[0] - cr[0]) / abs(cr[0]),
+            "mixed_fp16x2_c_sca": abs(ch[1] - cr[1]) / abs(cr[1]),
             "fp64_c_ext": abs(c64[0] - cr[0]) / abs(cr[0]), "fp64_c_sca": abs(c64[1] - cr[1]) / abs(cr[1]),
             "reference_residual": res_r},
     }
@@ -499,10 +555,12 @@ def run_precision(args):
 
 
 def serving_report(args, log):
-    """The 512-request serving stream (seed 4) through the batched server,
-    closed-loop (all queued at the start) and with Poisson arrivals at
-    args.rate per second, against the all-FP64 single-solve path one request
-    at a time.  Returns a dict."""
+    """The 512-request serving stream (seed 4) through the batched server:
+    closed loop (all queued at the start) with the inner iterations on the
+    two-term fp16 pair GEMM (complex64 accuracy; the headline), on the
+    one-term fp16 GEMM and on the complex64 GEMM; Poisson arrivals at
+    args.rate per second (two-term fp16); against the all-FP64 single-solve
+    path one request at a time.  Returns a dict."""
     import torch
 
     from yasf_amd import _native
@@ -518,25 +576,37 @@ def serving_report(args, log):
         for cls, particles in (("small", 400), ("large", 2000)):
             (waves, ctiles), ms = srv.autotune(L, particles=particles, reps=3)
             tiles[L][cls] = {"waves": waves, "column_tiles": ctiles}
-    log(f"autotuned tile shapes: {tiles}")
-    if args.warmup:
-        serve(srv, make_workload(8, seed=99, n_range=(20, 200)))
-    torch.cuda.synchronize()
+    log(f"autotuned complex64 tile shapes: {tiles}")
     device_setup = not args.host_setup
-    closed = serve(srv, reqs, progress=log, device_setup=device_setup)
-    log(f"served closed-loop: {closed.as_dict()}")
+
+    def summary(rep):
+        return {**rep.as_dict(),
+                "converged": int(sum(1 for r in rep.results.values() if r["info"] == 0)),
+                "mean_inner_iterations": float(np.mean([r["iterations"] for r in rep.results.values()])),
+                "mean_cycles": float(np.mean([r["cycles"] for r in rep.results.values()]))}
+
+    runs = {}
+    for name, prec in (("fp16x2", _native.FP16X2), ("fp16", _native.FP16), ("fp32", _native.FP32)):
+        srv.set_inner_precision(prec)
+        if args.warmup:
+            serve(srv, make_workload(8, seed=99, n_range=(20, 200)))
+        torch.cuda.synchronize()
+        runs[name] = serve(srv, reqs, progress=log, device_setup=device_setup)
+        log(f"served closed-loop, inner {name}: {runs[name].as_dict()}")
+    closed = runs["fp16x2"]
     out = {"requests": len(reqs), "restart": 50, "inner_rtol": args.inner_rtol, "tile_shapes": tiles,
            "setup": "device (Mie + plane wave at admission)" if device_setup else "host",
-           "closed_loop": {**closed.as_dict(),
-                           "converged": int(sum(1 for r in closed.results.values() if r["info"] == 0)),
-                           "mean_inner_iterations": float(np.mean([r["iterations"] for r in
-                                                                   closed.results.values()]))}}
+           "inner_precision": "fp16x2: pair factors and x as two fp16 terms each, fp16 MFMA, FP32 accumulation "
+                              "(complex64 accuracy); FP64 residuals decide convergence",
+           "closed_loop": summary(closed),
+           "closed_loop_inner_fp16": summary(runs["fp16"]),
+           "closed_loop_inner_fp32": summary(runs["fp32"])}
     if math.isfinite(args.rate) and args.rate > 0:
+        srv.set_inner_precision(_native.FP16X2)
         timed = make_workload(args.requests, seed=4, rate=args.rate)
         poisson = serve(srv, timed, progress=log, device_setup=device_setup)
         log(f"served Poisson {args.rate}/s: {poisson.as_dict()}")
-        out["poisson"] = {"arrival_rate_per_s": args.rate, **poisson.as_dict(),
-                          "converged": int(sum(1 for r in poisson.results.values() if r["info"] == 0))}
+        out["poisson"] = {"arrival_rate_per_s": args.rate, **summary(poisson)}
     srv.close()
     if not args.no_baseline:
         base = solve_sequential(reqs, progress=log)
@@ -544,10 +614,13 @@ def serving_report(args, log):
         out["fp64_single_solve"] = {**base.as_dict(),
                                     "converged": int(sum(1 for r in base.results.values() if r["info"] == 0))}
         out["speedup_vs_fp64_single_solve"] = closed.solves_per_s / base.solves_per_s
-        out["max_rel_diff_c_ext"] = max(abs(closed.results[i]["c_ext"] - base.results[i]["c_ext"])
-                                        / abs(base.results[i]["c_ext"]) for i in closed.results)
-        out["max_rel_diff_c_sca"] = max(abs(closed.results[i]["c_sca"] - base.results[i]["c_sca"])
-                                        / abs(base.results[i]["c_sca"]) for i in closed.results)
+        out["speedup_vs_fp64_single_solve_inner_fp16"] = runs["fp16"].solves_per_s / base.solves_per_s
+        out["speedup_vs_fp64_single_solve_inner_fp32"] = runs["fp32"].solves_per_s / base.solves_per_s
+        for name, rep in runs.items():
+            out[f"max_rel_diff_c_ext_{name}"] = max(abs(rep.results[i]["c_ext"] - base.results[i]["c_ext"])
+                                                    / abs(base.results[i]["c_ext"]) for i in rep.results)
+            out[f"max_rel_diff_c_sca_{name}"] = max(abs(rep.results[i]["c_sca"] - base.results[i]["c_sca"])
+                                                    / abs(base.results[i]["c_sca"]) for i in rep.results)
     return out
 
 
@@ -568,7 +641,7 @@ def run_serving(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 pair GEMM inside FP64 refinement (fp32/fp64)",
+        "dtype": "two-term fp16 pair GEMM (complex64 accuracy) inside FP64 refinement",
         "data": "synthetic (seeded request stream, seed 4)",
         "config": {"workload": "serving: 512 requests, N log-uniform [20, 3000], lmax 3..8, "
                                "lambda 300-1000 nm, random index and incidence, rtol 1e-6",
@@ -667,7 +740,7 @@ def main():
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="run a 1 GiB fill and copy first (reference kernels for rocprofv3 --pmc)")
     ap.add_argument("--requests", type=int, default=512, help="serving: number of requests")
-    ap.add_argument("--rate", type=float, default=16.0,
+    ap.add_argument("--rate", type=float, default=24.0,
                     help="serving: Poisson arrivals per second for the timed-arrival run (inf: closed loop only)")
     ap.add_argument("--no-serving", action="store_true",
                     help="solve: leave out the serving sub-object (the 512-request stream)")

@@ -240,7 +240,9 @@ struct yasfhip_session {
             if (matvec_mode == YASFHIP_MATVEC_AUTO) {
                 size_t free_b = 0, total_b = 0;
                 YH_CHECK(hipMemGetInfo(&free_b, &total_b));
-                YH_REQUIRE(bytes <= free_b / 4 * 3, "the fp16 pair table does not fit in device memory");
+                YH_REQUIRE(bytes <= free_b / 4 * 3, "the fp16 pair table (" + std::to_string(bytes >> 20) +
+                                                        " MiB) does not fit in three quarters of the free device "
+                                                        "memory (" + std::to_string(free_b >> 20) + " MiB)");
             }
             YH_CHECK(hipMalloc(&htable, bytes));
             htable_split = split;
