@@ -120,8 +120,12 @@ int yasfhip_master_matvec(yasfhip_session *s, int channel, const double *x, doub
 
 /* Solve (I - T W) x = b for one channel with GMRES on the device, following
  * SciPy's gmres as the reference calls it (solver.py:108-121: rtol=tol,
- * atol=0, restart, maxiter in restart cycles; left preconditioning, modified
- * Gram-Schmidt, LAPACK-style Givens rotations, true-residual outer check).
+ * atol=0, restart in [1, 800], maxiter in restart cycles; left
+ * preconditioning, LAPACK-style Givens rotations, true-residual outer check).
+ * The Arnoldi basis is orthogonalised by classical Gram-Schmidt with one
+ * reorthogonalisation (SciPy: modified Gram-Schmidt; the two agree to
+ * rounding), and each restart cycle runs on the device with one host
+ * synchronisation per cycle.
  * x_inout holds x0 on entry (the reference defaults x0 = b) and the solution
  * on exit.  use_precond != 0 applies the block-diagonal preconditioner built
  * by yasfhip_precond_build.  info: 0 converged, else maxiter (SciPy).
@@ -135,9 +139,9 @@ int yasfhip_gmres_solve(yasfhip_session *s, int channel, const double *b, double
 /* yasfhip_gmres_solve with b and x_inout in device memory of the session's
  * device (n complex128 values each): the same solve (solver.py:108-121) for
  * callers whose vectors are already in HBM.  b and x stay in device memory
- * (copied within HBM, never over PCIe); the solve's own scalars (Arnoldi
- * coefficients, residual norms) still reach the host as they do in
- * yasfhip_gmres_solve.  Host pointers, and buffers that do not hold n values
+ * (copied within HBM, never over PCIe); the solve's per-cycle scalars
+ * (residual norms, the cycle's residual estimates) reach the host as they do
+ * in yasfhip_gmres_solve.  Host pointers, and buffers that do not hold n values
  * inside one device allocation, are rejected (YASFHIP_INVALID_ARGUMENT)
  * before any device work; history stays a host array.  Returns when the
  * solve is done, as yasfhip_gmres_solve does. */

@@ -75,6 +75,37 @@ def test_gmres_matches_scipy_over_oracle():
     assert abs(its - it_ref) <= 1
 
 
+def test_gmres_history_matches_scipy_over_oracle():
+    """Several restart cycles (restart 10): the device cycles' residual
+    estimates follow SciPy's pr_norm history over the oracle operator (CGS2 on
+    the device, MGS in SciPy: equal to rounding), the cycle count and the
+    iterations agree within one per cycle, and the solutions agree."""
+    pos, radii, n, lam, nmed, L = _cfg1()
+    sim = build_sim(pos, radii, n, lam, nmed, L, 1e-8, restart=10)
+    b = sim.right_hand_side[:, :, 0].ravel()
+    k = 2 * np.pi / lam
+    t = sim.tmatrix_flat()[0]
+    x, info, iters, res, hist = sim.session.gmres(b, b, rtol=1e-8, restart=10, maxiter=100, history_cap=4096)
+    assert info == 0 and res <= 1e-8
+    from scipy.sparse.linalg import LinearOperator, gmres
+
+    ref_hist = []
+    A = LinearOperator((b.size, b.size), matvec=lambda v: R.master_matvec(L, pos, k, t, v), dtype=complex)
+    xr, info_r = gmres(A, b, b.copy(), restart=10, rtol=1e-8, atol=0, maxiter=100,
+                       callback=ref_hist.append, callback_type="pr_norm")
+    assert info_r == 0
+    ref_hist = np.asarray(ref_hist)
+    hist = np.asarray(hist)
+    assert len(hist) == iters
+    cycles = -(-iters // 10)
+    assert abs(iters - len(ref_hist)) <= max(1, cycles)
+    m = min(len(hist), len(ref_hist), 30)
+    npt.assert_allclose(hist[:m], ref_hist[:m], rtol=1e-6)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-6
+    with pytest.raises(ValueError):
+        sim.session.gmres(b, b, restart=801)
+
+
 def test_block_preconditioner_apply_and_solve():
     pos, radii, n, lam, nmed, L = _cfg1()
     pc = BlockDiagonalPreconditioner(edge_size_for(pos, 8))

@@ -1,8 +1,9 @@
 // C ABI (include/yasfhip.h): session state, device buffers and the GMRES
 // driver.  The GMRES control flow restates SciPy's gmres
 // (scipy/sparse/linalg/_isolve/iterative.py:692-840, the solver the reference
-// calls from yasfpy/solver.py:108-121) with every vector operation on the
-// device; only the (restart+1)-sized Hessenberg/Givens work runs on the host.
+// calls from yasfpy/solver.py:108-121); each restart cycle's Arnoldi steps,
+// Givens rotations and back substitution run on the device, and the host
+// takes the per-cycle decisions (true residual, SciPy's ptol update).
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -14,52 +15,6 @@
 #include "internal.hpp"
 
 namespace yasfhip {
-
-// krylov.hip
-struct Reducer {
-    double *partials;
-    unsigned *counter;
-};
-void launch_mgs_step(double2 *w, const double2 *v_prev, const double2 *h_prev, const double2 *v_next,
-                     long n, Reducer red, double *out, hipStream_t s);
-// chained modified-Gram-Schmidt steps; partials buffers of 512 x 4 doubles
-void launch_mgs_chain(double2 *w, const double2 *v_prev, const double *part_prev, const double2 *v_next,
-                      long n, double *part_out, double *h_out, hipStream_t s);
-void launch_partials_sum(const double *part, long n, double *out, hipStream_t s);
-void launch_scale(double2 *y, const double2 *x, double a, long n, hipStream_t s);
-void launch_add(double2 *x, const double2 *d, long n, hipStream_t s);
-void launch_update(double2 *x, const double2 *V, const double2 *y, int m, long n, long ld,
-                   hipStream_t s);
-void launch_residual(double2 *r, const double2 *b, const double2 *ax, long n, Reducer red, double *out,
-                     hipStream_t s);
-void launch_norm2(const double2 *x, long n, Reducer red, double *out, hipStream_t s);
-void launch_to_fp32(float *dst, const double *src, long count, hipStream_t s);
-void launch_to_fp64(double *dst, const float *src, long count, hipStream_t s);
-void launch_fill_test(double *dst, long count, hipStream_t s);  // timing input in [-1, 1)
-// observables.hip
-void launch_far_field(const double4 *pos, const double2 *f, int N, int lmax, double k,
-                      const double4 *dir, const double *polar, const double *azimuthal, int A,
-                      double2 *F, double2 *e_theta, double2 *e_phi, hipStream_t s);
-void csca_quadrature_grid(double kR, int lmax, int *n_theta, int *n_phi);
-size_t csca_quadrature_partials(int n_theta, int n_phi);
-void launch_csca_quadrature(const double4 *pos, const double2 *f, int N, int lmax, double k, int n_theta,
-                            int n_phi, double *partials, double *out, hipStream_t s);
-// half.hip: the pair GEMM on the fp16 matrix cores
-size_t half_table_term_bytes(const DevicePlan &dp, int N);
-size_t half_x_bytes(const DevicePlan &dp, int N);
-void launch_half_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, void *table, bool split,
-                             int *pexp, unsigned long long *dmin, hipStream_t s, double near2);
-void launch_half_x(const DevicePlan &dp, int N, const double2 *x, void *xh, bool split, unsigned *colmax, int *xexp,
-                   hipStream_t s);
-void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void *xh, bool split, const int *pexp,
-                      const int *xexp, void *Q, hipStream_t s);
-// precond.hip
-struct BlockPrecond;
-BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &ab5,
-                             const double *pos_host, int N, const int *box_of_particle, int num_boxes,
-                             double k, const double2 *t_dev, bool fp32_factors, hipStream_t s);
-void precond_destroy(BlockPrecond *p);
-void precond_apply(BlockPrecond *p, const double2 *r, double2 *z, hipStream_t s);
 
 namespace {
 
@@ -80,29 +35,6 @@ T *upload(const std::vector<T> &v, hipStream_t s) {
     if (!v.empty())
         YH_CHECK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
     return d;
-}
-
-// LAPACK zlartg semantics: [c s; -conj(s) c] [f; g] = [r; 0], c real.
-void lartg(cplx f, cplx g, double &c, cplx &s, cplx &r) {
-    if (g == cplx(0, 0)) {
-        c = 1.0;
-        s = cplx(0, 0);
-        r = f;
-        return;
-    }
-    if (f == cplx(0, 0)) {
-        c = 0.0;
-        const double ag = std::abs(g);
-        s = std::conj(g) / ag;
-        r = cplx(ag, 0.0);
-        return;
-    }
-    const double af = std::abs(f), ag = std::abs(g);
-    const double d = std::sqrt(af * af + ag * ag);
-    c = af / d;
-    const cplx sgn = f / af;
-    s = sgn * std::conj(g) / d;
-    r = sgn * d;
 }
 
 }  // namespace
@@ -127,14 +59,17 @@ struct yasfhip_session {
     float2 *x32 = nullptr;  // complex64 copy of the pair GEMM's x
     // GMRES workspace
     int vcap = 0;
-    double2 *V = nullptr, *bvec = nullptr, *xvec = nullptr, *rvec = nullptr, *avec = nullptr,
-            *ycoef = nullptr;
+    double2 *V = nullptr, *bvec = nullptr, *xvec = nullptr, *rvec = nullptr, *avec = nullptr;
     double2 *rb = nullptr, *rx = nullptr, *rr = nullptr;  // iterative refinement: b, x, residual
     double *scal = nullptr, *partials = nullptr;
-    double *mgs_part = nullptr;  // two launches' partials (kRedPartials each), used in turn
     unsigned *counter = nullptr;
     double *scal_host = nullptr;
-    double *hdev = nullptr, *hhost = nullptr;  // per-step MGS results, 4 doubles per slot
+    // Arnoldi state of the device GMRES (ArnoldiBuffers, sized for restart
+    // vcap - 1): Hessenberg, Givens, coefficients, partial sums, state word
+    void *arn_mem = nullptr;
+    ArnoldiState *arn_snap = nullptr;  // pinned: two polling slots and the cycle end
+    double *arn_hist = nullptr;        // pinned: a cycle's residual estimates
+    hipEvent_t poll_ev[2] = {nullptr, nullptr};
     BlockPrecond *precond = nullptr;
     int precond_channel = -1;
     bool precond_fp32 = true;  // LU factors stored in complex64 for the apply (FP64 arithmetic)
@@ -171,7 +106,7 @@ struct yasfhip_session {
         if (precond)
             precond_destroy(precond);
         void *bufs[] = {pos,  Q,    t,    xin,   yout,    x32,     V,    bvec,     xvec, rvec,
-                        avec, ycoef, scal, partials, counter, hdev, rb, rx, rr, mgs_part,
+                        avec, arn_mem, scal, partials, counter, rb, rx, rr,
                         htable, xh, hcolmax, hxexp, hpexp, hdmin};
         for (void *b : bufs)
             if (b)
@@ -184,9 +119,13 @@ struct yasfhip_session {
             (void)hipFree(slab);
         if (scal_host)
             (void)hipHostFree(scal_host);
-        if (hhost)
-            (void)hipHostFree(hhost);
+        for (void *h : {static_cast<void *>(arn_snap), static_cast<void *>(arn_hist)})
+            if (h)
+                (void)hipHostFree(h);
         for (auto &e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        for (auto &e : poll_ev)
             if (e)
                 (void)hipEventDestroy(e);
         if (stream)
@@ -292,22 +231,24 @@ struct yasfhip_session {
         return std::min(tiles, ntiles1);
     }
 
-    void pair_gemm(int channel, const void *xv, bool fp32) {
+    // gate (device int, may be null): every kernel does nothing unless
+    // *gate == 1 when it starts.
+    void pair_gemm(int channel, const void *xv, bool fp32, const int *gate = nullptr) {
         const int ix = fp32 ? 1 : 0;
         if (matvec_mode != YASFHIP_MATVEC_SLAB && table_for(channel, fp32)) {
-            launch_pair_table_gemm(dp, N, table[ix], xv, fp32, table_grouping[ix], Q, stream);
+            launch_pair_table_gemm(dp, N, table[ix], xv, fp32, table_grouping[ix], Q, stream, 0, -1, gate);
             return;
         }
         const int tiles = slab_tiles(fp32);
         if (tiles == 0) {
-            launch_pair(dp, pos, N, k[channel], xv, fp32, Q, stream, nullptr, near2);
+            launch_pair(dp, pos, N, k[channel], xv, fp32, Q, stream, gate, near2);
             return;
         }
         const int ntiles1 = (N + 15) / 16;
         for (int t0 = 0; t0 < ntiles1; t0 += tiles) {
             const int ntl = std::min(tiles, ntiles1 - t0);
-            launch_pair_table_build(dp, pos, N, k[channel], fp32, slab, stream, t0, ntl, nullptr, false, near2);
-            launch_pair_table_gemm(dp, N, slab, xv, fp32, table_grouping[ix], Q, stream, t0, ntl);
+            launch_pair_table_build(dp, pos, N, k[channel], fp32, slab, stream, t0, ntl, gate, false, near2);
+            launch_pair_table_gemm(dp, N, slab, xv, fp32, table_grouping[ix], Q, stream, t0, ntl, gate);
         }
     }
 
@@ -333,44 +274,84 @@ struct yasfhip_session {
     // YASFHIP_FP32: the pair GEMM runs in complex64 on a complex64 copy of x;
     // YASFHIP_FP16X2 / YASFHIP_FP16: on the fp16 matrix cores over the
     // fp16-term table and an fp16-term copy of x (complex64 partial sums);
-    // the ab5 contraction and the epilogue stay FP64.
-    void matvec_dev(int channel, const double2 *x, double2 *y, int mode, int precision) {
+    // the ab5 contraction and the epilogue stay FP64.  gate: as pair_gemm.
+    void matvec_dev(int channel, const double2 *x, double2 *y, int mode, int precision,
+                    const int *gate = nullptr) {
         const double2 *tt = t + static_cast<size_t>(channel) * n;
         if (precision == YASFHIP_FP16X2 || precision == YASFHIP_FP16) {
             const bool split = precision == YASFHIP_FP16X2;
             half_table_for(channel, split);
-            launch_half_x(dp, N, x, xh, split, hcolmax, hxexp, stream);
-            launch_half_gemm(dp, N, htable, xh, split, hpexp, hxexp, Q, stream);
-            launch_contract(dp, N, Q, x, tt, mode, y, true, false, stream);
+            launch_half_x(dp, N, x, xh, split, hcolmax, hxexp, stream, gate);
+            launch_half_gemm(dp, N, htable, xh, split, hpexp, hxexp, Q, stream, gate);
+            launch_contract(dp, N, Q, x, tt, mode, y, true, false, stream, 0, -1, gate);
         } else if (precision == YASFHIP_FP64) {
-            pair_gemm(channel, x, false);
-            launch_contract(dp, N, Q, x, tt, mode, y, false, false, stream);
+            pair_gemm(channel, x, false, gate);
+            launch_contract(dp, N, Q, x, tt, mode, y, false, false, stream, 0, -1, gate);
         } else {
             launch_to_fp32(reinterpret_cast<float *>(x32), reinterpret_cast<const double *>(x), 2 * n,
-                           stream);
-            pair_gemm(channel, x32, true);
-            launch_contract(dp, N, Q, x, tt, mode, y, true, false, stream);
+                           stream, gate);
+            pair_gemm(channel, x32, true, gate);
+            launch_contract(dp, N, Q, x, tt, mode, y, true, false, stream, 0, -1, gate);
         }
+    }
+
+    // Arnoldi buffers of restart capacity cap = vcap - 1, carved from arn_mem.
+    static size_t arnoldi_bytes(int cap, int nblk) {
+        const size_t c = cap, R1 = c + 1, R2 = c + 2;
+        return sizeof(double2) * (c * R1 + c + R1 + 2 * R1 + 2 * R2 * nblk + c) + sizeof(double) * (2 * c) +
+               sizeof(ArnoldiState) + 16 * 256;
+    }
+
+    ArnoldiBuffers arnoldi(int restart) const {
+        const int cap = vcap - 1;
+        const size_t c = cap, R1 = c + 1, R2 = c + 2;
+        ArnoldiBuffers a{};
+        a.V = V;
+        a.n = static_cast<long>(n);
+        a.restart = restart;
+        a.nblk = arnoldi_blocks(a.n);
+        char *p = static_cast<char *>(arn_mem);
+        auto take = [&](size_t bytes) {
+            char *q = p;
+            p += (bytes + 255) & ~size_t(255);
+            return q;
+        };
+        a.H = reinterpret_cast<double2 *>(take(sizeof(double2) * c * R1));
+        a.gs = reinterpret_cast<double2 *>(take(sizeof(double2) * c));
+        a.g = reinterpret_cast<double2 *>(take(sizeof(double2) * R1));
+        a.h1 = reinterpret_cast<double2 *>(take(sizeof(double2) * R1));
+        a.h2 = reinterpret_cast<double2 *>(take(sizeof(double2) * R1));
+        a.partA = reinterpret_cast<double2 *>(take(sizeof(double2) * R2 * a.nblk));
+        a.partB = reinterpret_cast<double2 *>(take(sizeof(double2) * R2 * a.nblk));
+        a.y = reinterpret_cast<double2 *>(take(sizeof(double2) * c));
+        a.gc = reinterpret_cast<double *>(take(sizeof(double) * c));
+        a.hist = reinterpret_cast<double *>(take(sizeof(double) * c));
+        a.st = reinterpret_cast<ArnoldiState *>(take(sizeof(ArnoldiState)));
+        return a;
     }
 
     void ensure_gmres(int restart) {
         if (vcap >= restart + 1)
             return;
-        if (V)
-            YH_CHECK(hipFree(V));
-        if (ycoef)
-            YH_CHECK(hipFree(ycoef));
-        if (hdev)
-            YH_CHECK(hipFree(hdev));
-        if (hhost)
-            YH_CHECK(hipHostFree(hhost));
+        for (void *b : {static_cast<void *>(V), arn_mem})
+            if (b)
+                YH_CHECK(hipFree(b));
+        V = nullptr;
+        arn_mem = nullptr;
+        if (arn_hist)
+            YH_CHECK(hipHostFree(arn_hist));
+        arn_hist = nullptr;
+        vcap = 0;
         V = dmalloc<double2>(static_cast<size_t>(restart + 1) * n);
-        if (!mgs_part)
-            mgs_part = dmalloc<double>(2 * kRedPartials);
-        ycoef = dmalloc<double2>(restart + 1);
-        hdev = dmalloc<double>(4 * static_cast<size_t>(restart + 2));
-        YH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&hhost), 4 * (restart + 2) * sizeof(double),
+        arn_mem = dmalloc<char>(arnoldi_bytes(restart, arnoldi_blocks(static_cast<long>(n))));
+        YH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&arn_hist), restart * sizeof(double),
                                hipHostMallocDefault));
+        if (!arn_snap)
+            YH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&arn_snap), 3 * sizeof(ArnoldiState),
+                                   hipHostMallocDefault));
+        for (auto &e : poll_ev)
+            if (!e)
+                YH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         vcap = restart + 1;
     }
 
@@ -386,9 +367,9 @@ struct yasfhip_session {
         return std::sqrt(scal_host[0]);
     }
 
-    void psolve(const double2 *r, double2 *z, bool use_precond) {
+    void psolve(const double2 *r, double2 *z, bool use_precond, const int *gate = nullptr) {
         if (use_precond)
-            precond_apply(precond, r, z, stream);
+            precond_apply(precond, r, z, stream, gate);
         else if (z != r)
             YH_CHECK(hipMemcpyAsync(z, r, n * sizeof(double2), hipMemcpyDeviceToDevice, stream));
     }
@@ -406,8 +387,20 @@ struct yasfhip_session {
     // may alias rvec, avec or V.  precision selects the pair GEMM of every
     // operator application (the Arnoldi process stays FP64).
     // history: presid/||b|| per inner iteration, up to history_cap values.
+    //
+    // A restart cycle runs on the device without host round trips: the host
+    // queues the cycle's Arnoldi steps (operator, preconditioner,
+    // launch_arnoldi_step) in groups of kStepsPerPoll, every kernel gated by
+    // the cycle's state word, and reads that word back one group behind, so
+    // the device does not wait for the host inside a cycle; steps queued past
+    // the device's stop decision return at once.  The host syncs once per
+    // cycle, for the true residual that drives SciPy's outer loop.
+    static constexpr int kStepsPerPoll = 4;
+
     GmresOutcome gmres_run(int channel, const double2 *bb, double2 *x, double rtol, int restart,
                            int maxiter, bool pc, int precision, double *history, int history_cap) {
+        YH_REQUIRE(restart >= 1 && restart <= arnoldi_max_restart(),
+                   "restart must be in [1, " + std::to_string(arnoldi_max_restart()) + "]");
         GmresOutcome out;
         hipStream_t st = stream;
         double2 *r = rvec, *av = avec;
@@ -426,10 +419,10 @@ struct yasfhip_session {
         double ptol_max_factor = 1.0;
         double ptol = Mb_nrm2 * std::min(ptol_max_factor, atol / bnrm2);
         double presid = 0.0;
-        std::vector<cplx> h(static_cast<size_t>(restart) * (restart + 1));
-        std::vector<double> gc(restart);
-        std::vector<cplx> gs(restart), S(restart + 1), y(restart);
-        auto H = [&](int col, int k) -> cplx & { return h[static_cast<size_t>(col) * (restart + 1) + k]; };
+        bool breakdown = false;
+        ensure_gmres(restart);
+        const ArnoldiBuffers a = arnoldi(restart);
+        const int *gate = &a.st->active;
 
         for (int iteration = 0; iteration < maxiter; ++iteration) {
             if (iteration == 0) {
@@ -451,98 +444,46 @@ struct yasfhip_session {
                     return out;
                 }
             }
+            // v[0] = M r / ||M r||, g = [||M r||, 0, ...]
             psolve(r, V, pc);
-            const double tmp0 = norm_dev(V);
-            launch_scale(V, V, 1.0 / tmp0, n, st);
-            std::fill(S.begin(), S.end(), cplx(0, 0));
-            S[0] = tmp0;
-            bool breakdown = false;
-            int col = 0;
-            for (col = 0; col < restart; ++col) {
-                double2 *vc = V + static_cast<size_t>(col) * n;
-                double2 *w = V + static_cast<size_t>(col + 1) * n;
-                if (pc) {
-                    matvec_dev(channel, vc, av, 1, precision);
-                    psolve(av, w, true);
-                } else {
-                    matvec_dev(channel, vc, w, 1, precision);
+            launch_norm2(V, n, reducer(), scal, st);
+            launch_arnoldi_start(a, scal, ptol, st);
+            int queued = 0;
+            for (int group = 0; queued < restart; ++group) {
+                const int end = std::min(restart, queued + kStepsPerPoll);
+                for (; queued < end; ++queued) {
+                    const double2 *vc = V + static_cast<size_t>(queued) * n;
+                    double2 *w = V + static_cast<size_t>(queued + 1) * n;
+                    if (pc) {
+                        matvec_dev(channel, vc, av, 1, precision, gate);
+                        psolve(av, w, true, gate);
+                    } else {
+                        matvec_dev(channel, vc, w, 1, precision, gate);
+                    }
+                    launch_arnoldi_step(a, queued, st);
                 }
-                // Modified Gram-Schmidt, one fused launch per basis vector:
-                // launch kq subtracts h[kq-1] v[kq-1] and forms the partials
-                // of <v[kq], w> (buffer kq % 2); the next launch sums them and
-                // stores (Re, Im, ||w||^2) of launch kq to slot kq of hdev.
-                auto part = [&](int launch) { return mgs_part + (launch & 1) * kRedPartials; };
-                for (int kq = 0; kq <= col; ++kq) {
-                    const double2 *vk = V + static_cast<size_t>(kq) * n;
-                    const double2 *vp = kq > 0 ? V + static_cast<size_t>(kq - 1) * n : nullptr;
-                    launch_mgs_chain(w, vp, part(kq + 1), vk, n, part(kq), kq > 0 ? hdev + 4 * (kq - 1) : nullptr,
-                                     st);
-                }
-                launch_mgs_chain(w, vc, part(col), nullptr, n, part(col + 1), hdev + 4 * col, st);
-                launch_partials_sum(part(col + 1), n, hdev + 4 * (col + 1), st);
-                YH_CHECK(hipMemcpyAsync(hhost, hdev, 4 * (col + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                YH_CHECK(hipMemcpyAsync(arn_snap + (group & 1), a.st, sizeof(ArnoldiState), hipMemcpyDeviceToHost,
                                         st));
-                YH_CHECK(hipStreamSynchronize(st));
-                const double h0 = std::sqrt(hhost[2]);
-                for (int kq = 0; kq <= col; ++kq)
-                    H(col, kq) = cplx(hhost[4 * kq], hhost[4 * kq + 1]);
-                const double h1 = std::sqrt(hhost[4 * (col + 1) + 2]);
-                H(col, col + 1) = h1;
-                if (h1 <= eps * h0) {
-                    H(col, col + 1) = 0.0;
-                    breakdown = true;
-                } else {
-                    launch_scale(w, w, 1.0 / h1, n, st);
-                }
-                for (int kq = 0; kq < col; ++kq) {
-                    const double c = gc[kq];
-                    const cplx sn = gs[kq];
-                    const cplx n0 = H(col, kq), n1 = H(col, kq + 1);
-                    H(col, kq) = c * n0 + sn * n1;
-                    H(col, kq + 1) = -std::conj(sn) * n0 + c * n1;
-                }
-                double c;
-                cplx sn, mag;
-                lartg(H(col, col), H(col, col + 1), c, sn, mag);
-                gc[col] = c;
-                gs[col] = sn;
-                H(col, col) = mag;
-                H(col, col + 1) = 0.0;
-                const cplx tmp = -std::conj(sn) * S[col];
-                S[col] = c * S[col];
-                S[col + 1] = tmp;
-                presid = std::abs(tmp);
-                ++out.iterations;
-                if (history && nhist < history_cap)
-                    history[nhist++] = presid / bnrm2;
-                if (presid <= ptol || breakdown)
-                    break;
-            }
-            if (col == restart)
-                col = restart - 1;
-            if (H(col, col) == cplx(0, 0))
-                S[col] = 0.0;
-            for (int kq = 0; kq <= col; ++kq)
-                y[kq] = S[kq];
-            for (int kq = col; kq > 0; --kq) {
-                if (y[kq] != cplx(0, 0)) {
-                    y[kq] /= H(kq, kq);
-                    const cplx tk = y[kq];
-                    for (int i = 0; i < kq; ++i)
-                        y[i] -= tk * H(kq, i);
+                YH_CHECK(hipEventRecord(poll_ev[group & 1], st));
+                if (group > 0) {
+                    YH_CHECK(hipEventSynchronize(poll_ev[(group - 1) & 1]));
+                    if (arn_snap[(group - 1) & 1].active != 1)
+                        break;
                 }
             }
-            if (y[0] != cplx(0, 0))
-                y[0] /= H(0, 0);
-            std::vector<double2> ydev(col + 1);
-            for (int kq = 0; kq <= col; ++kq)
-                ydev[kq] = make_double2(y[kq].real(), y[kq].imag());
-            YH_CHECK(hipMemcpyAsync(ycoef, ydev.data(), (col + 1) * sizeof(double2), hipMemcpyHostToDevice,
-                                    st));
-            launch_update(x, V, ycoef, col + 1, n, n, st);
+            // x += V y, y = H^-1 g; then the true residual
+            launch_arnoldi_end(a, x, st);
             matvec_dev(channel, x, av, 1, precision);
             launch_residual(r, bb, av, n, reducer(), scal, st);
+            YH_CHECK(hipMemcpyAsync(arn_snap + 2, a.st, sizeof(ArnoldiState), hipMemcpyDeviceToHost, st));
+            YH_CHECK(hipMemcpyAsync(arn_hist, a.hist, restart * sizeof(double), hipMemcpyDeviceToHost, st));
             fetch_scalars(1);
+            const ArnoldiState cycle = arn_snap[2];
+            out.iterations += cycle.j;
+            for (int k = 0; k < cycle.j && history && nhist < history_cap; ++k)
+                history[nhist++] = arn_hist[k] / bnrm2;
+            presid = cycle.presid;
+            breakdown = cycle.breakdown != 0;
             rnorm = std::sqrt(scal_host[0]);
             if (rnorm <= atol)
                 break;

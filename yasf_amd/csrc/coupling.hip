@@ -204,8 +204,11 @@ __global__ __launch_bounds__(256) void contract_kernel(const TQ *__restrict__ Q,
                                                        const double2 *__restrict__ cval, int N, int nmax,
                                                        const TX *__restrict__ x,
                                                        const double2 *__restrict__ t, int mode,
-                                                       TY *__restrict__ y, int t0, int t1) {
+                                                       TY *__restrict__ y, int t0, int t1,
+                                                       const int *__restrict__ gate) {
     // target tiles [t0, t1); Q holds them from its start
+    if (gate && *gate != 1)
+        return;
     const int groups = contract_groups(nmax);
     const int q = blockIdx.x >> 3, xcd = blockIdx.x & 7;
     const int s1t = t0 + (q / groups) * 8 + xcd;
@@ -254,7 +257,8 @@ size_t pair_lds_bytes(const DevicePlan &dp) { return 2 * static_cast<size_t>(dp.
 // otherwise complex128 (the mixed-precision operator reads FP32 Q with FP64
 // x and y, so x - t W x is formed in FP64).
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
-                     int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s, int t0, int ntl) {
+                     int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s, int t0, int ntl,
+                     const int *gate) {
     const int ntiles = ntl < 0 ? (N + 15) / 16 - t0 : ntl;
     const int t1 = t0 + ntiles;
     const int nblocks = (ntiles + 7) / 8 * 8 * contract_groups(dp.nmax);
@@ -262,15 +266,15 @@ void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, 
     if (io_fp32)
         hipLaunchKernelGGL((contract_kernel<float, float, float>), dim3(nblocks), dim3(256), 0, s,
                            static_cast<const float *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N, dp.nmax,
-                           static_cast<const float *>(x), t, mode, static_cast<float *>(y), t0, t1);
+                           static_cast<const float *>(x), t, mode, static_cast<float *>(y), t0, t1, gate);
     else if (q_fp32)
         hipLaunchKernelGGL((contract_kernel<float, double, double>), dim3(nblocks), dim3(256), 0, s,
                            static_cast<const float *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N, dp.nmax,
-                           static_cast<const double *>(x), t, mode, static_cast<double *>(y), t0, t1);
+                           static_cast<const double *>(x), t, mode, static_cast<double *>(y), t0, t1, gate);
     else
         hipLaunchKernelGGL((contract_kernel<double, double, double>), dim3(nblocks), dim3(256), 0, s,
                            static_cast<const double *>(Q), dp.nq, dp.crow, dp.cidx, dp.cval, N,
-                           dp.nmax, static_cast<const double *>(x), t, mode, static_cast<double *>(y), t0, t1);
+                           dp.nmax, static_cast<const double *>(x), t, mode, static_cast<double *>(y), t0, t1, gate);
     YH_CHECK(hipGetLastError());
 }
 

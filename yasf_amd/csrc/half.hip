@@ -84,13 +84,18 @@ __global__ __launch_bounds__(256) void half_table_build_kernel(
     });
 }
 
-__global__ void zero_u32_kernel(unsigned *v, int n) {
+__global__ void zero_u32_kernel(unsigned *v, int n, const int *__restrict__ gate) {
+    if (gate && *gate != 1)
+        return;
     for (int i = threadIdx.x; i < n; i += blockDim.x)
         v[i] = 0u;
 }
 
 __global__ __launch_bounds__(256) void half_colmax_kernel(const double2 *__restrict__ x, int N, int nmax,
-                                                          unsigned *__restrict__ colmax) {
+                                                          unsigned *__restrict__ colmax,
+                                                          const int *__restrict__ gate) {
+    if (gate && *gate != 1)
+        return;
     half_colmax_block(x, N, nmax, colmax, blockIdx.x);
 }
 
@@ -98,7 +103,10 @@ template <bool SPLIT>
 __global__ __launch_bounds__(256) void half_convert_kernel(const double2 *__restrict__ x, int N, int N32, int nmax,
                                                            const unsigned *__restrict__ colmax,
                                                            int *__restrict__ xexp, _Float16 *__restrict__ xhi,
-                                                           _Float16 *__restrict__ xlo) {
+                                                           _Float16 *__restrict__ xlo,
+                                                           const int *__restrict__ gate) {
+    if (gate && *gate != 1)
+        return;
     const long idx = blockIdx.x * 256L + threadIdx.x;
     if (idx < static_cast<long>(nmax + 1) * N32 * 32)
         half_convert_elem<SPLIT>(x, N, N32, nmax, colmax, xexp, xhi, xlo, idx);
@@ -111,7 +119,9 @@ __global__ __launch_bounds__(64 * WAVES) void half_gemm_kernel(
     const _Float16 *__restrict__ hi_tab, const _Float16 *__restrict__ lo_tab, const _Float16 *__restrict__ xhi,
     const _Float16 *__restrict__ xlo, const int *__restrict__ pexp, const int *__restrict__ xexp, int N32, int nmax,
     int lmax, const PairJob *__restrict__ jobs, int ntl, int t0, const int *__restrict__ colmap, int maxcols,
-    const int *__restrict__ dmoff, float *__restrict__ Q, int nq) {
+    const int *__restrict__ dmoff, float *__restrict__ Q, int nq, const int *__restrict__ gate) {
+    if (gate && *gate != 1)
+        return;
     const int jid = blockIdx.x / ntl;
     const int t = blockIdx.x - jid * ntl;
     const int dm = uniform(jobs[jid].dm);
@@ -174,24 +184,24 @@ void launch_half_table_build(const DevicePlan &dp, const double4 *pos, int N, do
 // x (N x nmax complex128) to its fp16 terms in xh (half_x_bytes) with the
 // column exponents xexp[nmax + 1]; colmax: nmax unsigned of device scratch.
 void launch_half_x(const DevicePlan &dp, int N, const double2 *x, void *xh, bool split, unsigned *colmax, int *xexp,
-                   hipStream_t s) {
+                   hipStream_t s, const int *gate) {
     const int N32 = (N + 31) / 32;
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, s, colmax, dp.nmax);
-    hipLaunchKernelGGL(half_colmax_kernel, dim3((N + 255) / 256), dim3(256), 0, s, x, N, dp.nmax, colmax);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, s, colmax, dp.nmax, gate);
+    hipLaunchKernelGGL(half_colmax_kernel, dim3((N + 255) / 256), dim3(256), 0, s, x, N, dp.nmax, colmax, gate);
     _Float16 *xhi = static_cast<_Float16 *>(xh);
     _Float16 *xlo = xhi + 2 * half_x_plane(dp.nmax, N32);
     const long count = static_cast<long>(dp.nmax + 1) * N32 * 32;
     const dim3 grid(static_cast<unsigned>((count + 255) / 256)), block(256);
     if (split)
-        hipLaunchKernelGGL(half_convert_kernel<true>, grid, block, 0, s, x, N, N32, dp.nmax, colmax, xexp, xhi, xlo);
+        hipLaunchKernelGGL(half_convert_kernel<true>, grid, block, 0, s, x, N, N32, dp.nmax, colmax, xexp, xhi, xlo, gate);
     else
-        hipLaunchKernelGGL(half_convert_kernel<false>, grid, block, 0, s, x, N, N32, dp.nmax, colmax, xexp, xhi, xlo);
+        hipLaunchKernelGGL(half_convert_kernel<false>, grid, block, 0, s, x, N, N32, dp.nmax, colmax, xexp, xhi, xlo, gate);
     YH_CHECK(hipGetLastError());
 }
 
 // Q (complex64, the table GEMM's layout) = G x over the fp16 table and x terms.
 void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void *xh, bool split, const int *pexp,
-                      const int *xexp, void *Q, hipStream_t s) {
+                      const int *xexp, void *Q, hipStream_t s, const int *gate) {
     const int ntiles1 = (N + 15) / 16, N32 = (N + 31) / 32;
     const _Float16 *hi = static_cast<const _Float16 *>(table);
     const _Float16 *lo = hi + half_table_term_bytes(dp, N) / sizeof(_Float16);
@@ -202,11 +212,11 @@ void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void
     if (split)
         hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, half_job_tiles(true), true>), grid, block, 0, s, hi, lo, xhi,
                            xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntiles1, 0, dp.colmap, dp.maxcols,
-                           dp.dmoff, static_cast<float *>(Q), dp.nq);
+                           dp.dmoff, static_cast<float *>(Q), dp.nq, gate);
     else
         hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, half_job_tiles(false), false>), grid, block, 0, s, hi, lo,
                            xhi, xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntiles1, 0, dp.colmap,
-                           dp.maxcols, dp.dmoff, static_cast<float *>(Q), dp.nq);
+                           dp.maxcols, dp.dmoff, static_cast<float *>(Q), dp.nq, gate);
     YH_CHECK(hipGetLastError());
 }
 

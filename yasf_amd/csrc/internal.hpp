@@ -153,9 +153,10 @@ void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double
                          void *Q, hipStream_t s);
 // ab5 contraction (FP64 accumulation) + optional master-operator epilogue.
 // t0, ntl: only the target tiles [t0, t0 + ntl) (ntl < 0: to the last),
-// read from Q from its start.
+// read from Q from its start.  gate: as launch_pair.
 void launch_contract(const DevicePlan &dp, int N, const void *Q, const void *x, const double2 *t,
-                     int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s, int t0 = 0, int ntl = -1);
+                     int mode, void *y, bool q_fp32, bool io_fp32, hipStream_t s, int t0 = 0, int ntl = -1,
+                     const int *gate = nullptr);
 // Resident pair-factor table: bytes, build, and the streaming pair GEMM.
 // Both work on target tiles [t0, t0 + ntl) (ntl < 0: to the last tile) of a
 // table whose pointer is at tile t0, so a slab of a few tiles can stand in
@@ -177,7 +178,99 @@ constexpr bool table_shape_offered(int shape, bool fp32) {
 }
 // grouping: a table GEMM shape (above).
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
-                            int grouping, void *Q, hipStream_t s, int t0 = 0, int ntl = -1);
+                            int grouping, void *Q, hipStream_t s, int t0 = 0, int ntl = -1,
+                            const int *gate = nullptr);
+
+// fp16 pair GEMM (half.hip): table and x-term sizes, the table build (pexp:
+// per-order exponents, dmin: device scratch for the smallest pair distance),
+// the conversion of x to its fp16 terms (colmax: nmax unsigned of scratch)
+// and the GEMM into the complex64 Q.
+size_t half_table_term_bytes(const DevicePlan &dp, int N);
+size_t half_x_bytes(const DevicePlan &dp, int N);
+void launch_half_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, void *table, bool split,
+                             int *pexp, unsigned long long *dmin, hipStream_t s, double near2);
+void launch_half_x(const DevicePlan &dp, int N, const double2 *x, void *xh, bool split, unsigned *colmax, int *xexp,
+                   hipStream_t s, const int *gate = nullptr);
+void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void *xh, bool split, const int *pexp,
+                      const int *xexp, void *Q, hipStream_t s, const int *gate = nullptr);
+
+// Vector kernels of the device GMRES (krylov.hip).  Reductions leave one row
+// of 4 doubles per block in partials (kRedBlocks rows); the last block sums
+// them in block order and resets counter.
+struct Reducer {
+    double *partials;
+    unsigned *counter;
+};
+// w -= h_prev v_prev (when v_prev); out = [Re, Im] <v_next, w> and ||w||^2.
+void launch_mgs_step(double2 *w, const double2 *v_prev, const double2 *h_prev, const double2 *v_next,
+                     long n, Reducer red, double *out, hipStream_t s);
+void launch_scale(double2 *y, const double2 *x, double a, long n, hipStream_t s);
+void launch_add(double2 *x, const double2 *d, long n, hipStream_t s);
+void launch_update(double2 *x, const double2 *V, const double2 *y, int m, long n, long ld, hipStream_t s);
+void launch_residual(double2 *r, const double2 *b, const double2 *ax, long n, Reducer red, double *out,
+                     hipStream_t s);
+void launch_norm2(const double2 *x, long n, Reducer red, double *out, hipStream_t s);
+void launch_to_fp32(float *dst, const double *src, long count, hipStream_t s, const int *gate = nullptr);
+void launch_to_fp64(double *dst, const float *src, long count, hipStream_t s);
+void launch_fill_test(double *dst, long count, hipStream_t s);  // timing input in [-1, 1)
+
+// Device state of one GMRES cycle of the single-solve driver.  The step
+// kernels (operator included) run only while active == 1; the Givens kernel
+// of the step that ends the cycle clears it.
+struct ArnoldiState {
+    int active;       // 1 while the cycle runs
+    int j;            // Arnoldi steps done in the cycle
+    int breakdown;    // the last step found an invariant subspace
+    int pad;
+    double target;    // stop once the residual estimate is at most this (SciPy's ptol)
+    double presid;    // residual estimate after the last step
+    double inv_norm;  // 1 / h[j+1, j] of the last step
+    double h0sq;      // ||w||^2 of the current step before orthogonalisation
+    double wsq;       // ||w||^2 of the current step after the first projection
+};
+
+// Krylov basis, Hessenberg and Givens state of the single-solve GMRES.
+struct ArnoldiBuffers {
+    double2 *V;              // [restart + 1][n] basis
+    long n;
+    int restart, nblk;       // nblk: blocks of the Gram-Schmidt passes
+    double2 *H;              // [restart][restart + 1] rotated Hessenberg columns
+    double *gc;              // [restart] Givens cosines
+    double2 *gs;             // [restart] Givens sines
+    double2 *g;              // [restart + 1] rotated right-hand side
+    double2 *h1, *h2;        // [restart + 1] Gram-Schmidt coefficients of the current step
+    double2 *partA, *partB;  // [restart + 2][nblk] per-block partial sums (slot-major)
+    double2 *y;              // [restart] cycle-end coefficients
+    double *hist;            // [restart] residual estimate after each step of the cycle
+    ArnoldiState *st;
+};
+int arnoldi_blocks(long n);         // nblk for vectors of n elements
+int arnoldi_max_restart();          // largest restart the step kernels take
+// Cycle start: V[0] /= sqrt(*nrm2) (device scalar ||V[0]||^2), g = [sqrt(*nrm2), 0, ...],
+// state = {active, j = 0, target}.
+void launch_arnoldi_start(const ArnoldiBuffers &a, const double *nrm2, double target, hipStream_t s);
+// Orthogonalisation, Givens update and normalisation of step j (w = V[j+1]
+// holds the operator applied to V[j]).
+void launch_arnoldi_step(const ArnoldiBuffers &a, int j, hipStream_t s);
+// Cycle end: y = H^-1 g over the cycle's steps, x += V y.
+void launch_arnoldi_end(const ArnoldiBuffers &a, double2 *x, hipStream_t s);
+
+// Block-diagonal preconditioner (precond.hip).
+struct BlockPrecond;
+BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &ab5,
+                             const double *pos_host, int N, const int *box_of_particle, int num_boxes,
+                             double k, const double2 *t_dev, bool fp32_factors, hipStream_t s);
+void precond_destroy(BlockPrecond *p);
+void precond_apply(BlockPrecond *p, const double2 *r, double2 *z, hipStream_t s, const int *gate = nullptr);
+
+// Observables (observables.hip).
+void launch_far_field(const double4 *pos, const double2 *f, int N, int lmax, double k,
+                      const double4 *dir, const double *polar, const double *azimuthal, int A,
+                      double2 *F, double2 *e_theta, double2 *e_phi, hipStream_t s);
+void csca_quadrature_grid(double kR, int lmax, int *n_theta, int *n_phi);
+size_t csca_quadrature_partials(int n_theta, int n_phi);
+void launch_csca_quadrature(const double4 *pos, const double2 *f, int N, int lmax, double k, int n_theta,
+                            int n_phi, double *partials, double *out, hipStream_t s);
 
 // Device allocation owned for a scope (freed on every exit path).
 template <typename T>

@@ -68,7 +68,9 @@ __global__ __launch_bounds__(64 * WAVES) void pair_table_gemm_kernel(
     const typename Cplx<T>::type *__restrict__ table, const typename Cplx<T>::type *__restrict__ x, int N,
     int N4, int nmax, const PairJob *__restrict__ jobs, int ntl, int t0, const int *__restrict__ colmap,
     int maxcols, int lmax, const int *__restrict__ dmoff, const int *__restrict__ order_tiles,
-    T *__restrict__ Q, int nq) {
+    T *__restrict__ Q, int nq, const int *__restrict__ gate) {
+    if (gate && *gate != 1)
+        return;
     const int jid = blockIdx.x / ntl;
     const int t = blockIdx.x - jid * ntl;  // tile within the slab
     const int dm = uniform(jobs[jid].dm);
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(64 * WAVES) void pair_table_gemm_kernel(
 
 template <typename T, int WAVES, int NTLMAX>
 void gemm_g(const DevicePlan &dp, int g, int N, const void *table, const void *x, void *Q, hipStream_t s,
-            int t0, int ntl) {
+            int t0, int ntl, const int *gate) {
     typedef typename Cplx<T>::type T2;
     const int N4 = (N + 3) / 4;
     const dim3 grid(dp.ntjobs[g] * ntl), block(64 * WAVES);
@@ -107,17 +109,17 @@ void gemm_g(const DevicePlan &dp, int g, int N, const void *table, const void *x
     if (2 * dp.lmax + 1 <= 15 || kNipLo == kNipHi)
         hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipLo, NTLMAX>), grid, block, 0, s, tb, xv, N, N4,
                            dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, dp.order_tiles, q,
-                           dp.nq);
+                           dp.nq, gate);
     else
         hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipHi, NTLMAX>), grid, block, 0, s, tb, xv, N, N4,
                            dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, dp.order_tiles, q,
-                           dp.nq);
+                           dp.nq, gate);
     YH_CHECK(hipGetLastError());
 }
 
 template <typename T>
 void gemm_t(const DevicePlan &dp, int shape, int N, const void *table, const void *x, void *Q, hipStream_t s,
-            int t0, int ntl) {
+            int t0, int ntl, const int *gate) {
     static_assert(kTableGrouping[0] == 2 && kTableGrouping[1] == 4 && kTableGrouping[2] == 8,
                   "instances follow the column-tile groupings");
     YH_REQUIRE(table_shape_offered(shape, sizeof(T) == sizeof(float)),
@@ -125,16 +127,16 @@ void gemm_t(const DevicePlan &dp, int shape, int N, const void *table, const voi
     const int g = shape % kTableGroupings;
     if (shape < kTableGroupings) {  // 8 waves
         if (g == 0)
-            gemm_g<T, 8, 2>(dp, g, N, table, x, Q, s, t0, ntl);
+            gemm_g<T, 8, 2>(dp, g, N, table, x, Q, s, t0, ntl, gate);
         else if (g == 1)
-            gemm_g<T, 8, 4>(dp, g, N, table, x, Q, s, t0, ntl);
+            gemm_g<T, 8, 4>(dp, g, N, table, x, Q, s, t0, ntl, gate);
         else if constexpr (sizeof(T) == sizeof(float))
-            gemm_g<T, 8, 8>(dp, g, N, table, x, Q, s, t0, ntl);
+            gemm_g<T, 8, 8>(dp, g, N, table, x, Q, s, t0, ntl, gate);
     } else {  // 4 waves
         if (g == 0)
-            gemm_g<T, 4, 2>(dp, g, N, table, x, Q, s, t0, ntl);
+            gemm_g<T, 4, 2>(dp, g, N, table, x, Q, s, t0, ntl, gate);
         else if constexpr (sizeof(T) == sizeof(float))
-            gemm_g<T, 4, 4>(dp, g, N, table, x, Q, s, t0, ntl);
+            gemm_g<T, 4, 4>(dp, g, N, table, x, Q, s, t0, ntl, gate);
     }
 }
 
@@ -175,7 +177,7 @@ void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, do
 }
 
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,
-                            int grouping, void *Q, hipStream_t s, int t0, int ntl) {
+                            int grouping, void *Q, hipStream_t s, int t0, int ntl, const int *gate) {
     YH_REQUIRE(grouping >= 0 && grouping < kTableShapes, "unknown table GEMM shape");
     const int ntiles1 = (N + 15) / 16;
     if (ntl < 0)
@@ -184,9 +186,9 @@ void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, cons
     YH_REQUIRE(table_gemm_extent_ok(N, dp.nmax, dp.lmax, fp32 ? sizeof(float2) : sizeof(double2)),
                "cluster too large for the table GEMM's 32-bit offsets");
     if (fp32)
-        gemm_t<float>(dp, grouping, N, table, x, Q, s, t0, ntl);
+        gemm_t<float>(dp, grouping, N, table, x, Q, s, t0, ntl, gate);
     else
-        gemm_t<double>(dp, grouping, N, table, x, Q, s, t0, ntl);
+        gemm_t<double>(dp, grouping, N, table, x, Q, s, t0, ntl, gate);
 }
 
 }  // namespace yasfhip

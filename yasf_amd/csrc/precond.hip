@@ -492,7 +492,10 @@ __global__ __launch_bounds__(kApplyThreads) void lu_apply_kernel(const int *__re
                                                        const int *__restrict__ dinv_off,
                                                        const double2 *__restrict__ dinv,
                                                        const double2 *__restrict__ r,
-                                                       double2 *__restrict__ z) {
+                                                       double2 *__restrict__ z,
+                                                       const int *__restrict__ gate) {
+    if (gate && *gate != 1)
+        return;
     extern __shared__ double2 y[];
     __shared__ double2 yb[16];
     __shared__ double2 Dsh[256];
@@ -813,15 +816,15 @@ void precond_destroy(BlockPrecond *p) {
     delete p;
 }
 
-void precond_apply(BlockPrecond *p, const double2 *r, double2 *z, hipStream_t s) {
+void precond_apply(BlockPrecond *p, const double2 *r, double2 *z, hipStream_t s, const int *gate) {
     if (p->blocks32)
         hipLaunchKernelGGL(lu_apply_kernel<float2>, dim3(p->nbox), dim3(kApplyThreads), p->max_m * sizeof(double2),
                            s, p->box_first, p->particles, p->nmax, p->blk_off, p->piv_off, p->blocks32, p->pivots,
-                           p->dinv_off, p->dinv, r, z);
+                           p->dinv_off, p->dinv, r, z, gate);
     else
         hipLaunchKernelGGL(lu_apply_kernel<double2>, dim3(p->nbox), dim3(kApplyThreads),
                            p->max_m * sizeof(double2), s, p->box_first, p->particles, p->nmax, p->blk_off,
-                           p->piv_off, p->blocks, p->pivots, p->dinv_off, p->dinv, r, z);
+                           p->piv_off, p->blocks, p->pivots, p->dinv_off, p->dinv, r, z, gate);
     YH_CHECK(hipGetLastError());
 }
 

@@ -36,38 +36,11 @@
 #include <memory>
 
 #include "../../include/yasfhip.h"
+#include "givens.hpp"
 #include "half_gemm.hpp"
 #include "table_gemm.hpp"
 
 namespace yasfhip {
-
-// krylov.hip / observables-side helpers reused per request
-struct Reducer {
-    double *partials;
-    unsigned *counter;
-};
-void launch_update(double2 *x, const double2 *V, const double2 *y, int m, long n, long ld,
-                   hipStream_t s);
-void launch_residual(double2 *r, const double2 *b, const double2 *ax, long n, Reducer red, double *out,
-                     hipStream_t s);
-void launch_norm2(const double2 *x, long n, Reducer red, double *out, hipStream_t s);
-void launch_mgs_step(double2 *w, const double2 *v_prev, const double2 *h_prev, const double2 *v_next,
-                     long n, Reducer red, double *out, hipStream_t s);
-void launch_to_fp32(float *dst, const double *src, long count, hipStream_t s);
-// half.hip: the pair GEMM on the fp16 matrix cores
-size_t half_table_term_bytes(const DevicePlan &dp, int N);
-size_t half_x_bytes(const DevicePlan &dp, int N);
-void launch_half_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, void *table, bool split,
-                             int *pexp, unsigned long long *dmin, hipStream_t s, double near2);
-void launch_half_x(const DevicePlan &dp, int N, const double2 *x, void *xh, bool split, unsigned *colmax, int *xexp,
-                   hipStream_t s);
-void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void *xh, bool split, const int *pexp,
-                      const int *xexp, void *Q, hipStream_t s);
-// observables.hip: C_sca by quadrature of the far field
-void csca_quadrature_grid(double kR, int lmax, int *n_theta, int *n_phi);
-size_t csca_quadrature_partials(int n_theta, int n_phi);
-void launch_csca_quadrature(const double4 *pos, const double2 *f, int N, int lmax, double k, int n_theta,
-                            int n_phi, double *partials, double *out, hipStream_t s);
 
 namespace {
 
@@ -481,32 +454,6 @@ __global__ __launch_bounds__(kMaxRestart + 1) void serve_reduce_kernel(const Bat
     }
 }
 
-__device__ __forceinline__ double cabs2(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
-
-// LAPACK zlartg semantics: [c s; -conj(s) c] [f; g] = [r; 0], c real.
-__device__ void lartg_dev(double2 f, double2 g, double &c, double2 &s, double2 &r) {
-    if (g.x == 0.0 && g.y == 0.0) {
-        c = 1.0;
-        s = make_double2(0.0, 0.0);
-        r = f;
-        return;
-    }
-    if (f.x == 0.0 && f.y == 0.0) {
-        c = 0.0;
-        const double ag = cabs2(g);
-        s = make_double2(g.x / ag, -g.y / ag);
-        r = make_double2(ag, 0.0);
-        return;
-    }
-    const double af = cabs2(f), ag = cabs2(g);
-    const double d = sqrt(af * af + ag * ag);
-    c = af / d;
-    const double2 sg = make_double2(f.x / af, f.y / af);  // f / |f|
-    // s = sg * conj(g) / d
-    s = make_double2((sg.x * g.x + sg.y * g.y) / d, (sg.y * g.x - sg.x * g.y) / d);
-    r = make_double2(sg.x * d, sg.y * d);
-}
-
 // Hessenberg column j, Givens rotation, residual estimate and the cycle's
 // stopping decision (SciPy's inner loop, iterative.py:755-795).  One thread
 // per request.
@@ -533,28 +480,7 @@ __global__ void serve_givens_kernel(const BatchReq *__restrict__ reqs, ReqState 
         col[j + 1] = make_double2(0.0, 0.0);
         breakdown = true;
     }
-    for (int k = 0; k < j; ++k) {
-        const double c = q.gc[k];
-        const double2 sn = q.gs[k];
-        const double2 n0 = col[k], n1 = col[k + 1];
-        // col[k] = c n0 + sn n1 ; col[k+1] = -conj(sn) n0 + c n1
-        col[k] = make_double2(c * n0.x + sn.x * n1.x - sn.y * n1.y, c * n0.y + sn.x * n1.y + sn.y * n1.x);
-        col[k + 1] = make_double2(-(sn.x * n0.x + sn.y * n0.y) + c * n1.x,
-                                  -(sn.x * n0.y - sn.y * n0.x) + c * n1.y);
-    }
-    double c;
-    double2 sn, mag;
-    lartg_dev(col[j], col[j + 1], c, sn, mag);
-    q.gc[j] = c;
-    q.gs[j] = sn;
-    col[j] = mag;
-    col[j + 1] = make_double2(0.0, 0.0);
-    const double2 gj = q.g[j];
-    // tmp = -conj(sn) g[j]
-    const double2 tmp = make_double2(-(sn.x * gj.x + sn.y * gj.y), -(sn.x * gj.y - sn.y * gj.x));
-    q.g[j] = make_double2(c * gj.x, c * gj.y);
-    q.g[j + 1] = tmp;
-    s.presid = cabs2(tmp);
+    s.presid = givens_column(col, j, q.gc, q.gs, q.g);
     s.inv_norm = breakdown ? 1.0 : 1.0 / hn;
     s.breakdown = breakdown ? 1 : 0;
     s.j = j + 1;
@@ -566,19 +492,7 @@ __global__ void serve_givens_kernel(const BatchReq *__restrict__ reqs, ReqState 
 // Cycle end: y = H^-1 g (upper triangular, m columns); one thread.
 __global__ void serve_backsolve_kernel(const double2 *__restrict__ H, const double2 *__restrict__ g, int m,
                                        int R1, double2 *__restrict__ y) {
-    for (int k = m - 1; k >= 0; --k) {
-        double2 v = g[k];
-        for (int i = k + 1; i < m; ++i) {
-            const double2 hik = H[static_cast<long>(i) * R1 + k];
-            const double2 yi = y[i];
-            v.x -= hik.x * yi.x - hik.y * yi.y;
-            v.y -= hik.x * yi.y + hik.y * yi.x;
-        }
-        const double2 d = H[static_cast<long>(k) * R1 + k];
-        const double dd = d.x * d.x + d.y * d.y;
-        y[k] = dd > 0.0 ? make_double2((v.x * d.x + v.y * d.y) / dd, (v.y * d.x - v.x * d.y) / dd)
-                        : make_double2(0.0, 0.0);
-    }
+    backsolve_dev(H, g, m, R1, y);
 }
 
 // b = t a, x = b (x0 = b), at admission.

@@ -215,3 +215,49 @@ def solve_sequential(requests: list[ServeRequest], rtol: float = 1e-6, restart: 
     lat = np.array([v["latency_s"] for v in results.values()])
     return ServeReport(len(results), makespan, len(results) / makespan, float(np.percentile(lat, 50)),
                        float(np.percentile(lat, 99)), results)
+
+
+def main(argv=None):
+    """Command-line driver: serves the seeded request stream on one server and
+    prints the report as one JSON line.  Example:
+    ``python3 -m yasf_amd.serving --requests 512 --precision fp16x2 --eager``."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--rate", type=float, default=math.inf, help="Poisson arrivals per second (inf: all at once)")
+    ap.add_argument("--precision", choices=["fp16x2", "fp16", "fp32"], default="fp16x2",
+                    help="inner operator precision")
+    ap.add_argument("--restart", type=int, default=50)
+    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch each iteration's kernels directly instead of replaying a captured graph")
+    args = ap.parse_args(argv)
+
+    import torch
+
+    torch.cuda.set_device(0)
+    from yasf_amd import _native
+
+    prec = {"fp16x2": _native.FP16X2, "fp16": _native.FP16, "fp32": _native.FP32}[args.precision]
+    reqs = make_workload(args.requests, seed=args.seed, rate=args.rate)
+    srv = _native.Server(restart=args.restart, inner_rtol=args.inner_rtol, use_graph=not args.eager)
+    try:
+        if not args.eager:
+            for lmax in range(3, 9):
+                for particles in (400, 2000):
+                    srv.autotune(lmax, particles=particles, reps=3)
+        srv.set_inner_precision(prec)
+        rep = serve(srv, reqs, device_setup=True, progress=lambda m: print(m, flush=True))
+    finally:
+        srv.close()
+    out = rep.as_dict()
+    out.update(precision=args.precision, rate=args.rate, graph=not args.eager,
+               converged=sum(1 for r in rep.results.values() if r.get("info") == 0))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
