@@ -94,6 +94,36 @@ def factorised_flops_per_pair(lmax: int) -> int:
     return 8 * len(triples)
 
 
+def trimmed_table_macs_per_pair(lmax: int) -> int:
+    """Complex MACs per particle pair that the FP64 table GEMM issues when
+    each order runs over only the column tiles it feeds (the one-order-at-a-
+    time shape): per (dm, order p), 16 columns per tile up to the last tile
+    holding a source column with a nonzero ab5 term at p (columns of a dm
+    ordered by l descending, as the coupling plan orders them)."""
+    from yasf_amd import _native
+
+    ab5 = _native.translation_table(lmax)
+    ls, ms = [], []
+    for _tau in (1, 2):
+        for l in range(1, lmax + 1):
+            for m in range(-l, l + 1):
+                ls.append(l)
+                ms.append(m)
+    ls, ms = np.array(ls), np.array(ms)
+    nz = ab5 != 0  # [n2][n1][p]
+    total = 0
+    for dm in range(-2 * lmax, 2 * lmax + 1):
+        cols = [n2 for n2 in range(len(ms)) if abs(ms[n2] - dm) <= lmax]
+        cols.sort(key=lambda n2: -ls[n2])  # stable: ties keep mode order
+        need = np.zeros(2 * lmax + 1, dtype=int)
+        for c, n2 in enumerate(cols):
+            n1 = np.nonzero(ms[n2] - ms == dm)[0]
+            ps = np.nonzero(nz[n2, n1, :].any(axis=0))[0]
+            need[ps] = np.maximum(need[ps], c // 16 + 1)
+        total += 16 * int(need.sum())
+    return total
+
+
 def direct_flops_per_pair(lmax: int) -> int:
     """8 T_nz + 8 nmax^2: the direct per-pair count of the reference loop."""
     from yasf_amd import _native
@@ -322,14 +352,20 @@ def run_solve(args):
     f_fact = factorised_flops_per_pair(lmax) * pairs
     f_direct = direct_flops_per_pair(lmax) * pairs
     macs_issued, _ = _native.plan_stats(lmax)
+    if tuned["fp64"] == 2:  # one order at a time over the tiles it feeds
+        macs_issued = trimmed_table_macs_per_pair(lmax)
     ntiles = (N + 15) // 16
     # real MFMA flops issued, tile padding included: 3 real products (6 flop)
     # per complex MAC in the FP64 GEMM (Gauss's form)
     f_issued = macs_issued * 6 * (ntiles * 16) * N
     achieved = f_fact / (pair_ms * 1e-3) / 1e12
     waves64, tiles64 = _native.Session.TABLE_SHAPES[tuned["fp64"]]
-    # instance name as rocprofv3 reports it: <T, waves, orders per wave (lmax <= 7), column tiles>
-    kernel64 = f"pair_table_gemm_kernel<double, {waves64}, {-(-15 // waves64)}, {tiles64}>"
+    # instance name as rocprofv3 reports it: <T, waves, orders per wave (lmax <= 7), column tiles,
+    # one order at a time>; FP64 shape 2 runs jobs of up to 6 column tiles one order at a time
+    if tuned["fp64"] == 2:
+        kernel64 = "pair_table_gemm_kernel<double, 8, 1, 6, true>"
+    else:
+        kernel64 = f"pair_table_gemm_kernel<double, {waves64}, {-(-15 // waves64)}, {tiles64}, false>"
     achieved32 = f_fact / (pair32_ms * 1e-3) / 1e12
 
     line = {
@@ -353,7 +389,9 @@ def run_solve(args):
                    "box_cap": args.box_cap if use_pc else None,
                    "matvec": "resident FP64 pair-factor table + ab5 contraction",
                    "table_gemm_shape": {k: {"waves": _native.Session.TABLE_SHAPES[v][0],
-                                            "column_tiles": _native.Session.TABLE_SHAPES[v][1]}
+                                            "column_tiles": 6 if (k == "fp64" and v == 2)
+                                            else _native.Session.TABLE_SHAPES[v][1],
+                                            "orders_at_once": 1 if (k == "fp64" and v == 2) else "all"}
                                         for k, v in tuned.items()}},
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),

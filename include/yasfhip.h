@@ -276,13 +276,14 @@ int yasfhip_set_slab_tiles(yasfhip_session *s, int tiles);
  * precision: times `reps` launches of each shape and keeps the fastest for
  * this session.  Shapes 0, 1, 2: 8 waves per block with jobs of at most 2, 4
  * or 8 16-column tiles per dm; 3, 4: 4 waves with 2 or 4 (fewer waves give
- * each more orders).  FP64 offers 0, 1 and 3, FP32 0-4: wider jobs stream
- * each table slice fewer times but hold more accumulators.  Returns the
+ * each more orders).  FP32 offers 0-4, FP64 0-3, where shape 2 runs blocks of
+ * 4 waves with one order per wave (8 tiles of two orders exceed a wave's
+ * registers): wider jobs stream each table slice fewer times but hold more
+ * accumulators.  Returns the
  * index in *grouping; ms[6] receives the mean times (-1 for shapes not
  * offered; all -1 when the session generates the pair factors on the fly, in
  * which case nothing changes).  The table (or first slab) is built first if
- * needed.  Until tuned (or set), a session uses shape 1 in FP64 and 2 in
- * FP32.  Results agree between shapes to rounding (the orders a block cuts
+ * needed.  Until tuned (or set), a session uses shape 2 in both.  Results agree between shapes to rounding (the orders a block cuts
  * along K depend on the job's order count). */
 int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps, int *grouping,
                         double *ms);
@@ -291,9 +292,10 @@ int yasfhip_tune_matvec(yasfhip_session *s, int channel, int precision, int reps
  * without timing.  No counterpart in the reference. */
 int yasfhip_set_table_shape(yasfhip_session *s, int precision, int shape);
 
-/* Timing of `reps` device-resident matvecs of one channel on the session
- * stream with HIP events: ms_pair = mean duration of the pair kernel,
- * ms_total = mean duration of the whole matvec (pair + contraction).  The
+/* Timing of device-resident matvecs of one channel on the session stream
+ * with HIP events around back-to-back launches: ms_pair = mean duration of
+ * the pair kernel over `reps` launches in a row, ms_total = mean duration of
+ * the whole matvec (pair + contraction) over `reps` in a row.  The
  * input is a deterministic pseudo-random vector (the GEMMs run measurably
  * faster on all-zero operands, which draw less power); the tuner above
  * times the same input. */

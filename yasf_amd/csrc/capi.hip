@@ -79,10 +79,11 @@ struct yasfhip_session {
     void *table[2] = {nullptr, nullptr};
     int table_channel[2] = {-1, -1};
     bool table_refused[2] = {false, false};
-    // table GEMM shape per precision (waves x column tiles) until the tuner
-    // picks one: 8 waves with jobs of 4 tiles (FP64) and of 8 (FP32), the
-    // choices at 2,000 spheres, lmax 6
-    int table_grouping[2] = {1, 2};
+    // table GEMM shape per precision until the tuner picks one: shape 2 in
+    // both (FP64: 8 waves, one order at a time over jobs of up to 6 tiles;
+    // FP32: 8 waves, jobs of 8 tiles), the fastest at 600 spheres lmax 4,
+    // 2,000 spheres lmax 6 and 2,771 spheres lmax 8
+    int table_grouping[2] = {2, 2};
     double near2 = kAllPairs;          // squared near-field radius of W (kAllPairs: every pair)
     // Slab of the pair table (a few target tiles), rebuilt tile range by tile
     // range inside every matvec when the full table does not fit.
@@ -1123,28 +1124,37 @@ int yasfhip_time_matvec(yasfhip_session *s, int channel, int precision, int reps
             s->half_table_for(channel, split);
             launch_half_x(s->dp, s->N, s->xin, s->xh, split, s->hcolmax, s->hxexp, st);
         }
-        for (int r = 0; r < reps; ++r) {
-            const void *xv = fp32 ? static_cast<const void *>(s->x32) : static_cast<const void *>(s->xin);
-            if (half)
-                s->half_table_for(channel, split);
-            else
-                s->table_for(channel, fp32);  // a (re)build is not part of the timed matvec
-            YH_CHECK(hipEventRecord(s->ev[0], st));
+        // Back-to-back launches between two events, as the solver issues
+        // them: reps pair GEMMs, then reps whole matvecs (pair GEMM +
+        // contraction).  A (re)build of the table is not part of either.
+        const void *xv = fp32 ? static_cast<const void *>(s->x32) : static_cast<const void *>(s->xin);
+        if (half)
+            s->half_table_for(channel, split);
+        else
+            s->table_for(channel, fp32);
+        auto pair = [&] {
             if (half)
                 launch_half_gemm(s->dp, s->N, s->htable, s->xh, split, s->hpexp, s->hxexp, s->Q, st);
             else
                 s->pair_gemm(channel, xv, fp32);
-            YH_CHECK(hipEventRecord(s->ev[1], st));
+        };
+        pair();  // warm-up
+        YH_CHECK(hipEventRecord(s->ev[0], st));
+        for (int r = 0; r < reps; ++r)
+            pair();
+        YH_CHECK(hipEventRecord(s->ev[1], st));
+        for (int r = 0; r < reps; ++r) {
+            pair();
             launch_contract(s->dp, s->N, s->Q, s->xin, s->t + static_cast<size_t>(channel) * s->n, 0,
                             s->yout, fp32 || half, false, st);
-            YH_CHECK(hipEventRecord(s->ev[2], st));
-            YH_CHECK(hipEventSynchronize(s->ev[2]));
-            float a = 0.f, b = 0.f;
-            YH_CHECK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
-            YH_CHECK(hipEventElapsedTime(&b, s->ev[0], s->ev[2]));
-            pair_ms += a;
-            total_ms += b;
         }
+        YH_CHECK(hipEventRecord(s->ev[2], st));
+        YH_CHECK(hipEventSynchronize(s->ev[2]));
+        float a = 0.f, b = 0.f;
+        YH_CHECK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+        YH_CHECK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
+        pair_ms = a;
+        total_ms = b;
         *ms_pair = pair_ms / reps;
         *ms_total = total_ms / reps;
     });

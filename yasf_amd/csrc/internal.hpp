@@ -84,6 +84,11 @@ constexpr int kRedPartials = kRedBlocks * 4;  // doubles in one launch's partial
 constexpr int kTableGroupings = 3;
 constexpr int kTableGrouping[kTableGroupings] = {2, 4, 8};
 constexpr int kTableMaxColTiles = 8;
+// Column tiles per job of the complex128 GEMM that computes one order at a
+// time (table shape 2 in FP64): 6 x 3 accumulators of 8 VGPRs and a
+// three-deep operand ring fit a wave's registers; every dm of lmax <= 6 has
+// at most 6 column tiles, so each table slice streams once.
+constexpr int kTableWideTiles64 = 6;
 
 constexpr int kWavesPerJob = 4;
 constexpr int kWaveMaxTiles = 8;                                // C tiles per wave
@@ -97,6 +102,7 @@ struct CouplingPlan {
     std::vector<int> colmap;          // [ndm][maxcols] -> source mode n2 or -1
     std::vector<PairJob> jobs;        // per 16-particle target tile, sorted by cost
     std::vector<PairJob> table_jobs[kTableGroupings];  // resident-table GEMM jobs, sorted by cost
+    std::vector<PairJob> table_jobs_w64;  // jobs of up to kTableWideTiles64 column tiles, sorted by cost
     int stage_elems = 0;              // complex elements of one pair-kernel LDS stage
     std::vector<int> crow;            // CSR row offsets per target mode n1 (nmax+1)
     std::vector<int> cidx;            // Q column index q*16 + col
@@ -121,6 +127,8 @@ struct DevicePlan {
     PairJob *jobs;
     PairJob *tjobs[kTableGroupings];
     int ntjobs[kTableGroupings];
+    PairJob *tjobs_w64;  // CouplingPlan::table_jobs_w64
+    int ntjobs_w64;
     int *colmap, *crow, *cidx;
     double2 *cval;
     double *la, *lb, *ld;
@@ -169,12 +177,14 @@ void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, do
                              const int *gate = nullptr, bool regular = false, double near2 = kAllPairs);
 // Shapes of the resident-table GEMM: shape % kTableGroupings indexes
 // kTableGrouping (column tiles per job), shape / kTableGroupings picks 8 or
-// 4 waves per block.  Complex128 offers 8 waves with 2 or 4 column tiles and
-// 4 waves with 2 (wider jobs do not fit a wave's registers).
+// 4 waves per block.  Complex128 offers 8 waves with 2 or 4 column tiles, 4
+// waves with 2, and as shape 2 8 waves that compute their orders one at a
+// time over jobs of up to kTableWideTiles64 tiles (two orders at once over 8
+// tiles do not fit a wave's registers); complex64 offers all but 4 waves
+// with 8 tiles.
 constexpr int kTableShapes = 2 * kTableGroupings;
 constexpr bool table_shape_offered(int shape, bool fp32) {
-    return shape >= 0 && shape < kTableShapes &&
-           (fp32 ? shape != 5 : (shape == 0 || shape == 1 || shape == 3));
+    return shape >= 0 && shape < kTableShapes && (fp32 ? shape != 5 : shape <= 3);
 }
 // grouping: a table GEMM shape (above).
 void launch_pair_table_gemm(const DevicePlan &dp, int N, const void *table, const void *x, bool fp32,

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void pair_table_build_kernel(
 // each wave more orders: more MFMAs per K-step to hide the operand loads
 // behind, at more accumulator registers.  The grid covers ntl target tiles
 // from t0 (table points at tile t0).
-template <typename T, int WAVES, int NIPMAX, int NTLMAX>
+template <typename T, int WAVES, int NIPMAX, int NTLMAX, bool SEQ>
 __global__ __launch_bounds__(64 * WAVES) void pair_table_gemm_kernel(
     const typename Cplx<T>::type *__restrict__ table, const typename Cplx<T>::type *__restrict__ x, int N,
     int N4, int nmax, const PairJob *__restrict__ jobs, int ntl, int t0, const int *__restrict__ colmap,
@@ -92,28 +92,32 @@ __global__ __launch_bounds__(64 * WAVES) void pair_table_gemm_kernel(
                                   order_tiles + d * P + (dm < 0 ? -dm : dm),
                                   colmap + d * maxcols + tile0 * 16};
     T *Qt = Q + 2 * (static_cast<size_t>(t0 + t) * nq + qbase) * 256;
-    table_gemm_block<T, WAVES, NIPMAX, NTLMAX>(op, nord, ntile, Qt, tstride);
+    table_gemm_block<T, WAVES, NIPMAX, NTLMAX, SEQ>(op, nord, ntile, Qt, tstride);
 }
 
-template <typename T, int WAVES, int NTLMAX>
-void gemm_g(const DevicePlan &dp, int g, int N, const void *table, const void *x, void *Q, hipStream_t s,
-            int t0, int ntl, const int *gate) {
+// jobs: the job list (count njobs) of the shape's column-tile grouping.
+// SEQ: each wave computes its whole orders one at a time (complex128 jobs
+// wider than 4 column tiles, whose two-orders-at-once accumulators exceed a
+// wave's registers).
+template <typename T, int WAVES, int NTLMAX, bool SEQ = false>
+void gemm_g(const DevicePlan &dp, const PairJob *jobs, int njobs, int N, const void *table, const void *x, void *Q,
+            hipStream_t s, int t0, int ntl, const int *gate) {
     typedef typename Cplx<T>::type T2;
     const int N4 = (N + 3) / 4;
-    const dim3 grid(dp.ntjobs[g] * ntl), block(64 * WAVES);
+    const dim3 grid(njobs * ntl), block(64 * WAVES);
     const T2 *tb = static_cast<const T2 *>(table);
     const T2 *xv = static_cast<const T2 *>(x);
     T *q = static_cast<T *>(Q);
     // whole orders per wave for 2L+1 <= 15 and <= 17 orders
-    constexpr int kNipLo = table_gemm_nipmax<WAVES>(15), kNipHi = table_gemm_nipmax<WAVES>(17);
+    constexpr int kNipLo = SEQ ? 1 : table_gemm_nipmax<WAVES>(15), kNipHi = SEQ ? 1 : table_gemm_nipmax<WAVES>(17);
     if (2 * dp.lmax + 1 <= 15 || kNipLo == kNipHi)
-        hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipLo, NTLMAX>), grid, block, 0, s, tb, xv, N, N4,
-                           dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, dp.order_tiles, q,
-                           dp.nq, gate);
+        hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipLo, NTLMAX, SEQ>), grid, block, 0, s, tb, xv, N, N4,
+                           dp.nmax, jobs, ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, dp.order_tiles, q, dp.nq,
+                           gate);
     else
-        hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipHi, NTLMAX>), grid, block, 0, s, tb, xv, N, N4,
-                           dp.nmax, dp.tjobs[g], ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, dp.order_tiles, q,
-                           dp.nq, gate);
+        hipLaunchKernelGGL((pair_table_gemm_kernel<T, WAVES, kNipHi, NTLMAX, SEQ>), grid, block, 0, s, tb, xv, N, N4,
+                           dp.nmax, jobs, ntl, t0, dp.colmap, dp.maxcols, dp.lmax, dp.dmoff, dp.order_tiles, q, dp.nq,
+                           gate);
     YH_CHECK(hipGetLastError());
 }
 
@@ -125,18 +129,22 @@ void gemm_t(const DevicePlan &dp, int shape, int N, const void *table, const voi
     YH_REQUIRE(table_shape_offered(shape, sizeof(T) == sizeof(float)),
                "table GEMM shape not offered in this precision");
     const int g = shape % kTableGroupings;
+    const PairJob *jobs = dp.tjobs[g];
+    const int nj = dp.ntjobs[g];
     if (shape < kTableGroupings) {  // 8 waves
         if (g == 0)
-            gemm_g<T, 8, 2>(dp, g, N, table, x, Q, s, t0, ntl, gate);
+            gemm_g<T, 8, 2>(dp, jobs, nj, N, table, x, Q, s, t0, ntl, gate);
         else if (g == 1)
-            gemm_g<T, 8, 4>(dp, g, N, table, x, Q, s, t0, ntl, gate);
+            gemm_g<T, 8, 4>(dp, jobs, nj, N, table, x, Q, s, t0, ntl, gate);
         else if constexpr (sizeof(T) == sizeof(float))
-            gemm_g<T, 8, 8>(dp, g, N, table, x, Q, s, t0, ntl, gate);
+            gemm_g<T, 8, 8>(dp, jobs, nj, N, table, x, Q, s, t0, ntl, gate);
+        else  // complex128: one order at a time, jobs of up to kTableWideTiles64 column tiles
+            gemm_g<T, 8, kTableWideTiles64, true>(dp, dp.tjobs_w64, dp.ntjobs_w64, N, table, x, Q, s, t0, ntl, gate);
     } else {  // 4 waves
         if (g == 0)
-            gemm_g<T, 4, 2>(dp, g, N, table, x, Q, s, t0, ntl, gate);
+            gemm_g<T, 4, 2>(dp, jobs, nj, N, table, x, Q, s, t0, ntl, gate);
         else if constexpr (sizeof(T) == sizeof(float))
-            gemm_g<T, 4, 4>(dp, g, N, table, x, Q, s, t0, ntl, gate);
+            gemm_g<T, 4, 4>(dp, jobs, nj, N, table, x, Q, s, t0, ntl, gate);
     }
 }
 

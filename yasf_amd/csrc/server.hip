@@ -226,8 +226,9 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_gemm_kernel(c
 // One (job, target tile t) block of a request's complex64 pair GEMM; tab
 // points at target tile t of the request's pair table (or of a slab of it).
 // Table GEMM shape of the server's FP64 products (residuals, C_sca): 8 waves
-// with jobs of up to 4 column tiles, 20% faster than jobs of 2 at lmax 4 and 6.
-constexpr int kServeShape64 = 1;
+// computing one order at a time over jobs of up to 6 column tiles, the
+// session default (each table slice streamed once).
+constexpr int kServeShape64 = 2;
 // Table GEMM shape of a request's single complex64 product (the residual of
 // x0 = b): 8 waves with jobs of up to 8 column tiles, the session default.
 constexpr int kServeShape32 = 2;

@@ -199,9 +199,10 @@ __device__ __forceinline__ void table_gemm_ntl(const TableGemmOperands<T> &op, i
                                                int k1, int ntile, Emit &emit) {
     if constexpr (NTL <= NTLMAX) {
         if (ntile == NTL) {
-            // three operand stages unless the accumulators leave no room (a
-            // complex128 wave with 2 x 4 C tiles holds 192 accumulator VGPRs)
-            constexpr int kDepth = (sizeof(T) == sizeof(double) && NIP * NTL >= 8) ? 2 : 3;
+            // three operand stages unless the accumulators and operands leave
+            // no room (a complex128 wave with 2 x 4 C tiles holds 192
+            // accumulator VGPRs; 5 or 6 tiles of one order, 144 and 24 of x)
+            constexpr int kDepth = (sizeof(T) == sizeof(double) && (NIP * NTL >= 8 || NTL >= 5)) ? 2 : 3;
             constexpr bool kGauss = sizeof(T) == sizeof(double) || NTLMAX >= 8;
             table_gemm_wave<T, NIP, NTL, kDepth, kGauss>(op, ip0, ipstep, k0, k1, emit);
             return;
@@ -239,7 +240,12 @@ constexpr int table_gemm_nipmax(int P) {
 // in LDS in SIMD (= K) order, one SIMD per round, so the result does not
 // depend on timing.  (Order-granular dealing leaves the SIMDs 80% busy at
 // lmax 4, 86% at lmax 6.)  Blocks of fewer than 4 waves deal whole orders.
-template <typename T, int WAVES, int NIPMAX, int NTLMAX>
+// SEQ: a wave computes its whole orders one after another (one order's
+// accumulators at a time; the x fragments are read again per order, the
+// table fragments still once) instead of all at once, each over only the
+// column tiles that order feeds (otile): C tiles past that prefix hold no
+// term the contraction reads, so they are neither computed nor written.
+template <typename T, int WAVES, int NIPMAX, int NTLMAX, bool SEQ = false>
 __device__ __forceinline__ void table_gemm_block(const TableGemmOperands<T> &op, int nord, int ntile,
                                                  T *__restrict__ Qt, int tstride) {
     typedef typename Cplx<T>::type T2;
@@ -261,7 +267,20 @@ __device__ __forceinline__ void table_gemm_block(const TableGemmOperands<T> &op,
     } else {
         const int a = nord / 4, b = nord - 4 * a;
         const int nfull = 4 * a > wave ? (4 * a - wave + WAVES - 1) / WAVES : 0;
-        {
+        // column tiles order ip feeds within the job
+        auto order_tiles_of = [&](int ip) {
+            const int t = uniform(op.otile[ip]) - op.tile0;
+            return t < 0 ? 0 : (t > ntile ? ntile : t);
+        };
+        if constexpr (SEQ) {
+            for (int j = 0; j < nfull; ++j) {
+                const int ip = wave + WAVES * j;
+                const int tn = order_tiles_of(ip);
+                auto emit = to_q(ip, WAVES);
+                if (tn > 0)
+                    table_gemm_dispatch<T, 1, 1, NTLMAX>(op, ip, WAVES, 1, 0, op.N4, tn, emit);
+            }
+        } else {
             auto emit = to_q(wave, WAVES);
             if (nfull > 0)
                 table_gemm_dispatch<T, 1, NIPMAX, NTLMAX>(op, wave, WAVES, nfull, 0, op.N4, ntile, emit);
@@ -298,7 +317,10 @@ __device__ __forceinline__ void table_gemm_block(const TableGemmOperands<T> &op,
                     hold[c][it][i][0] = re;
                     hold[c][it][i][1] = im;
                 };
-                table_gemm_dispatch<T, 1, 1, NTLMAX>(op, 4 * a + piece_order(pc), 0, 1, k0, k1, ntile, emit);
+                const int ip = 4 * a + piece_order(pc);
+                const int tn = SEQ ? order_tiles_of(ip) : ntile;  // tiles past tn stay zero in hold
+                if (tn > 0)
+                    table_gemm_dispatch<T, 1, 1, NTLMAX>(op, ip, 0, 1, k0, k1, tn, emit);
             }
         }
         __shared__ T2 part[3][NTLMAX][256];

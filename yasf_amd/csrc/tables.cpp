@@ -249,6 +249,16 @@ CouplingPlan build_coupling_plan(int lmax) {
                 t0 += nt;
             }
         }
+        {
+            const int njob = (ntiles + kTableWideTiles64 - 1) / kTableWideTiles64;
+            for (int j = 0, t0 = 0; j < njob; ++j) {
+                const int nt = ntiles / njob + (j < ntiles % njob ? 1 : 0);
+                const int nord = orders_for(dmi, t0);
+                if (nord > 0)
+                    plan.table_jobs_w64.push_back(PairJob{dm, t0, nt, q, nord * nt, ntiles, nord, 0});
+                t0 += nt;
+            }
+        }
         q += R * ntiles;
     }
     plan.nq = q;
@@ -257,6 +267,7 @@ CouplingPlan build_coupling_plan(int lmax) {
     std::stable_sort(plan.jobs.begin(), plan.jobs.end(), by_cost);
     for (auto &tj : plan.table_jobs)
         std::stable_sort(tj.begin(), tj.end(), by_cost);
+    std::stable_sort(plan.table_jobs_w64.begin(), plan.table_jobs_w64.end(), by_cost);
     // fp16 GEMM jobs: the jobs of up to 4 column tiles with at most
     // kHalfJobOrders orders each; a longer order range is split and the
     // piece's first order goes to pad
@@ -344,6 +355,8 @@ DevicePlan upload_device_plan(const CouplingPlan &pl, hipStream_t s) {
         dp.tjobs[g] = upload_vec(pl.table_jobs[g], s);
         dp.ntjobs[g] = static_cast<int>(pl.table_jobs[g].size());
     }
+    dp.tjobs_w64 = upload_vec(pl.table_jobs_w64, s);
+    dp.ntjobs_w64 = static_cast<int>(pl.table_jobs_w64.size());
     dp.colmap = upload_vec(pl.colmap, s);
     dp.crow = upload_vec(pl.crow, s);
     dp.cidx = upload_vec(pl.cidx, s);
@@ -370,9 +383,9 @@ DevicePlan upload_device_plan(const CouplingPlan &pl, hipStream_t s) {
 }
 
 void free_device_plan(DevicePlan &dp) {
-    void *bufs[] = {dp.jobs, dp.tjobs[0], dp.tjobs[1], dp.tjobs[2], dp.colmap, dp.crow, dp.cidx,
-                    dp.cval, dp.la,       dp.lb,       dp.ld,       dp.dmoff,  dp.order_tiles, dp.hjobs[0],
-                    dp.hjobs[1]};
+    void *bufs[] = {dp.jobs,  dp.tjobs[0], dp.tjobs[1], dp.tjobs[2], dp.tjobs_w64,     dp.colmap,
+                    dp.crow,  dp.cidx,     dp.cval,     dp.la,       dp.lb,          dp.ld,
+                    dp.dmoff, dp.order_tiles, dp.hjobs[0], dp.hjobs[1]};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
