@@ -145,9 +145,6 @@ __global__ __launch_bounds__(64 * WAVES) void half_gemm_kernel(
 
 constexpr int kHalfWaves = 8;  // 2 orders per wave: kHalfJobOrders
 static_assert(2 * kHalfWaves == kHalfJobOrders, "orders per fp16 job = 2 per wave");
-// Column tiles per job: 4 for the two-term GEMM (its registers hold no
-// more), 8 for the one-term GEMM (each table slice streamed once).
-constexpr int half_job_tiles(bool split) { return split ? 4 : 8; }
 
 }  // namespace
 
@@ -207,14 +204,14 @@ void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void
     const _Float16 *lo = hi + half_table_term_bytes(dp, N) / sizeof(_Float16);
     const _Float16 *xhi = static_cast<const _Float16 *>(xh);
     const _Float16 *xlo = xhi + 2 * half_x_plane(dp.nmax, N32);
-    const int h = split ? 0 : 1;
+    const int h = kHalfJobList;
     const dim3 grid(dp.nhjobs[h] * ntiles1), block(64 * kHalfWaves);
     if (split)
-        hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, half_job_tiles(true), true>), grid, block, 0, s, hi, lo, xhi,
+        hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, kHalfJobTiles, true>), grid, block, 0, s, hi, lo, xhi,
                            xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntiles1, 0, dp.colmap, dp.maxcols,
                            dp.dmoff, static_cast<float *>(Q), dp.nq, gate);
     else
-        hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, half_job_tiles(false), false>), grid, block, 0, s, hi, lo,
+        hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, kHalfJobTiles, false>), grid, block, 0, s, hi, lo,
                            xhi, xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntiles1, 0, dp.colmap,
                            dp.maxcols, dp.dmoff, static_cast<float *>(Q), dp.nq, gate);
     YH_CHECK(hipGetLastError());

@@ -38,6 +38,11 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kHalfTop = 14;  // scaled magnitudes stay below 2^kHalfTop
 
+// Column tiles per fp16 GEMM job in both forms, and the plan's job list
+// with that many (DevicePlan::hjobs): each table slice streamed once.
+constexpr int kHalfJobTiles = 8;
+constexpr int kHalfJobList = 1;
+
 // Exponent e with bound * 2^e < 2^kHalfTop (0 for a zero or non-finite bound).
 __host__ __device__ inline int half_exponent(double bound) {
     if (!(bound > 0.0) || !(bound < HUGE_VAL))
@@ -83,11 +88,16 @@ __device__ __forceinline__ h8_t neg_h8(h8_t a) {
 // two K-steps ahead into three register sets used in turn; the B fragments of
 // a K-step (the job's column tiles, shared by all waves) are loaded by the
 // whole block one K-step ahead into one of two LDS buffers.  Operands come
-// through raw buffer loads: a wave's second order slot, when it has none,
-// and the B pieces of absent column tiles read out of range, which returns
-// zero without touching memory, so every load is unconditional and the
-// compiler's waits stay static.  One barrier per K-step hands the LDS
-// buffers over.
+// through raw buffer loads: a wave's order slot, when it has none, and the B
+// pieces of absent column tiles read out of range, which returns zero
+// without touching memory, so every load is unconditional and the compiler's
+// waits stay static.  One barrier per K-step hands the LDS buffers over.
+//
+// A wave computes its two order slots together (one K sweep) when their
+// accumulators fit its registers, else one after the other (two K sweeps,
+// the second only when the job has more than WAVES orders): the two-term
+// form over 8 column tiles.  Either way each table fragment is read once;
+// the sequential form stages the x fragments once per sweep.
 template <int WAVES, int NTL, bool SPLIT>
 __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_tab, const _Float16 *__restrict__ lo_tab,
                                                 const _Float16 *__restrict__ xhi, const _Float16 *__restrict__ xlo,
@@ -95,7 +105,8 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
                                                 int ntile, const int *__restrict__ cols, const int *__restrict__ pexp,
                                                 const int *__restrict__ xexp, float *__restrict__ Qt, int tstride,
                                                 int tile0) {
-    constexpr int NIP = 2;
+    constexpr int NIP = (SPLIT && NTL > 4) ? 1 : 2;  // order slots per sweep
+    constexpr int NSWEEP = 2 / NIP;
     constexpr int NPL = SPLIT ? 4 : 2;  // planes: hi Re, hi Im, lo Re, lo Im
     constexpr int THREADS = WAVES * 64;
     constexpr int CHUNKS = NTL * NPL * 64;  // 16-byte B pieces of one K-step
@@ -135,145 +146,157 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
             if (bslot[u] >= 0)
                 (&bsh[buf][0][0][0])[bslot[u]] = breg[u];
     };
-    // this wave's orders: ip0 + wave and ip0 + wave + WAVES (absent: null offset)
     const size_t slice = static_cast<size_t>(N32) * R * 1024 * sizeof(_Float16);  // bytes of one term's slice
     const __amdgpu_buffer_rsrc_t rh = buffer_rsrc(hi_tab, slice);
     const __amdgpu_buffer_rsrc_t rl = buffer_rsrc(SPLIT ? lo_tab : hi_tab, slice);
-    unsigned aoff[NIP];
-#pragma unroll
-    for (int j = 0; j < NIP; ++j) {
-        const int ip = ip0 + wave + j * WAVES;
-        aoff[j] = wave + j * WAVES < nord ? static_cast<unsigned>((ip * 1024 + lane * 8) * sizeof(_Float16))
-                                          : kBufferNull;
-    }
     const unsigned kstride = static_cast<unsigned>(R) * 1024 * sizeof(_Float16);  // bytes per K-step
-    auto load_a = [&](int ks, h8_t (&a)[NIP][NPL]) {
-        const unsigned so = static_cast<unsigned>(ks) * kstride;
+    auto kc = [&](int k) { return k < N32 ? k : N32 - 1; };
+
+#pragma unroll 1
+    for (int sweep = 0; sweep < NSWEEP; ++sweep) {
+        if (sweep > 0 && nord <= WAVES * sweep)
+            break;  // block-uniform: no order for this sweep
+        // this sweep's order slots of the wave: ip0 + wave + WAVES (sweep + j)
+        // (absent: null offset)
+        int slot[NIP];
+        unsigned aoff[NIP];
 #pragma unroll
         for (int j = 0; j < NIP; ++j) {
-            a[j][0] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rh, aoff[j], so, 0));
-            a[j][1] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rh, aoff[j] + 1024, so, 0));
-            if constexpr (SPLIT) {
-                a[j][2] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rl, aoff[j], so, 0));
-                a[j][3] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rl, aoff[j] + 1024, so, 0));
-            }
+            slot[j] = wave + (sweep + j) * WAVES;
+            aoff[j] = slot[j] < nord ? static_cast<unsigned>(((ip0 + slot[j]) * 1024 + lane * 8) * sizeof(_Float16))
+                                     : kBufferNull;
         }
-    };
-    f4_t acc[NIP][NTL][2];
-#pragma unroll
-    for (int j = 0; j < NIP; ++j)
-#pragma unroll
-        for (int it = 0; it < NTL; ++it)
-            acc[j][it][0] = acc[j][it][1] = f4_t{0.f, 0.f, 0.f, 0.f};
-    // MFMAs of one K-step from the A fragments a and buffer buf of the B
-    // fragments (Im planes negated per column tile, for the real parts)
-    auto step = [&](const h8_t (&a)[NIP][NPL], int buf) {
-#pragma unroll
-        for (int it = 0; it < NTL; ++it) {
-            if (it >= ntile)
-                continue;
-            h8_t b[NPL], nb[NPL / 2];
-#pragma unroll
-            for (int pl = 0; pl < NPL; ++pl)
-                b[pl] = bsh[buf][it][pl][lane];
-            nb[0] = neg_h8(b[1]);
-            if constexpr (SPLIT)
-                nb[1] = neg_h8(b[3]);
+        auto load_a = [&](int ks, h8_t (&a)[NIP][NPL]) {
+            const unsigned so = static_cast<unsigned>(ks) * kstride;
 #pragma unroll
             for (int j = 0; j < NIP; ++j) {
-                if (wave + j * WAVES >= nord)
-                    continue;
-                f4_t cr = acc[j][it][0], ci = acc[j][it][1];
-                cr = mfma_h16(a[j][0], b[0], cr);
-                ci = mfma_h16(a[j][0], b[1], ci);
-                cr = mfma_h16(a[j][1], nb[0], cr);
-                ci = mfma_h16(a[j][1], b[0], ci);
+                a[j][0] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rh, aoff[j], so, 0));
+                a[j][1] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rh, aoff[j] + 1024, so, 0));
                 if constexpr (SPLIT) {
-                    cr = mfma_h16(a[j][0], b[2], cr);
-                    ci = mfma_h16(a[j][0], b[3], ci);
-                    cr = mfma_h16(a[j][1], nb[1], cr);
-                    ci = mfma_h16(a[j][1], b[2], ci);
-                    cr = mfma_h16(a[j][2], b[0], cr);
-                    ci = mfma_h16(a[j][2], b[1], ci);
-                    cr = mfma_h16(a[j][3], nb[0], cr);
-                    ci = mfma_h16(a[j][3], b[0], ci);
+                    a[j][2] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rl, aoff[j], so, 0));
+                    a[j][3] =
+                        __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rl, aoff[j] + 1024, so, 0));
                 }
-                acc[j][it][0] = cr;
-                acc[j][it][1] = ci;
             }
+        };
+        f4_t acc[NIP][NTL][2];
+#pragma unroll
+        for (int j = 0; j < NIP; ++j)
+#pragma unroll
+            for (int it = 0; it < NTL; ++it)
+                acc[j][it][0] = acc[j][it][1] = f4_t{0.f, 0.f, 0.f, 0.f};
+        // MFMAs of one K-step from the A fragments a and buffer buf of the B
+        // fragments (Im planes negated per column tile, for the real parts)
+        auto step = [&](const h8_t (&a)[NIP][NPL], int buf) {
+#pragma unroll
+            for (int it = 0; it < NTL; ++it) {
+                if (it >= ntile)
+                    continue;
+                h8_t b[NPL], nb[NPL / 2];
+#pragma unroll
+                for (int pl = 0; pl < NPL; ++pl)
+                    b[pl] = bsh[buf][it][pl][lane];
+                nb[0] = neg_h8(b[1]);
+                if constexpr (SPLIT)
+                    nb[1] = neg_h8(b[3]);
+#pragma unroll
+                for (int j = 0; j < NIP; ++j) {
+                    if (slot[j] >= nord)
+                        continue;
+                    f4_t cr = acc[j][it][0], ci = acc[j][it][1];
+                    cr = mfma_h16(a[j][0], b[0], cr);
+                    ci = mfma_h16(a[j][0], b[1], ci);
+                    cr = mfma_h16(a[j][1], nb[0], cr);
+                    ci = mfma_h16(a[j][1], b[0], ci);
+                    if constexpr (SPLIT) {
+                        cr = mfma_h16(a[j][0], b[2], cr);
+                        ci = mfma_h16(a[j][0], b[3], ci);
+                        cr = mfma_h16(a[j][1], nb[1], cr);
+                        ci = mfma_h16(a[j][1], b[2], ci);
+                        cr = mfma_h16(a[j][2], b[0], cr);
+                        ci = mfma_h16(a[j][2], b[1], ci);
+                        cr = mfma_h16(a[j][3], nb[0], cr);
+                        ci = mfma_h16(a[j][3], b[0], ci);
+                    }
+                    acc[j][it][0] = cr;
+                    acc[j][it][1] = ci;
+                }
+            }
+        };
+        // K-steps in threes over three A register sets (loaded two steps
+        // ahead); K-steps past the last reload the last one (the loads stay
+        // unconditional)
+        h8_t a0[NIP][NPL], a1[NIP][NPL], a2[NIP][NPL];
+        if (sweep > 0)
+            __syncthreads();  // the previous sweep's last LDS reads are done
+        load_b(0);
+        load_a(0, a0);
+        load_a(kc(1), a1);
+        store_b(0);
+        __syncthreads();
+        for (int ks = 0; ks < N32; ks += 3) {
+            load_b(kc(ks + 1));
+            load_a(kc(ks + 2), a2);
+            step(a0, 0);
+            store_b(1);
+            __syncthreads();
+            if (ks + 1 >= N32)
+                break;
+            load_b(kc(ks + 2));
+            load_a(kc(ks + 3), a0);
+            step(a1, 1);
+            store_b(0);
+            __syncthreads();
+            if (ks + 2 >= N32)
+                break;
+            load_b(kc(ks + 3));
+            load_a(kc(ks + 4), a1);
+            step(a2, 0);
+            store_b(1);
+            __syncthreads();
+            // buffers: the next round starts on LDS buffer 1; swap the roles
+            if (ks + 3 >= N32)
+                break;
+            load_b(kc(ks + 4));
+            load_a(kc(ks + 5), a2);
+            step(a0, 1);
+            store_b(0);
+            __syncthreads();
+            if (ks + 4 >= N32)
+                break;
+            load_b(kc(ks + 5));
+            load_a(kc(ks + 6), a0);
+            step(a1, 0);
+            store_b(1);
+            __syncthreads();
+            if (ks + 5 >= N32)
+                break;
+            load_b(kc(ks + 6));
+            load_a(kc(ks + 7), a1);
+            step(a2, 1);
+            store_b(0);
+            __syncthreads();
+            ks += 3;  // six K-steps per round: LDS buffers and A sets both back to their start
         }
-    };
-    // K-steps in threes over three A register sets (loaded two steps ahead);
-    // K-steps past the last reload the last one (the loads stay unconditional)
-    h8_t a0[NIP][NPL], a1[NIP][NPL], a2[NIP][NPL];
-    auto kc = [&](int k) { return k < N32 ? k : N32 - 1; };
-    load_b(0);
-    load_a(0, a0);
-    load_a(kc(1), a1);
-    store_b(0);
-    __syncthreads();
-    for (int ks = 0; ks < N32; ks += 3) {
-        load_b(kc(ks + 1));
-        load_a(kc(ks + 2), a2);
-        step(a0, 0);
-        store_b(1);
-        __syncthreads();
-        if (ks + 1 >= N32)
-            break;
-        load_b(kc(ks + 2));
-        load_a(kc(ks + 3), a0);
-        step(a1, 1);
-        store_b(0);
-        __syncthreads();
-        if (ks + 2 >= N32)
-            break;
-        load_b(kc(ks + 3));
-        load_a(kc(ks + 4), a1);
-        step(a2, 0);
-        store_b(1);
-        __syncthreads();
-        // buffers: the next round starts on LDS buffer 1; swap the roles
-        if (ks + 3 >= N32)
-            break;
-        load_b(kc(ks + 4));
-        load_a(kc(ks + 5), a2);
-        step(a0, 1);
-        store_b(0);
-        __syncthreads();
-        if (ks + 4 >= N32)
-            break;
-        load_b(kc(ks + 5));
-        load_a(kc(ks + 6), a0);
-        step(a1, 0);
-        store_b(1);
-        __syncthreads();
-        if (ks + 5 >= N32)
-            break;
-        load_b(kc(ks + 6));
-        load_a(kc(ks + 7), a1);
-        step(a2, 1);
-        store_b(0);
-        __syncthreads();
-        ks += 3;  // six K-steps per round: LDS buffers and A sets both back to their start
-    }
 #pragma unroll
-    for (int j = 0; j < NIP; ++j) {
-        if (wave + j * WAVES >= nord)
-            continue;
-        const int ip = ip0 + wave + j * WAVES;
+        for (int j = 0; j < NIP; ++j) {
+            if (slot[j] >= nord)
+                continue;
+            const int ip = ip0 + slot[j];
 #pragma unroll
-        for (int it = 0; it < NTL; ++it)
-            if (it < ntile) {
-                const int n2 = cols[it * 16 + lo];
-                const int e = pexp[ip] + xexp[n2 < 0 ? nmax : n2];
-                const size_t slot = static_cast<size_t>(ip) * tstride + tile0 + it;
-                float *dst = Qt + 2 * ((slot * 16 + lo) * 16 + 4 * hi);
+            for (int it = 0; it < NTL; ++it)
+                if (it < ntile) {
+                    const int n2 = cols[it * 16 + lo];
+                    const int e = pexp[ip] + xexp[n2 < 0 ? nmax : n2];
+                    const size_t qs = static_cast<size_t>(ip) * tstride + tile0 + it;
+                    float *dst = Qt + 2 * ((qs * 16 + lo) * 16 + 4 * hi);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    dst[2 * i] = ldexpf(acc[j][it][0][i], -e);
-                    dst[2 * i + 1] = ldexpf(acc[j][it][1][i], -e);
+                    for (int i = 0; i < 4; ++i) {
+                        dst[2 * i] = ldexpf(acc[j][it][0][i], -e);
+                        dst[2 * i + 1] = ldexpf(acc[j][it][1][i], -e);
+                    }
                 }
-            }
+        }
     }
 }
 

@@ -187,8 +187,6 @@ __global__ __launch_bounds__(256) void serve_half_convert_kernel(const BatchReq 
 }
 
 constexpr int kServeHalfWaves = 8;  // 2 orders per wave (the plan's fp16 jobs)
-// column tiles per fp16 job: 4 with two terms, 8 with one (the plan's job lists 0, 1)
-constexpr int serve_half_tiles(bool split) { return split ? 4 : 8; }
 
 template <bool SPLIT>
 __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_gemm_kernel(const BatchReq *__restrict__ reqs,
@@ -203,7 +201,7 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_gemm_kernel(c
     const int b = blockIdx.x - hs[h];
     const int jid = b / q.ntiles1;
     const int t = b - jid * q.ntiles1;
-    const PairJob *hjobs = q.hjobs[SPLIT ? 0 : 1];
+    const PairJob *hjobs = q.hjobs[kHalfJobList];
     const int dm = uniform(hjobs[jid].dm);
     const int tile0 = uniform(hjobs[jid].tile0);
     const int ntile = uniform(hjobs[jid].ntile);
@@ -218,7 +216,7 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_gemm_kernel(c
     const size_t term = static_cast<size_t>(q.ntiles1) * half_tile_halves(q.lmax, q.N32);
     const size_t xplane = half_x_plane(q.nmax, q.N32);
     float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
-    half_gemm_block<kServeHalfWaves, serve_half_tiles(SPLIT), SPLIT>(
+    half_gemm_block<kServeHalfWaves, kHalfJobTiles, SPLIT>(
         q.htab + f0, SPLIT ? q.htab + term + f0 : nullptr, q.xh, q.xh + 2 * xplane, xplane, q.N32, q.nmax,
         P - adm, ip0, nord, ntile, q.colmap + d * q.maxcols + tile0 * 16, q.pexp + adm, q.xexp, Qt, tstride, tile0);
 }
@@ -1417,7 +1415,7 @@ struct yasfhip_server {
                 hi[h] = half_reqs[h];
                 bc += (q.N + 255) / 256;
                 bv += static_cast<int>((static_cast<long>(q.nmax + 1) * N32 * 32 + 255) / 256);
-                bg += plan_for(q.L).dp.nhjobs[inner_precision == YASFHIP_FP16X2 ? 0 : 1] * q.ntiles1;
+                bg += plan_for(q.L).dp.nhjobs[kHalfJobList] * q.ntiles1;
             }
             hc[nh] = bc;
             hv[nh] = bv;

@@ -254,8 +254,12 @@ def main(argv=None):
     finally:
         srv.close()
     out = rep.as_dict()
-    out.update(precision=args.precision, rate=args.rate, graph=not args.eager,
-               converged=sum(1 for r in rep.results.values() if r.get("info") == 0))
+    res = list(rep.results.values())
+    out.update(precision=args.precision, rate=args.rate, graph=not args.eager, inner_rtol=args.inner_rtol,
+               converged=sum(1 for r in res if r.get("info") == 0),
+               mean_cycles=float(np.mean([r["cycles"] for r in res])),
+               mean_iterations=float(np.mean([r["iterations"] for r in res])),
+               max_residual=float(max(r["residual"] for r in res)))
     print(json.dumps(out), flush=True)
 
 
