@@ -406,6 +406,14 @@ int yasfhip_server_set_table_policy(yasfhip_server *s, int policy, int slab_tile
  * active). */
 int yasfhip_server_set_inner_precision(yasfhip_server *s, int precision);
 
+/* Reduction each restart cycle is asked for at least (the inner_rtol of
+ * yasfhip_server_create): cycles started from now on aim at
+ * max(inner_rtol, rtol ||b|| / (2 ||r||)).  With an operator of complex64
+ * accuracy (YASFHIP_FP32, YASFHIP_FP16X2) 1e-6 lets most requests converge
+ * after one cycle; the one-term YASFHIP_FP16 operator (relative error
+ * ~1e-4) gains nothing below 1e-4.  No counterpart in the reference. */
+int yasfhip_server_set_inner_rtol(yasfhip_server *s, double inner_rtol);
+
 /* C_sca method of the completions from now on (YASFHIP_CSCA_FAR_FIELD, the
  * default, or YASFHIP_CSCA_PAIR_SUM; see yasfhip_set_scattering_method). */
 int yasfhip_server_set_scattering_method(yasfhip_server *s, int method);

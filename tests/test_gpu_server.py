@@ -316,3 +316,22 @@ def test_half_inner_precision(mixed_requests, served, aggregate_requests, precis
     with pytest.raises(ValueError):  # not on a busy server
         srv.set_inner_precision(precision)
     srv.close()
+
+
+def test_inner_rtol_setter(mixed_requests, served):
+    # a smaller reduction per restart cycle with the fp16x2 inner operator:
+    # fewer cycles (FP64 residuals) in total, and every solution still meets
+    # the FP64 criterion
+    srv = _native.Server(memory_bytes=4 << 30, restart=50, inner_rtol=1e-4)
+    srv.set_inner_precision(_native.FP16X2)
+    srv.set_inner_rtol(1e-6)
+    with pytest.raises(ValueError):
+        srv.set_inner_rtol(0.0)
+    with pytest.raises(ValueError):
+        srv.set_inner_rtol(1.0)
+    rep = serve(srv, mixed_requests, keep_solutions=True)
+    srv.close()
+    for r in mixed_requests:
+        _fp64_check(r, rep.results[r.index])
+    cycles = sum(rep.results[r.index]["cycles"] for r in mixed_requests)
+    assert cycles <= sum(served.results[r.index]["cycles"] for r in mixed_requests)

@@ -108,6 +108,7 @@ def _load():
             "yasfhip_server_poll": ([P, llp, P, ctypes.c_longlong, ip, ip, ip, dp, dp, dp, dp], i),
             "yasfhip_server_autotune": ([P, i, i, i, ip, ip, P], i),
             "yasfhip_server_set_tile_shape": ([P, i, i, i], i),
+            "yasfhip_server_set_inner_rtol": ([P, d], i),
             "yasfhip_server_set_table_policy": ([P, i, i], i),
             "yasfhip_set_slab_tiles": ([P, i], i),
             "yasfhip_set_near_field_radius": ([P, d], i),
@@ -146,6 +147,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_time_matvec", "yasfhip_set_matvec_mode", "yasfhip_tune_matvec", "yasfhip_server_create",
         "yasfhip_server_destroy", "yasfhip_server_submit", "yasfhip_server_step", "yasfhip_server_poll",
         "yasfhip_server_autotune", "yasfhip_server_set_tile_shape", "yasfhip_server_set_table_policy",
+        "yasfhip_server_set_inner_rtol",
         "yasfhip_set_slab_tiles", "yasfhip_set_near_field_radius", "yasfhip_mie_coefficients",
         "yasfhip_planewave_coefficients", "yasfhip_server_submit_spheres", "yasfhip_set_table_shape",
         "yasfhip_set_precond_factor_precision", "yasfhip_coupling_matvec_noalloc",
@@ -540,6 +542,10 @@ class Server:
         _check(self._lib.yasfhip_server_set_tile_shape(self._h, int(lmax), int(waves), int(column_tiles)))
 
     TABLES_ON_THE_FLY, TABLES_AUTO, TABLES_SLABS = 0, 1, 2
+
+    def set_inner_rtol(self, inner_rtol: float):
+        """Reduction asked of each restart cycle started from now on (at least)."""
+        _check(self._lib.yasfhip_server_set_inner_rtol(self._h, float(inner_rtol)))
 
     def set_inner_precision(self, precision: int):
         """Pair GEMM of the inner iterations of operators admitted from now on:

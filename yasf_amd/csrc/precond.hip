@@ -134,9 +134,11 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // as LAPACK's izamax), one block of kLuWaves waves per box, in place on the
 // padded mp x mp block, with two levels of blocking:
 // * outer panels of kLuOuter columns; inside one, 16-column blocks: unblocked
-//   factorisation (pivot search, full-row swap, rank-1 updates within the 16
-//   columns), the 16x16 diagonal inverses, then the 16-row U = L^-1 A and
-//   the rank-16 update restricted to the outer panel's columns;
+//   factorisation of the 16-column panel (pivot search, rank-1 updates within
+//   the 16 columns; in registers, one row per thread, when the panel has at
+//   most kLuPanelRows rows), the row swaps outside the panel in one deferred
+//   pass, the 16x16 diagonal inverses, then the 16-row U = L^-1 A and the
+//   rank-16 update restricted to the outer panel's columns;
 // * then, for the columns right of the outer panel, U12 = L11^-1 A12 by block
 //   forward substitution (16-row blocks, stored inverses) and one rank-64
 //   update A22 -= L21 U12, with U12 staged in LDS kLuChunk columns at a time.
@@ -146,6 +148,188 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // lane & 15) is also its B operand layout, so solved tiles feed the next
 // products from registers.
 constexpr int kLuWaves = 8;     // waves per box
+// Rows of a 16-column panel the register form holds: one per thread.
+constexpr int kLuPanelRows = 64 * kLuWaves;
+
+// Argmax of |A(i, c)|_1 over the block, ties to the lower row: every thread
+// returns the same row.  (bv, bi): the thread's candidate.
+__device__ __forceinline__ int block_argmax(double bv, int bi, double *red_v, int *red_i) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_down(bv, off, 64);
+        const int oi = __shfl_down(bi, off, 64);
+        if (ov > bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    if (lane == 0) {
+        red_v[wave] = bv;
+        red_i[wave] = bi;
+    }
+    __syncthreads();
+    double v = red_v[0];
+    int ii = red_i[0];
+#pragma unroll
+    for (int q = 1; q < kLuWaves; ++q)
+        if (red_v[q] > v || (red_v[q] == v && red_i[q] < ii)) {
+            v = red_v[q];
+            ii = red_i[q];
+        }
+    return ii;
+}
+
+// Partial-pivoting LU of the 16-column panel [kb, kb + 16) of an mp x mp
+// row-major block, rows kb .. mp - 1 (at most kLuPanelRows of them) held in
+// registers: thread t owns row kb + t.  Per column one barrier publishes the
+// wave maxima (every thread then picks the same pivot row p) and one
+// publishes rows p and c, which swap; the elimination runs on the registers.
+// The row swaps outside the panel follow afterwards in one pass (each
+// thread applies the 16 swaps in order to its own columns), and piv[c]
+// records them (row c swapped with piv[c]).  (Two rows per thread, or a
+// second row in LDS, spilled to scratch and ran slower than the global form.)
+__device__ void panel_registers(double2 *__restrict__ A, int mp, int kb, int *__restrict__ piv, double *red_v,
+                                int *red_i, double2 *prow_sh, double2 *crow_sh) {
+    const int tid = threadIdx.x;
+    constexpr int NT = 64 * kLuWaves;
+    auto at = [&](int i, int j) -> double2 & { return A[static_cast<long>(i) * mp + j]; };
+    const int r = kb + tid;
+    double2 pr[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        pr[j] = r < mp ? at(r, kb + j) : make_double2(0.0, 0.0);
+    int pv[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int c = kb + j;
+        double bv = -1.0;
+        int bi = c;
+        if (r >= c && r < mp) {
+            bv = cabs1(pr[j]);
+            bi = r;
+        }
+        const int p = __builtin_amdgcn_readfirstlane(block_argmax(bv, bi, red_v, red_i));  // same in all threads
+        pv[j] = p;
+        // publish rows p (the pivot row) and c; p >= c, both in the panel
+        if (r == p) {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj)
+                prow_sh[jj] = pr[jj];
+        }
+        if (r == c) {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj)
+                crow_sh[jj] = pr[jj];
+        }
+        __syncthreads();
+        const double2 inv = cinv(prow_sh[j]);
+        if (r == c) {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj)
+                pr[jj] = prow_sh[jj];
+        } else if (r == p) {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj)
+                pr[jj] = crow_sh[jj];
+        }
+        const bool below = r > c && r < mp;
+        const double2 l = below ? cmul(pr[j], inv) : make_double2(0.0, 0.0);
+        if (below)
+            pr[j] = l;
+        // pr[jj] -= l prow[jj] (l = 0 for rows at or above c), the pivot row
+        // read four columns at a time: the scheduling barriers keep the
+        // compiler from holding all of it in registers beside the panel
+#pragma unroll
+        for (int j4 = (j + 1) & ~3; j4 < 16; j4 += 4) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int jj = j4; jj < j4 + 4; ++jj)
+                if (jj > j)
+                    pr[jj] = csub(pr[jj], cmul(l, prow_sh[jj]));
+        }
+        // the next column's maxima go to red_v / red_i, whose reads all ended
+        // before the barrier above; prow_sh / crow_sh are rewritten only after
+        // the next column's first barrier
+    }
+    if (r < mp)
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            at(r, kb + j) = pr[j];
+    if (tid < 16)
+        piv[kb + tid] = pv[tid];
+    // deferred row swaps: columns outside the panel, each thread its own
+    for (int jj = tid; jj < mp; jj += NT) {
+        if (jj >= kb && jj < kb + 16)
+            continue;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int c = kb + j, p = pv[j];
+            if (p != c) {
+                const double2 t0 = at(c, jj);
+                at(c, jj) = at(p, jj);
+                at(p, jj) = t0;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// The same panel for blocks of more rows, in global memory: per column a
+// block-wide pivot search, the row swap inside the panel and the elimination
+// within the panel, each followed by a barrier; the swaps outside the panel
+// are deferred as above.
+__device__ void panel_global(double2 *__restrict__ A, int mp, int kb, int *__restrict__ piv, double *red_v,
+                             int *red_i) {
+    const int tid = threadIdx.x;
+    auto at = [&](int i, int j) -> double2 & { return A[static_cast<long>(i) * mp + j]; };
+    int pv[16];
+    for (int j = 0; j < 16; ++j) {
+        const int c = kb + j;
+        double bv = -1.0;
+        int bi = c;
+        for (int i = c + tid; i < mp; i += blockDim.x) {
+            const double v = cabs1(at(i, c));
+            if (v > bv) {
+                bv = v;
+                bi = i;
+            }
+        }
+        const int p = __builtin_amdgcn_readfirstlane(block_argmax(bv, bi, red_v, red_i));
+        pv[j] = p;
+        if (p != c && tid < 16) {  // the swap inside the panel; the rest is deferred
+            const double2 t0 = at(c, kb + tid);
+            at(c, kb + tid) = at(p, kb + tid);
+            at(p, kb + tid) = t0;
+        }
+        __syncthreads();
+        const double2 inv = cinv(at(c, c));
+        const int bend = kb + 16;
+        for (int i = c + 1 + tid; i < mp; i += blockDim.x) {
+            const double2 l = cmul(at(i, c), inv);
+            at(i, c) = l;
+            for (int jj = c + 1; jj < bend; ++jj)
+                at(i, jj) = csub(at(i, jj), cmul(l, at(c, jj)));
+        }
+        __syncthreads();
+    }
+    if (tid < 16)
+        piv[kb + tid] = pv[tid];
+    for (int jj = tid; jj < mp; jj += blockDim.x) {
+        if (jj >= kb && jj < kb + 16)
+            continue;
+        for (int j = 0; j < 16; ++j) {
+            const int c = kb + j, p = pv[j];
+            if (p != c) {
+                const double2 t0 = at(c, jj);
+                at(c, jj) = at(p, jj);
+                at(p, jj) = t0;
+            }
+        }
+    }
+    __syncthreads();
+}
+
 constexpr int kLuOuter = 64;    // columns per outer panel
 constexpr int kLuChunk = 128;   // trailing columns per LDS stage of U12
 constexpr size_t kLuLds = static_cast<size_t>(kLuOuter) * kLuChunk * sizeof(double2);  // 128 KB
@@ -158,7 +342,7 @@ __global__ __launch_bounds__(64 * kLuWaves) void lu_blocked_kernel(const int *__
                                                          double2 *__restrict__ dinv) {
     __shared__ double red_v[kLuWaves];
     __shared__ int red_i[kLuWaves];
-    __shared__ int piv_sh;
+    __shared__ double2 prow_sh[16], crow_sh[16];
     __shared__ double2 Lsh[16][16];
     extern __shared__ double2 Ush[];  // [kLuOuter][kLuChunk]
     const int b = blockIdx.x;
@@ -174,72 +358,24 @@ __global__ __launch_bounds__(64 * kLuWaves) void lu_blocked_kernel(const int *__
     for (int K0 = 0; K0 < mp; K0 += kLuOuter) {
         const int pend = min(K0 + kLuOuter, mp);  // outer panel columns [K0, pend)
         for (int kb = K0; kb < pend; kb += 16) {
-            for (int j = 0; j < 16; ++j) {
-                const int c = kb + j;
-                double bv = -1.0;
-                int bi = c;
-                for (int i = c + tid; i < mp; i += blockDim.x) {
-                    const double v = cabs1(at(i, c));
-                    if (v > bv) {
-                        bv = v;
-                        bi = i;
-                    }
-                }
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    const double ov = __shfl_down(bv, off, 64);
-                    const int oi = __shfl_down(bi, off, 64);
-                    if (ov > bv || (ov == bv && oi < bi)) {
-                        bv = ov;
-                        bi = oi;
-                    }
-                }
-                if (lane == 0) {
-                    red_v[wave] = bv;
-                    red_i[wave] = bi;
-                }
-                __syncthreads();
-                if (tid == 0) {
-                    double v = red_v[0];
-                    int ii = red_i[0];
-                    for (int q = 1; q < kLuWaves; ++q)
-                        if (red_v[q] > v || (red_v[q] == v && red_i[q] < ii)) {
-                            v = red_v[q];
-                            ii = red_i[q];
-                        }
-                    piv_sh = ii;
-                    piv[c] = ii;
-                }
-                __syncthreads();
-                const int p = piv_sh;
-                if (p != c)
-                    for (int jj = tid; jj < mp; jj += blockDim.x) {
-                        const double2 t0 = at(c, jj);
-                        at(c, jj) = at(p, jj);
-                        at(p, jj) = t0;
-                    }
-                __syncthreads();
-                const double2 inv = cinv(at(c, c));
-                const int bend = kb + 16;
-                for (int i = c + 1 + tid; i < mp; i += blockDim.x) {
-                    const double2 l = cmul(at(i, c), inv);
-                    at(i, c) = l;
-                    for (int jj = c + 1; jj < bend; ++jj)
-                        at(i, jj) = csub(at(i, jj), cmul(l, at(c, jj)));
-                }
-                __syncthreads();
-            }
+            if (mp - kb <= kLuPanelRows)
+                panel_registers(A, mp, kb, piv, red_v, red_i, prow_sh, crow_sh);
+            else
+                panel_global(A, mp, kb, piv, red_v, red_i);
             // Inverses of the diagonal 16x16 factors, kept for the
             // substitutions: thread c < 16 solves L11 y = e_c (unit lower),
             // thread 16 + c solves U11 z = e_c (upper).
             double2 *Dinv = Dbox + static_cast<long>(kb / 16) * 512;
+            // (compile-time indices throughout, so yv / zv stay in registers;
+            // the terms outside a column's range are multiplied by zeros)
             if (tid < 16) {
                 const int c = tid;
                 double2 yv[16];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
-                    for (int q = c; q < r; ++q)
+#pragma unroll
+                    for (int q = 0; q < r; ++q)
                         v = csub(v, cmul(at(kb + r, kb + q), yv[q]));
                     yv[r] = r < c ? make_double2(0.0, 0.0) : v;
                     Lsh[r][c] = yv[r];
@@ -251,7 +387,8 @@ __global__ __launch_bounds__(64 * kLuWaves) void lu_blocked_kernel(const int *__
 #pragma unroll
                 for (int r = 15; r >= 0; --r) {
                     double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
-                    for (int q = r + 1; q <= c; ++q)
+#pragma unroll
+                    for (int q = r + 1; q < 16; ++q)
                         v = csub(v, cmul(at(kb + r, kb + q), zv[q]));
                     zv[r] = r > c ? make_double2(0.0, 0.0) : cmul(v, cinv(at(kb + r, kb + r)));
                     Dinv[256 + r * 16 + c] = zv[r];
@@ -397,14 +534,13 @@ __global__ __launch_bounds__(64 * kLuWaves) void lu_blocked_kernel(const int *__
                 // (Ar + Ai)(Br + Bi) from zero, so the MFMAs need not wait for
                 // the C tile, which is loaded first and combined at the end:
                 // C -= (P1 - P2) + i (P3 - P1 - P2)
-                double ar[kLuOuter / 4], ai[kLuOuter / 4], as[kLuOuter / 4];
+                double ar[kLuOuter / 4], ai[kLuOuter / 4];
 #pragma unroll
                 for (int kk = 0; kk < kLuOuter / 4; ++kk)
                     if (kk < nks) {
                         const double2 v = at(row0 + lo, K0 + 4 * kk + hi);
                         ar[kk] = v.x;
                         ai[kk] = v.y;
-                        as[kk] = v.x + v.y;
                     }
                 for (int ct = 0; ct < nct; ++ct) {
                     const int col0 = c0 + ct * 16;
@@ -413,13 +549,15 @@ __global__ __launch_bounds__(64 * kLuWaves) void lu_blocked_kernel(const int *__
                     for (int i = 0; i < 4; ++i)
                         cv[i] = at(row0 + 4 * i + hi, col0 + lo);
                     v4d p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
+                    // (the Gauss sums Ar + Ai are formed per MFMA: holding them
+                    // for all K-substeps pushed this loop past the registers)
+#pragma unroll 4
                     for (int kk = 0; kk < kLuOuter / 4; ++kk)
                         if (kk < nks) {
                             const double2 bv2 = Ush[(4 * kk + hi) * kLuChunk + ct * 16 + lo];
                             p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[kk], bv2.x, p1, 0, 0, 0);
                             p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[kk], bv2.y, p2, 0, 0, 0);
-                            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(as[kk], bv2.x + bv2.y, p3, 0, 0, 0);
+                            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[kk] + ai[kk], bv2.x + bv2.y, p3, 0, 0, 0);
                         }
 #pragma unroll
                     for (int i = 0; i < 4; ++i)

@@ -1963,6 +1963,14 @@ int yasfhip_server_set_inner_precision(yasfhip_server *s, int precision) {
     });
 }
 
+int yasfhip_server_set_inner_rtol(yasfhip_server *s, double inner_rtol) {
+    return sguard([&] {
+        YH_REQUIRE(s, "server is NULL");
+        YH_REQUIRE(inner_rtol > 0.0 && inner_rtol < 1.0, "inner_rtol must be in (0, 1)");
+        s->inner_rtol = inner_rtol;
+    });
+}
+
 int yasfhip_server_set_scattering_method(yasfhip_server *s, int method) {
     return sguard([&] {
         YH_REQUIRE(s, "server is NULL");
