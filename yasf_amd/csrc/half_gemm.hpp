@@ -85,9 +85,10 @@ __device__ __forceinline__ h8_t neg_h8(h8_t a) {
 // ip * tstride + tile0 + it past Qt (complex64).
 //
 // Pipelining: the table stream (the operand that comes from HBM) is loaded
-// two K-steps ahead into three register sets used in turn; the B fragments of
-// a K-step (the job's column tiles, shared by all waves) are loaded by the
-// whole block one K-step ahead into one of two LDS buffers.  Operands come
+// three K-steps ahead into four register sets used in turn; the B fragments
+// of a K-step (the job's column tiles, shared by all waves) are loaded by the
+// whole block two K-steps ahead into one of two register sets and stored one
+// K-step ahead into one of two LDS buffers.  Operands come
 // through raw buffer loads: a wave's order slot, when it has none, and the B
 // pieces of absent column tiles read out of range, which returns zero
 // without touching memory, so every load is unconditional and the compiler's
@@ -133,14 +134,13 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
         bslot[u] = q < CHUNKS ? q : -1;
     }
     (void)xlo;  // the lo terms follow the hi terms in one allocation
-    h8_t breg[PER];
-    auto load_b = [&](int ks) {
+    auto load_b = [&](int ks, h8_t (&breg)[PER]) {
         const unsigned so = static_cast<unsigned>(ks) * 32 * sizeof(_Float16);
 #pragma unroll
         for (int u = 0; u < PER; ++u)
             breg[u] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rx, boff[u], so, 0));
     };
-    auto store_b = [&](int buf) {
+    auto store_b = [&](int buf, const h8_t (&breg)[PER]) {
 #pragma unroll
         for (int u = 0; u < PER; ++u)
             if (bslot[u] >= 0)
@@ -223,60 +223,45 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
                 }
             }
         };
-        // K-steps in threes over three A register sets (loaded two steps
-        // ahead); K-steps past the last reload the last one (the loads stay
-        // unconditional)
-        h8_t a0[NIP][NPL], a1[NIP][NPL], a2[NIP][NPL];
+        // K-step ks computes with A register set ks % 4 and LDS buffer ks % 2.
+        // In it, the x pieces of ks + 2 are loaded into register set ks % 2
+        // before the table fragments of ks + 3 (set (ks + 3) % 4), and the x
+        // pieces of ks + 1 (loaded a step earlier) go to LDS buffer (ks + 1) % 2
+        // after the MFMAs.  The vector-memory counter completes in order, so
+        // waiting for x of ks + 1 only forces the table loads issued before
+        // it: each table fragment has three K-steps to arrive, each x piece
+        // two.  K-steps past the last reload the last one (the loads stay
+        // unconditional).
+        h8_t a0[NIP][NPL], a1[NIP][NPL], a2[NIP][NPL], a3[NIP][NPL];
+        h8_t bx0[PER], bx1[PER];
         if (sweep > 0)
             __syncthreads();  // the previous sweep's last LDS reads are done
-        load_b(0);
+        load_b(0, bx0);
+        store_b(0, bx0);
+        load_b(kc(1), bx1);
         load_a(0, a0);
         load_a(kc(1), a1);
-        store_b(0);
+        load_a(kc(2), a2);
         __syncthreads();
-        for (int ks = 0; ks < N32; ks += 3) {
-            load_b(kc(ks + 1));
-            load_a(kc(ks + 2), a2);
-            step(a0, 0);
-            store_b(1);
+        auto kstep = [&](int ks, const h8_t (&acur)[NIP][NPL], h8_t (&anext)[NIP][NPL], h8_t (&bload)[PER],
+                         const h8_t (&bstore)[PER], int buf) {
+            load_b(kc(ks + 2), bload);
+            load_a(kc(ks + 3), anext);
+            step(acur, buf);
+            store_b(buf ^ 1, bstore);
             __syncthreads();
+        };
+        for (int ks = 0; ks < N32; ks += 4) {
+            kstep(ks, a0, a3, bx0, bx1, 0);
             if (ks + 1 >= N32)
                 break;
-            load_b(kc(ks + 2));
-            load_a(kc(ks + 3), a0);
-            step(a1, 1);
-            store_b(0);
-            __syncthreads();
+            kstep(ks + 1, a1, a0, bx1, bx0, 1);
             if (ks + 2 >= N32)
                 break;
-            load_b(kc(ks + 3));
-            load_a(kc(ks + 4), a1);
-            step(a2, 0);
-            store_b(1);
-            __syncthreads();
-            // buffers: the next round starts on LDS buffer 1; swap the roles
+            kstep(ks + 2, a2, a1, bx0, bx1, 0);
             if (ks + 3 >= N32)
                 break;
-            load_b(kc(ks + 4));
-            load_a(kc(ks + 5), a2);
-            step(a0, 1);
-            store_b(0);
-            __syncthreads();
-            if (ks + 4 >= N32)
-                break;
-            load_b(kc(ks + 5));
-            load_a(kc(ks + 6), a0);
-            step(a1, 0);
-            store_b(1);
-            __syncthreads();
-            if (ks + 5 >= N32)
-                break;
-            load_b(kc(ks + 6));
-            load_a(kc(ks + 7), a1);
-            step(a2, 1);
-            store_b(0);
-            __syncthreads();
-            ks += 3;  // six K-steps per round: LDS buffers and A sets both back to their start
+            kstep(ks + 3, a3, a2, bx1, bx0, 1);
         }
 #pragma unroll
         for (int j = 0; j < NIP; ++j) {
