@@ -258,3 +258,43 @@ def test_mixed_precision_history_and_errors():
         s.gmres_refined(b, max_refinements=0)
     x, info, steps, iters, res, hist = s.gmres_refined(np.zeros_like(b))
     assert info == 0 and steps == 0 and np.all(x == 0)
+
+
+@pytest.mark.parametrize("restart", [1, 2, 7])
+def test_device_cycles_short_restarts_match_scipy(restart):
+    """Restart cycles of one, two and seven Arnoldi steps (every step ends or
+    nearly ends a cycle, so the device stop decision, the gated steps queued
+    past it and the per-cycle true residual are all exercised): the residual
+    history and the solution follow SciPy's GMRES over the oracle operator."""
+    from scipy.sparse.linalg import LinearOperator, gmres
+
+    pos, radii = cluster(3, 12, 0.1)
+    L, lam = 3, 600.0
+    sim = build_sim(pos, radii, np.full(12, 1.45 + 0.01j), lam, 1.0, L, 1e-7, restart=restart)
+    b = sim.right_hand_side[:, :, 0].ravel()
+    k = 2 * np.pi / lam
+    t = sim.tmatrix_flat()[0]
+    x, info, iters, res, hist = sim.session.gmres(b, b, rtol=1e-7, restart=restart, maxiter=400,
+                                                   history_cap=100000)
+    ref_hist = []
+    A = LinearOperator((b.size, b.size), matvec=lambda v: R.master_matvec(L, pos, k, t, v), dtype=complex)
+    xr, info_r = gmres(A, b, b.copy(), restart=restart, rtol=1e-7, atol=0, maxiter=400,
+                       callback=ref_hist.append, callback_type="pr_norm")
+    assert info == 0 and info_r == 0 and res <= 1e-7
+    assert len(hist) == iters
+    assert abs(iters - len(ref_hist)) <= max(2, iters // 10)
+    m = min(len(hist), len(ref_hist), 12)
+    npt.assert_allclose(np.asarray(hist[:m]), np.asarray(ref_hist[:m]), rtol=1e-6)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-6
+
+
+def test_device_cycles_single_sphere_and_breakdown():
+    """One sphere: W = 0, so A = I and the first Arnoldi step breaks down
+    (h[1,0] = 0): one step, x = b, as SciPy; and the smallest vector the
+    Arnoldi blocks handle (one partial block)."""
+    pos = np.zeros((1, 3))
+    sim = build_sim(pos, np.array([120.0]), np.array([1.5 + 0.02j]), 550.0, 1.0, 2, 1e-8, restart=20)
+    b = sim.right_hand_side[:, :, 0].ravel()
+    x, info, iters, res, hist = sim.session.gmres(b, np.zeros_like(b), rtol=1e-8, restart=20, maxiter=5)
+    assert info == 0 and iters == 1 and res < 1e-12
+    npt.assert_allclose(x, b, rtol=1e-13, atol=1e-15 * np.abs(b).max())
