@@ -724,9 +724,15 @@ def run_aggregates(args):
     reqs = make_aggregate_workload(counts, lams, args.directions, seed=5)
     log(f"workload: aggregates N={counts}, wavelengths {[round(v, 1) for v in lams]} nm, "
         f"{args.directions} directions: {len(reqs)} requests prepared in {time.perf_counter() - t0:.1f} s")
-    srv = _native.Server(restart=50, inner_rtol=mixed_inner_rtol(args))
+    # inner operator: complex64, or the fp16 pair GEMM in two terms (complex64
+    # accuracy) or one; reduction per restart cycle 1e-4, 1e-6 for fp16x2
+    prec = {"fp32": _native.FP32, "fp16x2": _native.FP16X2, "fp16": _native.FP16}[args.inner_precision]
+    inner_rtol = args.inner_rtol if args.inner_rtol is not None else (
+        1e-6 if args.inner_precision == "fp16x2" else 1e-4)
+    srv = _native.Server(restart=50, inner_rtol=inner_rtol)
     (waves, ctiles), ms = srv.autotune(4, particles=3000, reps=3)
     log(f"autotuned lmax-4 tile shape (large requests): {waves} waves, {ctiles} column tiles ({ms})")
+    srv.set_inner_precision(prec)
     if args.warmup:
         serve(srv, make_aggregate_workload([200], [500.0], 2, seed=99))
     torch.cuda.synchronize()
@@ -745,19 +751,23 @@ def run_aggregates(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 pair GEMM inside FP64 refinement (fp32/fp64)",
+        "dtype": {"fp32": "c64 pair GEMM inside FP64 refinement (fp32/fp64)",
+                  "fp16x2": "two-term fp16 pair GEMM (FP32 accumulation) inside FP64 refinement",
+                  "fp16": "one-term fp16 pair GEMM (FP32 accumulation) inside FP64 refinement"}[args.inner_precision],
         "data": "synthetic (seeded fractal aggregates, seed 5)",
         "config": {"workload": "serving at scale: fractal aggregates Df 1.8, kf 1.3, monomer 40 nm, "
                                "n 1.7+0.1i, lmax 4; sampled wavelengths x incidence directions, rtol 1e-6",
                    "particles": counts, "wavelengths_nm": lams, "directions": args.directions,
                    "requests": len(reqs), "operators": len(counts) * len(lams),
-                   "restart": 50, "inner_rtol": mixed_inner_rtol(args), "tile_shape": [waves, ctiles],
+                   "restart": 50, "inner_precision": args.inner_precision, "inner_rtol": inner_rtol,
+                   "tile_shape": [waves, ctiles],
                    "setup": "host" if args.host_setup else "device (Mie + plane wave at admission)"},
         "p50_latency_s": rep.p50_s,
         "p99_latency_s": rep.p99_s,
         "makespan_s": rep.makespan_s,
         "converged": int(sum(1 for i in infos if i == 0)),
         "mean_inner_iterations": float(np.mean([r["iterations"] for r in rep.results.values()])),
+        "mean_cycles": float(np.mean([r["cycles"] for r in rep.results.values()])),
     }
     if args.fp64_check:
         # first request of each operator on the all-FP64 single-solve path
@@ -806,6 +816,8 @@ def main():
     ap.add_argument("--particles", type=int, default=0, help="aggregates: fixed N instead of the seeded draw")
     ap.add_argument("--wavelengths", type=int, default=2, help="aggregates: wavelengths sampled of 64")
     ap.add_argument("--directions", type=int, default=16, help="aggregates: incidence directions")
+    ap.add_argument("--inner-precision", choices=["fp32", "fp16x2", "fp16"], default="fp32",
+                    help="aggregates: pair GEMM of the inner GMRES iterations (complex64, or fp16 terms)")
     ap.add_argument("--fp64-check", type=int, default=1,
                     help="aggregates: operators whose first request is re-solved in FP64 (0: none)")
     args = ap.parse_args()

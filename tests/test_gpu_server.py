@@ -318,6 +318,39 @@ def test_half_inner_precision(mixed_requests, served, aggregate_requests, precis
     srv.close()
 
 
+@pytest.mark.parametrize("precision", [_native.FP16X2, _native.FP16], ids=["fp16x2", "fp16"])
+def test_half_shared_and_slab_operators(aggregate_requests, precision):
+    # operators shared by key run their requests' fp16 GEMMs as one launch
+    # over the resident fp16 table, or over fp16 slabs rebuilt range by range
+    # (a slab of one FP64 tile holds 1 two-term or 3 one-term fp16 target
+    # tiles of the 90-sphere operators, so their GEMMs span several slabs);
+    # each block computes what the per-request GEMM computes, so every form
+    # gives the same solutions bit for bit
+    import copy
+    reqs = aggregate_requests
+    plain = [copy.copy(r) for r in reqs]
+    for r in plain:
+        r.operator_key = 0
+    runs = {}
+    for name, batch, slab_tiles in (("shared", reqs, 0), ("unshared", plain, 0), ("slabs", reqs, 1),
+                                    ("unshared_slabs", plain, 1)):
+        srv = _native.Server(memory_bytes=2 << 30, restart=50, inner_rtol=1e-4)
+        srv.set_inner_precision(precision)
+        if slab_tiles:
+            srv.set_table_policy(_native.Server.TABLES_SLABS, slab_tiles=slab_tiles)
+        runs[name] = serve(srv, batch, keep_solutions=True)
+        srv.close()
+    for r in reqs:
+        a = runs["shared"].results[r.index]
+        _fp64_check(r, a)
+        for name in ("unshared", "slabs", "unshared_slabs"):
+            b = runs[name].results[r.index]
+            assert b["info"] == 0
+            assert np.array_equal(a["x"], b["x"]) and a["iterations"] == b["iterations"], name
+            assert a["c_ext"] == b["c_ext"]
+            assert b["c_sca"] == pytest.approx(a["c_sca"], rel=1e-12)
+
+
 def test_inner_rtol_setter(mixed_requests, served):
     # a smaller reduction per restart cycle with the fp16x2 inner operator:
     # fewer cycles (FP64 residuals) in total, and every solution still meets

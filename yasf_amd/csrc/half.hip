@@ -38,13 +38,16 @@ __global__ __launch_bounds__(256) void min_distance_kernel(const double4 *__rest
 // the 8 sources of a lane's A fragment on consecutive threads so the fp16
 // stores of a wave are contiguous.  Builds target tiles t0.. (table points at
 // tile t0); sources up to N32 * 32 (zero past N).  pexp (block 0) receives
-// the order exponents.
+// the order exponents.  gate: as the GEMM's (returns unless *gate == 1).
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void half_table_build_kernel(
     const double4 *__restrict__ pos, int N, int N32, double k, int lmax, int ndm, const int *__restrict__ dmoff,
     const double *__restrict__ la, const double *__restrict__ lb, const double *__restrict__ ld,
     _Float16 *__restrict__ hi_tab, _Float16 *__restrict__ lo_tab, int t0,
-    const unsigned long long *__restrict__ dmin2, int *__restrict__ pexp, double near2) {
+    const unsigned long long *__restrict__ dmin2, int *__restrict__ pexp, double near2,
+    const int *__restrict__ gate) {
+    if (gate && *gate != 1)
+        return;
     __shared__ int es[kMaxOrders];
     const int P = 2 * lmax + 1;
     const double d2 = __longlong_as_double(static_cast<long long>(*dmin2));
@@ -138,9 +141,10 @@ __global__ __launch_bounds__(64 * WAVES) void half_gemm_kernel(
     const size_t f0 = (static_cast<size_t>(t) * P * P + dmoff[d]) * N32 * 1024;
     const size_t xplane = half_x_plane(nmax, N32);
     float *Qt = Q + 2 * (static_cast<size_t>(t0 + t) * nq + qbase) * 256;
-    half_gemm_block<WAVES, NTL, SPLIT>(hi_tab + f0, SPLIT ? lo_tab + f0 : nullptr, xhi, xlo, xplane, N32, nmax, R,
-                                       ip0, nord, ntile, colmap + d * maxcols + tile0 * 16, pexp + adm, xexp, Qt,
-                                       tstride, tile0);
+    (void)xlo;  // the lo terms follow the hi terms in one allocation
+    half_gemm_block<WAVES, NTL, SPLIT>(hi_tab + f0, SPLIT ? lo_tab + f0 : nullptr, HalfOneSource{xhi, xexp, Qt},
+                                       xplane, N32, nmax, R, ip0, nord, ntile, colmap + d * maxcols + tile0 * 16,
+                                       pexp + adm, tstride, tile0);
 }
 
 constexpr int kHalfWaves = 8;  // 2 orders per wave: kHalfJobOrders
@@ -157,25 +161,42 @@ size_t half_x_bytes(const DevicePlan &dp, int N) {
     return 4 * half_x_plane(dp.nmax, (N + 31) / 32) * sizeof(_Float16);  // 2 terms x 2 planes
 }
 
+// Smallest pair distance of the cluster (squared, as double bits) into dmin,
+// which the table builds read for the order exponents.
+void launch_half_min_distance(const double4 *pos, int N, unsigned long long *dmin, hipStream_t s) {
+    hipLaunchKernelGGL(init_min_kernel, dim3(1), dim3(1), 0, s, dmin);
+    hipLaunchKernelGGL(min_distance_kernel, dim3((N + 255) / 256), dim3(256), 0, s, pos, N, dmin);
+    YH_CHECK(hipGetLastError());
+}
+
+// Target tiles [t0, t0 + ntl) of the fp16 table into tab (hi term of tile t0
+// at tab, lo term term_halves further when split) and the order exponents
+// pexp[2L+1], from the smallest pair distance in dmin.
+void launch_half_table_range(const DevicePlan &dp, const double4 *pos, int N, double k, void *tab,
+                             size_t term_halves, bool split, int t0, int ntl, const unsigned long long *dmin,
+                             int *pexp, hipStream_t s, double near2, const int *gate) {
+    const int ntiles1 = (N + 15) / 16, N32 = (N + 31) / 32;
+    YH_REQUIRE(t0 >= 0 && ntl >= 1 && t0 + ntl <= ntiles1, "fp16 table range outside the target tiles");
+    const int ndm = 4 * dp.lmax + 1;
+    _Float16 *hi = static_cast<_Float16 *>(tab);
+    _Float16 *lo = split ? hi + term_halves : nullptr;
+    const dim3 grid(static_cast<unsigned>(ntl) * ndm * 2 * N32), block(256);
+    if (split)
+        hipLaunchKernelGGL(half_table_build_kernel<true>, grid, block, 0, s, pos, N, N32, k, dp.lmax, ndm, dp.dmoff,
+                           dp.la, dp.lb, dp.ld, hi, lo, t0, dmin, pexp, near2, gate);
+    else
+        hipLaunchKernelGGL(half_table_build_kernel<false>, grid, block, 0, s, pos, N, N32, k, dp.lmax, ndm, dp.dmoff,
+                           dp.la, dp.lb, dp.ld, hi, lo, t0, dmin, pexp, near2, gate);
+    YH_CHECK(hipGetLastError());
+}
+
 // Builds the fp16 table (hi term at table, lo term at table + term bytes when
 // split) and the order exponents pexp[2L+1]; dmin is device scratch.
 void launch_half_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, void *table, bool split,
                              int *pexp, unsigned long long *dmin, hipStream_t s, double near2) {
-    const int ntiles1 = (N + 15) / 16, N32 = (N + 31) / 32;
-    const int ndm = 4 * dp.lmax + 1;
-    hipLaunchKernelGGL(init_min_kernel, dim3(1), dim3(1), 0, s, dmin);
-    hipLaunchKernelGGL(min_distance_kernel, dim3((N + 255) / 256), dim3(256), 0, s, pos, N, dmin);
-    YH_CHECK(hipGetLastError());
-    _Float16 *hi = static_cast<_Float16 *>(table);
-    _Float16 *lo = split ? hi + half_table_term_bytes(dp, N) / sizeof(_Float16) : nullptr;
-    const dim3 grid(ntiles1 * ndm * 2 * N32), block(256);
-    if (split)
-        hipLaunchKernelGGL(half_table_build_kernel<true>, grid, block, 0, s, pos, N, N32, k, dp.lmax, ndm, dp.dmoff,
-                           dp.la, dp.lb, dp.ld, hi, lo, 0, dmin, pexp, near2);
-    else
-        hipLaunchKernelGGL(half_table_build_kernel<false>, grid, block, 0, s, pos, N, N32, k, dp.lmax, ndm, dp.dmoff,
-                           dp.la, dp.lb, dp.ld, hi, lo, 0, dmin, pexp, near2);
-    YH_CHECK(hipGetLastError());
+    launch_half_min_distance(pos, N, dmin, s);
+    launch_half_table_range(dp, pos, N, k, table, half_table_term_bytes(dp, N) / sizeof(_Float16), split, 0,
+                            (N + 15) / 16, dmin, pexp, s, near2, nullptr);
 }
 
 // x (N x nmax complex128) to its fp16 terms in xh (half_x_bytes) with the
@@ -196,23 +217,30 @@ void launch_half_x(const DevicePlan &dp, int N, const double2 *x, void *xh, bool
     YH_CHECK(hipGetLastError());
 }
 
-// Q (complex64, the table GEMM's layout) = G x over the fp16 table and x terms.
+// Q (complex64, the table GEMM's layout) = G x over the fp16 table and x
+// terms, for target tiles [t0, t0 + ntl) (ntl < 0: to the last); table holds
+// tile t0's hi term first and the lo terms term_halves further (0: the
+// whole table's term).
 void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void *xh, bool split, const int *pexp,
-                      const int *xexp, void *Q, hipStream_t s, const int *gate) {
+                      const int *xexp, void *Q, hipStream_t s, const int *gate, int t0, int ntl,
+                      size_t term_halves) {
     const int ntiles1 = (N + 15) / 16, N32 = (N + 31) / 32;
+    if (ntl < 0)
+        ntl = ntiles1 - t0;
+    YH_REQUIRE(t0 >= 0 && ntl >= 1 && t0 + ntl <= ntiles1, "fp16 GEMM range outside the target tiles");
     const _Float16 *hi = static_cast<const _Float16 *>(table);
-    const _Float16 *lo = hi + half_table_term_bytes(dp, N) / sizeof(_Float16);
+    const _Float16 *lo = hi + (term_halves ? term_halves : half_table_term_bytes(dp, N) / sizeof(_Float16));
     const _Float16 *xhi = static_cast<const _Float16 *>(xh);
     const _Float16 *xlo = xhi + 2 * half_x_plane(dp.nmax, N32);
     const int h = kHalfJobList;
-    const dim3 grid(dp.nhjobs[h] * ntiles1), block(64 * kHalfWaves);
+    const dim3 grid(dp.nhjobs[h] * ntl), block(64 * kHalfWaves);
     if (split)
         hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, kHalfJobTiles, true>), grid, block, 0, s, hi, lo, xhi,
-                           xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntiles1, 0, dp.colmap, dp.maxcols,
+                           xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntl, t0, dp.colmap, dp.maxcols,
                            dp.dmoff, static_cast<float *>(Q), dp.nq, gate);
     else
         hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, kHalfJobTiles, false>), grid, block, 0, s, hi, lo,
-                           xhi, xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntiles1, 0, dp.colmap,
+                           xhi, xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntl, t0, dp.colmap,
                            dp.maxcols, dp.dmoff, static_cast<float *>(Q), dp.nq, gate);
     YH_CHECK(hipGetLastError());
 }

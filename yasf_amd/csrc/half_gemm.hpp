@@ -75,14 +75,36 @@ __device__ __forceinline__ h8_t neg_h8(h8_t a) {
                                                                         0x80008000u});
 }
 
+// The source copies and outputs a block works on: one request's, or those
+// of several requests over one operator (Src::count() of them, each with the
+// job's ntile column tiles).  x(i): request i's copy (its hi term's plane 0,
+// the lo term 2 xplane halves further); exps(i): its column exponents;
+// out(i): its Q past the (job, target tile) offset; live(i): false for a
+// request whose copy is not current (its tiles read zeros and are not
+// stored).  Every i is wave-uniform where these are called.
+struct HalfOneSource {
+    static constexpr bool kMulti = false;  // one copy: one buffer descriptor for all B pieces
+    const _Float16 *xh;
+    const int *xexp;
+    float *Qt;
+    __device__ int count() const { return 1; }
+    __device__ bool live(int) const { return true; }
+    __device__ const _Float16 *x(int) const { return xh; }
+    __device__ const int *exps(int) const { return xexp; }
+    __device__ float *out(int) const { return Qt; }
+};
+
 // One (job, target tile) block of WAVES waves.  hi / lo point at the job's
-// (tile, dm) fragment 0 of the two terms (lo unused unless SPLIT); xhi / xlo
-// at the two terms' plane 0 of the source copy (plane 1 = Im follows at
-// xplane halves); cols: the job's 16 ntile source columns (-1: padding);
-// orders p = |dm| + ip for ip in [ip0, ip0 + nord), nord <= 2 WAVES, dealt
-// to the waves in turn (at most two each); pexp: order exponents from
-// p = |dm| on; xexp[n2] (xexp[nmax] = 0).  C tile (ip, it) goes to Q slot
-// ip * tstride + tile0 + it past Qt (complex64).
+// (tile, dm) fragment 0 of the two terms (lo unused unless SPLIT); src: the
+// source copies (plane 1 = Im follows plane 0 at xplane halves); cols: the
+// job's 16 ntile source columns (-1: padding), the same for every request of
+// src, whose column tiles follow one another (request i's tile c is block
+// tile i ntile + c; src.count() ntile <= NTL); orders p = |dm| + ip for ip
+// in [ip0, ip0 + nord), nord <= 2 WAVES, dealt to the waves in turn (at most
+// two each); pexp: order exponents from p = |dm| on; exps(i)[n2] (entry
+// nmax = 0).  C tile (ip, c) of request i goes to Q slot ip * tstride +
+// tile0 + c past out(i) (complex64).  Each table fragment is loaded once
+// and used for every request of src.
 //
 // Pipelining: the table stream (the operand that comes from HBM) is loaded
 // three K-steps ahead into four register sets used in turn; the B fragments
@@ -99,13 +121,11 @@ __device__ __forceinline__ h8_t neg_h8(h8_t a) {
 // the second only when the job has more than WAVES orders): the two-term
 // form over 8 column tiles.  Either way each table fragment is read once;
 // the sequential form stages the x fragments once per sweep.
-template <int WAVES, int NTL, bool SPLIT>
+template <int WAVES, int NTL, bool SPLIT, class Src>
 __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_tab, const _Float16 *__restrict__ lo_tab,
-                                                const _Float16 *__restrict__ xhi, const _Float16 *__restrict__ xlo,
-                                                size_t xplane, int N32, int nmax, int R, int ip0, int nord,
-                                                int ntile, const int *__restrict__ cols, const int *__restrict__ pexp,
-                                                const int *__restrict__ xexp, float *__restrict__ Qt, int tstride,
-                                                int tile0) {
+                                                const Src &src, size_t xplane, int N32, int nmax, int R, int ip0,
+                                                int nord, int ntile, const int *__restrict__ cols,
+                                                const int *__restrict__ pexp, int tstride, int tile0) {
     constexpr int NIP = (SPLIT && NTL > 4) ? 1 : 2;  // order slots per sweep
     constexpr int NSWEEP = 2 / NIP;
     constexpr int NPL = SPLIT ? 4 : 2;  // planes: hi Re, hi Im, lo Re, lo Im
@@ -117,28 +137,36 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
     const int lane = tid & 63, lo = lane & 15, hi = lane >> 4;
     const int wave = uniform(tid >> 6);
     const size_t S32 = static_cast<size_t>(N32) * 32;
-    // B pieces of this thread: byte offset (K-step 0) into the fp16 source
-    // copy (xhi .. 4 planes) and LDS slot
-    const __amdgpu_buffer_rsrc_t rx = buffer_rsrc(xhi, 4 * xplane * sizeof(_Float16));
+    const int nreq = src.count();
+    const int ntt = nreq * ntile;  // column tiles of the block
+    // B pieces of this thread: the copy they come from (a wave's 64 pieces
+    // of one u share a column tile, hence a request), byte offset (K-step 0)
+    // into that copy (4 planes) and LDS slot
+    constexpr int NRX = Src::kMulti ? PER : 1;
+    __amdgpu_buffer_rsrc_t rx[NRX];
     unsigned boff[PER];
     int bslot[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int q = tid + u * THREADS;
-        const int it = q / (NPL * 64), pl = (q / 64) % NPL, ln = q % 64;
-        const int n2 = (q < CHUNKS && it < ntile) ? cols[it * 16 + (ln & 15)] : -1;
+        const int itw = (wave * 64 + u * THREADS) / (NPL * 64);  // = q / (NPL * 64), wave-uniform
+        const int rq = min(itw / ntile, nreq - 1), c = itw - rq * ntile;
+        if (Src::kMulti || u == 0)
+            rx[Src::kMulti ? u : 0] = buffer_rsrc(src.x(rq), 4 * xplane * sizeof(_Float16));
+        const int pl = (q / 64) % NPL, ln = q % 64;
+        const int n2 = (q < CHUNKS && itw < ntt && src.live(rq)) ? cols[c * 16 + (ln & 15)] : -1;
         const size_t plane = (pl < 2 ? 0 : 2 * xplane) + (pl & 1) * xplane;
         boff[u] = n2 < 0 ? kBufferNull
                          : static_cast<unsigned>((plane + static_cast<size_t>(n2) * S32 + (ln >> 4) * 8) *
                                                  sizeof(_Float16));
         bslot[u] = q < CHUNKS ? q : -1;
     }
-    (void)xlo;  // the lo terms follow the hi terms in one allocation
     auto load_b = [&](int ks, h8_t (&breg)[PER]) {
         const unsigned so = static_cast<unsigned>(ks) * 32 * sizeof(_Float16);
 #pragma unroll
         for (int u = 0; u < PER; ++u)
-            breg[u] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rx, boff[u], so, 0));
+            breg[u] = __builtin_bit_cast(
+                h8_t, __builtin_amdgcn_raw_buffer_load_b128(rx[Src::kMulti ? u : 0], boff[u], so, 0));
     };
     auto store_b = [&](int buf, const h8_t (&breg)[PER]) {
 #pragma unroll
@@ -190,7 +218,7 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
         auto step = [&](const h8_t (&a)[NIP][NPL], int buf) {
 #pragma unroll
             for (int it = 0; it < NTL; ++it) {
-                if (it >= ntile)
+                if (it >= ntt)
                     continue;
                 h8_t b[NPL], nb[NPL / 2];
 #pragma unroll
@@ -270,11 +298,14 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
             const int ip = ip0 + slot[j];
 #pragma unroll
             for (int it = 0; it < NTL; ++it)
-                if (it < ntile) {
-                    const int n2 = cols[it * 16 + lo];
-                    const int e = pexp[ip] + xexp[n2 < 0 ? nmax : n2];
-                    const size_t qs = static_cast<size_t>(ip) * tstride + tile0 + it;
-                    float *dst = Qt + 2 * ((qs * 16 + lo) * 16 + 4 * hi);
+                if (it < ntt) {
+                    const int rq = it / ntile, c = it - rq * ntile;
+                    if (!src.live(rq))
+                        continue;
+                    const int n2 = cols[c * 16 + lo];
+                    const int e = pexp[ip] + src.exps(rq)[n2 < 0 ? nmax : n2];
+                    const size_t qs = static_cast<size_t>(ip) * tstride + tile0 + c;
+                    float *dst = src.out(rq) + 2 * ((qs * 16 + lo) * 16 + 4 * hi);
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         dst[2 * i] = ldexpf(acc[j][it][0][i], -e);

@@ -217,8 +217,74 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_gemm_kernel(c
     const size_t xplane = half_x_plane(q.nmax, q.N32);
     float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
     half_gemm_block<kServeHalfWaves, kHalfJobTiles, SPLIT>(
-        q.htab + f0, SPLIT ? q.htab + term + f0 : nullptr, q.xh, q.xh + 2 * xplane, xplane, q.N32, q.nmax,
-        P - adm, ip0, nord, ntile, q.colmap + d * q.maxcols + tile0 * 16, q.pexp + adm, q.xexp, Qt, tstride, tile0);
+        q.htab + f0, SPLIT ? q.htab + term + f0 : nullptr, HalfOneSource{q.xh, q.xexp, Qt}, xplane, q.N32, q.nmax,
+        P - adm, ip0, nord, ntile, q.colmap + d * q.maxcols + tile0 * 16, q.pexp + adm, tstride, tile0);
+}
+
+// Requests r0 .. r0+n-1 of an operator group (ridx[]) as the sources of one
+// fp16 GEMM block; qoff: the (job, target tile) offset into each one's Q.
+struct HalfGroupSource {
+    static constexpr bool kMulti = true;
+    const BatchReq *reqs;
+    const ReqState *st;
+    const int *ridx;
+    int r0, n;
+    size_t qoff;
+    __device__ int req(int i) const { return uniform(ridx[r0 + i]); }
+    __device__ int count() const { return n; }
+    __device__ bool live(int i) const { return st[req(i)].state == kArnoldi; }
+    __device__ const _Float16 *x(int i) const { return reqs[req(i)].xh; }
+    __device__ const int *exps(int i) const { return reqs[req(i)].xexp; }
+    __device__ float *out(int i) const { return reqs[req(i)].Q + qoff; }
+};
+
+// The same fp16 GEMM for the nr requests ridx[] of one operator over its
+// fp16 table or a slab of it: target tiles t0 .. t0+ntl-1, tab at tile t0's
+// hi term, the lo term term halves further.  A block computes one (job,
+// tile) for up to per requests (per x the job's column tiles <=
+// kHalfJobTiles), each table fragment loaded once for all of them.  Logical
+// block lb runs request group lb % ngrp of (job, tile) lb / ngrp, so the
+// groups' blocks of one table slice run together; the hardware hands block b
+// to XCD b % 8, so block b takes lb = (b % 8) (gridDim / 8) + b / 8 and
+// those blocks also share one XCD's L2 (the grid is padded to a multiple of
+// 8; blocks past total return).
+template <bool SPLIT>
+__global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_shared_gemm_kernel(
+    const BatchReq *__restrict__ reqs, const ReqState *__restrict__ st, const int *__restrict__ ridx, int nr,
+    int per, const _Float16 *__restrict__ tab, size_t term, int t0, int ntl, int total) {
+    const int lb = static_cast<int>(blockIdx.x % 8) * static_cast<int>(gridDim.x / 8) + static_cast<int>(blockIdx.x / 8);
+    if (lb >= total)
+        return;
+    const int ngrp = (nr + per - 1) / per;
+    const int grp = lb % ngrp;
+    const int rest = lb / ngrp;
+    const int tl = rest % ntl;
+    const int jid = rest / ntl;
+    const int r0 = grp * per, n = min(per, nr - r0);
+    int live = 0;
+    for (int i = 0; i < n; ++i)
+        live |= st[uniform(ridx[r0 + i])].state == kArnoldi;
+    if (!live)
+        return;
+    const BatchReq &q = reqs[uniform(ridx[r0])];  // operator data: the same in every request of the group
+    const PairJob *hjobs = q.hjobs[kHalfJobList];
+    const int dm = uniform(hjobs[jid].dm);
+    const int tile0 = uniform(hjobs[jid].tile0);
+    const int ntile = uniform(hjobs[jid].ntile);
+    const int qbase = uniform(hjobs[jid].qbase);
+    const int tstride = uniform(hjobs[jid].tstride);
+    const int nord = uniform(hjobs[jid].nord);
+    const int ip0 = uniform(hjobs[jid].pad);
+    const int P = 2 * q.lmax + 1;
+    const int adm = dm < 0 ? -dm : dm;
+    const int d = dm + 2 * q.lmax;
+    const size_t f0 = (static_cast<size_t>(tl) * P * P + q.dmoff[d]) * q.N32 * 1024;
+    const size_t xplane = half_x_plane(q.nmax, q.N32);
+    const HalfGroupSource src{reqs, st, ridx, r0, n, 2 * (static_cast<size_t>(t0 + tl) * q.nq + qbase) * 256};
+    half_gemm_block<kServeHalfWaves, kHalfJobTiles, SPLIT>(tab + f0, SPLIT ? tab + term + f0 : nullptr, src, xplane,
+                                                           q.N32, q.nmax, P - adm, ip0, nord, ntile,
+                                                           q.colmap + d * q.maxcols + tile0 * 16, q.pexp + adm,
+                                                           tstride, tile0);
 }
 
 // One (job, target tile t) block of a request's complex64 pair GEMM; tab
@@ -606,10 +672,14 @@ struct SharedOp {
     void *slab = nullptr;
     int slab_tiles64 = 0;
     double kR = 0.0;  // k times the largest distance of a sphere from the origin (far-field quadrature)
-    // fp16-term table in place of the complex64 one (inner precision
-    // FP16X2 / FP16): its terms, order exponents and scratch
+    // fp16 terms in place of the complex64 table or slab (inner precision
+    // FP16X2 / FP16): the resident table, or slabs of hslab_tiles target
+    // tiles in the slab buffer; hterm: halves from the hi to the lo term;
+    // order exponents and scratch
     bool half = false, half_split = false;
     void *htab = nullptr;
+    int hslab_tiles = 0;
+    size_t hterm = 0;
     int *pexp = nullptr;
     unsigned long long *dmin = nullptr;
 };
@@ -705,8 +775,9 @@ struct yasfhip_server {
     int slab_tiles_req = 0;  // FP64 target tiles per slab (0: an eighth of the arena)
     int csca_method = YASFHIP_CSCA_FAR_FIELD;  // how completions compute C_sca
     int inner_precision = YASFHIP_FP32;         // pair GEMM of the inner GMRES iterations
-    std::vector<int> half_reqs;                 // active requests whose operator has an fp16 table
+    std::vector<int> half_reqs;                 // active requests whose operator has fp16 terms
     int half_index = 0, half_colmax_blocks = 0, half_convert_blocks = 0, half_gemm_blocks = 0;
+    int half_gemm_count = 0;  // of them, those with an unshared resident fp16 table
     std::deque<Request *> finished;
     long long completed = 0;
     // batched launch data
@@ -926,8 +997,15 @@ struct yasfhip_server {
             attempt(false, false, 0, cap);
         if (!o->block)
             return nullptr;
-        if (!o->table32)
-            o->half = false;  // the fp16 form replaces the complex64 table only
+        if (o->half && !o->table32) {  // fp16 slabs in the slab buffer, as many tiles as fit
+            const size_t tile = half_tile_halves(q.L, (q.N + 31) / 32) * sizeof(_Float16) * (o->half_split ? 2 : 1);
+            const size_t bytes = o->slab_tiles64 ? tile64 * o->slab_tiles64 : 0;
+            o->hslab_tiles = static_cast<int>(std::min<size_t>(bytes / tile, ntiles1));
+            if (o->hslab_tiles < 1)
+                o->half = false;  // no slab (pair factors on the fly): complex64
+        }
+        if (o->half)
+            o->hterm = (o->table32 ? ntiles1 : o->hslab_tiles) * half_tile_halves(q.L, (q.N + 31) / 32);
         layout_op(*o, true);
         hipStream_t st = stream;
         YH_CHECK(hipMemcpyAsync(o->pos_d, q.pos.data(), sizeof(double4) * q.pos.size(), hipMemcpyHostToDevice,
@@ -939,8 +1017,11 @@ struct yasfhip_server {
         } else {
             YH_CHECK(hipMemcpyAsync(o->t_d, q.t.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
         }
+        if (o->half)
+            launch_half_min_distance(o->pos_d, o->N, o->dmin, st);
         if (o->table32 && o->half)
-            launch_half_table_build(dp, o->pos_d, o->N, o->k, o->htab, o->half_split, o->pexp, o->dmin, st, kAllPairs);
+            launch_half_table_range(dp, o->pos_d, o->N, o->k, o->htab, o->hterm, o->half_split, 0, o->ntiles1, o->dmin,
+                                    o->pexp, st, kAllPairs);
         else if (o->table32)
             launch_pair_table_build(dp, o->pos_d, o->N, o->k, true, o->table, st);
         if (o->table64)
@@ -1099,6 +1180,23 @@ struct yasfhip_server {
             if (o.table32 && o.half) {
                 launch_half_x(dp, q->N, q->x_d, q->xh, o.half_split, q->colmax, q->xexp, st);
                 launch_half_gemm(dp, q->N, o.htab, q->xh, o.half_split, o.pexp, q->xexp, q->Q, st);
+            } else if (o.slab && o.half) {  // fp16 slabs, one build for the operator's requests
+                for (size_t j = i; j < todo.size(); ++j)
+                    if (todo[j]->op == q->op && todo[j]->approx_residual)
+                        launch_half_x(dp, q->N, todo[j]->x_d, todo[j]->xh, o.half_split, todo[j]->colmax,
+                                      todo[j]->xexp, st);
+                for (int t0 = 0; t0 < o.ntiles1; t0 += o.hslab_tiles) {
+                    const int ntl = std::min(o.hslab_tiles, o.ntiles1 - t0);
+                    launch_half_table_range(dp, o.pos_d, o.N, o.k, o.slab, o.hterm, o.half_split, t0, ntl, o.dmin,
+                                            o.pexp, st, kAllPairs);
+                    for (size_t j = i; j < todo.size(); ++j)
+                        if (todo[j]->op == q->op && todo[j]->approx_residual)
+                            launch_half_gemm(dp, q->N, o.slab, todo[j]->xh, o.half_split, o.pexp, todo[j]->xexp,
+                                             todo[j]->Q, st, nullptr, t0, ntl, o.hterm);
+                }
+                for (size_t j = i; j < todo.size(); ++j)
+                    if (todo[j]->op == q->op && todo[j]->approx_residual)
+                        gemm_done[j] = 1;
             } else if (o.table32) {
                 launch_pair_table_gemm(dp, q->N, o.table, q->x32, true, kServeShape32, q->Q, st);
             } else if (o.slab) {  // complex64 slabs (twice the FP64 tiles), one build for the operator's requests
@@ -1394,32 +1492,41 @@ struct yasfhip_server {
             grp_count[gi] = cnt;
             grp_blocks[gi] = blocks;
         }
-        // requests over an fp16-term table: block ranges of the three
-        // batched steps (column maxima, fp16 terms, GEMM) and their indices
+        // requests over fp16 terms: block ranges of the batched column maxima
+        // and fp16 terms (all of them) and of the per-request GEMM (those
+        // with an unshared resident table; the others run in operator
+        // groups below), with their indices
         half_reqs.clear();
+        half_gemm_count = 0;
         for (int r = 0; r < nreq; ++r)
-            if (active[r]->op->half)
+            if (active[r]->op->half) {
                 half_reqs.push_back(r);
+                half_gemm_count += active[r]->op->table32 && op_group_size(r) < 2;
+            }
         {
-            const int nh = static_cast<int>(half_reqs.size());
+            const int nh = static_cast<int>(half_reqs.size()), ng = half_gemm_count;
             half_index = static_cast<int>(h_index.size());
-            h_index.resize(h_index.size() + 4 * (nh + 1), 0);
-            int *hc = h_index.data() + half_index, *hv = hc + nh + 1, *hg = hv + nh + 1, *hi = hg + nh + 1;
-            int bc = 0, bv = 0, bg = 0;
+            h_index.resize(h_index.size() + 3 * (nh + 1) + 2 * (ng + 1), 0);
+            int *hc = h_index.data() + half_index, *hv = hc + nh + 1, *hi = hv + nh + 1, *hg = hi + nh + 1,
+                *hgi = hg + ng + 1;
+            int bc = 0, bv = 0, bg = 0, g = 0;
             for (int h = 0; h < nh; ++h) {
                 const Request &q = *active[half_reqs[h]];
                 const int N32 = (q.N + 31) / 32;
                 hc[h] = bc;
                 hv[h] = bv;
-                hg[h] = bg;
                 hi[h] = half_reqs[h];
                 bc += (q.N + 255) / 256;
                 bv += static_cast<int>((static_cast<long>(q.nmax + 1) * N32 * 32 + 255) / 256);
-                bg += plan_for(q.L).dp.nhjobs[kHalfJobList] * q.ntiles1;
+                if (q.op->table32 && op_group_size(half_reqs[h]) < 2) {
+                    hg[g] = bg;
+                    hgi[g++] = half_reqs[h];
+                    bg += plan_for(q.L).dp.nhjobs[kHalfJobList] * q.ntiles1;
+                }
             }
             hc[nh] = bc;
             hv[nh] = bv;
-            hg[nh] = bg;
+            hg[ng] = bg;
             half_colmax_blocks = bc;
             half_convert_blocks = bv;
             half_gemm_blocks = bg;
@@ -1428,7 +1535,7 @@ struct yasfhip_server {
         slab_groups.clear();
         for (int r = 0; r < nreq; ++r) {
             SharedOp *o = active[r]->op;
-            if (o->half || (o->table32 && op_group_size(r) < 2))
+            if (o->table32 && op_group_size(r) < 2)
                 continue;  // in the batched GEMMs above
             if (!o->table32 && !o->slab) {
                 otf.push_back(r);
@@ -1511,6 +1618,33 @@ struct yasfhip_server {
         return c;
     }
 
+    // The fp16 GEMM of an operator group's requests over target tiles [t0,
+    // t0 + ntl) of its resident fp16 table (tab, all tiles) or of a slab.
+    void launch_half_shared_gemm(const SlabGroup &g, const void *tab, int t0, int ntl) {
+        const SharedOp &o = *g.op;
+        const PlanEntry &pe = plan_for(o.L);
+        const DevicePlan &dp = pe.dp;
+        // requests per block: as many as the widest job's column tiles allow
+        int maxnt = 1;
+        for (const PairJob &j : pe.plan.half_jobs[kHalfJobList])
+            maxnt = std::max(maxnt, j.ntile);
+        const int per = std::max(1, std::min(g.count, kHalfJobTiles / maxnt));
+        YH_REQUIRE(per * maxnt <= kHalfJobTiles || per == 1, "fp16 group blocks exceed the column tiles");
+        const long total = static_cast<long>(dp.nhjobs[kHalfJobList]) * ntl * ((g.count + per - 1) / per);
+        YH_REQUIRE(total < (1L << 31) - 8, "fp16 operator group grid too large");
+        const int blocks = static_cast<int>((total + 7) / 8 * 8);
+        const int *ridx = d_index + g.first;
+        const _Float16 *h = static_cast<const _Float16 *>(tab);
+        if (o.half_split)
+            hipLaunchKernelGGL(serve_half_shared_gemm_kernel<true>, dim3(blocks), dim3(64 * kServeHalfWaves), 0, stream,
+                               d_reqs, d_state, ridx, g.count, per, h, o.hterm, t0, ntl, static_cast<int>(total));
+        else
+            hipLaunchKernelGGL(serve_half_shared_gemm_kernel<false>, dim3(blocks), dim3(64 * kServeHalfWaves), 0,
+                               stream, d_reqs, d_state, ridx, g.count, per, h, o.hterm, t0, ntl,
+                               static_cast<int>(total));
+        YH_CHECK(hipGetLastError());
+    }
+
     // The complex64 GEMM of an operator group's requests over target tiles
     // [t0, t0 + ntl) of its resident table (all tiles) or of its slab.
     void launch_slab_gemm(const SlabGroup &g, int t0, int ntl) {
@@ -1563,23 +1697,25 @@ struct yasfhip_server {
             const int *gidx = gstart + nreq + 1;
             launch_gemm(gi, gstart, gidx, grp_count[gi], grp_blocks[gi]);
         }
-        if (!half_reqs.empty()) {  // the fp16 pair GEMM of requests over fp16-term tables
-            const int nh = static_cast<int>(half_reqs.size());
-            const int *hc = d_index + half_index, *hv = hc + nh + 1, *hg = hv + nh + 1, *hi = hg + nh + 1;
+        if (!half_reqs.empty()) {  // fp16 terms of the requests over fp16 operators, and the GEMMs of unshared tables
+            const int nh = static_cast<int>(half_reqs.size()), ng = half_gemm_count;
+            const int *hc = d_index + half_index, *hv = hc + nh + 1, *hi = hv + nh + 1, *hg = hi + nh + 1,
+                      *hgi = hg + ng + 1;
             const bool split = inner_precision == YASFHIP_FP16X2;
             hipLaunchKernelGGL(serve_half_colmax_kernel, dim3(half_colmax_blocks), dim3(256), 0, st, d_reqs, d_state,
                                hc, hi, nh);
-            if (split) {
+            if (split)
                 hipLaunchKernelGGL(serve_half_convert_kernel<true>, dim3(half_convert_blocks), dim3(256), 0, st,
                                    d_reqs, d_state, hv, hi, nh);
-                hipLaunchKernelGGL(serve_half_gemm_kernel<true>, dim3(half_gemm_blocks), dim3(64 * kServeHalfWaves),
-                                   0, st, d_reqs, d_state, hg, hi, nh);
-            } else {
+            else
                 hipLaunchKernelGGL(serve_half_convert_kernel<false>, dim3(half_convert_blocks), dim3(256), 0, st,
                                    d_reqs, d_state, hv, hi, nh);
+            if (ng > 0 && split)
+                hipLaunchKernelGGL(serve_half_gemm_kernel<true>, dim3(half_gemm_blocks), dim3(64 * kServeHalfWaves),
+                                   0, st, d_reqs, d_state, hg, hgi, ng);
+            else if (ng > 0)
                 hipLaunchKernelGGL(serve_half_gemm_kernel<false>, dim3(half_gemm_blocks), dim3(64 * kServeHalfWaves),
-                                   0, st, d_reqs, d_state, hg, hi, nh);
-            }
+                                   0, st, d_reqs, d_state, hg, hgi, ng);
             YH_CHECK(hipGetLastError());
         }
         // operator groups: the requests' GEMMs block-interleaved over the
@@ -1588,13 +1724,25 @@ struct yasfhip_server {
             const SharedOp &o = *g.op;
             const DevicePlan &dp = plan_for(o.L).dp;
             if (o.table32) {
-                launch_slab_gemm(g, 0, o.ntiles1);
+                if (o.half)
+                    launch_half_shared_gemm(g, o.htab, 0, o.ntiles1);
+                else
+                    launch_slab_gemm(g, 0, o.ntiles1);
                 continue;
             }
             int *flag = d_flags + g.flag;
             hipLaunchKernelGGL(serve_any_active_kernel, dim3(1), dim3(64), 0, st, d_state, d_index + g.first,
                                g.count, flag);
             YH_CHECK(hipGetLastError());
+            if (o.half) {
+                for (int t0 = 0; t0 < o.ntiles1; t0 += o.hslab_tiles) {
+                    const int ntl = std::min(o.hslab_tiles, o.ntiles1 - t0);
+                    launch_half_table_range(dp, o.pos_d, o.N, o.k, o.slab, o.hterm, o.half_split, t0, ntl, o.dmin,
+                                            o.pexp, st, kAllPairs, flag);
+                    launch_half_shared_gemm(g, o.slab, t0, ntl);
+                }
+                continue;
+            }
             const int tiles = 2 * o.slab_tiles64;
             for (int t0 = 0; t0 < o.ntiles1; t0 += tiles) {
                 const int ntl = std::min(tiles, o.ntiles1 - t0);
