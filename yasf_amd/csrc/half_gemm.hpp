@@ -106,14 +106,12 @@ struct HalfOneSource {
 // tile0 + c past out(i) (complex64).  Each table fragment is loaded once
 // and used for every request of src.
 //
-// Pipelining (one request): the table stream (the operand that comes from
-// HBM) is loaded three K-steps ahead into four register sets used in turn;
-// the B fragments of a K-step (the job's column tiles, shared by all waves)
-// are loaded by the whole block two K-steps ahead into one of two register
-// sets and stored one K-step ahead into one of two LDS buffers.  With
-// several requests the B fragments of all of them (which no longer fit one
-// XCD's L2) take the longer lead: both operands in three register sets,
-// each with two K-steps to arrive.  Operands come
+// Pipelining: the table stream (the operand that comes from HBM) and the B
+// fragments of a K-step (the column tiles of the job's requests, shared by
+// all waves, loaded by the whole block and stored to one of two LDS
+// buffers a K-step ahead) each have three register sets and two K-steps to
+// arrive.  (Table fragments three K-steps ahead with the x pieces one step
+// in flight were 5-10% slower.)  Operands come
 // through raw buffer loads: a wave's order slot, when it has none, and the B
 // pieces of absent column tiles read out of range, which returns zero
 // without touching memory, so every load is unconditional and the compiler's
@@ -256,79 +254,39 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
         };
         if (sweep > 0)
             __syncthreads();  // the previous sweep's last LDS reads are done
-        if constexpr (Src::kMulti) {
-            // Several requests: the x pieces (a block's whole B, from L2 or
-            // beyond) take the longer lead.  K-step ks computes with A set
-            // ks % 3 and LDS buffer ks % 2; it loads the table fragments of
-            // ks + 2 (set (ks + 2) % 3) before its MFMAs, then stores the x
-            // pieces of ks + 1 (B set (ks + 1) % 3) to LDS and reloads that
-            // set with ks + 4.  Waiting for A of ks forces the x pieces of
-            // ks + 1 (issued before it): both have two K-steps to arrive.
-            h8_t a0[NIP][NPL], a1[NIP][NPL], a2[NIP][NPL];
-            h8_t b0[PER], b1[PER], b2[PER];
-            load_b(0, b0);
-            store_b(0, b0);
-            load_b(kc(1), b1);
-            load_b(kc(2), b2);
-            load_a(0, a0);
-            load_a(kc(1), a1);
-            load_b(kc(3), b0);
+        // K-step ks computes with A set ks % 3 and LDS buffer ks % 2; it
+        // loads the table fragments of ks + 2 (set (ks + 2) % 3) before its
+        // MFMAs, then stores the x pieces of ks + 1 (B set (ks + 1) % 3) to
+        // LDS and reloads that set with ks + 4.  The vector-memory counter
+        // completes in order, so waiting for A of ks forces the x pieces of
+        // ks + 1 (issued before it): both have two K-steps to arrive.
+        // K-steps past the last reload the last one (the loads stay
+        // unconditional).
+        h8_t a0[NIP][NPL], a1[NIP][NPL], a2[NIP][NPL];
+        h8_t b0[PER], b1[PER], b2[PER];
+        load_b(0, b0);
+        store_b(0, b0);
+        load_b(kc(1), b1);
+        load_b(kc(2), b2);
+        load_a(0, a0);
+        load_a(kc(1), a1);
+        load_b(kc(3), b0);
+        __syncthreads();
+        auto kstep = [&](int ks, const h8_t (&acur)[NIP][NPL], h8_t (&anext)[NIP][NPL], h8_t (&bnext)[PER]) {
+            load_a(kc(ks + 2), anext);
+            step(acur, ks & 1);
+            store_b((ks + 1) & 1, bnext);
+            load_b(kc(ks + 4), bnext);
             __syncthreads();
-            auto kstep = [&](int ks, const h8_t (&acur)[NIP][NPL], h8_t (&anext)[NIP][NPL], h8_t (&bnext)[PER]) {
-                load_a(kc(ks + 2), anext);
-                step(acur, ks & 1);
-                store_b((ks + 1) & 1, bnext);
-                load_b(kc(ks + 4), bnext);
-                __syncthreads();
-            };
-            for (int ks = 0; ks < N32; ks += 3) {
-                kstep(ks, a0, a2, b1);
-                if (ks + 1 >= N32)
-                    break;
-                kstep(ks + 1, a1, a0, b2);
-                if (ks + 2 >= N32)
-                    break;
-                kstep(ks + 2, a2, a1, b0);
-            }
-        } else {
-            // K-step ks computes with A register set ks % 4 and LDS buffer
-            // ks % 2.  In it, the x pieces of ks + 2 are loaded into register
-            // set ks % 2 before the table fragments of ks + 3 (set (ks + 3) %
-            // 4), and the x pieces of ks + 1 (loaded a step earlier) go to LDS
-            // buffer (ks + 1) % 2 after the MFMAs.  The vector-memory counter
-            // completes in order, so waiting for x of ks + 1 only forces the
-            // table loads issued before it: each table fragment has three
-            // K-steps to arrive, each x piece two.  K-steps past the last
-            // reload the last one (the loads stay unconditional).
-            h8_t a0[NIP][NPL], a1[NIP][NPL], a2[NIP][NPL], a3[NIP][NPL];
-            h8_t bx0[PER], bx1[PER];
-            load_b(0, bx0);
-            store_b(0, bx0);
-            load_b(kc(1), bx1);
-            load_a(0, a0);
-            load_a(kc(1), a1);
-            load_a(kc(2), a2);
-            __syncthreads();
-            auto kstep = [&](int ks, const h8_t (&acur)[NIP][NPL], h8_t (&anext)[NIP][NPL], h8_t (&bload)[PER],
-                             const h8_t (&bstore)[PER], int buf) {
-                load_b(kc(ks + 2), bload);
-                load_a(kc(ks + 3), anext);
-                step(acur, buf);
-                store_b(buf ^ 1, bstore);
-                __syncthreads();
-            };
-            for (int ks = 0; ks < N32; ks += 4) {
-                kstep(ks, a0, a3, bx0, bx1, 0);
-                if (ks + 1 >= N32)
-                    break;
-                kstep(ks + 1, a1, a0, bx1, bx0, 1);
-                if (ks + 2 >= N32)
-                    break;
-                kstep(ks + 2, a2, a1, bx0, bx1, 0);
-                if (ks + 3 >= N32)
-                    break;
-                kstep(ks + 3, a3, a2, bx1, bx0, 1);
-            }
+        };
+        for (int ks = 0; ks < N32; ks += 3) {
+            kstep(ks, a0, a2, b1);
+            if (ks + 1 >= N32)
+                break;
+            kstep(ks + 1, a1, a0, b2);
+            if (ks + 2 >= N32)
+                break;
+            kstep(ks + 2, a2, a1, b0);
         }
 #pragma unroll
         for (int j = 0; j < NIP; ++j) {
