@@ -400,10 +400,11 @@ int yasfhip_server_set_table_policy(yasfhip_server *s, int policy, int slab_tile
 /* Pair GEMM of the inner GMRES iterations of operators admitted from now
  * on: YASFHIP_FP32 (default: complex64 tables and GEMMs), YASFHIP_FP16X2 or
  * YASFHIP_FP16 (fp16-term tables and the fp16 pair GEMM; see the precision
- * enum).  Applies to operators that keep a resident table; slab and
- * on-the-fly operators keep the complex64 GEMM.  FP64 residuals decide
- * convergence in every case.  Needs an idle server (nothing queued or
- * active). */
+ * enum).  Applies to operators that keep a resident table or a slab (the
+ * slab then holds fp16 target tiles; the requests of a shared operator run
+ * one fp16 GEMM, several per block); on-the-fly operators keep the
+ * complex64 GEMM.  FP64 residuals decide convergence in every case.  Needs
+ * an idle server (nothing queued or active). */
 int yasfhip_server_set_inner_precision(yasfhip_server *s, int precision);
 
 /* Reduction each restart cycle is asked for at least (the inner_rtol of
