@@ -977,11 +977,20 @@ struct yasfhip_server {
                 o->block = arena.alloc(bytes);
             return o->block != nullptr;
         };
+        // the inner table with an FP64 slab for the residuals: the largest
+        // slab that can be carved now, down to 16 tiles (smaller slabs
+        // starve the FP64 GEMM of blocks; without a slab the residuals run
+        // on the fly, 1.5x the slab form's time)
+        auto table_with_slab = [&]() {
+            for (int t = slab64s; t >= 1 && t >= std::min(16, slab64s); t /= 2)
+                if (attempt(true, false, t, cap / 2 + tile64 * t))
+                    return true;
+            return false;
+        };
         bool table_form = false;  // policy 1 and the complex64 table fits the arena's share
         if (table_policy == 1) {
             table_form = true;
-            if (!attempt(true, true, 0, cap / 4) && !(slab64s >= 1 && attempt(true, false, slab64s, cap / 2 + tile64 * slab64s)) &&
-                !attempt(true, false, 0, cap / 2)) {
+            if (!attempt(true, true, 0, cap / 4) && !table_with_slab() && !attempt(true, false, 0, cap / 2)) {
                 o->table32 = true;
                 o->table64 = false;
                 o->slab_tiles64 = 0;
