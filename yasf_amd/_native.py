@@ -406,7 +406,8 @@ class Session:
     def tune_matvec(self, channel: int = 0, precision: int = FP64, reps: int = 3):
         """Picks the table GEMM's shape; returns (index, ms per shape: 8 waves with
         2/4/8 column tiles, then 4 waves with 2/4/8; -1 where not offered).
-        FP64 shape 2 runs 4 waves of one order each over 8 column tiles."""
+        FP64 shape 2 runs 8 waves that compute their orders one at a time over
+        jobs of up to 6 column tiles."""
         g = ctypes.c_int()
         ms = np.zeros(6)
         _check(self._lib.yasfhip_tune_matvec(self._h, int(channel), int(precision), int(reps),
