@@ -34,28 +34,33 @@ __global__ __launch_bounds__(256) void min_distance_kernel(const double4 *__rest
         atomicMin(out, static_cast<unsigned long long>(__double_as_longlong(best)));
 }
 
+// The order exponents pexp[p] (p < P) from the smallest squared pair
+// distance (double bits in dmin2).  One block.
+__global__ void half_exponents_kernel(const unsigned long long *__restrict__ dmin2, double k, int P,
+                                      int *__restrict__ pexp) {
+    const double d2 = __longlong_as_double(static_cast<long long>(*dmin2));
+    for (int p = threadIdx.x; p < P; p += blockDim.x)
+        pexp[p] = half_order_exponent(p, k, d2);
+}
+
 // One block per (target tile, dm, 16-source chunk); thread = one pair, with
 // the 8 sources of a lane's A fragment on consecutive threads so the fp16
 // stores of a wave are contiguous.  Builds target tiles t0.. (table points at
-// tile t0); sources up to N32 * 32 (zero past N).  pexp (block 0) receives
-// the order exponents.  gate: as the GEMM's (returns unless *gate == 1).
+// tile t0); sources up to N32 * 32 (zero past N); pexp: the order exponents.
+// gate: as the GEMM's (returns unless *gate == 1).
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void half_table_build_kernel(
     const double4 *__restrict__ pos, int N, int N32, double k, int lmax, int ndm, const int *__restrict__ dmoff,
     const double *__restrict__ la, const double *__restrict__ lb, const double *__restrict__ ld,
-    _Float16 *__restrict__ hi_tab, _Float16 *__restrict__ lo_tab, int t0,
-    const unsigned long long *__restrict__ dmin2, int *__restrict__ pexp, double near2,
-    const int *__restrict__ gate) {
+    _Float16 *__restrict__ hi_tab, _Float16 *__restrict__ lo_tab, int t0, const int *__restrict__ pexp,
+    double near2, const int *__restrict__ gate) {
     if (gate && *gate != 1)
         return;
     __shared__ int es[kMaxOrders];
     const int P = 2 * lmax + 1;
-    const double d2 = __longlong_as_double(static_cast<long long>(*dmin2));
     if (threadIdx.x < P)
-        es[threadIdx.x] = half_order_exponent(threadIdx.x, k, d2);
+        es[threadIdx.x] = pexp[threadIdx.x];
     __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x < P && pexp)
-        pexp[threadIdx.x] = es[threadIdx.x];
     const int nchunk = 2 * N32;
     const int c = blockIdx.x % nchunk;
     const int d = (blockIdx.x / nchunk) % ndm;
@@ -73,16 +78,26 @@ __global__ __launch_bounds__(256) void half_table_build_kernel(
     const bool valid = (s1 < N) && (s2 < N) && s1 != s2 && dx * dx + dy * dy + dz * dz <= near2;
     const size_t base = ((static_cast<size_t>(t) * SR + dmoff[d]) * N32 + static_cast<size_t>(ks) * R) * 1024 +
                         lane * 8 + i;
+    // fp16 terms through FP32: two conversion instructions instead of the
+    // correctly rounded FP64 -> fp16 integer sequence (the empty asm keeps
+    // the compiler from folding the two roundings back into that one).  hi
+    // may differ from the nearest fp16 in a rare double-rounding tie; lo =
+    // v - hi, formed in FP64, takes up the difference.
+    auto to_half = [](double v) {
+        float f = static_cast<float>(v);
+        asm volatile("" : "+v"(f));
+        return static_cast<_Float16>(f);
+    };
     pair_series_to(dx, dy, dz, k, dm, P, valid, la, lb, ld, [&](int ip, double gr, double gi) {
         const int e = es[adm + ip];
         const double vr = ldexp(gr, e), vi = ldexp(gi, e);
-        const _Float16 hr = static_cast<_Float16>(vr), hm = static_cast<_Float16>(vi);
+        const _Float16 hr = to_half(vr), hm = to_half(vi);
         const size_t o = base + static_cast<size_t>(ip) * 1024;
         hi_tab[o] = hr;
         hi_tab[o + 512] = hm;
         if constexpr (SPLIT) {
-            lo_tab[o] = static_cast<_Float16>(vr - static_cast<double>(hr));
-            lo_tab[o + 512] = static_cast<_Float16>(vi - static_cast<double>(hm));
+            lo_tab[o] = to_half(vr - static_cast<double>(hr));
+            lo_tab[o + 512] = to_half(vi - static_cast<double>(hm));
         }
     });
 }
@@ -161,20 +176,22 @@ size_t half_x_bytes(const DevicePlan &dp, int N) {
     return 4 * half_x_plane(dp.nmax, (N + 31) / 32) * sizeof(_Float16);  // 2 terms x 2 planes
 }
 
-// Smallest pair distance of the cluster (squared, as double bits) into dmin,
-// which the table builds read for the order exponents.
-void launch_half_min_distance(const double4 *pos, int N, unsigned long long *dmin, hipStream_t s) {
+// The order exponents pexp[2L+1] of a cluster: its smallest pair distance
+// (into the scratch word dmin), then the bound of each order.
+void launch_half_exponents(const DevicePlan &dp, const double4 *pos, int N, double k, unsigned long long *dmin,
+                           int *pexp, hipStream_t s) {
     hipLaunchKernelGGL(init_min_kernel, dim3(1), dim3(1), 0, s, dmin);
     hipLaunchKernelGGL(min_distance_kernel, dim3((N + 255) / 256), dim3(256), 0, s, pos, N, dmin);
+    hipLaunchKernelGGL(half_exponents_kernel, dim3(1), dim3(64), 0, s, dmin, k, 2 * dp.lmax + 1, pexp);
     YH_CHECK(hipGetLastError());
 }
 
 // Target tiles [t0, t0 + ntl) of the fp16 table into tab (hi term of tile t0
-// at tab, lo term term_halves further when split) and the order exponents
-// pexp[2L+1], from the smallest pair distance in dmin.
+// at tab, lo term term_halves further when split), scaled by the order
+// exponents pexp (launch_half_exponents).
 void launch_half_table_range(const DevicePlan &dp, const double4 *pos, int N, double k, void *tab,
-                             size_t term_halves, bool split, int t0, int ntl, const unsigned long long *dmin,
-                             int *pexp, hipStream_t s, double near2, const int *gate) {
+                             size_t term_halves, bool split, int t0, int ntl, const int *pexp, hipStream_t s,
+                             double near2, const int *gate) {
     const int ntiles1 = (N + 15) / 16, N32 = (N + 31) / 32;
     YH_REQUIRE(t0 >= 0 && ntl >= 1 && t0 + ntl <= ntiles1, "fp16 table range outside the target tiles");
     const int ndm = 4 * dp.lmax + 1;
@@ -183,10 +200,10 @@ void launch_half_table_range(const DevicePlan &dp, const double4 *pos, int N, do
     const dim3 grid(static_cast<unsigned>(ntl) * ndm * 2 * N32), block(256);
     if (split)
         hipLaunchKernelGGL(half_table_build_kernel<true>, grid, block, 0, s, pos, N, N32, k, dp.lmax, ndm, dp.dmoff,
-                           dp.la, dp.lb, dp.ld, hi, lo, t0, dmin, pexp, near2, gate);
+                           dp.la, dp.lb, dp.ld, hi, lo, t0, pexp, near2, gate);
     else
         hipLaunchKernelGGL(half_table_build_kernel<false>, grid, block, 0, s, pos, N, N32, k, dp.lmax, ndm, dp.dmoff,
-                           dp.la, dp.lb, dp.ld, hi, lo, t0, dmin, pexp, near2, gate);
+                           dp.la, dp.lb, dp.ld, hi, lo, t0, pexp, near2, gate);
     YH_CHECK(hipGetLastError());
 }
 
@@ -194,9 +211,9 @@ void launch_half_table_range(const DevicePlan &dp, const double4 *pos, int N, do
 // split) and the order exponents pexp[2L+1]; dmin is device scratch.
 void launch_half_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, void *table, bool split,
                              int *pexp, unsigned long long *dmin, hipStream_t s, double near2) {
-    launch_half_min_distance(pos, N, dmin, s);
+    launch_half_exponents(dp, pos, N, k, dmin, pexp, s);
     launch_half_table_range(dp, pos, N, k, table, half_table_term_bytes(dp, N) / sizeof(_Float16), split, 0,
-                            (N + 15) / 16, dmin, pexp, s, near2, nullptr);
+                            (N + 15) / 16, pexp, s, near2, nullptr);
 }
 
 // x (N x nmax complex128) to its fp16 terms in xh (half_x_bytes) with the

@@ -199,12 +199,14 @@ size_t half_table_term_bytes(const DevicePlan &dp, int N);
 size_t half_x_bytes(const DevicePlan &dp, int N);
 void launch_half_table_build(const DevicePlan &dp, const double4 *pos, int N, double k, void *table, bool split,
                              int *pexp, unsigned long long *dmin, hipStream_t s, double near2);
-// The build in two parts, for slabs: the smallest pair distance once, then
-// target tiles [t0, t0 + ntl) into tab (lo term term_halves past the hi term).
-void launch_half_min_distance(const double4 *pos, int N, unsigned long long *dmin, hipStream_t s);
+// The build in two parts, for slabs: the order exponents once (dmin:
+// scratch), then target tiles [t0, t0 + ntl) into tab (lo term term_halves
+// past the hi term).
+void launch_half_exponents(const DevicePlan &dp, const double4 *pos, int N, double k, unsigned long long *dmin,
+                           int *pexp, hipStream_t s);
 void launch_half_table_range(const DevicePlan &dp, const double4 *pos, int N, double k, void *tab,
-                             size_t term_halves, bool split, int t0, int ntl, const unsigned long long *dmin,
-                             int *pexp, hipStream_t s, double near2, const int *gate = nullptr);
+                             size_t term_halves, bool split, int t0, int ntl, const int *pexp, hipStream_t s,
+                             double near2, const int *gate = nullptr);
 void launch_half_x(const DevicePlan &dp, int N, const double2 *x, void *xh, bool split, unsigned *colmax, int *xexp,
                    hipStream_t s, const int *gate = nullptr);
 void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void *xh, bool split, const int *pexp,

@@ -1027,9 +1027,9 @@ struct yasfhip_server {
             YH_CHECK(hipMemcpyAsync(o->t_d, q.t.data(), sizeof(double2) * q.n, hipMemcpyHostToDevice, st));
         }
         if (o->half)
-            launch_half_min_distance(o->pos_d, o->N, o->dmin, st);
+            launch_half_exponents(dp, o->pos_d, o->N, o->k, o->dmin, o->pexp, st);
         if (o->table32 && o->half)
-            launch_half_table_range(dp, o->pos_d, o->N, o->k, o->htab, o->hterm, o->half_split, 0, o->ntiles1, o->dmin,
+            launch_half_table_range(dp, o->pos_d, o->N, o->k, o->htab, o->hterm, o->half_split, 0, o->ntiles1,
                                     o->pexp, st, kAllPairs);
         else if (o->table32)
             launch_pair_table_build(dp, o->pos_d, o->N, o->k, true, o->table, st);
@@ -1196,8 +1196,8 @@ struct yasfhip_server {
                                       todo[j]->xexp, st);
                 for (int t0 = 0; t0 < o.ntiles1; t0 += o.hslab_tiles) {
                     const int ntl = std::min(o.hslab_tiles, o.ntiles1 - t0);
-                    launch_half_table_range(dp, o.pos_d, o.N, o.k, o.slab, o.hterm, o.half_split, t0, ntl, o.dmin,
-                                            o.pexp, st, kAllPairs);
+                    launch_half_table_range(dp, o.pos_d, o.N, o.k, o.slab, o.hterm, o.half_split, t0, ntl, o.pexp,
+                                            st, kAllPairs);
                     for (size_t j = i; j < todo.size(); ++j)
                         if (todo[j]->op == q->op && todo[j]->approx_residual)
                             launch_half_gemm(dp, q->N, o.slab, todo[j]->xh, o.half_split, o.pexp, todo[j]->xexp,
@@ -1746,8 +1746,8 @@ struct yasfhip_server {
             if (o.half) {
                 for (int t0 = 0; t0 < o.ntiles1; t0 += o.hslab_tiles) {
                     const int ntl = std::min(o.hslab_tiles, o.ntiles1 - t0);
-                    launch_half_table_range(dp, o.pos_d, o.N, o.k, o.slab, o.hterm, o.half_split, t0, ntl, o.dmin,
-                                            o.pexp, st, kAllPairs, flag);
+                    launch_half_table_range(dp, o.pos_d, o.N, o.k, o.slab, o.hterm, o.half_split, t0, ntl, o.pexp,
+                                            st, kAllPairs, flag);
                     launch_half_shared_gemm(g, o.slab, t0, ntl);
                 }
                 continue;
