@@ -187,14 +187,27 @@ __device__ __forceinline__ int block_argmax(double bv, int bi, double *red_v, in
 // publishes rows p and c, which swap; the elimination runs on the registers.
 // The row swaps outside the panel follow afterwards in one pass (each
 // thread applies the 16 swaps in order to its own columns), and piv[c]
-// records them (row c swapped with piv[c]).  (Two rows per thread, or a
-// second row in LDS, spilled to scratch and ran slower than the global form.)
+// records them (row c swapped with piv[c]).  (Two rows per thread in
+// registers spilled to scratch and ran slower than the global form.)
+// EXTRA: up to kLuExtraRows more rows, kb + NT + t for thread t, live in LDS
+// (xrow, rows kLuExtraStride double2 apart so that a wave's rows fall on
+// distinct banks) and are eliminated there four columns at a time.
+constexpr int kLuExtraRows = 256;
+constexpr int kLuExtraStride = 17;
+template <bool EXTRA>
 __device__ void panel_registers(double2 *__restrict__ A, int mp, int kb, int *__restrict__ piv, double *red_v,
-                                int *red_i, double2 *prow_sh, double2 *crow_sh) {
+                                int *red_i, double2 *prow_sh, double2 *crow_sh, double2 *xrow) {
     const int tid = threadIdx.x;
     constexpr int NT = 64 * kLuWaves;
     auto at = [&](int i, int j) -> double2 & { return A[static_cast<long>(i) * mp + j]; };
     const int r = kb + tid;
+    const int r2 = kb + NT + tid;  // this thread's LDS row (EXTRA)
+    const bool has2 = EXTRA && tid < kLuExtraRows && r2 < mp;
+    double2 *xr = EXTRA ? xrow + tid * kLuExtraStride : nullptr;
+    if (has2)
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            xr[j] = at(r2, kb + j);
     double2 pr[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j)
@@ -209,6 +222,13 @@ __device__ void panel_registers(double2 *__restrict__ A, int mp, int kb, int *__
             bv = cabs1(pr[j]);
             bi = r;
         }
+        if (has2) {  // r2 > r: it wins only on a strictly larger value
+            const double v2 = cabs1(xr[j]);
+            if (v2 > bv) {
+                bv = v2;
+                bi = r2;
+            }
+        }
         const int p = __builtin_amdgcn_readfirstlane(block_argmax(bv, bi, red_v, red_i));  // same in all threads
         pv[j] = p;
         // publish rows p (the pivot row) and c; p >= c, both in the panel
@@ -216,6 +236,11 @@ __device__ void panel_registers(double2 *__restrict__ A, int mp, int kb, int *__
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj)
                 prow_sh[jj] = pr[jj];
+        }
+        if (has2 && r2 == p) {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj)
+                prow_sh[jj] = xr[jj];
         }
         if (r == c) {
 #pragma unroll
@@ -232,6 +257,23 @@ __device__ void panel_registers(double2 *__restrict__ A, int mp, int kb, int *__
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj)
                 pr[jj] = crow_sh[jj];
+        }
+        if (has2) {  // the LDS row (below every register row): swap, then eliminate in LDS
+            if (r2 == p) {
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj)
+                    xr[jj] = crow_sh[jj];
+            }
+            const double2 l2 = cmul(xr[j], inv);
+            xr[j] = l2;
+#pragma unroll
+            for (int j4 = (j + 1) & ~3; j4 < 16; j4 += 4) {
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int jj = j4; jj < j4 + 4; ++jj)
+                    if (jj > j)
+                        xr[jj] = csub(xr[jj], cmul(l2, prow_sh[jj]));
+            }
         }
         const bool below = r > c && r < mp;
         const double2 l = below ? cmul(pr[j], inv) : make_double2(0.0, 0.0);
@@ -256,6 +298,10 @@ __device__ void panel_registers(double2 *__restrict__ A, int mp, int kb, int *__
 #pragma unroll
         for (int j = 0; j < 16; ++j)
             at(r, kb + j) = pr[j];
+    if (has2)
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            at(r2, kb + j) = xr[j];
     if (tid < 16)
         piv[kb + tid] = pv[tid];
     // deferred row swaps: columns outside the panel, each thread its own
@@ -333,240 +379,284 @@ __device__ void panel_global(double2 *__restrict__ A, int mp, int kb, int *__res
 constexpr int kLuOuter = 64;    // columns per outer panel
 constexpr int kLuChunk = 128;   // trailing columns per LDS stage of U12
 constexpr size_t kLuLds = static_cast<size_t>(kLuOuter) * kLuChunk * sizeof(double2);  // 128 KB
-__global__ __launch_bounds__(64 * kLuWaves) void lu_blocked_kernel(const int *__restrict__ box_first, int nmax,
-                                                         const long long *__restrict__ blk_off,
-                                                         const int *__restrict__ piv_off,
-                                                         double2 *__restrict__ blocks,
-                                                         int *__restrict__ pivots,
-                                                         const int *__restrict__ dinv_off,
-                                                         double2 *__restrict__ dinv) {
+constexpr size_t kLuPanelLds = static_cast<size_t>(kLuExtraRows) * kLuExtraStride * sizeof(double2);
+
+// The LU runs as three kernels per step (one block per box each), so that
+// each phase gets its own register allocation: in one kernel the phases'
+// live ranges met and spilled (196 VGPRs, and 1,165 with the panel's LDS
+// rows), while each phase alone fits.  Per 16-column block kb of an outer
+// panel K0: lu_panel_kernel (pivoting and elimination of the 16 columns),
+// then lu_block_kernel (diagonal inverses, U rows inside the outer panel,
+// rank-16 update); per outer panel: lu_trailing_kernel (U12 and the rank-64
+// update of the trailing matrix).  Boxes smaller than the step return.
+
+// The 16-column panel at kb: in registers (and LDS rows) up to
+// kLuPanelRows + kLuExtraRows rows, else in global memory.
+__global__ __launch_bounds__(64 * kLuWaves) void lu_panel_kernel(const int *__restrict__ box_first, int nmax,
+                                                       const long long *__restrict__ blk_off,
+                                                       const int *__restrict__ piv_off,
+                                                       double2 *__restrict__ blocks, int *__restrict__ pivots,
+                                                       int kb) {
     __shared__ double red_v[kLuWaves];
     __shared__ int red_i[kLuWaves];
     __shared__ double2 prow_sh[16], crow_sh[16];
+    extern __shared__ double2 xrow[];  // [kLuExtraRows][kLuExtraStride]
+    const int b = blockIdx.x;
+    const int mp = padded16((box_first[b + 1] - box_first[b]) * nmax);
+    if (kb >= mp)
+        return;
+    double2 *A = blocks + blk_off[b];
+    int *piv = pivots + piv_off[b];
+    if (mp - kb <= kLuPanelRows + kLuExtraRows)
+        panel_registers<true>(A, mp, kb, piv, red_v, red_i, prow_sh, crow_sh, xrow);
+    else
+        panel_global(A, mp, kb, piv, red_v, red_i);
+}
+
+// After the panel at kb (in the outer panel from K0): the inverses of its
+// diagonal 16x16 factors, its U rows for the columns inside the outer panel
+// and the rank-16 update of those columns below it.
+__global__ __launch_bounds__(64 * kLuWaves) void lu_block_kernel(const int *__restrict__ box_first, int nmax,
+                                                       const long long *__restrict__ blk_off,
+                                                       double2 *__restrict__ blocks,
+                                                       const int *__restrict__ dinv_off,
+                                                       double2 *__restrict__ dinv, int K0, int kb) {
     __shared__ double2 Lsh[16][16];
-    extern __shared__ double2 Ush[];  // [kLuOuter][kLuChunk]
     const int b = blockIdx.x;
     const int m = (box_first[b + 1] - box_first[b]) * nmax;
     const int mp = padded16(m);
+    if (kb >= mp)
+        return;
     double2 *A = blocks + blk_off[b];
-    int *piv = pivots + piv_off[b];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lo = lane & 15, hi = lane >> 4;
     auto at = [&](int i, int j) -> double2 & { return A[static_cast<long>(i) * mp + j]; };
     double2 *Dbox = dinv + static_cast<long>(dinv_off[b]) * 512;
+    const int pend = min(K0 + kLuOuter, mp);  // outer panel columns [K0, pend)
 
-    for (int K0 = 0; K0 < mp; K0 += kLuOuter) {
-        const int pend = min(K0 + kLuOuter, mp);  // outer panel columns [K0, pend)
-        for (int kb = K0; kb < pend; kb += 16) {
-            if (mp - kb <= kLuPanelRows)
-                panel_registers(A, mp, kb, piv, red_v, red_i, prow_sh, crow_sh);
-            else
-                panel_global(A, mp, kb, piv, red_v, red_i);
-            // Inverses of the diagonal 16x16 factors, kept for the
-            // substitutions: thread c < 16 solves L11 y = e_c (unit lower),
-            // thread 16 + c solves U11 z = e_c (upper).
-            double2 *Dinv = Dbox + static_cast<long>(kb / 16) * 512;
-            // (compile-time indices throughout, so yv / zv stay in registers;
-            // the terms outside a column's range are multiplied by zeros)
-            if (tid < 16) {
-                const int c = tid;
-                double2 yv[16];
+    // Inverses of the diagonal 16x16 factors, kept for the
+    // substitutions: thread c < 16 solves L11 y = e_c (unit lower),
+    // thread 16 + c solves U11 z = e_c (upper).
+    double2 *Dinv = Dbox + static_cast<long>(kb / 16) * 512;
+    // (compile-time indices throughout, so yv / zv stay in registers;
+    // the terms outside a column's range are multiplied by zeros)
+    if (tid < 16) {
+        const int c = tid;
+        double2 yv[16];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+        for (int r = 0; r < 16; ++r) {
+            double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
 #pragma unroll
-                    for (int q = 0; q < r; ++q)
-                        v = csub(v, cmul(at(kb + r, kb + q), yv[q]));
-                    yv[r] = r < c ? make_double2(0.0, 0.0) : v;
-                    Lsh[r][c] = yv[r];
-                    Dinv[r * 16 + c] = yv[r];
-                }
-            } else if (tid < 32) {
-                const int c = tid - 16;
-                double2 zv[16];
+            for (int q = 0; q < r; ++q)
+                v = csub(v, cmul(at(kb + r, kb + q), yv[q]));
+            yv[r] = r < c ? make_double2(0.0, 0.0) : v;
+            Lsh[r][c] = yv[r];
+            Dinv[r * 16 + c] = yv[r];
+        }
+    } else if (tid < 32) {
+        const int c = tid - 16;
+        double2 zv[16];
 #pragma unroll
-                for (int r = 15; r >= 0; --r) {
-                    double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+        for (int r = 15; r >= 0; --r) {
+            double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
 #pragma unroll
-                    for (int q = r + 1; q < 16; ++q)
-                        v = csub(v, cmul(at(kb + r, kb + q), zv[q]));
-                    zv[r] = r > c ? make_double2(0.0, 0.0) : cmul(v, cinv(at(kb + r, kb + r)));
-                    Dinv[256 + r * 16 + c] = zv[r];
-                }
+            for (int q = r + 1; q < 16; ++q)
+                v = csub(v, cmul(at(kb + r, kb + q), zv[q]));
+            zv[r] = r > c ? make_double2(0.0, 0.0) : cmul(v, cinv(at(kb + r, kb + r)));
+            Dinv[256 + r * 16 + c] = zv[r];
+        }
+    }
+    __syncthreads();
+    const int ntp = (pend - kb - 16) >> 4;  // column tiles right of the block, inside the panel
+    if (ntp == 0)
+        return;
+    // U = Linv A for the block's 16 rows (each wave owns whole column
+    // tiles, so reading and overwriting a tile needs no barrier).
+    for (int cb = wave; cb < ntp; cb += kLuWaves) {
+        const int col0 = kb + 16 + cb * 16;
+        double2 bv[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+            bv[kk] = at(kb + 4 * kk + hi, col0 + lo);
+        v4d cre = {0.0, 0.0, 0.0, 0.0}, cim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const double2 a = Lsh[lo][4 * kk + hi];
+            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bv[kk].x, cre, 0, 0, 0);
+            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bv[kk].y, cre, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bv[kk].y, cim, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bv[kk].x, cim, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            at(kb + 4 * i + hi, col0 + lo) = make_double2(cre[i], cim[i]);
+    }
+    __syncthreads();
+    // rank-16 update of the panel's remaining columns, all rows below
+    const int nrb = (mp - kb - 16) >> 4;
+    for (int rb = wave; rb < nrb; rb += kLuWaves) {
+        const int row0 = kb + 16 + rb * 16;
+        double ar[4], ai[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const double2 v = at(row0 + lo, kb + 4 * kk + hi);
+            ar[kk] = v.x;
+            ai[kk] = v.y;
+        }
+        for (int cb = 0; cb < ntp; ++cb) {
+            const int col0 = kb + 16 + cb * 16;
+            v4d cre, cim;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double2 v = at(row0 + 4 * i + hi, col0 + lo);
+                cre[i] = v.x;
+                cim[i] = v.y;
             }
-            __syncthreads();
-            const int ntp = (pend - kb - 16) >> 4;  // column tiles right of the block, inside the panel
-            if (ntp == 0)
-                continue;
-            // U = Linv A for the block's 16 rows (each wave owns whole column
-            // tiles, so reading and overwriting a tile needs no barrier).
-            for (int cb = wave; cb < ntp; cb += kLuWaves) {
-                const int col0 = kb + 16 + cb * 16;
-                double2 bv[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const double2 bv2 = at(kb + 4 * kk + hi, col0 + lo);
+                cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.x, cre, 0, 0, 0);
+                cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[kk], bv2.y, cre, 0, 0, 0);
+                cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.y, cim, 0, 0, 0);
+                cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[kk], bv2.x, cim, 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                at(row0 + 4 * i + hi, col0 + lo) = make_double2(cre[i], cim[i]);
+        }
+    }
+
+}
+
+// After the outer panel [K0, K0 + kLuOuter): U12 = L11^-1 A12 and
+// A22 -= L21 U12 for the trailing matrix.
+__global__ __launch_bounds__(64 * kLuWaves) void lu_trailing_kernel(const int *__restrict__ box_first, int nmax,
+                                                          const long long *__restrict__ blk_off,
+                                                          double2 *__restrict__ blocks,
+                                                          const int *__restrict__ dinv_off,
+                                                          const double2 *__restrict__ dinv, int K0) {
+    extern __shared__ double2 Ush[];  // [kLuOuter][kLuChunk]
+    const int b = blockIdx.x;
+    const int m = (box_first[b + 1] - box_first[b]) * nmax;
+    const int mp = padded16(m);
+    const int pend = min(K0 + kLuOuter, mp);
+    if (pend >= mp)
+        return;  // no trailing matrix (also K0 >= mp)
+    double2 *A = blocks + blk_off[b];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lo = lane & 15, hi = lane >> 4;
+    auto at = [&](int i, int j) -> double2 & { return A[static_cast<long>(i) * mp + j]; };
+    const double2 *Dbox = dinv + static_cast<long>(dinv_off[b]) * 512;
+
+    const int nbi = (pend - K0) >> 4;  // 16-row blocks of the outer panel (<= 4)
+    // U12 = L11^-1 A12 for the columns right of the panel: per column
+    // tile, block forward substitution t_i = Linv_i (A12_i - sum_{j<i}
+    // L_ij t_j), the tiles held in registers (element kk of a tile is
+    // row 4 kk + hi: also the B operand of K-substep kk).
+    const int ntc = (mp - pend) >> 4;
+    for (int cb = wave; cb < ntc; cb += kLuWaves) {
+        const int col0 = pend + cb * 16;
+        double2 tv[kLuOuter / 16][4];
+#pragma unroll
+        for (int i = 0; i < kLuOuter / 16; ++i)
+            if (i < nbi)
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk)
-                    bv[kk] = at(kb + 4 * kk + hi, col0 + lo);
-                v4d cre = {0.0, 0.0, 0.0, 0.0}, cim = {0.0, 0.0, 0.0, 0.0};
+                    tv[i][kk] = at(K0 + 16 * i + 4 * kk + hi, col0 + lo);
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const double2 a = Lsh[lo][4 * kk + hi];
-                    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bv[kk].x, cre, 0, 0, 0);
-                    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bv[kk].y, cre, 0, 0, 0);
-                    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bv[kk].y, cim, 0, 0, 0);
-                    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bv[kk].x, cim, 0, 0, 0);
-                }
+        for (int i = 0; i < kLuOuter / 16; ++i) {
+            if (i >= nbi)
+                break;
+            v4d cre, cim;
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    at(kb + 4 * i + hi, col0 + lo) = make_double2(cre[i], cim[i]);
+            for (int kk = 0; kk < 4; ++kk) {
+                cre[kk] = tv[i][kk].x;
+                cim[kk] = tv[i][kk].y;
             }
-            __syncthreads();
-            // rank-16 update of the panel's remaining columns, all rows below
-            const int nrb = (mp - kb - 16) >> 4;
-            for (int rb = wave; rb < nrb; rb += kLuWaves) {
-                const int row0 = kb + 16 + rb * 16;
-                double ar[4], ai[4];
+#pragma unroll
+            for (int j = 0; j < i; ++j)
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
-                    const double2 v = at(row0 + lo, kb + 4 * kk + hi);
+                    const double2 a = at(K0 + 16 * i + lo, K0 + 16 * j + 4 * kk + hi);
+                    const double2 bt = tv[j][kk];
+                    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.x, bt.x, cre, 0, 0, 0);
+                    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bt.y, cre, 0, 0, 0);
+                    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.x, bt.y, cim, 0, 0, 0);
+                    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bt.x, cim, 0, 0, 0);
+                }
+            const double2 *Li = Dbox + static_cast<long>(K0 / 16 + i) * 512;  // unit-lower inverse
+            v4d ure = {0.0, 0.0, 0.0, 0.0}, uim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const double2 a = Li[lo * 16 + 4 * kk + hi];
+                ure = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, cre[kk], ure, 0, 0, 0);
+                ure = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, cim[kk], ure, 0, 0, 0);
+                uim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, cim[kk], uim, 0, 0, 0);
+                uim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, cre[kk], uim, 0, 0, 0);
+            }
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                tv[i][kk] = make_double2(ure[kk], uim[kk]);
+        }
+#pragma unroll
+        for (int i = 0; i < kLuOuter / 16; ++i)
+            if (i < nbi)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+                    at(K0 + 16 * i + 4 * kk + hi, col0 + lo) = tv[i][kk];
+    }
+    __syncthreads();
+    // A22 -= L21 U12 (K = pend - K0), U12 staged in LDS kLuChunk columns at
+    // a time; each wave holds the L21 fragments of one row block.
+    const int nks = (pend - K0) >> 2;  // K-substeps of 4
+    const int nrb = (mp - pend) >> 4;
+    for (int c0 = pend; c0 < mp; c0 += kLuChunk) {
+        const int cw = min(kLuChunk, mp - c0);
+        for (int e = tid; e < (pend - K0) * cw; e += blockDim.x) {
+            const int r = e / cw, c = e - r * cw;
+            Ush[r * kLuChunk + c] = at(K0 + r, c0 + c);
+        }
+        __syncthreads();
+        const int nct = cw >> 4;
+        for (int rb = wave; rb < nrb; rb += kLuWaves) {
+            const int row0 = pend + rb * 16;
+            // L21 fragments (ar, ai) and their sums for the three-MFMA
+            // complex product (Gauss): P1 = Ar Br, P2 = Ai Bi, P3 =
+            // (Ar + Ai)(Br + Bi) from zero, so the MFMAs need not wait for
+            // the C tile, which is loaded first and combined at the end:
+            // C -= (P1 - P2) + i (P3 - P1 - P2)
+            double ar[kLuOuter / 4], ai[kLuOuter / 4];
+#pragma unroll
+            for (int kk = 0; kk < kLuOuter / 4; ++kk)
+                if (kk < nks) {
+                    const double2 v = at(row0 + lo, K0 + 4 * kk + hi);
                     ar[kk] = v.x;
                     ai[kk] = v.y;
                 }
-                for (int cb = 0; cb < ntp; ++cb) {
-                    const int col0 = kb + 16 + cb * 16;
-                    v4d cre, cim;
+            for (int ct = 0; ct < nct; ++ct) {
+                const int col0 = c0 + ct * 16;
+                double2 cv[4];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const double2 v = at(row0 + 4 * i + hi, col0 + lo);
-                        cre[i] = v.x;
-                        cim[i] = v.y;
-                    }
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) {
-                        const double2 bv2 = at(kb + 4 * kk + hi, col0 + lo);
-                        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.x, cre, 0, 0, 0);
-                        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[kk], bv2.y, cre, 0, 0, 0);
-                        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[kk], bv2.y, cim, 0, 0, 0);
-                        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[kk], bv2.x, cim, 0, 0, 0);
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        at(row0 + 4 * i + hi, col0 + lo) = make_double2(cre[i], cim[i]);
-                }
-            }
-            __syncthreads();
-        }
-        if (pend >= mp)
-            break;
-        const int nbi = (pend - K0) >> 4;  // 16-row blocks of the outer panel (<= 4)
-        // U12 = L11^-1 A12 for the columns right of the panel: per column
-        // tile, block forward substitution t_i = Linv_i (A12_i - sum_{j<i}
-        // L_ij t_j), the tiles held in registers (element kk of a tile is
-        // row 4 kk + hi: also the B operand of K-substep kk).
-        const int ntc = (mp - pend) >> 4;
-        for (int cb = wave; cb < ntc; cb += kLuWaves) {
-            const int col0 = pend + cb * 16;
-            double2 tv[kLuOuter / 16][4];
-#pragma unroll
-            for (int i = 0; i < kLuOuter / 16; ++i)
-                if (i < nbi)
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk)
-                        tv[i][kk] = at(K0 + 16 * i + 4 * kk + hi, col0 + lo);
-#pragma unroll
-            for (int i = 0; i < kLuOuter / 16; ++i) {
-                if (i >= nbi)
-                    break;
-                v4d cre, cim;
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    cre[kk] = tv[i][kk].x;
-                    cim[kk] = tv[i][kk].y;
-                }
-#pragma unroll
-                for (int j = 0; j < i; ++j)
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) {
-                        const double2 a = at(K0 + 16 * i + lo, K0 + 16 * j + 4 * kk + hi);
-                        const double2 bt = tv[j][kk];
-                        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.x, bt.x, cre, 0, 0, 0);
-                        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bt.y, cre, 0, 0, 0);
-                        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.x, bt.y, cim, 0, 0, 0);
-                        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bt.x, cim, 0, 0, 0);
-                    }
-                const double2 *Li = Dbox + static_cast<long>(K0 / 16 + i) * 512;  // unit-lower inverse
-                v4d ure = {0.0, 0.0, 0.0, 0.0}, uim = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const double2 a = Li[lo * 16 + 4 * kk + hi];
-                    ure = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, cre[kk], ure, 0, 0, 0);
-                    ure = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, cim[kk], ure, 0, 0, 0);
-                    uim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, cim[kk], uim, 0, 0, 0);
-                    uim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, cre[kk], uim, 0, 0, 0);
-                }
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk)
-                    tv[i][kk] = make_double2(ure[kk], uim[kk]);
-            }
-#pragma unroll
-            for (int i = 0; i < kLuOuter / 16; ++i)
-                if (i < nbi)
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk)
-                        at(K0 + 16 * i + 4 * kk + hi, col0 + lo) = tv[i][kk];
-        }
-        __syncthreads();
-        // A22 -= L21 U12 (K = pend - K0), U12 staged in LDS kLuChunk columns at
-        // a time; each wave holds the L21 fragments of one row block.
-        const int nks = (pend - K0) >> 2;  // K-substeps of 4
-        const int nrb = (mp - pend) >> 4;
-        for (int c0 = pend; c0 < mp; c0 += kLuChunk) {
-            const int cw = min(kLuChunk, mp - c0);
-            for (int e = tid; e < (pend - K0) * cw; e += blockDim.x) {
-                const int r = e / cw, c = e - r * cw;
-                Ush[r * kLuChunk + c] = at(K0 + r, c0 + c);
-            }
-            __syncthreads();
-            const int nct = cw >> 4;
-            for (int rb = wave; rb < nrb; rb += kLuWaves) {
-                const int row0 = pend + rb * 16;
-                // L21 fragments (ar, ai) and their sums for the three-MFMA
-                // complex product (Gauss): P1 = Ar Br, P2 = Ai Bi, P3 =
-                // (Ar + Ai)(Br + Bi) from zero, so the MFMAs need not wait for
-                // the C tile, which is loaded first and combined at the end:
-                // C -= (P1 - P2) + i (P3 - P1 - P2)
-                double ar[kLuOuter / 4], ai[kLuOuter / 4];
-#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    cv[i] = at(row0 + 4 * i + hi, col0 + lo);
+                v4d p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
+                // (the Gauss sums Ar + Ai are formed per MFMA: holding them
+                // for all K-substeps pushed this loop past the registers)
+#pragma unroll 4
                 for (int kk = 0; kk < kLuOuter / 4; ++kk)
                     if (kk < nks) {
-                        const double2 v = at(row0 + lo, K0 + 4 * kk + hi);
-                        ar[kk] = v.x;
-                        ai[kk] = v.y;
+                        const double2 bv2 = Ush[(4 * kk + hi) * kLuChunk + ct * 16 + lo];
+                        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[kk], bv2.x, p1, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[kk], bv2.y, p2, 0, 0, 0);
+                        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[kk] + ai[kk], bv2.x + bv2.y, p3, 0, 0, 0);
                     }
-                for (int ct = 0; ct < nct; ++ct) {
-                    const int col0 = c0 + ct * 16;
-                    double2 cv[4];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        cv[i] = at(row0 + 4 * i + hi, col0 + lo);
-                    v4d p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0}, p3 = {0.0, 0.0, 0.0, 0.0};
-                    // (the Gauss sums Ar + Ai are formed per MFMA: holding them
-                    // for all K-substeps pushed this loop past the registers)
-#pragma unroll 4
-                    for (int kk = 0; kk < kLuOuter / 4; ++kk)
-                        if (kk < nks) {
-                            const double2 bv2 = Ush[(4 * kk + hi) * kLuChunk + ct * 16 + lo];
-                            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[kk], bv2.x, p1, 0, 0, 0);
-                            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[kk], bv2.y, p2, 0, 0, 0);
-                            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[kk] + ai[kk], bv2.x + bv2.y, p3, 0, 0, 0);
-                        }
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        at(row0 + 4 * i + hi, col0 + lo) =
-                            make_double2(cv[i].x - (p1[i] - p2[i]), cv[i].y - (p3[i] - p1[i] - p2[i]));
-                }
+                for (int i = 0; i < 4; ++i)
+                    at(row0 + 4 * i + hi, col0 + lo) =
+                        make_double2(cv[i].x - (p1[i] - p2[i]), cv[i].y - (p3[i] - p1[i] - p2[i]));
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
 }
 
@@ -905,8 +995,11 @@ BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &
         YH_CHECK(hipGetLastError());
         static bool lu_lds_opt_in = false;  // gfx950: up to 160 KB of LDS per workgroup
         if (!lu_lds_opt_in) {  // set here, at the build, so that later applies can be graph-captured
-            YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&lu_blocked_kernel),
+            YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&lu_trailing_kernel),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLuLds)));
+            YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&lu_panel_kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         static_cast<int>(kLuPanelLds)));
             const int dyn = static_cast<int>(kMaxBlock * sizeof(double2));
             YH_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&lu_apply_kernel<float2>),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
@@ -914,8 +1007,17 @@ BlockPrecond *precond_create(const CouplingPlan &plan, const std::vector<cplx> &
                                          hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
             lu_lds_opt_in = true;
         }
-        hipLaunchKernelGGL(lu_blocked_kernel, dim3(bp->nbox), dim3(64 * kLuWaves), kLuLds, s, bp->box_first, nmax,
-                           bp->blk_off, bp->piv_off, bp->blocks, bp->pivots, bp->dinv_off, bp->dinv);
+        const int mpmax = bp->max_m;  // padded to 16 rows
+        for (int K0 = 0; K0 < mpmax; K0 += kLuOuter) {
+            for (int kb = K0; kb < std::min(K0 + kLuOuter, mpmax); kb += 16) {
+                hipLaunchKernelGGL(lu_panel_kernel, dim3(bp->nbox), dim3(64 * kLuWaves), kLuPanelLds, s,
+                                   bp->box_first, nmax, bp->blk_off, bp->piv_off, bp->blocks, bp->pivots, kb);
+                hipLaunchKernelGGL(lu_block_kernel, dim3(bp->nbox), dim3(64 * kLuWaves), 0, s, bp->box_first, nmax,
+                                   bp->blk_off, bp->blocks, bp->dinv_off, bp->dinv, K0, kb);
+            }
+            hipLaunchKernelGGL(lu_trailing_kernel, dim3(bp->nbox), dim3(64 * kLuWaves), kLuLds, s, bp->box_first,
+                               nmax, bp->blk_off, bp->blocks, bp->dinv_off, bp->dinv, K0);
+        }
         YH_CHECK(hipGetLastError());
         hipLaunchKernelGGL(pivots_to_permutation_kernel, dim3(bp->nbox), dim3(256), bp->max_m * sizeof(int), s,
                            bp->box_first, nmax, bp->piv_off, bp->pivots);
