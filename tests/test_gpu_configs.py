@@ -11,7 +11,8 @@ sections to 1e-6 of a 1e-10 FP64 solve.  Workloads:
     solve (FP64 refinement) and its cross sections;
   * the 512-request serving stream, seed 4: its largest requests served;
   * a 1,200-monomer fractal aggregate at lmax 4: 16 incidence directions
-    sharing one operator in slab form, equal to unshared solves.
+    sharing one operator in slab form, equal to unshared solves, with the
+    complex64 and with the two-term fp16 inner operator.
 """
 
 import copy
@@ -166,6 +167,37 @@ def test_aggregate_16_directions_shared_slab_operator():
         a, u = shared.results[r.index], unshared.results[r.index]
         assert a["info"] == 0 and np.array_equal(a["x"], u["x"])
         assert a["iterations"] == u["iterations"] and a["c_ext"] == u["c_ext"] and a["c_sca"] == u["c_sca"]
+        assert _rel_residual(s, r.t, r.a_inc, a["x"]) <= 1e-6
+        ce, cs = s.cross_sections(r.a_inc, a["x"])
+        assert a["c_ext"] == pytest.approx(ce, rel=1e-10) and a["c_sca"] == pytest.approx(cs, rel=1e-10)
+    s.close()
+
+
+def test_aggregate_16_directions_fp16x2_operator_groups():
+    # the same aggregate with the two-term fp16 inner operator: the operator
+    # keeps fp16 slabs (4 FP64 tiles: 7 fp16 target tiles of 75), and the 16
+    # requests run their fp16 GEMMs two per block over each slab; unshared
+    # solves (one per block) give the same bits, and every solution meets the
+    # FP64 criterion with cross sections matching the FP64 operator's
+    reqs = make_aggregate_workload([1200], [500.0], 16, seed=5)
+    runs = []
+    for keyed in (True, False):
+        batch = reqs if keyed else [copy.copy(r) for r in reqs]
+        if not keyed:
+            for r in batch:
+                r.operator_key = 0
+        srv = _native.Server(restart=50, inner_rtol=1e-6)
+        srv.set_inner_precision(_native.FP16X2)
+        srv.set_table_policy(_native.Server.TABLES_SLABS, slab_tiles=4)
+        runs.append(serve(srv, batch, keep_solutions=True))
+        srv.close()
+    shared, unshared = runs
+    r0 = reqs[0]
+    s = _native.Session(r0.positions, r0.lmax, [r0.k])
+    s.set_tmatrix(r0.t[np.newaxis, :])
+    for r in reqs:
+        a, u = shared.results[r.index], unshared.results[r.index]
+        assert a["info"] == 0 and np.array_equal(a["x"], u["x"]) and a["iterations"] == u["iterations"]
         assert _rel_residual(s, r.t, r.a_inc, a["x"]) <= 1e-6
         ce, cs = s.cross_sections(r.a_inc, a["x"])
         assert a["c_ext"] == pytest.approx(ce, rel=1e-10) and a["c_sca"] == pytest.approx(cs, rel=1e-10)
