@@ -816,8 +816,9 @@ def main():
     ap.add_argument("--particles", type=int, default=0, help="aggregates: fixed N instead of the seeded draw")
     ap.add_argument("--wavelengths", type=int, default=2, help="aggregates: wavelengths sampled of 64")
     ap.add_argument("--directions", type=int, default=16, help="aggregates: incidence directions")
-    ap.add_argument("--inner-precision", choices=["fp32", "fp16x2", "fp16"], default="fp32",
-                    help="aggregates: pair GEMM of the inner GMRES iterations (complex64, or fp16 terms)")
+    ap.add_argument("--inner-precision", choices=["fp32", "fp16x2", "fp16"], default="fp16x2",
+                    help="aggregates: pair GEMM of the inner GMRES iterations: two fp16 terms (complex64 "
+                         "accuracy, the default), complex64, or one fp16 term")
     ap.add_argument("--fp64-check", type=int, default=1,
                     help="aggregates: operators whose first request is re-solved in FP64 (0: none)")
     args = ap.parse_args()
