@@ -135,10 +135,11 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // padded mp x mp block, with two levels of blocking:
 // * outer panels of kLuOuter columns; inside one, 16-column blocks: unblocked
 //   factorisation of the 16-column panel (pivot search, rank-1 updates within
-//   the 16 columns; in registers, one row per thread, when the panel has at
-//   most kLuPanelRows rows), the row swaps outside the panel in one deferred
-//   pass, the 16x16 diagonal inverses, then the 16-row U = L^-1 A and the
-//   rank-16 update restricted to the outer panel's columns;
+//   the 16 columns; in registers, one row per thread, with up to
+//   kLuExtraRows more rows in LDS, else in global memory), the row swaps
+//   outside the panel in one deferred pass, the 16x16 diagonal inverses,
+//   then the 16-row U = L^-1 A and the rank-16 update restricted to the
+//   outer panel's columns;
 // * then, for the columns right of the outer panel, U12 = L11^-1 A12 by block
 //   forward substitution (16-row blocks, stored inverses) and one rank-64
 //   update A22 -= L21 U12, with U12 staged in LDS kLuChunk columns at a time.
@@ -146,7 +147,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // once per 16 columns.  Complex products are four real FP64 MFMAs
 // (16x16x4); the C tile layout of the MFMA (rows 4 i + (lane >> 4), column
 // lane & 15) is also its B operand layout, so solved tiles feed the next
-// products from registers.
+// products from registers.  The phases run as separate kernels (below).
 constexpr int kLuWaves = 8;     // waves per box
 // Rows of a 16-column panel the register form holds: one per thread.
 constexpr int kLuPanelRows = 64 * kLuWaves;
