@@ -351,6 +351,45 @@ def test_half_shared_and_slab_operators(aggregate_requests, precision):
             assert b["c_sca"] == pytest.approx(a["c_sca"], rel=1e-12)
 
 
+@pytest.mark.parametrize("precision", [_native.FP16X2, _native.FP16], ids=["fp16x2", "fp16"])
+def test_half_operator_groups_across_orders(precision):
+    # five requests per operator at lmax 3 (two column tiles per job: four
+    # requests per fp16 GEMM block, then a group of one) and at lmax 6 (six
+    # tiles: one request per block), each with its own incident field; the
+    # groups give the unshared solutions bit for bit
+    import copy
+    from yasf_amd.coefficients import planewave_coefficients
+    rng = np.random.default_rng(21)
+    reqs = []
+    for key, lmax in ((31, 3), (32, 6)):
+        base = make_request(rng, 0, n_range=(60, 70), lmax_range=(lmax, lmax))
+        for d in range(5):
+            r = copy.copy(base)
+            r.index = 100 * key + d
+            r.operator_key = key
+            beta, alpha = float(np.arccos(rng.uniform(-1, 1))), float(rng.uniform(0, 2 * np.pi))
+            k = np.array([r.k])
+            r.a_inc = np.ascontiguousarray(
+                planewave_coefficients(lmax, r.positions, k, beta, alpha, 1)[:, :, 0]).ravel()
+            reqs.append(r)
+    plain = [copy.copy(r) for r in reqs]
+    for r in plain:
+        r.operator_key = 0
+    runs = []
+    for batch in (reqs, plain):
+        srv = _native.Server(memory_bytes=2 << 30, restart=50, inner_rtol=1e-4)
+        srv.set_inner_precision(precision)
+        runs.append(serve(srv, batch, keep_solutions=True))
+        srv.close()
+    shared, unshared = runs
+    for r in reqs:
+        a, b = shared.results[r.index], unshared.results[r.index]
+        assert a["info"] == 0 and np.array_equal(a["x"], b["x"]) and a["iterations"] == b["iterations"]
+        assert a["c_ext"] == b["c_ext"] and a["c_sca"] == b["c_sca"]
+    for key in (31, 32):
+        _fp64_check(reqs[0] if key == 31 else reqs[5], shared.results[100 * key])
+
+
 def test_inner_rtol_setter(mixed_requests, served):
     # a smaller reduction per restart cycle with the fp16x2 inner operator:
     # fewer cycles (FP64 residuals) in total, and every solution still meets
