@@ -389,7 +389,8 @@ int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, in
  * the FP64 residual and C_sca), else a slab of the table (a range of target
  * tiles, at most an eighth of the arena) rebuilt range by range in every
  * pair GEMM, once for all requests of the operator; 2: slabs only; 0: no
- * tables or slabs, every GEMM generates the pair factors on the fly.  The
+ * tables or slabs, every GEMM generates the pair factors on the fly (FP64
+ * residuals use the residual pool when one is reserved, see below).  The
  * complex64 GEMMs of requests sharing an operator run as one launch whose
  * blocks interleave the requests, so each table slice is streamed once for
  * all of them.
@@ -422,6 +423,22 @@ int yasfhip_server_set_scattering_method(yasfhip_server *s, int method);
 /* Fixes the tile shape used for lmax, both size classes (one of the
  * candidates above). */
 int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int column_tiles);
+
+/* Residual pool: one FP64 slab, carved from the arena, that the FP64
+ * residuals of operators holding neither an FP64 table nor a slab of their
+ * own use in turn (residuals run one after another on the server's stream),
+ * under every table policy, where it holds at least min(16, target tiles) of
+ * the operator's FP64 tiles; else those residuals generate their pair factors
+ * on the fly.  Servers with an arena of 32 GiB or more reserve
+ * memory_bytes / 32 at creation.  bytes = 0 releases the pool; a pool that
+ * does not fit the arena's free memory is an error.  Needs an idle server.
+ * No counterpart in the reference. */
+int yasfhip_server_set_residual_pool(yasfhip_server *s, size_t bytes);
+
+/* FP64 residual pair GEMMs run since creation, by form: forms[0] resident
+ * FP64 table, [1] the operator's own slab, [2] the residual pool, [3] on the
+ * fly. */
+int yasfhip_server_residual_forms(yasfhip_server *s, long long forms[4]);
 
 #ifdef __cplusplus
 }

@@ -5,10 +5,11 @@ FP64 operator (itself pinned to the oracle at 1e-12): FP64 relative residual
 <= 1e-6, the reported residual and cross sections consistent with that
 operator, and C_ext / C_sca within 1e-6 of a tight (1e-10) all-FP64 solve.
 Batch composition must not change any result: runs with and without the HIP
-graph agree bit for bit, and a small arena (forcing queueing and the on-the-fly
-FP64 residual) agrees to rounding.  Requests that share an operator by key give
-the same solutions bit for bit as unshared ones; without pair tables (on-the-fly
-complex64 GEMMs) the solutions still meet the FP64 criterion."""
+graph agree bit for bit, and a small arena (forcing queueing and FP64
+residuals from slabs or on the fly) agrees to rounding, as does the residual
+pool's slab.  Requests that share an operator by key give the same solutions
+bit for bit as unshared ones; without pair tables (on-the-fly complex64 GEMMs)
+the solutions still meet the FP64 criterion."""
 
 import numpy as np
 import pytest
@@ -68,6 +69,8 @@ def test_graph_eager_and_small_arena_agree(mixed_requests, served):
     # an arena that holds about two of these requests at a time
     small = _native.Server(memory_bytes=160 << 20, restart=50, inner_rtol=1e-4)
     rep_small = serve(small, mixed_requests, keep_solutions=True)
+    forms = small.residual_forms()
+    assert forms["slab"] + forms["on_the_fly"] > 0
     small.close()
     for r in mixed_requests:
         a = served.results[r.index]
@@ -75,13 +78,44 @@ def test_graph_eager_and_small_arena_agree(mixed_requests, served):
         assert e["info"] == 0 and np.array_equal(a["x"], e["x"])
         assert a["iterations"] == e["iterations"] and a["cycles"] == e["cycles"]
         assert a["c_ext"] == e["c_ext"] and a["c_sca"] == e["c_sca"]
-        # in the small arena the FP64 residual is computed without the
-        # resident FP64 table (on-the-fly pair factors): same sums, agreement
-        # to rounding
+        # in the small arena some FP64 residuals are computed without the
+        # resident FP64 table (from FP64 slabs, or on the fly): same sums,
+        # agreement to rounding
         sm = rep_small.results[r.index]
         assert sm["info"] == 0
         assert np.linalg.norm(sm["x"] - a["x"]) <= 1e-12 * np.linalg.norm(a["x"])
         assert abs(sm["c_ext"] - a["c_ext"]) <= 1e-12 * abs(a["c_ext"])
+
+
+def test_residual_pool_agrees(mixed_requests):
+    # no per-operator tables or slabs (on-the-fly complex64 GEMMs) with and
+    # without a residual pool of 128 MB (10 FP64 tiles of the largest request,
+    # 160 spheres at lmax 8): with it every FP64 residual builds the pool's
+    # slab in turn, without it generates the pair factors on the fly; same
+    # sums, agreement to rounding
+    reps, forms = [], []
+    for pool in (0, 128 << 20):
+        srv = _native.Server(memory_bytes=2 << 30, restart=50, inner_rtol=1e-4)
+        srv.set_table_policy(_native.Server.TABLES_ON_THE_FLY)
+        if pool:
+            with pytest.raises(ValueError):
+                srv.set_residual_pool(4 << 30)  # beyond the arena
+            srv.set_residual_pool(pool)
+        reps.append(serve(srv, mixed_requests, keep_solutions=True))
+        forms.append(srv.residual_forms())
+        if pool:
+            r0 = mixed_requests[0]
+            srv.submit(r0.positions, r0.lmax, r0.k, r0.t, r0.a_inc)
+            with pytest.raises(ValueError):
+                srv.set_residual_pool(0)  # needs an idle server
+        srv.close()
+    assert forms[0]["on_the_fly"] > 0 and forms[0]["on_the_fly"] == sum(forms[0].values())
+    assert forms[1]["pool"] == forms[0]["on_the_fly"] and forms[1]["on_the_fly"] == 0
+    for r in mixed_requests:
+        a, p = reps[0].results[r.index], reps[1].results[r.index]
+        assert a["info"] == 0 and p["info"] == 0
+        assert np.linalg.norm(p["x"] - a["x"]) <= 1e-12 * np.linalg.norm(a["x"])
+        assert abs(p["c_ext"] - a["c_ext"]) <= 1e-12 * abs(a["c_ext"])
 
 
 def test_sequential_reference_agrees(mixed_requests, served):

@@ -124,6 +124,8 @@ def _load():
             "yasfhip_set_inner_precision": ([P, i], i),
             "yasfhip_server_set_inner_precision": ([P, i], i),
             "yasfhip_server_set_scattering_method": ([P, i], i),
+            "yasfhip_server_set_residual_pool": ([P, ctypes.c_size_t], i),
+            "yasfhip_server_residual_forms": ([P, llp], i),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)
@@ -152,7 +154,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_planewave_coefficients", "yasfhip_server_submit_spheres", "yasfhip_set_table_shape",
         "yasfhip_set_precond_factor_precision", "yasfhip_coupling_matvec_noalloc",
         "yasfhip_set_scattering_method", "yasfhip_server_set_scattering_method", "yasfhip_set_inner_precision",
-        "yasfhip_server_set_inner_precision",
+        "yasfhip_server_set_inner_precision", "yasfhip_server_set_residual_pool", "yasfhip_server_residual_forms",
     ]
 
 
@@ -562,3 +564,15 @@ class Server:
         slabs; TABLES_SLABS: slabs only; TABLES_ON_THE_FLY: neither.
         slab_tiles > 0 fixes the FP64 target tiles per slab."""
         _check(self._lib.yasfhip_server_set_table_policy(self._h, int(policy), int(slab_tiles)))
+
+    def set_residual_pool(self, nbytes: int):
+        """Reserves an FP64 slab of nbytes from the arena for the FP64
+        residuals of operators without an FP64 table or slab of their own (0
+        releases it).  Arenas of 32 GiB or more start with memory_bytes / 32."""
+        _check(self._lib.yasfhip_server_set_residual_pool(self._h, int(nbytes)))
+
+    def residual_forms(self) -> dict:
+        """FP64 residual pair GEMMs run so far, by form."""
+        forms = (ctypes.c_longlong * 4)()
+        _check(self._lib.yasfhip_server_residual_forms(self._h, forms))
+        return dict(zip(("table", "slab", "pool", "on_the_fly"), (int(v) for v in forms)))

@@ -796,6 +796,14 @@ struct yasfhip_server {
     int cap_scal = 0;
     double *partials = nullptr;
     unsigned *counter = nullptr;
+    // FP64 slab shared by the residuals of operators that hold neither an
+    // FP64 table nor a slab of their own (refreshes run one after another on
+    // the stream, so one buffer serves them all); carved from large arenas
+    void *res_pool = nullptr;
+    size_t res_pool_bytes = 0;
+    // FP64 residual pair GEMMs run so far: resident table, own slab, residual
+    // pool, on the fly
+    long long residual_forms[4] = {};
     hipEvent_t ev[2] = {nullptr, nullptr};
 
     ~yasfhip_server() {
@@ -1040,6 +1048,16 @@ struct yasfhip_server {
         return o.release();
     }
 
+    // FP64 target tiles the residual pool holds for the operator, or 0 when
+    // fewer than 16 fit (smaller slabs starve the GEMM of blocks)
+    int pool_slab_tiles(const SharedOp &o) {
+        if (!res_pool)
+            return 0;
+        const size_t tile64 = pair_table_tile_bytes(plan_for(o.L).dp, o.N, false);
+        const int tiles = static_cast<int>(std::min<size_t>(res_pool_bytes / tile64, o.ntiles1));
+        return tiles >= std::min(16, o.ntiles1) ? tiles : 0;
+    }
+
     void release_op(SharedOp *o) {
         if (--o->refs > 0)
             return;
@@ -1234,22 +1252,29 @@ struct yasfhip_server {
             const DevicePlan &dp = plan_for(q->L).dp;
             if (gemm_done[i])
                 continue;
+            const int pool_tiles = o.table64 || o.slab ? 0 : pool_slab_tiles(o);
             if (o.table64) {
                 launch_pair_table_gemm(dp, q->N, o.tab64, q->x_d, false, kServeShape64, q->Q, st);
-            } else if (o.slab) {
-                for (int t0 = 0; t0 < o.ntiles1; t0 += o.slab_tiles64) {
-                    const int ntl = std::min(o.slab_tiles64, o.ntiles1 - t0);
-                    launch_pair_table_build(dp, o.pos_d, o.N, o.k, false, o.slab, st, t0, ntl);
+                ++residual_forms[0];
+            } else if (o.slab || pool_tiles) {  // one build per slab for all of the operator's requests
+                void *slab = o.slab ? o.slab : res_pool;
+                const int tiles = o.slab ? o.slab_tiles64 : pool_tiles;
+                for (int t0 = 0; t0 < o.ntiles1; t0 += tiles) {
+                    const int ntl = std::min(tiles, o.ntiles1 - t0);
+                    launch_pair_table_build(dp, o.pos_d, o.N, o.k, false, slab, st, t0, ntl);
                     for (size_t j = i; j < todo.size(); ++j)
                         if (todo[j]->op == q->op && !todo[j]->approx_residual)
-                            launch_pair_table_gemm(dp, q->N, o.slab, todo[j]->x_d, false, kServeShape64, todo[j]->Q,
+                            launch_pair_table_gemm(dp, q->N, slab, todo[j]->x_d, false, kServeShape64, todo[j]->Q,
                                                    st, t0, ntl);
                 }
                 for (size_t j = i; j < todo.size(); ++j)
-                    if (todo[j]->op == q->op && !todo[j]->approx_residual)
+                    if (todo[j]->op == q->op && !todo[j]->approx_residual) {
                         gemm_done[j] = 1;
+                        ++residual_forms[o.slab ? 1 : 2];
+                    }
             } else {
                 launch_pair(dp, o.pos_d, q->N, q->k, q->x_d, false, q->Q, st);
+                ++residual_forms[3];
             }
             gemm_done[i] = 1;
         }
@@ -2011,6 +2036,10 @@ int yasfhip_server_create(size_t memory_bytes, int restart, double inner_rtol, i
             memory_bytes = free_b / 10 * 7;
         }
         s->arena.init(memory_bytes);
+        if (memory_bytes >= (size_t(32) << 30)) {
+            s->res_pool_bytes = memory_bytes / 32 / 256 * 256;
+            s->res_pool = s->arena.alloc(s->res_pool_bytes);
+        }
         YH_CHECK(hipMalloc(&s->partials, kRedPartials * sizeof(double)));
         YH_CHECK(hipMalloc(&s->counter, sizeof(unsigned)));
         YH_CHECK(hipMemsetAsync(s->counter, 0, sizeof(unsigned), s->stream));
@@ -2144,6 +2173,30 @@ int yasfhip_server_set_table_policy(yasfhip_server *s, int policy, int slab_tile
         YH_REQUIRE(slab_tiles >= 0, "slab_tiles must be >= 0");
         s->table_policy = policy;
         s->slab_tiles_req = slab_tiles;
+    });
+}
+
+int yasfhip_server_set_residual_pool(yasfhip_server *s, size_t bytes) {
+    return sguard([&] {
+        YH_REQUIRE(s, "NULL argument");
+        YH_REQUIRE(s->queue.empty() && s->active.empty(), "the residual pool needs an idle server");
+        YH_CHECK(hipStreamSynchronize(s->stream));
+        if (s->res_pool)
+            s->arena.release(s->res_pool);
+        s->res_pool = nullptr;
+        s->res_pool_bytes = 0;
+        if (bytes) {
+            s->res_pool = s->arena.alloc(bytes);
+            YH_REQUIRE(s->res_pool, "the residual pool does not fit the arena's free memory");
+            s->res_pool_bytes = bytes;
+        }
+    });
+}
+
+int yasfhip_server_residual_forms(yasfhip_server *s, long long *forms) {
+    return sguard([&] {
+        YH_REQUIRE(s && forms, "NULL argument");
+        std::copy(s->residual_forms, s->residual_forms + 4, forms);
     });
 }
 
