@@ -130,40 +130,27 @@ __global__ __launch_bounds__(256) void half_convert_kernel(const double2 *__rest
         half_convert_elem<SPLIT>(x, N, N32, nmax, colmax, xexp, xhi, xlo, idx);
 }
 
-// One block per (job, target tile) of tiles [t0, t0 + ntl); hi / lo point at
+// One block per (bin, target tile) of tiles [t0, t0 + ntl); hi / lo point at
 // the table's tile t0.
 template <int WAVES, int NTL, bool SPLIT>
 __global__ __launch_bounds__(64 * WAVES) void half_gemm_kernel(
     const _Float16 *__restrict__ hi_tab, const _Float16 *__restrict__ lo_tab, const _Float16 *__restrict__ xhi,
-    const _Float16 *__restrict__ xlo, const int *__restrict__ pexp, const int *__restrict__ xexp, int N32, int nmax,
-    int lmax, const PairJob *__restrict__ jobs, int ntl, int t0, const int *__restrict__ colmap, int maxcols,
+    const int *__restrict__ pexp, const int *__restrict__ xexp, int N32, int nmax, int lmax,
+    const HalfBin *__restrict__ bins, int ntl, int t0, const int *__restrict__ colmap,
     const int *__restrict__ dmoff, float *__restrict__ Q, int nq, const int *__restrict__ gate) {
     if (gate && *gate != 1)
         return;
-    const int jid = blockIdx.x / ntl;
-    const int t = blockIdx.x - jid * ntl;
-    const int dm = uniform(jobs[jid].dm);
-    const int tile0 = uniform(jobs[jid].tile0);
-    const int ntile = uniform(jobs[jid].ntile);
-    const int qbase = uniform(jobs[jid].qbase);
-    const int tstride = uniform(jobs[jid].tstride);
-    const int nord = uniform(jobs[jid].nord);
-    const int ip0 = uniform(jobs[jid].pad);  // first order of the job's order range
+    const int bid = blockIdx.x / ntl;
+    const int t = blockIdx.x - bid * ntl;
     const int P = 2 * lmax + 1;
-    const int adm = dm < 0 ? -dm : dm;
-    const int R = P - adm;
-    const int d = dm + 2 * lmax;
-    const size_t f0 = (static_cast<size_t>(t) * P * P + dmoff[d]) * N32 * 1024;
+    const size_t f0 = static_cast<size_t>(t) * P * P * N32 * 1024;
     const size_t xplane = half_x_plane(nmax, N32);
-    float *Qt = Q + 2 * (static_cast<size_t>(t0 + t) * nq + qbase) * 256;
-    (void)xlo;  // the lo terms follow the hi terms in one allocation
+    float *Qt = Q + 2 * static_cast<size_t>(t0 + t) * nq * 256;
     half_gemm_block<WAVES, NTL, SPLIT>(hi_tab + f0, SPLIT ? lo_tab + f0 : nullptr, HalfOneSource{xhi, xexp, Qt},
-                                       xplane, N32, nmax, R, ip0, nord, ntile, colmap + d * maxcols + tile0 * 16,
-                                       pexp + adm, tstride, tile0);
+                                       xplane, N32, nmax, lmax, bins + bid, colmap, dmoff, pexp);
 }
 
-constexpr int kHalfWaves = 8;  // 2 orders per wave: kHalfJobOrders
-static_assert(2 * kHalfWaves == kHalfJobOrders, "orders per fp16 job = 2 per wave");
+constexpr int kHalfWaves = kHalfBinWaves;
 
 }  // namespace
 
@@ -248,17 +235,16 @@ void launch_half_gemm(const DevicePlan &dp, int N, const void *table, const void
     const _Float16 *hi = static_cast<const _Float16 *>(table);
     const _Float16 *lo = hi + (term_halves ? term_halves : half_table_term_bytes(dp, N) / sizeof(_Float16));
     const _Float16 *xhi = static_cast<const _Float16 *>(xh);
-    const _Float16 *xlo = xhi + 2 * half_x_plane(dp.nmax, N32);
-    const int h = kHalfJobList;
-    const dim3 grid(dp.nhjobs[h] * ntl), block(64 * kHalfWaves);
+    const int f = split ? 0 : 1;
+    const dim3 grid(static_cast<unsigned>(dp.nhbins[f][0]) * ntl), block(64 * kHalfWaves);
     if (split)
         hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, kHalfJobTiles, true>), grid, block, 0, s, hi, lo, xhi,
-                           xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntl, t0, dp.colmap, dp.maxcols,
-                           dp.dmoff, static_cast<float *>(Q), dp.nq, gate);
+                           pexp, xexp, N32, dp.nmax, dp.lmax, dp.hbins[f][0], ntl, t0, dp.colmap, dp.dmoff,
+                           static_cast<float *>(Q), dp.nq, gate);
     else
         hipLaunchKernelGGL((half_gemm_kernel<kHalfWaves, kHalfJobTiles, false>), grid, block, 0, s, hi, lo,
-                           xhi, xlo, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hjobs[h], ntl, t0, dp.colmap,
-                           dp.maxcols, dp.dmoff, static_cast<float *>(Q), dp.nq, gate);
+                           xhi, pexp, xexp, N32, dp.nmax, dp.lmax, dp.hbins[f][0], ntl, t0, dp.colmap, dp.dmoff,
+                           static_cast<float *>(Q), dp.nq, gate);
     YH_CHECK(hipGetLastError());
 }
 

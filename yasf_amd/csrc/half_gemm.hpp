@@ -38,10 +38,8 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kHalfTop = 14;  // scaled magnitudes stay below 2^kHalfTop
 
-// Column tiles per fp16 GEMM job in both forms, and the plan's job list
-// with that many (DevicePlan::hjobs): each table slice streamed once.
-constexpr int kHalfJobTiles = 8;
-constexpr int kHalfJobList = 1;
+// Column tiles of an fp16 GEMM block's B stage (the widest bins).
+constexpr int kHalfJobTiles = kHalfBinTiles;
 
 // Exponent e with bound * 2^e < 2^kHalfTop (0 for a zero or non-finite bound).
 __host__ __device__ inline int half_exponent(double bound) {
@@ -77,9 +75,9 @@ __device__ __forceinline__ h8_t neg_h8(h8_t a) {
 
 // The source copies and outputs a block works on: one request's, or those
 // of several requests over one operator (Src::count() of them, each with the
-// job's ntile column tiles).  x(i): request i's copy (its hi term's plane 0,
+// bin's ntile column tiles).  x(i): request i's copy (its hi term's plane 0,
 // the lo term 2 xplane halves further); exps(i): its column exponents;
-// out(i): its Q past the (job, target tile) offset; live(i): false for a
+// out(i): its Q at the target tile (complex64 slots); live(i): false for a
 // request whose copy is not current (its tiles read zeros and are not
 // stored).  Every i is wave-uniform where these are called.
 struct HalfOneSource {
@@ -94,41 +92,34 @@ struct HalfOneSource {
     __device__ float *out(int) const { return Qt; }
 };
 
-// One (job, target tile) block of WAVES waves.  hi / lo point at the job's
-// (tile, dm) fragment 0 of the two terms (lo unused unless SPLIT); src: the
-// source copies (plane 1 = Im follows plane 0 at xplane halves); cols: the
-// job's 16 ntile source columns (-1: padding), the same for every request of
-// src, whose column tiles follow one another (request i's tile c is block
-// tile i ntile + c; src.count() ntile <= NTL); orders p = |dm| + ip for ip
-// in [ip0, ip0 + nord), nord <= 2 WAVES, dealt to the waves in turn (at most
-// two each); pexp: order exponents from p = |dm| on; exps(i)[n2] (entry
-// nmax = 0).  C tile (ip, c) of request i goes to Q slot ip * tstride +
-// tile0 + c past out(i) (complex64).  Each table fragment is loaded once
-// and used for every request of src.
+// One (bin, target tile) block of WAVES = kHalfBinWaves waves.  hi / lo
+// point at the target tile of the two terms' tables (lo unused unless
+// SPLIT), laid out as above; src: the source copies (plane 1 = Im follows
+// plane 0 at xplane halves), whose column tiles follow one another (request
+// i's bin tile k is block tile i ntile + k; src.count() ntile <= NTL); bin:
+// the block's jobs (see HalfBin); colmap, dmoff: the plan's; pexp: the order
+// exponents.  Wave w computes its orders wip over its bin tiles [wtb, wte)
+// for every request of src; C tile (order, k) of request i goes to Q slot
+// wq + k past out(i) (complex64).  Each table fragment is loaded once and
+// used for every request of src.
 //
 // Pipelining: the table stream (the operand that comes from HBM) and the B
-// fragments of a K-step (the column tiles of the job's requests, shared by
-// all waves, loaded by the whole block and stored to one of two LDS
-// buffers a K-step ahead) each have three register sets and two K-steps to
-// arrive.  (Table fragments three K-steps ahead with the x pieces one step
-// in flight were 5-10% slower.)  Operands come
-// through raw buffer loads: a wave's order slot, when it has none, and the B
-// pieces of absent column tiles read out of range, which returns zero
-// without touching memory, so every load is unconditional and the compiler's
-// waits stay static.  One barrier per K-step hands the LDS buffers over.
-//
-// A wave computes its two order slots together (one K sweep) when their
-// accumulators fit its registers, else one after the other (two K sweeps,
-// the second only when the job has more than WAVES orders): the two-term
-// form over 8 column tiles.  Either way each table fragment is read once;
-// the sequential form stages the x fragments once per sweep.
+// fragments of a K-step (the column tiles of the bin's requests, shared by
+// all waves, loaded by the whole block and stored to one of two LDS buffers
+// a K-step ahead) each have three register sets and two K-steps to arrive.
+// (Table fragments three K-steps ahead with the x pieces one step in flight
+// were 5-10% slower.)  Operands come through raw buffer loads: a wave's
+// order slot, when it has none, and the B pieces of absent column tiles read
+// out of range, which returns zero without touching memory, so every load is
+// unconditional and the compiler's waits stay static.  One barrier per
+// K-step hands the LDS buffers over.
 template <int WAVES, int NTL, bool SPLIT, class Src>
 __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_tab, const _Float16 *__restrict__ lo_tab,
-                                                const Src &src, size_t xplane, int N32, int nmax, int R, int ip0,
-                                                int nord, int ntile, const int *__restrict__ cols,
-                                                const int *__restrict__ pexp, int tstride, int tile0) {
-    constexpr int NIP = (SPLIT && NTL > 4) ? 1 : 2;  // order slots per sweep
-    constexpr int NSWEEP = 2 / NIP;
+                                                const Src &src, size_t xplane, int N32, int nmax, int lmax,
+                                                const HalfBin *__restrict__ bin, const int *__restrict__ colmap,
+                                                const int *__restrict__ dmoff, const int *__restrict__ pexp) {
+    static_assert(WAVES == kHalfBinWaves && NTL == kHalfBinTiles, "fp16 GEMM blocks are bins");
+    constexpr int NIP = SPLIT ? 1 : 2;  // orders per wave
     constexpr int NPL = SPLIT ? 4 : 2;  // planes: hi Re, hi Im, lo Re, lo Im
     constexpr int THREADS = WAVES * 64;
     constexpr int CHUNKS = NTL * NPL * 64;  // 16-byte B pieces of one K-step
@@ -139,7 +130,26 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
     const int wave = uniform(tid >> 6);
     const size_t S32 = static_cast<size_t>(N32) * 32;
     const int nreq = src.count();
+    const int ntile = uniform(bin->ntile);
     const int ntt = nreq * ntile;  // column tiles of the block
+    // this wave's dm, orders and tiles: bit it of mine is set when block
+    // tile it (request it / ntile, bin tile it % ntile) is the wave's
+    const int d = uniform(bin->wd[wave]);
+    const int adm = d < 0 ? 0 : abs(d - 2 * lmax);
+    const int R = 2 * lmax + 1 - adm;
+    const int tb = uniform(bin->wtb[wave]), te = uniform(bin->wte[wave]);
+    int wip[NIP];
+#pragma unroll
+    for (int j = 0; j < NIP; ++j)
+        wip[j] = d < 0 ? -1 : uniform(bin->wip[wave][j]);
+    unsigned mine = 0;
+    if (d >= 0)
+        for (int it = 0; it < ntt; ++it) {
+            const int k = it % ntile;
+            if (k >= tb && k < te)
+                mine |= 1u << it;
+        }
+    mine = uniform(mine);
     // B pieces of this thread: the copy they come from (a wave's 64 pieces
     // of one u share a column tile, hence a request), byte offset (K-step 0)
     // into that copy (4 planes) and LDS slot
@@ -151,11 +161,12 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
     for (int u = 0; u < PER; ++u) {
         const int q = tid + u * THREADS;
         const int itw = (wave * 64 + u * THREADS) / (NPL * 64);  // = q / (NPL * 64), wave-uniform
-        const int rq = min(itw / ntile, nreq - 1), c = itw - rq * ntile;
+        const int rq = min(itw / ntile, nreq - 1), k = itw - rq * ntile;
         if (Src::kMulti || u == 0)
             rx[Src::kMulti ? u : 0] = buffer_rsrc(src.x(rq), 4 * xplane * sizeof(_Float16));
         const int pl = (q / 64) % NPL, ln = q % 64;
-        const int n2 = (q < CHUNKS && itw < ntt && src.live(rq)) ? cols[c * 16 + (ln & 15)] : -1;
+        const int tc = (q < CHUNKS && itw < ntt && src.live(rq)) ? uniform(bin->tcol[k]) : -1;
+        const int n2 = tc >= 0 ? colmap[tc + (ln & 15)] : -1;
         const size_t plane = (pl < 2 ? 0 : 2 * xplane) + (pl & 1) * xplane;
         boff[u] = n2 < 0 ? kBufferNull
                          : static_cast<unsigned>((plane + static_cast<size_t>(n2) * S32 + (ln >> 4) * 8) *
@@ -175,141 +186,128 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
             if (bslot[u] >= 0)
                 (&bsh[buf][0][0][0])[bslot[u]] = breg[u];
     };
-    const size_t slice = static_cast<size_t>(N32) * R * 1024 * sizeof(_Float16);  // bytes of one term's slice
-    const __amdgpu_buffer_rsrc_t rh = buffer_rsrc(hi_tab, slice);
-    const __amdgpu_buffer_rsrc_t rl = buffer_rsrc(SPLIT ? lo_tab : hi_tab, slice);
+    // the wave's dm slice of each term (an idle wave: an empty range)
+    const size_t f0 = d < 0 ? 0 : static_cast<size_t>(dmoff[d]) * N32 * 1024;
+    const size_t slice = d < 0 ? 0 : static_cast<size_t>(N32) * R * 1024 * sizeof(_Float16);
+    const __amdgpu_buffer_rsrc_t rh = buffer_rsrc(hi_tab + f0, slice);
+    const __amdgpu_buffer_rsrc_t rl = buffer_rsrc((SPLIT ? lo_tab : hi_tab) + f0, slice);
     const unsigned kstride = static_cast<unsigned>(R) * 1024 * sizeof(_Float16);  // bytes per K-step
     auto kc = [&](int k) { return k < N32 ? k : N32 - 1; };
-
-#pragma unroll 1
-    for (int sweep = 0; sweep < NSWEEP; ++sweep) {
-        if (sweep > 0 && nord <= WAVES * sweep)
-            break;  // block-uniform: no order for this sweep
-        // this sweep's order slots of the wave: ip0 + wave + WAVES (sweep + j)
-        // (absent: null offset)
-        int slot[NIP];
-        unsigned aoff[NIP];
+    unsigned aoff[NIP];
+#pragma unroll
+    for (int j = 0; j < NIP; ++j)
+        aoff[j] = wip[j] >= 0 ? static_cast<unsigned>((wip[j] * 1024 + lane * 8) * sizeof(_Float16)) : kBufferNull;
+    auto load_a = [&](int ks, h8_t (&a)[NIP][NPL]) {
+        const unsigned so = static_cast<unsigned>(ks) * kstride;
 #pragma unroll
         for (int j = 0; j < NIP; ++j) {
-            slot[j] = wave + (sweep + j) * WAVES;
-            aoff[j] = slot[j] < nord ? static_cast<unsigned>(((ip0 + slot[j]) * 1024 + lane * 8) * sizeof(_Float16))
-                                     : kBufferNull;
+            a[j][0] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rh, aoff[j], so, 0));
+            a[j][1] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rh, aoff[j] + 1024, so, 0));
+            if constexpr (SPLIT) {
+                a[j][2] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rl, aoff[j], so, 0));
+                a[j][3] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rl, aoff[j] + 1024, so, 0));
+            }
         }
-        auto load_a = [&](int ks, h8_t (&a)[NIP][NPL]) {
-            const unsigned so = static_cast<unsigned>(ks) * kstride;
+    };
+    f4_t acc[NIP][NTL][2];
+#pragma unroll
+    for (int j = 0; j < NIP; ++j)
+#pragma unroll
+        for (int it = 0; it < NTL; ++it)
+            acc[j][it][0] = acc[j][it][1] = f4_t{0.f, 0.f, 0.f, 0.f};
+    // MFMAs of one K-step from the A fragments a and buffer buf of the B
+    // fragments (Im planes negated per column tile, for the real parts)
+    auto step = [&](const h8_t (&a)[NIP][NPL], int buf) {
+#pragma unroll
+        for (int it = 0; it < NTL; ++it) {
+            if (!((mine >> it) & 1u))
+                continue;
+            h8_t b[NPL], nb[NPL / 2];
+#pragma unroll
+            for (int pl = 0; pl < NPL; ++pl)
+                b[pl] = bsh[buf][it][pl][lane];
+            nb[0] = neg_h8(b[1]);
+            if constexpr (SPLIT)
+                nb[1] = neg_h8(b[3]);
 #pragma unroll
             for (int j = 0; j < NIP; ++j) {
-                a[j][0] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rh, aoff[j], so, 0));
-                a[j][1] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rh, aoff[j] + 1024, so, 0));
-                if constexpr (SPLIT) {
-                    a[j][2] = __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rl, aoff[j], so, 0));
-                    a[j][3] =
-                        __builtin_bit_cast(h8_t, __builtin_amdgcn_raw_buffer_load_b128(rl, aoff[j] + 1024, so, 0));
-                }
-            }
-        };
-        f4_t acc[NIP][NTL][2];
-#pragma unroll
-        for (int j = 0; j < NIP; ++j)
-#pragma unroll
-            for (int it = 0; it < NTL; ++it)
-                acc[j][it][0] = acc[j][it][1] = f4_t{0.f, 0.f, 0.f, 0.f};
-        // MFMAs of one K-step from the A fragments a and buffer buf of the B
-        // fragments (Im planes negated per column tile, for the real parts)
-        auto step = [&](const h8_t (&a)[NIP][NPL], int buf) {
-#pragma unroll
-            for (int it = 0; it < NTL; ++it) {
-                if (it >= ntt)
+                if (wip[j] < 0)
                     continue;
-                h8_t b[NPL], nb[NPL / 2];
+                f4_t cr = acc[j][it][0], ci = acc[j][it][1];
+                cr = mfma_h16(a[j][0], b[0], cr);
+                ci = mfma_h16(a[j][0], b[1], ci);
+                cr = mfma_h16(a[j][1], nb[0], cr);
+                ci = mfma_h16(a[j][1], b[0], ci);
+                if constexpr (SPLIT) {
+                    cr = mfma_h16(a[j][0], b[2], cr);
+                    ci = mfma_h16(a[j][0], b[3], ci);
+                    cr = mfma_h16(a[j][1], nb[1], cr);
+                    ci = mfma_h16(a[j][1], b[2], ci);
+                    cr = mfma_h16(a[j][2], b[0], cr);
+                    ci = mfma_h16(a[j][2], b[1], ci);
+                    cr = mfma_h16(a[j][3], nb[0], cr);
+                    ci = mfma_h16(a[j][3], b[0], ci);
+                }
+                acc[j][it][0] = cr;
+                acc[j][it][1] = ci;
+            }
+        }
+    };
+    // K-step ks computes with A set ks % 3 and LDS buffer ks % 2; it loads
+    // the table fragments of ks + 2 (set (ks + 2) % 3) before its MFMAs, then
+    // stores the x pieces of ks + 1 (B set (ks + 1) % 3) to LDS and reloads
+    // that set with ks + 4.  The vector-memory counter completes in order, so
+    // waiting for A of ks forces the x pieces of ks + 1 (issued before it):
+    // both have two K-steps to arrive.  K-steps past the last reload the last
+    // one (the loads stay unconditional).
+    h8_t a0[NIP][NPL], a1[NIP][NPL], a2[NIP][NPL];
+    h8_t b0[PER], b1[PER], b2[PER];
+    load_b(0, b0);
+    store_b(0, b0);
+    load_b(kc(1), b1);
+    load_b(kc(2), b2);
+    load_a(0, a0);
+    load_a(kc(1), a1);
+    load_b(kc(3), b0);
+    __syncthreads();
+    auto kstep = [&](int ks, const h8_t (&acur)[NIP][NPL], h8_t (&anext)[NIP][NPL], h8_t (&bnext)[PER]) {
+        load_a(kc(ks + 2), anext);
+        step(acur, ks & 1);
+        store_b((ks + 1) & 1, bnext);
+        load_b(kc(ks + 4), bnext);
+        __syncthreads();
+    };
+    for (int ks = 0; ks < N32; ks += 3) {
+        kstep(ks, a0, a2, b1);
+        if (ks + 1 >= N32)
+            break;
+        kstep(ks + 1, a1, a0, b2);
+        if (ks + 2 >= N32)
+            break;
+        kstep(ks + 2, a2, a1, b0);
+    }
 #pragma unroll
-                for (int pl = 0; pl < NPL; ++pl)
-                    b[pl] = bsh[buf][it][pl][lane];
-                nb[0] = neg_h8(b[1]);
-                if constexpr (SPLIT)
-                    nb[1] = neg_h8(b[3]);
+    for (int j = 0; j < NIP; ++j) {
+        if (wip[j] < 0)
+            continue;
+        const int wq = uniform(bin->wq[wave][j]);
+        const int eo = pexp[adm + wip[j]];
 #pragma unroll
-                for (int j = 0; j < NIP; ++j) {
-                    if (slot[j] >= nord)
-                        continue;
-                    f4_t cr = acc[j][it][0], ci = acc[j][it][1];
-                    cr = mfma_h16(a[j][0], b[0], cr);
-                    ci = mfma_h16(a[j][0], b[1], ci);
-                    cr = mfma_h16(a[j][1], nb[0], cr);
-                    ci = mfma_h16(a[j][1], b[0], ci);
-                    if constexpr (SPLIT) {
-                        cr = mfma_h16(a[j][0], b[2], cr);
-                        ci = mfma_h16(a[j][0], b[3], ci);
-                        cr = mfma_h16(a[j][1], nb[1], cr);
-                        ci = mfma_h16(a[j][1], b[2], ci);
-                        cr = mfma_h16(a[j][2], b[0], cr);
-                        ci = mfma_h16(a[j][2], b[1], ci);
-                        cr = mfma_h16(a[j][3], nb[0], cr);
-                        ci = mfma_h16(a[j][3], b[0], ci);
-                    }
-                    acc[j][it][0] = cr;
-                    acc[j][it][1] = ci;
+        for (int it = 0; it < NTL; ++it)
+            if ((mine >> it) & 1u) {
+                const int rq = it / ntile, k = it - rq * ntile;
+                if (!src.live(rq))
+                    continue;
+                const int n2 = colmap[uniform(bin->tcol[k]) + lo];
+                const int e = eo + src.exps(rq)[n2 < 0 ? nmax : n2];
+                const size_t qs = static_cast<size_t>(wq + k);
+                float *dst = src.out(rq) + 2 * ((qs * 16 + lo) * 16 + 4 * hi);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    dst[2 * i] = ldexpf(acc[j][it][0][i], -e);
+                    dst[2 * i + 1] = ldexpf(acc[j][it][1][i], -e);
                 }
             }
-        };
-        if (sweep > 0)
-            __syncthreads();  // the previous sweep's last LDS reads are done
-        // K-step ks computes with A set ks % 3 and LDS buffer ks % 2; it
-        // loads the table fragments of ks + 2 (set (ks + 2) % 3) before its
-        // MFMAs, then stores the x pieces of ks + 1 (B set (ks + 1) % 3) to
-        // LDS and reloads that set with ks + 4.  The vector-memory counter
-        // completes in order, so waiting for A of ks forces the x pieces of
-        // ks + 1 (issued before it): both have two K-steps to arrive.
-        // K-steps past the last reload the last one (the loads stay
-        // unconditional).
-        h8_t a0[NIP][NPL], a1[NIP][NPL], a2[NIP][NPL];
-        h8_t b0[PER], b1[PER], b2[PER];
-        load_b(0, b0);
-        store_b(0, b0);
-        load_b(kc(1), b1);
-        load_b(kc(2), b2);
-        load_a(0, a0);
-        load_a(kc(1), a1);
-        load_b(kc(3), b0);
-        __syncthreads();
-        auto kstep = [&](int ks, const h8_t (&acur)[NIP][NPL], h8_t (&anext)[NIP][NPL], h8_t (&bnext)[PER]) {
-            load_a(kc(ks + 2), anext);
-            step(acur, ks & 1);
-            store_b((ks + 1) & 1, bnext);
-            load_b(kc(ks + 4), bnext);
-            __syncthreads();
-        };
-        for (int ks = 0; ks < N32; ks += 3) {
-            kstep(ks, a0, a2, b1);
-            if (ks + 1 >= N32)
-                break;
-            kstep(ks + 1, a1, a0, b2);
-            if (ks + 2 >= N32)
-                break;
-            kstep(ks + 2, a2, a1, b0);
-        }
-#pragma unroll
-        for (int j = 0; j < NIP; ++j) {
-            if (slot[j] >= nord)
-                continue;
-            const int ip = ip0 + slot[j];
-#pragma unroll
-            for (int it = 0; it < NTL; ++it)
-                if (it < ntt) {
-                    const int rq = it / ntile, c = it - rq * ntile;
-                    if (!src.live(rq))
-                        continue;
-                    const int n2 = cols[c * 16 + lo];
-                    const int e = pexp[ip] + src.exps(rq)[n2 < 0 ? nmax : n2];
-                    const size_t qs = static_cast<size_t>(ip) * tstride + tile0 + c;
-                    float *dst = src.out(rq) + 2 * ((qs * 16 + lo) * 16 + 4 * hi);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        dst[2 * i] = ldexpf(acc[j][it][0][i], -e);
-                        dst[2 * i + 1] = ldexpf(acc[j][it][1][i], -e);
-                    }
-                }
-        }
     }
 }
 

@@ -94,7 +94,28 @@ constexpr int kWavesPerJob = 4;
 constexpr int kWaveMaxTiles = 8;                                // C tiles per wave
 constexpr int kJobMaxTiles = kWavesPerJob * kWaveMaxTiles;      // C tiles per job
 constexpr int kJobMaxColTiles = 2;                              // column tiles per job
-constexpr int kHalfJobOrders = 16;  // orders per fp16 GEMM job (8 waves x 2)
+
+// A block of the fp16 pair GEMM ("bin"): the orders of one or more jobs (a
+// dm's group of column tiles and a range of its orders) dealt to the block's
+// kHalfBinWaves waves, one order per wave in the two-term form and two in the
+// one-term form, the jobs' column tiles side by side in the block's B stage
+// (at most kHalfBinCap[c] per request).  Packing the short order ranges of
+// large |dm| together keeps the waves busy: a block's time is a fixed part
+// per K-step plus its busiest wave's MFMAs.  Bins of every capacity cover
+// the same (order, column tile) products, each once.
+constexpr int kHalfBinWaves = 8;
+constexpr int kHalfBinTiles = 8;
+constexpr int kHalfBinCaps = 3;
+constexpr int kHalfBinCap[kHalfBinCaps] = {8, 4, 2};
+struct HalfBin {
+    int ntile;                           // column tiles of the bin
+    int tcol[kHalfBinTiles];             // colmap offset of tile k's 16 source columns (-1: none)
+    int wd[kHalfBinWaves];               // wave's dm + 2L (-1: no order)
+    int wtb[kHalfBinWaves];              // wave's tiles [wtb, wte)
+    int wte[kHalfBinWaves];
+    int wip[kHalfBinWaves][2];           // wave's orders p - |dm| (-1: none)
+    int wq[kHalfBinWaves][2];            // Q slot of bin tile 0 for that order (tile k: wq + k)
+};
 
 // Everything the coupling kernels need for one lmax.
 struct CouplingPlan {
@@ -111,9 +132,11 @@ struct CouplingPlan {
     std::vector<double> leg_beta;     // [(2L+1)^2] Legendre recurrence beta(l, m)
     std::vector<double> leg_diag;     // [2L+1] P_m^m = leg_diag[m] * sin^m
     std::vector<int> order_tiles;     // [ndm][2L+1]: column tiles (prefix) of the dm that order p feeds
-    // fp16 GEMM jobs of up to 4 and of up to 8 column tiles (pad = first
-    // order of the job's order range, at most kHalfJobOrders orders)
-    std::vector<PairJob> half_jobs[2];
+    // fp16 GEMM bins, [0] two-term (one order per wave), [1] one-term (two),
+    // per capacity kHalfBinCap[c]; half_bin_tiles: their largest job widths
+    // summed (the busiest waves' column tiles)
+    std::vector<HalfBin> half_bins[2][kHalfBinCaps];
+    int half_bin_tiles[2][kHalfBinCaps] = {};
     long long pair_macs_per_pair = 0; // complex MACs the on-the-fly pair kernel issues per pair
     long long nnz_ab5 = 0;
 };
@@ -134,8 +157,9 @@ struct DevicePlan {
     double *la, *lb, *ld;
     int *dmoff;  // [ndm] first order slot of each dm in a pair-table target tile
     int *order_tiles;  // [ndm][2L+1] column tiles that order p feeds (a prefix of the dm's tiles)
-    PairJob *hjobs[2];  // fp16 GEMM jobs of up to 4 / 8 column tiles
-    int nhjobs[2];
+    HalfBin *hbins[2][kHalfBinCaps];  // CouplingPlan::half_bins
+    int nhbins[2][kHalfBinCaps];
+    int hbin_tiles[2][kHalfBinCaps];
 };
 
 // Records the message of the calling thread's last failure (yasfhip_last_error)

@@ -82,11 +82,12 @@ struct BatchReq {
     const int *colmap, *crow, *cidx, *dmoff, *order_tiles;
     const double2 *cval;
     // operators with an fp16-term table (the fp16 pair GEMM): the table, its
-    // order exponents, the job list it runs, and the request's fp16 copy of
-    // the basis vector with its column maxima and exponents
+    // order exponents, the bins of up to 8 column tiles it runs, and the
+    // request's fp16 copy of the basis vector with its column maxima and
+    // exponents
     const _Float16 *htab;
     const int *pexp;
-    const PairJob *hjobs[2];
+    const HalfBin *hbins;
     _Float16 *xh;
     unsigned *colmax;
     int *xexp;
@@ -199,30 +200,20 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_gemm_kernel(c
         return;
     const BatchReq &q = reqs[r];
     const int b = blockIdx.x - hs[h];
-    const int jid = b / q.ntiles1;
-    const int t = b - jid * q.ntiles1;
-    const PairJob *hjobs = q.hjobs[kHalfJobList];
-    const int dm = uniform(hjobs[jid].dm);
-    const int tile0 = uniform(hjobs[jid].tile0);
-    const int ntile = uniform(hjobs[jid].ntile);
-    const int qbase = uniform(hjobs[jid].qbase);
-    const int tstride = uniform(hjobs[jid].tstride);
-    const int nord = uniform(hjobs[jid].nord);
-    const int ip0 = uniform(hjobs[jid].pad);
+    const int bid = b / q.ntiles1;
+    const int t = b - bid * q.ntiles1;
     const int P = 2 * q.lmax + 1;
-    const int adm = dm < 0 ? -dm : dm;
-    const int d = dm + 2 * q.lmax;
-    const size_t f0 = (static_cast<size_t>(t) * P * P + q.dmoff[d]) * q.N32 * 1024;
+    const size_t f0 = static_cast<size_t>(t) * P * P * q.N32 * 1024;
     const size_t term = static_cast<size_t>(q.ntiles1) * half_tile_halves(q.lmax, q.N32);
     const size_t xplane = half_x_plane(q.nmax, q.N32);
-    float *Qt = q.Q + 2 * (static_cast<size_t>(t) * q.nq + qbase) * 256;
-    half_gemm_block<kServeHalfWaves, kHalfJobTiles, SPLIT>(
-        q.htab + f0, SPLIT ? q.htab + term + f0 : nullptr, HalfOneSource{q.xh, q.xexp, Qt}, xplane, q.N32, q.nmax,
-        P - adm, ip0, nord, ntile, q.colmap + d * q.maxcols + tile0 * 16, q.pexp + adm, tstride, tile0);
+    float *Qt = q.Q + 2 * static_cast<size_t>(t) * q.nq * 256;
+    half_gemm_block<kServeHalfWaves, kHalfJobTiles, SPLIT>(q.htab + f0, SPLIT ? q.htab + term + f0 : nullptr,
+                                                           HalfOneSource{q.xh, q.xexp, Qt}, xplane, q.N32, q.nmax,
+                                                           q.lmax, q.hbins + bid, q.colmap, q.dmoff, q.pexp);
 }
 
 // Requests r0 .. r0+n-1 of an operator group (ridx[]) as the sources of one
-// fp16 GEMM block; qoff: the (job, target tile) offset into each one's Q.
+// fp16 GEMM block; qoff: the target tile's offset into each one's Q.
 struct HalfGroupSource {
     static constexpr bool kMulti = true;
     const BatchReq *reqs;
@@ -240,18 +231,19 @@ struct HalfGroupSource {
 
 // The same fp16 GEMM for the nr requests ridx[] of one operator over its
 // fp16 table or a slab of it: target tiles t0 .. t0+ntl-1, tab at tile t0's
-// hi term, the lo term term halves further.  A block computes one (job,
-// tile) for up to per requests (per x the job's column tiles <=
-// kHalfJobTiles), each table fragment loaded once for all of them.  Logical
-// block lb runs request group lb % ngrp of (job, tile) lb / ngrp, so the
-// groups' blocks of one table slice run together; the hardware hands block b
-// to XCD b % 8, so block b takes lb = (b % 8) (gridDim / 8) + b / 8 and
-// those blocks also share one XCD's L2 (the grid is padded to a multiple of
-// 8; blocks past total return).
+// hi term, the lo term term halves further.  A block computes one (bin,
+// tile) of bins (the plan's bins of at most kHalfJobTiles / per column
+// tiles) for up to per requests, each table fragment loaded once for all of
+// them.  Logical block lb runs request group lb % ngrp of (bin, tile) lb /
+// ngrp, so the groups' blocks of one table slice run together; the hardware
+// hands block b to XCD b % 8, so block b takes lb = (b % 8) (gridDim / 8) +
+// b / 8 and those blocks also share one XCD's L2 (the grid is padded to a
+// multiple of 8; blocks past total return).
 template <bool SPLIT>
 __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_shared_gemm_kernel(
     const BatchReq *__restrict__ reqs, const ReqState *__restrict__ st, const int *__restrict__ ridx, int nr,
-    int per, const _Float16 *__restrict__ tab, size_t term, int t0, int ntl, int total) {
+    int per, const HalfBin *__restrict__ bins, const _Float16 *__restrict__ tab, size_t term, int t0, int ntl,
+    int total) {
     const int lb = static_cast<int>(blockIdx.x % 8) * static_cast<int>(gridDim.x / 8) + static_cast<int>(blockIdx.x / 8);
     if (lb >= total)
         return;
@@ -259,7 +251,7 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_shared_gemm_k
     const int grp = lb % ngrp;
     const int rest = lb / ngrp;
     const int tl = rest % ntl;
-    const int jid = rest / ntl;
+    const int bid = rest / ntl;
     const int r0 = grp * per, n = min(per, nr - r0);
     int live = 0;
     for (int i = 0; i < n; ++i)
@@ -267,24 +259,13 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_shared_gemm_k
     if (!live)
         return;
     const BatchReq &q = reqs[uniform(ridx[r0])];  // operator data: the same in every request of the group
-    const PairJob *hjobs = q.hjobs[kHalfJobList];
-    const int dm = uniform(hjobs[jid].dm);
-    const int tile0 = uniform(hjobs[jid].tile0);
-    const int ntile = uniform(hjobs[jid].ntile);
-    const int qbase = uniform(hjobs[jid].qbase);
-    const int tstride = uniform(hjobs[jid].tstride);
-    const int nord = uniform(hjobs[jid].nord);
-    const int ip0 = uniform(hjobs[jid].pad);
     const int P = 2 * q.lmax + 1;
-    const int adm = dm < 0 ? -dm : dm;
-    const int d = dm + 2 * q.lmax;
-    const size_t f0 = (static_cast<size_t>(tl) * P * P + q.dmoff[d]) * q.N32 * 1024;
+    const size_t f0 = static_cast<size_t>(tl) * P * P * q.N32 * 1024;
     const size_t xplane = half_x_plane(q.nmax, q.N32);
-    const HalfGroupSource src{reqs, st, ridx, r0, n, 2 * (static_cast<size_t>(t0 + tl) * q.nq + qbase) * 256};
+    const HalfGroupSource src{reqs, st, ridx, r0, n, 2 * static_cast<size_t>(t0 + tl) * q.nq * 256};
     half_gemm_block<kServeHalfWaves, kHalfJobTiles, SPLIT>(tab + f0, SPLIT ? tab + term + f0 : nullptr, src, xplane,
-                                                           q.N32, q.nmax, P - adm, ip0, nord, ntile,
-                                                           q.colmap + d * q.maxcols + tile0 * 16, q.pexp + adm,
-                                                           tstride, tile0);
+                                                           q.N32, q.nmax, q.lmax, bins + bid, q.colmap, q.dmoff,
+                                                           q.pexp);
 }
 
 // One (job, target tile t) block of a request's complex64 pair GEMM; tab
@@ -1484,8 +1465,7 @@ struct yasfhip_server {
             const bool half = q.op->half;
             b.htab = half ? static_cast<const _Float16 *>(q.op->htab) : nullptr;
             b.pexp = half ? q.op->pexp : nullptr;
-            b.hjobs[0] = dp.hjobs[0];
-            b.hjobs[1] = dp.hjobs[1];
+            b.hbins = half ? dp.hbins[q.op->half_split ? 0 : 1][0] : nullptr;
             b.xh = half ? static_cast<_Float16 *>(q.xh) : nullptr;
             b.colmax = half ? q.colmax : nullptr;
             b.xexp = half ? q.xexp : nullptr;
@@ -1555,7 +1535,7 @@ struct yasfhip_server {
                 if (q.op->table32 && op_group_size(half_reqs[h]) < 2) {
                     hg[g] = bg;
                     hgi[g++] = half_reqs[h];
-                    bg += plan_for(q.L).dp.nhjobs[kHalfJobList] * q.ntiles1;
+                    bg += plan_for(q.L).dp.nhbins[q.op->half_split ? 0 : 1][0] * q.ntiles1;
                 }
             }
             hc[nh] = bc;
@@ -1658,23 +1638,37 @@ struct yasfhip_server {
         const SharedOp &o = *g.op;
         const PlanEntry &pe = plan_for(o.L);
         const DevicePlan &dp = pe.dp;
-        // requests per block: as many as the widest job's column tiles allow
-        int maxnt = 1;
-        for (const PairJob &j : pe.plan.half_jobs[kHalfJobList])
-            maxnt = std::max(maxnt, j.ntile);
-        const int per = std::max(1, std::min(g.count, kHalfJobTiles / maxnt));
-        YH_REQUIRE(per * maxnt <= kHalfJobTiles || per == 1, "fp16 group blocks exceed the column tiles");
-        const long total = static_cast<long>(dp.nhjobs[kHalfJobList]) * ntl * ((g.count + per - 1) / per);
+        // bin capacity and requests per block: cap column tiles per request,
+        // kHalfJobTiles / cap requests per block.  A block costs a fixed part
+        // (about 4 column tiles of one wave's MFMAs) plus its busiest wave's
+        // tiles for each of its requests; the cheapest capacity wins.
+        const int f = o.half_split ? 0 : 1;
+        int cbest = 0, per = 1;
+        double best = 0.0;
+        for (int c = 0; c < kHalfBinCaps; ++c) {
+            const int pc = std::max(1, std::min(g.count, kHalfJobTiles / kHalfBinCap[c]));
+            const double cost = (4.0 * dp.nhbins[f][c] + static_cast<double>(pc) * dp.hbin_tiles[f][c]) *
+                                ((g.count + pc - 1) / pc);
+            if (c == 0 || cost < best) {
+                best = cost;
+                cbest = c;
+                per = pc;
+            }
+        }
+        YH_REQUIRE(per * kHalfBinCap[cbest] <= kHalfJobTiles, "fp16 group blocks exceed the column tiles");
+        const long total = static_cast<long>(dp.nhbins[f][cbest]) * ntl * ((g.count + per - 1) / per);
         YH_REQUIRE(total < (1L << 31) - 8, "fp16 operator group grid too large");
         const int blocks = static_cast<int>((total + 7) / 8 * 8);
         const int *ridx = d_index + g.first;
         const _Float16 *h = static_cast<const _Float16 *>(tab);
+        const HalfBin *bins = dp.hbins[f][cbest];
         if (o.half_split)
             hipLaunchKernelGGL(serve_half_shared_gemm_kernel<true>, dim3(blocks), dim3(64 * kServeHalfWaves), 0, stream,
-                               d_reqs, d_state, ridx, g.count, per, h, o.hterm, t0, ntl, static_cast<int>(total));
+                               d_reqs, d_state, ridx, g.count, per, bins, h, o.hterm, t0, ntl,
+                               static_cast<int>(total));
         else
             hipLaunchKernelGGL(serve_half_shared_gemm_kernel<false>, dim3(blocks), dim3(64 * kServeHalfWaves), 0,
-                               stream, d_reqs, d_state, ridx, g.count, per, h, o.hterm, t0, ntl,
+                               stream, d_reqs, d_state, ridx, g.count, per, bins, h, o.hterm, t0, ntl,
                                static_cast<int>(total));
         YH_CHECK(hipGetLastError());
     }

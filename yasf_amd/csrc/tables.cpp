@@ -268,16 +268,106 @@ CouplingPlan build_coupling_plan(int lmax) {
     for (auto &tj : plan.table_jobs)
         std::stable_sort(tj.begin(), tj.end(), by_cost);
     std::stable_sort(plan.table_jobs_w64.begin(), plan.table_jobs_w64.end(), by_cost);
-    // fp16 GEMM jobs: the jobs of up to 4 column tiles with at most
-    // kHalfJobOrders orders each; a longer order range is split and the
-    // piece's first order goes to pad
-    for (int h = 0; h < 2; ++h) {  // jobs of up to 4 (h = 0) or 8 (h = 1) column tiles
-        for (const PairJob &j : plan.table_jobs[1 + h])
-            for (int ip0 = 0; ip0 < j.nord; ip0 += kHalfJobOrders) {
-                const int n = std::min(kHalfJobOrders, j.nord - ip0);
-                plan.half_jobs[h].push_back(PairJob{j.dm, j.tile0, j.ntile, j.qbase, n * j.ntile, j.tstride, n, ip0});
+    // fp16 GEMM bins.  Jobs of the grouping that matches the capacity are
+    // cut into sub-jobs of at most 8 (two-term) or 16 (one-term, two per
+    // wave) orders, and the sub-jobs are packed first-fit, widest first, into
+    // bins of kHalfBinWaves waves and cap column tiles.  A sub-job of w waves
+    // deals its orders ip0 + i and ip0 + i + w to its wave i.
+    for (int form = 0; form < 2; ++form) {
+        const int nip = form + 1;
+        for (int c = 0; c < kHalfBinCaps; ++c) {
+            const int cap = kHalfBinCap[c];
+            int g = 0;
+            while (kTableGrouping[g] != cap)
+                ++g;
+            struct Sub {
+                const PairJob *j;
+                int ip0, n, w;
+            };
+            std::vector<Sub> subs;
+            for (const PairJob &j : plan.table_jobs[g])
+                for (int ip0 = 0; ip0 < j.nord; ip0 += kHalfBinWaves * nip) {
+                    const int n = std::min(kHalfBinWaves * nip, j.nord - ip0);
+                    subs.push_back(Sub{&j, ip0, n, (n + nip - 1) / nip});
+                }
+            std::stable_sort(subs.begin(), subs.end(), [](const Sub &a, const Sub &b) {
+                return a.j->ntile != b.j->ntile ? a.j->ntile > b.j->ntile : a.w > b.w;
+            });
+            std::vector<HalfBin> bins;
+            std::vector<int> waves_used, widest;
+            for (const Sub &sj : subs) {
+                const int nt = sj.j->ntile;
+                size_t bi = 0;
+                while (bi < bins.size() &&
+                       (waves_used[bi] + sj.w > kHalfBinWaves || bins[bi].ntile + nt > cap))
+                    ++bi;
+                if (bi == bins.size()) {
+                    HalfBin nb{};
+                    std::fill(std::begin(nb.tcol), std::end(nb.tcol), -1);
+                    std::fill(std::begin(nb.wd), std::end(nb.wd), -1);
+                    for (int w = 0; w < kHalfBinWaves; ++w)
+                        nb.wip[w][0] = nb.wip[w][1] = nb.wq[w][0] = nb.wq[w][1] = -1;
+                    bins.push_back(nb);
+                    waves_used.push_back(0);
+                    widest.push_back(0);
+                }
+                HalfBin &b = bins[bi];
+                const int dmi = sj.j->dm + 2 * lmax, tb = b.ntile;
+                for (int k = 0; k < nt; ++k)
+                    b.tcol[tb + k] = dmi * maxcols + (sj.j->tile0 + k) * 16;
+                for (int i = 0; i < sj.w; ++i) {
+                    const int w = waves_used[bi] + i;
+                    b.wd[w] = dmi;
+                    b.wtb[w] = tb;
+                    b.wte[w] = tb + nt;
+                    for (int o = 0; o < nip; ++o) {
+                        const int io = i + o * sj.w;
+                        if (io >= sj.n)
+                            continue;
+                        const int ip = sj.ip0 + io;
+                        b.wip[w][o] = ip;
+                        b.wq[w][o] = sj.j->qbase + ip * sj.j->tstride + sj.j->tile0 - tb;
+                    }
+                }
+                waves_used[bi] += sj.w;
+                b.ntile += nt;
+                widest[bi] = std::max(widest[bi], nt);
             }
-        std::stable_sort(plan.half_jobs[h].begin(), plan.half_jobs[h].end(), by_cost);
+            // widest bins first (the longest blocks start first)
+            std::vector<size_t> order(bins.size());
+            for (size_t i = 0; i < order.size(); ++i)
+                order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return widest[a] > widest[b]; });
+            int tiles = 0;
+            for (size_t i : order) {
+                plan.half_bins[form][c].push_back(bins[i]);
+                tiles += widest[i];
+            }
+            plan.half_bin_tiles[form][c] = tiles;
+            // every (order, column tile) product of the grouping's jobs
+            // exactly once, with the column tile the job gives it
+            std::vector<int> want(static_cast<size_t>(q), -1), seen(static_cast<size_t>(q), 0);
+            for (const PairJob &j : plan.table_jobs[g])
+                for (int ip = 0; ip < j.nord; ++ip)
+                    for (int k = 0; k < j.ntile; ++k)
+                        want[j.qbase + ip * j.tstride + j.tile0 + k] = (j.dm + 2 * lmax) * maxcols + (j.tile0 + k) * 16;
+            for (const HalfBin &b : plan.half_bins[form][c])
+                for (int w = 0; w < kHalfBinWaves; ++w)
+                    for (int o = 0; o < 2; ++o) {
+                        if (b.wd[w] < 0 || b.wip[w][o] < 0)
+                            continue;
+                        for (int k = b.wtb[w]; k < b.wte[w]; ++k) {
+                            const int slot = b.wq[w][o] + k;
+                            if (slot < 0 || slot >= q || want[slot] != b.tcol[k] ||
+                                b.tcol[k] / maxcols != b.wd[w])
+                                throw Error(kInternal, "fp16 GEMM bin outside the pair plan");
+                            ++seen[slot];
+                        }
+                    }
+            for (int slot = 0; slot < q; ++slot)
+                if (seen[slot] != (want[slot] >= 0 ? 1 : 0))
+                    throw Error(kInternal, "fp16 GEMM bins do not cover the pair plan once");
+        }
     }
 
     // Contraction lists: for each n1, the nonzero ab5[n2][n1][p].
@@ -304,6 +394,20 @@ CouplingPlan build_coupling_plan(int lmax) {
         }
         plan.crow[n1 + 1] = static_cast<int>(plan.cidx.size());
     }
+    // every Q slot the contraction reads is computed by every bin list
+    for (int form = 0; form < 2; ++form)
+        for (int c = 0; c < kHalfBinCaps; ++c) {
+            std::vector<char> cov(static_cast<size_t>(plan.nq), 0);
+            for (const HalfBin &b : plan.half_bins[form][c])
+                for (int w = 0; w < kHalfBinWaves; ++w)
+                    for (int o = 0; o < 2; ++o)
+                        if (b.wd[w] >= 0 && b.wip[w][o] >= 0)
+                            for (int k = b.wtb[w]; k < b.wte[w]; ++k)
+                                cov[b.wq[w][o] + k] = 1;
+            for (int ci : plan.cidx)
+                if (!cov[ci / 16])
+                    throw Error(kInternal, "fp16 GEMM bins miss a Q slot of the contraction");
+        }
 
     // Normalised Legendre recurrences (legendre_normalized_trigon.py:59-91):
     //   P_m^m = sqrt((2m+1)/2/(2m)!) (2m-1)!! sin^m
@@ -375,17 +479,20 @@ DevicePlan upload_device_plan(const CouplingPlan &pl, hipStream_t s) {
     }
     dp.dmoff = upload_vec(dmoff, s);
     dp.order_tiles = upload_vec(pl.order_tiles, s);
-    for (int h = 0; h < 2; ++h) {
-        dp.hjobs[h] = upload_vec(pl.half_jobs[h], s);
-        dp.nhjobs[h] = static_cast<int>(pl.half_jobs[h].size());
-    }
+    for (int f = 0; f < 2; ++f)
+        for (int c = 0; c < kHalfBinCaps; ++c) {
+            dp.hbins[f][c] = upload_vec(pl.half_bins[f][c], s);
+            dp.nhbins[f][c] = static_cast<int>(pl.half_bins[f][c].size());
+            dp.hbin_tiles[f][c] = pl.half_bin_tiles[f][c];
+        }
     return dp;
 }
 
 void free_device_plan(DevicePlan &dp) {
-    void *bufs[] = {dp.jobs,  dp.tjobs[0], dp.tjobs[1], dp.tjobs[2], dp.tjobs_w64,     dp.colmap,
-                    dp.crow,  dp.cidx,     dp.cval,     dp.la,       dp.lb,          dp.ld,
-                    dp.dmoff, dp.order_tiles, dp.hjobs[0], dp.hjobs[1]};
+    void *bufs[] = {dp.jobs,         dp.tjobs[0],     dp.tjobs[1],     dp.tjobs[2],     dp.tjobs_w64,
+                    dp.colmap,       dp.crow,         dp.cidx,         dp.cval,         dp.la,
+                    dp.lb,           dp.ld,           dp.dmoff,        dp.order_tiles,  dp.hbins[0][0],
+                    dp.hbins[0][1],  dp.hbins[0][2],  dp.hbins[1][0],  dp.hbins[1][1],  dp.hbins[1][2]};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
