@@ -419,6 +419,8 @@ def test_half_operator_groups_across_orders(precision):
     for r in reqs:
         a, b = shared.results[r.index], unshared.results[r.index]
         assert a["info"] == 0 and np.array_equal(a["x"], b["x"]) and a["iterations"] == b["iterations"]
+        # C_sca: a key's completions share their quadrature launches, the
+        # unkeyed ones run one each; the same bits
         assert a["c_ext"] == b["c_ext"] and a["c_sca"] == b["c_sca"]
     for key in (31, 32):
         _fp64_check(reqs[0] if key == 31 else reqs[5], shared.results[100 * key])

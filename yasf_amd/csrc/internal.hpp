@@ -314,6 +314,13 @@ void csca_quadrature_grid(double kR, int lmax, int *n_theta, int *n_phi);
 size_t csca_quadrature_partials(int n_theta, int n_phi);
 void launch_csca_quadrature(const double4 *pos, const double2 *f, int N, int lmax, double k, int n_theta,
                             int n_phi, double *partials, double *out, hipStream_t s);
+// The same for nreq <= csca_quadrature_batch_max(lmax) solutions f[i] of one
+// geometry in one launch (the phase factors computed once for all), each
+// with its own partials[i] and out[i]; the results equal separate launches'.
+int csca_quadrature_batch_max(int lmax);
+void launch_csca_quadrature_batch(const double4 *pos, const double2 *const *f, double *const *partials,
+                                  double *const *out, int nreq, int N, int lmax, double k, int n_theta, int n_phi,
+                                  hipStream_t s);
 
 // Device allocation owned for a scope (freed on every exit path).
 template <typename T>

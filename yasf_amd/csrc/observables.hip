@@ -217,19 +217,28 @@ __device__ inline void gauss_legendre_node(int i, int n, double &x, double &w) {
 //   C_sca k^2 = int |E|^2 dOmega = sum_{i,j} w_i (2 pi / n_phi) |E(x_i, phi_j)|^2,
 // Gauss-Legendre in cos th (n_theta nodes) and the trapezoid rule in phi
 // (n_phi points), exact for the band-limited far field of the cluster.  One
-// block (4 waves) per 16 directions: the F rows of its directions (the far
-// field GEMM of far_field_gemm_kernel, all modes) go to LDS, then 16 threads
-// assemble E and the block leaves the weighted sum of its 16 directions (in
-// direction order) in partials[block].
-constexpr int kQuadColTiles = (2 * kMaxLmax * (kMaxLmax + 2) + 15) / 16;  // mode tiles at lmax 8
-__global__ __launch_bounds__(256) void csca_quadrature_kernel(const double4 *__restrict__ pos,
-                                                              const double2 *__restrict__ f, int N, int lmax,
-                                                              int nmax, double k, int n_theta, int n_phi,
-                                                              double *__restrict__ partials) {
+// block (4 waves) per 16 directions and up to kQuadMaxReq solutions f of one
+// geometry (their mode tiles, at most kQuadMaxTiles, dealt to the waves in
+// turn): per chunk of 16 sources the waves compute the 16 x 16 phase factors
+// once into LDS (each wave 4 sources), then every wave runs the chunk's four
+// K-steps of the far-field GEMM (far_field_gemm_kernel's) for its tiles.
+// The F rows of the block's directions go to LDS, 16 threads per solution
+// assemble E, and the block leaves the weighted sum of its 16 directions (in
+// direction order) in partials[i][block] of solution i.
+constexpr int kQuadMaxTiles = 12;  // mode tiles of a block: 3 per wave
+constexpr int kQuadMaxReq = 12;
+struct QuadBatch {
+    const double2 *f[kQuadMaxReq];
+    double *partials[kQuadMaxReq];
+};
+__global__ __launch_bounds__(256) void csca_quadrature_kernel(const double4 *__restrict__ pos, QuadBatch batch,
+                                                              int nreq, int N, int lmax, int nmax, double k,
+                                                              int n_theta, int n_phi) {
     typedef Mfma<double>::acc_t acc_t;
-    constexpr int kPerWave = (kQuadColTiles + 3) / 4;
-    __shared__ double2 Fs[16][kQuadColTiles * 16];
-    __shared__ double Is[16];
+    constexpr int kPerWave = kQuadMaxTiles / 4;
+    __shared__ double2 Fs[16][kQuadMaxTiles * 16];
+    __shared__ double2 Ph[16][16];  // phase factor of direction r and source k0 + c
+    __shared__ double Is[kQuadMaxReq][16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lo = lane & 15, hi = lane >> 4;
     const int A = n_theta * n_phi;
@@ -243,60 +252,73 @@ __global__ __launch_bounds__(256) void csca_quadrature_kernel(const double4 *__r
     sincos(ph, &sph, &cph);
     const double ex = st * cph, ey = st * sph, ez = x;
     const int ntiles = (nmax + 15) / 16;
+    const int ntt = nreq * ntiles;  // the block's mode tiles: solution ct / ntiles, tile ct % ntiles
     acc_t acc[kPerWave][2];
 #pragma unroll
     for (int t = 0; t < kPerWave; ++t) {
         acc[t][0] = acc_t{0, 0, 0, 0};
         acc[t][1] = acc_t{0, 0, 0, 0};
     }
-    for (int k0 = 0; k0 < N; k0 += 4) {
-        const int s = k0 + hi;
-        double ar = 0.0, ai = 0.0;
-        if (s < N) {
-            const double4 r = pos[s];
-            sincos(-k * (r.x * ex + r.y * ey + r.z * ez), &ai, &ar);
+    for (int k0 = 0; k0 < N; k0 += 16) {
+        {
+            const int s = k0 + 4 * wave + hi;
+            double ar = 0.0, ai = 0.0;
+            if (s < N) {
+                const double4 r = pos[s];
+                sincos(-k * (r.x * ex + r.y * ey + r.z * ez), &ai, &ar);
+            }
+            Ph[lo][4 * wave + hi] = make_double2(ar, ai);
         }
+        __syncthreads();
 #pragma unroll
-        for (int t = 0; t < kPerWave; ++t) {
-            const int ct = wave + 4 * t;
-            if (ct < ntiles) {
-                const int n = ct * 16 + lo;
-                double2 b = make_double2(0.0, 0.0);
-                if (s < N && n < nmax)
-                    b = f[static_cast<size_t>(s) * nmax + n];
-                acc[t][0] = Mfma<double>::mac(ar, b.x, acc[t][0]);
-                acc[t][0] = Mfma<double>::mac(-ai, b.y, acc[t][0]);
-                acc[t][1] = Mfma<double>::mac(ar, b.y, acc[t][1]);
-                acc[t][1] = Mfma<double>::mac(ai, b.x, acc[t][1]);
+        for (int kk = 0; kk < 4; ++kk) {
+            const int s = k0 + 4 * kk + hi;
+            const double2 e = Ph[lo][4 * kk + hi];
+            const double ar = e.x, ai = e.y;
+#pragma unroll
+            for (int t = 0; t < kPerWave; ++t) {
+                const int ct = wave + 4 * t;
+                if (ct < ntt) {
+                    const int rq = ct / ntiles;
+                    const int n = (ct - rq * ntiles) * 16 + lo;
+                    double2 b = make_double2(0.0, 0.0);
+                    if (s < N && n < nmax)
+                        b = batch.f[rq][static_cast<size_t>(s) * nmax + n];
+                    acc[t][0] = Mfma<double>::mac(ar, b.x, acc[t][0]);
+                    acc[t][0] = Mfma<double>::mac(-ai, b.y, acc[t][0]);
+                    acc[t][1] = Mfma<double>::mac(ar, b.y, acc[t][1]);
+                    acc[t][1] = Mfma<double>::mac(ai, b.x, acc[t][1]);
+                }
             }
         }
+        __syncthreads();  // the phases of this chunk are read
     }
 #pragma unroll
     for (int t = 0; t < kPerWave; ++t) {
         const int ct = wave + 4 * t;
-        if (ct < ntiles) {
+        if (ct < ntt) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 Fs[Mfma<double>::row(i, hi)][ct * 16 + lo] = make_double2(acc[t][0][i], acc[t][1][i]);
         }
     }
     __syncthreads();
-    if (threadIdx.x < 16) {
-        const int r = threadIdx.x;  // direction r of the tile: its own node (lanes 0..15 hold them)
+    if (threadIdx.x < 16 * nreq) {
+        const int r = threadIdx.x & 15, rq = threadIdx.x >> 4;  // direction r of the tile, solution rq
         double v = 0.0;
-        if (a < A) {
+        if (a < A) {  // lanes 0..15 and their copies in each group of 16 hold direction r's node
             double2 eth, eph;
-            assemble_direction(Fs[r], lmax, x, st, ph, eth, eph);
+            assemble_direction(Fs[r] + rq * ntiles * 16, lmax, x, st, ph, eth, eph);
             v = w * (2.0 * M_PI / n_phi) * (eth.x * eth.x + eth.y * eth.y + eph.x * eph.x + eph.y * eph.y);
         }
-        Is[r] = v;
+        Is[rq][r] = v;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < nreq) {
         double sum = 0.0;
         for (int r = 0; r < 16; ++r)
-            sum += Is[r];
-        partials[blockIdx.x] = sum;
+            sum += Is[threadIdx.x][r];
+        batch.partials[threadIdx.x][blockIdx.x] = sum;
     }
 }
 
@@ -343,12 +365,30 @@ size_t csca_quadrature_partials(int n_theta, int n_phi) {
 // doubles of device scratch.
 void launch_csca_quadrature(const double4 *pos, const double2 *f, int N, int lmax, double k, int n_theta,
                             int n_phi, double *partials, double *out, hipStream_t s) {
+    launch_csca_quadrature_batch(pos, &f, &partials, &out, 1, N, lmax, k, n_theta, n_phi, s);
+}
+
+int csca_quadrature_batch_max(int lmax) {
+    const int ntiles = (2 * lmax * (lmax + 2) + 15) / 16;
+    return std::max(1, std::min(kQuadMaxReq, kQuadMaxTiles / ntiles));
+}
+
+void launch_csca_quadrature_batch(const double4 *pos, const double2 *const *f, double *const *partials,
+                                  double *const *out, int nreq, int N, int lmax, double k, int n_theta, int n_phi,
+                                  hipStream_t s) {
     const int nmax = 2 * lmax * (lmax + 2);
+    YH_REQUIRE(nreq >= 1 && nreq <= csca_quadrature_batch_max(lmax), "C_sca quadrature batch too large");
     const int blocks = static_cast<int>(csca_quadrature_partials(n_theta, n_phi));
-    hipLaunchKernelGGL(csca_quadrature_kernel, dim3(blocks), dim3(256), 0, s, pos, f, N, lmax, nmax, k, n_theta,
-                       n_phi, partials);
+    QuadBatch batch{};
+    for (int i = 0; i < nreq; ++i) {
+        batch.f[i] = f[i];
+        batch.partials[i] = partials[i];
+    }
+    hipLaunchKernelGGL(csca_quadrature_kernel, dim3(blocks), dim3(256), 0, s, pos, batch, nreq, N, lmax, nmax, k,
+                       n_theta, n_phi);
     YH_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, partials, blocks, 1.0 / M_PI, out);
+    for (int i = 0; i < nreq; ++i)
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, partials[i], blocks, 1.0 / M_PI, out[i]);
     YH_CHECK(hipGetLastError());
 }
 

@@ -274,6 +274,8 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_shared_gemm_k
 // computing one order at a time over jobs of up to 6 column tiles, the
 // session default (each table slice streamed once).
 constexpr int kServeShape64 = 2;
+// completions whose C_sca quadratures share one launch, at most
+constexpr int kMaxQuadBatch = 12;
 // Table GEMM shape of a request's single complex64 product (the residual of
 // x0 = b): 8 waves with jobs of up to 8 column tiles, the session default.
 constexpr int kServeShape32 = 2;
@@ -1316,18 +1318,37 @@ struct yasfhip_server {
         // else on the fly
         std::vector<char> regular_done(done.size(), 0);
         if (csca_method == YASFHIP_CSCA_FAR_FIELD) {
-            for (auto *q : done) {
-                if (!q->observables)
+            // the completions of one operator share their quadrature launches
+            // (phase factors computed once per batch)
+            std::vector<char> batched(done.size(), 0);
+            for (size_t i = 0; i < done.size(); ++i) {
+                Request *q = done[i];
+                if (!q->observables || batched[i])
                     continue;
                 int nth = 0, nph = 0;
                 csca_quadrature_grid(q->op->kR, q->L, &nth, &nph);
-                YH_REQUIRE(csca_quadrature_partials(nth, nph) * sizeof(double) <=
-                               static_cast<size_t>(restart + 1) * q->n * sizeof(double2),
-                           "far-field quadrature partials exceed the request's Krylov basis");
-                launch_csca_quadrature(q->op->pos_d, q->x_d, q->N, q->L, q->k, nth, nph,
-                                       reinterpret_cast<double *>(q->V), scal + 8 * q->slot + 2, st);
-                launch_mgs_step(q->x_d, nullptr, nullptr, q->a_d, q->n, reducer(), scal + 8 * q->slot + 5, st);
+                const double2 *f[kMaxQuadBatch];
+                double *part[kMaxQuadBatch], *out[kMaxQuadBatch];
+                const int bmax = std::min(kMaxQuadBatch, csca_quadrature_batch_max(q->L));
+                int nb = 0;
+                for (size_t j = i; j < done.size() && nb < bmax; ++j) {
+                    Request *r = done[j];
+                    if (!r->observables || batched[j] || r->op != q->op)
+                        continue;
+                    YH_REQUIRE(csca_quadrature_partials(nth, nph) * sizeof(double) <=
+                                   static_cast<size_t>(restart + 1) * r->n * sizeof(double2),
+                               "far-field quadrature partials exceed the request's Krylov basis");
+                    f[nb] = r->x_d;
+                    part[nb] = reinterpret_cast<double *>(r->V);
+                    out[nb] = scal + 8 * r->slot + 2;
+                    ++nb;
+                    batched[j] = 1;
+                }
+                launch_csca_quadrature_batch(q->op->pos_d, f, part, out, nb, q->N, q->L, q->k, nth, nph, st);
             }
+            for (auto *q : done)
+                if (q->observables)
+                    launch_mgs_step(q->x_d, nullptr, nullptr, q->a_d, q->n, reducer(), scal + 8 * q->slot + 5, st);
         }
         for (size_t i = 0; i < done.size() && csca_method == YASFHIP_CSCA_PAIR_SUM; ++i) {
             Request *q = done[i];
