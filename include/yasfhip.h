@@ -424,6 +424,13 @@ int yasfhip_server_set_scattering_method(yasfhip_server *s, int method);
  * candidates above). */
 int yasfhip_server_set_tile_shape(yasfhip_server *s, int lmax, int waves, int column_tiles);
 
+/* Batched Arnoldi iterations per yasfhip_server_step (1..16, default 1; a
+ * HIP graph holds them all): the host synchronises, refreshes, completes and
+ * admits once per step.  A request whose cycle ends inside a step waits for
+ * the step's end (its kernels skip it); results do not depend on the
+ * setting.  No counterpart in the reference. */
+int yasfhip_server_set_iterations_per_step(yasfhip_server *s, int iterations);
+
 /* Residual pool: one FP64 slab, carved from the arena, that the FP64
  * residuals of operators holding neither an FP64 table nor a slab of their
  * own use in turn (residuals run one after another on the server's stream),

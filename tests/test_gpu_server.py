@@ -87,6 +87,25 @@ def test_graph_eager_and_small_arena_agree(mixed_requests, served):
         assert abs(sm["c_ext"] - a["c_ext"]) <= 1e-12 * abs(a["c_ext"])
 
 
+def test_iterations_per_step_agree(mixed_requests, served):
+    # several batched Arnoldi iterations per step (one graph replay or one
+    # eager sequence, one host synchronisation): each request runs the same
+    # iterations, only its refreshes wait for the step's end, so the results
+    # agree bit for bit
+    for k, use_graph in ((4, True), (3, False)):
+        srv = _native.Server(memory_bytes=4 << 30, restart=50, inner_rtol=1e-4, use_graph=use_graph)
+        srv.set_iterations_per_step(k)
+        rep = serve(srv, mixed_requests, keep_solutions=True)
+        srv.close()
+        for r in mixed_requests:
+            a, b = served.results[r.index], rep.results[r.index]
+            assert b["info"] == 0 and np.array_equal(a["x"], b["x"])
+            assert a["iterations"] == b["iterations"] and a["cycles"] == b["cycles"]
+            assert a["c_ext"] == b["c_ext"] and a["c_sca"] == b["c_sca"]
+    with pytest.raises(ValueError):
+        _native.Server(memory_bytes=1 << 30).set_iterations_per_step(17)
+
+
 def test_residual_pool_agrees(mixed_requests):
     # no per-operator tables or slabs (on-the-fly complex64 GEMMs) with and
     # without a residual pool of 128 MB (10 FP64 tiles of the largest request,

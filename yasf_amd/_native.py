@@ -125,6 +125,7 @@ def _load():
             "yasfhip_server_set_inner_precision": ([P, i], i),
             "yasfhip_server_set_scattering_method": ([P, i], i),
             "yasfhip_server_set_residual_pool": ([P, ctypes.c_size_t], i),
+            "yasfhip_server_set_iterations_per_step": ([P, i], i),
             "yasfhip_server_residual_forms": ([P, llp], i),
         }
         for name, (args, res) in sig.items():
@@ -155,6 +156,7 @@ def exported_symbols() -> list[str]:
         "yasfhip_set_precond_factor_precision", "yasfhip_coupling_matvec_noalloc",
         "yasfhip_set_scattering_method", "yasfhip_server_set_scattering_method", "yasfhip_set_inner_precision",
         "yasfhip_server_set_inner_precision", "yasfhip_server_set_residual_pool", "yasfhip_server_residual_forms",
+        "yasfhip_server_set_iterations_per_step",
     ]
 
 
@@ -570,6 +572,10 @@ class Server:
         residuals of operators without an FP64 table or slab of their own (0
         releases it).  Arenas of 32 GiB or more start with memory_bytes / 32."""
         _check(self._lib.yasfhip_server_set_residual_pool(self._h, int(nbytes)))
+
+    def set_iterations_per_step(self, iterations: int):
+        """Batched Arnoldi iterations per step (1..16) between host synchronisations."""
+        _check(self._lib.yasfhip_server_set_iterations_per_step(self._h, int(iterations)))
 
     def residual_forms(self) -> dict:
         """FP64 residual pair GEMMs run so far, by form."""

@@ -755,6 +755,7 @@ struct yasfhip_server {
     // 1: resident tables where they fit, else slabs; 2: slabs only; 0: pair
     // factors on the fly only
     int table_policy = 1;
+    int iters_per_step = 1;  // batched Arnoldi iterations between host syncs
     int slab_tiles_req = 0;  // FP64 target tiles per slab (0: an eighth of the arena)
     int csca_method = YASFHIP_CSCA_FAR_FIELD;  // how completions compute C_sca
     int inner_precision = YASFHIP_FP32;         // pair GEMM of the inner GMRES iterations
@@ -1934,18 +1935,24 @@ struct yasfhip_server {
             return 0;
         const int nreq = static_cast<int>(active.size());
         YH_CHECK(hipMemcpyAsync(d_state, h_state, sizeof(ReqState) * nreq, hipMemcpyHostToDevice, stream));
+        // iters_per_step Arnoldi iterations per step: every kernel of an
+        // iteration skips the requests whose device state word left
+        // kArnoldi, so a cycle that ends inside the step idles its request
+        // until the host refreshes it
         if (use_graph) {
             if (!graph) {
                 hipGraph_t g = nullptr;
                 YH_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                launch_iteration();
+                for (int i = 0; i < iters_per_step; ++i)
+                    launch_iteration();
                 YH_CHECK(hipStreamEndCapture(stream, &g));
                 YH_CHECK(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
                 YH_CHECK(hipGraphDestroy(g));
             }
             YH_CHECK(hipGraphLaunch(graph, stream));
         } else {
-            launch_iteration();
+            for (int i = 0; i < iters_per_step; ++i)
+                launch_iteration();
         }
         YH_CHECK(hipMemcpyAsync(h_state, d_state, sizeof(ReqState) * nreq, hipMemcpyDeviceToHost, stream));
         YH_CHECK(hipStreamSynchronize(stream));
@@ -2188,6 +2195,17 @@ int yasfhip_server_set_table_policy(yasfhip_server *s, int policy, int slab_tile
         YH_REQUIRE(slab_tiles >= 0, "slab_tiles must be >= 0");
         s->table_policy = policy;
         s->slab_tiles_req = slab_tiles;
+    });
+}
+
+int yasfhip_server_set_iterations_per_step(yasfhip_server *s, int iterations) {
+    return sguard([&] {
+        YH_REQUIRE(s, "NULL argument");
+        YH_REQUIRE(iterations >= 1 && iterations <= 16, "iterations per step must be in [1, 16]");
+        if (iterations != s->iters_per_step) {
+            s->iters_per_step = iterations;
+            s->set_changed = true;  // the graph is captured again
+        }
     });
 }
 

@@ -234,6 +234,8 @@ def main(argv=None):
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
     ap.add_argument("--eager", action="store_true",
                     help="launch each iteration's kernels directly instead of replaying a captured graph")
+    ap.add_argument("--iterations-per-step", type=int, default=None,
+                    help="batched Arnoldi iterations between host synchronisations (server default if omitted)")
     args = ap.parse_args(argv)
 
     import torch
@@ -250,6 +252,8 @@ def main(argv=None):
                 for particles in (400, 2000):
                     srv.autotune(lmax, particles=particles, reps=3)
         srv.set_inner_precision(prec)
+        if args.iterations_per_step is not None:
+            srv.set_iterations_per_step(args.iterations_per_step)
         rep = serve(srv, reqs, device_setup=True, progress=lambda m: print(m, flush=True))
     finally:
         srv.close()
