@@ -263,15 +263,13 @@ __device__ __forceinline__ void pair_series_regular(double dx, double dy, double
 }
 
 // (W x)[s1, n1] = sum_e ab5_e Q[e] for the 16 targets of tile s1t and the 16
-// target modes n1 = 16 g + threadIdx.x/16 of mode group g, then either
-// y = W x (mode 0) or the master operator y = x - t (W x) (mode 1,
+// target modes of contraction rows 16 g .. 16 g + 15 (mode group g; the
+// plan's crow gives each row's mode, rows grouped by m1), then either y = W x
+// (mode 0) or the master operator y = x - t (W x) (mode 1,
 // matrix_apply.py:104-110); FP64 accumulation.  256 threads: lane&15 =
 // target in tile, so the 16 targets of one Q slot are read as one coalesced
-// row.  A launch covers (target tile, mode group) pairs: ceil(nmax/16) blocks
-// per tile.
-constexpr int kContractModes = 16;  // target modes per block
-
-__host__ __device__ inline int contract_groups(int nmax) { return (nmax + kContractModes - 1) / kContractModes; }
+// row.  A launch covers (target tile, mode group) pairs: contract_groups(nmax)
+// blocks per tile.
 
 template <typename TQ, typename TX, typename TY>
 __device__ __forceinline__ void contract_tile(const TQ *__restrict__ Q, int nq, const int *__restrict__ crow,
@@ -282,18 +280,19 @@ __device__ __forceinline__ void contract_tile(const TQ *__restrict__ Q, int nq, 
     // Q holds the target tiles from qt0 on (qt0 = 0: all of them)
     const int r = threadIdx.x & 15;
     const int s1 = s1t * 16 + r;
-    const int n1 = group * kContractModes + (threadIdx.x >> 4);
-    if (n1 >= nmax)
+    const int row = group * kContractModes + (threadIdx.x >> 4);
+    const int n1 = crow[contract_groups(nmax) * kContractModes + 1 + row];
+    if (n1 < 0)
         return;
     const size_t qrow = static_cast<size_t>(s1t - qt0) * nq * 16;
     double sr = 0.0, si = 0.0;
-    const int e1 = crow[n1 + 1];
+    const int e1 = crow[row + 1];
     // batches of kBatch entries: all their index, coefficient and Q loads are
     // in flight together (one entry at a time waits two memory round trips
     // per entry); entries past the row's end repeat its first with a zero
     // coefficient, and the sum runs in entry order as before
     constexpr int kBatch = 8;
-    for (int e = crow[n1]; e < e1; e += kBatch) {
+    for (int e = crow[row]; e < e1; e += kBatch) {
         const int cnt = min(kBatch, e1 - e);
         int ci[kBatch];
         double2 a[kBatch];

@@ -95,6 +95,11 @@ constexpr int kWaveMaxTiles = 8;                                // C tiles per w
 constexpr int kJobMaxTiles = kWavesPerJob * kWaveMaxTiles;      // C tiles per job
 constexpr int kJobMaxColTiles = 2;                              // column tiles per job
 
+// Target modes per block of the ab5 contraction, and its mode groups (blocks
+// per target tile): the plan packs the modes of each m1 into them.
+constexpr int kContractModes = 16;
+__host__ __device__ inline int contract_groups(int nmax) { return (nmax + kContractModes - 1) / kContractModes; }
+
 // A block of the fp16 pair GEMM ("bin"): the orders of one or more jobs (a
 // dm's group of column tiles and a range of its orders) dealt to the block's
 // kHalfBinWaves waves, one order per wave in the two-term form and two in the
@@ -125,7 +130,10 @@ struct CouplingPlan {
     std::vector<PairJob> table_jobs[kTableGroupings];  // resident-table GEMM jobs, sorted by cost
     std::vector<PairJob> table_jobs_w64;  // jobs of up to kTableWideTiles64 column tiles, sorted by cost
     int stage_elems = 0;              // complex elements of one pair-kernel LDS stage
-    std::vector<int> crow;            // CSR row offsets per target mode n1 (nmax+1)
+    // CSR of the contraction: rows k < R = contract_groups(nmax) x
+    // kContractModes, crow[k] .. crow[k + 1] its entries, crow[R + 1 + k]
+    // its target mode n1 (-1: padding row); rows grouped by m1
+    std::vector<int> crow;
     std::vector<int> cidx;            // Q column index q*16 + col
     std::vector<double> cval;         // ab5 entry (re, im) interleaved
     std::vector<double> leg_alpha;    // [(2L+1)^2] Legendre recurrence alpha(l, m)

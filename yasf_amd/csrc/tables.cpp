@@ -370,9 +370,50 @@ CouplingPlan build_coupling_plan(int lmax) {
         }
     }
 
-    // Contraction lists: for each n1, the nonzero ab5[n2][n1][p].
-    plan.crow.assign(plan.nmax + 1, 0);
-    for (int n1 = 0; n1 < plan.nmax; ++n1) {
+    // Contraction lists: for each target mode n1, the nonzero ab5[n2][n1][p]
+    // in (n2, p) order, as rows of a CSR.  A Q slot (dm, p, column n2) feeds
+    // only the modes with m1 = m2 - dm, so the rows are ordered to keep each
+    // m1's modes inside one block of kContractModes rows: the m1 sets, largest
+    // first, are packed first-fit into contract_groups(nmax) blocks (padding
+    // rows: mode -1), and each Q row is then read by one block.
+    // crow[k], crow[k + 1]: row k's entries; crow[R + 1 + k]: its mode
+    // (R = contract_groups(nmax) kContractModes rows).
+    const int crows = contract_groups(plan.nmax) * kContractModes;
+    std::vector<int> row_mode(crows, -1);
+    {
+        std::vector<std::pair<int, int>> sets;  // (mode count, m1)
+        for (int m = -lmax; m <= lmax; ++m) {
+            int c = 0;
+            for (const auto &md : modes)
+                c += md.m == m;
+            sets.push_back({c, m});
+        }
+        std::stable_sort(sets.begin(), sets.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+        std::vector<std::vector<int>> groups;
+        for (const auto &st : sets) {
+            size_t g = 0;
+            while (g < groups.size() && groups[g].size() + st.first > static_cast<size_t>(kContractModes))
+                ++g;
+            if (g == groups.size())
+                groups.emplace_back();
+            for (int n1 = 0; n1 < plan.nmax; ++n1)
+                if (modes[n1].m == st.second)
+                    groups[g].push_back(n1);
+        }
+        if (static_cast<int>(groups.size()) != contract_groups(plan.nmax))
+            throw Error(kInternal, "contraction modes do not pack into their blocks");
+        for (size_t g = 0; g < groups.size(); ++g)
+            for (size_t i = 0; i < groups[g].size(); ++i)
+                row_mode[g * kContractModes + i] = groups[g][i];
+    }
+    plan.crow.assign(2 * static_cast<size_t>(crows) + 1, 0);
+    for (int k = 0; k < crows; ++k) {
+        const int n1 = row_mode[k];
+        plan.crow[crows + 1 + k] = n1;
+        if (n1 < 0) {
+            plan.crow[k + 1] = static_cast<int>(plan.cidx.size());
+            continue;
+        }
         const int m1 = modes[n1].m;
         for (int n2 = 0; n2 < plan.nmax; ++n2) {
             const int dm = modes[n2].m - m1;
@@ -392,7 +433,7 @@ CouplingPlan build_coupling_plan(int lmax) {
                 ++plan.nnz_ab5;
             }
         }
-        plan.crow[n1 + 1] = static_cast<int>(plan.cidx.size());
+        plan.crow[k + 1] = static_cast<int>(plan.cidx.size());
     }
     // every Q slot the contraction reads is computed by every bin list
     for (int form = 0; form < 2; ++form)
