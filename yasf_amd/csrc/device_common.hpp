@@ -75,6 +75,8 @@ struct Cplx<float> {
 };
 
 // Pair geometry of the reference's mutual_lookup (misc.py:149-229):
+constexpr int kMaxOrders = 2 * kMaxLmax + 1;  // p = 0 .. 2 lmax
+
 // d = r_s1 - r_s2, cos th = d_z/|d| clipped (0 at d = 0), sin th =
 // sqrt(1 - cos^2), phi = atan2(d_y, d_x); (er, ei) = e^{i dm phi} by binary
 // powering of e^{i phi}; x = k |d| (1 at d = 0, unused there).
@@ -165,6 +167,91 @@ __device__ __forceinline__ void pair_series_to(double dx, double dy, double dz, 
     }
 }
 
+// G_{s1 s2}[p, dm] for every dm in [-2L, 2L] and p in [|dm|, 2L], handed to
+// sink(dm + 2L, p - |dm|, re, im): pair_series_to's values for each dm (the
+// same operations; the compiler may fuse a multiply-add differently, so the
+// last bit can differ), with the distance, angles and Hankel functions
+// formed once per pair instead of once per dm.
+template <typename Sink>
+__device__ __forceinline__ void pair_series_all(double dx, double dy, double dz, double k, int lmax, bool valid,
+                                                const double *__restrict__ la, const double *__restrict__ lb,
+                                                const double *__restrict__ ld, Sink &&sink) {
+    const int P = 2 * lmax + 1;
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    const bool zero_dist = !(d > 0.0);
+    double ct = zero_dist ? 0.0 : dz / d;
+    ct = fmin(1.0, fmax(-1.0, ct));
+    const double st = sqrt(1.0 - ct * ct);
+    const double rho2 = dx * dx + dy * dy;
+    double c1 = 1.0, s1 = 0.0;
+    if (rho2 > 0.0) {
+        const double ir = 1.0 / sqrt(rho2);
+        c1 = dx * ir;
+        s1 = dy * ir;
+    }
+    const double x = zero_dist ? 1.0 : k * d;
+    // h_p, p < P, by pair_series_to's recurrence
+    double hre[kMaxOrders], him[kMaxOrders];
+    {
+        double sx, cx;
+        sincos(x, &sx, &cx);
+        const double inv = 1.0 / x;
+        double ar = sx * inv, ai = -cx * inv;
+        double hr = (ar - cx) * inv, hi = (ai - sx) * inv;
+#pragma unroll
+        for (int p = 0; p < kMaxOrders; ++p) {
+            hre[p] = ar;
+            him[p] = ai;
+            const double f = (2 * p + 3) * inv;
+            const double nr = f * hr - ar, ni = f * hi - ai;
+            ar = hr;
+            ai = hi;
+            hr = nr;
+            hi = ni;
+        }
+    }
+    for (int dmi = 0; dmi < 4 * lmax + 1; ++dmi) {
+        const int dm = dmi - 2 * lmax;
+        const int adm = dm < 0 ? -dm : dm;
+        double er = 1.0, ei = 0.0, br = c1, bi = s1;
+        for (int n = adm; n; n >>= 1) {
+            if (n & 1) {
+                const double t = er * br - ei * bi;
+                ei = er * bi + ei * br;
+                er = t;
+            }
+            const double t = br * br - bi * bi;
+            bi = 2.0 * br * bi;
+            br = t;
+        }
+        if (dm < 0)
+            ei = -ei;
+        double cur = ld[adm], prv = 0.0;
+        for (int i = 0; i < adm; ++i)
+            cur *= st;
+#pragma unroll
+        for (int p = 0; p < kMaxOrders; ++p) {
+            if (p < P && p >= adm) {
+                const double phr = cur * hre[p], phi = cur * him[p];
+                double gr = phr * er - phi * ei;
+                double gi = phr * ei + phi * er;
+                if (zero_dist) {  // h_p(0) = j_p(0) = delta_p0, phi = 0, cos th = 0
+                    gr = (p == 0) ? cur : 0.0;
+                    gi = 0.0;
+                }
+                if (!valid) {
+                    gr = 0.0;
+                    gi = 0.0;
+                }
+                sink(dmi, p - adm, gr, gi);
+                const double nx = la[p * P + adm] * ct * cur - lb[p * P + adm] * prv;
+                prv = cur;
+                cur = nx;
+            }
+        }
+    }
+}
+
 // As above, stored to dst[(p - |dm|) * stride] (float2 or double2).
 template <typename T2>
 __device__ __forceinline__ void pair_series(double dx, double dy, double dz, double k, int dm, int P,
@@ -178,8 +265,6 @@ __device__ __forceinline__ void pair_series(double dx, double dy, double dz, dou
         dst[ip * stride] = v;
     });
 }
-
-constexpr int kMaxOrders = 2 * kMaxLmax + 1;  // p = 0 .. 2 lmax
 
 // Spherical Bessel j_p(x), p < kMaxOrders, x > 0.  The upward recurrence is
 // stable while p < x, so it serves x >= kMaxOrders; below that Miller's

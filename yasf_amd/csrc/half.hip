@@ -43,9 +43,10 @@ __global__ void half_exponents_kernel(const unsigned long long *__restrict__ dmi
         pexp[p] = half_order_exponent(p, k, d2);
 }
 
-// One block per (target tile, dm, 16-source chunk); thread = one pair, with
-// the 8 sources of a lane's A fragment on consecutive threads so the fp16
-// stores of a wave are contiguous.  Builds target tiles t0.. (table points at
+// One block per (target tile, 16-source chunk); thread = one pair, all of
+// its dm (the pair's geometry and Hankel functions formed once), with the 8
+// sources of a lane's A fragment on consecutive threads so the fp16 stores of
+// a wave are contiguous.  Builds target tiles t0.. (table points at
 // tile t0); sources up to N32 * 32 (zero past N); pexp: the order exponents.
 // gate: as the GEMM's (returns unless *gate == 1).
 template <bool SPLIT>
@@ -57,18 +58,17 @@ __global__ __launch_bounds__(256) void half_table_build_kernel(
     if (gate && *gate != 1)
         return;
     __shared__ int es[kMaxOrders];
+    __shared__ int dms[4 * kMaxLmax + 1];
     const int P = 2 * lmax + 1;
     if (threadIdx.x < P)
         es[threadIdx.x] = pexp[threadIdx.x];
+    if (threadIdx.x < ndm)
+        dms[threadIdx.x] = dmoff[threadIdx.x];
     __syncthreads();
     const int nchunk = 2 * N32;
     const int c = blockIdx.x % nchunk;
-    const int d = (blockIdx.x / nchunk) % ndm;
-    const int t = blockIdx.x / (nchunk * ndm);
+    const int t = blockIdx.x / nchunk;
     const int SR = P * P;
-    const int dm = d - 2 * lmax;
-    const int adm = dm < 0 ? -dm : dm;
-    const int R = P - adm;
     const int i = threadIdx.x & 7, r = (threadIdx.x >> 3) & 15, g = threadIdx.x >> 7;
     const int s1 = (t0 + t) * 16 + r, s2 = c * 16 + g * 8 + i;
     const int ks = s2 >> 5, lane = ((s2 & 31) >> 3) * 16 + r;
@@ -76,8 +76,7 @@ __global__ __launch_bounds__(256) void half_table_build_kernel(
     const double4 p2 = pos[s2 < N ? s2 : 0];
     const double dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
     const bool valid = (s1 < N) && (s2 < N) && s1 != s2 && dx * dx + dy * dy + dz * dz <= near2;
-    const size_t base = ((static_cast<size_t>(t) * SR + dmoff[d]) * N32 + static_cast<size_t>(ks) * R) * 1024 +
-                        lane * 8 + i;
+    const size_t tbase = static_cast<size_t>(t) * SR * N32 * 1024 + lane * 8 + i;
     // fp16 terms through FP32: two conversion instructions instead of the
     // correctly rounded FP64 -> fp16 integer sequence (the empty asm keeps
     // the compiler from folding the two roundings back into that one).  hi
@@ -88,11 +87,13 @@ __global__ __launch_bounds__(256) void half_table_build_kernel(
         asm volatile("" : "+v"(f));
         return static_cast<_Float16>(f);
     };
-    pair_series_to(dx, dy, dz, k, dm, P, valid, la, lb, ld, [&](int ip, double gr, double gi) {
+    pair_series_all(dx, dy, dz, k, lmax, valid, la, lb, ld, [&](int d, int ip, double gr, double gi) {
+        const int adm = d < 2 * lmax ? 2 * lmax - d : d - 2 * lmax;
+        const int R = P - adm;
         const int e = es[adm + ip];
         const double vr = ldexp(gr, e), vi = ldexp(gi, e);
         const _Float16 hr = to_half(vr), hm = to_half(vi);
-        const size_t o = base + static_cast<size_t>(ip) * 1024;
+        const size_t o = tbase + (static_cast<size_t>(dms[d]) * N32 + static_cast<size_t>(ks) * R + ip) * 1024;
         hi_tab[o] = hr;
         hi_tab[o + 512] = hm;
         if constexpr (SPLIT) {
@@ -184,7 +185,7 @@ void launch_half_table_range(const DevicePlan &dp, const double4 *pos, int N, do
     const int ndm = 4 * dp.lmax + 1;
     _Float16 *hi = static_cast<_Float16 *>(tab);
     _Float16 *lo = split ? hi + term_halves : nullptr;
-    const dim3 grid(static_cast<unsigned>(ntl) * ndm * 2 * N32), block(256);
+    const dim3 grid(static_cast<unsigned>(ntl) * 2 * N32), block(256);
     if (split)
         hipLaunchKernelGGL(half_table_build_kernel<true>, grid, block, 0, s, pos, N, N32, k, dp.lmax, ndm, dp.dmoff,
                            dp.la, dp.lb, dp.ld, hi, lo, t0, pexp, near2, gate);

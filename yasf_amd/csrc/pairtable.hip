@@ -16,7 +16,9 @@ namespace yasfhip {
 
 namespace {
 
-// One block per (target tile, dm, 16-source chunk); thread = one pair.
+// One block per (target tile, 16-source chunk) -- per (target tile, dm,
+// 16-source chunk) for the regular factors; thread = one pair (the pair
+// factors of all its dm: geometry and Hankel functions formed once).
 // Builds target tiles t0.. of the table (table points at tile t0: a slab of
 // the full table when t0 > 0 or fewer tiles are launched).  gate: run only
 // while *gate == 1 (batched callers), null: always.
@@ -30,14 +32,16 @@ __global__ __launch_bounds__(256) void pair_table_build_kernel(
     double near2) {
     if (gate && *gate != 1)
         return;
+    __shared__ int dms[4 * kMaxLmax + 1];
+    if (threadIdx.x < ndm)
+        dms[threadIdx.x] = dmoff[threadIdx.x];
+    __syncthreads();
     const int nchunk = (N + 15) / 16;
     const int c = blockIdx.x % nchunk;
-    const int d = (blockIdx.x / nchunk) % ndm;
-    const int t = blockIdx.x / (nchunk * ndm);
+    const int d = Regular ? (blockIdx.x / nchunk) % ndm : 0;
+    const int t = blockIdx.x / (nchunk * (Regular ? ndm : 1));
     const int P = 2 * lmax + 1;
     const int SR = P * P;
-    const int dm = d - 2 * lmax;
-    const int R = P - (dm < 0 ? -dm : dm);
     const int r = threadIdx.x & 15, kk = threadIdx.x >> 4;
     const int s1 = (t0 + t) * 16 + r, s2 = c * 16 + kk;
     const int ks = s2 >> 2;
@@ -49,12 +53,21 @@ __global__ __launch_bounds__(256) void pair_table_build_kernel(
     const double dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
     // near2: pairs farther apart than the near-field radius do not couple
     const bool valid = (s1 < N) && (s2 < N) && (Regular || (s1 != s2 && dx * dx + dy * dy + dz * dz <= near2));
-    T2 *dst = table + ((static_cast<size_t>(t) * SR + dmoff[d]) * N4 + static_cast<size_t>(ks) * R) * 64 +
-              lane;
-    if constexpr (Regular)
-        pair_series_regular<T2>(dx, dy, dz, k, dm, P, valid, la, lb, ld, dst, 64);
-    else
-        pair_series<T2>(dx, dy, dz, k, dm, P, valid, la, lb, ld, dst, 64);
+    T2 *tile = table + static_cast<size_t>(t) * SR * N4 * 64 + lane;
+    if constexpr (Regular) {
+        const int dm = d - 2 * lmax;
+        const int R = P - (dm < 0 ? -dm : dm);
+        pair_series_regular<T2>(dx, dy, dz, k, dm, P, valid, la, lb, ld,
+                                tile + (static_cast<size_t>(dms[d]) * N4 + static_cast<size_t>(ks) * R) * 64, 64);
+    } else {
+        pair_series_all(dx, dy, dz, k, lmax, valid, la, lb, ld, [&](int dd, int ip, double gr, double gi) {
+            const int R = P - (dd < 2 * lmax ? 2 * lmax - dd : dd - 2 * lmax);
+            T2 v;
+            v.x = gr;
+            v.y = gi;
+            tile[(static_cast<size_t>(dms[dd]) * N4 + static_cast<size_t>(ks) * R + ip) * 64] = v;
+        });
+    }
 }
 
 // One block of WAVES waves per (job, target tile), which deals the job's
@@ -170,7 +183,7 @@ void launch_pair_table_build(const DevicePlan &dp, const double4 *pos, int N, do
         ntl = ntiles1 - t0;
     YH_REQUIRE(t0 >= 0 && ntl >= 1 && t0 + ntl <= ntiles1, "table tile range outside the targets");
     const int ndm = 4 * dp.lmax + 1;
-    const dim3 grid(ntl * ndm * nchunk), block(256);
+    const dim3 grid(ntl * (regular ? ndm : 1) * nchunk), block(256);
     YH_REQUIRE(!(regular && fp32), "the regular table is FP64 only");
     if (regular)
         hipLaunchKernelGGL((pair_table_build_kernel<double2, true>), grid, block, 0, s, pos, N, N4, k, dp.lmax,
