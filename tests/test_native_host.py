@@ -41,7 +41,11 @@ def test_translation_table_fixture(golden):
 
 def test_plan_covers_every_ab5_entry():
     # nnz reported by the plan equals the exact count; pair work per pair is
-    # below the reference loop's direct count 8*T_nz + 8*nmax^2.
+    # below the reference loop's direct count 8*T_nz + 8*nmax^2.  Building the
+    # plan also runs its own checks (an error if one fails): every fp16 bin
+    # list computes each of its jobs' (order, column tile) products once and
+    # every Q slot the contraction reads, and the contraction rows pack each
+    # m1's modes into one block.
     for lmax in range(1, 9):
         macs, nnz = _native.plan_stats(lmax)
         assert nnz == np.count_nonzero(R.translation_table(lmax))
