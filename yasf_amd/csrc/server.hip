@@ -274,6 +274,9 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_shared_gemm_k
 // computing one order at a time over jobs of up to 6 column tiles, the
 // session default (each table slice streamed once).
 constexpr int kServeShape64 = 2;
+// Share of the arena an operator's resident inner table (complex64 or fp16
+// terms) may take before the operator streams slabs instead
+constexpr double kInnerShare = 0.875;
 // completions whose C_sca quadratures share one launch, at most
 constexpr int kMaxQuadBatch = 12;
 // Table GEMM shape of a request's single complex64 product (the residual of
@@ -925,8 +928,9 @@ struct yasfhip_server {
 
     // A new operator for request q.  Policy 1 takes the first that fits the
     // arena's share: complex64 + FP64 tables within a quarter of the arena;
-    // complex64 within half of it, with an FP64 slab of up to a sixteenth
-    // (else alone, FP64 on the fly); else a slab of at most an eighth.  Null
+    // the inner table within kInnerShare of it, with an FP64 slab of up to a
+    // sixteenth (else alone: FP64 residuals over the residual pool or on the
+    // fly); else a slab of at most an eighth.  Null
     // when the chosen form cannot be allocated now: the request waits for
     // memory rather than running in a slower form -- except in an idle
     // arena, where smaller slabs and at last the tableless form (on-the-fly
@@ -975,18 +979,19 @@ struct yasfhip_server {
         // on the fly, 1.5x the slab form's time)
         auto table_with_slab = [&]() {
             for (int t = slab64s; t >= 1 && t >= std::min(16, slab64s); t /= 2)
-                if (attempt(true, false, t, cap / 2 + tile64 * t))
+                if (attempt(true, false, t, static_cast<size_t>(kInnerShare * cap) + tile64 * t))
                     return true;
             return false;
         };
-        bool table_form = false;  // policy 1 and the complex64 table fits the arena's share
+        bool table_form = false;  // policy 1 and the inner table fits the arena's share
         if (table_policy == 1) {
             table_form = true;
-            if (!attempt(true, true, 0, cap / 4) && !table_with_slab() && !attempt(true, false, 0, cap / 2)) {
+            if (!attempt(true, true, 0, cap / 4) && !table_with_slab() &&
+                !attempt(true, false, 0, static_cast<size_t>(kInnerShare * cap))) {
                 o->table32 = true;
                 o->table64 = false;
                 o->slab_tiles64 = 0;
-                table_form = layout_op(*o, false) <= cap / 2;
+                table_form = layout_op(*o, false) <= static_cast<size_t>(kInnerShare * cap);
             }
         }
         if (!o->block && table_form && !idle)
