@@ -171,7 +171,7 @@ __device__ __forceinline__ void half_gemm_block(const _Float16 *__restrict__ hi_
         boff[u] = n2 < 0 ? kBufferNull
                          : static_cast<unsigned>((plane + static_cast<size_t>(n2) * S32 + (ln >> 4) * 8) *
                                                  sizeof(_Float16));
-        bslot[u] = q < CHUNKS ? q : -1;
+        bslot[u] = q < CHUNKS && itw < ntt ? q : -1;  // LDS tiles past the block's are never read
     }
     auto load_b = [&](int ks, h8_t (&breg)[PER]) {
         const unsigned so = static_cast<unsigned>(ks) * 32 * sizeof(_Float16);
