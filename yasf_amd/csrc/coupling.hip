@@ -245,8 +245,8 @@ void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, cons
 }
 
 void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
-                         void *Q, hipStream_t s) {
-    launch_pair_t<double, true>(dp, pos, N, k, x, Q, s);
+                         void *Q, hipStream_t s, int t0, int ntl) {
+    launch_pair_t<double, true>(dp, pos, N, k, x, Q, s, nullptr, kAllPairs, t0, ntl);
 }
 
 // Largest LDS the pair kernel asks for, bytes (2 stages of complex128).

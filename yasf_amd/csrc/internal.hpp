@@ -188,9 +188,10 @@ constexpr double kAllPairs = HUGE_VAL;
 void launch_pair(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x, bool fp32,
                  void *Q, hipStream_t s, const int *gate = nullptr, double near2 = kAllPairs, int t0 = 0,
                  int ntl = -1);
-// Q of the regular (J-translation) operator, coincident pairs included, FP64.
+// Q of the regular (J-translation) operator, coincident pairs included, FP64;
+// t0, ntl as launch_pair's.
 void launch_pair_regular(const DevicePlan &dp, const double4 *pos, int N, double k, const void *x,
-                         void *Q, hipStream_t s);
+                         void *Q, hipStream_t s, int t0 = 0, int ntl = -1);
 // ab5 contraction (FP64 accumulation) + optional master-operator epilogue.
 // t0, ntl: only the target tiles [t0, t0 + ntl) (ntl < 0: to the last),
 // read from Q from its start.  gate: as launch_pair.

@@ -280,7 +280,16 @@ constexpr int kServeShape64 = 2;
 // 172 GB table stayed resident but only part of its 16 requests fit beside
 // it, and the rest ran after them (complex64 inner 0.514 -> 0.441 solves/s,
 // fp16x2 1.026 -> 1.046).
-constexpr double kInnerShare = 0.5;
+constexpr double kInnerShare = 0.875;
+// Target tiles per FP64 range: a request's Q holds the complex64 products of
+// all its tiles and the FP64 ones of this many, so its FP64 products (the
+// residuals, the pair-sum C_sca) are formed and contracted range by range --
+// half of the tiles where the FP64 products of all of them would take more
+// than 256 MB, all of them below (one range: no extra launches)
+inline int fp64_range_tiles(int ntiles1, int nq) {
+    const size_t all = static_cast<size_t>(ntiles1) * nq * 256 * sizeof(double2);
+    return all <= (size_t(256) << 20) ? ntiles1 : (ntiles1 + 1) / 2;
+}
 // completions whose C_sca quadratures share one launch, at most
 constexpr int kMaxQuadBatch = 12;
 // Table GEMM shape of a request's single complex64 product (the residual of
@@ -897,7 +906,9 @@ struct yasfhip_server {
         const size_t o_xh = half ? take(half_x_bytes(dp, q.N)) : 0;
         const size_t o_cm = half ? take(sizeof(unsigned) * q.nmax) : 0;
         const size_t o_xe = half ? take(sizeof(int) * (q.nmax + 1)) : 0;
-        const size_t o_Q = take(sizeof(double2) * q.ntiles1 * dp.nq * 256);
+        // Q: the complex64 products of all target tiles, or the FP64 ones of
+        // a range of fp64_range_tiles tiles (FP64 GEMMs run range by range)
+        const size_t o_Q = take(sizeof(double2) * fp64_range_tiles(q.ntiles1, dp.nq) * dp.nq * 256);
         const size_t o_V = take(sizeof(double2) * R1 * q.n);
         const size_t o_part = take(sizeof(double2) * R1 * q.chunks);
         const size_t o_np = take(sizeof(double) * 2 * q.chunks);
@@ -1237,28 +1248,44 @@ struct yasfhip_server {
             }
             gemm_done[i] = 1;
         }
-        // FP64 pair GEMMs of A x: resident FP64 table, slab (one build per
-        // slab for all of an operator's requests here), or on the fly
+        // FP64 residuals r = b - A x into V[0], A x into V[1]: pair GEMMs over
+        // the resident FP64 table, a slab (one build per slab for all of an
+        // operator's requests here), the residual pool or on the fly, range
+        // by range of target tiles (fp64_range_tiles: what Q holds in FP64),
+        // each range contracted before the next overwrites Q
+        auto contract64 = [&](Request *r, int t0, int ntl) {
+            launch_contract(plan_for(r->L).dp, r->N, r->Q, r->x_d, r->op->t_d, 1, r->V + r->n, false, false, st,
+                            t0, ntl);
+        };
         for (size_t i = 0; i < todo.size(); ++i) {
             Request *q = todo[i];
             const SharedOp &o = *q->op;
             const DevicePlan &dp = plan_for(q->L).dp;
             if (gemm_done[i])
                 continue;
-            const int pool_tiles = o.table64 || o.slab ? 0 : pool_slab_tiles(o);
+            const int range = fp64_range_tiles(o.ntiles1, dp.nq);
+            const int pool_tiles = o.table64 || o.slab ? 0 : std::min(pool_slab_tiles(o), range);
             if (o.table64) {
-                launch_pair_table_gemm(dp, q->N, o.tab64, q->x_d, false, kServeShape64, q->Q, st);
+                const size_t tile_elems = pair_table_tile_bytes(dp, q->N, false) / sizeof(double2);
+                for (int t0 = 0; t0 < o.ntiles1; t0 += range) {
+                    const int ntl = std::min(range, o.ntiles1 - t0);
+                    launch_pair_table_gemm(dp, q->N, static_cast<const double2 *>(o.tab64) + t0 * tile_elems,
+                                           q->x_d, false, kServeShape64, q->Q, st, 0, ntl);
+                    contract64(q, t0, ntl);
+                }
                 ++residual_forms[0];
             } else if (o.slab || pool_tiles) {  // one build per slab for all of the operator's requests
                 void *slab = o.slab ? o.slab : res_pool;
-                const int tiles = o.slab ? o.slab_tiles64 : pool_tiles;
+                const int tiles = o.slab ? std::min(o.slab_tiles64, range) : pool_tiles;
                 for (int t0 = 0; t0 < o.ntiles1; t0 += tiles) {
                     const int ntl = std::min(tiles, o.ntiles1 - t0);
                     launch_pair_table_build(dp, o.pos_d, o.N, o.k, false, slab, st, t0, ntl);
                     for (size_t j = i; j < todo.size(); ++j)
-                        if (todo[j]->op == q->op && !todo[j]->approx_residual)
+                        if (todo[j]->op == q->op && !todo[j]->approx_residual) {
                             launch_pair_table_gemm(dp, q->N, slab, todo[j]->x_d, false, kServeShape64, todo[j]->Q,
-                                                   st, t0, ntl);
+                                                   st, 0, ntl);
+                            contract64(todo[j], t0, ntl);
+                        }
                 }
                 for (size_t j = i; j < todo.size(); ++j)
                     if (todo[j]->op == q->op && !todo[j]->approx_residual) {
@@ -1266,17 +1293,21 @@ struct yasfhip_server {
                         ++residual_forms[o.slab ? 1 : 2];
                     }
             } else {
-                launch_pair(dp, o.pos_d, q->N, q->k, q->x_d, false, q->Q, st);
+                for (int t0 = 0; t0 < o.ntiles1; t0 += range) {
+                    const int ntl = std::min(range, o.ntiles1 - t0);
+                    launch_pair(dp, o.pos_d, q->N, q->k, q->x_d, false, q->Q, st, nullptr, kAllPairs, t0, ntl);
+                    contract64(q, t0, ntl);
+                }
                 ++residual_forms[3];
             }
             gemm_done[i] = 1;
         }
         for (auto *q : todo) {
             const DevicePlan &dp = plan_for(q->L).dp;
-            // r = b - A x (FP64) into V[0]; V[1] holds A x
             double2 *ax = q->V + q->n;
             const SharedOp &o = *q->op;
-            launch_contract(dp, q->N, q->Q, q->x_d, o.t_d, 1, ax, q->approx_residual, false, st);
+            if (q->approx_residual)  // the inner operator's complex64 products of all tiles
+                launch_contract(dp, q->N, q->Q, q->x_d, o.t_d, 1, ax, true, false, st);
             launch_residual(q->V, q->b_d, ax, q->n, reducer(), scal + 8 * q->slot, st);
             if (q->first_refresh)
                 launch_norm2(q->b_d, q->n, reducer(), scal + 8 * q->slot + 1, st);
@@ -1360,35 +1391,48 @@ struct yasfhip_server {
                 if (q->observables)
                     launch_mgs_step(q->x_d, nullptr, nullptr, q->a_d, q->n, reducer(), scal + 8 * q->slot + 5, st);
         }
+        // C_sca by the regular pair sum: W_J x into V[1] range by range of
+        // target tiles (slabs, one build for all of an operator's requests,
+        // or on the fly), each range contracted before the next overwrites Q
+        auto contract_regular = [&](Request *r, int t0, int ntl) {
+            launch_contract(plan_for(r->L).dp, r->N, r->Q, r->x_d, r->op->t_d, 0, r->V + r->n, false, false, st,
+                            t0, ntl);
+        };
         for (size_t i = 0; i < done.size() && csca_method == YASFHIP_CSCA_PAIR_SUM; ++i) {
             Request *q = done[i];
             if (!q->observables || regular_done[i])
                 continue;
             const SharedOp &o = *q->op;
             const DevicePlan &dp = plan_for(q->L).dp;
+            const int range = fp64_range_tiles(o.ntiles1, dp.nq);
             if (o.slab) {
-                for (int t0 = 0; t0 < o.ntiles1; t0 += o.slab_tiles64) {
-                    const int ntl = std::min(o.slab_tiles64, o.ntiles1 - t0);
+                const int tiles = std::min(o.slab_tiles64, range);
+                for (int t0 = 0; t0 < o.ntiles1; t0 += tiles) {
+                    const int ntl = std::min(tiles, o.ntiles1 - t0);
                     launch_pair_table_build(dp, o.pos_d, o.N, o.k, false, o.slab, st, t0, ntl, nullptr, true);
                     for (size_t j = i; j < done.size(); ++j)
-                        if (done[j]->op == q->op && done[j]->observables)
+                        if (done[j]->op == q->op && done[j]->observables) {
                             launch_pair_table_gemm(dp, q->N, o.slab, done[j]->x_d, false, kServeShape64, done[j]->Q,
-                                                   st, t0, ntl);
+                                                   st, 0, ntl);
+                            contract_regular(done[j], t0, ntl);
+                        }
                 }
                 for (size_t j = i; j < done.size(); ++j)
                     if (done[j]->op == q->op)
                         regular_done[j] = 1;
             } else {
-                launch_pair_regular(dp, o.pos_d, q->N, q->k, q->x_d, q->Q, st);
+                for (int t0 = 0; t0 < o.ntiles1; t0 += range) {
+                    const int ntl = std::min(range, o.ntiles1 - t0);
+                    launch_pair_regular(dp, o.pos_d, q->N, q->k, q->x_d, q->Q, st, t0, ntl);
+                    contract_regular(q, t0, ntl);
+                }
             }
             regular_done[i] = 1;
         }
         for (auto *q : done) {
             if (!q->observables || csca_method != YASFHIP_CSCA_PAIR_SUM)
                 continue;
-            const DevicePlan &dp = plan_for(q->L).dp;
             double2 *y = q->V + q->n;
-            launch_contract(dp, q->N, q->Q, q->x_d, q->op->t_d, 0, y, false, false, st);
             launch_mgs_step(y, nullptr, nullptr, q->x_d, q->n, reducer(), scal + 8 * q->slot + 2, st);
             launch_mgs_step(q->x_d, nullptr, nullptr, q->a_d, q->n, reducer(), scal + 8 * q->slot + 5, st);
         }
