@@ -385,7 +385,7 @@ int yasfhip_server_autotune(yasfhip_server *s, int lmax, int particle_number, in
 
 /* Pair-factor memory of operators admitted from now on.  policy 1 (default):
  * resident tables where they fit (complex64 and FP64 within a quarter of the
- * arena; the inner table (complex64 or fp16 terms) within half, with an
+ * arena; the inner table (complex64 or fp16 terms) within 7/8, with an
  * FP64 slab of up to a sixteenth for the FP64 residual and C_sca), else a
  * slab of the table (a range of target
  * tiles, at most an eighth of the arena) rebuilt range by range in every

@@ -275,11 +275,9 @@ __global__ __launch_bounds__(64 * kServeHalfWaves) void serve_half_shared_gemm_k
 // session default (each table slice streamed once).
 constexpr int kServeShape64 = 2;
 // Share of the arena an operator's resident inner table (complex64 or fp16
-// terms) may take before the operator streams slabs instead.  Half leaves
-// room for the requests that share it: at 7/8 a 16,295-monomer aggregate's
-// 172 GB table stayed resident but only part of its 16 requests fit beside
-// it, and the rest ran after them (complex64 inner 0.514 -> 0.441 solves/s,
-// fp16x2 1.026 -> 1.046).
+// terms) may take before the operator streams slabs instead: a
+// 16,295-monomer aggregate's 172 GB table and its 16 requests (Q sized for
+// ranges of FP64 products, fp64_range_tiles) fit together.
 constexpr double kInnerShare = 0.875;
 // Target tiles per FP64 range: a request's Q holds the complex64 products of
 // all its tiles and the FP64 ones of this many, so its FP64 products (the
